@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Benchmark of the PPPM time step on one MI355X.
+
+Default workload (the configuration the metric is quoted on): plain PPPM with motion —
+64^3 grid, static subdivided cube (768 tris) + icosphere level 4 (5120 tris) translating
+(0.3 h, 200 Hz) and rotating (2 rad/s), band-limited noise on half of the vertices,
+4000 steps, FP32 stepping with FP64-built FP16 weights. One "step" = update_mesh (re-bin,
+re-list, recompute dirty weight-cache entries) + update_grid_and_face, as the reference's
+demo loop does per audio sample (experiments/demo/main.cu:170-205).
+
+Prints one JSON line. value = grid cell-updates/s (N^3 per step); inputs (vertex
+positions per step, Neumann series) are resident in HBM before the timed region.
+"""
+import argparse
+import json
+import math
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "pppm-tdbem_amd")
+sys.path.insert(0, PKG)
+
+HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
+
+
+def algorithmic_bytes(N, st):
+    """Algorithmic HBM bytes per launch of each step kernel.
+
+    fdtd      : read p^n, p^{n-1}, write p^{n+1}                  12 B / cell
+    far       : per (cell, tri) pair one 72 B FP16 record + 4 B index; per C3 cell read p, write far
+    dirichlet : per (tri, tri) pair 72 B record + 4 B index; per triangle 204 B (two 16-tap
+                history windows 128 B, 8 interpolation weights 32 B, 8 far reads 32 B,
+                area 4 B, N and D writes 8 B)
+    near      : as far, reading far and writing p
+    """
+    P3, C3, P4, T = st["P3"], st["C3"], st["P4"], st["T"]
+    return {
+        "fdtd": 12 * N ** 3,
+        "far": 76 * P3 + 8 * C3,
+        "dirichlet": 76 * P4 + 204 * T,
+        "near": 76 * P3 + 8 * C3,
+    }
+
+
+def cpu_baseline(sc, steps, threads):
+    """The CPU oracle (a port of the reference algorithm; the reference itself is CUDA-only)
+    on the host cores: `steps` steps of the same workload after an untimed set_mesh."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    import oracle
+
+    o = oracle.Oracle(sc.res, sc.dl, sc.dt, sc.min_pos, sc.c, 0.83, oracle.FP64)
+    o.set_mesh(sc.motion(0) if sc.motion else sc.verts, sc.tris)
+    t0 = time.perf_counter()
+    for i in range(steps):
+        if sc.motion:
+            o.update_mesh(sc.motion(i))
+        o.step(sc.neumann[i])
+    dt = time.perf_counter() - t0
+    return sc.res ** 3 * steps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's 4000)")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="c2_moving", choices=["c2_moving", "c1_static_sphere"])
+    ap.add_argument("--timing-steps", type=int, default=200, help="extra steps timed per kernel with HIP events")
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design note, "
+                                   "not an implemented path", "n_gpus_requested": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    import pppm_amd
+    import scenes
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU"}))
+        sys.exit(1)
+
+    default_steps = 4000 if args.workload == "c2_moving" else 1000
+    K = args.steps or default_steps
+    W = args.warmup
+    total = W + K + args.timing_steps
+    sc = scenes.c2_moving(steps=total) if args.workload == "c2_moving" else scenes.c1_static_sphere(steps=total)
+    moving = sc.motion is not None
+    N = sc.res
+
+    # inputs resident in HBM
+    neumann = torch.from_numpy(sc.neumann).cuda()
+    if moving:
+        verts_all = torch.from_numpy(np.stack([sc.motion(i) for i in range(total)])).cuda()
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.Stream()
+    solver = pppm_amd.PPPMSolver(sc.res, sc.dl, sc.dt, sc.min_pos, 0, 0.83, sc.c)
+    solver.set_stream(stream.cuda_stream)
+    solver.set_mesh(verts_all[0] if moving else sc.verts, torch.from_numpy(sc.tris).cuda() if moving else sc.tris)
+    lis = [sc.listener]
+    out = torch.zeros(total, 1, device="cuda")
+
+    def run_steps(i0, n):
+        if moving:
+            for i in range(i0, i0 + n):
+                solver.update_mesh(verts_all[i])
+                solver.update_grid_and_face(neumann[i])
+        else:
+            solver.run(neumann[i0:i0 + n], n, lis, out[i0:i0 + n], use_graph=True)
+
+    with torch.cuda.stream(stream):
+        run_steps(0, W)
+        stream.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run_steps(W, K)
+        stream.synchronize()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        elapsed = t1 - t0
+        st = solver.stats()
+        # per-kernel durations: HIP events around each launch, on the solver's stream
+        solver.set_timing(True)
+        i0 = W + K
+        if moving:
+            run_steps(i0, args.timing_steps)
+        else:
+            solver.run(neumann[i0:], args.timing_steps, lis, out[i0:], use_graph=False)
+        solver.set_timing(False)
+        phases = solver.get_timing()
+        stream.synchronize()
+
+    listener = float(solver.read_cells(lis)[0])
+    ms_step = elapsed * 1e3 / K
+    value = N ** 3 * K / elapsed
+    bytes_ = algorithmic_bytes(N, st)
+    step_bytes = sum(bytes_.values())
+    avg = {k: (phases[k][0] / phases[k][1] if phases[k][1] else None) for k in phases}
+    step_kernels = ["fdtd", "far", "dirichlet", "near"]
+    dom = max(step_kernels, key=lambda k: avg[k] or 0.0)
+    achieved = bytes_[dom] / (avg[dom] * 1e-3) / 1e9
+    phase_share = {k: round(phases[k][0] / max(sum(p[0] for p in phases.values()), 1e-12), 4) for k in phases}
+
+    res = {
+        "metric": "grid cell-updates/s",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": sc.name,
+            "grid": f"{N}^3",
+            "triangles": int(st["T"]),
+            "moving": moving,
+            "c3_cells": st["C3"], "p3_pairs": st["P3"], "p4_pairs": st["P4"],
+            "dt": sc.dt, "h": sc.dl,
+        },
+        "hbm_gbps_effective_step_kernels": step_bytes / (sum(avg[k] for k in step_kernels) * 1e-3) / 1e9,
+        "roofline": {
+            "kernel": {"fdtd": "k_fdtd_abc", "far": "k_far", "dirichlet": "k_dirichlet", "near": "k_near"}[dom],
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": None,
+            "bytes_per_launch": bytes_[dom],
+            "avg_launch_us": avg[dom] * 1e3,
+        },
+        "phase_avg_us": {k: (v * 1e3 if v else None) for k, v in avg.items()},
+        "phase_time_share": phase_share,
+        "listener_final": listener,
+    }
+    if not args.no_cpu:
+        cpu_rate, cpu_s = cpu_baseline(sc, args.cpu_steps, args.cpu_threads)
+        res["cpu_baseline"] = {
+            "value": cpu_rate,
+            "unit": "cell-updates/s",
+            "cores": args.cpu_threads,
+            "kind": "port",
+            "sample": f"CPU oracle (OpenMP, FP64 weights), first {args.cpu_steps} steps of {sc.name} incl. "
+                      f"update_mesh, after an untimed set_mesh; {cpu_s:.1f} s",
+        }
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

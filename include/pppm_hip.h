@@ -1,0 +1,133 @@
+/*
+ * pppm_hip.h — C ABI of the MI355X-native PPPM time step (libpppm_hip.so).
+ *
+ * Drop-in for the reference's PPPMSolver / FDTD solver API (src/algorithm/pppm.h,
+ * src/algorithm/fdtd.h). Plain pointers and sizes only; no C++ types cross the ABI.
+ *
+ * Conventions
+ *  - Every entry point returns an int status (PPPM_OK == 0, negative on error);
+ *    pppm_last_error() returns a message for the last failure on that handle.
+ *    No exceptions cross the ABI (the reference throws std::runtime_error from
+ *    cuSynchronize, src/core/macro.h:119-166).
+ *  - One handle per host thread. All device work of a handle runs on its stream
+ *    (pppm_set_stream; default: the legacy null stream, as the reference).
+ *  - Arrays passed in are copied (host or device, selected by on_device); device
+ *    pointers returned by *_ptr calls are borrowed and stay valid until the next
+ *    set_mesh/update_mesh/destroy.
+ *  - Grid layout is the reference's GArr3D: x-major, z contiguous,
+ *    index = (x*N + y)*N + z (src/core/array3D.h:54-87). History rings are 32 floats
+ *    per triangle, slot = t mod 32 (src/algorithm/bem.h:55, src/core/array.h:39).
+ */
+#ifndef PPPM_HIP_H
+#define PPPM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pppm_solver pppm_solver;
+
+enum pppm_status
+{
+    PPPM_OK = 0,
+    PPPM_ERR_ARG = -1,         /* bad argument */
+    PPPM_ERR_HIP = -2,         /* HIP runtime error */
+    PPPM_ERR_OUT_OF_GRID = -3, /* a triangle lies outside cells [2, N-3] */
+    PPPM_ERR_STATE = -4,       /* call order (e.g. step before set_mesh) */
+    PPPM_ERR_CAPACITY = -5     /* an internal capacity was exceeded */
+};
+
+enum pppm_flags
+{
+    /* Honour pml_width in the FDTD. The reference's PPPMSolver drops it
+     * (src/algorithm/particle_grid.h:115), so the default follows that. */
+    PPPM_FLAG_HONOR_PML = 1u << 0,
+};
+
+/* PPPMSolver::PPPMSolver(res, dl, dt, min_pos, pml_width, reflect_coeff)
+ * (src/algorithm/pppm.h:39-55) plus TDBEM::init (src/algorithm/bem.h:264-277).
+ * c is the speed of sound, hard-coded as AIR_WAVE_SPEED 340.3 in the reference
+ * (src/core/macro.h:82). min_xyz may be NULL (origin). */
+int pppm_create(pppm_solver **out, int res, float dl, float dt, const float *min_xyz, int pml_width,
+                float reflect_coeff, float c, unsigned flags);
+/* PPPMSolver::clear (src/algorithm/pppm.h:111-125) + delete */
+int pppm_destroy(pppm_solver *s);
+/* all subsequent work of this handle is enqueued on stream (a hipStream_t) */
+int pppm_set_stream(pppm_solver *s, void *hip_stream);
+const char *pppm_last_error(const pppm_solver *s);
+
+/* PPPMSolver::set_mesh (src/algorithm/pppm.h:75-94): verts nv x 3 floats, tris nt x 3 ints.
+ * Bins triangles, builds the 3x3x3 / 4x4x4 lists and the FP64 weight caches. */
+int pppm_set_mesh(pppm_solver *s, const float *verts, int nv, const int *tris, int nt, int on_device);
+/* PPPMSolver::update_mesh (src/algorithm/pppm.h:96-109): new vertex positions, same topology.
+ * Re-bins and recomputes the cache entries whose geometry moved past the thresholds
+ * (src/algorithm/pppm_cache.h:64-69, 122-127). */
+int pppm_update_mesh(pppm_solver *s, const float *verts, int on_device);
+
+/* PPPMSolver::update_grid_and_face / update_step (src/algorithm/pppm.h:137-152):
+ * FDTD step, far field, Neumann input (nt floats), Dirichlet update, near field. */
+int pppm_step(pppm_solver *s, const float *neumann, int on_device);
+/* Runs `steps` consecutive update_steps. neumann_series is device memory holding
+ * steps x stride floats (row i = Neumann input of step i, nt used). After each step
+ * the pressure at the n_listen cells (xyz triples, host array) is written to
+ * out_dev[i*n_listen + l] (device memory). use_graph != 0 replays the step as a
+ * captured HIP graph. Replaces the demo loop's per-step one-float D2H read
+ * (experiments/demo/main.cu:266, src/core/array.h:276-288). */
+int pppm_run(pppm_solver *s, const float *neumann_series, long long stride, int steps, const int *listen_xyz,
+             int n_listen, float *out_dev, int use_graph);
+
+/* Phase entry points used by the reference's tests (src/algorithm/pppm.h:127-135,
+ * src/algorithm/fdtd.h:73-87). */
+int pppm_fdtd_step(pppm_solver *s);                              /* FDTD::step */
+int pppm_solve_far(pppm_solver *s);                              /* solve_fdtd_far */
+int pppm_set_neumann(pppm_solver *s, const float *neumann, int on_device); /* set_neumann_condition */
+int pppm_update_dirichlet(pppm_solver *s);                       /* update_dirichlet */
+int pppm_solve_near(pppm_solver *s);                             /* solve_fdtd_near */
+/* fdtd.reset(); neumann.reset(); dirichlet.reset() (experiments/demo/main.cu:251-254) */
+int pppm_reset(pppm_solver *s);
+
+/* time_idx() (src/algorithm/pppm.h:71); t starts at -1 (src/algorithm/fdtd.h:55) */
+int pppm_time_idx(const pppm_solver *s, int *t);
+/* fdtd.grids[t + rel] device pointer (N^3 floats); get_grid() is rel = 0 (pppm.h:154) */
+int pppm_grid_ptr(pppm_solver *s, int rel, float **dev_ptr);
+/* copies grids[t] at n cells (xyz triples, host) to host out — the listener read
+ * without a full-grid transfer */
+int pppm_read_cells(pppm_solver *s, const int *xyz, int n, float *host_out);
+/* device pointers of the Neumann and Dirichlet rings (nt x 32 floats each, slot t mod 32) */
+int pppm_history_ptrs(pppm_solver *s, float **neumann, float **dirichlet);
+/* get_triangles() (pppm.h:156): 13 floats per triangle to host:
+ * center(3) normal(3) area cell(3) base(3) */
+int pppm_triangles(pppm_solver *s, float *host13);
+/* counts: [0]=T [1]=C3 cells (cells within +-1 of a triangle cell) [2]=P3 pairs
+ * [3]=non-empty base cells [4]=P4 pairs [5]=grid-cache recomputes at last update
+ * [6]=face-cache recomputes at last update [7]=face-cache pool entries */
+int pppm_stats(pppm_solver *s, long long *out8);
+
+/* Per-phase GPU timing with HIP events on the handle's stream, the counterpart of the
+ * reference's log_time / LOG_TIME phase timers (src/core/macro.h:35-48, pppm.cu:45-50).
+ * Phases: 0 FDTD, 1 far field (+ Neumann input), 2 Dirichlet update, 3 near field,
+ * 4 update_mesh. Only eagerly launched work is timed (graph replays are not).
+ * pppm_get_timing returns accumulated milliseconds and launch counts, then resets them. */
+int pppm_set_timing(pppm_solver *s, int enable);
+int pppm_get_timing(pppm_solver *s, double *ms8, long long *count8);
+
+/* Cache inspection / loading in the reference's LayerWeightHalf layout (72 B per
+ * record, src/algorithm/bem.h:163-169). Grid cache: records (tri*27+o)*2 + {0: fdtd, 1: bem}
+ * (GridCache::CacheElement, pppm_cache.h:13-20). Face cache: one record per
+ * (triangle c, entry of the 4x4x4 list of c's base cell), CSR over c
+ * (FaceCache::cache, pppm_cache.h:81-155); start has nt+1 entries. */
+int pppm_get_grid_weights(pppm_solver *s, void *host_recs72);
+int pppm_set_grid_weights(pppm_solver *s, const void *host_recs72);
+int pppm_get_face_weights(pppm_solver *s, int *host_start, int *host_nbr, void *host_recs72);
+int pppm_set_face_weights(pppm_solver *s, const void *host_recs72);
+int pppm_get_interp_weights(pppm_solver *s, float *host_out8);
+/* ParticleGrid lists (src/algorithm/particle_grid.h:100-112): C3 cells with their
+ * 3x3x3 pair lists (entries tri*27 + weight offset), non-empty base cells with their
+ * 4x4x4 triangle lists. Pass NULL to skip an array; sizes come from pppm_stats. */
+int pppm_get_c3(pppm_solver *s, int *cells, int *start, int *pairs);
+int pppm_get_b4(pppm_solver *s, int *cells, int *start, int *tris);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

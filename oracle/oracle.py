@@ -1,0 +1,303 @@
+"""ctypes binding of the CPU oracle (liboracle.so) — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this module,
+as the checker. The product (libpppm_hip.so and its bindings) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF32, FP64 = 0, 1
+SINGLE, DOUBLE = 0, 1
+
+LWH_DTYPE = np.dtype([("s", "<u2", (16,)), ("ss", "<f4"), ("d", "<u2", (16,)), ("sd", "<f4")])
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-C", HERE, "-s"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        vp, i, f, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double
+        sig = {
+            "orc_create": ([i, f, f, vp, f, f, i, i], vp),
+            "orc_destroy": ([vp], None),
+            "orc_set_mesh": ([vp, vp, i, vp, i], i),
+            "orc_update_mesh": ([vp, vp], i),
+            "orc_step": ([vp, vp], i),
+            "orc_fdtd_step": ([vp], i),
+            "orc_solve_far": ([vp], i),
+            "orc_set_neumann": ([vp, vp], i),
+            "orc_update_dirichlet": ([vp], i),
+            "orc_solve_near": ([vp], i),
+            "orc_solve_far_direct": ([vp], i),
+            "orc_solve_near_direct": ([vp], i),
+            "orc_time_idx": ([vp], i),
+            "orc_get_grid": ([vp, i, vp], i),
+            "orc_set_grid": ([vp, i, vp], i),
+            "orc_get_far": ([vp, vp], i),
+            "orc_read_cell": ([vp, i, i, i], f),
+            "orc_get_history": ([vp, vp, vp], i),
+            "orc_set_history": ([vp, vp, vp], i),
+            "orc_reset": ([vp], None),
+            "orc_stats": ([vp, vp], i),
+            "orc_get_triangles": ([vp, vp], i),
+            "orc_get_c3": ([vp, vp, vp, vp], i),
+            "orc_get_b4": ([vp, vp, vp, vp], i),
+            "orc_export_grid_weights": ([vp, vp], i),
+            "orc_import_grid_weights": ([vp, vp], i),
+            "orc_export_face_weights": ([vp, vp, vp, vp], i),
+            "orc_import_face_weights": ([vp, vp], i),
+            "orc_get_interp_weights": ([vp, vp], i),
+            "orc_f2h": ([f], ctypes.c_ushort),
+            "orc_h2f": ([ctypes.c_ushort], f),
+            "orc_tdbem_init": ([i, f, f, vp, vp], None),
+            "orc_scaled_fft": ([i, vp, d, vp], None),
+            "orc_scaled_idft": ([vp, d, vp], None),
+            "orc_face2point": ([i, vp, vp, d, d, i, vp], None),
+            "orc_face2face": ([i, vp, vp, vp, d, d, i, f, vp], None),
+            "orc_laplace_weight_point": ([i, f, f, f, vp, vp, i, vp], None),
+            "orc_singular_fp64": ([vp, vp, vp, d, d, i, i, vp], None),
+        }
+        for name, (a, r) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = a
+            fn.restype = r
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Oracle:
+    """CPU restatement of PPPMSolver (src/algorithm/pppm.h) for parity checks."""
+
+    def __init__(self, res, dl, dt, min_pos=(0, 0, 0), c=343.0, reflect_coeff=0.83, mode=FP64, pml_width=0):
+        self.L = lib()
+        mn = np.asarray(min_pos, dtype=np.float32)
+        self.s = self.L.orc_create(int(res), float(dl), float(dt), _p(mn), float(c), float(reflect_coeff),
+                                   int(mode), int(pml_width))
+        if not self.s:
+            raise ValueError("orc_create failed")
+        self.N = int(res)
+        self.nt = 0
+
+    def __del__(self):
+        try:
+            if self.s:
+                self.L.orc_destroy(self.s)
+        except Exception:
+            pass
+
+    def _ok(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"oracle {what} failed ({rc})")
+
+    def set_mesh(self, verts, tris):
+        v = np.ascontiguousarray(verts, dtype=np.float32)
+        t = np.ascontiguousarray(tris, dtype=np.int32)
+        self._ok(self.L.orc_set_mesh(self.s, _p(v), v.shape[0], _p(t), t.shape[0]), "set_mesh")
+        self.nt = t.shape[0]
+
+    def update_mesh(self, verts):
+        v = np.ascontiguousarray(verts, dtype=np.float32)
+        self._ok(self.L.orc_update_mesh(self.s, _p(v)), "update_mesh")
+
+    def step(self, neumann):
+        n = np.ascontiguousarray(neumann, dtype=np.float32)
+        self._ok(self.L.orc_step(self.s, _p(n)), "step")
+
+    def fdtd_step(self):
+        self._ok(self.L.orc_fdtd_step(self.s), "fdtd_step")
+
+    def solve_far(self):
+        self._ok(self.L.orc_solve_far(self.s), "solve_far")
+
+    def set_neumann(self, neumann):
+        n = np.ascontiguousarray(neumann, dtype=np.float32)
+        self._ok(self.L.orc_set_neumann(self.s, _p(n)), "set_neumann")
+
+    def update_dirichlet(self):
+        self._ok(self.L.orc_update_dirichlet(self.s), "update_dirichlet")
+
+    def solve_near(self):
+        self._ok(self.L.orc_solve_near(self.s), "solve_near")
+
+    def solve_far_direct(self):
+        self._ok(self.L.orc_solve_far_direct(self.s), "solve_far_direct")
+
+    def solve_near_direct(self):
+        self._ok(self.L.orc_solve_near_direct(self.s), "solve_near_direct")
+
+    def reset(self):
+        self.L.orc_reset(self.s)
+
+    def time_idx(self):
+        return self.L.orc_time_idx(self.s)
+
+    def grid(self, rel=0):
+        out = np.empty(self.N ** 3, dtype=np.float32)
+        self.L.orc_get_grid(self.s, int(rel), _p(out))
+        return out.reshape(self.N, self.N, self.N)
+
+    def set_grid(self, rel, g):
+        g = np.ascontiguousarray(g, dtype=np.float32)
+        self.L.orc_set_grid(self.s, int(rel), _p(g))
+
+    def far(self):
+        out = np.empty(self.N ** 3, dtype=np.float32)
+        self.L.orc_get_far(self.s, _p(out))
+        return out.reshape(self.N, self.N, self.N)
+
+    def read(self, x, y, z):
+        return self.L.orc_read_cell(self.s, int(x), int(y), int(z))
+
+    def history(self):
+        n = np.empty((self.nt, 32), dtype=np.float32)
+        d = np.empty((self.nt, 32), dtype=np.float32)
+        self.L.orc_get_history(self.s, _p(n), _p(d))
+        return n, d
+
+    def set_history(self, n, d):
+        n = np.ascontiguousarray(n, dtype=np.float32)
+        d = np.ascontiguousarray(d, dtype=np.float32)
+        self.L.orc_set_history(self.s, _p(n), _p(d))
+
+    def stats(self):
+        out = np.zeros(8, dtype=np.int64)
+        self.L.orc_stats(self.s, _p(out))
+        keys = ["T", "C3", "P3", "B4", "P4", "grid_recomputes", "face_recomputes"]
+        return dict(zip(keys, (int(x) for x in out[:7])))
+
+    def triangles(self):
+        out = np.zeros((self.nt, 13), dtype=np.float32)
+        self.L.orc_get_triangles(self.s, _p(out))
+        return out
+
+    def c3(self):
+        st = self.stats()
+        cells = np.zeros(max(st["C3"], 1), np.int32)
+        start = np.zeros(st["C3"] + 1, np.int32)
+        pairs = np.zeros(max(st["P3"], 1), np.int32)
+        self.L.orc_get_c3(self.s, _p(cells), _p(start), _p(pairs))
+        return cells[:st["C3"]], start, pairs[:st["P3"]]
+
+    def b4(self):
+        st = self.stats()
+        cells = np.zeros(max(st["B4"], 1), np.int32)
+        start = np.zeros(st["B4"] + 1, np.int32)
+        self.L.orc_get_b4(self.s, _p(cells), _p(start), None)
+        tris = np.zeros(max(int(start[-1]), 1), np.int32)
+        self.L.orc_get_b4(self.s, None, None, _p(tris))
+        return cells[:st["B4"]], start, tris[:int(start[-1])]
+
+    def grid_weights(self):
+        rec = np.zeros(self.nt * 27 * 2, dtype=LWH_DTYPE)
+        self.L.orc_export_grid_weights(self.s, _p(rec))
+        return rec
+
+    def set_grid_weights(self, rec):
+        rec = np.ascontiguousarray(rec, dtype=LWH_DTYPE)
+        self.L.orc_import_grid_weights(self.s, _p(rec))
+
+    def face_weights(self):
+        st = self.stats()
+        start = np.zeros(self.nt + 1, np.int32)
+        nbr = np.zeros(max(st["P4"], 1), np.int32)
+        rec = np.zeros(max(st["P4"], 1), dtype=LWH_DTYPE)
+        self.L.orc_export_face_weights(self.s, _p(start), _p(nbr), _p(rec))
+        return start, nbr[:st["P4"]], rec[:st["P4"]]
+
+    def set_face_weights(self, rec):
+        rec = np.ascontiguousarray(rec, dtype=LWH_DTYPE)
+        self.L.orc_import_face_weights(self.s, _p(rec))
+
+    def interp_weights(self):
+        out = np.zeros((self.nt, 8), np.float32)
+        self.L.orc_get_interp_weights(self.s, _p(out))
+        return out
+
+
+# ---------------------------------------------------------------------- primitives
+def f2h(x: float) -> int:
+    return int(lib().orc_f2h(float(x)))
+
+
+def h2f(h: int) -> float:
+    return float(lib().orc_h2f(int(h)))
+
+
+def tdbem_init(mode, dt, c):
+    lam = ctypes.c_double()
+    wave = np.zeros(32, np.float64)
+    lib().orc_tdbem_init(int(mode), float(dt), float(c), ctypes.byref(lam), _p(wave))
+    return lam.value, wave[0::2] + 1j * wave[1::2]
+
+
+def scaled_fft(mode, v16, lam):
+    v = np.zeros(32, np.float64)
+    v[0::2] = np.real(v16)
+    v[1::2] = np.imag(v16)
+    out = np.zeros(16, np.float64)
+    lib().orc_scaled_fft(int(mode), _p(v), float(lam), _p(out))
+    return out
+
+
+def scaled_idft(v16, lam):
+    v = np.zeros(32, np.float64)
+    v[0::2] = np.real(v16)
+    v[1::2] = np.imag(v16)
+    out = np.zeros(16, np.float64)
+    lib().orc_scaled_idft(_p(v), float(lam), _p(out))
+    return out
+
+
+def face2point(mode, tri9, pt3, k, layer):
+    tri = np.ascontiguousarray(tri9, np.float32).reshape(9)
+    pt = np.ascontiguousarray(pt3, np.float32).reshape(3)
+    out = np.zeros(2, np.float64)
+    lib().orc_face2point(int(mode), _p(tri), _p(pt), float(np.real(k)), float(np.imag(k)), int(layer), _p(out))
+    return out[0] + 1j * out[1]
+
+
+def face2face(mode, verts, src3, trg3, k, layer, threshold):
+    v = np.ascontiguousarray(verts, np.float32).reshape(-1)
+    s = np.ascontiguousarray(src3, np.int32)
+    t = np.ascontiguousarray(trg3, np.int32)
+    out = np.zeros(2, np.float64)
+    lib().orc_face2face(int(mode), _p(v), _p(s), _p(t), float(np.real(k)), float(np.imag(k)), int(layer),
+                        float(threshold), _p(out))
+    return out[0] + 1j * out[1]
+
+
+def singular_fp64(verts, src3, trg3, k, layer, gauss_n):
+    v = np.ascontiguousarray(verts, np.float32).reshape(-1)
+    s = np.ascontiguousarray(src3, np.int32)
+    t = np.ascontiguousarray(trg3, np.int32)
+    out = np.zeros(2, np.float64)
+    lib().orc_singular_fp64(_p(v), _p(s), _p(t), float(np.real(k)), float(np.imag(k)), int(layer), int(gauss_n),
+                            _p(out))
+    return out[0] + 1j * out[1]
+
+
+def laplace_weight_point(mode, dt, c, h, tri9, pt3, layer):
+    tri = np.ascontiguousarray(tri9, np.float32).reshape(9)
+    pt = np.ascontiguousarray(pt3, np.float32).reshape(3)
+    out = np.zeros(16, np.float64)
+    lib().orc_laplace_weight_point(int(mode), float(dt), float(c), float(h), _p(tri), _p(pt), int(layer), _p(out))
+    return out

@@ -1,0 +1,308 @@
+// FDTD pressure update for gfx950: 7-point leapfrog stencil (FP64 arithmetic on FP32
+// storage) fused with the first-order absorbing boundary, or the split-field PML.
+//
+// Reference behaviour: FDTD::step (src/algorithm/fdtd.h:73-87), fdtd_inner_kernel and
+// laplacian (src/algorithm/fdtd.cu:7-33), ab_condition_solve / Solve4x4Linear
+// (fdtd.cu:36-137), fdtd_boundary_kernel (fdtd.cu:139-150), pml_step_kernel (fdtd.cu:162-232).
+//
+// One launch covers the whole step: the first blocks stream the interior (indices
+// 2..N-3 in every dimension, z contiguous so a wave reads 256 B rows), the remaining
+// blocks solve the 4x4 ABC system for the index-1 / N-2 shell. Interior and shell
+// write disjoint cells of level t+1; the shell also writes the ghost layer of level t,
+// which no thread of this launch reads except the writing thread itself (masked
+// terms of its own system), so the fusion is race-free. The last block to finish
+// advances t.
+#include "internal.hpp"
+
+namespace pppm_amd
+{
+
+namespace
+{
+
+constexpr int BLK = 256;
+
+__device__ inline float *level(float *g3, int t, int rel, size_t nn)
+{
+    int l = ((t + rel) % 3 + 3) % 3;
+    return g3 + (size_t)l * nn;
+}
+
+__device__ inline void advance_t(StepState *st)
+{
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        __threadfence();
+        unsigned prev = atomicAdd(&st->done, 1u);
+        if (prev == gridDim.x - 1)
+        {
+            atomicAdd(&st->t, 1);
+            atomicExch(&st->done, 0u);
+            __threadfence();
+        }
+    }
+}
+
+// Solve4x4Linear (fdtd.cu:36-86): cofactor inverse in FP32, FP64 reciprocal of the determinant.
+__device__ inline void solve4(const float a[4][4], const float b[4], float res[4])
+{
+    float s0 = a[0][0] * a[1][1] - a[1][0] * a[0][1];
+    float s1 = a[0][0] * a[1][2] - a[1][0] * a[0][2];
+    float s2 = a[0][0] * a[1][3] - a[1][0] * a[0][3];
+    float s3 = a[0][1] * a[1][2] - a[1][1] * a[0][2];
+    float s4 = a[0][1] * a[1][3] - a[1][1] * a[0][3];
+    float s5 = a[0][2] * a[1][3] - a[1][2] * a[0][3];
+    float c5 = a[2][2] * a[3][3] - a[3][2] * a[2][3];
+    float c4 = a[2][1] * a[3][3] - a[3][1] * a[2][3];
+    float c3 = a[2][1] * a[3][2] - a[3][1] * a[2][2];
+    float c2 = a[2][0] * a[3][3] - a[3][0] * a[2][3];
+    float c1 = a[2][0] * a[3][2] - a[3][0] * a[2][2];
+    float c0 = a[2][0] * a[3][1] - a[3][0] * a[2][1];
+    float invdet = (float)(1.0 / (double)(s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 + s5 * c0));
+    float inv[4][4];
+    inv[0][0] = (a[1][1] * c5 - a[1][2] * c4 + a[1][3] * c3) * invdet;
+    inv[0][1] = (-a[0][1] * c5 + a[0][2] * c4 - a[0][3] * c3) * invdet;
+    inv[0][2] = (a[3][1] * s5 - a[3][2] * s4 + a[3][3] * s3) * invdet;
+    inv[0][3] = (-a[2][1] * s5 + a[2][2] * s4 - a[2][3] * s3) * invdet;
+    inv[1][0] = (-a[1][0] * c5 + a[1][2] * c2 - a[1][3] * c1) * invdet;
+    inv[1][1] = (a[0][0] * c5 - a[0][2] * c2 + a[0][3] * c1) * invdet;
+    inv[1][2] = (-a[3][0] * s5 + a[3][2] * s2 - a[3][3] * s1) * invdet;
+    inv[1][3] = (a[2][0] * s5 - a[2][2] * s2 + a[2][3] * s1) * invdet;
+    inv[2][0] = (a[1][0] * c4 - a[1][1] * c2 + a[1][3] * c0) * invdet;
+    inv[2][1] = (-a[0][0] * c4 + a[0][1] * c2 - a[0][3] * c0) * invdet;
+    inv[2][2] = (a[3][0] * s4 - a[3][1] * s2 + a[3][3] * s0) * invdet;
+    inv[2][3] = (-a[2][0] * s4 + a[2][1] * s2 - a[2][3] * s0) * invdet;
+    inv[3][0] = (-a[1][0] * c3 + a[1][1] * c1 - a[1][2] * c0) * invdet;
+    inv[3][1] = (a[0][0] * c3 - a[0][1] * c1 + a[0][2] * c0) * invdet;
+    inv[3][2] = (-a[3][0] * s3 + a[3][1] * s1 - a[3][2] * s0) * invdet;
+    inv[3][3] = (a[2][0] * s3 - a[2][1] * s1 + a[2][2] * s0) * invdet;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+    {
+        float r = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            r += inv[i][j] * b[j];
+        res[i] = r;
+    }
+}
+
+// ab_condition_solve (fdtd.cu:88-137) for one shell cell
+__device__ inline void abc_cell(float *g, const float *gm, float *gp, int N, int i, int j, int k, const FdtdParams &p)
+{
+    const int co[3] = {i, j, k};
+    int judge[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+        judge[d] = -((int)(co[d] == 1)) + ((int)(co[d] == N - 2));
+    const int strd[3] = {N * N, N, 1};
+    const int me = lin3(i, j, k, N);
+    const float dt = p.dt, h = p.h, c = p.c;
+    float a[4][4], b[4];
+    float coef = c * c * dt * dt / (h * h);
+    a[3][3] = 1;
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+        a[3][d] = -coef * (float)(judge[d] != 0);
+    const float gc = g[me], gmc = gm[me];
+    b[3] = -gmc + 2 * gc - coef * (6 * gc);
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+        b[3] += coef * g[me + strd[d]] * (float)(judge[d] <= 0) + coef * g[me - strd[d]] * (float)(judge[d] >= 0);
+    float coef1 = 1 / (dt * dt), coef2 = c / (2 * h * dt), coef3 = -(c * c) / (2 * h * h);
+#pragma unroll
+    for (int idx = 0; idx < 3; idx++)
+    {
+        const int x = idx, y = (idx + 1) % 3, z = (idx + 2) % 3;
+        a[idx][3] = coef1;
+        a[idx][x] = coef2;
+        a[idx][y] = coef3 * (float)(judge[y] != 0);
+        a[idx][z] = coef3 * (float)(judge[z] != 0);
+        b[idx] = -coef1 * (gmc - 2 * gc) +
+                 -coef2 * (g[me + strd[x]] * (float)(-(int)(judge[x] < 0)) +
+                           g[me - strd[x]] * (float)(-(int)(judge[x] > 0)) + gm[me + strd[x]] * (float)(-judge[x]) +
+                           gm[me - strd[x]] * (float)judge[x]) +
+                 -coef3 * (g[me + strd[y]] * (float)(judge[y] <= 0) + g[me - strd[y]] * (float)(judge[y] >= 0) +
+                           g[me + strd[z]] * (float)(judge[z] <= 0) + g[me - strd[z]] * (float)(judge[z] >= 0) -
+                           4 * gc);
+    }
+    float r[4];
+    solve4(a, b, r);
+    gp[me] = r[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+        if (judge[d] != 0)
+            g[me + judge[d] * strd[d]] = r[d];
+}
+
+__global__ __launch_bounds__(BLK) void k_fdtd_abc(float *__restrict__ grid3, FdtdParams p, StepState *st,
+                                                  int nb_inner)
+{
+    const int N = p.N;
+    const size_t nn = (size_t)N * N * N;
+    const int t = st->t;
+    float *g = level(grid3, t, 0, nn);
+    const float *gm = level(grid3, t, -1, nn);
+    float *gp = level(grid3, t, 1, nn);
+    if ((int)blockIdx.x < nb_inner)
+    {
+        const int M = N - 4;
+        const long long idx = (long long)blockIdx.x * BLK + threadIdx.x;
+        if (idx < (long long)M * M * M)
+        {
+            const int x = 2 + (int)(idx / (M * M));
+            const int rem = (int)(idx % (M * M));
+            const int y = 2 + rem / M, z = 2 + rem % M;
+            const int me = lin3(x, y, z, N);
+            // laplacian + update in FP64, sum order x-, x+, y-, y+, z-, z+, -6*centre (fdtd.cu:7-33)
+            double sum = 0;
+            sum += g[me - N * N];
+            sum += g[me + N * N];
+            sum += g[me - N];
+            sum += g[me + N];
+            sum += g[me - 1];
+            sum += g[me + 1];
+            const float gc = g[me];
+            sum -= 6 * gc;
+            const double lap = sum / p.hh;
+            double v = 2 * gc;
+            v += p.coef * lap;
+            v += -gm[me];
+            gp[me] = (float)v;
+        }
+    }
+    else
+    {
+        const int S = N - 2;
+        long long idx = (long long)(blockIdx.x - nb_inner) * BLK + threadIdx.x;
+        const long long fx = 2LL * S * S, fy = 2LL * (S - 2) * S, fz = 2LL * (S - 2) * (S - 2);
+        int i = -1, j = 0, k = 0;
+        if (idx < fx)
+        {
+            int side = (int)(idx / ((long long)S * S));
+            int r = (int)(idx % ((long long)S * S));
+            i = side ? N - 2 : 1;
+            j = 1 + r / S;
+            k = 1 + r % S;
+        }
+        else if ((idx -= fx) < fy)
+        {
+            int side = (int)(idx / ((long long)(S - 2) * S));
+            int r = (int)(idx % ((long long)(S - 2) * S));
+            j = side ? N - 2 : 1;
+            i = 2 + r / S;
+            k = 1 + r % S;
+        }
+        else if ((idx -= fy) < fz)
+        {
+            int side = (int)(idx / ((long long)(S - 2) * (S - 2)));
+            int r = (int)(idx % ((long long)(S - 2) * (S - 2)));
+            k = side ? N - 2 : 1;
+            i = 2 + r / (S - 2);
+            j = 2 + r % (S - 2);
+        }
+        if (i >= 0)
+            abc_cell(g, gm, gp, N, i, j, k, p);
+    }
+    advance_t(st);
+}
+
+// get_damp (fdtd.cu:162-169)
+__device__ inline float pml_damp(const FdtdParams &p, int x, int y, int z)
+{
+    const int N = p.N, w = p.pml;
+    if (x < w || x >= N - w || y < w || y >= N - w || z < w || z >= N - w)
+        return p.damp;
+    return 0.0f;
+}
+
+// pml_step_kernel (fdtd.cu:179-232); PML state: 6 floats per cell (v1[3], v2[3]) per level
+__global__ __launch_bounds__(BLK) void k_fdtd_pml(float *__restrict__ grid3, float *__restrict__ pml3, FdtdParams p,
+                                                  StepState *st)
+{
+    const int N = p.N;
+    const size_t nn = (size_t)N * N * N;
+    const int t = st->t;
+    const float *g = level(grid3, t, 0, nn);
+    const float *gm = level(grid3, t, -1, nn);
+    float *gp = level(grid3, t, 1, nn);
+    const int l0 = ((t % 3) + 3) % 3, l1 = (((t + 1) % 3) + 3) % 3;
+    const float *P = pml3 + (size_t)l0 * nn * 6;
+    float *Pn = pml3 + (size_t)l1 * nn * 6;
+    const int S = N - 2;
+    const long long idx = (long long)blockIdx.x * BLK + threadIdx.x;
+    if (idx < (long long)S * S * S)
+    {
+        const int x = 1 + (int)(idx / (S * S));
+        const int rem = (int)(idx % (S * S));
+        const int y = 1 + rem / S, z = 1 + rem % S;
+        const int co[3] = {x, y, z};
+        const int strd[3] = {N * N, N, 1};
+        int sg[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            sg[i] = co[i] > N / 2 ? 1 : -1;
+        const float cd = pml_damp(p, x, y, z);
+        const int me = lin3(x, y, z, N);
+        const float c = p.c, h = p.h, dt = p.dt;
+        auto dmp = [&](int i, int d) {
+            int q[3] = {x, y, z};
+            q[i] += d;
+            return pml_damp(p, q[0], q[1], q[2]);
+        };
+        float rhs = 0;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+        {
+            rhs += 1 / (h * h) * (g[me + strd[i]] - 2 * g[me] + g[me - strd[i]]);
+            rhs += 1 / h *
+                   (cd * P[(size_t)(me + sg[i] * strd[i]) * 6 + 3 + i] -
+                    dmp(i, -sg[i]) * P[(size_t)(me - sg[i] * strd[i]) * 6 + i]);
+        }
+        gp[me] = g[me] * 2 - gm[me] + c * c * dt * dt * rhs;
+        if (cd != 0)
+        {
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+            {
+                float r = 0;
+                r += -1.0f / 2.0f * (dmp(i, -sg[i]) * P[(size_t)(me - sg[i] * strd[i]) * 6 + i] + cd * P[(size_t)me * 6 + i]);
+                r += -1.0f / (h * 2.0f) * (g[me + sg[i] * strd[i]] - g[me - sg[i] * strd[i]]);
+                Pn[(size_t)me * 6 + i] = P[(size_t)me * 6 + i] + c * dt * r;
+            }
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+            {
+                float r = 0;
+                r += -1.0f / 2.0f *
+                     (dmp(i, -sg[i]) * P[(size_t)me * 6 + 3 + i] + cd * P[(size_t)(me + sg[i] * strd[i]) * 6 + 3 + i]);
+                r += -1.0f / (h * 2.0f) * (g[me + sg[i] * strd[i]] - g[me - sg[i] * strd[i]]);
+                Pn[(size_t)me * 6 + 3 + i] = P[(size_t)me * 6 + 3 + i] + c * dt * r;
+            }
+        }
+    }
+    advance_t(st);
+}
+
+}  // namespace
+
+hipError_t launch_fdtd(float *grid3, float *pml3, const FdtdParams &p, StepState *st, hipStream_t s)
+{
+    const int N = p.N;
+    if (p.pml > 0)
+    {
+        long long cells = (long long)(N - 2) * (N - 2) * (N - 2);
+        int nb = (int)((cells + BLK - 1) / BLK);
+        k_fdtd_pml<<<nb, BLK, 0, s>>>(grid3, pml3, p, st);
+        return hipGetLastError();
+    }
+    const long long M = N - 4, S = N - 2;
+    const long long inner = M * M * M, shell = S * S * S - inner;
+    const int nb_inner = (int)((inner + BLK - 1) / BLK);
+    const int nb_shell = (int)((shell + BLK - 1) / BLK);
+    k_fdtd_abc<<<nb_inner + nb_shell, BLK, 0, s>>>(grid3, p, st, nb_inner);
+    return hipGetLastError();
+}
+
+}  // namespace pppm_amd

@@ -1,0 +1,1127 @@
+// C ABI of the MI355X PPPM library: state, orchestration of the build and step
+// kernels, HIP-graph replay of the time step. Declarations: include/pppm_hip.h.
+//
+// Reference behaviour: PPPMSolver (src/algorithm/pppm.h:11-164), ParticleGrid::init /
+// set_mesh / update_mesh / construct_neighbor_lists (src/algorithm/particle_grid.h:114-148,
+// particle_grid.cu:133-241), GridCache / FaceCache::update_cache (pppm_cache.cu:125-134,
+// 273-289), FDTD::init / step / reset (src/algorithm/fdtd.h:45-110).
+#include "pppm_hip.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace pppm_amd;
+
+namespace
+{
+
+template <class T>
+struct DBuf
+{
+    T *p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t m)
+    {
+        if (m == 0)
+            m = 1;
+        if (p && m <= n)
+            return hipSuccess;
+        release();
+        hipError_t e = hipMalloc((void **)&p, m * sizeof(T));
+        if (e != hipSuccess)
+        {
+            p = nullptr;
+            n = 0;
+            return e;
+        }
+        n = m;
+        return hipSuccess;
+    }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct LWH72
+{
+    uint16_t s[STEPS];
+    float ss;
+    uint16_t d[STEPS];
+    float sd;
+};
+static_assert(sizeof(LWH72) == 72, "LayerWeightHalf layout");
+
+}  // namespace
+
+struct pppm_solver
+{
+    int N = 0;
+    float h = 0, dt = 0, c = 0, rc = 0;
+    float minp[3] = {0, 0, 0};
+    int pml_width = 0;
+    unsigned flags = 0;
+    bool use_pml = false;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    int t_host = -1;
+    Tdbem bem;
+
+    DBuf<float> grid, pmlbuf;
+    DBuf<StepState> state;
+
+    int nv = 0, nt = 0;
+    bool mesh_set = false;
+    DBuf<float> verts;
+    DBuf<int> tris, err_flag;
+    DBuf<TriGeom> geom;
+
+    DBuf<int> cell_cnt, base_cnt, cell_start, base_start, cell_fill, base_fill, cell_items, base_items;
+    DBuf<int> flag, pos, scan_tmp;
+    int n_c3 = 0, n_p3 = 0, n_b4 = 0, n_p4 = 0;
+    DBuf<int> c3_cells, c3_index, c3_cnt, c3_start, c3_pairs;
+    DBuf<int> b4_cells, b4_index, b4_cnt, b4_start, b4_tris;
+    DBuf<int> face_cnt, face_start, face_nbr, face_slot, face_owner, face_dirty, dirty_list;
+    DBuf<int> corner;
+
+    DBuf<uint32_t> wf, wb;
+    DBuf<float2> sf, sb;
+    DBuf<float> gmeta, interp;
+    DBuf<int> gflag, glist;
+    DBuf<double> denom, ftmp;
+
+    int pool_cap = 0, hcap = 0, pool_count = 0;
+    DBuf<uint32_t> fw;
+    DBuf<float2> fsc, fmeta;
+    DBuf<unsigned long long> hkeys;
+    DBuf<int> hvals, pool_counter;
+
+    DBuf<float> hn, hd, far, nstage;
+    DBuf<int> lcell, lc3;
+    long long last_grid_recomputes = 0, last_face_recomputes = 0;
+
+    // per-phase GPU timing with HIP events on the handle's stream (the reference's
+    // log_time / LOG_TIME per phase, src/core/macro.h:35-48)
+    bool timing = false;
+    struct Span
+    {
+        int phase;
+        hipEvent_t a, b;
+    };
+    std::vector<Span> spans;
+    std::vector<hipEvent_t> pool;
+    double phase_ms[8] = {0};
+    long long phase_n[8] = {0};
+};
+
+namespace
+{
+
+int fail(pppm_solver *s, int code, const std::string &msg)
+{
+    if (s)
+        s->err = msg;
+    return code;
+}
+
+#define CK(expr)                                                                                 \
+    do                                                                                           \
+    {                                                                                            \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail(s, PPPM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+FdtdParams fdtd_params(const pppm_solver *s)
+{
+    FdtdParams p;
+    p.N = s->N;
+    const double c = s->c, dt = s->dt, h = s->h;
+    p.coef = c * c * dt * dt;
+    p.hh = h * h;
+    p.c = s->c;
+    p.dt = s->dt;
+    p.h = s->h;
+    p.pml = s->use_pml ? s->pml_width : 0;
+    p.damp = 2.0f / s->h;
+    return p;
+}
+
+WeightArgs weight_args(const pppm_solver *s)
+{
+    WeightArgs a;
+    a.N = s->N;
+    a.nt = s->nt;
+    a.h = s->h;
+    a.c = s->c;
+    a.dt = s->dt;
+    a.minx = s->minp[0];
+    a.miny = s->minp[1];
+    a.minz = s->minp[2];
+    return a;
+}
+
+StepArgs step_args(const pppm_solver *s, int t0, long long stride, int n_listen)
+{
+    StepArgs a;
+    a.N = s->N;
+    a.nt = s->nt;
+    a.n_c3 = s->n_c3;
+    a.rc = s->rc;
+    a.t0 = t0;
+    a.stride = stride;
+    a.n_listen = n_listen;
+    return a;
+}
+
+int read_int(pppm_solver *s, const int *dev, int *out)
+{
+    CK(hipMemcpyAsync(out, dev, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    CK(hipStreamSynchronize(s->stream));
+    return PPPM_OK;
+}
+
+// exclusive scan of n ints into out (n+1 entries); returns the total on the host
+int scan_total(pppm_solver *s, const int *in, int *out, int n, int *total)
+{
+    CK(s->scan_tmp.ensure(scan_tmp_ints(n)));
+    CK(scan_exclusive(in, out, n, s->scan_tmp.p, s->stream));
+    return read_int(s, out + n, total);
+}
+
+// grows the face-cache pool (keeping its contents) and the hash table (rehashing)
+int ensure_pool(pppm_solver *s, int need)
+{
+    if (need > s->pool_cap)
+    {
+        int ncap = std::max(need, s->pool_cap * 2);
+        DBuf<uint32_t> nfw;
+        DBuf<float2> nfsc, nfmeta;
+        CK(nfw.ensure((size_t)ncap * STEPS));
+        CK(nfsc.ensure(ncap));
+        CK(nfmeta.ensure(ncap));
+        if (s->pool_count > 0)
+        {
+            CK(hipMemcpyAsync(nfw.p, s->fw.p, (size_t)s->pool_count * STEPS * sizeof(uint32_t),
+                              hipMemcpyDeviceToDevice, s->stream));
+            CK(hipMemcpyAsync(nfsc.p, s->fsc.p, (size_t)s->pool_count * sizeof(float2), hipMemcpyDeviceToDevice,
+                              s->stream));
+            CK(hipMemcpyAsync(nfmeta.p, s->fmeta.p, (size_t)s->pool_count * sizeof(float2),
+                              hipMemcpyDeviceToDevice, s->stream));
+        }
+        CK(hipStreamSynchronize(s->stream));
+        s->fw.release();
+        s->fsc.release();
+        s->fmeta.release();
+        s->fw = nfw;
+        s->fsc = nfsc;
+        s->fmeta = nfmeta;
+        nfw.p = nullptr;
+        nfsc.p = nullptr;
+        nfmeta.p = nullptr;
+        s->pool_cap = ncap;
+    }
+    int want = 1024;
+    while (want < 2 * s->pool_cap)
+        want <<= 1;
+    if (want > s->hcap)
+    {
+        DBuf<unsigned long long> nk;
+        DBuf<int> nvv;
+        CK(nk.ensure(want));
+        CK(nvv.ensure(want));
+        CK(hipMemsetAsync(nk.p, 0xff, (size_t)want * sizeof(unsigned long long), s->stream));
+        if (s->hcap > 0)
+            CK(launch_rehash(s->hkeys.p, s->hvals.p, s->hcap, nk.p, nvv.p, want, s->stream));
+        CK(hipStreamSynchronize(s->stream));
+        s->hkeys.release();
+        s->hvals.release();
+        s->hkeys = nk;
+        s->hvals = nvv;
+        nk.p = nullptr;
+        nvv.p = nullptr;
+        s->hcap = want;
+    }
+    return PPPM_OK;
+}
+
+// Geometry, binning, lists and weight caches for the current vertex positions.
+int rebuild(pppm_solver *s, bool first)
+{
+    const int N = s->N, nt = s->nt;
+    const int NN = N * N * N;
+    hipStream_t st = s->stream;
+    int rc;
+
+    // ---- triangle geometry (update_triangles_kernel)
+    CK(s->err_flag.ensure(1));
+    CK(hipMemsetAsync(s->err_flag.p, 0, sizeof(int), st));
+    BuildArgs ba{N, nt, s->nv, s->h, s->minp[0], s->minp[1], s->minp[2]};
+    CK(launch_tri_geometry(s->verts.p, s->tris.p, s->geom.p, ba, s->err_flag.p, st));
+    int ef = 0;
+    if ((rc = read_int(s, s->err_flag.p, &ef)))
+        return rc;
+    if (ef & 2)
+        return fail(s, PPPM_ERR_ARG, "triangle references a vertex index out of range");
+    if (ef & 1)
+        return fail(s, PPPM_ERR_OUT_OF_GRID, "a triangle centre lies outside grid cells [2, N-3]");
+
+    // ---- cell / base-cell CSR
+    CK(s->cell_cnt.ensure(NN));
+    CK(s->base_cnt.ensure(NN));
+    CK(s->cell_fill.ensure(NN));
+    CK(s->base_fill.ensure(NN));
+    CK(s->cell_start.ensure(NN + 1));
+    CK(s->base_start.ensure(NN + 1));
+    CK(s->cell_items.ensure(nt));
+    CK(s->base_items.ensure(nt));
+    CK(hipMemsetAsync(s->cell_cnt.p, 0, (size_t)NN * sizeof(int), st));
+    CK(hipMemsetAsync(s->base_cnt.p, 0, (size_t)NN * sizeof(int), st));
+    CK(hipMemsetAsync(s->cell_fill.p, 0, (size_t)NN * sizeof(int), st));
+    CK(hipMemsetAsync(s->base_fill.p, 0, (size_t)NN * sizeof(int), st));
+    CK(launch_count_cells(s->geom.p, nt, s->cell_cnt.p, s->base_cnt.p, st));
+    CK(s->scan_tmp.ensure(scan_tmp_ints(NN)));
+    CK(scan_exclusive(s->cell_cnt.p, s->cell_start.p, NN, s->scan_tmp.p, st));
+    CK(scan_exclusive(s->base_cnt.p, s->base_start.p, NN, s->scan_tmp.p, st));
+    CK(launch_fill_cells(s->geom.p, nt, s->cell_start.p, s->base_start.p, s->cell_fill.p, s->base_fill.p,
+                         s->cell_items.p, s->base_items.p, st));
+    CK(launch_sort_cells(s->cell_start.p, s->cell_items.p, NN, st));
+    CK(launch_sort_cells(s->base_start.p, s->base_items.p, NN, st));
+
+    // ---- C3 cells: within +-1 of a triangle cell, linear order
+    CK(s->flag.ensure(NN));
+    CK(s->pos.ensure(NN + 1));
+    CK(hipMemsetAsync(s->flag.p, 0, (size_t)NN * sizeof(int), st));
+    CK(launch_mark27(s->geom.p, nt, N, s->flag.p, st));
+    if ((rc = scan_total(s, s->flag.p, s->pos.p, NN, &s->n_c3)))
+        return rc;
+    CK(s->c3_cells.ensure(s->n_c3));
+    CK(s->c3_index.ensure(NN));
+    CK(launch_scatter_selected(s->flag.p, s->pos.p, NN, s->c3_cells.p, s->c3_index.p, st));
+    CK(s->c3_cnt.ensure(s->n_c3));
+    CK(s->c3_start.ensure(s->n_c3 + 1));
+    CK(launch_count_nbr(s->c3_cells.p, s->n_c3, s->cell_start.p, N, 3, s->c3_cnt.p, st));
+    if ((rc = scan_total(s, s->c3_cnt.p, s->c3_start.p, s->n_c3, &s->n_p3)))
+        return rc;
+    CK(s->c3_pairs.ensure(s->n_p3));
+    CK(launch_fill_c3(s->c3_cells.p, s->n_c3, s->cell_start.p, s->cell_items.p, s->c3_start.p, N, s->c3_pairs.p, st));
+    CK(s->far.ensure(s->n_c3));
+
+    // ---- non-empty base cells and their 4x4x4 lists
+    CK(launch_flag_base(s->base_cnt.p, N, s->flag.p, st));
+    if ((rc = scan_total(s, s->flag.p, s->pos.p, NN, &s->n_b4)))
+        return rc;
+    CK(s->b4_cells.ensure(s->n_b4));
+    CK(s->b4_index.ensure(NN));
+    CK(launch_scatter_selected(s->flag.p, s->pos.p, NN, s->b4_cells.p, s->b4_index.p, st));
+    CK(s->b4_cnt.ensure(s->n_b4));
+    CK(s->b4_start.ensure(s->n_b4 + 1));
+    CK(launch_count_nbr(s->b4_cells.p, s->n_b4, s->cell_start.p, N, 4, s->b4_cnt.p, st));
+    int n_b4_items = 0;
+    if ((rc = scan_total(s, s->b4_cnt.p, s->b4_start.p, s->n_b4, &n_b4_items)))
+        return rc;
+    CK(s->b4_tris.ensure(n_b4_items));
+    CK(launch_fill_b4(s->b4_cells.p, s->n_b4, s->cell_start.p, s->cell_items.p, s->b4_start.p, N, s->b4_tris.p, st));
+
+    // ---- per-triangle face lists (the 4x4x4 list of its base cell)
+    CK(s->face_cnt.ensure(nt));
+    CK(s->face_start.ensure(nt + 1));
+    CK(launch_face_counts(s->geom.p, nt, s->b4_index.p, s->b4_start.p, s->face_cnt.p, st));
+    if ((rc = scan_total(s, s->face_cnt.p, s->face_start.p, nt, &s->n_p4)))
+        return rc;
+    CK(s->face_nbr.ensure(s->n_p4));
+    CK(s->face_slot.ensure(s->n_p4));
+    CK(s->face_owner.ensure(s->n_p4));
+    CK(s->face_dirty.ensure(s->n_p4));
+    CK(launch_face_fill(s->geom.p, nt, s->b4_index.p, s->b4_start.p, s->b4_tris.p, s->face_start.p, s->face_nbr.p,
+                        st));
+    CK(launch_face_owner(s->face_start.p, nt, s->face_owner.p, st));
+    CK(s->corner.ensure((size_t)nt * 8));
+    CK(launch_corner_map(s->geom.p, nt, N, s->c3_index.p, s->corner.p, st));
+
+    // ---- grid cache (dirty triangles only)
+    const int scratch_n = std::max(NN, std::max(nt, s->n_p4));
+    CK(s->flag.ensure(scratch_n));
+    CK(s->pos.ensure(scratch_n + 1));
+    const WeightArgs wa = weight_args(s);
+    CK(s->gflag.ensure(nt));
+    CK(s->glist.ensure(nt));
+    CK(launch_grid_dirty(s->geom.p, s->gmeta.p, nt, s->h, first ? 1 : 0, s->gflag.p, st));
+    int n_dirty = 0;
+    if ((rc = scan_total(s, s->gflag.p, s->pos.p, nt, &n_dirty)))
+        return rc;
+    CK(launch_scatter_selected(s->gflag.p, s->pos.p, nt, s->glist.p, nullptr, st));
+    CK(launch_grid_weights(s->verts.p, s->tris.p, s->geom.p, s->glist.p, n_dirty, wa, s->bem, s->wf.p, s->sf.p,
+                           s->wb.p, s->sb.p, s->gmeta.p, st));
+    s->last_grid_recomputes = n_dirty;
+
+    // ---- interpolation weights (every update)
+    CK(launch_interp_weights(s->verts.p, s->tris.p, s->geom.p, wa, s->interp.p, st));
+
+    // ---- face cache: persistent (c, j) slots, dirty pairs recomputed
+    if ((rc = ensure_pool(s, s->pool_count + s->n_p4)))
+        return rc;
+    CK(s->pool_counter.ensure(1));
+    CK(hipMemcpyAsync(s->pool_counter.p, &s->pool_count, sizeof(int), hipMemcpyHostToDevice, st));
+    CK(launch_face_lookup(s->geom.p, s->tris.p, s->face_owner.p, s->face_nbr.p, s->n_p4, s->hkeys.p, s->hvals.p,
+                          s->hcap, s->pool_counter.p, s->fmeta.p, s->face_slot.p, s->face_dirty.p, st));
+    if ((rc = read_int(s, s->pool_counter.p, &s->pool_count)))
+        return rc;
+    if (s->pool_count > s->pool_cap)
+        return fail(s, PPPM_ERR_CAPACITY, "face-cache pool overflow");
+    CK(launch_flag_nonzero(s->face_dirty.p, s->n_p4, s->flag.p, st));
+    int n_fd = 0;
+    if ((rc = scan_total(s, s->flag.p, s->pos.p, s->n_p4, &n_fd)))
+        return rc;
+    CK(s->dirty_list.ensure(n_fd));
+    CK(launch_scatter_selected(s->flag.p, s->pos.p, s->n_p4, s->dirty_list.p, nullptr, st));
+    CK(s->ftmp.ensure((size_t)n_fd * 36));
+    CK(launch_face_weights(s->verts.p, s->tris.p, s->geom.p, s->dirty_list.p, n_fd, s->face_owner.p, s->face_nbr.p,
+                           s->face_slot.p, s->interp.p, wa, s->bem, s->ftmp.p, s->fw.p, s->fsc.p, st));
+    s->last_face_recomputes = n_fd;
+    CK(launch_face_denom(s->geom.p, nt, s->face_start.p, s->face_slot.p, s->fw.p, s->fsc.p, s->denom.p, st));
+    CK(hipStreamSynchronize(st));
+    return PPPM_OK;
+}
+
+int upload(pppm_solver *s, void *dst, const void *src, size_t bytes, int on_device)
+{
+    CK(hipMemcpyAsync(dst, src, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s->stream));
+    return PPPM_OK;
+}
+
+// one update_grid_and_face on the handle's stream; neumann is device memory (row t - t0)
+hipEvent_t take_event(pppm_solver *s)
+{
+    if (!s->pool.empty())
+    {
+        hipEvent_t e = s->pool.back();
+        s->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess)
+        return nullptr;
+    return e;
+}
+
+// marks the start of a timed phase; returns the span index or -1
+int span_begin(pppm_solver *s, int phase)
+{
+    if (!s->timing)
+        return -1;
+    pppm_solver::Span sp{phase, take_event(s), take_event(s)};
+    if (!sp.a || !sp.b || hipEventRecord(sp.a, s->stream) != hipSuccess)
+        return -1;
+    s->spans.push_back(sp);
+    return (int)s->spans.size() - 1;
+}
+
+void span_end(pppm_solver *s, int idx)
+{
+    if (idx >= 0)
+        (void)hipEventRecord(s->spans[idx].b, s->stream);
+}
+
+int collect_spans(pppm_solver *s)
+{
+    if (s->spans.empty())
+        return PPPM_OK;
+    CK(hipStreamSynchronize(s->stream));
+    for (auto &sp : s->spans)
+    {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, sp.a, sp.b) == hipSuccess)
+        {
+            s->phase_ms[sp.phase] += ms;
+            s->phase_n[sp.phase]++;
+        }
+        s->pool.push_back(sp.a);
+        s->pool.push_back(sp.b);
+    }
+    s->spans.clear();
+    return PPPM_OK;
+}
+
+int enqueue_step(pppm_solver *s, const float *neumann, int t0, long long stride, int n_listen, float *out)
+{
+    const FdtdParams fp = fdtd_params(s);
+    int sp = span_begin(s, 0);
+    CK(launch_fdtd(s->grid.p, s->pmlbuf.p, fp, s->state.p, s->stream));
+    span_end(s, sp);
+    if (!s->mesh_set)
+    {
+        if (out)
+        {
+            StepArgs a = step_args(s, t0, stride, n_listen);
+            a.n_c3 = 0;
+            CK(launch_listen_direct(a, s->state.p, s->grid.p, s->lcell.p, out, s->stream));
+        }
+        return PPPM_OK;
+    }
+    const StepArgs a = step_args(s, t0, stride, n_listen);
+    sp = span_begin(s, 1);
+    CK(launch_far(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_pairs.p, s->wf.p, s->sf.p, s->hn.p,
+                  s->hd.p, s->far.p, neumann, s->n_p3, s->stream));
+    span_end(s, sp);
+    sp = span_begin(s, 2);
+    CK(launch_dirichlet(a, s->state.p, s->face_start.p, s->face_nbr.p, s->face_slot.p, s->fw.p, s->fsc.p, s->hn.p,
+                        s->hd.p, s->interp.p, s->corner.p, s->far.p, s->denom.p, s->n_p4, s->stream));
+    span_end(s, sp);
+    sp = span_begin(s, 3);
+    CK(launch_near(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_pairs.p, s->wb.p, s->sb.p, s->hn.p,
+                   s->hd.p, s->far.p, s->lcell.p, s->lc3.p, out, s->n_p3, s->stream));
+    span_end(s, sp);
+    if (s->spans.size() > 4096)
+        return collect_spans(s);
+    return PPPM_OK;
+}
+
+int check_handle(pppm_solver *s) { return s ? PPPM_OK : PPPM_ERR_ARG; }
+
+}  // namespace
+
+// ============================================================================ C ABI
+extern "C" {
+
+int pppm_create(pppm_solver **out, int res, float dl, float dt, const float *min_xyz, int pml_width,
+                float reflect_coeff, float c, unsigned flags)
+{
+    if (!out)
+        return PPPM_ERR_ARG;
+    *out = nullptr;
+    if (res < 8 || !(dl > 0) || !(dt > 0) || !(c > 0) || pml_width < 0 || 2 * pml_width >= res)
+        return PPPM_ERR_ARG;
+    pppm_solver *s = new pppm_solver();
+    s->N = res;
+    s->h = dl;
+    s->dt = dt;
+    s->c = c;
+    s->rc = reflect_coeff;
+    if (min_xyz)
+        std::memcpy(s->minp, min_xyz, 3 * sizeof(float));
+    s->pml_width = pml_width;
+    s->flags = flags;
+    s->use_pml = (flags & PPPM_FLAG_HONOR_PML) && pml_width > 0;
+    tdbem_init(s->bem, dt, c, dl);
+    hipError_t e = hipStreamCreate(&s->stream);
+    if (e != hipSuccess)
+    {
+        delete s;
+        return PPPM_ERR_HIP;
+    }
+    s->own_stream = true;
+    const size_t NN = (size_t)res * res * res;
+    if (s->grid.ensure(3 * NN) != hipSuccess || s->state.ensure(1) != hipSuccess ||
+        (s->use_pml && s->pmlbuf.ensure(3 * NN * 6) != hipSuccess))
+    {
+        pppm_destroy(s);
+        return PPPM_ERR_HIP;
+    }
+    if (pppm_reset(s) != PPPM_OK)
+    {
+        pppm_destroy(s);
+        return PPPM_ERR_HIP;
+    }
+    *out = s;
+    return PPPM_OK;
+}
+
+int pppm_destroy(pppm_solver *s)
+{
+    if (!s)
+        return PPPM_ERR_ARG;
+    if (s->stream)
+        (void)hipStreamSynchronize(s->stream);
+    DBuf<float> *fb[] = {&s->grid, &s->pmlbuf, &s->verts, &s->gmeta, &s->interp, &s->hn, &s->hd, &s->far, &s->nstage};
+    for (auto b : fb)
+        b->release();
+    DBuf<int> *ib[] = {&s->tris,       &s->err_flag,  &s->cell_cnt,   &s->base_cnt,   &s->cell_start, &s->base_start,
+                       &s->cell_fill,  &s->base_fill, &s->cell_items, &s->base_items, &s->flag,       &s->pos,
+                       &s->scan_tmp,   &s->c3_cells,  &s->c3_index,   &s->c3_cnt,     &s->c3_start,   &s->c3_pairs,
+                       &s->b4_cells,   &s->b4_index,  &s->b4_cnt,     &s->b4_start,   &s->b4_tris,    &s->face_cnt,
+                       &s->face_start, &s->face_nbr,  &s->face_slot,  &s->face_owner, &s->face_dirty, &s->dirty_list,
+                       &s->corner,     &s->gflag,     &s->glist,      &s->hvals,      &s->pool_counter, &s->lcell,
+                       &s->lc3};
+    for (auto b : ib)
+        b->release();
+    s->state.release();
+    s->geom.release();
+    s->wf.release();
+    s->wb.release();
+    s->sf.release();
+    s->sb.release();
+    s->denom.release();
+    s->ftmp.release();
+    s->fw.release();
+    s->fsc.release();
+    s->fmeta.release();
+    s->hkeys.release();
+    for (auto &sp : s->spans)
+    {
+        (void)hipEventDestroy(sp.a);
+        (void)hipEventDestroy(sp.b);
+    }
+    for (auto e : s->pool)
+        (void)hipEventDestroy(e);
+    if (s->own_stream && s->stream)
+        (void)hipStreamDestroy(s->stream);
+    delete s;
+    return PPPM_OK;
+}
+
+int pppm_set_stream(pppm_solver *s, void *hip_stream)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    CK(hipStreamSynchronize(s->stream));
+    if (s->own_stream)
+        CK(hipStreamDestroy(s->stream));
+    s->own_stream = false;
+    s->stream = (hipStream_t)hip_stream;
+    return PPPM_OK;
+}
+
+const char *pppm_last_error(const pppm_solver *s) { return s ? s->err.c_str() : "null handle"; }
+
+int pppm_set_mesh(pppm_solver *s, const float *verts, int nv, const int *tris, int nt, int on_device)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    if (!verts || !tris || nv < 3 || nt < 1)
+        return fail(s, PPPM_ERR_ARG, "set_mesh: empty mesh");
+    s->nv = nv;
+    s->nt = nt;
+    s->mesh_set = false;
+    CK(s->verts.ensure((size_t)nv * 3));
+    CK(s->tris.ensure((size_t)nt * 3));
+    CK(s->geom.ensure(nt));
+    int rc;
+    if ((rc = upload(s, s->verts.p, verts, (size_t)nv * 3 * sizeof(float), on_device)))
+        return rc;
+    if ((rc = upload(s, s->tris.p, tris, (size_t)nt * 3 * sizeof(int), on_device)))
+        return rc;
+    // fresh caches (GridCache::init / FaceCache::init, pppm_cache.h:29-33, 115-124)
+    CK(s->wf.ensure((size_t)nt * GC * STEPS));
+    CK(s->wb.ensure((size_t)nt * GC * STEPS));
+    CK(s->sf.ensure((size_t)nt * GC));
+    CK(s->sb.ensure((size_t)nt * GC));
+    CK(s->gmeta.ensure((size_t)nt * 7));
+    CK(s->interp.ensure((size_t)nt * 8));
+    CK(s->denom.ensure(nt));
+    s->pool_count = 0;
+    if (s->hcap > 0)
+        CK(hipMemsetAsync(s->hkeys.p, 0xff, (size_t)s->hcap * sizeof(unsigned long long), s->stream));
+    if ((rc = rebuild(s, true)))
+        return rc;
+    // dirichlet / neumann histories reset (pppm.h:88-91)
+    CK(s->hn.ensure((size_t)nt * RING));
+    CK(s->hd.ensure((size_t)nt * RING));
+    CK(s->nstage.ensure(nt));
+    CK(hipMemsetAsync(s->hn.p, 0, (size_t)nt * RING * sizeof(float), s->stream));
+    CK(hipMemsetAsync(s->hd.p, 0, (size_t)nt * RING * sizeof(float), s->stream));
+    CK(hipStreamSynchronize(s->stream));
+    s->mesh_set = true;
+    return PPPM_OK;
+}
+
+int pppm_update_mesh(pppm_solver *s, const float *verts, int on_device)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    if (!s->mesh_set)
+        return fail(s, PPPM_ERR_STATE, "update_mesh before set_mesh");
+    int rc;
+    if ((rc = upload(s, s->verts.p, verts, (size_t)s->nv * 3 * sizeof(float), on_device)))
+        return rc;
+    const int sp = span_begin(s, 4);
+    rc = rebuild(s, false);
+    span_end(s, sp);
+    return rc;
+}
+
+int pppm_set_timing(pppm_solver *s, int enable)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    int rc = collect_spans(s);
+    s->timing = enable != 0;
+    return rc;
+}
+
+int pppm_get_timing(pppm_solver *s, double *ms8, long long *count8)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    int rc = collect_spans(s);
+    if (rc)
+        return rc;
+    for (int i = 0; i < 8; i++)
+    {
+        if (ms8)
+            ms8[i] = s->phase_ms[i];
+        if (count8)
+            count8[i] = s->phase_n[i];
+        s->phase_ms[i] = 0;
+        s->phase_n[i] = 0;
+    }
+    return PPPM_OK;
+}
+
+int pppm_step(pppm_solver *s, const float *neumann, int on_device)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    if (!s->mesh_set)
+        return fail(s, PPPM_ERR_STATE, "step before set_mesh");
+    const float *nd = neumann;
+    int rc;
+    if (neumann && !on_device)
+    {
+        if ((rc = upload(s, s->nstage.p, neumann, (size_t)s->nt * sizeof(float), 0)))
+            return rc;
+        nd = s->nstage.p;
+    }
+    const int t_new = s->t_host + 1;
+    if ((rc = enqueue_step(s, nd, t_new, 0, 0, nullptr)))
+        return rc;
+    s->t_host = t_new;
+    return PPPM_OK;
+}
+
+int pppm_run(pppm_solver *s, const float *neumann_series, long long stride, int steps, const int *listen_xyz,
+             int n_listen, float *out_dev, int use_graph)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    if (steps < 0 || n_listen < 0 || n_listen > 64 || (n_listen > 0 && (!listen_xyz || !out_dev)))
+        return fail(s, PPPM_ERR_ARG, "run: bad arguments (at most 64 listeners)");
+    if (s->mesh_set && !neumann_series)
+        return fail(s, PPPM_ERR_ARG, "run: neumann_series required");
+    if (steps == 0)
+        return PPPM_OK;
+    const int N = s->N;
+    std::vector<int> cells(n_listen), c3(n_listen, -1);
+    for (int l = 0; l < n_listen; l++)
+    {
+        const int x = listen_xyz[3 * l], y = listen_xyz[3 * l + 1], z = listen_xyz[3 * l + 2];
+        if (x < 0 || y < 0 || z < 0 || x >= N || y >= N || z >= N)
+            return fail(s, PPPM_ERR_ARG, "run: listener outside the grid");
+        cells[l] = lin3(x, y, z, N);
+        if (s->mesh_set)
+        {
+            int v = -1;
+            int rc = read_int(s, s->c3_index.p + cells[l], &v);
+            if (rc)
+                return rc;
+            c3[l] = v;
+        }
+    }
+    CK(s->lcell.ensure(std::max(n_listen, 1)));
+    CK(s->lc3.ensure(std::max(n_listen, 1)));
+    if (n_listen > 0)
+    {
+        CK(hipMemcpyAsync(s->lcell.p, cells.data(), n_listen * sizeof(int), hipMemcpyHostToDevice, s->stream));
+        CK(hipMemcpyAsync(s->lc3.p, c3.data(), n_listen * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    }
+    const int t0 = s->t_host + 1;
+    float *out = n_listen > 0 ? out_dev : nullptr;
+    int rc;
+    if (!use_graph)
+    {
+        for (int i = 0; i < steps; i++)
+            if ((rc = enqueue_step(s, neumann_series, t0, stride, n_listen, out)))
+                return rc;
+    }
+    else
+    {
+        // one captured graph holds `chunk` steps; kernels read t from device memory.
+        // Event timing is not captured: spans are only recorded by eager launches.
+        const bool timing = s->timing;
+        s->timing = false;
+        struct Restore
+        {
+            pppm_solver *s;
+            bool v;
+            ~Restore() { s->timing = v; }
+        } restore{s, timing};
+        const int chunk = std::min(steps, 32);
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        CK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < chunk; i++)
+        {
+            rc = enqueue_step(s, neumann_series, t0, stride, n_listen, out);
+            if (rc)
+            {
+                hipGraph_t tmp;
+                (void)hipStreamEndCapture(s->stream, &tmp);
+                if (tmp)
+                    (void)hipGraphDestroy(tmp);
+                return rc;
+            }
+        }
+        CK(hipStreamEndCapture(s->stream, &g));
+        CK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        int done = 0;
+        while (done + chunk <= steps)
+        {
+            CK(hipGraphLaunch(ge, s->stream));
+            done += chunk;
+        }
+        for (; done < steps; done++)
+            if ((rc = enqueue_step(s, neumann_series, t0, stride, n_listen, out)))
+                return rc;
+        CK(hipStreamSynchronize(s->stream));
+        (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(g);
+    }
+    s->t_host += steps;
+    return PPPM_OK;
+}
+
+int pppm_fdtd_step(pppm_solver *s)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    const FdtdParams fp = fdtd_params(s);
+    CK(launch_fdtd(s->grid.p, s->pmlbuf.p, fp, s->state.p, s->stream));
+    s->t_host++;
+    return PPPM_OK;
+}
+
+int pppm_solve_far(pppm_solver *s)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    if (!s->mesh_set)
+        return fail(s, PPPM_ERR_STATE, "solve_far before set_mesh");
+    const StepArgs a = step_args(s, s->t_host, 0, 0);
+    CK(launch_far(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_pairs.p, s->wf.p, s->sf.p, s->hn.p,
+                  s->hd.p, s->far.p, nullptr, s->n_p3, s->stream));
+    return PPPM_OK;
+}
+
+int pppm_set_neumann(pppm_solver *s, const float *neumann, int on_device)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    if (!s->mesh_set || !neumann)
+        return fail(s, PPPM_ERR_STATE, "set_neumann before set_mesh");
+    const float *nd = neumann;
+    int rc;
+    if (!on_device)
+    {
+        if ((rc = upload(s, s->nstage.p, neumann, (size_t)s->nt * sizeof(float), 0)))
+            return rc;
+        nd = s->nstage.p;
+    }
+    const StepArgs a = step_args(s, s->t_host, 0, 0);
+    CK(launch_set_neumann(a, s->state.p, s->hn.p, nd, s->stream));
+    return PPPM_OK;
+}
+
+int pppm_update_dirichlet(pppm_solver *s)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    if (!s->mesh_set)
+        return fail(s, PPPM_ERR_STATE, "update_dirichlet before set_mesh");
+    const StepArgs a = step_args(s, s->t_host, 0, 0);
+    CK(launch_dirichlet(a, s->state.p, s->face_start.p, s->face_nbr.p, s->face_slot.p, s->fw.p, s->fsc.p, s->hn.p,
+                        s->hd.p, s->interp.p, s->corner.p, s->far.p, s->denom.p, s->n_p4, s->stream));
+    return PPPM_OK;
+}
+
+int pppm_solve_near(pppm_solver *s)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    if (!s->mesh_set)
+        return fail(s, PPPM_ERR_STATE, "solve_near before set_mesh");
+    const StepArgs a = step_args(s, s->t_host, 0, 0);
+    CK(launch_near(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_pairs.p, s->wb.p, s->sb.p, s->hn.p,
+                   s->hd.p, s->far.p, nullptr, nullptr, nullptr, s->n_p3, s->stream));
+    return PPPM_OK;
+}
+
+int pppm_reset(pppm_solver *s)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    const size_t NN = (size_t)s->N * s->N * s->N;
+    CK(hipMemsetAsync(s->grid.p, 0, 3 * NN * sizeof(float), s->stream));
+    if (s->use_pml)
+        CK(hipMemsetAsync(s->pmlbuf.p, 0, 3 * NN * 6 * sizeof(float), s->stream));
+    if (s->mesh_set)
+    {
+        CK(hipMemsetAsync(s->hn.p, 0, (size_t)s->nt * RING * sizeof(float), s->stream));
+        CK(hipMemsetAsync(s->hd.p, 0, (size_t)s->nt * RING * sizeof(float), s->stream));
+    }
+    StepState init{-1, 0u};
+    CK(hipMemcpyAsync(s->state.p, &init, sizeof(init), hipMemcpyHostToDevice, s->stream));
+    CK(hipStreamSynchronize(s->stream));
+    s->t_host = -1;
+    return PPPM_OK;
+}
+
+int pppm_time_idx(const pppm_solver *s, int *t)
+{
+    if (!s || !t)
+        return PPPM_ERR_ARG;
+    *t = s->t_host;
+    return PPPM_OK;
+}
+
+int pppm_grid_ptr(pppm_solver *s, int rel, float **dev_ptr)
+{
+    if (check_handle(s) || !dev_ptr)
+        return PPPM_ERR_ARG;
+    const size_t NN = (size_t)s->N * s->N * s->N;
+    const int l = (((s->t_host + rel) % 3) + 3) % 3;
+    *dev_ptr = s->grid.p + (size_t)l * NN;
+    return PPPM_OK;
+}
+
+int pppm_read_cells(pppm_solver *s, const int *xyz, int n, float *host_out)
+{
+    if (check_handle(s) || (n > 0 && (!xyz || !host_out)))
+        return PPPM_ERR_ARG;
+    float *g = nullptr;
+    pppm_grid_ptr(s, 0, &g);
+    const int N = s->N;
+    for (int i = 0; i < n; i++)
+    {
+        const int x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        if (x < 0 || y < 0 || z < 0 || x >= N || y >= N || z >= N)
+            return fail(s, PPPM_ERR_ARG, "read_cells: cell outside the grid");
+        CK(hipMemcpyAsync(host_out + i, g + lin3(x, y, z, N), sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    }
+    CK(hipStreamSynchronize(s->stream));
+    return PPPM_OK;
+}
+
+int pppm_history_ptrs(pppm_solver *s, float **neumann, float **dirichlet)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    if (!s->mesh_set)
+        return fail(s, PPPM_ERR_STATE, "no mesh");
+    if (neumann)
+        *neumann = s->hn.p;
+    if (dirichlet)
+        *dirichlet = s->hd.p;
+    return PPPM_OK;
+}
+
+int pppm_triangles(pppm_solver *s, float *host13)
+{
+    if (check_handle(s) || !host13)
+        return PPPM_ERR_ARG;
+    if (!s->mesh_set)
+        return fail(s, PPPM_ERR_STATE, "no mesh");
+    std::vector<TriGeom> g(s->nt);
+    CK(hipMemcpyAsync(g.data(), s->geom.p, (size_t)s->nt * sizeof(TriGeom), hipMemcpyDeviceToHost, s->stream));
+    CK(hipStreamSynchronize(s->stream));
+    for (int i = 0; i < s->nt; i++)
+    {
+        const TriGeom &t = g[i];
+        float v[13] = {t.center.x, t.center.y, t.center.z, t.normal.x, t.normal.y, t.normal.z, t.area,
+                       (float)t.cell3.x, (float)t.cell3.y, (float)t.cell3.z, (float)t.base3.x, (float)t.base3.y,
+                       (float)t.base3.z};
+        std::memcpy(host13 + 13 * i, v, sizeof(v));
+    }
+    return PPPM_OK;
+}
+
+int pppm_stats(pppm_solver *s, long long *o)
+{
+    if (check_handle(s) || !o)
+        return PPPM_ERR_ARG;
+    o[0] = s->nt;
+    o[1] = s->n_c3;
+    o[2] = s->n_p3;
+    o[3] = s->n_b4;
+    o[4] = s->n_p4;
+    o[5] = s->last_grid_recomputes;
+    o[6] = s->last_face_recomputes;
+    o[7] = s->pool_count;
+    return PPPM_OK;
+}
+
+static void to72(const uint32_t *w16, float2 sc, LWH72 &r)
+{
+    for (int k = 0; k < STEPS; k++)
+    {
+        r.s[k] = (uint16_t)(w16[k] & 0xffffu);
+        r.d[k] = (uint16_t)(w16[k] >> 16);
+    }
+    r.ss = sc.x;
+    r.sd = sc.y;
+}
+static void from72(const LWH72 &r, uint32_t *w16, float2 &sc)
+{
+    for (int k = 0; k < STEPS; k++)
+        w16[k] = (uint32_t)r.s[k] | ((uint32_t)r.d[k] << 16);
+    sc = make_float2(r.ss, r.sd);
+}
+
+int pppm_get_grid_weights(pppm_solver *s, void *host_recs72)
+{
+    if (check_handle(s) || !host_recs72)
+        return PPPM_ERR_ARG;
+    if (!s->mesh_set)
+        return fail(s, PPPM_ERR_STATE, "no mesh");
+    const size_t R = (size_t)s->nt * GC;
+    std::vector<uint32_t> wf(R * STEPS), wb(R * STEPS);
+    std::vector<float2> sf(R), sb(R);
+    CK(hipMemcpyAsync(wf.data(), s->wf.p, R * STEPS * 4, hipMemcpyDeviceToHost, s->stream));
+    CK(hipMemcpyAsync(wb.data(), s->wb.p, R * STEPS * 4, hipMemcpyDeviceToHost, s->stream));
+    CK(hipMemcpyAsync(sf.data(), s->sf.p, R * sizeof(float2), hipMemcpyDeviceToHost, s->stream));
+    CK(hipMemcpyAsync(sb.data(), s->sb.p, R * sizeof(float2), hipMemcpyDeviceToHost, s->stream));
+    CK(hipStreamSynchronize(s->stream));
+    LWH72 *o = (LWH72 *)host_recs72;
+    for (size_t r = 0; r < R; r++)
+    {
+        to72(&wf[r * STEPS], sf[r], o[2 * r]);
+        to72(&wb[r * STEPS], sb[r], o[2 * r + 1]);
+    }
+    return PPPM_OK;
+}
+
+int pppm_set_grid_weights(pppm_solver *s, const void *host_recs72)
+{
+    if (check_handle(s) || !host_recs72)
+        return PPPM_ERR_ARG;
+    if (!s->mesh_set)
+        return fail(s, PPPM_ERR_STATE, "no mesh");
+    const size_t R = (size_t)s->nt * GC;
+    std::vector<uint32_t> wf(R * STEPS), wb(R * STEPS);
+    std::vector<float2> sf(R), sb(R);
+    const LWH72 *in = (const LWH72 *)host_recs72;
+    for (size_t r = 0; r < R; r++)
+    {
+        from72(in[2 * r], &wf[r * STEPS], sf[r]);
+        from72(in[2 * r + 1], &wb[r * STEPS], sb[r]);
+    }
+    CK(hipMemcpyAsync(s->wf.p, wf.data(), R * STEPS * 4, hipMemcpyHostToDevice, s->stream));
+    CK(hipMemcpyAsync(s->wb.p, wb.data(), R * STEPS * 4, hipMemcpyHostToDevice, s->stream));
+    CK(hipMemcpyAsync(s->sf.p, sf.data(), R * sizeof(float2), hipMemcpyHostToDevice, s->stream));
+    CK(hipMemcpyAsync(s->sb.p, sb.data(), R * sizeof(float2), hipMemcpyHostToDevice, s->stream));
+    CK(hipStreamSynchronize(s->stream));
+    return PPPM_OK;
+}
+
+int pppm_get_face_weights(pppm_solver *s, int *host_start, int *host_nbr, void *host_recs72)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    if (!s->mesh_set)
+        return fail(s, PPPM_ERR_STATE, "no mesh");
+    const int P = s->n_p4;
+    std::vector<int> start(s->nt + 1), nbr(P), slot(P);
+    CK(hipMemcpyAsync(start.data(), s->face_start.p, (s->nt + 1) * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    CK(hipMemcpyAsync(nbr.data(), s->face_nbr.p, P * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    CK(hipMemcpyAsync(slot.data(), s->face_slot.p, P * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    std::vector<uint32_t> fw((size_t)s->pool_count * STEPS);
+    std::vector<float2> fsc(s->pool_count);
+    if (s->pool_count > 0)
+    {
+        CK(hipMemcpyAsync(fw.data(), s->fw.p, fw.size() * 4, hipMemcpyDeviceToHost, s->stream));
+        CK(hipMemcpyAsync(fsc.data(), s->fsc.p, fsc.size() * sizeof(float2), hipMemcpyDeviceToHost, s->stream));
+    }
+    CK(hipStreamSynchronize(s->stream));
+    if (host_start)
+        std::memcpy(host_start, start.data(), start.size() * sizeof(int));
+    if (host_nbr)
+        std::memcpy(host_nbr, nbr.data(), nbr.size() * sizeof(int));
+    if (host_recs72)
+    {
+        LWH72 *o = (LWH72 *)host_recs72;
+        for (int p = 0; p < P; p++)
+            to72(&fw[(size_t)slot[p] * STEPS], fsc[slot[p]], o[p]);
+    }
+    return PPPM_OK;
+}
+
+int pppm_set_face_weights(pppm_solver *s, const void *host_recs72)
+{
+    if (check_handle(s) || !host_recs72)
+        return PPPM_ERR_ARG;
+    if (!s->mesh_set)
+        return fail(s, PPPM_ERR_STATE, "no mesh");
+    const int P = s->n_p4;
+    std::vector<int> slot(P);
+    CK(hipMemcpyAsync(slot.data(), s->face_slot.p, P * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    std::vector<uint32_t> fw((size_t)s->pool_count * STEPS);
+    std::vector<float2> fsc(s->pool_count);
+    CK(hipMemcpyAsync(fw.data(), s->fw.p, fw.size() * 4, hipMemcpyDeviceToHost, s->stream));
+    CK(hipMemcpyAsync(fsc.data(), s->fsc.p, fsc.size() * sizeof(float2), hipMemcpyDeviceToHost, s->stream));
+    CK(hipStreamSynchronize(s->stream));
+    const LWH72 *in = (const LWH72 *)host_recs72;
+    for (int p = 0; p < P; p++)
+        from72(in[p], &fw[(size_t)slot[p] * STEPS], fsc[slot[p]]);
+    CK(hipMemcpyAsync(s->fw.p, fw.data(), fw.size() * 4, hipMemcpyHostToDevice, s->stream));
+    CK(hipMemcpyAsync(s->fsc.p, fsc.data(), fsc.size() * sizeof(float2), hipMemcpyHostToDevice, s->stream));
+    CK(launch_face_denom(s->geom.p, s->nt, s->face_start.p, s->face_slot.p, s->fw.p, s->fsc.p, s->denom.p, s->stream));
+    CK(hipStreamSynchronize(s->stream));
+    return PPPM_OK;
+}
+
+int pppm_get_interp_weights(pppm_solver *s, float *host_out8)
+{
+    if (check_handle(s) || !host_out8)
+        return PPPM_ERR_ARG;
+    if (!s->mesh_set)
+        return fail(s, PPPM_ERR_STATE, "no mesh");
+    CK(hipMemcpyAsync(host_out8, s->interp.p, (size_t)s->nt * 8 * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    CK(hipStreamSynchronize(s->stream));
+    return PPPM_OK;
+}
+
+int pppm_get_c3(pppm_solver *s, int *cells, int *start, int *pairs)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    if (!s->mesh_set)
+        return fail(s, PPPM_ERR_STATE, "no mesh");
+    if (cells)
+        CK(hipMemcpyAsync(cells, s->c3_cells.p, s->n_c3 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    if (start)
+        CK(hipMemcpyAsync(start, s->c3_start.p, (s->n_c3 + 1) * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    if (pairs)
+        CK(hipMemcpyAsync(pairs, s->c3_pairs.p, (size_t)s->n_p3 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    CK(hipStreamSynchronize(s->stream));
+    return PPPM_OK;
+}
+
+int pppm_get_b4(pppm_solver *s, int *cells, int *start, int *tris)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    if (!s->mesh_set)
+        return fail(s, PPPM_ERR_STATE, "no mesh");
+    int items = 0;
+    int rc = read_int(s, s->b4_start.p + s->n_b4, &items);
+    if (rc)
+        return rc;
+    if (cells)
+        CK(hipMemcpyAsync(cells, s->b4_cells.p, s->n_b4 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    if (start)
+        CK(hipMemcpyAsync(start, s->b4_start.p, (s->n_b4 + 1) * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    if (tris)
+        CK(hipMemcpyAsync(tris, s->b4_tris.p, (size_t)items * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    CK(hipStreamSynchronize(s->stream));
+    return PPPM_OK;
+}
+
+}  // extern "C"

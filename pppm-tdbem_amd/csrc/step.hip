@@ -1,0 +1,336 @@
+// Per-step near-field kernels: far-field extraction, Dirichlet update, near-field
+// correction. Sparse and HBM-bound: no MFMA.
+//
+// Reference behaviour: solve_fdtd_far_kernel (src/algorithm/pppm.cu:8-51),
+// set_neumann_condition_kernel (pppm.cu:206-225), solve_face_far_kernel,
+// solve_face_near_kernel and update_dirichlet_kernel (pppm.cu:116-204),
+// solve_fdtd_near_kernel (pppm.cu:53-114), LayerWeightHalf::convolution (bem.h:203-213).
+//
+// Segmented gather with a fixed summation order. A "segment" is a C3 cell (far / near)
+// or a triangle (Dirichlet); its pairs are contiguous in a CSR list. Wave w owns the
+// segments whose first pair lies in [64w, 64w+64) — at most 64, one leader lane each.
+// The wave walks the owned pair range in rounds of 64: lane i evaluates the 16-tap
+// convolution of pair (round base + i) sequentially (k = 0..15, as the reference's
+// loop), parks the value in LDS, and every leader adds the values of its own pairs in
+// list order. Every sum therefore has the order of a sequential loop, whatever the
+// segment lengths, and lanes stay busy across segment boundaries.
+//
+// Weights: 64 B record per pair (16 x half2 {S_k, D_k}) + float2 scales. History: the
+// reference's 32-slot rings (slot = t mod 32), one 128 B line per triangle and field.
+//
+// Time index: all kernels read t from device memory (StepState), so one captured HIP
+// graph replays any step.
+#include "internal.hpp"
+
+namespace pppm_amd
+{
+
+namespace
+{
+constexpr int WPB = 4;            // waves per block
+constexpr int TB = WPB * WAVE;    // threads per block
+
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// smallest s in [0, n] with start[s] >= p (start has n+1 non-decreasing entries)
+__device__ inline int lower_seg(const int *__restrict__ start, int n, int p)
+{
+    int lo = 0, hi = n;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi) >> 1;
+        if (start[mid] >= p)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+// LayerWeightHalf::convolution (bem.h:203-213) for one pair, taps k0..15 in order.
+// dzero_tap0: the Dirichlet value of step t is read as 0 (pppm.cu:143).
+template <int K0, bool DZERO0>
+__device__ inline float conv_pair(const uint32_t *__restrict__ w16, float2 sc, const float *__restrict__ nring,
+                                  const float *__restrict__ dring, int t)
+{
+    uint4 q[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        q[i] = reinterpret_cast<const uint4 *>(w16)[i];
+    const uint32_t *w = reinterpret_cast<const uint32_t *>(q);
+    float r = 0.0f;
+#pragma unroll
+    for (int k = K0; k < STEPS; k++)
+    {
+        const int slot = (t - k) & (RING - 1);
+        const float ws = f16_bits_to_f32(w[k]);
+        const float wd = f16_bits_to_f32(w[k] >> 16);
+        const float n = nring[slot];
+        const float d = (DZERO0 && k == 0) ? 0.0f : dring[slot];
+        const float a = -ws * sc.x * n;
+        const float b = wd * sc.y * d;
+        r += a + b;
+    }
+    return r;
+}
+
+// far(c) = p(c) - sum_pairs W^fdtd (*) (N, D). W^fdtd has tap 0 == 0 (pppm_cache.cu:111),
+// so taps 1..15 only (adding the zero tap leaves the sum bit-identical); the Neumann slot
+// of step t is therefore free to be written by the extra waves of this launch.
+__global__ __launch_bounds__(TB) void k_far(StepArgs a, const StepState *__restrict__ st,
+                                            const float *__restrict__ grid3, const int *__restrict__ c3_cells,
+                                            const int *__restrict__ c3_start, const int *__restrict__ c3_pairs,
+                                            const uint32_t *__restrict__ wf, const float2 *__restrict__ sf,
+                                            float *__restrict__ hn, const float *__restrict__ hd,
+                                            float *__restrict__ far, const float *__restrict__ neumann,
+                                            int gather_waves)
+{
+    __shared__ float vals[WPB][WAVE];
+    const int t = st->t;
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * WPB + wid;
+    if (wave < gather_waves)
+    {
+        const int c0 = lower_seg(c3_start, a.n_c3, wave * WAVE);
+        const int c1 = lower_seg(c3_start, a.n_c3, wave * WAVE + WAVE);
+        if (c0 >= c1)
+            return;
+        const int q0 = c3_start[c0], q1 = c3_start[c1];
+        const int cs = c0 + lane;
+        const bool leader = cs < c1;
+        const int m0 = leader ? c3_start[cs] : 0, m1 = leader ? c3_start[cs + 1] : 0;
+        float acc = 0.0f;
+        for (int base = q0; base < q1; base += WAVE)
+        {
+            const int p = base + lane;
+            float v = 0.0f;
+            if (p < q1)
+            {
+                const int rec = c3_pairs[p];
+                const int tri = rec / GC;
+                v = conv_pair<1, false>(wf + (size_t)rec * STEPS, sf[rec], hn + tri * RING, hd + tri * RING, t);
+            }
+            vals[wid][lane] = v;
+            wave_sync();
+            if (leader)
+            {
+                const int e = min(m1, base + WAVE);
+                for (int q = max(m0, base); q < e; q++)
+                    acc += vals[wid][q - base];
+            }
+            wave_sync();
+        }
+        if (leader)
+        {
+            const size_t nn = (size_t)a.N * a.N * a.N;
+            const float *g = grid3 + (size_t)(((t % 3) + 3) % 3) * nn;
+            far[cs] = g[c3_cells[cs]] - acc;
+        }
+    }
+    else
+    {
+        // Neumann input of step t into the ring (set_neumann_condition_kernel)
+        const int i = (wave - gather_waves) * WAVE + lane;
+        if (neumann && i < a.nt)
+            hn[i * RING + (t & (RING - 1))] = neumann[(size_t)(t - a.t0) * a.stride + i];
+    }
+}
+
+// Face far field (8-corner interpolation, pppm.cu:116-144), face near field against the
+// 4x4x4 list with D[t] = 0, then D_c[t] = (far*rc + near) / (0.5*area - sum_j D_0(c, j)).
+__global__ __launch_bounds__(TB) void k_dirichlet(StepArgs a, const StepState *__restrict__ st,
+                                                  const int *__restrict__ face_start,
+                                                  const int *__restrict__ face_nbr,
+                                                  const int *__restrict__ face_slot,
+                                                  const uint32_t *__restrict__ fw, const float2 *__restrict__ fsc,
+                                                  const float *__restrict__ hn, float *__restrict__ hd,
+                                                  const float *__restrict__ interp, const int *__restrict__ corner,
+                                                  const float *__restrict__ far, const double *__restrict__ denom)
+{
+    __shared__ float vals[WPB][WAVE];
+    const int t = st->t;
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * WPB + wid;
+    const int c0 = lower_seg(face_start, a.nt, wave * WAVE);
+    const int c1 = lower_seg(face_start, a.nt, wave * WAVE + WAVE);
+    if (c0 >= c1)
+        return;
+    const int q0 = face_start[c0], q1 = face_start[c1];
+    const int cs = c0 + lane;
+    const bool leader = cs < c1;
+    const int m0 = leader ? face_start[cs] : 0, m1 = leader ? face_start[cs + 1] : 0;
+    float acc = 0.0f;
+    for (int base = q0; base < q1; base += WAVE)
+    {
+        const int p = base + lane;
+        float v = 0.0f;
+        if (p < q1)
+        {
+            const int j = face_nbr[p];
+            const int ws = face_slot[p];
+            v = conv_pair<0, true>(fw + (size_t)ws * STEPS, fsc[ws], hn + j * RING, hd + j * RING, t);
+        }
+        vals[wid][lane] = v;
+        wave_sync();
+        if (leader)
+        {
+            const int e = min(m1, base + WAVE);
+            for (int q = max(m0, base); q < e; q++)
+                acc += vals[wid][q - base];
+        }
+        wave_sync();
+    }
+    if (leader)
+    {
+        float ff = 0.0f;
+#pragma unroll
+        for (int w = 0; w < 8; w++)
+            ff += interp[8 * cs + w] * far[corner[8 * cs + w]];
+        const float num = ff * a.rc + acc;
+        hd[cs * RING + (t & (RING - 1))] = (float)((double)num / denom[cs]);
+    }
+}
+
+// p(c) = far + 0.98 * sum W^bem (*) (N, D) over all 16 taps; records listener samples.
+__global__ __launch_bounds__(TB) void k_near(StepArgs a, const StepState *__restrict__ st, float *__restrict__ grid3,
+                                             const int *__restrict__ c3_cells, const int *__restrict__ c3_start,
+                                             const int *__restrict__ c3_pairs, const uint32_t *__restrict__ wb,
+                                             const float2 *__restrict__ sb, const float *__restrict__ hn,
+                                             const float *__restrict__ hd, const float *__restrict__ far,
+                                             const int *__restrict__ listen_cell, const int *__restrict__ listen_c3,
+                                             float *__restrict__ out, int gather_waves)
+{
+    __shared__ float vals[WPB][WAVE];
+    const int t = st->t;
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * WPB + wid;
+    const size_t nn = (size_t)a.N * a.N * a.N;
+    float *g = grid3 + (size_t)(((t % 3) + 3) % 3) * nn;
+    if (wave < gather_waves)
+    {
+        const int c0 = lower_seg(c3_start, a.n_c3, wave * WAVE);
+        const int c1 = lower_seg(c3_start, a.n_c3, wave * WAVE + WAVE);
+        if (c0 >= c1)
+            return;
+        const int q0 = c3_start[c0], q1 = c3_start[c1];
+        const int cs = c0 + lane;
+        const bool leader = cs < c1;
+        const int m0 = leader ? c3_start[cs] : 0, m1 = leader ? c3_start[cs + 1] : 0;
+        float acc = 0.0f;
+        for (int base = q0; base < q1; base += WAVE)
+        {
+            const int p = base + lane;
+            float v = 0.0f;
+            if (p < q1)
+            {
+                const int rec = c3_pairs[p];
+                const int tri = rec / GC;
+                v = conv_pair<0, false>(wb + (size_t)rec * STEPS, sb[rec], hn + tri * RING, hd + tri * RING, t);
+            }
+            vals[wid][lane] = v;
+            wave_sync();
+            if (leader)
+            {
+                const int e = min(m1, base + WAVE);
+                for (int q = max(m0, base); q < e; q++)
+                    acc += vals[wid][q - base];
+            }
+            wave_sync();
+        }
+        if (leader)
+        {
+            const float v = (float)((double)far[cs] + (double)acc * 0.98);
+            g[c3_cells[cs]] = v;
+            if (out)
+                for (int l = 0; l < a.n_listen; l++)
+                    if (listen_c3[l] == cs)
+                        out[(size_t)(t - a.t0) * a.n_listen + l] = v;
+        }
+    }
+    else if (out && wave == gather_waves)
+    {
+        // listeners outside the corrected region read the grid directly (no wave writes them)
+        if (lane < a.n_listen && listen_c3[lane] < 0)
+            out[(size_t)(t - a.t0) * a.n_listen + lane] = g[listen_cell[lane]];
+    }
+}
+
+__global__ void k_listen_direct(StepArgs a, const StepState *__restrict__ st, const float *__restrict__ grid3,
+                                const int *__restrict__ listen_cell, float *__restrict__ out)
+{
+    const int t = st->t;
+    const int l = threadIdx.x;
+    if (l >= a.n_listen)
+        return;
+    const size_t nn = (size_t)a.N * a.N * a.N;
+    const float *g = grid3 + (size_t)(((t % 3) + 3) % 3) * nn;
+    out[(size_t)(t - a.t0) * a.n_listen + l] = g[listen_cell[l]];
+}
+
+__global__ void k_set_neumann(StepArgs a, const StepState *__restrict__ st, float *__restrict__ hn,
+                              const float *__restrict__ neumann)
+{
+    const int t = st->t;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.nt)
+        hn[i * RING + (t & (RING - 1))] = neumann[(size_t)(t - a.t0) * a.stride + i];
+}
+
+inline int blocks_for_waves(long long waves) { return waves > 0 ? (int)((waves + WPB - 1) / WPB) : 1; }
+// waves needed to own every segment start in [0, pairs]
+inline int gather_waves_for(long long pairs) { return (int)(pairs / WAVE) + 1; }
+}  // namespace
+
+hipError_t launch_far(const StepArgs &a, const StepState *st, const float *grid3, const int *c3_cells,
+                      const int *c3_start, const int *c3_pairs, const uint32_t *wf, const float2 *sf, float *hn,
+                      const float *hd, float *far, const float *neumann, int n_pairs, hipStream_t s)
+{
+    const int gw = gather_waves_for(n_pairs);
+    const long long waves = (long long)gw + (neumann ? (a.nt + WAVE - 1) / WAVE : 0);
+    k_far<<<blocks_for_waves(waves), TB, 0, s>>>(a, st, grid3, c3_cells, c3_start, c3_pairs, wf, sf, hn, hd, far,
+                                                  neumann, gw);
+    return hipGetLastError();
+}
+
+hipError_t launch_dirichlet(const StepArgs &a, const StepState *st, const int *face_start, const int *face_nbr,
+                            const int *face_slot, const uint32_t *fw, const float2 *fsc, const float *hn, float *hd,
+                            const float *interp, const int *corner, const float *far, const double *denom,
+                            int n_pairs, hipStream_t s)
+{
+    k_dirichlet<<<blocks_for_waves(gather_waves_for(n_pairs)), TB, 0, s>>>(a, st, face_start, face_nbr, face_slot, fw,
+                                                                          fsc, hn, hd, interp, corner, far, denom);
+    return hipGetLastError();
+}
+
+hipError_t launch_near(const StepArgs &a, const StepState *st, float *grid3, const int *c3_cells,
+                       const int *c3_start, const int *c3_pairs, const uint32_t *wb, const float2 *sb,
+                       const float *hn, const float *hd, const float *far, const int *listen_cell,
+                       const int *listen_c3, float *out, int n_pairs, hipStream_t s)
+{
+    const int gw = gather_waves_for(n_pairs);
+    const long long waves = (long long)gw + (out ? 1 : 0);
+    k_near<<<blocks_for_waves(waves), TB, 0, s>>>(a, st, grid3, c3_cells, c3_start, c3_pairs, wb, sb, hn, hd, far,
+                                                   listen_cell, listen_c3, out, gw);
+    return hipGetLastError();
+}
+
+hipError_t launch_listen_direct(const StepArgs &a, const StepState *st, const float *grid3, const int *listen_cell,
+                                float *out, hipStream_t s)
+{
+    k_listen_direct<<<1, 64, 0, s>>>(a, st, grid3, listen_cell, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_neumann(const StepArgs &a, const StepState *st, float *hn, const float *neumann,
+                              hipStream_t s)
+{
+    k_set_neumann<<<(a.nt + 255) / 256 > 0 ? (a.nt + 255) / 256 : 1, 256, 0, s>>>(a, st, hn, neumann);
+    return hipGetLastError();
+}
+
+}  // namespace pppm_amd

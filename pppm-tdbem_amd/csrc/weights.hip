@@ -1,0 +1,709 @@
+// FP64 construction of the near-field TDBEM weights on the GPU.
+//
+// Reference behaviour (computed there in FP32 complex arithmetic):
+//   TDBEM::init / scaledFFT / laplace_weight     src/algorithm/bem.h:264-351
+//   Helmholtz layer potentials                    src/helmholtz/potential.h:22-101
+//   face2Point / face2Face integrands (1-point,   src/helmholtz/integrand.h:52-267
+//     regular 1x1, Duffy-singular 3^4 Gauss)
+//   GridCache  (27 offsets x {fdtd, bem})         src/algorithm/pppm_cache.cu:53-134
+//   trilinear interpolation weights               src/algorithm/pppm_cache.cu:139-181
+//   FaceCache  (face pair weights + dirty test)   src/algorithm/pppm_cache.cu:183-289
+//   LayerWeightHalf::set (FP16 + FP32 scale)      src/algorithm/bem.h:171-191
+//
+// Here every integral, the wave numbers, lambda and the inverse DFT are FP64; the
+// weight series is rounded to FP32 and quantised to FP16 with a max-abs FP32 scale,
+// exactly as the reference stores it. Storage: per record 16 x half2 {S_k, D_k}
+// (64 B, one half cache line) plus a float2 {scale_S, scale_D}.
+#include "internal.hpp"
+
+#include <cmath>
+
+namespace pppm_amd
+{
+
+namespace
+{
+constexpr int B = 128;
+
+__constant__ double c_gx[3] = {0.774596669241483, 0.000000000000000, -0.774596669241483};  // gauss/line.h:26-37
+__constant__ double c_gw[3] = {0.555555555555554, 0.888888888888889, 0.555555555555554};
+constexpr double TRI1_X = 0.3333333333333330;  // gauss/triangle.h:6-14
+constexpr double TRI1_W = 1.0;
+
+struct cdbl
+{
+    double re, im;
+};
+
+__device__ inline cdbl cexp_d(double re, double im)
+{
+    double e = exp(re);
+    double sn, cs;
+    sincos(im, &sn, &cs);
+    return {e * cs, e * sn};
+}
+
+// single / double layer kernels, potential.h:22-47
+__device__ inline cdbl layer64(d3 src, d3 trg, d3 n, double kr, double ki, int type)
+{
+    if (type == 0)
+    {
+        d3 s = trg - src;
+        double r2 = dotd(s, s);
+        if (r2 == 0)
+            return {0, 0};
+        double r = sqrt(r2);
+        cdbl e = cexp_d(-r * ki, r * kr);
+        double q = 4 * PI_R * r;
+        return {e.re / q, e.im / q};
+    }
+    d3 s = src - trg;
+    double r2 = dotd(s, s);
+    if (r2 == 0)
+        return {0, 0};
+    double r = sqrt(r2);
+    double zr = -r * ki, zi = r * kr;
+    cdbl e = cexp_d(zr, zi);
+    double q = 4 * PI_R * r2 * r;
+    double ar = -e.re / q, ai = -e.im / q;
+    double br = 1 - zr, bi = -zi;
+    double pr = ar * br - ai * bi, pi = ar * bi + ai * br;
+    double dn = dotd(s, n);
+    return {pr * dn, pi * dn};
+}
+
+struct TriD
+{
+    d3 v[3];
+    d3 n;
+    double jac;
+};
+
+__device__ inline f3 ld3(const float *v, int i) { return {v[3 * i], v[3 * i + 1], v[3 * i + 2]}; }
+
+__device__ inline TriD tri_d(const float *verts, int i0, int i1, int i2)
+{
+    TriD t;
+    t.v[0] = D(ld3(verts, i0));
+    t.v[1] = D(ld3(verts, i1));
+    t.v[2] = D(ld3(verts, i2));
+    d3 cr = crossd(t.v[1] - t.v[0], t.v[2] - t.v[0]);
+    t.jac = sqrt(dotd(cr, cr));
+    t.n = {cr.x / t.jac, cr.y / t.jac, cr.z / t.jac};
+    return t;
+}
+__device__ inline d3 l2g(double x1, double x2, const d3 *v) { return (1 - x1 - x2) * v[0] + x1 * v[1] + x2 * v[2]; }
+__device__ inline d3 l2g2(double x1, double x2, const d3 *v) { return (1 - x1) * v[0] + (x1 - x2) * v[1] + x2 * v[2]; }
+
+// potential_integrand over a batch of scaled targets (integrand.h:209-222)
+__device__ inline cdbl f2p64(const TriD &t, const d3 *pts, const double *scale, int npts, double kr, double ki,
+                             int type)
+{
+    d3 p = l2g(TRI1_X, TRI1_X, t.v);
+    double sr = 0, si = 0;
+    for (int i = 0; i < npts; i++)
+    {
+        cdbl g = layer64(p, pts[i], t.n, kr, ki, type);
+        sr += g.re * scale[i];
+        si += g.im * scale[i];
+    }
+    double f = 0.5 * TRI1_W * t.jac;
+    return {f * sr, f * si};
+}
+
+// singular_point (integrand.h:52-147); trial = tv, test = sv
+__device__ inline cdbl sing_point64(double xsi, double e1, double e2, double e3, double w, const d3 *tv,
+                                    const d3 *sv, d3 tn, int nn, double kr, double ki, int type)
+{
+    double rr = 0, ri = 0;
+    xsi = 0.5 * (xsi + 1);
+    e1 = 0.5 * (e1 + 1);
+    e2 = 0.5 * (e2 + 1);
+    e3 = 0.5 * (e3 + 1);
+    double jw = 0;
+#define ADD(A, Bp, F)                                      \
+    {                                                      \
+        cdbl g_ = layer64((A), (Bp), tn, kr, ki, type);    \
+        rr += g_.re * (F);                                 \
+        ri += g_.im * (F);                                 \
+    }
+    if (nn == 3)
+    {
+        jw = xsi * xsi * xsi * e1 * e1 * e2;
+        double e12 = e1 * e2, e123 = e1 * e2 * e3;
+        ADD(l2g2(xsi, xsi * (1.0 - e1 + e12), tv), l2g2(xsi * (1.0 - e123), xsi * (1.0 - e1), sv), 1.0);
+        ADD(l2g2(xsi * (1.0 - e123), xsi * (1.0 - e1), tv), l2g2(xsi, xsi * (1.0 - e1 + e12), sv), 1.0);
+        ADD(l2g2(xsi, xsi * (e1 - e12 + e123), tv), l2g2(xsi * (1.0 - e12), xsi * (e1 - e12), sv), 1.0);
+        ADD(l2g2(xsi * (1.0 - e12), xsi * (e1 - e12), tv), l2g2(xsi, xsi * (e1 - e12 + e123), sv), 1.0);
+        ADD(l2g2(xsi * (1.0 - e123), xsi * (e1 - e123), tv), l2g2(xsi, xsi * (e1 - e12), sv), 1.0);
+        ADD(l2g2(xsi, xsi * (e1 - e12), tv), l2g2(xsi * (1.0 - e123), xsi * (e1 - e123), sv), 1.0);
+    }
+    else if (nn == 2)
+    {
+        jw = xsi * xsi * xsi * e1 * e1;
+        double e12 = e1 * e2, e123 = e1 * e2 * e3;
+        ADD(l2g2(xsi, xsi * e1 * e3, tv), l2g2(xsi * (1.0 - e12), xsi * e1 * (1.0 - e2), sv), 1.0);
+        ADD(l2g2(xsi, xsi * e1, tv), l2g2(xsi * (1.0 - e123), xsi * e1 * e2 * (1 - e3), sv), e2);
+        ADD(l2g2(xsi * (1.0 - e12), xsi * e1 * (1.0 - e2), tv), l2g2(xsi, xsi * e123, sv), e2);
+        ADD(l2g2(xsi * (1.0 - e123), xsi * e12 * (1.0 - e3), tv), l2g2(xsi, xsi * e1, sv), e2);
+        ADD(l2g2(xsi * (1.0 - e123), xsi * e1 * (1.0 - e2 * e3), tv), l2g2(xsi, xsi * e12, sv), e2);
+    }
+    else if (nn == 1)
+    {
+        jw = xsi * xsi * xsi * e2;
+        ADD(l2g2(xsi, xsi * e1, tv), l2g2(xsi * e2, xsi * e2 * e3, sv), 1.0);
+        ADD(l2g2(xsi * e2, xsi * e2 * e3, tv), l2g2(xsi, xsi * e1, sv), 1.0);
+    }
+    else
+        return {0, 0};
+#undef ADD
+    double f = jw * w;
+    return {rr * f, ri * f};
+}
+
+__device__ inline int common_vertices(int a0, int a1, int a2, int b0, int b1, int b2)
+{
+    return (a0 == b0) + (a0 == b1) + (a0 == b2) + (a1 == b0) + (a1 == b1) + (a1 == b2) + (a2 == b0) + (a2 == b1) +
+           (a2 == b2);
+}
+
+// face2FaceIntegrand (integrand.h:228-259); the near-coincidence test stays in FP32
+__device__ inline cdbl f2f64(const float *verts, const int *src, const int *trg, double kr, double ki, int type,
+                             float threshold)
+{
+    const f3 s0 = ld3(verts, src[0]), s1 = ld3(verts, src[1]), s2 = ld3(verts, src[2]);
+    const f3 t0 = ld3(verts, trg[0]), t1 = ld3(verts, trg[1]), t2 = ld3(verts, trg[2]);
+    const int nn = common_vertices(src[0], src[1], src[2], trg[0], trg[1], trg[2]);
+    const float dist = lengthf((s0 + s1 + s2) / 3.0f - (t0 + t1 + t2) / 3.0f);
+    if (dist < threshold && nn == 0)
+        return {0, 0};
+    const TriD S = tri_d(verts, src[0], src[1], src[2]);
+    const TriD T = tri_d(verts, trg[0], trg[1], trg[2]);
+    if (nn == 0)
+    {
+        d3 a = l2g(TRI1_X, TRI1_X, S.v), b = l2g(TRI1_X, TRI1_X, T.v);
+        cdbl g = layer64(a, b, S.n, kr, ki, type);
+        double f = 0.25 * TRI1_W * TRI1_W * S.jac * T.jac;
+        return {f * g.re, f * g.im};
+    }
+    double rr = 0, ri = 0;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+            for (int c = 0; c < 3; c++)
+                for (int d = 0; d < 3; d++)
+                {
+                    cdbl p = sing_point64(c_gx[a], c_gx[b], c_gx[c], c_gx[d], c_gw[a] * c_gw[b] * c_gw[c] * c_gw[d],
+                                          S.v, T.v, S.n, nn, kr, ki, type);
+                    rr += p.re;
+                    ri += p.im;
+                }
+    double f = S.jac * T.jac / 16;
+    return {rr * f, ri * f};
+}
+
+// Radix-2 decimation-in-time inverse transform with the reference's butterfly order
+// (FFT<step>, bem.h:12-38): level STEP transforms the stride-STEP subsequence. The
+// order matters: with PI = 3.14159265359 the twiddle angles carry ~1e-13 rad errors and
+// lambda^-k (up to ~5.6e3 at k = 15) amplifies them to FP16-rounding scale, so another
+// DFT factorisation would flip stored FP16 codes. Twiddles come from a host table.
+template <int STEP>
+__device__ inline void fft_level(const cdbl *in, cdbl *out, cdbl *buf, const Tdbem &b)
+{
+    if constexpr (STEP == STEPS)
+    {
+        out[0] = in[0];
+    }
+    else
+    {
+        constexpr int NS = STEP * 2;
+        fft_level<NS>(in, out, buf, b);
+        fft_level<NS>(in + STEP, out + STEP, buf + STEP, b);
+#pragma unroll
+        for (int i = 0; i < STEPS; i += NS)
+        {
+            buf[i / 2] = out[i];
+            buf[(STEPS + i) / 2] = out[i + STEP];
+        }
+#pragma unroll
+        for (int i = 0; i < STEPS / 2; i += STEP)
+        {
+            const double cr = b.tw_re[i], ci = b.tw_im[i];
+            const cdbl x = buf[i + STEPS / 2];
+            const double tr = cr * x.re - ci * x.im;
+            const double ti = cr * x.im + ci * x.re;
+            const cdbl a = buf[i];
+            out[i] = {a.re + tr, a.im + ti};
+            out[i + STEPS / 2] = {a.re - tr, a.im - ti};
+        }
+    }
+}
+
+// scaledFFT (bem.h:292-304) of a conjugate-symmetric series given by its first 9 entries
+__device__ inline void scaled_idft(const cdbl *v9, const Tdbem &b, double *out)
+{
+    cdbl in[STEPS], o[STEPS], buf[STEPS];
+#pragma unroll
+    for (int k = 0; k < STEPS; k++)
+        in[k] = k <= 8 ? v9[k] : cdbl{v9[16 - k].re, -v9[16 - k].im};
+    fft_level<1>(in, o, buf, b);
+    double temp = 1.0;
+#pragma unroll
+    for (int n = 0; n < STEPS; n++)
+    {
+        out[n] = o[n].re / STEPS * temp;
+        temp /= b.lambda;
+    }
+}
+
+// LayerWeightHalf::set (bem.h:171-191) into the half2 {S, D} record layout
+__device__ inline void quantize_store(const float *s, const float *d, uint32_t *w16, float2 *sc)
+{
+    float ms = 0, md = 0;
+    for (int i = 0; i < STEPS; i++)
+    {
+        ms = fmaxf(ms, fabsf(s[i]));
+        md = fmaxf(md, fabsf(d[i]));
+    }
+    if (ms == 0)
+        ms = 1;
+    if (md == 0)
+        md = 1;
+    for (int i = 0; i < STEPS; i++)
+    {
+        uint32_t hs = f32_to_f16_bits(s[i] / ms);
+        uint32_t hd = f32_to_f16_bits(d[i] / md);
+        w16[i] = hs | (hd << 16);
+    }
+    *sc = make_float2(ms, md);
+}
+
+__device__ inline d3 cell_center_d(int x, int y, int z, const WeightArgs &a)
+{
+    return {((double)x + 0.5) * a.h + a.minx, ((double)y + 0.5) * a.h + a.miny, ((double)z + 0.5) * a.h + a.minz};
+}
+__device__ inline f3 cell_center_f(int x, int y, int z, const WeightArgs &a)
+{
+    return f3{((float)x + 0.5f) * a.h, ((float)y + 0.5f) * a.h, ((float)z + 0.5f) * a.h} + f3{a.minx, a.miny, a.minz};
+}
+
+// ---------------------------------------------------------------- grid cache
+// GridCache::need_recompute (pppm_cache.h:64-69); meta = center(3) normal(3) area per triangle
+__global__ __launch_bounds__(256) void k_grid_dirty(const TriGeom *__restrict__ geom, const float *__restrict__ meta,
+                                                    int nt, float h, int first, int *flag)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nt)
+        return;
+    if (first)
+    {
+        flag[i] = 1;
+        return;
+    }
+    const TriGeom g = geom[i];
+    const float *m = meta + 7 * i;
+    const f3 oc = {m[0], m[1], m[2]}, on = {m[3], m[4], m[5]};
+    const float oa = m[6];
+    const bool d = (double)lengthf(oc - g.center) > RECOMPUTE_THRESHOLD * (double)h ||
+                   (double)lengthf(on - g.normal) > RECOMPUTE_THRESHOLD ||
+                   (double)fabsf(oa - g.area) > RECOMPUTE_THRESHOLD * (double)g.area;
+    flag[i] = d ? 1 : 0;
+}
+
+// update_grid_cache_kernel (pppm_cache.cu:64-123): one thread per (dirty triangle, offset)
+__global__ __launch_bounds__(B) void k_grid_weights(const float *__restrict__ verts, const int *__restrict__ tris,
+                                                    const TriGeom *__restrict__ geom, const int *__restrict__ list,
+                                                    int n, WeightArgs a, Tdbem bem, uint32_t *__restrict__ wf,
+                                                    float2 *__restrict__ sf, uint32_t *__restrict__ wb,
+                                                    float2 *__restrict__ sb, float *__restrict__ meta)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n * GC)
+        return;
+    const int tri = list[i / GC], o = i % GC;
+    const TriGeom g = geom[tri];
+    const TriD T = tri_d(verts, tris[3 * tri], tris[3 * tri + 1], tris[3 * tri + 2]);
+    const int3_ c = g.cell3;
+    const int3_ nb = {c.x + o % 3 - 1, c.y + o / 3 % 3 - 1, c.z + o / 9 - 1};
+    const float s = a.c * a.c * a.dt * a.dt / (a.h * a.h);
+    const int dx[8] = {0, -1, 1, 0, 0, 0, 0, 0}, dy[8] = {0, 0, 0, -1, 1, 0, 0, 0}, dz[8] = {0, 0, 0, 0, 0, -1, 1, 0};
+    const float dsc[8] = {2, s, s, s, s, s, s, -6 * s};
+    d3 pts[8];
+    double scs[8];
+    int np = 0;
+    for (int q = 0; q < 8; q++)
+    {
+        const int x = nb.x + dx[q], y = nb.y + dy[q], z = nb.z + dz[q];
+        if (x == c.x && y == c.y && z == c.z)
+            continue;  // triangle_in_grid_cell (pppm_cache.cu:7-19)
+        pts[np] = cell_center_d(x, y, z, a);
+        scs[np] = dsc[q];
+        np++;
+    }
+    const bool self = (nb.x == c.x && nb.y == c.y && nb.z == c.z);
+    cdbl vs[9], vd[9], us[9], ud[9];
+    const d3 pc = cell_center_d(nb.x, nb.y, nb.z, a);
+    const double one = 1.0;
+    for (int k = 0; k <= 8; k++)
+    {
+        vs[k] = f2p64(T, pts, scs, np, bem.wave_re[k], bem.wave_im[k], 0);
+        vd[k] = f2p64(T, pts, scs, np, bem.wave_re[k], bem.wave_im[k], 1);
+        if (!self)
+        {
+            us[k] = f2p64(T, &pc, &one, 1, bem.wave_re[k], bem.wave_im[k], 0);
+            ud[k] = f2p64(T, &pc, &one, 1, bem.wave_re[k], bem.wave_im[k], 1);
+        }
+        else
+            us[k] = ud[k] = {0, 0};
+    }
+    double ws[STEPS], wd[STEPS], gs[STEPS], gd[STEPS];
+    scaled_idft(vs, bem, ws);
+    scaled_idft(vd, bem, wd);
+    if (!self)
+    {
+        scaled_idft(us, bem, gs);
+        scaled_idft(ud, bem, gd);
+    }
+    else
+        for (int q = 0; q < STEPS; q++)
+            gs[q] = gd[q] = 0;
+    float fs[STEPS], fd[STEPS], bs[STEPS], bd[STEPS];
+    // w.move(-1) then w.add(G, -1, -2) (bem.h:73-80, 128-140)
+    for (int q = 0; q < STEPS; q++)
+    {
+        double x = q >= 1 ? ws[q - 1] : 0.0;
+        double y = q >= 1 ? wd[q - 1] : 0.0;
+        if (!self && q >= 2)
+        {
+            x += gs[q - 2] * -1.0;
+            y += gd[q - 2] * -1.0;
+        }
+        fs[q] = (float)x;
+        fd[q] = (float)y;
+        bs[q] = (float)gs[q];
+        bd[q] = (float)gd[q];
+    }
+    const size_t rec = (size_t)tri * GC + o;
+    quantize_store(fs, fd, wf + rec * STEPS, sf + rec);
+    quantize_store(bs, bd, wb + rec * STEPS, sb + rec);
+    if (o == 0)
+    {
+        float *m = meta + 7 * tri;
+        m[0] = g.center.x;
+        m[1] = g.center.y;
+        m[2] = g.center.z;
+        m[3] = g.normal.x;
+        m[4] = g.normal.y;
+        m[5] = g.normal.z;
+        m[6] = g.area;
+    }
+}
+
+// update_interpolation_weight_kernel (pppm_cache.cu:157-181), FP64 then rounded
+__global__ __launch_bounds__(256) void k_interp(const float *__restrict__ verts, const int *__restrict__ tris,
+                                                const TriGeom *__restrict__ geom, WeightArgs a,
+                                                float *__restrict__ interp)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.nt)
+        return;
+    const TriD T = tri_d(verts, tris[3 * i], tris[3 * i + 1], tris[3 * i + 2]);
+    const d3 p = l2g(TRI1_X, TRI1_X, T.v);
+    const int3_ b = geom[i].base3;
+    const d3 bc = cell_center_d(b.x, b.y, b.z, a);
+    const double h = a.h;
+    const double dx = (p.x - bc.x) / h, dy = (p.y - bc.y) / h, dz = (p.z - bc.z) / h;
+    const double w[8] = {(1 - dx) * (1 - dy) * (1 - dz), (1 - dx) * (1 - dy) * dz, (1 - dx) * dy * (1 - dz),
+                         (1 - dx) * dy * dz,             dx * (1 - dy) * (1 - dz), dx * (1 - dy) * dz,
+                         dx * dy * (1 - dz),             dx * dy * dz};
+    const double sc = 0.5 * TRI1_W * T.jac;
+    for (int q = 0; q < 8; q++)
+        interp[8 * i + q] = (float)(sc * w[q]);
+}
+
+// ---------------------------------------------------------------- face cache
+__device__ inline unsigned hash64(unsigned long long k, int cap)
+{
+    return (unsigned)((k * 0x9E3779B97F4A7C15ull) >> 32) & (unsigned)(cap - 1);
+}
+constexpr unsigned long long EMPTY_KEY = ~0ull;
+
+// compute_face_compute_list_kernel + get_cache_element_with_check (pppm_cache.cu:183-232,
+// pppm_cache.h:134-147): (c, j) -> persistent slot; dirty if new or moved past the thresholds
+__global__ __launch_bounds__(256) void k_face_lookup(const TriGeom *__restrict__ geom, const int *__restrict__ tris,
+                                                     const int *__restrict__ owner, const int *__restrict__ face_nbr,
+                                                     int n_p4, unsigned long long *hkeys, int *hvals, int hcap,
+                                                     int *pool_count, float2 *fmeta, int *face_slot, int *dirty)
+{
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= n_p4)
+        return;
+    const int c = owner[p], j = face_nbr[p];
+    const unsigned long long key = ((unsigned long long)c << 32) | (unsigned)j;
+    unsigned h = hash64(key, hcap);
+    int slot = -1;
+    bool fresh = false;
+    for (int probe = 0; probe < hcap; probe++)
+    {
+        unsigned long long k = hkeys[h];
+        if (k == key)
+        {
+            slot = hvals[h];
+            break;
+        }
+        if (k == EMPTY_KEY)
+        {
+            unsigned long long prev = atomicCAS(&hkeys[h], EMPTY_KEY, key);
+            if (prev == EMPTY_KEY)
+            {
+                slot = atomicAdd(pool_count, 1);
+                hvals[h] = slot;
+                fresh = true;
+                break;
+            }
+            if (prev == key)
+            {
+                slot = hvals[h];
+                break;
+            }
+        }
+        h = (h + 1) & (unsigned)(hcap - 1);
+    }
+    face_slot[p] = slot;
+    if (slot < 0)
+    {
+        dirty[p] = 0;
+        return;
+    }
+    const TriGeom gc = geom[c], gj = geom[j];
+    const float normal_angle = dotf(gj.normal, gc.normal);
+    const float distance = lengthf(gj.center - gc.center);
+    bool d = fresh;
+    if (!fresh)
+    {
+        const float2 m = fmeta[slot];
+        d = (double)fabsf(m.x - normal_angle) > RECOMPUTE_THRESHOLD ||
+            (double)fabsf(m.y - distance) > RECOMPUTE_THRESHOLD * (double)distance;
+    }
+    if (d)
+    {
+        fmeta[slot] = make_float2(normal_angle, distance);
+        const int nn = common_vertices(tris[3 * j], tris[3 * j + 1], tris[3 * j + 2], tris[3 * c], tris[3 * c + 1],
+                                       tris[3 * c + 2]);
+        dirty[p] = 1 + (nn > 3 ? 3 : nn);
+    }
+    else
+        dirty[p] = 0;
+}
+
+// one thread per (dirty pair, k <= 8, layer): face-face integral plus the interpolated
+// far-field representation subtracted for corners farther than 1.5h (pppm_cache.cu:234-271)
+__global__ __launch_bounds__(B) void k_face_integrals(const float *__restrict__ verts, const int *__restrict__ tris,
+                                                      const TriGeom *__restrict__ geom,
+                                                      const int *__restrict__ dirty_pairs, int n,
+                                                      const int *__restrict__ owner, const int *__restrict__ face_nbr,
+                                                      const float *__restrict__ interp, WeightArgs a, Tdbem bem,
+                                                      double *__restrict__ tmp)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n * 18)
+        return;
+    const int d = i / 18, r = i % 18, layer = r / 9, k = r % 9;
+    const int p = dirty_pairs[d];
+    const int c = owner[p], j = face_nbr[p];
+    const int src[3] = {tris[3 * j], tris[3 * j + 1], tris[3 * j + 2]};
+    const int trg[3] = {tris[3 * c], tris[3 * c + 1], tris[3 * c + 2]};
+    const double kr = bem.wave_re[k], ki = bem.wave_im[k];
+    cdbl v = f2f64(verts, src, trg, kr, ki, layer, bem.threshold);
+    // batch_grid_near_field with the corners farther than 1.5h in any axis, scale -interp
+    const TriGeom gc = geom[c], gj = geom[j];
+    d3 pts[8];
+    double scs[8];
+    int np = 0;
+    const double lim = (double)a.h * 1.5;
+    for (int w = 0; w < 8; w++)
+    {
+        const int x = gc.base3.x + (w >> 2), y = gc.base3.y + ((w >> 1) & 1), z = gc.base3.z + (w & 1);
+        const f3 ctr = cell_center_f(x, y, z, a);
+        if ((double)fabsf(ctr.x - gj.center.x) > lim || (double)fabsf(ctr.y - gj.center.y) > lim ||
+            (double)fabsf(ctr.z - gj.center.z) > lim)
+        {
+            pts[np] = cell_center_d(x, y, z, a);
+            scs[np] = (double)(-interp[8 * c + w]);
+            np++;
+        }
+    }
+    if (np > 0)
+    {
+        const TriD J = tri_d(verts, src[0], src[1], src[2]);
+        cdbl u = f2p64(J, pts, scs, np, kr, ki, layer);
+        v.re += u.re;
+        v.im += u.im;
+    }
+    tmp[2 * (size_t)i] = v.re;
+    tmp[2 * (size_t)i + 1] = v.im;
+}
+
+__global__ __launch_bounds__(256) void k_face_finish(const int *__restrict__ dirty_pairs, int n,
+                                                     const int *__restrict__ face_slot, Tdbem bem,
+                                                     const double *__restrict__ tmp, uint32_t *__restrict__ fw,
+                                                     float2 *__restrict__ fsc)
+{
+    const int d = blockIdx.x * 256 + threadIdx.x;
+    if (d >= n)
+        return;
+    const int slot = face_slot[dirty_pairs[d]];
+    cdbl vs[9], vd[9];
+    for (int k = 0; k < 9; k++)
+    {
+        vs[k] = {tmp[2 * ((size_t)d * 18 + k)], tmp[2 * ((size_t)d * 18 + k) + 1]};
+        vd[k] = {tmp[2 * ((size_t)d * 18 + 9 + k)], tmp[2 * ((size_t)d * 18 + 9 + k) + 1]};
+    }
+    double ws[STEPS], wd[STEPS];
+    scaled_idft(vs, bem, ws);
+    scaled_idft(vd, bem, wd);
+    float fs[STEPS], fd[STEPS];
+    for (int q = 0; q < STEPS; q++)
+    {
+        fs[q] = (float)ws[q];
+        fd[q] = (float)wd[q];
+    }
+    quantize_store(fs, fd, fw + (size_t)slot * STEPS, fsc + slot);
+}
+
+// denominators 0.5*area - sum_j D_0(c, j) (pppm.cu:167, 181-187)
+__global__ __launch_bounds__(256) void k_face_denom(const TriGeom *__restrict__ geom, int nt,
+                                                    const int *__restrict__ face_start,
+                                                    const int *__restrict__ face_slot, const uint32_t *__restrict__ fw,
+                                                    const float2 *__restrict__ fsc, double *__restrict__ denom)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= nt)
+        return;
+    float fsum = 0;
+    for (int p = face_start[c]; p < face_start[c + 1]; p++)
+    {
+        const int slot = face_slot[p];
+        fsum += f16_bits_to_f32(fw[(size_t)slot * STEPS] >> 16) * fsc[slot].y;
+    }
+    denom[c] = 0.5 * (double)geom[c].area - (double)fsum;
+}
+
+__global__ __launch_bounds__(256) void k_face_owner(const int *__restrict__ face_start, int nt, int *owner)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= nt)
+        return;
+    for (int p = face_start[c]; p < face_start[c + 1]; p++)
+        owner[p] = c;
+}
+
+__global__ __launch_bounds__(256) void k_rehash(const unsigned long long *__restrict__ okeys,
+                                                const int *__restrict__ ovals, int ocap, unsigned long long *nkeys,
+                                                int *nvals, int ncap)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= ocap)
+        return;
+    const unsigned long long key = okeys[i];
+    if (key == EMPTY_KEY)
+        return;
+    unsigned h = hash64(key, ncap);
+    for (int probe = 0; probe < ncap; probe++)
+    {
+        if (atomicCAS(&nkeys[h], EMPTY_KEY, key) == EMPTY_KEY)
+        {
+            nvals[h] = ovals[i];
+            return;
+        }
+        h = (h + 1) & (unsigned)(ncap - 1);
+    }
+}
+
+inline int nb(long long n, int b) { return n > 0 ? (int)((n + b - 1) / b) : 1; }
+}  // namespace
+
+void tdbem_init(Tdbem &b, float dt, float c, float grid_size)
+{
+    // TDBEM::init (bem.h:264-277) in FP64
+    const double dc = (double)dt * (double)c;
+    const double lam = std::max(std::pow(dc, 3.0 / STEPS), std::pow((double)EPS_F, 1.0 / (2 * STEPS)));
+    b.lambda = lam;
+    for (int k = 0; k < STEPS; k++)
+    {
+        const double ang = (-2 * PI_R / STEPS) * k;
+        const double sr = lam * std::cos(ang), si = lam * std::sin(ang);
+        const double tr = sr * sr - si * si - 4 * sr + 3, ti = 2 * sr * si - 4 * si;  // BDF2 (macro.h:90)
+        const double br = 0.5 * tr / dc, bi = 0.5 * ti / dc;
+        b.wave_re[k] = -bi;  // i * BDF2(s_k) / (c dt)
+        b.wave_im[k] = br;
+    }
+    for (int j = 0; j < STEPS / 2; j++)
+    {
+        const double ang = 2 * PI_R * j / STEPS;
+        b.tw_re[j] = std::cos(ang);
+        b.tw_im[j] = std::sin(ang);
+    }
+    b.threshold = (float)(grid_size * 0.1);
+}
+
+hipError_t launch_grid_dirty(const TriGeom *geom, float *meta, int nt, float h, int first, int *flag, hipStream_t s)
+{
+    k_grid_dirty<<<nb(nt, 256), 256, 0, s>>>(geom, meta, nt, h, first, flag);
+    return hipGetLastError();
+}
+hipError_t launch_grid_weights(const float *verts, const int *tris, const TriGeom *geom, const int *list, int n,
+                               const WeightArgs &a, const Tdbem &b, uint32_t *wf, float2 *sf, uint32_t *wb,
+                               float2 *sb, float *meta, hipStream_t s)
+{
+    if (n <= 0)
+        return hipSuccess;
+    k_grid_weights<<<nb((long long)n * GC, B), B, 0, s>>>(verts, tris, geom, list, n, a, b, wf, sf, wb, sb, meta);
+    return hipGetLastError();
+}
+hipError_t launch_interp_weights(const float *verts, const int *tris, const TriGeom *geom, const WeightArgs &a,
+                                 float *interp, hipStream_t s)
+{
+    k_interp<<<nb(a.nt, 256), 256, 0, s>>>(verts, tris, geom, a, interp);
+    return hipGetLastError();
+}
+hipError_t launch_face_lookup(const TriGeom *geom, const int *tris, const int *owner, const int *face_nbr, int n_p4,
+                              unsigned long long *hkeys, int *hvals, int hcap, int *pool_count, float2 *fmeta,
+                              int *face_slot, int *dirty, hipStream_t s)
+{
+    if (n_p4 <= 0)
+        return hipSuccess;
+    k_face_lookup<<<nb(n_p4, 256), 256, 0, s>>>(geom, tris, owner, face_nbr, n_p4, hkeys, hvals, hcap, pool_count,
+                                                 fmeta, face_slot, dirty);
+    return hipGetLastError();
+}
+hipError_t launch_face_weights(const float *verts, const int *tris, const TriGeom *geom, const int *dirty_pairs,
+                               int n, const int *owner, const int *face_nbr, const int *face_slot,
+                               const float *interp, const WeightArgs &a, const Tdbem &b, double *tmp, uint32_t *fw,
+                               float2 *fsc, hipStream_t s)
+{
+    if (n <= 0)
+        return hipSuccess;
+    k_face_integrals<<<nb((long long)n * 18, B), B, 0, s>>>(verts, tris, geom, dirty_pairs, n, owner, face_nbr,
+                                                            interp, a, b, tmp);
+    k_face_finish<<<nb(n, 256), 256, 0, s>>>(dirty_pairs, n, face_slot, b, tmp, fw, fsc);
+    return hipGetLastError();
+}
+hipError_t launch_face_owner(const int *face_start, int nt, int *owner, hipStream_t s)
+{
+    k_face_owner<<<nb(nt, 256), 256, 0, s>>>(face_start, nt, owner);
+    return hipGetLastError();
+}
+hipError_t launch_face_denom(const TriGeom *geom, int nt, const int *face_start, const int *face_slot,
+                             const uint32_t *fw, const float2 *fsc, double *denom, hipStream_t s)
+{
+    k_face_denom<<<nb(nt, 256), 256, 0, s>>>(geom, nt, face_start, face_slot, fw, fsc, denom);
+    return hipGetLastError();
+}
+hipError_t launch_rehash(const unsigned long long *okeys, const int *ovals, int ocap, unsigned long long *nkeys,
+                         int *nvals, int ncap, hipStream_t s)
+{
+    k_rehash<<<nb(ocap, 256), 256, 0, s>>>(okeys, ovals, ocap, nkeys, nvals, ncap);
+    return hipGetLastError();
+}
+
+}  // namespace pppm_amd

@@ -1,0 +1,353 @@
+"""Python binding of libpppm_hip.so (ctypes over the C ABI in include/pppm_hip.h).
+
+`PPPMSolver` mirrors the reference's solver interface (src/algorithm/pppm.h:11-164):
+set_mesh / update_mesh / update_grid_and_face (alias update_step) / solve_fdtd_far /
+set_neumann_condition / update_dirichlet / solve_fdtd_near / dt / dl / res / get_grid /
+get_triangles / clear, plus `run` (many steps with on-device listener recording).
+
+Arrays may be numpy arrays (host) or torch tensors on the GPU (device pointers are
+passed through). The library must be built (``make -C pppm-tdbem_amd``); there is no
+fallback: a missing library raises at import of the solver.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpppm_hip.so")
+
+PPPM_FLAG_HONOR_PML = 1
+
+EXPORTS = [
+    "pppm_create", "pppm_destroy", "pppm_set_stream", "pppm_last_error", "pppm_set_mesh",
+    "pppm_update_mesh", "pppm_step", "pppm_run", "pppm_fdtd_step", "pppm_solve_far",
+    "pppm_set_neumann", "pppm_update_dirichlet", "pppm_solve_near", "pppm_reset",
+    "pppm_time_idx", "pppm_grid_ptr", "pppm_read_cells", "pppm_history_ptrs", "pppm_triangles",
+    "pppm_stats", "pppm_get_grid_weights", "pppm_set_grid_weights", "pppm_get_face_weights",
+    "pppm_set_face_weights", "pppm_get_interp_weights", "pppm_get_c3", "pppm_get_b4",
+    "pppm_set_timing", "pppm_get_timing",
+]
+
+# LayerWeightHalf record (src/algorithm/bem.h:163-169): 16 x fp16, f32, 16 x fp16, f32
+LWH_DTYPE = np.dtype([("s", "<u2", (16,)), ("ss", "<f4"), ("d", "<u2", (16,)), ("sd", "<f4")])
+assert LWH_DTYPE.itemsize == 72
+
+_lib = None
+
+
+class PPPMError(RuntimeError):
+    pass
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libpppm_hip.so and declares the C signatures. Raises if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise PPPMError(f"{path} is not built (run `make -C {_HERE}`)")
+    lib = ctypes.CDLL(path)
+    vp, i, f, ll = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_longlong
+    P = ctypes.POINTER
+    sig = {
+        "pppm_create": ([P(vp), i, f, f, vp, i, f, f, ctypes.c_uint], i),
+        "pppm_destroy": ([vp], i),
+        "pppm_set_stream": ([vp, vp], i),
+        "pppm_last_error": ([vp], ctypes.c_char_p),
+        "pppm_set_mesh": ([vp, vp, i, vp, i, i], i),
+        "pppm_update_mesh": ([vp, vp, i], i),
+        "pppm_step": ([vp, vp, i], i),
+        "pppm_run": ([vp, vp, ll, i, vp, i, vp, i], i),
+        "pppm_fdtd_step": ([vp], i),
+        "pppm_solve_far": ([vp], i),
+        "pppm_set_neumann": ([vp, vp, i], i),
+        "pppm_update_dirichlet": ([vp], i),
+        "pppm_solve_near": ([vp], i),
+        "pppm_reset": ([vp], i),
+        "pppm_time_idx": ([vp, P(i)], i),
+        "pppm_grid_ptr": ([vp, i, P(vp)], i),
+        "pppm_read_cells": ([vp, vp, i, vp], i),
+        "pppm_history_ptrs": ([vp, P(vp), P(vp)], i),
+        "pppm_triangles": ([vp, vp], i),
+        "pppm_stats": ([vp, vp], i),
+        "pppm_get_grid_weights": ([vp, vp], i),
+        "pppm_set_grid_weights": ([vp, vp], i),
+        "pppm_get_face_weights": ([vp, vp, vp, vp], i),
+        "pppm_set_face_weights": ([vp, vp], i),
+        "pppm_get_interp_weights": ([vp, vp], i),
+        "pppm_get_c3": ([vp, vp, vp, vp], i),
+        "pppm_get_b4": ([vp, vp, vp, vp], i),
+        "pppm_set_timing": ([vp, i], i),
+        "pppm_get_timing": ([vp, vp, vp], i),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    """(pointer, on_device) for a numpy array or a torch tensor."""
+    if a is None:
+        return None, 0
+    if hasattr(a, "data_ptr"):  # torch tensor
+        if not a.is_contiguous():
+            raise PPPMError("tensor must be contiguous")
+        return ctypes.c_void_p(a.data_ptr()), int(a.is_cuda)
+    arr = np.ascontiguousarray(a)
+    return arr.ctypes.data_as(ctypes.c_void_p), 0
+
+
+class PPPMSolver:
+    """MI355X PPPM solver; the constructor mirrors PPPMSolver(res, dl, dt, min_pos, pml_width,
+    reflect_coeff) (src/algorithm/pppm.h:39-55) with the speed of sound as a parameter."""
+
+    def __init__(self, res, dl, dt, min_pos=(0.0, 0.0, 0.0), pml_width=0, reflect_coeff=0.83,
+                 c=343.0, honor_pml=False):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        mn = np.asarray(min_pos, dtype=np.float32)
+        self._mn = mn
+        flags = PPPM_FLAG_HONOR_PML if honor_pml else 0
+        rc = self.lib.pppm_create(ctypes.byref(h), int(res), float(dl), float(dt),
+                                  mn.ctypes.data_as(ctypes.c_void_p), int(pml_width),
+                                  float(reflect_coeff), float(c), flags)
+        if rc != 0:
+            raise PPPMError(f"pppm_create failed ({rc})")
+        self.h = h
+        self._res, self._dl, self._dt, self._c = int(res), float(dl), float(dt), float(c)
+        self.min_pos = tuple(float(x) for x in mn)
+        self.nt = 0
+        self.nv = 0
+
+    # ------------------------------------------------------------------ plumbing
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.pppm_last_error(self.h)
+            raise PPPMError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def clear(self):
+        if getattr(self, "h", None):
+            self.lib.pppm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.clear()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_handle: int):
+        self._check(self.lib.pppm_set_stream(self.h, ctypes.c_void_p(stream_handle)), "set_stream")
+
+    # ------------------------------------------------------------------ reference-shaped API
+    def dt(self):
+        return self._dt
+
+    def dl(self):
+        return self._dl
+
+    def res(self):
+        return self._res
+
+    def min_coord(self):
+        return self.min_pos
+
+    def time_idx(self):
+        t = ctypes.c_int()
+        self._check(self.lib.pppm_time_idx(self.h, ctypes.byref(t)), "time_idx")
+        return t.value
+
+    def set_mesh(self, verts, tris):
+        if hasattr(verts, "data_ptr"):
+            nv, nt = verts.shape[0], tris.shape[0]
+            v, dv = _ptr(verts)
+            t, dt_ = _ptr(tris)
+            if dv != dt_:
+                raise PPPMError("verts and tris must both be host or both device")
+        else:
+            verts = np.ascontiguousarray(verts, dtype=np.float32)
+            tris = np.ascontiguousarray(tris, dtype=np.int32)
+            nv, nt = verts.shape[0], tris.shape[0]
+            v, dv = _ptr(verts)
+            t, _ = _ptr(tris)
+        self._check(self.lib.pppm_set_mesh(self.h, v, nv, t, nt, dv), "set_mesh")
+        self.nv, self.nt = nv, nt
+
+    def update_mesh(self, verts):
+        if not hasattr(verts, "data_ptr"):
+            verts = np.ascontiguousarray(verts, dtype=np.float32)
+        v, dv = _ptr(verts)
+        self._check(self.lib.pppm_update_mesh(self.h, v, dv), "update_mesh")
+
+    def update_grid_and_face(self, neumann):
+        if not hasattr(neumann, "data_ptr"):
+            neumann = np.ascontiguousarray(neumann, dtype=np.float32)
+        p, d = _ptr(neumann)
+        self._check(self.lib.pppm_step(self.h, p, d), "step")
+
+    update_step = update_grid_and_face
+
+    def run(self, neumann_series_dev, steps, listeners, out_dev, use_graph=True, stride=None):
+        """steps x stride device Neumann series -> out_dev[steps, n_listen] (device)."""
+        lx = np.ascontiguousarray(np.asarray(listeners, dtype=np.int32).reshape(-1, 3))
+        stride = self.nt if stride is None else stride
+        np_, _ = _ptr(neumann_series_dev)
+        op, _ = _ptr(out_dev)
+        self._check(self.lib.pppm_run(self.h, np_, int(stride), int(steps), lx.ctypes.data_as(ctypes.c_void_p),
+                                      lx.shape[0], op, int(use_graph)), "run")
+
+    def fdtd_step(self):
+        self._check(self.lib.pppm_fdtd_step(self.h), "fdtd_step")
+
+    def solve_fdtd_far(self):
+        self._check(self.lib.pppm_solve_far(self.h), "solve_far")
+
+    def set_neumann_condition(self, neumann):
+        if not hasattr(neumann, "data_ptr"):
+            neumann = np.ascontiguousarray(neumann, dtype=np.float32)
+        p, d = _ptr(neumann)
+        self._check(self.lib.pppm_set_neumann(self.h, p, d), "set_neumann")
+
+    def update_dirichlet(self):
+        self._check(self.lib.pppm_update_dirichlet(self.h), "update_dirichlet")
+
+    def solve_fdtd_near(self):
+        self._check(self.lib.pppm_solve_near(self.h), "solve_near")
+
+    PHASES = ("fdtd", "far", "dirichlet", "near", "update_mesh")
+
+    def set_timing(self, enable=True):
+        self._check(self.lib.pppm_set_timing(self.h, int(enable)), "set_timing")
+
+    def get_timing(self):
+        """{phase: (total_ms, launches)} accumulated since the last call (HIP events)."""
+        ms = np.zeros(8, np.float64)
+        n = np.zeros(8, np.int64)
+        self._check(self.lib.pppm_get_timing(self.h, ms.ctypes.data_as(ctypes.c_void_p),
+                                             n.ctypes.data_as(ctypes.c_void_p)), "get_timing")
+        return {name: (float(ms[i]), int(n[i])) for i, name in enumerate(self.PHASES)}
+
+    def reset(self):
+        self._check(self.lib.pppm_reset(self.h), "reset")
+
+    def grid_ptr(self, rel=0):
+        p = ctypes.c_void_p()
+        self._check(self.lib.pppm_grid_ptr(self.h, int(rel), ctypes.byref(p)), "grid_ptr")
+        return p.value
+
+    def get_grid(self, rel=0):
+        """Host copy of grids[t + rel] as an (N, N, N) float32 array (x, y, z; z contiguous)."""
+        n = self._res ** 3
+        out = np.empty(n, dtype=np.float32)
+        rc = _hip_runtime().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(self.grid_ptr(rel)),
+                                      ctypes.c_size_t(4 * n), 2)  # hipMemcpyDeviceToHost (syncs the handle's stream)
+        if rc != 0:
+            raise PPPMError(f"hipMemcpy failed ({rc})")
+        return out.reshape(self._res, self._res, self._res)
+
+    def set_grid(self, rel, values):
+        """Overwrites grids[t + rel] from a host array of N^3 floats."""
+        a = np.ascontiguousarray(values, dtype=np.float32).reshape(-1)
+        if a.size != self._res ** 3:
+            raise PPPMError("set_grid: wrong size")
+        rc = _hip_runtime().hipMemcpy(ctypes.c_void_p(self.grid_ptr(rel)), a.ctypes.data_as(ctypes.c_void_p),
+                                      ctypes.c_size_t(a.nbytes), 1)  # hipMemcpyHostToDevice
+        if rc != 0:
+            raise PPPMError(f"hipMemcpy failed ({rc})")
+
+    def get_history(self):
+        """Host copies of the Neumann and Dirichlet rings, (T, 32) each."""
+        pn, pd = ctypes.c_void_p(), ctypes.c_void_p()
+        self._check(self.lib.pppm_history_ptrs(self.h, ctypes.byref(pn), ctypes.byref(pd)), "history_ptrs")
+        out = []
+        for p in (pn, pd):
+            a = np.empty((self.nt, 32), dtype=np.float32)
+            rc = _hip_runtime().hipMemcpy(a.ctypes.data_as(ctypes.c_void_p), p, ctypes.c_size_t(a.nbytes), 2)
+            if rc != 0:
+                raise PPPMError(f"hipMemcpy failed ({rc})")
+            out.append(a)
+        return out[0], out[1]
+
+    def read_cells(self, xyz):
+        xyz = np.ascontiguousarray(np.asarray(xyz, dtype=np.int32).reshape(-1, 3))
+        out = np.zeros(xyz.shape[0], dtype=np.float32)
+        self._check(self.lib.pppm_read_cells(self.h, xyz.ctypes.data_as(ctypes.c_void_p), xyz.shape[0],
+                                             out.ctypes.data_as(ctypes.c_void_p)), "read_cells")
+        return out
+
+    def get_triangles(self):
+        out = np.zeros((self.nt, 13), dtype=np.float32)
+        self._check(self.lib.pppm_triangles(self.h, out.ctypes.data_as(ctypes.c_void_p)), "triangles")
+        return out
+
+    def stats(self):
+        out = np.zeros(8, dtype=np.int64)
+        self._check(self.lib.pppm_stats(self.h, out.ctypes.data_as(ctypes.c_void_p)), "stats")
+        keys = ["T", "C3", "P3", "B4", "P4", "grid_recomputes", "face_recomputes", "pool"]
+        return dict(zip(keys, (int(x) for x in out)))
+
+    def get_grid_weights(self):
+        rec = np.zeros(self.nt * 27 * 2, dtype=LWH_DTYPE)
+        self._check(self.lib.pppm_get_grid_weights(self.h, rec.ctypes.data_as(ctypes.c_void_p)), "get_grid_weights")
+        return rec
+
+    def set_grid_weights(self, rec):
+        rec = np.ascontiguousarray(rec, dtype=LWH_DTYPE)
+        self._check(self.lib.pppm_set_grid_weights(self.h, rec.ctypes.data_as(ctypes.c_void_p)), "set_grid_weights")
+
+    def get_face_weights(self):
+        st = self.stats()
+        start = np.zeros(self.nt + 1, dtype=np.int32)
+        nbr = np.zeros(max(st["P4"], 1), dtype=np.int32)
+        rec = np.zeros(max(st["P4"], 1), dtype=LWH_DTYPE)
+        self._check(self.lib.pppm_get_face_weights(self.h, start.ctypes.data_as(ctypes.c_void_p),
+                                                   nbr.ctypes.data_as(ctypes.c_void_p),
+                                                   rec.ctypes.data_as(ctypes.c_void_p)), "get_face_weights")
+        return start, nbr[:st["P4"]], rec[:st["P4"]]
+
+    def set_face_weights(self, rec):
+        rec = np.ascontiguousarray(rec, dtype=LWH_DTYPE)
+        self._check(self.lib.pppm_set_face_weights(self.h, rec.ctypes.data_as(ctypes.c_void_p)), "set_face_weights")
+
+    def get_interp_weights(self):
+        out = np.zeros((self.nt, 8), dtype=np.float32)
+        self._check(self.lib.pppm_get_interp_weights(self.h, out.ctypes.data_as(ctypes.c_void_p)), "interp")
+        return out
+
+    def get_c3(self):
+        st = self.stats()
+        cells = np.zeros(max(st["C3"], 1), dtype=np.int32)
+        start = np.zeros(st["C3"] + 1, dtype=np.int32)
+        pairs = np.zeros(max(st["P3"], 1), dtype=np.int32)
+        self._check(self.lib.pppm_get_c3(self.h, cells.ctypes.data_as(ctypes.c_void_p),
+                                         start.ctypes.data_as(ctypes.c_void_p),
+                                         pairs.ctypes.data_as(ctypes.c_void_p)), "get_c3")
+        return cells[:st["C3"]], start, pairs[:st["P3"]]
+
+    def get_b4(self):
+        st = self.stats()
+        cells = np.zeros(max(st["B4"], 1), dtype=np.int32)
+        start = np.zeros(st["B4"] + 1, dtype=np.int32)
+        self._check(self.lib.pppm_get_b4(self.h, cells.ctypes.data_as(ctypes.c_void_p),
+                                         start.ctypes.data_as(ctypes.c_void_p), None), "get_b4")
+        tris = np.zeros(max(int(start[-1]), 1), dtype=np.int32)
+        self._check(self.lib.pppm_get_b4(self.h, None, None, tris.ctypes.data_as(ctypes.c_void_p)), "get_b4")
+        return cells[:st["B4"]], start, tris[:int(start[-1])]
+
+
+_hip = None
+
+
+def _hip_runtime():
+    global _hip
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        _hip.hipMemcpy.restype = ctypes.c_int
+    return _hip

@@ -1,0 +1,212 @@
+"""GPU parity: libpppm_hip.so (through its C ABI) against the CPU oracle.
+
+Bars: bit-exact for index work (binning, lists) and for the FDTD update (FP64 stencil and
+FP32 ABC are evaluated with the same IEEE operations in the same order); FP16 weight codes
+equal except for rare one-ulp flips from FP64 transcendental differences; listener within
+1e-5 relative L2 (the gate), full grids within 1e-5 relative L2.
+"""
+import numpy as np
+import pytest
+
+from conftest import random_small_triangles
+
+pytestmark = pytest.mark.gpu
+
+LISTEN_TOL = 1e-5
+
+
+def _pair(amd, O, sc, mode=None):
+    mode = O.FP64 if mode is None else mode
+    g = amd.PPPMSolver(sc.res, sc.dl, sc.dt, sc.min_pos, 0, 0.83, sc.c)
+    g.set_mesh(sc.verts, sc.tris)
+    o = O.Oracle(sc.res, sc.dl, sc.dt, sc.min_pos, sc.c, 0.83, mode)
+    o.set_mesh(sc.verts, sc.tris)
+    return g, o
+
+
+@pytest.fixture(scope="module")
+def c1_short(scenes_mod):
+    return scenes_mod.c1_static_sphere(steps=1000)
+
+
+def test_fdtd_abc_bitexact(amd, oracle_mod):
+    """FDTD::step with the ABC (fdtd.cu:20-150): random levels t-1, t; 4 steps; bit-exact."""
+    rng = np.random.default_rng(1)
+    N, h = 20, 0.01
+    dt = h / (np.sqrt(3) * 343 * 1.01)
+    g = amd.PPPMSolver(N, h, dt, (0, 0, 0), 0, 0.83, 343.0)
+    o = oracle_mod.Oracle(N, h, dt, (0, 0, 0), 343.0, 0.83, oracle_mod.FP64)
+    for rel in (-1, 0):
+        x = rng.standard_normal(N ** 3).astype(np.float32)
+        g.set_grid(rel, x)
+        o.set_grid(rel, x)
+    for _ in range(4):
+        g.fdtd_step()
+        o.fdtd_step()
+        for rel in (0, -1):
+            a, b = g.get_grid(rel), o.grid(rel)
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), f"level {rel} differs"
+    assert g.time_idx() == o.time_idx() == 3
+
+
+def test_fdtd_pml_matches(amd, oracle_mod):
+    """pml_step_kernel (fdtd.cu:179-232) with the PML honoured; FP32, bit-exact."""
+    rng = np.random.default_rng(2)
+    N, h, w = 20, 0.01, 4
+    dt = h / (np.sqrt(3) * 343 * 1.01)
+    g = amd.PPPMSolver(N, h, dt, (0, 0, 0), w, 0.83, 343.0, honor_pml=True)
+    o = oracle_mod.Oracle(N, h, dt, (0, 0, 0), 343.0, 0.83, oracle_mod.FP64, pml_width=w)
+    x = rng.standard_normal(N ** 3).astype(np.float32)
+    for rel in (-1, 0):
+        g.set_grid(rel, x)
+        o.set_grid(rel, x)
+    for _ in range(6):
+        g.fdtd_step()
+        o.fdtd_step()
+    a, b = g.get_grid(0), o.grid(0)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_binning_and_lists_bitexact(amd, oracle_mod, c1_short):
+    g, o = _pair(amd, oracle_mod, c1_short)
+    tg, to = g.get_triangles(), o.triangles()
+    assert np.array_equal(tg.view(np.uint32), to.view(np.uint32))
+    gs, os_ = g.stats(), o.stats()
+    for k in ("T", "C3", "P3", "B4", "P4"):
+        assert gs[k] == os_[k], k
+    for a, b in zip(g.get_c3(), o.c3()):
+        assert np.array_equal(a, b)
+    for a, b in zip(g.get_b4(), o.b4()):
+        assert np.array_equal(a, b)
+
+
+def _code_diff(a, b):
+    """max |fp16 code difference| between two LayerWeightHalf arrays, and the flip fraction."""
+    da = np.abs(a["s"].astype(np.int32) - b["s"].astype(np.int32))
+    dd = np.abs(a["d"].astype(np.int32) - b["d"].astype(np.int32))
+    m = max(int(da.max()), int(dd.max()))
+    frac = (np.count_nonzero(da) + np.count_nonzero(dd)) / (da.size + dd.size)
+    return m, frac
+
+
+def test_grid_weights_fp64(amd, oracle_mod, c1_short):
+    g, o = _pair(amd, oracle_mod, c1_short)
+    a, b = g.get_grid_weights(), o.grid_weights()
+    m, frac = _code_diff(a, b)
+    print(f"grid-cache FP16 codes: max |diff| {m}, flipped fraction {frac:.2e} of {a.size * 32}")
+    assert frac < 1e-3 and m <= 1, (m, frac)
+    for k in ("ss", "sd"):
+        assert np.allclose(a[k], b[k], rtol=1e-6, atol=0)
+    ia, ib = g.get_interp_weights(), o.interp_weights()
+    assert np.allclose(ia, ib, rtol=1e-6, atol=1e-12)
+
+
+def test_face_weights_fp64(amd, oracle_mod, c1_short):
+    g, o = _pair(amd, oracle_mod, c1_short)
+    sa, na, ra = g.get_face_weights()
+    sb, nb, rb = o.face_weights()
+    assert np.array_equal(sa, sb) and np.array_equal(na, nb)
+    m, frac = _code_diff(ra, rb)
+    print(f"face-cache FP16 codes: max |diff| {m}, flipped fraction {frac:.2e} of {ra.size * 32}")
+    assert frac < 1e-3 and m <= 1, (m, frac)
+    for k in ("ss", "sd"):
+        assert np.allclose(ra[k], rb[k], rtol=1e-6, atol=0)
+
+
+def test_step_with_shared_weights(amd, oracle_mod, c1_short):
+    """Step kernels alone: the oracle's weights loaded into the GPU solver, 1000 steps.
+    Every sum runs in the oracle's order, so grids, rings and listener are bit-exact."""
+    sc = c1_short
+    g, o = _pair(amd, oracle_mod, sc)
+    g.set_grid_weights(o.grid_weights())
+    g.set_face_weights(o.face_weights()[2])
+    sig_g, sig_o = [], []
+    for i in range(sc.steps):
+        g.update_grid_and_face(sc.neumann[i])
+        o.step(sc.neumann[i])
+        if i % 50 == 49:
+            sig_g.append(g.read_cells([sc.listener])[0])
+            sig_o.append(o.read(*sc.listener))
+    assert np.array_equal(np.array(sig_g, np.float32), np.array(sig_o, np.float32))
+    for rel in (0, -1):
+        assert np.array_equal(g.get_grid(rel).view(np.uint32), o.grid(rel).view(np.uint32)), rel
+    ha, hb = g.get_history(), o.history()
+    assert np.array_equal(ha[0].view(np.uint32), hb[0].view(np.uint32))
+    assert np.array_equal(ha[1].view(np.uint32), hb[1].view(np.uint32))
+
+
+def test_c1_listener_parity(amd, oracle_mod, scenes_mod, c1_short):
+    """Config 1, 1000 steps, GPU-built FP64 weights vs the FP64 oracle: the 1e-5 gate."""
+    import torch
+    sc = c1_short
+    g, o = _pair(amd, oracle_mod, sc)
+    dev_n = torch.from_numpy(sc.neumann).cuda()
+    out = torch.zeros(sc.steps, 1, device="cuda")
+    g.run(dev_n, sc.steps, [sc.listener], out, use_graph=True)
+    sig_g = out[:, 0].cpu().numpy()
+    sig_o = []
+    for i in range(sc.steps):
+        o.step(sc.neumann[i])
+        sig_o.append(o.read(*sc.listener))
+    sig_o = np.array(sig_o)
+    err = scenes_mod.rel_l2(sig_g, sig_o)
+    print(f"C1 listener rel L2 GPU vs FP64 oracle: {err:.3e}")
+    assert err < LISTEN_TOL
+    grid_err = scenes_mod.rel_l2(g.get_grid(0), o.grid(0))
+    assert grid_err < LISTEN_TOL
+
+
+def test_graph_equals_eager(amd, c1_short):
+    import torch
+    sc = c1_short
+    steps = 70
+    outs = []
+    for use_graph in (False, True):
+        g = amd.PPPMSolver(sc.res, sc.dl, sc.dt, sc.min_pos, 0, 0.83, sc.c)
+        g.set_mesh(sc.verts, sc.tris)
+        dev_n = torch.from_numpy(sc.neumann[:steps]).cuda()
+        out = torch.zeros(steps, 1, device="cuda")
+        g.run(dev_n, steps, [sc.listener], out, use_graph=use_graph)
+        outs.append((out.cpu().numpy(), g.get_grid(0)))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_moving_mesh_parity(amd, oracle_mod, scenes_mod):
+    """Random small triangles moved every step (ParticleGridTest-style moves, seeded):
+    update_mesh re-binning, dirty recomputes and the step, vs the oracle."""
+    rng = np.random.default_rng(7)
+    N, h = 24, 0.005
+    dt = h / (np.sqrt(3) * 343)
+    verts, tris = random_small_triangles(rng, 40, N, h, size=1.0, margin=5)
+    g = amd.PPPMSolver(N, h, dt, (0, 0, 0), 0, 0.83, 343.0)
+    o = oracle_mod.Oracle(N, h, dt, (0, 0, 0), 343.0, 0.83, oracle_mod.FP64)
+    g.set_mesh(verts, tris)
+    o.set_mesh(verts, tris)
+    lis = (18, 18, 18)
+    sg, so = [], []
+    v = verts.copy()
+    for i in range(60):
+        v = (v + (rng.random(v.shape) - 0.5).astype(np.float32) * np.float32(0.02 * h)).astype(np.float32)
+        g.update_mesh(v)
+        o.update_mesh(v)
+        assert g.stats()["grid_recomputes"] == o.stats()["grid_recomputes"]
+        assert g.stats()["face_recomputes"] == o.stats()["face_recomputes"]
+        nm = rng.standard_normal(tris.shape[0]).astype(np.float32)
+        g.update_grid_and_face(nm)
+        o.step(nm)
+        sg.append(g.read_cells([lis])[0])
+        so.append(o.read(*lis))
+    for a, b in zip(g.get_c3(), o.c3()):
+        assert np.array_equal(a, b)
+    err = scenes_mod.rel_l2(sg, so)
+    assert err < LISTEN_TOL, err
+
+
+def test_error_paths(amd):
+    g = amd.PPPMSolver(16, 0.01, 1e-5, (0, 0, 0), 0, 0.83, 343.0)
+    with pytest.raises(amd.PPPMError):
+        g.update_grid_and_face(np.zeros(1, np.float32))  # before set_mesh
+    v = np.array([[0, 0, 0], [0.001, 0, 0], [0, 0.001, 0]], np.float32)  # cell 0: outside [2, N-3]
+    with pytest.raises(amd.PPPMError):
+        g.set_mesh(v, np.array([[0, 1, 2]], np.int32))
