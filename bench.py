@@ -110,13 +110,15 @@ def main():
     lis = [sc.listener]
     out = torch.zeros(total, 1, device="cuda")
 
-    def run_steps(i0, n):
+    def run_steps(i0, n, use_graph=True):
         if moving:
+            # the demo loop's per-sample UpdateSound: update_mesh, then update_grid_and_face
+            # (experiments/demo/main.cu:170-205); rows are already in HBM
             for i in range(i0, i0 + n):
                 solver.update_mesh(verts_all[i])
                 solver.update_grid_and_face(neumann[i])
         else:
-            solver.run(neumann[i0:i0 + n], n, lis, out[i0:i0 + n], use_graph=True)
+            solver.run(neumann[i0:i0 + n], n, lis, out[i0:i0 + n], use_graph=use_graph)
 
     with torch.cuda.stream(stream):
         run_steps(0, W)
@@ -131,11 +133,7 @@ def main():
         st = solver.stats()
         # per-kernel durations: HIP events around each launch, on the solver's stream
         solver.set_timing(True)
-        i0 = W + K
-        if moving:
-            run_steps(i0, args.timing_steps)
-        else:
-            solver.run(neumann[i0:], args.timing_steps, lis, out[i0:], use_graph=False)
+        run_steps(W + K, args.timing_steps, use_graph=False)
         solver.set_timing(False)
         phases = solver.get_timing()
         stream.synchronize()

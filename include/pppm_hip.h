@@ -76,6 +76,15 @@ int pppm_step(pppm_solver *s, const float *neumann, int on_device);
 int pppm_run(pppm_solver *s, const float *neumann_series, long long stride, int steps, const int *listen_xyz,
              int n_listen, float *out_dev, int use_graph);
 
+/* pppm_run for a moving mesh: before step i the vertices are replaced by row i of
+ * verts_series (device memory, steps x vstride floats, vstride >= 3 nv) and the mesh is
+ * updated as by pppm_update_mesh — the demo loop's UpdateSound (experiments/demo/main.cu:
+ * 170-205: object transform, update_mesh, update_grid_and_face). Each update is
+ * synchronous (its status is read back and capacities grow as needed), so use_graph is
+ * ignored for moving meshes. */
+int pppm_run_moving(pppm_solver *s, const float *verts_series, long long vstride, const float *neumann_series,
+                    long long stride, int steps, const int *listen_xyz, int n_listen, float *out_dev, int use_graph);
+
 /* Phase entry points used by the reference's tests (src/algorithm/pppm.h:127-135,
  * src/algorithm/fdtd.h:73-87). */
 int pppm_fdtd_step(pppm_solver *s);                              /* FDTD::step */

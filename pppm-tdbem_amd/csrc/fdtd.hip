@@ -10,8 +10,8 @@
 // blocks solve the 4x4 ABC system for the index-1 / N-2 shell. Interior and shell
 // write disjoint cells of level t+1; the shell also writes the ghost layer of level t,
 // which no thread of this launch reads except the writing thread itself (masked
-// terms of its own system), so the fusion is race-free. The last block to finish
-// advances t.
+// terms of its own system), so the fusion is race-free. t is read from device memory
+// (so the launch can be replayed from a graph); a one-thread kernel advances it after.
 #include "internal.hpp"
 
 namespace pppm_amd
@@ -26,22 +26,6 @@ __device__ inline float *level(float *g3, int t, int rel, size_t nn)
 {
     int l = ((t + rel) % 3 + 3) % 3;
     return g3 + (size_t)l * nn;
-}
-
-__device__ inline void advance_t(StepState *st)
-{
-    __syncthreads();
-    if (threadIdx.x == 0)
-    {
-        __threadfence();
-        unsigned prev = atomicAdd(&st->done, 1u);
-        if (prev == gridDim.x - 1)
-        {
-            atomicAdd(&st->t, 1);
-            atomicExch(&st->done, 0u);
-            __threadfence();
-        }
-    }
 }
 
 // Solve4x4Linear (fdtd.cu:36-86): cofactor inverse in FP32, FP64 reciprocal of the determinant.
@@ -136,7 +120,7 @@ __device__ inline void abc_cell(float *g, const float *gm, float *gp, int N, int
             g[me + judge[d] * strd[d]] = r[d];
 }
 
-__global__ __launch_bounds__(BLK) void k_fdtd_abc(float *__restrict__ grid3, FdtdParams p, StepState *st,
+__global__ __launch_bounds__(BLK) void k_fdtd_abc(float *__restrict__ grid3, FdtdParams p, const StepState *__restrict__ st,
                                                   int nb_inner)
 {
     const int N = p.N;
@@ -205,7 +189,6 @@ __global__ __launch_bounds__(BLK) void k_fdtd_abc(float *__restrict__ grid3, Fdt
         if (i >= 0)
             abc_cell(g, gm, gp, N, i, j, k, p);
     }
-    advance_t(st);
 }
 
 // get_damp (fdtd.cu:162-169)
@@ -219,7 +202,7 @@ __device__ inline float pml_damp(const FdtdParams &p, int x, int y, int z)
 
 // pml_step_kernel (fdtd.cu:179-232); PML state: 6 floats per cell (v1[3], v2[3]) per level
 __global__ __launch_bounds__(BLK) void k_fdtd_pml(float *__restrict__ grid3, float *__restrict__ pml3, FdtdParams p,
-                                                  StepState *st)
+                                                  const StepState *__restrict__ st)
 {
     const int N = p.N;
     const size_t nn = (size_t)N * N * N;
@@ -282,12 +265,11 @@ __global__ __launch_bounds__(BLK) void k_fdtd_pml(float *__restrict__ grid3, flo
             }
         }
     }
-    advance_t(st);
 }
 
 }  // namespace
 
-hipError_t launch_fdtd(float *grid3, float *pml3, const FdtdParams &p, StepState *st, hipStream_t s)
+hipError_t launch_fdtd(float *grid3, float *pml3, const FdtdParams &p, const StepState *st, hipStream_t s)
 {
     const int N = p.N;
     if (p.pml > 0)

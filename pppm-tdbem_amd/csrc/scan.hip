@@ -76,11 +76,8 @@ __global__ __launch_bounds__(SB) void k_scan_add(int *__restrict__ out, int n, c
     }
 }
 
-__global__ void k_write_total(int *out, int n, const int *sums_scanned, int nb)
-{
-    // out[n] = total = exclusive(sums)[nb-1] + sums[nb-1] is stored by the level below in sums_scanned[nb]
-    out[n] = sums_scanned[nb];
-}
+// out[n] = total, read from src[nb] (the level below stores its total there); src == null -> 0
+__global__ void k_write_total(int *out, int n, const int *src, int nb) { out[n] = src ? src[nb] : 0; }
 
 }  // namespace
 
@@ -99,12 +96,14 @@ size_t scan_tmp_ints(int n)
     return tot + 16;
 }
 
-// out must hold n+1 ints
+// out must hold n+1 ints. Only kernels are enqueued (no memset/memcpy nodes), so the
+// scan can be captured in a HIP graph.
 hipError_t scan_exclusive(const int *in, int *out, int n, int *tmp, hipStream_t s)
 {
     if (n <= 0)
     {
-        return hipMemsetAsync(out, 0, sizeof(int), s);
+        k_write_total<<<1, 1, 0, s>>>(out, 0, nullptr, 0);
+        return hipGetLastError();
     }
     int nb = (n + SPB - 1) / SPB;
     int *sums = tmp;            // nb
@@ -113,7 +112,8 @@ hipError_t scan_exclusive(const int *in, int *out, int n, int *tmp, hipStream_t 
     if (nb == 1)
     {
         // total = sums[0]
-        return hipMemcpyAsync(out + n, sums, sizeof(int), hipMemcpyDeviceToDevice, s);
+        k_write_total<<<1, 1, 0, s>>>(out, n, sums, 0);
+        return hipGetLastError();
     }
     hipError_t e = scan_exclusive(sums, sums_ex, nb, tmp + 2 * nb + 1, s);
     if (e != hipSuccess)

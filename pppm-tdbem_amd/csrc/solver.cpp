@@ -5,6 +5,11 @@
 // set_mesh / update_mesh / construct_neighbor_lists (src/algorithm/particle_grid.h:114-148,
 // particle_grid.cu:133-241), GridCache / FaceCache::update_cache (pppm_cache.cu:125-134,
 // 273-289), FDTD::init / step / reset (src/algorithm/fdtd.h:45-110).
+//
+// A mesh update is one asynchronous kernel sequence: every array has a host-known capacity
+// and every live count stays in device memory, so no device-to-host round trip is needed
+// and a moving-mesh time step (update + step) can be captured once and replayed. Counts
+// and error bits are gathered into a status block that the host reads at sync points.
 #include "pppm_hip.h"
 
 #include <algorithm>
@@ -20,11 +25,16 @@ using namespace pppm_amd;
 namespace
 {
 
+// owning device buffer; grows on demand, contents are not preserved by ensure()
 template <class T>
 struct DBuf
 {
     T *p = nullptr;
     size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
     hipError_t ensure(size_t m)
     {
         if (m == 0)
@@ -49,6 +59,14 @@ struct DBuf
         p = nullptr;
         n = 0;
     }
+    void take(DBuf &o)
+    {
+        release();
+        p = o.p;
+        n = o.n;
+        o.p = nullptr;
+        o.n = 0;
+    }
 };
 
 struct LWH72
@@ -59,6 +77,20 @@ struct LWH72
     float sd;
 };
 static_assert(sizeof(LWH72) == 72, "LayerWeightHalf layout");
+
+// status block layout
+enum
+{
+    ST_ERR = 0,
+    ST_C3 = 1,
+    ST_B4 = 2,
+    ST_B4ITEMS = 3,
+    ST_P4 = 4,
+    ST_GDIRTY = 5,
+    ST_FDIRTY = 6,
+    ST_POOL = 7,
+    ST_N = 8
+};
 
 }  // namespace
 
@@ -82,22 +114,27 @@ struct pppm_solver
     int nv = 0, nt = 0;
     bool mesh_set = false;
     DBuf<float> verts;
-    DBuf<int> tris, err_flag;
+    DBuf<int> tris;
     DBuf<TriGeom> geom;
 
-    DBuf<int> cell_cnt, base_cnt, cell_start, base_start, cell_fill, base_fill, cell_items, base_items;
-    DBuf<int> flag, pos, scan_tmp;
-    int n_c3 = 0, n_p3 = 0, n_b4 = 0, n_p4 = 0;
-    DBuf<int> c3_cells, c3_index, c3_cnt, c3_start, c3_pairs;
+    // capacities (host-known) and derived launch sizes
+    int cap_c3 = 0, cap_b4items = 0, cap_p4 = 0;
+    int c3_waves = 0, face_waves = 0;
+
+    DBuf<int> counters;  // [cell_cnt | base_cnt | cell_fill | base_fill], 4 x N^3
+    DBuf<int> cell_start, base_start, cell_items, base_items, cell_rank, base_rank;
+    DBuf<int> flag, c3_pos, b4_pos, scan_tmp;
+    DBuf<int> c3_cells, c3_index, c3_cnt, c3_start, c3_pairs, c3_wave;
     DBuf<int> b4_cells, b4_index, b4_cnt, b4_start, b4_tris;
-    DBuf<int> face_cnt, face_start, face_nbr, face_slot, face_owner, face_dirty, dirty_list;
-    DBuf<int> corner;
+    DBuf<int> face_cnt, face_start, face_nbr, face_slot, face_owner, face_dirty, face_flag, face_pos, dirty_list;
+    DBuf<int> face_wave, corner;
+    DBuf<int> gflag, gpos, glist;
+    DBuf<char> gtmp, ftmp;
+    DBuf<int> errflag, status;
 
     DBuf<uint32_t> wf, wb;
     DBuf<float2> sf, sb;
     DBuf<float> gmeta, interp;
-    DBuf<int> gflag, glist;
-    DBuf<double> denom, ftmp;
 
     int pool_cap = 0, hcap = 0, pool_count = 0;
     DBuf<uint32_t> fw;
@@ -107,6 +144,7 @@ struct pppm_solver
 
     DBuf<float> hn, hd, far, nstage;
     DBuf<int> lcell, lc3;
+    int st_h[ST_N] = {0};
     long long last_grid_recomputes = 0, last_face_recomputes = 0;
 
     // per-phase GPU timing with HIP events on the handle's stream (the reference's
@@ -139,6 +177,14 @@ int fail(pppm_solver *s, int code, const std::string &msg)
         hipError_t e_ = (expr);                                                                  \
         if (e_ != hipSuccess)                                                                    \
             return fail(s, PPPM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+#define RC(expr)          \
+    do                    \
+    {                     \
+        int rc_ = (expr); \
+        if (rc_)          \
+            return rc_;   \
     } while (0)
 
 FdtdParams fdtd_params(const pppm_solver *s)
@@ -175,7 +221,6 @@ StepArgs step_args(const pppm_solver *s, int t0, long long stride, int n_listen)
     StepArgs a;
     a.N = s->N;
     a.nt = s->nt;
-    a.n_c3 = s->n_c3;
     a.rc = s->rc;
     a.t0 = t0;
     a.stride = stride;
@@ -183,223 +228,7 @@ StepArgs step_args(const pppm_solver *s, int t0, long long stride, int n_listen)
     return a;
 }
 
-int read_int(pppm_solver *s, const int *dev, int *out)
-{
-    CK(hipMemcpyAsync(out, dev, sizeof(int), hipMemcpyDeviceToHost, s->stream));
-    CK(hipStreamSynchronize(s->stream));
-    return PPPM_OK;
-}
-
-// exclusive scan of n ints into out (n+1 entries); returns the total on the host
-int scan_total(pppm_solver *s, const int *in, int *out, int n, int *total)
-{
-    CK(s->scan_tmp.ensure(scan_tmp_ints(n)));
-    CK(scan_exclusive(in, out, n, s->scan_tmp.p, s->stream));
-    return read_int(s, out + n, total);
-}
-
-// grows the face-cache pool (keeping its contents) and the hash table (rehashing)
-int ensure_pool(pppm_solver *s, int need)
-{
-    if (need > s->pool_cap)
-    {
-        int ncap = std::max(need, s->pool_cap * 2);
-        DBuf<uint32_t> nfw;
-        DBuf<float2> nfsc, nfmeta;
-        CK(nfw.ensure((size_t)ncap * STEPS));
-        CK(nfsc.ensure(ncap));
-        CK(nfmeta.ensure(ncap));
-        if (s->pool_count > 0)
-        {
-            CK(hipMemcpyAsync(nfw.p, s->fw.p, (size_t)s->pool_count * STEPS * sizeof(uint32_t),
-                              hipMemcpyDeviceToDevice, s->stream));
-            CK(hipMemcpyAsync(nfsc.p, s->fsc.p, (size_t)s->pool_count * sizeof(float2), hipMemcpyDeviceToDevice,
-                              s->stream));
-            CK(hipMemcpyAsync(nfmeta.p, s->fmeta.p, (size_t)s->pool_count * sizeof(float2),
-                              hipMemcpyDeviceToDevice, s->stream));
-        }
-        CK(hipStreamSynchronize(s->stream));
-        s->fw.release();
-        s->fsc.release();
-        s->fmeta.release();
-        s->fw = nfw;
-        s->fsc = nfsc;
-        s->fmeta = nfmeta;
-        nfw.p = nullptr;
-        nfsc.p = nullptr;
-        nfmeta.p = nullptr;
-        s->pool_cap = ncap;
-    }
-    int want = 1024;
-    while (want < 2 * s->pool_cap)
-        want <<= 1;
-    if (want > s->hcap)
-    {
-        DBuf<unsigned long long> nk;
-        DBuf<int> nvv;
-        CK(nk.ensure(want));
-        CK(nvv.ensure(want));
-        CK(hipMemsetAsync(nk.p, 0xff, (size_t)want * sizeof(unsigned long long), s->stream));
-        if (s->hcap > 0)
-            CK(launch_rehash(s->hkeys.p, s->hvals.p, s->hcap, nk.p, nvv.p, want, s->stream));
-        CK(hipStreamSynchronize(s->stream));
-        s->hkeys.release();
-        s->hvals.release();
-        s->hkeys = nk;
-        s->hvals = nvv;
-        nk.p = nullptr;
-        nvv.p = nullptr;
-        s->hcap = want;
-    }
-    return PPPM_OK;
-}
-
-// Geometry, binning, lists and weight caches for the current vertex positions.
-int rebuild(pppm_solver *s, bool first)
-{
-    const int N = s->N, nt = s->nt;
-    const int NN = N * N * N;
-    hipStream_t st = s->stream;
-    int rc;
-
-    // ---- triangle geometry (update_triangles_kernel)
-    CK(s->err_flag.ensure(1));
-    CK(hipMemsetAsync(s->err_flag.p, 0, sizeof(int), st));
-    BuildArgs ba{N, nt, s->nv, s->h, s->minp[0], s->minp[1], s->minp[2]};
-    CK(launch_tri_geometry(s->verts.p, s->tris.p, s->geom.p, ba, s->err_flag.p, st));
-    int ef = 0;
-    if ((rc = read_int(s, s->err_flag.p, &ef)))
-        return rc;
-    if (ef & 2)
-        return fail(s, PPPM_ERR_ARG, "triangle references a vertex index out of range");
-    if (ef & 1)
-        return fail(s, PPPM_ERR_OUT_OF_GRID, "a triangle centre lies outside grid cells [2, N-3]");
-
-    // ---- cell / base-cell CSR
-    CK(s->cell_cnt.ensure(NN));
-    CK(s->base_cnt.ensure(NN));
-    CK(s->cell_fill.ensure(NN));
-    CK(s->base_fill.ensure(NN));
-    CK(s->cell_start.ensure(NN + 1));
-    CK(s->base_start.ensure(NN + 1));
-    CK(s->cell_items.ensure(nt));
-    CK(s->base_items.ensure(nt));
-    CK(hipMemsetAsync(s->cell_cnt.p, 0, (size_t)NN * sizeof(int), st));
-    CK(hipMemsetAsync(s->base_cnt.p, 0, (size_t)NN * sizeof(int), st));
-    CK(hipMemsetAsync(s->cell_fill.p, 0, (size_t)NN * sizeof(int), st));
-    CK(hipMemsetAsync(s->base_fill.p, 0, (size_t)NN * sizeof(int), st));
-    CK(launch_count_cells(s->geom.p, nt, s->cell_cnt.p, s->base_cnt.p, st));
-    CK(s->scan_tmp.ensure(scan_tmp_ints(NN)));
-    CK(scan_exclusive(s->cell_cnt.p, s->cell_start.p, NN, s->scan_tmp.p, st));
-    CK(scan_exclusive(s->base_cnt.p, s->base_start.p, NN, s->scan_tmp.p, st));
-    CK(launch_fill_cells(s->geom.p, nt, s->cell_start.p, s->base_start.p, s->cell_fill.p, s->base_fill.p,
-                         s->cell_items.p, s->base_items.p, st));
-    CK(launch_sort_cells(s->cell_start.p, s->cell_items.p, NN, st));
-    CK(launch_sort_cells(s->base_start.p, s->base_items.p, NN, st));
-
-    // ---- C3 cells: within +-1 of a triangle cell, linear order
-    CK(s->flag.ensure(NN));
-    CK(s->pos.ensure(NN + 1));
-    CK(hipMemsetAsync(s->flag.p, 0, (size_t)NN * sizeof(int), st));
-    CK(launch_mark27(s->geom.p, nt, N, s->flag.p, st));
-    if ((rc = scan_total(s, s->flag.p, s->pos.p, NN, &s->n_c3)))
-        return rc;
-    CK(s->c3_cells.ensure(s->n_c3));
-    CK(s->c3_index.ensure(NN));
-    CK(launch_scatter_selected(s->flag.p, s->pos.p, NN, s->c3_cells.p, s->c3_index.p, st));
-    CK(s->c3_cnt.ensure(s->n_c3));
-    CK(s->c3_start.ensure(s->n_c3 + 1));
-    CK(launch_count_nbr(s->c3_cells.p, s->n_c3, s->cell_start.p, N, 3, s->c3_cnt.p, st));
-    if ((rc = scan_total(s, s->c3_cnt.p, s->c3_start.p, s->n_c3, &s->n_p3)))
-        return rc;
-    CK(s->c3_pairs.ensure(s->n_p3));
-    CK(launch_fill_c3(s->c3_cells.p, s->n_c3, s->cell_start.p, s->cell_items.p, s->c3_start.p, N, s->c3_pairs.p, st));
-    CK(s->far.ensure(s->n_c3));
-
-    // ---- non-empty base cells and their 4x4x4 lists
-    CK(launch_flag_base(s->base_cnt.p, N, s->flag.p, st));
-    if ((rc = scan_total(s, s->flag.p, s->pos.p, NN, &s->n_b4)))
-        return rc;
-    CK(s->b4_cells.ensure(s->n_b4));
-    CK(s->b4_index.ensure(NN));
-    CK(launch_scatter_selected(s->flag.p, s->pos.p, NN, s->b4_cells.p, s->b4_index.p, st));
-    CK(s->b4_cnt.ensure(s->n_b4));
-    CK(s->b4_start.ensure(s->n_b4 + 1));
-    CK(launch_count_nbr(s->b4_cells.p, s->n_b4, s->cell_start.p, N, 4, s->b4_cnt.p, st));
-    int n_b4_items = 0;
-    if ((rc = scan_total(s, s->b4_cnt.p, s->b4_start.p, s->n_b4, &n_b4_items)))
-        return rc;
-    CK(s->b4_tris.ensure(n_b4_items));
-    CK(launch_fill_b4(s->b4_cells.p, s->n_b4, s->cell_start.p, s->cell_items.p, s->b4_start.p, N, s->b4_tris.p, st));
-
-    // ---- per-triangle face lists (the 4x4x4 list of its base cell)
-    CK(s->face_cnt.ensure(nt));
-    CK(s->face_start.ensure(nt + 1));
-    CK(launch_face_counts(s->geom.p, nt, s->b4_index.p, s->b4_start.p, s->face_cnt.p, st));
-    if ((rc = scan_total(s, s->face_cnt.p, s->face_start.p, nt, &s->n_p4)))
-        return rc;
-    CK(s->face_nbr.ensure(s->n_p4));
-    CK(s->face_slot.ensure(s->n_p4));
-    CK(s->face_owner.ensure(s->n_p4));
-    CK(s->face_dirty.ensure(s->n_p4));
-    CK(launch_face_fill(s->geom.p, nt, s->b4_index.p, s->b4_start.p, s->b4_tris.p, s->face_start.p, s->face_nbr.p,
-                        st));
-    CK(launch_face_owner(s->face_start.p, nt, s->face_owner.p, st));
-    CK(s->corner.ensure((size_t)nt * 8));
-    CK(launch_corner_map(s->geom.p, nt, N, s->c3_index.p, s->corner.p, st));
-
-    // ---- grid cache (dirty triangles only)
-    const int scratch_n = std::max(NN, std::max(nt, s->n_p4));
-    CK(s->flag.ensure(scratch_n));
-    CK(s->pos.ensure(scratch_n + 1));
-    const WeightArgs wa = weight_args(s);
-    CK(s->gflag.ensure(nt));
-    CK(s->glist.ensure(nt));
-    CK(launch_grid_dirty(s->geom.p, s->gmeta.p, nt, s->h, first ? 1 : 0, s->gflag.p, st));
-    int n_dirty = 0;
-    if ((rc = scan_total(s, s->gflag.p, s->pos.p, nt, &n_dirty)))
-        return rc;
-    CK(launch_scatter_selected(s->gflag.p, s->pos.p, nt, s->glist.p, nullptr, st));
-    CK(launch_grid_weights(s->verts.p, s->tris.p, s->geom.p, s->glist.p, n_dirty, wa, s->bem, s->wf.p, s->sf.p,
-                           s->wb.p, s->sb.p, s->gmeta.p, st));
-    s->last_grid_recomputes = n_dirty;
-
-    // ---- interpolation weights (every update)
-    CK(launch_interp_weights(s->verts.p, s->tris.p, s->geom.p, wa, s->interp.p, st));
-
-    // ---- face cache: persistent (c, j) slots, dirty pairs recomputed
-    if ((rc = ensure_pool(s, s->pool_count + s->n_p4)))
-        return rc;
-    CK(s->pool_counter.ensure(1));
-    CK(hipMemcpyAsync(s->pool_counter.p, &s->pool_count, sizeof(int), hipMemcpyHostToDevice, st));
-    CK(launch_face_lookup(s->geom.p, s->tris.p, s->face_owner.p, s->face_nbr.p, s->n_p4, s->hkeys.p, s->hvals.p,
-                          s->hcap, s->pool_counter.p, s->fmeta.p, s->face_slot.p, s->face_dirty.p, st));
-    if ((rc = read_int(s, s->pool_counter.p, &s->pool_count)))
-        return rc;
-    if (s->pool_count > s->pool_cap)
-        return fail(s, PPPM_ERR_CAPACITY, "face-cache pool overflow");
-    CK(launch_flag_nonzero(s->face_dirty.p, s->n_p4, s->flag.p, st));
-    int n_fd = 0;
-    if ((rc = scan_total(s, s->flag.p, s->pos.p, s->n_p4, &n_fd)))
-        return rc;
-    CK(s->dirty_list.ensure(n_fd));
-    CK(launch_scatter_selected(s->flag.p, s->pos.p, s->n_p4, s->dirty_list.p, nullptr, st));
-    CK(s->ftmp.ensure((size_t)n_fd * 36));
-    CK(launch_face_weights(s->verts.p, s->tris.p, s->geom.p, s->dirty_list.p, n_fd, s->face_owner.p, s->face_nbr.p,
-                           s->face_slot.p, s->interp.p, wa, s->bem, s->ftmp.p, s->fw.p, s->fsc.p, st));
-    s->last_face_recomputes = n_fd;
-    CK(launch_face_denom(s->geom.p, nt, s->face_start.p, s->face_slot.p, s->fw.p, s->fsc.p, s->denom.p, st));
-    CK(hipStreamSynchronize(st));
-    return PPPM_OK;
-}
-
-int upload(pppm_solver *s, void *dst, const void *src, size_t bytes, int on_device)
-{
-    CK(hipMemcpyAsync(dst, src, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s->stream));
-    return PPPM_OK;
-}
-
-// one update_grid_and_face on the handle's stream; neumann is device memory (row t - t0)
+// ---------------------------------------------------------------- timing spans
 hipEvent_t take_event(pppm_solver *s)
 {
     if (!s->pool.empty())
@@ -414,7 +243,6 @@ hipEvent_t take_event(pppm_solver *s)
     return e;
 }
 
-// marks the start of a timed phase; returns the span index or -1
 int span_begin(pppm_solver *s, int phase)
 {
     if (!s->timing)
@@ -452,34 +280,312 @@ int collect_spans(pppm_solver *s)
     return PPPM_OK;
 }
 
+// ---------------------------------------------------------------- capacities
+int scan(pppm_solver *s, const int *in, int *out, int n)
+{
+    CK(scan_exclusive(in, out, n, s->scan_tmp.p, s->stream));
+    return PPPM_OK;
+}
+
+// (re)allocates every capacity-sized array for the current caps
+int allocate(pppm_solver *s)
+{
+    const int N = s->N, nt = s->nt;
+    const int NN = N * N * N;
+    s->cap_c3 = std::min(27 * nt, NN);
+    const int p3 = 27 * nt;
+    s->c3_waves = p3 / WAVE + 1;
+    s->face_waves = s->cap_p4 / WAVE + 1;
+    const int scan_n = std::max({NN, s->cap_c3, s->cap_p4, nt});
+    CK(s->counters.ensure((size_t)4 * NN));
+    CK(s->cell_start.ensure(NN + 1));
+    CK(s->base_start.ensure(NN + 1));
+    CK(s->cell_items.ensure(nt));
+    CK(s->base_items.ensure(nt));
+    CK(s->cell_rank.ensure(nt));
+    CK(s->base_rank.ensure(nt));
+    CK(s->flag.ensure(NN));
+    CK(s->c3_pos.ensure(NN + 1));
+    CK(s->b4_pos.ensure(NN + 1));
+    CK(s->scan_tmp.ensure(scan_tmp_ints(scan_n)));
+    CK(s->c3_cells.ensure(s->cap_c3));
+    CK(s->c3_index.ensure(NN));
+    CK(s->c3_cnt.ensure(s->cap_c3));
+    CK(s->c3_start.ensure(s->cap_c3 + 1));
+    CK(s->c3_pairs.ensure(p3));
+    CK(s->c3_wave.ensure(s->c3_waves + 1));
+    CK(s->far.ensure(s->cap_c3));
+    CK(s->b4_cells.ensure(nt));
+    CK(s->b4_index.ensure(NN));
+    CK(s->b4_cnt.ensure(nt));
+    CK(s->b4_start.ensure(nt + 1));
+    CK(s->b4_tris.ensure(s->cap_b4items));
+    CK(s->face_cnt.ensure(nt));
+    CK(s->face_start.ensure(nt + 1));
+    CK(s->face_nbr.ensure(s->cap_p4));
+    CK(s->face_slot.ensure(s->cap_p4));
+    CK(s->face_owner.ensure(s->cap_p4));
+    CK(s->face_dirty.ensure(s->cap_p4));
+    CK(s->face_flag.ensure(s->cap_p4));
+    CK(s->face_pos.ensure(s->cap_p4 + 1));
+    CK(s->dirty_list.ensure(s->cap_p4));
+    CK(s->face_wave.ensure(s->face_waves + 1));
+    CK(s->corner.ensure((size_t)nt * 8));
+    CK(s->gflag.ensure(nt));
+    CK(s->gpos.ensure(nt + 1));
+    CK(s->glist.ensure(nt));
+    CK(s->gtmp.ensure(grid_tmp_bytes(nt)));
+    CK(s->ftmp.ensure(face_tmp_bytes(s->cap_p4)));
+    CK(s->errflag.ensure(1));
+    CK(s->status.ensure(16));
+    return PPPM_OK;
+}
+
+// grows the face-cache pool (keeping its contents) and the hash table (rehashing)
+int ensure_pool(pppm_solver *s, int need)
+{
+    if (need > s->pool_cap)
+    {
+        const int ncap = std::max(need, s->pool_cap + s->pool_cap / 2);
+        DBuf<uint32_t> nfw;
+        DBuf<float2> nfsc, nfmeta;
+        CK(nfw.ensure((size_t)ncap * STEPS));
+        CK(nfsc.ensure(ncap));
+        CK(nfmeta.ensure(ncap));
+        if (s->pool_count > 0)
+        {
+            CK(hipMemcpyAsync(nfw.p, s->fw.p, (size_t)s->pool_count * STEPS * sizeof(uint32_t),
+                              hipMemcpyDeviceToDevice, s->stream));
+            CK(hipMemcpyAsync(nfsc.p, s->fsc.p, (size_t)s->pool_count * sizeof(float2), hipMemcpyDeviceToDevice,
+                              s->stream));
+            CK(hipMemcpyAsync(nfmeta.p, s->fmeta.p, (size_t)s->pool_count * sizeof(float2),
+                              hipMemcpyDeviceToDevice, s->stream));
+        }
+        CK(hipStreamSynchronize(s->stream));
+        s->fw.take(nfw);
+        s->fsc.take(nfsc);
+        s->fmeta.take(nfmeta);
+        s->pool_cap = ncap;
+    }
+    int want = 1024;
+    while (want < 2 * s->pool_cap)
+        want <<= 1;
+    if (want > s->hcap)
+    {
+        DBuf<unsigned long long> nk;
+        DBuf<int> nvv;
+        CK(nk.ensure(want));
+        CK(nvv.ensure(want));
+        CK(launch_fill_u64(nk.p, want, ~0ull, s->stream));
+        if (s->hcap > 0)
+            CK(launch_rehash(s->hkeys.p, s->hvals.p, s->hcap, nk.p, nvv.p, want, s->stream));
+        CK(hipStreamSynchronize(s->stream));
+        s->hkeys.take(nk);
+        s->hvals.take(nvv);
+        s->hcap = want;
+    }
+    return PPPM_OK;
+}
+
+// ---------------------------------------------------------------- the update sequence
+// Geometry, binning, lists and weight caches for the vertex positions in s->verts (or, with
+// a series, row t+1-t0 of it). Asynchronous: only kernels are enqueued.
+int enqueue_rebuild(pppm_solver *s, bool first, const float *verts_series, long long vstride, int t0)
+{
+    const int N = s->N, nt = s->nt;
+    const int NN = N * N * N;
+    hipStream_t st = s->stream;
+    int *err = s->errflag.p;
+
+    if (verts_series)
+        CK(launch_copy_verts(verts_series, vstride, t0, s->state.p, s->verts.p, s->nv * 3, st));
+    // triangle geometry (update_triangles_kernel, particle_grid.cu:40-56)
+    const BuildArgs ba{N, nt, s->nv, s->h, s->minp[0], s->minp[1], s->minp[2]};
+    CK(launch_tri_geometry(s->verts.p, s->tris.p, s->geom.p, ba, err, st));
+
+    // cell / base-cell CSR (grid_face_list / base_coord_face_list)
+    int *cell_cnt = s->counters.p, *base_cnt = cell_cnt + NN, *cell_fill = base_cnt + NN, *base_fill = cell_fill + NN;
+    CK(launch_fill_int(s->counters.p, 4LL * NN, 0, st));
+    CK(launch_count_cells(s->geom.p, nt, cell_cnt, base_cnt, st));
+    RC(scan(s, cell_cnt, s->cell_start.p, NN));
+    RC(scan(s, base_cnt, s->base_start.p, NN));
+    CK(launch_fill_cells(s->geom.p, nt, s->cell_start.p, s->base_start.p, cell_fill, base_fill, s->cell_items.p,
+                         s->base_items.p, s->cell_rank.p, s->base_rank.p, st));
+    CK(launch_sort_cells(s->geom.p, nt, s->cell_start.p, s->cell_items.p, s->cell_rank.p, 0, st));
+    CK(launch_sort_cells(s->geom.p, nt, s->base_start.p, s->base_items.p, s->base_rank.p, 1, st));
+
+    // C3 cells: within +-1 of a triangle cell, linear order, with their 3x3x3 pair lists
+    CK(launch_fill_int(s->flag.p, NN, 0, st));
+    CK(launch_mark27(s->geom.p, nt, N, s->flag.p, st));
+    RC(scan(s, s->flag.p, s->c3_pos.p, NN));
+    CK(launch_scatter_selected(s->flag.p, s->c3_pos.p, NN, s->c3_cells.p, s->c3_index.p, st));
+    CK(launch_nbr_count(s->c3_cells.p, s->c3_pos.p + NN, s->cap_c3, s->cell_start.p, N, 3, s->c3_cnt.p, st));
+    RC(scan(s, s->c3_cnt.p, s->c3_start.p, s->cap_c3));
+    CK(launch_nbr_fill(s->c3_cells.p, s->c3_pos.p + NN, s->cap_c3, s->cell_start.p, s->cell_items.p, N, 3,
+                       s->c3_start.p, 27 * nt, s->c3_pairs.p, err, st));
+    CK(launch_wave_map(s->c3_start.p, s->cap_c3, s->c3_pos.p + NN, s->c3_waves, s->c3_wave.p, st));
+
+    // non-empty base cells and their 4x4x4 lists
+    CK(launch_flag_base(base_cnt, N, s->flag.p, st));
+    RC(scan(s, s->flag.p, s->b4_pos.p, NN));
+    CK(launch_scatter_selected(s->flag.p, s->b4_pos.p, NN, s->b4_cells.p, s->b4_index.p, st));
+    CK(launch_nbr_count(s->b4_cells.p, s->b4_pos.p + NN, nt, s->cell_start.p, N, 4, s->b4_cnt.p, st));
+    RC(scan(s, s->b4_cnt.p, s->b4_start.p, nt));
+    CK(launch_nbr_fill(s->b4_cells.p, s->b4_pos.p + NN, nt, s->cell_start.p, s->cell_items.p, N, 4, s->b4_start.p,
+                       s->cap_b4items, s->b4_tris.p, err, st));
+
+    // per-triangle face lists (the 4x4x4 list of its base cell)
+    CK(launch_face_counts(s->geom.p, nt, s->b4_index.p, s->b4_start.p, s->face_cnt.p, st));
+    RC(scan(s, s->face_cnt.p, s->face_start.p, nt));
+    CK(launch_face_fill(s->geom.p, nt, s->b4_index.p, s->b4_start.p, s->b4_tris.p, s->face_start.p, s->cap_p4,
+                        s->face_nbr.p, s->face_owner.p, err, st));
+    CK(launch_wave_map(s->face_start.p, nt, nullptr, s->face_waves, s->face_wave.p, st));
+    CK(launch_corner_map(s->geom.p, nt, N, s->c3_index.p, s->corner.p, st));
+
+    // grid cache: dirty triangles only (GridCache::update_cache, pppm_cache.cu:125-134)
+    const WeightArgs wa = weight_args(s);
+    CK(launch_grid_dirty(s->geom.p, s->gmeta.p, nt, s->h, first ? 1 : 0, s->gflag.p, st));
+    RC(scan(s, s->gflag.p, s->gpos.p, nt));
+    CK(launch_scatter_selected(s->gflag.p, s->gpos.p, nt, s->glist.p, nullptr, st));
+    CK(launch_grid_weights(s->verts.p, s->tris.p, s->geom.p, s->glist.p, s->gpos.p + nt, nt, wa, s->bem, s->gtmp.p,
+                           s->wf.p, s->sf.p, s->wb.p, s->sb.p, s->gmeta.p, err, st));
+
+    // interpolation weights, every update (update_interpolation_weight_kernel)
+    CK(launch_interp_weights(s->verts.p, s->tris.p, s->geom.p, wa, s->interp.p, err, st));
+
+    // face cache: persistent (c, j) slots, dirty pairs recomputed (FaceCache::update_cache)
+    CK(launch_face_lookup(s->geom.p, s->tris.p, s->face_owner.p, s->face_nbr.p, s->face_start.p + nt, s->cap_p4,
+                          s->hkeys.p, s->hvals.p, s->hcap, s->pool_counter.p, s->pool_cap, s->fmeta.p, s->face_slot.p,
+                          s->face_dirty.p, err, nt, st));
+    CK(launch_flag_nonzero(s->face_dirty.p, s->cap_p4, s->face_flag.p, st));
+    RC(scan(s, s->face_flag.p, s->face_pos.p, s->cap_p4));
+    CK(launch_scatter_selected(s->face_flag.p, s->face_pos.p, s->cap_p4, s->dirty_list.p, nullptr, st));
+    CK(launch_face_weights(s->verts.p, s->tris.p, s->geom.p, s->dirty_list.p, s->face_pos.p + s->cap_p4, s->cap_p4,
+                           s->face_owner.p, s->face_nbr.p, s->face_slot.p, s->interp.p, wa, s->bem, s->ftmp.p, s->fw.p,
+                           s->fsc.p, s->fmeta.p, err, st));
+
+    // status block
+    StatusSrc src{};
+    src.n = ST_N;
+    src.p[ST_ERR] = err;
+    src.p[ST_C3] = s->c3_pos.p + NN;
+    src.p[ST_B4] = s->b4_pos.p + NN;
+    src.p[ST_B4ITEMS] = s->b4_start.p + nt;
+    src.p[ST_P4] = s->face_start.p + nt;
+    src.p[ST_GDIRTY] = s->gpos.p + nt;
+    src.p[ST_FDIRTY] = s->face_pos.p + s->cap_p4;
+    src.p[ST_POOL] = s->pool_counter.p;
+    CK(launch_status(src, s->status.p, st));
+    return PPPM_OK;
+}
+
+int read_status(pppm_solver *s)
+{
+    CK(hipMemcpyAsync(s->st_h, s->status.p, sizeof(s->st_h), hipMemcpyDeviceToHost, s->stream));
+    CK(hipStreamSynchronize(s->stream));
+    s->pool_count = s->st_h[ST_POOL];
+    s->last_grid_recomputes = s->st_h[ST_GDIRTY];
+    s->last_face_recomputes = s->st_h[ST_FDIRTY];
+    return PPPM_OK;
+}
+
+// maps device error bits to a status (capacity problems are returned as PPPM_ERR_CAPACITY)
+int status_error(pppm_solver *s)
+{
+    const int e = s->st_h[ST_ERR];
+    if (e & ERR_BAD_INDEX)
+        return fail(s, PPPM_ERR_ARG, "triangle references a vertex index out of range");
+    if (e & ERR_OUT_OF_GRID)
+        return fail(s, PPPM_ERR_OUT_OF_GRID, "a triangle centre lies outside grid cells [2, N-3]");
+    if (e & ERR_CAPACITY)
+        return fail(s, PPPM_ERR_CAPACITY, "an internal list or the face-cache pool ran out of capacity");
+    return PPPM_OK;
+}
+
+bool needs_growth(const pppm_solver *s)
+{
+    return s->st_h[ST_B4ITEMS] > s->cap_b4items || s->st_h[ST_P4] > s->cap_p4 ||
+           s->pool_count + s->cap_p4 > s->pool_cap;
+}
+
+// grows capacities to 1.25x the last observed needs (and the pool to hold one more update)
+int grow(pppm_solver *s)
+{
+    bool realloc = false;
+    if (s->st_h[ST_B4ITEMS] > s->cap_b4items || s->cap_b4items == 0)
+    {
+        s->cap_b4items = std::max(s->st_h[ST_B4ITEMS], 1) * 5 / 4 + 1024;
+        realloc = true;
+    }
+    if (s->st_h[ST_P4] > s->cap_p4 || s->cap_p4 == 0)
+    {
+        s->cap_p4 = std::max(s->st_h[ST_P4], 1) * 5 / 4 + 1024;
+        realloc = true;
+    }
+    if (realloc)
+        RC(allocate(s));
+    return ensure_pool(s, s->pool_count + 2 * s->cap_p4);
+}
+
+// synchronous update used by set_mesh / update_mesh: runs the sequence, retries once with
+// grown capacities if a list or the pool overflowed
+int rebuild_sync(pppm_solver *s, bool first)
+{
+    for (int attempt = 0; attempt < 3; attempt++)
+    {
+        CK(launch_fill_int(s->errflag.p, 1, 0, s->stream));
+        if (first)
+        {
+            // fresh caches (GridCache::init / FaceCache::init, pppm_cache.h:29-33, 115-124)
+            s->pool_count = 0;
+            CK(launch_fill_u64(s->hkeys.p, s->hcap, ~0ull, s->stream));
+            CK(hipMemcpyAsync(s->pool_counter.p, &s->pool_count, sizeof(int), hipMemcpyHostToDevice, s->stream));
+        }
+        RC(enqueue_rebuild(s, first, nullptr, 0, 0));
+        RC(read_status(s));
+        const int e = s->st_h[ST_ERR];
+        if ((e & ~ERR_CAPACITY) != 0)
+            return status_error(s);
+        if (!(e & ERR_CAPACITY) && !needs_growth(s))
+            return PPPM_OK;
+        RC(grow(s));
+    }
+    return status_error(s);
+}
+
+int upload(pppm_solver *s, void *dst, const void *src, size_t bytes, int on_device)
+{
+    CK(hipMemcpyAsync(dst, src, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s->stream));
+    return PPPM_OK;
+}
+
+// one update_grid_and_face on the handle's stream; neumann is device memory (row t - t0)
 int enqueue_step(pppm_solver *s, const float *neumann, int t0, long long stride, int n_listen, float *out)
 {
     const FdtdParams fp = fdtd_params(s);
     int sp = span_begin(s, 0);
     CK(launch_fdtd(s->grid.p, s->pmlbuf.p, fp, s->state.p, s->stream));
     span_end(s, sp);
+    CK(launch_advance(s->state.p, s->stream));
+    const StepArgs a = step_args(s, t0, stride, n_listen);
     if (!s->mesh_set)
     {
         if (out)
-        {
-            StepArgs a = step_args(s, t0, stride, n_listen);
-            a.n_c3 = 0;
             CK(launch_listen_direct(a, s->state.p, s->grid.p, s->lcell.p, out, s->stream));
-        }
         return PPPM_OK;
     }
-    const StepArgs a = step_args(s, t0, stride, n_listen);
     sp = span_begin(s, 1);
-    CK(launch_far(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_pairs.p, s->wf.p, s->sf.p, s->hn.p,
-                  s->hd.p, s->far.p, neumann, s->n_p3, s->stream));
+    CK(launch_far(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_wave.p, s->c3_waves, s->c3_pairs.p,
+                  s->wf.p, s->sf.p, s->hn.p, s->hd.p, s->far.p, neumann, s->stream));
     span_end(s, sp);
     sp = span_begin(s, 2);
-    CK(launch_dirichlet(a, s->state.p, s->face_start.p, s->face_nbr.p, s->face_slot.p, s->fw.p, s->fsc.p, s->hn.p,
-                        s->hd.p, s->interp.p, s->corner.p, s->far.p, s->denom.p, s->n_p4, s->stream));
+    CK(launch_dirichlet(a, s->state.p, s->face_start.p, s->face_wave.p, s->face_waves, s->face_nbr.p, s->face_slot.p,
+                        s->fw.p, s->fsc.p, s->hn.p, s->hd.p, s->interp.p, s->corner.p, s->far.p, s->geom.p,
+                        s->errflag.p, s->stream));
     span_end(s, sp);
     sp = span_begin(s, 3);
-    CK(launch_near(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_pairs.p, s->wb.p, s->sb.p, s->hn.p,
-                   s->hd.p, s->far.p, s->lcell.p, s->lc3.p, out, s->n_p3, s->stream));
+    CK(launch_near(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_wave.p, s->c3_waves, s->c3_pairs.p,
+                   s->wb.p, s->sb.p, s->hn.p, s->hd.p, s->far.p, s->lcell.p, s->lc3.p, out, s->stream));
     span_end(s, sp);
     if (s->spans.size() > 4096)
         return collect_spans(s);
@@ -487,6 +593,163 @@ int enqueue_step(pppm_solver *s, const float *neumann, int t0, long long stride,
 }
 
 int check_handle(pppm_solver *s) { return s ? PPPM_OK : PPPM_ERR_ARG; }
+
+int set_listeners(pppm_solver *s, const int *listen_xyz, int n_listen)
+{
+    const int N = s->N;
+    std::vector<int> cells(n_listen), c3(n_listen, -1);
+    for (int l = 0; l < n_listen; l++)
+    {
+        const int x = listen_xyz[3 * l], y = listen_xyz[3 * l + 1], z = listen_xyz[3 * l + 2];
+        if (x < 0 || y < 0 || z < 0 || x >= N || y >= N || z >= N)
+            return fail(s, PPPM_ERR_ARG, "listener outside the grid");
+        cells[l] = lin3(x, y, z, N);
+    }
+    CK(s->lcell.ensure(std::max(n_listen, 1)));
+    CK(s->lc3.ensure(std::max(n_listen, 1)));
+    if (n_listen > 0)
+    {
+        if (s->mesh_set)
+        {
+            CK(hipStreamSynchronize(s->stream));
+            for (int l = 0; l < n_listen; l++)
+                CK(hipMemcpy(&c3[l], s->c3_index.p + cells[l], sizeof(int), hipMemcpyDeviceToHost));
+        }
+        CK(hipMemcpyAsync(s->lcell.p, cells.data(), n_listen * sizeof(int), hipMemcpyHostToDevice, s->stream));
+        CK(hipMemcpyAsync(s->lc3.p, c3.data(), n_listen * sizeof(int), hipMemcpyHostToDevice, s->stream));
+        CK(hipStreamSynchronize(s->stream));
+    }
+    return PPPM_OK;
+}
+
+// steps of (optional mesh update +) update_grid_and_face; graph-captured in chunks
+int run_impl(pppm_solver *s, const float *verts_series, long long vstride, const float *neumann_series,
+             long long stride, int steps, const int *listen_xyz, int n_listen, float *out_dev, int use_graph)
+{
+    if (steps < 0 || n_listen < 0 || n_listen > 64 || (n_listen > 0 && (!listen_xyz || !out_dev)))
+        return fail(s, PPPM_ERR_ARG, "run: bad arguments (at most 64 listeners)");
+    if (s->mesh_set && !neumann_series)
+        return fail(s, PPPM_ERR_ARG, "run: neumann_series required");
+    if (verts_series && !s->mesh_set)
+        return fail(s, PPPM_ERR_STATE, "run: moving mesh before set_mesh");
+    if (steps == 0)
+        return PPPM_OK;
+    // a listener's C3 index can change with the lists: moving runs read the grid directly
+    RC(set_listeners(s, listen_xyz, n_listen));
+    if (verts_series && n_listen > 0)
+    {
+        std::vector<int> none(n_listen, -1);
+        CK(hipMemcpy(s->lc3.p, none.data(), n_listen * sizeof(int), hipMemcpyHostToDevice));
+    }
+    float *out = n_listen > 0 ? out_dev : nullptr;
+    const int t0 = s->t_host + 1;
+    // Moving meshes take the synchronous update of pppm_update_mesh before each step (row i
+    // of the series, status read back, capacities grown when needed). The fully
+    // asynchronous update (row picked on the device, capturable in a graph) stays off until
+    // its graph replay has been verified on the device.
+    constexpr bool kAsyncMovingStep = false;
+    if (verts_series && !kAsyncMovingStep)
+        use_graph = 0;
+    auto one_step = [&](int i) -> int {
+        if (verts_series)
+        {
+            const int sp = span_begin(s, 4);
+            if (kAsyncMovingStep)
+                RC(enqueue_rebuild(s, false, verts_series, vstride, t0));
+            else
+            {
+                RC(upload(s, s->verts.p, verts_series + (size_t)i * vstride, (size_t)s->nv * 3 * sizeof(float), 1));
+                RC(rebuild_sync(s, false));
+            }
+            span_end(s, sp);
+        }
+        if (!verts_series)
+            return enqueue_step(s, neumann_series, t0, stride, n_listen, out);
+        // with moving lists the listener is read by its own launch after the near field
+        RC(enqueue_step(s, neumann_series, t0, stride, n_listen, nullptr));
+        if (out)
+        {
+            const StepArgs a = step_args(s, t0, stride, n_listen);
+            CK(launch_listen_direct(a, s->state.p, s->grid.p, s->lcell.p, out, s->stream));
+        }
+        return PPPM_OK;
+    };
+    if (verts_series)
+    {
+        CK(launch_fill_int(s->errflag.p, 1, 0, s->stream));
+        RC(ensure_pool(s, s->pool_count + 2 * s->cap_p4));
+    }
+    int done = 0;
+    if (!use_graph)
+    {
+        for (; done < steps; done++)
+        {
+            RC(one_step(done));
+            if (verts_series && (done % 64 == 63 || done == steps - 1))
+            {
+                RC(read_status(s));
+                RC(status_error(s));
+                if (needs_growth(s))
+                    RC(grow(s));
+            }
+        }
+    }
+    else
+    {
+        // one captured graph holds `chunk` steps; kernels read t (and the input rows) from
+        // device memory. Event timing is not captured.
+        const bool timing = s->timing;
+        s->timing = false;
+        const int chunk = std::min(steps, 32);
+        while (done < steps)
+        {
+            hipGraph_t g = nullptr;
+            hipGraphExec_t ge = nullptr;
+            CK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+            int rc = PPPM_OK;
+            for (int i = 0; i < chunk && rc == PPPM_OK; i++)
+                rc = one_step(done + i);
+            hipError_t ee = hipStreamEndCapture(s->stream, &g);
+            if (rc || ee != hipSuccess)
+            {
+                if (g)
+                    (void)hipGraphDestroy(g);
+                s->timing = timing;
+                return rc ? rc : fail(s, PPPM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ee));
+            }
+            CK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+            bool regrow = false;
+            while (done + chunk <= steps && !regrow)
+            {
+                CK(hipGraphLaunch(ge, s->stream));
+                done += chunk;
+                if (verts_series)
+                {
+                    RC(read_status(s));
+                    RC(status_error(s));
+                    regrow = needs_growth(s);
+                }
+            }
+            CK(hipStreamSynchronize(s->stream));
+            (void)hipGraphExecDestroy(ge);
+            (void)hipGraphDestroy(g);
+            if (regrow)
+                RC(grow(s));  // buffers moved: capture again
+            else
+                break;
+        }
+        for (; done < steps; done++)
+            RC(one_step(done));
+        s->timing = timing;
+        if (verts_series)
+        {
+            RC(read_status(s));
+            RC(status_error(s));
+        }
+    }
+    s->t_host += steps;
+    return PPPM_OK;
+}
 
 }  // namespace
 
@@ -542,30 +805,6 @@ int pppm_destroy(pppm_solver *s)
         return PPPM_ERR_ARG;
     if (s->stream)
         (void)hipStreamSynchronize(s->stream);
-    DBuf<float> *fb[] = {&s->grid, &s->pmlbuf, &s->verts, &s->gmeta, &s->interp, &s->hn, &s->hd, &s->far, &s->nstage};
-    for (auto b : fb)
-        b->release();
-    DBuf<int> *ib[] = {&s->tris,       &s->err_flag,  &s->cell_cnt,   &s->base_cnt,   &s->cell_start, &s->base_start,
-                       &s->cell_fill,  &s->base_fill, &s->cell_items, &s->base_items, &s->flag,       &s->pos,
-                       &s->scan_tmp,   &s->c3_cells,  &s->c3_index,   &s->c3_cnt,     &s->c3_start,   &s->c3_pairs,
-                       &s->b4_cells,   &s->b4_index,  &s->b4_cnt,     &s->b4_start,   &s->b4_tris,    &s->face_cnt,
-                       &s->face_start, &s->face_nbr,  &s->face_slot,  &s->face_owner, &s->face_dirty, &s->dirty_list,
-                       &s->corner,     &s->gflag,     &s->glist,      &s->hvals,      &s->pool_counter, &s->lcell,
-                       &s->lc3};
-    for (auto b : ib)
-        b->release();
-    s->state.release();
-    s->geom.release();
-    s->wf.release();
-    s->wb.release();
-    s->sf.release();
-    s->sb.release();
-    s->denom.release();
-    s->ftmp.release();
-    s->fw.release();
-    s->fsc.release();
-    s->fmeta.release();
-    s->hkeys.release();
     for (auto &sp : s->spans)
     {
         (void)hipEventDestroy(sp.a);
@@ -575,7 +814,7 @@ int pppm_destroy(pppm_solver *s)
         (void)hipEventDestroy(e);
     if (s->own_stream && s->stream)
         (void)hipStreamDestroy(s->stream);
-    delete s;
+    delete s;  // device buffers are released by their owners
     return PPPM_OK;
 }
 
@@ -605,24 +844,22 @@ int pppm_set_mesh(pppm_solver *s, const float *verts, int nv, const int *tris, i
     CK(s->verts.ensure((size_t)nv * 3));
     CK(s->tris.ensure((size_t)nt * 3));
     CK(s->geom.ensure(nt));
-    int rc;
-    if ((rc = upload(s, s->verts.p, verts, (size_t)nv * 3 * sizeof(float), on_device)))
-        return rc;
-    if ((rc = upload(s, s->tris.p, tris, (size_t)nt * 3 * sizeof(int), on_device)))
-        return rc;
-    // fresh caches (GridCache::init / FaceCache::init, pppm_cache.h:29-33, 115-124)
+    RC(upload(s, s->verts.p, verts, (size_t)nv * 3 * sizeof(float), on_device));
+    RC(upload(s, s->tris.p, tris, (size_t)nt * 3 * sizeof(int), on_device));
     CK(s->wf.ensure((size_t)nt * GC * STEPS));
     CK(s->wb.ensure((size_t)nt * GC * STEPS));
     CK(s->sf.ensure((size_t)nt * GC));
     CK(s->sb.ensure((size_t)nt * GC));
     CK(s->gmeta.ensure((size_t)nt * 7));
     CK(s->interp.ensure((size_t)nt * 8));
-    CK(s->denom.ensure(nt));
+    CK(s->pool_counter.ensure(1));
+    // first guess for the 4x4x4 list capacities; rebuild_sync grows them if needed
+    s->cap_b4items = std::max(s->cap_b4items, 16 * nt + 1024);
+    s->cap_p4 = std::max(s->cap_p4, 16 * nt + 1024);
+    RC(allocate(s));
     s->pool_count = 0;
-    if (s->hcap > 0)
-        CK(hipMemsetAsync(s->hkeys.p, 0xff, (size_t)s->hcap * sizeof(unsigned long long), s->stream));
-    if ((rc = rebuild(s, true)))
-        return rc;
+    RC(ensure_pool(s, 2 * s->cap_p4));
+    RC(rebuild_sync(s, true));
     // dirichlet / neumann histories reset (pppm.h:88-91)
     CK(s->hn.ensure((size_t)nt * RING));
     CK(s->hd.ensure((size_t)nt * RING));
@@ -640,11 +877,9 @@ int pppm_update_mesh(pppm_solver *s, const float *verts, int on_device)
         return PPPM_ERR_ARG;
     if (!s->mesh_set)
         return fail(s, PPPM_ERR_STATE, "update_mesh before set_mesh");
-    int rc;
-    if ((rc = upload(s, s->verts.p, verts, (size_t)s->nv * 3 * sizeof(float), on_device)))
-        return rc;
+    RC(upload(s, s->verts.p, verts, (size_t)s->nv * 3 * sizeof(float), on_device));
     const int sp = span_begin(s, 4);
-    rc = rebuild(s, false);
+    const int rc = rebuild_sync(s, false);
     span_end(s, sp);
     return rc;
 }
@@ -653,7 +888,7 @@ int pppm_set_timing(pppm_solver *s, int enable)
 {
     if (check_handle(s))
         return PPPM_ERR_ARG;
-    int rc = collect_spans(s);
+    const int rc = collect_spans(s);
     s->timing = enable != 0;
     return rc;
 }
@@ -662,9 +897,7 @@ int pppm_get_timing(pppm_solver *s, double *ms8, long long *count8)
 {
     if (check_handle(s))
         return PPPM_ERR_ARG;
-    int rc = collect_spans(s);
-    if (rc)
-        return rc;
+    RC(collect_spans(s));
     for (int i = 0; i < 8; i++)
     {
         if (ms8)
@@ -684,16 +917,13 @@ int pppm_step(pppm_solver *s, const float *neumann, int on_device)
     if (!s->mesh_set)
         return fail(s, PPPM_ERR_STATE, "step before set_mesh");
     const float *nd = neumann;
-    int rc;
     if (neumann && !on_device)
     {
-        if ((rc = upload(s, s->nstage.p, neumann, (size_t)s->nt * sizeof(float), 0)))
-            return rc;
+        RC(upload(s, s->nstage.p, neumann, (size_t)s->nt * sizeof(float), 0));
         nd = s->nstage.p;
     }
     const int t_new = s->t_host + 1;
-    if ((rc = enqueue_step(s, nd, t_new, 0, 0, nullptr)))
-        return rc;
+    RC(enqueue_step(s, nd, t_new, 0, 0, nullptr));
     s->t_host = t_new;
     return PPPM_OK;
 }
@@ -703,90 +933,18 @@ int pppm_run(pppm_solver *s, const float *neumann_series, long long stride, int 
 {
     if (check_handle(s))
         return PPPM_ERR_ARG;
-    if (steps < 0 || n_listen < 0 || n_listen > 64 || (n_listen > 0 && (!listen_xyz || !out_dev)))
-        return fail(s, PPPM_ERR_ARG, "run: bad arguments (at most 64 listeners)");
-    if (s->mesh_set && !neumann_series)
-        return fail(s, PPPM_ERR_ARG, "run: neumann_series required");
-    if (steps == 0)
-        return PPPM_OK;
-    const int N = s->N;
-    std::vector<int> cells(n_listen), c3(n_listen, -1);
-    for (int l = 0; l < n_listen; l++)
-    {
-        const int x = listen_xyz[3 * l], y = listen_xyz[3 * l + 1], z = listen_xyz[3 * l + 2];
-        if (x < 0 || y < 0 || z < 0 || x >= N || y >= N || z >= N)
-            return fail(s, PPPM_ERR_ARG, "run: listener outside the grid");
-        cells[l] = lin3(x, y, z, N);
-        if (s->mesh_set)
-        {
-            int v = -1;
-            int rc = read_int(s, s->c3_index.p + cells[l], &v);
-            if (rc)
-                return rc;
-            c3[l] = v;
-        }
-    }
-    CK(s->lcell.ensure(std::max(n_listen, 1)));
-    CK(s->lc3.ensure(std::max(n_listen, 1)));
-    if (n_listen > 0)
-    {
-        CK(hipMemcpyAsync(s->lcell.p, cells.data(), n_listen * sizeof(int), hipMemcpyHostToDevice, s->stream));
-        CK(hipMemcpyAsync(s->lc3.p, c3.data(), n_listen * sizeof(int), hipMemcpyHostToDevice, s->stream));
-    }
-    const int t0 = s->t_host + 1;
-    float *out = n_listen > 0 ? out_dev : nullptr;
-    int rc;
-    if (!use_graph)
-    {
-        for (int i = 0; i < steps; i++)
-            if ((rc = enqueue_step(s, neumann_series, t0, stride, n_listen, out)))
-                return rc;
-    }
-    else
-    {
-        // one captured graph holds `chunk` steps; kernels read t from device memory.
-        // Event timing is not captured: spans are only recorded by eager launches.
-        const bool timing = s->timing;
-        s->timing = false;
-        struct Restore
-        {
-            pppm_solver *s;
-            bool v;
-            ~Restore() { s->timing = v; }
-        } restore{s, timing};
-        const int chunk = std::min(steps, 32);
-        hipGraph_t g = nullptr;
-        hipGraphExec_t ge = nullptr;
-        CK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < chunk; i++)
-        {
-            rc = enqueue_step(s, neumann_series, t0, stride, n_listen, out);
-            if (rc)
-            {
-                hipGraph_t tmp;
-                (void)hipStreamEndCapture(s->stream, &tmp);
-                if (tmp)
-                    (void)hipGraphDestroy(tmp);
-                return rc;
-            }
-        }
-        CK(hipStreamEndCapture(s->stream, &g));
-        CK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-        int done = 0;
-        while (done + chunk <= steps)
-        {
-            CK(hipGraphLaunch(ge, s->stream));
-            done += chunk;
-        }
-        for (; done < steps; done++)
-            if ((rc = enqueue_step(s, neumann_series, t0, stride, n_listen, out)))
-                return rc;
-        CK(hipStreamSynchronize(s->stream));
-        (void)hipGraphExecDestroy(ge);
-        (void)hipGraphDestroy(g);
-    }
-    s->t_host += steps;
-    return PPPM_OK;
+    return run_impl(s, nullptr, 0, neumann_series, stride, steps, listen_xyz, n_listen, out_dev, use_graph);
+}
+
+int pppm_run_moving(pppm_solver *s, const float *verts_series, long long vstride, const float *neumann_series,
+                    long long stride, int steps, const int *listen_xyz, int n_listen, float *out_dev, int use_graph)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    if (!verts_series || vstride < (long long)s->nv * 3)
+        return fail(s, PPPM_ERR_ARG, "run_moving: vertex series required (stride >= 3 nv)");
+    return run_impl(s, verts_series, vstride, neumann_series, stride, steps, listen_xyz, n_listen, out_dev,
+                    use_graph);
 }
 
 int pppm_fdtd_step(pppm_solver *s)
@@ -795,6 +953,7 @@ int pppm_fdtd_step(pppm_solver *s)
         return PPPM_ERR_ARG;
     const FdtdParams fp = fdtd_params(s);
     CK(launch_fdtd(s->grid.p, s->pmlbuf.p, fp, s->state.p, s->stream));
+    CK(launch_advance(s->state.p, s->stream));
     s->t_host++;
     return PPPM_OK;
 }
@@ -806,8 +965,8 @@ int pppm_solve_far(pppm_solver *s)
     if (!s->mesh_set)
         return fail(s, PPPM_ERR_STATE, "solve_far before set_mesh");
     const StepArgs a = step_args(s, s->t_host, 0, 0);
-    CK(launch_far(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_pairs.p, s->wf.p, s->sf.p, s->hn.p,
-                  s->hd.p, s->far.p, nullptr, s->n_p3, s->stream));
+    CK(launch_far(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_wave.p, s->c3_waves, s->c3_pairs.p,
+                  s->wf.p, s->sf.p, s->hn.p, s->hd.p, s->far.p, nullptr, s->stream));
     return PPPM_OK;
 }
 
@@ -818,11 +977,9 @@ int pppm_set_neumann(pppm_solver *s, const float *neumann, int on_device)
     if (!s->mesh_set || !neumann)
         return fail(s, PPPM_ERR_STATE, "set_neumann before set_mesh");
     const float *nd = neumann;
-    int rc;
     if (!on_device)
     {
-        if ((rc = upload(s, s->nstage.p, neumann, (size_t)s->nt * sizeof(float), 0)))
-            return rc;
+        RC(upload(s, s->nstage.p, neumann, (size_t)s->nt * sizeof(float), 0));
         nd = s->nstage.p;
     }
     const StepArgs a = step_args(s, s->t_host, 0, 0);
@@ -837,8 +994,9 @@ int pppm_update_dirichlet(pppm_solver *s)
     if (!s->mesh_set)
         return fail(s, PPPM_ERR_STATE, "update_dirichlet before set_mesh");
     const StepArgs a = step_args(s, s->t_host, 0, 0);
-    CK(launch_dirichlet(a, s->state.p, s->face_start.p, s->face_nbr.p, s->face_slot.p, s->fw.p, s->fsc.p, s->hn.p,
-                        s->hd.p, s->interp.p, s->corner.p, s->far.p, s->denom.p, s->n_p4, s->stream));
+    CK(launch_dirichlet(a, s->state.p, s->face_start.p, s->face_wave.p, s->face_waves, s->face_nbr.p, s->face_slot.p,
+                        s->fw.p, s->fsc.p, s->hn.p, s->hd.p, s->interp.p, s->corner.p, s->far.p, s->geom.p,
+                        s->errflag.p, s->stream));
     return PPPM_OK;
 }
 
@@ -849,8 +1007,8 @@ int pppm_solve_near(pppm_solver *s)
     if (!s->mesh_set)
         return fail(s, PPPM_ERR_STATE, "solve_near before set_mesh");
     const StepArgs a = step_args(s, s->t_host, 0, 0);
-    CK(launch_near(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_pairs.p, s->wb.p, s->sb.p, s->hn.p,
-                   s->hd.p, s->far.p, nullptr, nullptr, nullptr, s->n_p3, s->stream));
+    CK(launch_near(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_wave.p, s->c3_waves, s->c3_pairs.p,
+                   s->wb.p, s->sb.p, s->hn.p, s->hd.p, s->far.p, nullptr, nullptr, nullptr, s->stream));
     return PPPM_OK;
 }
 
@@ -867,7 +1025,7 @@ int pppm_reset(pppm_solver *s)
         CK(hipMemsetAsync(s->hn.p, 0, (size_t)s->nt * RING * sizeof(float), s->stream));
         CK(hipMemsetAsync(s->hd.p, 0, (size_t)s->nt * RING * sizeof(float), s->stream));
     }
-    StepState init{-1, 0u};
+    StepState init{-1, 0};
     CK(hipMemcpyAsync(s->state.p, &init, sizeof(init), hipMemcpyHostToDevice, s->stream));
     CK(hipStreamSynchronize(s->stream));
     s->t_host = -1;
@@ -947,11 +1105,13 @@ int pppm_stats(pppm_solver *s, long long *o)
 {
     if (check_handle(s) || !o)
         return PPPM_ERR_ARG;
+    if (s->mesh_set)
+        RC(read_status(s));
     o[0] = s->nt;
-    o[1] = s->n_c3;
-    o[2] = s->n_p3;
-    o[3] = s->n_b4;
-    o[4] = s->n_p4;
+    o[1] = s->mesh_set ? s->st_h[ST_C3] : 0;
+    o[2] = s->mesh_set ? 27LL * s->nt : 0;
+    o[3] = s->mesh_set ? s->st_h[ST_B4] : 0;
+    o[4] = s->mesh_set ? s->st_h[ST_P4] : 0;
     o[5] = s->last_grid_recomputes;
     o[6] = s->last_face_recomputes;
     o[7] = s->pool_count;
@@ -1027,7 +1187,8 @@ int pppm_get_face_weights(pppm_solver *s, int *host_start, int *host_nbr, void *
         return PPPM_ERR_ARG;
     if (!s->mesh_set)
         return fail(s, PPPM_ERR_STATE, "no mesh");
-    const int P = s->n_p4;
+    RC(read_status(s));
+    const int P = s->st_h[ST_P4];
     std::vector<int> start(s->nt + 1), nbr(P), slot(P);
     CK(hipMemcpyAsync(start.data(), s->face_start.p, (s->nt + 1) * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     CK(hipMemcpyAsync(nbr.data(), s->face_nbr.p, P * sizeof(int), hipMemcpyDeviceToHost, s->stream));
@@ -1059,7 +1220,8 @@ int pppm_set_face_weights(pppm_solver *s, const void *host_recs72)
         return PPPM_ERR_ARG;
     if (!s->mesh_set)
         return fail(s, PPPM_ERR_STATE, "no mesh");
-    const int P = s->n_p4;
+    RC(read_status(s));
+    const int P = s->st_h[ST_P4];
     std::vector<int> slot(P);
     CK(hipMemcpyAsync(slot.data(), s->face_slot.p, P * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     std::vector<uint32_t> fw((size_t)s->pool_count * STEPS);
@@ -1072,7 +1234,6 @@ int pppm_set_face_weights(pppm_solver *s, const void *host_recs72)
         from72(in[p], &fw[(size_t)slot[p] * STEPS], fsc[slot[p]]);
     CK(hipMemcpyAsync(s->fw.p, fw.data(), fw.size() * 4, hipMemcpyHostToDevice, s->stream));
     CK(hipMemcpyAsync(s->fsc.p, fsc.data(), fsc.size() * sizeof(float2), hipMemcpyHostToDevice, s->stream));
-    CK(launch_face_denom(s->geom.p, s->nt, s->face_start.p, s->face_slot.p, s->fw.p, s->fsc.p, s->denom.p, s->stream));
     CK(hipStreamSynchronize(s->stream));
     return PPPM_OK;
 }
@@ -1094,12 +1255,14 @@ int pppm_get_c3(pppm_solver *s, int *cells, int *start, int *pairs)
         return PPPM_ERR_ARG;
     if (!s->mesh_set)
         return fail(s, PPPM_ERR_STATE, "no mesh");
+    RC(read_status(s));
+    const int n = s->st_h[ST_C3];
     if (cells)
-        CK(hipMemcpyAsync(cells, s->c3_cells.p, s->n_c3 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        CK(hipMemcpyAsync(cells, s->c3_cells.p, n * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     if (start)
-        CK(hipMemcpyAsync(start, s->c3_start.p, (s->n_c3 + 1) * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        CK(hipMemcpyAsync(start, s->c3_start.p, (n + 1) * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     if (pairs)
-        CK(hipMemcpyAsync(pairs, s->c3_pairs.p, (size_t)s->n_p3 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        CK(hipMemcpyAsync(pairs, s->c3_pairs.p, (size_t)27 * s->nt * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     CK(hipStreamSynchronize(s->stream));
     return PPPM_OK;
 }
@@ -1110,14 +1273,12 @@ int pppm_get_b4(pppm_solver *s, int *cells, int *start, int *tris)
         return PPPM_ERR_ARG;
     if (!s->mesh_set)
         return fail(s, PPPM_ERR_STATE, "no mesh");
-    int items = 0;
-    int rc = read_int(s, s->b4_start.p + s->n_b4, &items);
-    if (rc)
-        return rc;
+    RC(read_status(s));
+    const int n = s->st_h[ST_B4], items = s->st_h[ST_B4ITEMS];
     if (cells)
-        CK(hipMemcpyAsync(cells, s->b4_cells.p, s->n_b4 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        CK(hipMemcpyAsync(cells, s->b4_cells.p, n * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     if (start)
-        CK(hipMemcpyAsync(start, s->b4_start.p, (s->n_b4 + 1) * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        CK(hipMemcpyAsync(start, s->b4_start.p, (n + 1) * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     if (tris)
         CK(hipMemcpyAsync(tris, s->b4_tris.p, (size_t)items * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     CK(hipStreamSynchronize(s->stream));

@@ -8,12 +8,13 @@
 //
 // Segmented gather with a fixed summation order. A "segment" is a C3 cell (far / near)
 // or a triangle (Dirichlet); its pairs are contiguous in a CSR list. Wave w owns the
-// segments whose first pair lies in [64w, 64w+64) — at most 64, one leader lane each.
-// The wave walks the owned pair range in rounds of 64: lane i evaluates the 16-tap
-// convolution of pair (round base + i) sequentially (k = 0..15, as the reference's
-// loop), parks the value in LDS, and every leader adds the values of its own pairs in
-// list order. Every sum therefore has the order of a sequential loop, whatever the
-// segment lengths, and lanes stay busy across segment boundaries.
+// segments whose first pair lies in [64w, 64w+64) — at most 64, one leader lane each;
+// the owned range comes from a wave map built with the lists. The wave walks its pair
+// range in rounds of 64: lane i evaluates the 16-tap convolution of pair (round base + i)
+// sequentially (k = 0..15, as the reference's loop), parks the value in LDS, and every
+// leader adds the values of its own pairs in list order. Every sum therefore has the
+// order of a sequential loop, whatever the segment lengths, and lanes stay busy across
+// segment boundaries.
 //
 // Weights: 64 B record per pair (16 x half2 {S_k, D_k}) + float2 scales. History: the
 // reference's 32-slot rings (slot = t mod 32), one 128 B line per triangle and field.
@@ -37,32 +38,19 @@ __device__ inline void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// smallest s in [0, n] with start[s] >= p (start has n+1 non-decreasing entries)
-__device__ inline int lower_seg(const int *__restrict__ start, int n, int p)
-{
-    int lo = 0, hi = n;
-    while (lo < hi)
-    {
-        const int mid = (lo + hi) >> 1;
-        if (start[mid] >= p)
-            hi = mid;
-        else
-            lo = mid + 1;
-    }
-    return lo;
-}
-
-// LayerWeightHalf::convolution (bem.h:203-213) for one pair, taps k0..15 in order.
-// dzero_tap0: the Dirichlet value of step t is read as 0 (pppm.cu:143).
+// LayerWeightHalf::convolution (bem.h:203-213) for one pair, taps K0..15 in order.
+// DZERO0: the Dirichlet value of step t is read as 0 (pppm.cu:143).
 template <int K0, bool DZERO0>
 __device__ inline float conv_pair(const uint32_t *__restrict__ w16, float2 sc, const float *__restrict__ nring,
-                                  const float *__restrict__ dring, int t)
+                                  const float *__restrict__ dring, int t, uint32_t *tap0 = nullptr)
 {
     uint4 q[4];
 #pragma unroll
     for (int i = 0; i < 4; i++)
         q[i] = reinterpret_cast<const uint4 *>(w16)[i];
     const uint32_t *w = reinterpret_cast<const uint32_t *>(q);
+    if (tap0)
+        *tap0 = w[0];
     float r = 0.0f;
 #pragma unroll
     for (int k = K0; k < STEPS; k++)
@@ -84,11 +72,11 @@ __device__ inline float conv_pair(const uint32_t *__restrict__ w16, float2 sc, c
 // of step t is therefore free to be written by the extra waves of this launch.
 __global__ __launch_bounds__(TB) void k_far(StepArgs a, const StepState *__restrict__ st,
                                             const float *__restrict__ grid3, const int *__restrict__ c3_cells,
-                                            const int *__restrict__ c3_start, const int *__restrict__ c3_pairs,
+                                            const int *__restrict__ c3_start, const int *__restrict__ c3_wave,
+                                            int gather_waves, const int *__restrict__ c3_pairs,
                                             const uint32_t *__restrict__ wf, const float2 *__restrict__ sf,
                                             float *__restrict__ hn, const float *__restrict__ hd,
-                                            float *__restrict__ far, const float *__restrict__ neumann,
-                                            int gather_waves)
+                                            float *__restrict__ far, const float *__restrict__ neumann)
 {
     __shared__ float vals[WPB][WAVE];
     const int t = st->t;
@@ -96,8 +84,7 @@ __global__ __launch_bounds__(TB) void k_far(StepArgs a, const StepState *__restr
     const int wave = blockIdx.x * WPB + wid;
     if (wave < gather_waves)
     {
-        const int c0 = lower_seg(c3_start, a.n_c3, wave * WAVE);
-        const int c1 = lower_seg(c3_start, a.n_c3, wave * WAVE + WAVE);
+        const int c0 = c3_wave[wave], c1 = c3_wave[wave + 1];
         if (c0 >= c1)
             return;
         const int q0 = c3_start[c0], q1 = c3_start[c1];
@@ -129,7 +116,8 @@ __global__ __launch_bounds__(TB) void k_far(StepArgs a, const StepState *__restr
         {
             const size_t nn = (size_t)a.N * a.N * a.N;
             const float *g = grid3 + (size_t)(((t % 3) + 3) % 3) * nn;
-            far[cs] = g[c3_cells[cs]] - acc;
+            const unsigned cell = (unsigned)c3_cells[cs];
+            far[cs] = cell < nn ? g[cell] - acc : 0.0f;
         }
     }
     else
@@ -142,46 +130,59 @@ __global__ __launch_bounds__(TB) void k_far(StepArgs a, const StepState *__restr
 }
 
 // Face far field (8-corner interpolation, pppm.cu:116-144), face near field against the
-// 4x4x4 list with D[t] = 0, then D_c[t] = (far*rc + near) / (0.5*area - sum_j D_0(c, j)).
+// 4x4x4 list with D[t] = 0 and its tap-0 factors (pppm.cu:146-170), then
+// D_c[t] = (far*rc + near) / (0.5*area - sum_j D_0(c, j)) (pppm.cu:172-193).
 __global__ __launch_bounds__(TB) void k_dirichlet(StepArgs a, const StepState *__restrict__ st,
                                                   const int *__restrict__ face_start,
+                                                  const int *__restrict__ face_wave, int gather_waves,
                                                   const int *__restrict__ face_nbr,
                                                   const int *__restrict__ face_slot,
                                                   const uint32_t *__restrict__ fw, const float2 *__restrict__ fsc,
                                                   const float *__restrict__ hn, float *__restrict__ hd,
                                                   const float *__restrict__ interp, const int *__restrict__ corner,
-                                                  const float *__restrict__ far, const double *__restrict__ denom)
+                                                  const float *__restrict__ far, const TriGeom *__restrict__ geom,
+                                                  const int *__restrict__ err)
 {
     __shared__ float vals[WPB][WAVE];
+    __shared__ float facs[WPB][WAVE];
     const int t = st->t;
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wave = blockIdx.x * WPB + wid;
-    const int c0 = lower_seg(face_start, a.nt, wave * WAVE);
-    const int c1 = lower_seg(face_start, a.nt, wave * WAVE + WAVE);
+    // after a flagged mesh update the face lists may be incomplete: leave D[t] untouched
+    if (wave >= gather_waves || *err != 0)
+        return;
+    const int c0 = face_wave[wave], c1 = face_wave[wave + 1];
     if (c0 >= c1)
         return;
     const int q0 = face_start[c0], q1 = face_start[c1];
     const int cs = c0 + lane;
     const bool leader = cs < c1;
     const int m0 = leader ? face_start[cs] : 0, m1 = leader ? face_start[cs + 1] : 0;
-    float acc = 0.0f;
+    float acc = 0.0f, fac = 0.0f;
     for (int base = q0; base < q1; base += WAVE)
     {
         const int p = base + lane;
-        float v = 0.0f;
+        float v = 0.0f, f = 0.0f;
         if (p < q1)
         {
             const int j = face_nbr[p];
             const int ws = face_slot[p];
-            v = conv_pair<0, true>(fw + (size_t)ws * STEPS, fsc[ws], hn + j * RING, hd + j * RING, t);
+            const float2 sc = fsc[ws];
+            uint32_t w0;
+            v = conv_pair<0, true>(fw + (size_t)ws * STEPS, sc, hn + j * RING, hd + j * RING, t, &w0);
+            f = f16_bits_to_f32(w0 >> 16) * sc.y;  // face_factor = (float)D_0 * scale_D
         }
         vals[wid][lane] = v;
+        facs[wid][lane] = f;
         wave_sync();
         if (leader)
         {
             const int e = min(m1, base + WAVE);
             for (int q = max(m0, base); q < e; q++)
+            {
                 acc += vals[wid][q - base];
+                fac += facs[wid][q - base];
+            }
         }
         wave_sync();
     }
@@ -190,20 +191,25 @@ __global__ __launch_bounds__(TB) void k_dirichlet(StepArgs a, const StepState *_
         float ff = 0.0f;
 #pragma unroll
         for (int w = 0; w < 8; w++)
-            ff += interp[8 * cs + w] * far[corner[8 * cs + w]];
+        {
+            const int ci = corner[8 * cs + w];  // corners of a base cell are C3 cells (>= 0)
+            ff += interp[8 * cs + w] * (ci >= 0 ? far[ci] : 0.0f);
+        }
         const float num = ff * a.rc + acc;
-        hd[cs * RING + (t & (RING - 1))] = (float)((double)num / denom[cs]);
+        const double den = 0.5 * (double)geom[cs].area - (double)fac;
+        hd[cs * RING + (t & (RING - 1))] = (float)((double)num / den);
     }
 }
 
 // p(c) = far + 0.98 * sum W^bem (*) (N, D) over all 16 taps; records listener samples.
 __global__ __launch_bounds__(TB) void k_near(StepArgs a, const StepState *__restrict__ st, float *__restrict__ grid3,
                                              const int *__restrict__ c3_cells, const int *__restrict__ c3_start,
+                                             const int *__restrict__ c3_wave, int gather_waves,
                                              const int *__restrict__ c3_pairs, const uint32_t *__restrict__ wb,
                                              const float2 *__restrict__ sb, const float *__restrict__ hn,
                                              const float *__restrict__ hd, const float *__restrict__ far,
                                              const int *__restrict__ listen_cell, const int *__restrict__ listen_c3,
-                                             float *__restrict__ out, int gather_waves)
+                                             float *__restrict__ out)
 {
     __shared__ float vals[WPB][WAVE];
     const int t = st->t;
@@ -213,8 +219,7 @@ __global__ __launch_bounds__(TB) void k_near(StepArgs a, const StepState *__rest
     float *g = grid3 + (size_t)(((t % 3) + 3) % 3) * nn;
     if (wave < gather_waves)
     {
-        const int c0 = lower_seg(c3_start, a.n_c3, wave * WAVE);
-        const int c1 = lower_seg(c3_start, a.n_c3, wave * WAVE + WAVE);
+        const int c0 = c3_wave[wave], c1 = c3_wave[wave + 1];
         if (c0 >= c1)
             return;
         const int q0 = c3_start[c0], q1 = c3_start[c1];
@@ -245,7 +250,9 @@ __global__ __launch_bounds__(TB) void k_near(StepArgs a, const StepState *__rest
         if (leader)
         {
             const float v = (float)((double)far[cs] + (double)acc * 0.98);
-            g[c3_cells[cs]] = v;
+            const unsigned cell = (unsigned)c3_cells[cs];
+            if (cell < nn)
+                g[cell] = v;
             if (out)
                 for (int l = 0; l < a.n_listen; l++)
                     if (listen_c3[l] == cs)
@@ -282,40 +289,37 @@ __global__ void k_set_neumann(StepArgs a, const StepState *__restrict__ st, floa
 }
 
 inline int blocks_for_waves(long long waves) { return waves > 0 ? (int)((waves + WPB - 1) / WPB) : 1; }
-// waves needed to own every segment start in [0, pairs]
-inline int gather_waves_for(long long pairs) { return (int)(pairs / WAVE) + 1; }
 }  // namespace
 
 hipError_t launch_far(const StepArgs &a, const StepState *st, const float *grid3, const int *c3_cells,
-                      const int *c3_start, const int *c3_pairs, const uint32_t *wf, const float2 *sf, float *hn,
-                      const float *hd, float *far, const float *neumann, int n_pairs, hipStream_t s)
+                      const int *c3_start, const int *c3_wave, int c3_waves, const int *c3_pairs, const uint32_t *wf,
+                      const float2 *sf, float *hn, const float *hd, float *far, const float *neumann, hipStream_t s)
 {
-    const int gw = gather_waves_for(n_pairs);
-    const long long waves = (long long)gw + (neumann ? (a.nt + WAVE - 1) / WAVE : 0);
-    k_far<<<blocks_for_waves(waves), TB, 0, s>>>(a, st, grid3, c3_cells, c3_start, c3_pairs, wf, sf, hn, hd, far,
-                                                  neumann, gw);
+    const long long waves = (long long)c3_waves + (neumann ? (a.nt + WAVE - 1) / WAVE : 0);
+    k_far<<<blocks_for_waves(waves), TB, 0, s>>>(a, st, grid3, c3_cells, c3_start, c3_wave, c3_waves, c3_pairs, wf,
+                                                  sf, hn, hd, far, neumann);
     return hipGetLastError();
 }
 
-hipError_t launch_dirichlet(const StepArgs &a, const StepState *st, const int *face_start, const int *face_nbr,
-                            const int *face_slot, const uint32_t *fw, const float2 *fsc, const float *hn, float *hd,
-                            const float *interp, const int *corner, const float *far, const double *denom,
-                            int n_pairs, hipStream_t s)
+hipError_t launch_dirichlet(const StepArgs &a, const StepState *st, const int *face_start, const int *face_wave,
+                            int face_waves, const int *face_nbr, const int *face_slot, const uint32_t *fw,
+                            const float2 *fsc, const float *hn, float *hd, const float *interp, const int *corner,
+                            const float *far, const TriGeom *geom, const int *err, hipStream_t s)
 {
-    k_dirichlet<<<blocks_for_waves(gather_waves_for(n_pairs)), TB, 0, s>>>(a, st, face_start, face_nbr, face_slot, fw,
-                                                                          fsc, hn, hd, interp, corner, far, denom);
+    k_dirichlet<<<blocks_for_waves(face_waves), TB, 0, s>>>(a, st, face_start, face_wave, face_waves, face_nbr,
+                                                             face_slot, fw, fsc, hn, hd, interp, corner, far, geom,
+                                                             err);
     return hipGetLastError();
 }
 
 hipError_t launch_near(const StepArgs &a, const StepState *st, float *grid3, const int *c3_cells,
-                       const int *c3_start, const int *c3_pairs, const uint32_t *wb, const float2 *sb,
-                       const float *hn, const float *hd, const float *far, const int *listen_cell,
-                       const int *listen_c3, float *out, int n_pairs, hipStream_t s)
+                       const int *c3_start, const int *c3_wave, int c3_waves, const int *c3_pairs, const uint32_t *wb,
+                       const float2 *sb, const float *hn, const float *hd, const float *far, const int *listen_cell,
+                       const int *listen_c3, float *out, hipStream_t s)
 {
-    const int gw = gather_waves_for(n_pairs);
-    const long long waves = (long long)gw + (out ? 1 : 0);
-    k_near<<<blocks_for_waves(waves), TB, 0, s>>>(a, st, grid3, c3_cells, c3_start, c3_pairs, wb, sb, hn, hd, far,
-                                                   listen_cell, listen_c3, out, gw);
+    const long long waves = (long long)c3_waves + (out ? 1 : 0);
+    k_near<<<blocks_for_waves(waves), TB, 0, s>>>(a, st, grid3, c3_cells, c3_start, c3_wave, c3_waves, c3_pairs, wb,
+                                                   sb, hn, hd, far, listen_cell, listen_c3, out);
     return hipGetLastError();
 }
 

@@ -16,6 +16,7 @@
 // (64 B, one half cache line) plus a float2 {scale_S, scale_D}.
 #include "internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace pppm_amd
@@ -201,56 +202,48 @@ __device__ inline cdbl f2f64(const float *verts, const int *src, const int *trg,
     return {rr * f, ri * f};
 }
 
-// Radix-2 decimation-in-time inverse transform with the reference's butterfly order
-// (FFT<step>, bem.h:12-38): level STEP transforms the stride-STEP subsequence. The
-// order matters: with PI = 3.14159265359 the twiddle angles carry ~1e-13 rad errors and
-// lambda^-k (up to ~5.6e3 at k = 15) amplifies them to FP16-rounding scale, so another
-// DFT factorisation would flip stored FP16 codes. Twiddles come from a host table.
-template <int STEP>
-__device__ inline void fft_level(const cdbl *in, cdbl *out, cdbl *buf, const Tdbem &b)
-{
-    if constexpr (STEP == STEPS)
-    {
-        out[0] = in[0];
-    }
-    else
-    {
-        constexpr int NS = STEP * 2;
-        fft_level<NS>(in, out, buf, b);
-        fft_level<NS>(in + STEP, out + STEP, buf + STEP, b);
-#pragma unroll
-        for (int i = 0; i < STEPS; i += NS)
-        {
-            buf[i / 2] = out[i];
-            buf[(STEPS + i) / 2] = out[i + STEP];
-        }
-#pragma unroll
-        for (int i = 0; i < STEPS / 2; i += STEP)
-        {
-            const double cr = b.tw_re[i], ci = b.tw_im[i];
-            const cdbl x = buf[i + STEPS / 2];
-            const double tr = cr * x.re - ci * x.im;
-            const double ti = cr * x.im + ci * x.re;
-            const cdbl a = buf[i];
-            out[i] = {a.re + tr, a.im + ti};
-            out[i + STEPS / 2] = {a.re - tr, a.im - ti};
-        }
-    }
-}
+// Inverse transform with the reference's radix-2 decimation-in-time butterflies
+// (FFT<step>, bem.h:12-38), written in place on bit-reversed input. Each butterfly
+// a +- w*b uses the same operands, twiddle and operation order as the recursive form,
+// so results are bit-identical to it. The order matters: with PI = 3.14159265359 the
+// twiddle angles carry ~1e-13 rad errors and lambda^-k (up to ~5.6e3 at k = 15)
+// amplifies them to FP16-rounding scale, so another DFT factorisation would flip stored
+// FP16 codes. Twiddles come from a host-computed table.
+__device__ inline int bitrev4(int j) { return ((j & 1) << 3) | ((j & 2) << 1) | ((j & 4) >> 1) | ((j & 8) >> 3); }
 
 // scaledFFT (bem.h:292-304) of a conjugate-symmetric series given by its first 9 entries
 __device__ inline void scaled_idft(const cdbl *v9, const Tdbem &b, double *out)
 {
-    cdbl in[STEPS], o[STEPS], buf[STEPS];
+    cdbl x[STEPS];
 #pragma unroll
-    for (int k = 0; k < STEPS; k++)
-        in[k] = k <= 8 ? v9[k] : cdbl{v9[16 - k].re, -v9[16 - k].im};
-    fft_level<1>(in, o, buf, b);
+    for (int j = 0; j < STEPS; j++)
+    {
+        const int k = bitrev4(j);
+        x[j] = k <= 8 ? v9[k] : cdbl{v9[16 - k].re, -v9[16 - k].im};
+    }
+#pragma unroll
+    for (int L = 2; L <= STEPS; L <<= 1)
+    {
+        const int half = L >> 1, tws = STEPS / L;
+#pragma unroll
+        for (int g = 0; g < STEPS; g += L)
+#pragma unroll
+            for (int m = 0; m < half; m++)
+            {
+                const double cr = b.tw_re[m * tws], ci = b.tw_im[m * tws];
+                const cdbl y = x[g + m + half];
+                const double tr = cr * y.re - ci * y.im;
+                const double ti = cr * y.im + ci * y.re;
+                const cdbl a = x[g + m];
+                x[g + m] = {a.re + tr, a.im + ti};
+                x[g + m + half] = {a.re - tr, a.im - ti};
+            }
+    }
     double temp = 1.0;
 #pragma unroll
     for (int n = 0; n < STEPS; n++)
     {
-        out[n] = o[n].re / STEPS * temp;
+        out[n] = x[n].re / STEPS * temp;
         temp /= b.lambda;
     }
 }
@@ -286,6 +279,12 @@ __device__ inline f3 cell_center_f(int x, int y, int z, const WeightArgs &a)
     return f3{((float)x + 0.5f) * a.h, ((float)y + 0.5f) * a.h, ((float)z + 0.5f) * a.h} + f3{a.minx, a.miny, a.minz};
 }
 
+
+// A device-side count, or 0 when the mesh update has flagged an error (an invalid vertex
+// index, or a list that ran out of capacity and was left incomplete): downstream kernels
+// then do no work instead of reading unwritten entries.
+__device__ inline int live_count(const int *n_ptr, const int *err) { return *err != 0 ? 0 : *n_ptr; }
+
 // ---------------------------------------------------------------- grid cache
 // GridCache::need_recompute (pppm_cache.h:64-69); meta = center(3) normal(3) area per triangle
 __global__ __launch_bounds__(256) void k_grid_dirty(const TriGeom *__restrict__ geom, const float *__restrict__ meta,
@@ -309,102 +308,138 @@ __global__ __launch_bounds__(256) void k_grid_dirty(const TriGeom *__restrict__ 
     flag[i] = d ? 1 : 0;
 }
 
-// update_grid_cache_kernel (pppm_cache.cu:64-123): one thread per (dirty triangle, offset)
-__global__ __launch_bounds__(B) void k_grid_weights(const float *__restrict__ verts, const int *__restrict__ tris,
-                                                    const TriGeom *__restrict__ geom, const int *__restrict__ list,
-                                                    int n, WeightArgs a, Tdbem bem, uint32_t *__restrict__ wf,
-                                                    float2 *__restrict__ sf, uint32_t *__restrict__ wb,
-                                                    float2 *__restrict__ sb, float *__restrict__ meta)
+// update_grid_cache_kernel (pppm_cache.cu:64-123), integral part: one thread per
+// (dirty triangle, offset o, wave number k <= 8) evaluates the 8-target FDTD-stencil batch
+// and the single cell-centre target, both layers -> 4 complex values
+__global__ __launch_bounds__(B) void k_grid_integrals(const float *__restrict__ verts, const int *__restrict__ tris,
+                                                      const TriGeom *__restrict__ geom, const int *__restrict__ list,
+                                                      const int *__restrict__ n_ptr, WeightArgs a, Tdbem bem,
+                                                      cdbl *__restrict__ tmp, const int *__restrict__ err)
 {
-    const int i = blockIdx.x * B + threadIdx.x;
-    if (i >= n * GC)
-        return;
-    const int tri = list[i / GC], o = i % GC;
-    const TriGeom g = geom[tri];
-    const TriD T = tri_d(verts, tris[3 * tri], tris[3 * tri + 1], tris[3 * tri + 2]);
-    const int3_ c = g.cell3;
-    const int3_ nb = {c.x + o % 3 - 1, c.y + o / 3 % 3 - 1, c.z + o / 9 - 1};
-    const float s = a.c * a.c * a.dt * a.dt / (a.h * a.h);
-    const int dx[8] = {0, -1, 1, 0, 0, 0, 0, 0}, dy[8] = {0, 0, 0, -1, 1, 0, 0, 0}, dz[8] = {0, 0, 0, 0, 0, -1, 1, 0};
-    const float dsc[8] = {2, s, s, s, s, s, s, -6 * s};
-    d3 pts[8];
-    double scs[8];
-    int np = 0;
-    for (int q = 0; q < 8; q++)
+    const long long items = (long long)live_count(n_ptr, err) * GC * 9;
+    for (long long it = (long long)blockIdx.x * B + threadIdx.x; it < items; it += (long long)gridDim.x * B)
     {
-        const int x = nb.x + dx[q], y = nb.y + dy[q], z = nb.z + dz[q];
-        if (x == c.x && y == c.y && z == c.z)
-            continue;  // triangle_in_grid_cell (pppm_cache.cu:7-19)
-        pts[np] = cell_center_d(x, y, z, a);
-        scs[np] = dsc[q];
-        np++;
-    }
-    const bool self = (nb.x == c.x && nb.y == c.y && nb.z == c.z);
-    cdbl vs[9], vd[9], us[9], ud[9];
-    const d3 pc = cell_center_d(nb.x, nb.y, nb.z, a);
-    const double one = 1.0;
-    for (int k = 0; k <= 8; k++)
-    {
-        vs[k] = f2p64(T, pts, scs, np, bem.wave_re[k], bem.wave_im[k], 0);
-        vd[k] = f2p64(T, pts, scs, np, bem.wave_re[k], bem.wave_im[k], 1);
+        const int k = (int)(it % 9);
+        const int o = (int)((it / 9) % GC);
+        const int tri = list[it / (9 * GC)];
+        const TriGeom g = geom[tri];
+        const TriD T = tri_d(verts, tris[3 * tri], tris[3 * tri + 1], tris[3 * tri + 2]);
+        const int3_ c = g.cell3;
+        const int3_ nb = {c.x + o % 3 - 1, c.y + o / 3 % 3 - 1, c.z + o / 9 - 1};
+        const float s = a.c * a.c * a.dt * a.dt / (a.h * a.h);
+        const int dx[8] = {0, -1, 1, 0, 0, 0, 0, 0}, dy[8] = {0, 0, 0, -1, 1, 0, 0, 0},
+                  dz[8] = {0, 0, 0, 0, 0, -1, 1, 0};
+        const float dsc[8] = {2, s, s, s, s, s, s, -6 * s};
+        d3 pts[8];
+        double scs[8];
+        int np = 0;
+        for (int q = 0; q < 8; q++)
+        {
+            const int x = nb.x + dx[q], y = nb.y + dy[q], z = nb.z + dz[q];
+            if (x == c.x && y == c.y && z == c.z)
+                continue;  // triangle_in_grid_cell (pppm_cache.cu:7-19)
+            pts[np] = cell_center_d(x, y, z, a);
+            scs[np] = dsc[q];
+            np++;
+        }
+        const bool self = (nb.x == c.x && nb.y == c.y && nb.z == c.z);
+        const double kr = bem.wave_re[k], ki = bem.wave_im[k];
+        cdbl *o4 = tmp + it * 4;
+        o4[0] = f2p64(T, pts, scs, np, kr, ki, 0);
+        o4[1] = f2p64(T, pts, scs, np, kr, ki, 1);
         if (!self)
         {
-            us[k] = f2p64(T, &pc, &one, 1, bem.wave_re[k], bem.wave_im[k], 0);
-            ud[k] = f2p64(T, &pc, &one, 1, bem.wave_re[k], bem.wave_im[k], 1);
+            const d3 pc = cell_center_d(nb.x, nb.y, nb.z, a);
+            const double one = 1.0;
+            o4[2] = f2p64(T, &pc, &one, 1, kr, ki, 0);
+            o4[3] = f2p64(T, &pc, &one, 1, kr, ki, 1);
         }
         else
-            us[k] = ud[k] = {0, 0};
+            o4[2] = o4[3] = {0, 0};
     }
-    double ws[STEPS], wd[STEPS], gs[STEPS], gd[STEPS];
-    scaled_idft(vs, bem, ws);
-    scaled_idft(vd, bem, wd);
-    if (!self)
+}
+
+// finish: scaled IDFTs, move(-1), add(G, -1, -2) (bem.h:73-80, 128-140), FP16 quantisation
+__global__ __launch_bounds__(B) void k_grid_finish(const TriGeom *__restrict__ geom, const int *__restrict__ list,
+                                                   const int *__restrict__ n_ptr, Tdbem bem,
+                                                   const cdbl *__restrict__ tmp, uint32_t *__restrict__ wf,
+                                                   float2 *__restrict__ sf, uint32_t *__restrict__ wb,
+                                                   float2 *__restrict__ sb, float *__restrict__ meta,
+                                                   const int *__restrict__ err)
+{
+    const long long items = (long long)live_count(n_ptr, err) * GC;
+    for (long long it = (long long)blockIdx.x * B + threadIdx.x; it < items; it += (long long)gridDim.x * B)
     {
-        scaled_idft(us, bem, gs);
-        scaled_idft(ud, bem, gd);
-    }
-    else
-        for (int q = 0; q < STEPS; q++)
-            gs[q] = gd[q] = 0;
-    float fs[STEPS], fd[STEPS], bs[STEPS], bd[STEPS];
-    // w.move(-1) then w.add(G, -1, -2) (bem.h:73-80, 128-140)
-    for (int q = 0; q < STEPS; q++)
-    {
-        double x = q >= 1 ? ws[q - 1] : 0.0;
-        double y = q >= 1 ? wd[q - 1] : 0.0;
-        if (!self && q >= 2)
+        const int o = (int)(it % GC);
+        const int tri = list[it / GC];
+        const TriGeom g = geom[tri];
+        const int3_ c = g.cell3;
+        const bool self = (o == 13);  // offset (0, 0, 0): the triangle's own cell
+        (void)c;
+        const cdbl *src = tmp + it * 9 * 4;
+        cdbl v[9];
+        double ws[STEPS], wd[STEPS], gs[STEPS], gd[STEPS];
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+            v[k] = src[4 * k + 0];
+        scaled_idft(v, bem, ws);
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+            v[k] = src[4 * k + 1];
+        scaled_idft(v, bem, wd);
+        if (!self)
         {
-            x += gs[q - 2] * -1.0;
-            y += gd[q - 2] * -1.0;
+#pragma unroll
+            for (int k = 0; k < 9; k++)
+                v[k] = src[4 * k + 2];
+            scaled_idft(v, bem, gs);
+#pragma unroll
+            for (int k = 0; k < 9; k++)
+                v[k] = src[4 * k + 3];
+            scaled_idft(v, bem, gd);
         }
-        fs[q] = (float)x;
-        fd[q] = (float)y;
-        bs[q] = (float)gs[q];
-        bd[q] = (float)gd[q];
-    }
-    const size_t rec = (size_t)tri * GC + o;
-    quantize_store(fs, fd, wf + rec * STEPS, sf + rec);
-    quantize_store(bs, bd, wb + rec * STEPS, sb + rec);
-    if (o == 0)
-    {
-        float *m = meta + 7 * tri;
-        m[0] = g.center.x;
-        m[1] = g.center.y;
-        m[2] = g.center.z;
-        m[3] = g.normal.x;
-        m[4] = g.normal.y;
-        m[5] = g.normal.z;
-        m[6] = g.area;
+        else
+            for (int q = 0; q < STEPS; q++)
+                gs[q] = gd[q] = 0;
+        float fs[STEPS], fd[STEPS], bs[STEPS], bd[STEPS];
+        for (int q = 0; q < STEPS; q++)
+        {
+            double x = q >= 1 ? ws[q - 1] : 0.0;
+            double y = q >= 1 ? wd[q - 1] : 0.0;
+            if (!self && q >= 2)
+            {
+                x += gs[q - 2] * -1.0;
+                y += gd[q - 2] * -1.0;
+            }
+            fs[q] = (float)x;
+            fd[q] = (float)y;
+            bs[q] = (float)gs[q];
+            bd[q] = (float)gd[q];
+        }
+        const size_t rec = (size_t)tri * GC + o;
+        quantize_store(fs, fd, wf + rec * STEPS, sf + rec);
+        quantize_store(bs, bd, wb + rec * STEPS, sb + rec);
+        if (o == 0)
+        {
+            float *m = meta + 7 * tri;
+            m[0] = g.center.x;
+            m[1] = g.center.y;
+            m[2] = g.center.z;
+            m[3] = g.normal.x;
+            m[4] = g.normal.y;
+            m[5] = g.normal.z;
+            m[6] = g.area;
+        }
     }
 }
 
 // update_interpolation_weight_kernel (pppm_cache.cu:157-181), FP64 then rounded
 __global__ __launch_bounds__(256) void k_interp(const float *__restrict__ verts, const int *__restrict__ tris,
                                                 const TriGeom *__restrict__ geom, WeightArgs a,
-                                                float *__restrict__ interp)
+                                                float *__restrict__ interp, const int *__restrict__ err)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= a.nt)
+    if (i >= a.nt || *err != 0)  // vertex indices are only trusted once validated
         return;
     const TriD T = tri_d(verts, tris[3 * i], tris[3 * i + 1], tris[3 * i + 2]);
     const d3 p = l2g(TRI1_X, TRI1_X, T.v);
@@ -428,16 +463,32 @@ __device__ inline unsigned hash64(unsigned long long k, int cap)
 constexpr unsigned long long EMPTY_KEY = ~0ull;
 
 // compute_face_compute_list_kernel + get_cache_element_with_check (pppm_cache.cu:183-232,
-// pppm_cache.h:134-147): (c, j) -> persistent slot; dirty if new or moved past the thresholds
+// pppm_cache.h:134-147): (c, j) -> persistent slot; dirty if new or moved past the thresholds.
+// Pairs past the live count write dirty = 0 (the flags are scanned over the capacity).
 __global__ __launch_bounds__(256) void k_face_lookup(const TriGeom *__restrict__ geom, const int *__restrict__ tris,
                                                      const int *__restrict__ owner, const int *__restrict__ face_nbr,
-                                                     int n_p4, unsigned long long *hkeys, int *hvals, int hcap,
-                                                     int *pool_count, float2 *fmeta, int *face_slot, int *dirty)
+                                                     const int *__restrict__ n_ptr, int cap,
+                                                     unsigned long long *hkeys, int *hvals, int hcap, int *pool_count,
+                                                     int pool_cap, float2 *fmeta, int *face_slot, int *dirty,
+                                                     int *err_flag, int nt)
 {
     const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= n_p4)
+    if (p >= cap)
         return;
+    // lists past an overflow (or an invalid mesh) were not written: touch nothing
+    if (p >= live_count(n_ptr, err_flag))
+    {
+        dirty[p] = 0;
+        return;
+    }
     const int c = owner[p], j = face_nbr[p];
+    if ((unsigned)c >= (unsigned)nt || (unsigned)j >= (unsigned)nt)
+    {
+        atomicOr(err_flag, ERR_CAPACITY);
+        dirty[p] = 0;
+        face_slot[p] = -1;
+        return;
+    }
     const unsigned long long key = ((unsigned long long)c << 32) | (unsigned)j;
     unsigned h = hash64(key, hcap);
     int slot = -1;
@@ -445,26 +496,30 @@ __global__ __launch_bounds__(256) void k_face_lookup(const TriGeom *__restrict__
     for (int probe = 0; probe < hcap; probe++)
     {
         unsigned long long k = hkeys[h];
-        if (k == key)
-        {
-            slot = hvals[h];
-            break;
-        }
         if (k == EMPTY_KEY)
         {
             unsigned long long prev = atomicCAS(&hkeys[h], EMPTY_KEY, key);
             if (prev == EMPTY_KEY)
+                k = key, fresh = true;
+            else
+                k = prev;
+        }
+        if (k == key)
+        {
+            if (!fresh)
+                slot = hvals[h];
+            if (slot < 0)  // new key, or a key left without a slot by an earlier overflow
             {
                 slot = atomicAdd(pool_count, 1);
+                if (slot >= pool_cap)
+                {
+                    atomicOr(err_flag, ERR_CAPACITY);
+                    slot = -1;
+                }
                 hvals[h] = slot;
                 fresh = true;
-                break;
             }
-            if (prev == key)
-            {
-                slot = hvals[h];
-                break;
-            }
+            break;
         }
         h = (h + 1) & (unsigned)(hcap - 1);
     }
@@ -486,7 +541,11 @@ __global__ __launch_bounds__(256) void k_face_lookup(const TriGeom *__restrict__
     }
     if (d)
     {
-        fmeta[slot] = make_float2(normal_angle, distance);
+        // the slot's meta is written by k_face_finish once its weights exist; a new slot
+        // holds +inf until then, so an update that stops before the weights are computed
+        // leaves the pair dirty for the next one
+        if (fresh)
+            fmeta[slot] = make_float2(INFINITY, INFINITY);
         const int nn = common_vertices(tris[3 * j], tris[3 * j + 1], tris[3 * j + 2], tris[3 * c], tris[3 * c + 1],
                                        tris[3 * c + 2]);
         dirty[p] = 1 + (nn > 3 ? 3 : nn);
@@ -499,102 +558,81 @@ __global__ __launch_bounds__(256) void k_face_lookup(const TriGeom *__restrict__
 // far-field representation subtracted for corners farther than 1.5h (pppm_cache.cu:234-271)
 __global__ __launch_bounds__(B) void k_face_integrals(const float *__restrict__ verts, const int *__restrict__ tris,
                                                       const TriGeom *__restrict__ geom,
-                                                      const int *__restrict__ dirty_pairs, int n,
-                                                      const int *__restrict__ owner, const int *__restrict__ face_nbr,
+                                                      const int *__restrict__ dirty_pairs,
+                                                      const int *__restrict__ n_ptr, const int *__restrict__ owner,
+                                                      const int *__restrict__ face_nbr,
                                                       const float *__restrict__ interp, WeightArgs a, Tdbem bem,
-                                                      double *__restrict__ tmp)
+                                                      cdbl *__restrict__ tmp, const int *__restrict__ err)
 {
-    const int i = blockIdx.x * B + threadIdx.x;
-    if (i >= n * 18)
-        return;
-    const int d = i / 18, r = i % 18, layer = r / 9, k = r % 9;
-    const int p = dirty_pairs[d];
-    const int c = owner[p], j = face_nbr[p];
-    const int src[3] = {tris[3 * j], tris[3 * j + 1], tris[3 * j + 2]};
-    const int trg[3] = {tris[3 * c], tris[3 * c + 1], tris[3 * c + 2]};
-    const double kr = bem.wave_re[k], ki = bem.wave_im[k];
-    cdbl v = f2f64(verts, src, trg, kr, ki, layer, bem.threshold);
-    // batch_grid_near_field with the corners farther than 1.5h in any axis, scale -interp
-    const TriGeom gc = geom[c], gj = geom[j];
-    d3 pts[8];
-    double scs[8];
-    int np = 0;
-    const double lim = (double)a.h * 1.5;
-    for (int w = 0; w < 8; w++)
+    const long long items = (long long)live_count(n_ptr, err) * 18;
+    for (long long i = (long long)blockIdx.x * B + threadIdx.x; i < items; i += (long long)gridDim.x * B)
     {
-        const int x = gc.base3.x + (w >> 2), y = gc.base3.y + ((w >> 1) & 1), z = gc.base3.z + (w & 1);
-        const f3 ctr = cell_center_f(x, y, z, a);
-        if ((double)fabsf(ctr.x - gj.center.x) > lim || (double)fabsf(ctr.y - gj.center.y) > lim ||
-            (double)fabsf(ctr.z - gj.center.z) > lim)
+        const int d = (int)(i / 18), r = (int)(i % 18), layer = r / 9, k = r % 9;
+        const int p = dirty_pairs[d];
+        const int c = owner[p], j = face_nbr[p];
+        const int src[3] = {tris[3 * j], tris[3 * j + 1], tris[3 * j + 2]};
+        const int trg[3] = {tris[3 * c], tris[3 * c + 1], tris[3 * c + 2]};
+        const double kr = bem.wave_re[k], ki = bem.wave_im[k];
+        cdbl v = f2f64(verts, src, trg, kr, ki, layer, bem.threshold);
+        const TriGeom gc = geom[c], gj = geom[j];
+        d3 pts[8];
+        double scs[8];
+        int np = 0;
+        const double lim = (double)a.h * 1.5;
+        for (int w = 0; w < 8; w++)
         {
-            pts[np] = cell_center_d(x, y, z, a);
-            scs[np] = (double)(-interp[8 * c + w]);
-            np++;
+            const int x = gc.base3.x + (w >> 2), y = gc.base3.y + ((w >> 1) & 1), z = gc.base3.z + (w & 1);
+            const f3 ctr = cell_center_f(x, y, z, a);
+            if ((double)fabsf(ctr.x - gj.center.x) > lim || (double)fabsf(ctr.y - gj.center.y) > lim ||
+                (double)fabsf(ctr.z - gj.center.z) > lim)
+            {
+                pts[np] = cell_center_d(x, y, z, a);
+                scs[np] = (double)(-interp[8 * c + w]);
+                np++;
+            }
         }
+        if (np > 0)
+        {
+            const TriD J = tri_d(verts, src[0], src[1], src[2]);
+            cdbl u = f2p64(J, pts, scs, np, kr, ki, layer);
+            v.re += u.re;
+            v.im += u.im;
+        }
+        tmp[i] = v;
     }
-    if (np > 0)
-    {
-        const TriD J = tri_d(verts, src[0], src[1], src[2]);
-        cdbl u = f2p64(J, pts, scs, np, kr, ki, layer);
-        v.re += u.re;
-        v.im += u.im;
-    }
-    tmp[2 * (size_t)i] = v.re;
-    tmp[2 * (size_t)i + 1] = v.im;
 }
 
-__global__ __launch_bounds__(256) void k_face_finish(const int *__restrict__ dirty_pairs, int n,
-                                                     const int *__restrict__ face_slot, Tdbem bem,
-                                                     const double *__restrict__ tmp, uint32_t *__restrict__ fw,
-                                                     float2 *__restrict__ fsc)
+__global__ __launch_bounds__(B) void k_face_finish(const int *__restrict__ dirty_pairs,
+                                                   const int *__restrict__ n_ptr, const int *__restrict__ face_slot,
+                                                   const TriGeom *__restrict__ geom, const int *__restrict__ owner,
+                                                   const int *__restrict__ face_nbr, Tdbem bem,
+                                                   const cdbl *__restrict__ tmp, uint32_t *__restrict__ fw,
+                                                   float2 *__restrict__ fsc, float2 *__restrict__ fmeta,
+                                                   const int *__restrict__ err)
 {
-    const int d = blockIdx.x * 256 + threadIdx.x;
-    if (d >= n)
-        return;
-    const int slot = face_slot[dirty_pairs[d]];
-    cdbl vs[9], vd[9];
-    for (int k = 0; k < 9; k++)
+    const int n = live_count(n_ptr, err);
+    for (int d = blockIdx.x * B + threadIdx.x; d < n; d += gridDim.x * B)
     {
-        vs[k] = {tmp[2 * ((size_t)d * 18 + k)], tmp[2 * ((size_t)d * 18 + k) + 1]};
-        vd[k] = {tmp[2 * ((size_t)d * 18 + 9 + k)], tmp[2 * ((size_t)d * 18 + 9 + k) + 1]};
-    }
-    double ws[STEPS], wd[STEPS];
-    scaled_idft(vs, bem, ws);
-    scaled_idft(vd, bem, wd);
-    float fs[STEPS], fd[STEPS];
-    for (int q = 0; q < STEPS; q++)
-    {
-        fs[q] = (float)ws[q];
-        fd[q] = (float)wd[q];
-    }
-    quantize_store(fs, fd, fw + (size_t)slot * STEPS, fsc + slot);
-}
-
-// denominators 0.5*area - sum_j D_0(c, j) (pppm.cu:167, 181-187)
-__global__ __launch_bounds__(256) void k_face_denom(const TriGeom *__restrict__ geom, int nt,
-                                                    const int *__restrict__ face_start,
-                                                    const int *__restrict__ face_slot, const uint32_t *__restrict__ fw,
-                                                    const float2 *__restrict__ fsc, double *__restrict__ denom)
-{
-    const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= nt)
-        return;
-    float fsum = 0;
-    for (int p = face_start[c]; p < face_start[c + 1]; p++)
-    {
+        const int p = dirty_pairs[d];
         const int slot = face_slot[p];
-        fsum += f16_bits_to_f32(fw[(size_t)slot * STEPS] >> 16) * fsc[slot].y;
+        const TriGeom gc = geom[owner[p]], gj = geom[face_nbr[p]];
+        fmeta[slot] = make_float2(dotf(gj.normal, gc.normal), lengthf(gj.center - gc.center));
+        cdbl v[9];
+        double ws[STEPS], wd[STEPS];
+        for (int k = 0; k < 9; k++)
+            v[k] = tmp[(size_t)d * 18 + k];
+        scaled_idft(v, bem, ws);
+        for (int k = 0; k < 9; k++)
+            v[k] = tmp[(size_t)d * 18 + 9 + k];
+        scaled_idft(v, bem, wd);
+        float fs[STEPS], fd[STEPS];
+        for (int q = 0; q < STEPS; q++)
+        {
+            fs[q] = (float)ws[q];
+            fd[q] = (float)wd[q];
+        }
+        quantize_store(fs, fd, fw + (size_t)slot * STEPS, fsc + slot);
     }
-    denom[c] = 0.5 * (double)geom[c].area - (double)fsum;
-}
-
-__global__ __launch_bounds__(256) void k_face_owner(const int *__restrict__ face_start, int nt, int *owner)
-{
-    const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= nt)
-        return;
-    for (int p = face_start[c]; p < face_start[c + 1]; p++)
-        owner[p] = c;
 }
 
 __global__ __launch_bounds__(256) void k_rehash(const unsigned long long *__restrict__ okeys,
@@ -619,7 +657,16 @@ __global__ __launch_bounds__(256) void k_rehash(const unsigned long long *__rest
     }
 }
 
+__global__ __launch_bounds__(256) void k_fill_u64(unsigned long long *a, long long n, unsigned long long v)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n)
+        a[i] = v;
+}
+
 inline int nb(long long n, int b) { return n > 0 ? (int)((n + b - 1) / b) : 1; }
+// grid-stride launches for work whose size lives in device memory: cap bounds the items
+inline int nb_capped(long long cap_items, int b) { return (int)std::min<long long>(nb(cap_items, b), 8192); }
 }  // namespace
 
 void tdbem_init(Tdbem &b, float dt, float c, float grid_size)
@@ -646,63 +693,60 @@ void tdbem_init(Tdbem &b, float dt, float c, float grid_size)
     b.threshold = (float)(grid_size * 0.1);
 }
 
+size_t grid_tmp_bytes(int max_dirty) { return (size_t)max_dirty * GC * 9 * 4 * sizeof(cdbl); }
+size_t face_tmp_bytes(int max_dirty) { return (size_t)max_dirty * 18 * sizeof(cdbl); }
+
 hipError_t launch_grid_dirty(const TriGeom *geom, float *meta, int nt, float h, int first, int *flag, hipStream_t s)
 {
     k_grid_dirty<<<nb(nt, 256), 256, 0, s>>>(geom, meta, nt, h, first, flag);
     return hipGetLastError();
 }
-hipError_t launch_grid_weights(const float *verts, const int *tris, const TriGeom *geom, const int *list, int n,
-                               const WeightArgs &a, const Tdbem &b, uint32_t *wf, float2 *sf, uint32_t *wb,
-                               float2 *sb, float *meta, hipStream_t s)
+hipError_t launch_grid_weights(const float *verts, const int *tris, const TriGeom *geom, const int *list,
+                               const int *n_ptr, int cap, const WeightArgs &a, const Tdbem &b, void *tmp,
+                               uint32_t *wf, float2 *sf, uint32_t *wb, float2 *sb, float *meta, const int *err,
+                               hipStream_t s)
 {
-    if (n <= 0)
-        return hipSuccess;
-    k_grid_weights<<<nb((long long)n * GC, B), B, 0, s>>>(verts, tris, geom, list, n, a, b, wf, sf, wb, sb, meta);
+    k_grid_integrals<<<nb_capped((long long)cap * GC * 9, B), B, 0, s>>>(verts, tris, geom, list, n_ptr, a, b,
+                                                                          (cdbl *)tmp, err);
+    k_grid_finish<<<nb_capped((long long)cap * GC, B), B, 0, s>>>(geom, list, n_ptr, b, (const cdbl *)tmp, wf, sf, wb,
+                                                                    sb, meta, err);
     return hipGetLastError();
 }
 hipError_t launch_interp_weights(const float *verts, const int *tris, const TriGeom *geom, const WeightArgs &a,
-                                 float *interp, hipStream_t s)
+                                 float *interp, const int *err, hipStream_t s)
 {
-    k_interp<<<nb(a.nt, 256), 256, 0, s>>>(verts, tris, geom, a, interp);
+    k_interp<<<nb(a.nt, 256), 256, 0, s>>>(verts, tris, geom, a, interp, err);
     return hipGetLastError();
 }
-hipError_t launch_face_lookup(const TriGeom *geom, const int *tris, const int *owner, const int *face_nbr, int n_p4,
-                              unsigned long long *hkeys, int *hvals, int hcap, int *pool_count, float2 *fmeta,
-                              int *face_slot, int *dirty, hipStream_t s)
+hipError_t launch_face_lookup(const TriGeom *geom, const int *tris, const int *owner, const int *face_nbr,
+                              const int *n_ptr, int cap, unsigned long long *hkeys, int *hvals, int hcap,
+                              int *pool_count, int pool_cap, float2 *fmeta, int *face_slot, int *dirty, int *err_flag,
+                              int nt, hipStream_t s)
 {
-    if (n_p4 <= 0)
-        return hipSuccess;
-    k_face_lookup<<<nb(n_p4, 256), 256, 0, s>>>(geom, tris, owner, face_nbr, n_p4, hkeys, hvals, hcap, pool_count,
-                                                 fmeta, face_slot, dirty);
+    k_face_lookup<<<nb(cap, 256), 256, 0, s>>>(geom, tris, owner, face_nbr, n_ptr, cap, hkeys, hvals, hcap,
+                                               pool_count, pool_cap, fmeta, face_slot, dirty, err_flag, nt);
     return hipGetLastError();
 }
 hipError_t launch_face_weights(const float *verts, const int *tris, const TriGeom *geom, const int *dirty_pairs,
-                               int n, const int *owner, const int *face_nbr, const int *face_slot,
-                               const float *interp, const WeightArgs &a, const Tdbem &b, double *tmp, uint32_t *fw,
-                               float2 *fsc, hipStream_t s)
+                               const int *n_ptr, int cap, const int *owner, const int *face_nbr, const int *face_slot,
+                               const float *interp, const WeightArgs &a, const Tdbem &b, void *tmp, uint32_t *fw,
+                               float2 *fsc, float2 *fmeta, const int *err, hipStream_t s)
 {
-    if (n <= 0)
-        return hipSuccess;
-    k_face_integrals<<<nb((long long)n * 18, B), B, 0, s>>>(verts, tris, geom, dirty_pairs, n, owner, face_nbr,
-                                                            interp, a, b, tmp);
-    k_face_finish<<<nb(n, 256), 256, 0, s>>>(dirty_pairs, n, face_slot, b, tmp, fw, fsc);
-    return hipGetLastError();
-}
-hipError_t launch_face_owner(const int *face_start, int nt, int *owner, hipStream_t s)
-{
-    k_face_owner<<<nb(nt, 256), 256, 0, s>>>(face_start, nt, owner);
-    return hipGetLastError();
-}
-hipError_t launch_face_denom(const TriGeom *geom, int nt, const int *face_start, const int *face_slot,
-                             const uint32_t *fw, const float2 *fsc, double *denom, hipStream_t s)
-{
-    k_face_denom<<<nb(nt, 256), 256, 0, s>>>(geom, nt, face_start, face_slot, fw, fsc, denom);
+    k_face_integrals<<<nb_capped((long long)cap * 18, B), B, 0, s>>>(verts, tris, geom, dirty_pairs, n_ptr, owner,
+                                                                      face_nbr, interp, a, b, (cdbl *)tmp, err);
+    k_face_finish<<<nb_capped(cap, B), B, 0, s>>>(dirty_pairs, n_ptr, face_slot, geom, owner, face_nbr, b,
+                                                  (const cdbl *)tmp, fw, fsc, fmeta, err);
     return hipGetLastError();
 }
 hipError_t launch_rehash(const unsigned long long *okeys, const int *ovals, int ocap, unsigned long long *nkeys,
                          int *nvals, int ncap, hipStream_t s)
 {
     k_rehash<<<nb(ocap, 256), 256, 0, s>>>(okeys, ovals, ocap, nkeys, nvals, ncap);
+    return hipGetLastError();
+}
+hipError_t launch_fill_u64(unsigned long long *a, long long n, unsigned long long v, hipStream_t s)
+{
+    k_fill_u64<<<nb(n, 256), 256, 0, s>>>(a, n, v);
     return hipGetLastError();
 }
 

@@ -28,7 +28,7 @@ EXPORTS = [
     "pppm_time_idx", "pppm_grid_ptr", "pppm_read_cells", "pppm_history_ptrs", "pppm_triangles",
     "pppm_stats", "pppm_get_grid_weights", "pppm_set_grid_weights", "pppm_get_face_weights",
     "pppm_set_face_weights", "pppm_get_interp_weights", "pppm_get_c3", "pppm_get_b4",
-    "pppm_set_timing", "pppm_get_timing",
+    "pppm_set_timing", "pppm_get_timing", "pppm_run_moving",
 ]
 
 # LayerWeightHalf record (src/algorithm/bem.h:163-169): 16 x fp16, f32, 16 x fp16, f32
@@ -82,6 +82,7 @@ def load_library(path: str = LIB_PATH):
         "pppm_get_b4": ([vp, vp, vp, vp], i),
         "pppm_set_timing": ([vp, i], i),
         "pppm_get_timing": ([vp, vp, vp], i),
+        "pppm_run_moving": ([vp, vp, ll, vp, ll, i, vp, i, vp, i], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -201,6 +202,16 @@ class PPPMSolver:
         op, _ = _ptr(out_dev)
         self._check(self.lib.pppm_run(self.h, np_, int(stride), int(steps), lx.ctypes.data_as(ctypes.c_void_p),
                                       lx.shape[0], op, int(use_graph)), "run")
+
+    def run_moving(self, verts_series_dev, neumann_series_dev, steps, listeners, out_dev, use_graph=True):
+        """Moving mesh: before step i the vertices become verts_series_dev[i] (device, steps x nv x 3)."""
+        lx = np.ascontiguousarray(np.asarray(listeners, dtype=np.int32).reshape(-1, 3))
+        vp_, _ = _ptr(verts_series_dev)
+        np_, _ = _ptr(neumann_series_dev)
+        op, _ = _ptr(out_dev)
+        self._check(self.lib.pppm_run_moving(self.h, vp_, int(self.nv * 3), np_, int(self.nt), int(steps),
+                                             lx.ctypes.data_as(ctypes.c_void_p), lx.shape[0], op, int(use_graph)),
+                    "run_moving")
 
     def fdtd_step(self):
         self._check(self.lib.pppm_fdtd_step(self.h), "fdtd_step")
