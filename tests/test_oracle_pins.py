@@ -1,0 +1,500 @@
+"""Pins of the CPU oracle (test infrastructure) against facts it does not compute itself:
+IEEE half rounding, numpy's FFT, the closed-form convolution-quadrature weights of the
+retarded Green's function, analytic potentials of a triangle, the discrete plane wave of the
+leapfrog stencil, brute-force binning, trilinear exactness and the cached-vs-direct identity
+of the reference's own tests (GridCacheTest.cu, FaceCacheTest.cu, ParticleGridTest.cu,
+FFTTest.cu). The reference ships no golden vectors for this path, so each check is chosen so
+that a dropped term, a wrong sign or index, or a transposed operand fails it.
+"""
+import math
+
+import mpmath as mp
+import numpy as np
+import pytest
+
+from conftest import random_small_triangles
+
+PI_REF = 3.14159265359          # macro.h:80
+EPS_F = float(np.float32(1e-8))  # macro.h:100
+
+
+# ------------------------------------------------------------------ FP16 codec (bem.h:171-191)
+def test_f2h_is_ieee_round_to_nearest_even(oracle_mod):
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([
+        rng.standard_normal(4000).astype(np.float32) * np.float32(10.0) ** rng.integers(-9, 5, 4000),
+        np.float32([0.0, -0.0, 65504.0, 65519.99, 65520.0, 1e6, -1e6, 6.1e-5, 5.96e-8, 2.98e-8, 2.99e-8,
+                    1.0 + 2.0 ** -11, 1.0 + 3 * 2.0 ** -11, -(1.0 + 2.0 ** -11)]),
+    ]).astype(np.float32)
+    got = np.array([oracle_mod.f2h(float(x)) for x in xs], np.uint16)
+    with np.errstate(over="ignore"):
+        want = xs.astype(np.float16).view(np.uint16)
+    assert np.array_equal(got, want)
+
+
+def test_h2f_decodes_every_half(oracle_mod):
+    codes = np.arange(65536, dtype=np.uint32)
+    got = np.array([oracle_mod.h2f(int(c)) for c in codes], np.float32)
+    want = codes.astype(np.uint16).view(np.float16).astype(np.float32)
+    nan = np.isnan(want)
+    assert np.array_equal(np.isnan(got), nan)
+    assert np.array_equal(got[~nan].view(np.uint32), want[~nan].view(np.uint32))
+
+
+# ------------------------------------------------------------------ TDBEM transforms (bem.h:253-304)
+def test_lambda_rule_both_branches(oracle_mod):
+    for dt, c in [(1.05e-5, 343.0), (2e-4, 343.0), (1e-3, 340.3)]:
+        lam, _ = oracle_mod.tdbem_init(oracle_mod.FP64, dt, c)
+        dc = float(np.float32(dt)) * float(np.float32(c))   # dt and c arrive as float
+        assert lam == pytest.approx(max(dc ** (3 / 16), EPS_F ** (1 / 32)), rel=1e-14)
+    assert oracle_mod.tdbem_init(oracle_mod.FP64, 1.05e-5, 343.0)[0] == pytest.approx(EPS_F ** (1 / 32))
+    assert oracle_mod.tdbem_init(oracle_mod.FP64, 2e-4, 343.0)[0] > EPS_F ** (1 / 32)
+
+
+@pytest.mark.parametrize("mode", ["FP64", "REF32"])
+def test_scaled_fft_is_scaled_inverse_dft(oracle_mod, mode):
+    """out[n] = Re(sum_k v_k e^{+2 pi i k n / 16}) / 16 * lambda^-n, numpy's IFFT as the truth."""
+    rng = np.random.default_rng(2)
+    m = getattr(oracle_mod, mode)
+    for lam in (0.5623, 0.81, 1.0):
+        v = rng.standard_normal(16) + 1j * rng.standard_normal(16)
+        want = np.real(np.fft.ifft(v)) * lam ** -np.arange(16.0)
+        got = oracle_mod.scaled_fft(m, v, lam)
+        tol = 1e-10 if mode == "FP64" else 2e-5
+        assert np.max(np.abs(got - want)) <= tol * np.max(np.abs(want)), (mode, lam)
+        # FFTTest.cu: the radix-2 FFT and the direct scaled IDFT agree (abs 1e-3 there)
+        assert np.max(np.abs(oracle_mod.scaled_idft(v, lam) - want)) <= 1e-10 * np.max(np.abs(want))
+
+
+def _cq_single(r, dt, c, lam, alias=4):
+    """Exact BDF2 convolution-quadrature weights of e^{-s r / c} / (4 pi r), including the
+    aliasing of the 16-point scaled transform: w_n = sum_q lambda^{16 q} omega_{n + 16 q},
+    omega_n = [zeta^n] exp(-a (3 - 4 zeta + zeta^2)) / (4 pi r), a = r / (2 c dt)."""
+    mp.mp.dps = 40
+    a = mp.mpf(r) / (2 * mp.mpf(c) * mp.mpf(dt))
+
+    def omega(n):
+        s = mp.mpf(0)
+        for m in range(n // 2 + 1):
+            j = n - 2 * m
+            s += (4 * a) ** j / mp.factorial(j) * (-a) ** m / mp.factorial(m)
+        return mp.e ** (-3 * a) * s / (4 * mp.pi * mp.mpf(r))
+
+    return np.array([float(sum(mp.mpf(lam) ** (16 * q) * omega(n + 16 * q) for q in range(alias)))
+                     for n in range(16)])
+
+
+@pytest.mark.parametrize("mode,tol", [("FP64", 1e-6), ("REF32", 2e-4)])
+def test_laplace_weights_match_closed_form_cq(oracle_mod, mode, tol):
+    """laplace_weight (bem.h:306-316) for a face->point pair against the closed-form
+    convolution-quadrature weights: single layer = area * w_n(r); double layer = the
+    derivative of that along the source normal (central difference)."""
+    h = 0.00625
+    c = 343.0
+    dt = h / (math.sqrt(3) * c)
+    lam, _ = oracle_mod.tdbem_init(oracle_mod.FP64, dt, c)
+    m = getattr(oracle_mod, mode)
+    rng = np.random.default_rng(4)
+    for _ in range(3):
+        tri = (rng.random((3, 3)) * h * 0.6).astype(np.float32)
+        pt = (rng.random(3) * 4 * h - 2 * h + np.array([3 * h, 0, 0])).astype(np.float32)
+        T = tri.astype(np.float64)
+        y = T[0] + 0.3333333333333330 * (T[1] - T[0]) + 0.3333333333333330 * (T[2] - T[0])
+        n = np.cross(T[1] - T[0], T[2] - T[0])
+        area = 0.5 * np.linalg.norm(n)
+        n /= np.linalg.norm(n)
+        x = pt.astype(np.float64)
+        ws = area * _cq_single(np.linalg.norm(x - y), dt, c, lam)
+        got = oracle_mod.laplace_weight_point(m, dt, c, h, tri, pt, 0)
+        assert np.max(np.abs(got - ws)) <= tol * np.max(np.abs(ws))
+        eps = 1e-6 * h
+        wd = area * (_cq_single(np.linalg.norm(x - (y + eps * n)), dt, c, lam)
+                     - _cq_single(np.linalg.norm(x - (y - eps * n)), dt, c, lam)) / (2 * eps)
+        got = oracle_mod.laplace_weight_point(m, dt, c, h, tri, pt, 1)
+        assert np.max(np.abs(got - wd)) <= max(tol, 1e-5) * np.max(np.abs(wd))
+
+
+# ------------------------------------------------------------------ quadrature (integrand.h, potential.h)
+def _G(k, x, y):
+    r = np.linalg.norm(x - y)
+    return np.exp(1j * k * r) / (4 * math.pi * r)
+
+
+def _dG_dny(k, x, y, n, eps=1e-7):
+    s = eps * np.linalg.norm(x - y)
+    return (_G(k, x, y + s * n) - _G(k, x, y - s * n)) / (2 * s)
+
+
+def _tri(T):
+    n = np.cross(T[1] - T[0], T[2] - T[0])
+    area = 0.5 * np.linalg.norm(n)
+    return T[0] + 0.3333333333333330 * (T[1] - T[0]) + 0.3333333333333330 * (T[2] - T[0]), n / (2 * area), area
+
+
+def test_face2point_is_area_times_kernel(oracle_mod):
+    """1-point rule (gauss/triangle.h): single = area G(y_c, x); double = area dG/dn_y."""
+    rng = np.random.default_rng(5)
+    for _ in range(5):
+        tri = (rng.random((3, 3)) * 0.01).astype(np.float32)
+        pt = (rng.random(3) * 0.03 + 0.02).astype(np.float32)
+        k = complex(rng.uniform(-300, 300), rng.uniform(0, 300))
+        y, n, area = _tri(tri.astype(np.float64))
+        x = pt.astype(np.float64)
+        assert oracle_mod.face2point(oracle_mod.FP64, tri, pt, k, 0) == pytest.approx(area * _G(k, x, y), rel=1e-9)
+        assert oracle_mod.face2point(oracle_mod.FP64, tri, pt, k, 1) == pytest.approx(area * _dG_dny(k, x, y, n),
+                                                                                    rel=1e-6)
+
+
+def test_regular_face2face_is_product_rule(oracle_mod):
+    """No shared vertex, centres >= threshold apart: area_s area_t K(y_s, y_t) with the trial
+    (source) normal."""
+    rng = np.random.default_rng(6)
+    verts = (rng.random((6, 3)) * 0.01).astype(np.float32)
+    verts[3:] += np.float32(0.02)
+    k = complex(120.0, 80.0)
+    ys, ns, As = _tri(verts[:3].astype(np.float64))
+    yt, _, At = _tri(verts[3:].astype(np.float64))
+    got = oracle_mod.face2face(oracle_mod.FP64, verts, [0, 1, 2], [3, 4, 5], k, 0, 0.001)
+    assert got == pytest.approx(As * At * _G(k, yt, ys), rel=1e-9)
+    got = oracle_mod.face2face(oracle_mod.FP64, verts, [0, 1, 2], [3, 4, 5], k, 1, 0.001)
+    assert got == pytest.approx(As * At * _dG_dny(k, yt, ys, ns), rel=1e-6)
+
+
+@pytest.mark.parametrize("mode", ["FP64", "REF32"])
+def test_near_coincident_unconnected_pair_is_zero(oracle_mod, mode):
+    """integrand.h:240-246: centres closer than the threshold and no shared vertex route to
+    the singular rule with neighbour count 0, whose switch has no case -> exactly 0."""
+    verts = np.array([[0, 0, 0], [1e-3, 0, 0], [0, 1e-3, 0],
+                      [1e-5, 1e-5, 0], [1.01e-3, 1e-5, 0], [1e-5, 1.01e-3, 0]], np.float32)
+    for layer in (0, 1):
+        got = oracle_mod.face2face(getattr(oracle_mod, mode), verts, [0, 1, 2], [3, 4, 5], 100 + 50j, layer, 1e-3)
+        assert got == 0
+
+
+# analytic potentials of a flat triangle, used as the truth for the singular rules
+def _gl01(n):
+    x, w = np.polynomial.legendre.leggauss(n)
+    return 0.5 * (x + 1), 0.5 * w
+
+
+def _tri_rule(V, n):
+    s, ws = _gl01(n)
+    S, T = np.meshgrid(s, s, indexing="ij")
+    W = np.outer(ws, ws)
+    P = V[0] + S[..., None] * (V[1] - V[0]) + (S * T)[..., None] * (V[2] - V[1])
+    area2 = np.linalg.norm(np.cross(V[1] - V[0], V[2] - V[0]))
+    return P.reshape(-1, 3), (W * S * area2).reshape(-1)
+
+
+def _static_single(x, V):
+    """int_T 1/|x - y| dS_y (planar polygon formula, Wilton et al., IEEE TAP 1984)."""
+    n = np.cross(V[1] - V[0], V[2] - V[0])
+    n /= np.linalg.norm(n)
+    d = (x - V[0]) @ n
+    rho = x - d[:, None] * n
+    out = np.zeros(len(x))
+    ad = np.abs(d)
+    for a, b in ((V[0], V[1]), (V[1], V[2]), (V[2], V[0])):
+        sv = (b - a) / np.linalg.norm(b - a)
+        u = np.cross(sv, n)
+        lm, lp, t0 = (a - rho) @ sv, (b - rho) @ sv, (a - rho) @ u
+        R0sq = t0 ** 2 + d ** 2
+        Rm, Rp = np.linalg.norm(x - a, axis=1), np.linalg.norm(x - b, axis=1)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            lg = np.where(np.abs(t0) > 0, np.log((Rp + lp) / (Rm + lm)), 0.0)
+        out += t0 * lg - ad * (np.arctan2(t0 * lp, R0sq + ad * Rp) - np.arctan2(t0 * lm, R0sq + ad * Rm))
+    return out
+
+
+def _static_double(x, V):
+    """int_T (y - x).n / |y - x|^3 dS_y = solid angle of T from x (Van Oosterom & Strackee,
+    IEEE TBME 1983), for x off the plane of T."""
+    a, b, c = V[0] - x, V[1] - x, V[2] - x
+    la, lb, lc = (np.linalg.norm(q, axis=1) for q in (a, b, c))
+    num = np.einsum("ij,ij->i", a, np.cross(b, c))
+    den = la * lb * lc + np.einsum("ij,ij->i", a, b) * lc + np.einsum("ij,ij->i", a, c) * lb \
+        + np.einsum("ij,ij->i", b, c) * la
+    return 2 * np.arctan2(num, den)
+
+
+def _exact_pair(Vtr, Vte, k, layer, n_out=96, n4=20):
+    """int_test int_trial K_k(y, x) dS_y dS_x, split into the static kernel (analytic inner
+    integral, Gauss outer) and the bounded remainder K_k - K_0 (4-D product Gauss)."""
+    ntr = np.cross(Vtr[1] - Vtr[0], Vtr[2] - Vtr[0])
+    ntr /= np.linalg.norm(ntr)
+    P, W = _tri_rule(Vte, n_out)
+    if layer == 0:
+        st = np.sum(W * _static_single(P, Vtr))
+    else:
+        st = -np.sum(W * _static_double(P, Vtr))
+    Pa, Wa = _tri_rule(Vtr, n4)
+    Pb, Wb = _tri_rule(Vte, n4)
+    D = Pa[None, :, :] - Pb[:, None, :]          # y - x
+    r = np.linalg.norm(D, axis=-1)
+    ikr = 1j * k * r
+    with np.errstate(invalid="ignore", divide="ignore"):
+        if layer == 0:
+            f = np.where(r > 0, (np.exp(ikr) - 1) / r, 1j * k)
+        else:
+            f = np.where(r > 0, -(np.exp(ikr) * (1 - ikr) - 1) * (D @ ntr) / r ** 3, 0)
+    rem = np.sum(Wb[:, None] * Wa[None, :] * f)
+    return (st + rem) / (4 * math.pi)
+
+
+def test_analytic_triangle_potentials_match_brute_force():
+    rng = np.random.default_rng(8)
+    V = rng.random((3, 3))
+    P, W = _tri_rule(V, 60)
+    n = np.cross(V[1] - V[0], V[2] - V[0])
+    n /= np.linalg.norm(n)
+    for x in (np.array([2.0, 1.0, 0.5]), np.array([0.3, 0.3, 1.0]), V.mean(0) + 0.2 * n):
+        dv = P - x
+        r = np.linalg.norm(dv, axis=1)
+        assert _static_single(x[None], V)[0] == pytest.approx(np.sum(W / r), rel=1e-10)
+        assert _static_double(x[None], V)[0] == pytest.approx(np.sum(W * (dv @ n) / r ** 3), rel=1e-9)
+
+
+SINGULAR_CASES = [("identical", [0, 1, 2], [0, 1, 2]),
+                  ("common edge", [0, 1, 2], [0, 1, 3]),
+                  ("common vertex", [0, 1, 2], [0, 3, 4])]
+
+
+@pytest.mark.parametrize("name,trial,test", SINGULAR_CASES)
+@pytest.mark.parametrize("layer", [0, 1])
+def test_singular_duffy_rules_converge_to_the_integral(oracle_mod, name, trial, test, layer):
+    """Sauter-Schwab regions of singular_point (integrand.h:51-149) with the shared vertices
+    where the reference's parametrisation puts them (common vertex = v0 of both, common edge
+    = v0 -> v1 of both): at a high Gauss order the rule equals the analytic split to the
+    accuracy of the split itself. The reference's 3-point rule is the same mapping at order 3."""
+    h = 0.00625
+    c = 343.0
+    dt = h / (math.sqrt(3) * c)
+    _, wn = oracle_mod.tdbem_init(oracle_mod.FP64, dt, c)
+    k = wn[3]
+    verts = (np.array([[0, 0, 0], [1.0, 0.1, 0], [0.3, 0.9, 0.2], [0.5, -0.8, 0.4], [-0.7, 0.2, -0.6]]) * h
+             ).astype(np.float32)
+    V = verts.astype(np.float64)
+    hi = oracle_mod.singular_fp64(verts, trial, test, k, layer, 16)
+    if name == "identical" and layer == 1:
+        # in-plane: (y - x).n_y = 0 everywhere on the pair
+        assert abs(hi) == 0.0
+        assert oracle_mod.face2face(oracle_mod.FP64, verts, trial, test, k, 1, 0.1 * h) == 0
+        return
+    exact = _exact_pair(V[trial], V[test], k, layer)
+    assert abs(hi - exact) <= 2e-4 * abs(exact), (hi, exact)
+    prod = oracle_mod.face2face(oracle_mod.FP64, verts, trial, test, k, layer, 0.1 * h)
+    assert prod == pytest.approx(oracle_mod.singular_fp64(verts, trial, test, k, layer, 3), rel=1e-12)
+
+
+# ------------------------------------------------------------------ FDTD (fdtd.cu:7-33)
+def test_leapfrog_plane_wave_follows_discrete_dispersion(oracle_mod):
+    N, h, c = 40, 0.005, 343.0
+    dt = 0.9 * h / (math.sqrt(3) * c)
+    C = c * dt / h
+    kv = (2 * math.pi / (11 * h), 2 * math.pi / (17 * h), 2 * math.pi / (23 * h))
+    w = math.acos(1 - 2 * C * C * sum(math.sin(q * h / 2) ** 2 for q in kv)) / dt
+    i = np.arange(N) * h
+    X, Y, Z = np.meshgrid(i, i, i, indexing="ij")
+
+    def field(t):
+        return np.cos(kv[0] * X + kv[1] * Y + kv[2] * Z - w * t).astype(np.float32)
+
+    o = oracle_mod.Oracle(N, h, dt, (0, 0, 0), c, 0.83, oracle_mod.FP64)
+    o.set_grid(0, field(0.0))
+    o.set_grid(-1, field(-dt))
+    steps = 10
+    for _ in range(steps):
+        o.fdtd_step()
+    g = o.grid(0).reshape(N, N, N)
+    m = steps + 3                       # outside the boundary's domain of influence
+    sl = (slice(m, N - m),) * 3
+    assert np.max(np.abs(g[sl] - field(steps * dt)[sl])) < 2e-6
+    # the continuum phase is measurably different: the check resolves the stencil
+    cont = np.cos(kv[0] * X + kv[1] * Y + kv[2] * Z - c * math.sqrt(sum(q * q for q in kv)) * steps * dt)
+    assert np.max(np.abs(g[sl] - cont[sl])) > 1e-3
+
+
+def test_fdtd_zero_in_zero_out_and_axis_symmetry(oracle_mod):
+    N, h, c = 24, 0.005, 343.0
+    dt = h / (math.sqrt(3) * c)
+    o = oracle_mod.Oracle(N, h, dt, (0, 0, 0), c, 0.83, oracle_mod.FP64)
+    for _ in range(3):
+        o.fdtd_step()
+    assert not np.any(o.grid(0))
+    rng = np.random.default_rng(9)
+    p0 = np.zeros((N, N, N), np.float32)
+    p0[8:16, 8:16, 8:16] = rng.standard_normal((8, 8, 8))
+    outs = []
+    for perm in ((0, 1, 2), (2, 0, 1), (1, 2, 0)):
+        o = oracle_mod.Oracle(N, h, dt, (0, 0, 0), c, 0.83, oracle_mod.FP64)
+        q = np.ascontiguousarray(np.transpose(p0, perm))
+        o.set_grid(0, q)
+        o.set_grid(-1, q)
+        for _ in range(12):
+            o.fdtd_step()
+        inv = np.argsort(perm)
+        outs.append(np.transpose(o.grid(0).reshape(N, N, N), inv))
+    scale = np.max(np.abs(outs[0]))
+    for other in outs[1:]:
+        assert np.max(np.abs(other - outs[0])) <= 1e-5 * scale
+
+
+def _pulse_run(oracle_mod, N, steps, pml=0, rigid=False):
+    """Gaussian pulse released at the box centre; records the central 8^3 block per step.
+    rigid=True clamps the two outer layers to zero after every step (a reflecting box)."""
+    h, c = 0.005, 343.0
+    dt = h / (math.sqrt(3) * c)
+    o = oracle_mod.Oracle(N, h, dt, (0, 0, 0), c, 0.83, oracle_mod.FP64, pml)
+    i = (np.arange(N) - (N - 1) / 2.0)
+    X, Y, Z = np.meshgrid(i, i, i, indexing="ij")
+    p0 = np.exp(-(X ** 2 + Y ** 2 + Z ** 2) / (2 * 1.5 ** 2)).astype(np.float32)
+    o.set_grid(0, p0)
+    o.set_grid(-1, p0)
+    rec = []
+    for _ in range(steps):
+        o.fdtd_step()
+        g = o.grid(0).reshape(N, N, N)
+        if rigid:
+            g = g.copy()
+            for ax in range(3):
+                sl = [slice(None)] * 3
+                sl[ax] = slice(0, 2)
+                g[tuple(sl)] = 0
+                sl[ax] = slice(N - 2, N)
+                g[tuple(sl)] = 0
+            o.set_grid(0, g)
+        rec.append(g[N // 2 - 4:N // 2 + 4, N // 2 - 4:N // 2 + 4, N // 2 - 4:N // 2 + 4].copy())
+    return np.array(rec)
+
+
+@pytest.mark.parametrize("pml", [0, 6])
+def test_boundary_absorbs_an_outgoing_pulse(oracle_mod, pml):
+    """ABC (fdtd.cu:36-150) / PML (fdtd.cu:162-237): after the pulse has crossed the boundary
+    of a 32^3 box and its echo has come back, the box centre stays close to the centre of a
+    96^3 box (where no echo can have returned yet); a reflecting 32^3 box misses by about ten
+    times more."""
+    steps = 70
+    small = _pulse_run(oracle_mod, 32, steps, pml)
+    big = _pulse_run(oracle_mod, 96, steps, pml)
+    walls = _pulse_run(oracle_mod, 32, steps, rigid=True)
+    late = slice(45, steps)            # the boundary response is back at the centre
+    err = np.max(np.abs(small[late] - big[late]))
+    err_rigid = np.max(np.abs(walls[late] - big[late]))
+    assert err < 0.2 * err_rigid, (err, err_rigid)
+    early = slice(0, 15)               # before any boundary influence: identical
+    assert np.max(np.abs(small[early] - big[early])) <= 1e-6
+
+
+# ------------------------------------------------------------------ particle grid (ParticleGridTest.cu)
+def _small_scene(oracle_mod, seed=12, n=60, N=24):
+    rng = np.random.default_rng(seed)
+    h = 0.005
+    verts, tris = random_small_triangles(rng, n, N, h, size=1.0, margin=4)
+    dt = h / (math.sqrt(3) * 343.0)
+    o = oracle_mod.Oracle(N, h, dt, (0, 0, 0), 343.0, 0.83, oracle_mod.FP64)
+    o.set_mesh(verts, tris)
+    return o, verts, tris, N, h
+
+
+def test_binning_and_neighbour_lists_by_brute_force(oracle_mod):
+    o, verts, tris, N, h = _small_scene(oracle_mod)
+    tri = o.triangles()                          # centre(3) normal(3) area cell(3) base(3)
+    V = verts[tris].astype(np.float32)
+    centre = ((V[:, 0] + V[:, 1] + V[:, 2]) / np.float32(3)).astype(np.float32)
+    cross = np.cross(V[:, 1] - V[:, 0], V[:, 2] - V[:, 0])
+    assert np.allclose(tri[:, 0:3], centre, rtol=1e-6, atol=1e-9)
+    assert np.allclose(tri[:, 6], 0.5 * np.linalg.norm(cross, axis=1), rtol=1e-5)
+    assert np.allclose(tri[:, 3:6], cross / np.linalg.norm(cross, axis=1)[:, None], atol=1e-5)
+    cell = tri[:, 7:10].astype(int)
+    base = tri[:, 10:13].astype(int)
+    # the centre lies in its cell; the base cell is the lower corner of the 2x2x2 block
+    # of cell centres around it (particle_grid.h:155-161)
+    assert np.all(cell == np.floor(centre / np.float32(h)).astype(int))
+    cc = (base + 0.5) * h
+    assert np.all(centre >= cc - 1e-7) and np.all(centre < cc + h + 1e-7)
+    lin = lambda p: (p[..., 0] * N + p[..., 1]) * N + p[..., 2]
+
+    # C3: cells within +-1 of a triangle cell, each with the triangles of its 3x3x3 block
+    cells, start, pairs = o.c3()
+    want = set()
+    for c in cell:
+        for d in np.ndindex(3, 3, 3):
+            want.add(int(lin(c + np.array(d) - 1)))
+    assert sorted(want) == list(cells)
+    for q, cq in enumerate(cells):
+        recs = pairs[start[q]:start[q + 1]]
+        tris_q, offs = recs // 27, recs % 27
+        xyz = np.array([cq // (N * N), cq // N % N, cq % N])
+        expect = {t for t in range(len(cell)) if np.all(np.abs(cell[t] - xyz) <= 1)}
+        assert set(tris_q.tolist()) == expect and len(tris_q) == len(expect)
+        # record offset o names the target relative to the triangle's cell
+        d = np.stack([offs % 3 - 1, offs // 3 % 3 - 1, offs // 9 - 1], 1)
+        assert np.array_equal(cell[tris_q] + d, np.broadcast_to(xyz, d.shape))
+
+    # B4: non-empty base cells, each with the triangles whose cell is in its 4x4x4 block
+    bcells, bstart, btris = o.b4()
+    assert sorted({int(lin(b)) for b in base}) == list(bcells)
+    for q, bq in enumerate(bcells):
+        xyz = np.array([bq // (N * N), bq // N % N, bq % N])
+        got = btris[bstart[q]:bstart[q + 1]]
+        expect = {t for t in range(len(cell)) if np.all((cell[t] - xyz >= -1) & (cell[t] - xyz <= 2))}
+        assert set(got.tolist()) == expect and len(got) == len(expect)
+
+
+def test_interpolation_weights_reproduce_trilinear_fields(oracle_mod):
+    """FaceCacheTest.cu:37-71: the 8 corner weights integrate any trilinear field exactly
+    under the 1-point rule (weights = area x trilinear coefficients at the centroid)."""
+    o, verts, tris, N, h = _small_scene(oracle_mod, seed=13)
+    w = o.interp_weights()
+    tri = o.triangles()
+    base = tri[:, 10:13].astype(int)
+    rng = np.random.default_rng(14)
+    prm = rng.random(8)
+
+    def f(p):
+        x, y, z = p[..., 0] / h, p[..., 1] / h, p[..., 2] / h
+        return prm[0] * x + prm[1] * y + prm[2] * z + prm[3] * x * y + prm[4] * x * z + prm[5] * y * z \
+            + prm[6] * x * y * z + prm[7]
+
+    corners = np.array([[(j >> 2) & 1, (j >> 1) & 1, j & 1] for j in range(8)])
+    for i in range(len(tris)):
+        pts = (base[i] + corners + 0.5) * h
+        V = verts[tris[i]].astype(np.float64)
+        y = V[0] + 0.3333333333333330 * (V[1] - V[0]) + 0.3333333333333330 * (V[2] - V[0])
+        area = 0.5 * np.linalg.norm(np.cross(V[1] - V[0], V[2] - V[0]))
+        assert np.dot(w[i], f(pts)) == pytest.approx(area * f(y), rel=1e-5, abs=area * 1e-5)
+
+
+def test_cached_equals_direct_far_and_near(oracle_mod):
+    """GridCacheTest.cu:16-60: the cached far/near corrections equal the brute-force direct
+    path (weights built per (triangle, cell) on the fly, pppm_direct.cu:6-48) with random
+    Dirichlet history; the cached weights are FP16 with a per-record scale."""
+    rng = np.random.default_rng(15)
+    a, verts, tris, N, h = _small_scene(oracle_mod, seed=16, n=40)
+    b, _, _, _, _ = _small_scene(oracle_mod, seed=16, n=40)
+    T = len(tris)
+    nh = np.zeros((T, 32), np.float32)
+    dh = np.zeros((T, 32), np.float32)
+    dh[:, 17:32] = rng.integers(1, 6, (T, 15))    # D[t - k], k = 1..15, with t = 0 after the step
+    nh[:, 17:32] = rng.integers(1, 6, (T, 15))
+    for o in (a, b):
+        o.set_history(nh, dh)
+    a.solve_far_direct()
+    a.solve_near_direct()
+    b.fdtd_step()
+    b.solve_far()
+    b.solve_near()
+    fa, fb = a.far().ravel(), b.far().ravel()
+    ga, gb = a.grid(0).ravel(), b.grid(0).ravel()
+    assert a.time_idx() == b.time_idx() == 0
+    c3 = b.c3()[0]
+    assert len(c3) > 100
+    scale = np.max(np.abs(fa[c3]))
+    assert scale > 0
+    assert np.max(np.abs(fa[c3] - fb[c3])) <= 2e-3 * scale
+    # the direct correction adds the near field with factor 1 (pppm_direct.cu:6-18), the
+    # cached one with 0.98 (pppm.cu:53-114): compare the near fields themselves
+    near_direct = ga[c3] - fa[c3]
+    near_cached = (gb[c3].astype(np.float64) - fb[c3]) / 0.98
+    scale = np.max(np.abs(near_direct))
+    assert np.max(np.abs(near_direct - near_cached)) <= 2e-3 * scale
