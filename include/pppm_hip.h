@@ -136,6 +136,27 @@ int pppm_get_interp_weights(pppm_solver *s, float *host_out8);
 int pppm_get_c3(pppm_solver *s, int *cells, int *start, int *pairs);
 int pppm_get_b4(pppm_solver *s, int *cells, int *start, int *tris);
 
+/* Cell classification and fresh cells for moving meshes — the ghost-cell solver's cell data
+ * (GhostCellSolver::precompute_cell_data / fill_in_fresh_cell / set_solid_cell_zero,
+ * src/algorithm/ghost_cell.h:69-212, ghost_cell_util.cu:7-230, ghost_cell.cu:51-218,547-560)
+ * computed on the handle's current mesh and grid. Cell types: 0 AIR, 1 SOLID, 2 GHOST,
+ * 3 UNKNOWN (ghost_cell.h:11-17).
+ * pppm_classify: labels every cell (nearest triangle in the 7^3 window, side of the surface,
+ *   ghost cells, column fill, ghost indices in linear order); the previous labels are kept
+ *   for pppm_fill_fresh (all AIR before the first call). thin_shell: the reference's
+ *   thin-shell test (condition_number_threshold == 0, ghost_cell.cu:236).
+ * pppm_fill_fresh: cells that turned AIR get values at t and t-1 from the cells around their
+ *   mirror point, with the per-triangle accelerations now / one step before (nt floats each).
+ *   *fresh_count = their number, or -1 if one had no nearest triangle (then 1e10 is written
+ *   at t, as the reference's error path).
+ * pppm_set_solid_zero: SOLID cells of grid level t set to 0.
+ * pppm_get_cells: host copies (N^3 entries, x-major; any pointer may be NULL). */
+int pppm_classify(pppm_solver *s, int thin_shell, int *ghost_count);
+int pppm_fill_fresh(pppm_solver *s, const float *acc_now, const float *acc_old, int on_device, int *fresh_count);
+int pppm_set_solid_zero(pppm_solver *s);
+int pppm_get_cells(pppm_solver *s, int *type, int *ghost_idx, int *nearest, float *point3, float *reflect3,
+                   float *dist);
+
 #ifdef __cplusplus
 }
 #endif

@@ -71,6 +71,12 @@ def lib():
             "orc_face2face": ([i, vp, vp, vp, d, d, i, f, vp], None),
             "orc_laplace_weight_point": ([i, f, f, f, vp, vp, i, vp], None),
             "orc_singular_fp64": ([vp, vp, vp, d, d, i, i, vp], None),
+            "orc_classify": ([vp, i], i),
+            "orc_get_cells": ([vp, vp, vp, vp, vp, vp, vp], i),
+            "orc_fill_fresh": ([vp, vp, vp], i),
+            "orc_set_solid_zero": ([vp], i),
+            "orc_vertex_normals": ([vp, vp], i),
+            "orc_closest_coeff": ([vp, vp, vp], None),
         }
         for name, (a, r) in sig.items():
             fn = getattr(L, name)
@@ -232,6 +238,36 @@ class Oracle:
         self.L.orc_get_interp_weights(self.s, _p(out))
         return out
 
+    # cell classification / fresh cells (ghost-cell solver)
+    def classify(self, thin_shell=False):
+        r = self.L.orc_classify(self.s, int(bool(thin_shell)))
+        if r < 0:
+            raise RuntimeError("oracle classify failed")
+        return r
+
+    def cells(self):
+        NN = self.N ** 3
+        out = {"type": np.zeros(NN, np.int32), "ghost_idx": np.zeros(NN, np.int32),
+               "nearest": np.zeros(NN, np.int32), "point": np.zeros((NN, 3), np.float32),
+               "reflect": np.zeros((NN, 3), np.float32), "dist": np.zeros(NN, np.float32)}
+        self._ok(self.L.orc_get_cells(self.s, *(_p(out[k]) for k in
+                                                 ("type", "ghost_idx", "nearest", "point", "reflect", "dist"))),
+                 "get_cells")
+        return out
+
+    def fill_fresh(self, acc_now, acc_old):
+        a = np.ascontiguousarray(acc_now, np.float32)
+        b = np.ascontiguousarray(acc_old, np.float32)
+        return int(self.L.orc_fill_fresh(self.s, _p(a), _p(b)))
+
+    def set_solid_zero(self):
+        self._ok(self.L.orc_set_solid_zero(self.s), "set_solid_zero")
+
+    def vertex_normals(self, nv):
+        out = np.zeros((nv, 3), np.float32)
+        self._ok(self.L.orc_vertex_normals(self.s, _p(out)), "vertex_normals")
+        return out
+
 
 # ---------------------------------------------------------------------- primitives
 def f2h(x: float) -> int:
@@ -300,4 +336,12 @@ def laplace_weight_point(mode, dt, c, h, tri9, pt3, layer):
     pt = np.ascontiguousarray(pt3, np.float32).reshape(3)
     out = np.zeros(16, np.float64)
     lib().orc_laplace_weight_point(int(mode), float(dt), float(c), float(h), _p(tri), _p(pt), int(layer), _p(out))
+    return out
+
+
+def closest_coeff(p3, tri9):
+    p = np.ascontiguousarray(p3, np.float32).reshape(3)
+    t = np.ascontiguousarray(tri9, np.float32).reshape(9)
+    out = np.zeros(3, np.float32)
+    lib().orc_closest_coeff(_p(p), _p(t), _p(out))
     return out

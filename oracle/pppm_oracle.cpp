@@ -777,6 +777,13 @@ struct orc_scene
     std::vector<float> hn, hd;
     std::vector<float> face_far;
 
+    // cell classification (GhostCellSolver::cell_data / cell_data_old, ghost_cell.h:125-127)
+    std::vector<int> ctype, ctype_old, cghost, cnearest;
+    std::vector<f3> cpoint, creflect;
+    std::vector<float> cdist;
+    int ghost_count = 0;
+    bool classified = false;
+
     inline int lin(int x, int y, int z) const { return (x * N + y) * N + z; }
     inline int lin(i3 c) const { return lin(c.x, c.y, c.z); }
     inline std::vector<float> &G(int rel) { return grid[((t + rel) % 3 + 3) % 3]; }
@@ -1596,6 +1603,371 @@ static float near_field_direct(const orc_scene &s, i3 src_center, i3 dst, int t)
     return v;
 }
 
+// ------------------------------------------------------------------ cell classification
+// GhostCellSolver's cell data (ghost_cell_util.cu:7-230) and fresh cells (ghost_cell.cu:51-218,
+// 547-560). Cell types follow the reference's enum (ghost_cell.h:11-17).
+enum : int
+{
+    CELL_AIR = 0,
+    CELL_SOLID = 1,
+    CELL_GHOST = 2,
+    CELL_UNKNOWN = 3
+};
+
+struct P2
+{
+    float s, t;
+};
+
+// Closest point of a triangle to a point, as barycentric weights of (v0, v1, v2): Eberly,
+// "Distance from a Point to a Triangle" (Geometric Tools, 2020) — minimise
+// |v0 + s e0 + t e1 - p|^2 over the triangle by following the zero set of the gradient's
+// directional derivative. Same float operations as get_nearest_triangle_point_coeff
+// (helper_math.h:1603-1800), on which the labels depend.
+static inline P2 min_on_edge02(float a11, float b1)
+{
+    const int neg = b1 < 0, past = a11 + b1 <= 0;
+    return {0.0f, (float)neg * ((float)past - (float)(1 - past) * (b1 / a11))};
+}
+static inline P2 min_on_edge12(float a01, float a11, float b1, float f10, float f01)
+{
+    const float g0 = a01 + b1 - f10, g1 = a11 + b1 - f01;
+    const int neg = g0 < 0, past = g1 <= 0;
+    const float t = (float)neg * ((float)past + (float)(1 - past) * g0 / (g0 - g1));
+    return {1 - t, t};
+}
+static inline P2 min_inside(P2 q0, float g0, P2 q1, float g1)
+{
+    const float z = g0 / (g0 - g1), w = 1 - z;
+    return {w * q0.s + z * q1.s, w * q0.t + z * q1.t};
+}
+static f3 closest_coeff(f3 p, f3 v0, f3 v1, f3 v2)
+{
+    const f3 d = p - v0, e0 = v1 - v0, e1 = v2 - v0;
+    const float a00 = dot(e0, e0), a01 = dot(e0, e1), a11 = dot(e1, e1);
+    const float b0 = -dot(d, e0), b1 = -dot(d, e1);
+    const float f00 = b0, f10 = b0 + a00, f01 = b0 + a01;
+    P2 q0{0, 0}, q1{0, 0}, r{0, 0};
+    float g0, g1;
+    if (f00 >= 0)
+    {
+        if (f01 >= 0)
+            r = min_on_edge02(a11, b1);
+        else
+        {
+            q0 = {0, f00 / (f00 - f01)};
+            q1.s = f01 / (f01 - f10);
+            q1.t = 1 - q1.s;
+            const float dt1 = q1.t - q0.t;
+            g0 = dt1 * (a11 * q0.t + b1);
+            if (g0 >= 0)
+                r = min_on_edge02(a11, b1);
+            else
+            {
+                g1 = dt1 * (a01 * q1.s + a11 * q1.t + b1);
+                r = g1 <= 0 ? min_on_edge12(a01, a11, b1, f10, f01) : min_inside(q0, g0, q1, g1);
+            }
+        }
+    }
+    else if (f01 <= 0)
+    {
+        if (f10 <= 0)
+            r = min_on_edge12(a01, a11, b1, f10, f01);
+        else
+        {
+            q0 = {f00 / (f00 - f10), 0};
+            q1.s = f01 / (f01 - f10);
+            q1.t = 1 - q1.s;
+            g0 = q1.t * (a01 * q0.s + b1);
+            if (g0 >= 0)
+                r = q0;
+            else
+            {
+                g1 = q1.t * (a01 * q1.s + a11 * q1.t + b1);
+                r = g1 <= 0 ? min_on_edge12(a01, a11, b1, f10, f01) : min_inside(q0, g0, q1, g1);
+            }
+        }
+    }
+    else if (f10 <= 0)
+    {
+        q0 = {0, f00 / (f00 - f01)};
+        q1.s = f01 / (f01 - f10);
+        q1.t = 1 - q1.s;
+        const float dt1 = q1.t - q0.t;
+        g0 = dt1 * (a11 * q0.t + b1);
+        if (g0 >= 0)
+            r = min_on_edge02(a11, b1);
+        else
+        {
+            g1 = dt1 * (a01 * q1.s + a11 * q1.t + b1);
+            r = g1 <= 0 ? min_on_edge12(a01, a11, b1, f10, f01) : min_inside(q0, g0, q1, g1);
+        }
+    }
+    else
+    {
+        q0 = {f00 / (f00 - f10), 0};
+        q1 = {0, f00 / (f00 - f01)};
+        g0 = q1.t * (a01 * q0.s + b1);
+        if (g0 >= 0)
+            r = q0;
+        else
+        {
+            g1 = q1.t * (a11 * q1.t + b1);
+            r = g1 <= 0 ? min_on_edge02(a11, b1) : min_inside(q0, g0, q1, g1);
+        }
+    }
+    return {1 - r.s - r.t, r.s, r.t};
+}
+
+// angle-weighted vertex normals over the faces that use each vertex, in ascending face
+// order (fill_in_nearest_kernel, ghost_cell_util.cu:56-80; the reference's list order is
+// that of atomic appends)
+static std::vector<f3> vertex_normals(const orc_scene &s)
+{
+    const int nv = (int)s.verts.size();
+    std::vector<std::vector<int>> faces(nv);
+    for (int i = 0; i < s.nt; i++)
+        for (int k = 0; k < 3; k++)
+            faces[s.tris[i].idx[k]].push_back(i);
+    std::vector<f3> out(nv, f3{0, 0, 0});
+    for (int v = 0; v < nv; v++)
+    {
+        f3 n{0, 0, 0};
+        for (int f : faces[v])
+        {
+            const Tri &t = s.tris[f];
+            int o[2], m = 0;
+            for (int k = 0; k < 3; k++)
+                if (t.idx[k] != v)
+                    o[m++] = t.idx[k];
+            const f3 e1 = s.verts[o[0]] - s.verts[v], e2 = s.verts[o[1]] - s.verts[v];
+            const float ang = (float)std::acos((double)(dot(e1, e2) / (length(e1) * length(e2))));
+            n = n + t.normal * ang;
+        }
+        out[v] = n * (1.0f / sqrtf(dot(n, n)));
+    }
+    return out;
+}
+
+// fill_cell_data (ghost_cell_util.cu:207-230): nearest triangle in the 7^3 window, side of
+// the surface, ghost cells, column fill of unknown cells, ghost indices
+static int classify(orc_scene &s, bool thin_shell)
+{
+    const int N = s.N, NN = N * N * N;
+    if (s.classified)
+        s.ctype_old = s.ctype;
+    else
+        s.ctype_old.assign(NN, CELL_AIR);
+    s.ctype.assign(NN, CELL_UNKNOWN);
+    s.cnearest.assign(NN, -1);
+    s.cpoint.assign(NN, f3{0, 0, 0});
+    s.creflect.assign(NN, f3{0, 0, 0});
+    s.cdist.assign(NN, 3.402823466e+38F);
+    const std::vector<f3> vn = vertex_normals(s);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int x = 0; x < N; x++)
+        for (int y = 0; y < N; y++)
+            for (int z = 0; z < N; z++)
+            {
+                const f3 ctr = s.center_f(i3{x, y, z});
+                float best = 3.402823466e+38F;
+                int bi = -1;
+                f3 bp{0, 0, 0}, bc{0, 0, 0};
+                for (int dx = -3; dx <= 3; dx++)
+                    for (int dy = -3; dy <= 3; dy++)
+                        for (int dz = -3; dz <= 3; dz++)
+                        {
+                            const int cx = x + dx, cy = y + dy, cz = z + dz;
+                            if (cx < 0 || cy < 0 || cz < 0 || cx >= N || cy >= N || cz >= N)
+                                continue;
+                            const int cl = s.lin(cx, cy, cz);
+                            for (int q = s.cell_start[cl]; q < s.cell_start[cl + 1]; q++)
+                            {
+                                const int f = s.cell_items[q];
+                                const Tri &t = s.tris[f];
+                                const f3 v0 = s.verts[t.idx[0]], v1 = s.verts[t.idx[1]], v2 = s.verts[t.idx[2]];
+                                const f3 cf3 = closest_coeff(ctr, v0, v1, v2);
+                                const f3 np = cf3.x * v0 + cf3.y * v1 + cf3.z * v2;
+                                const float len = length(ctr - np);
+                                if (len < best)
+                                {
+                                    best = len;
+                                    bi = f;
+                                    bp = np;
+                                    bc = cf3;
+                                }
+                            }
+                        }
+                const int c = s.lin(x, y, z);
+                if (bi < 0)
+                    continue;  // UNKNOWN
+                s.cdist[c] = best;
+                s.cnearest[c] = bi;
+                s.cpoint[c] = bp;
+                s.creflect[c] = 2.0f * bp - ctr;
+                const Tri &t = s.tris[bi];
+                const f3 nrm = bc.x * vn[t.idx[0]] + bc.y * vn[t.idx[1]] + bc.z * vn[t.idx[2]];
+                const float direction = dot(nrm, ctr - bp);
+                const f3 dv = bp - ctr;
+                const float hh = s.h / 2;
+                const bool shell = thin_shell && std::fabs(dv.x) < hh && std::fabs(dv.y) < hh && std::fabs(dv.z) < hh;
+                s.ctype[c] = (direction <= 0 || shell) ? CELL_SOLID : CELL_AIR;
+            }
+    // ghost cells: SOLID with an AIR 6-neighbour (cell_classfication_kernel)
+    std::vector<int> typ(s.ctype), flag(NN, 0);
+    for (int x = 0; x < N; x++)
+        for (int y = 0; y < N; y++)
+            for (int z = 0; z < N; z++)
+            {
+                const int c = s.lin(x, y, z);
+                if (s.ctype[c] != CELL_SOLID)
+                    continue;
+                const int nb[6][3] = {{x + 1, y, z}, {x - 1, y, z}, {x, y + 1, z},
+                                      {x, y - 1, z}, {x, y, z + 1}, {x, y, z - 1}};
+                for (auto &q : nb)
+                    if (q[0] >= 0 && q[1] >= 0 && q[2] >= 0 && q[0] < N && q[1] < N && q[2] < N &&
+                        s.ctype[s.lin(q[0], q[1], q[2])] == CELL_AIR)
+                        typ[c] = CELL_GHOST;
+                flag[c] = typ[c] == CELL_GHOST;
+            }
+    // column fill along z (post_cell_classification_kernel)
+    for (int x = 0; x < N; x++)
+        for (int y = 0; y < N; y++)
+        {
+            int cur = CELL_AIR;
+            bool found = false;
+            for (int z = 0; z < N; z++)
+            {
+                const int c = s.lin(x, y, z);
+                if (!found && typ[c] != CELL_UNKNOWN)
+                {
+                    found = true;
+                    cur = typ[c];
+                    for (int k = 0; k < z; k++)
+                        typ[s.lin(x, y, k)] = cur;
+                    continue;
+                }
+                if (typ[c] == CELL_SOLID || typ[c] == CELL_GHOST)
+                    cur = CELL_SOLID;
+                else if (typ[c] == CELL_AIR)
+                    cur = CELL_AIR;
+                else
+                    typ[c] = cur;
+            }
+        }
+    // ghost index = inclusive scan of the pre-fill ghost flags - 1 (apply_cell_type_kernel)
+    s.cghost.assign(NN, -1);
+    int run = 0;
+    for (int c = 0; c < NN; c++)
+    {
+        run += flag[c];
+        s.ctype[c] = typ[c];
+        if (typ[c] == CELL_GHOST)
+            s.cghost[c] = run - 1;
+    }
+    s.ghost_count = run;
+    s.classified = true;
+    return run;
+}
+
+// get_fresh_cell_list + solve_fresh_history (ghost_cell.cu:51-196): cells that turned AIR
+// get values at t and t-1 from the AIR (old and new) cells around their mirror point.
+// Returns the number of fresh cells, or -1 if one of them has no nearest triangle (the
+// reference then writes the 1e10 sentinel into every fresh cell at t).
+static int fill_fresh(orc_scene &s, const float *acc_now, const float *acc_old)
+{
+    const int N = s.N, NN = N * N * N;
+    std::vector<int> fresh;
+    for (int c = 0; c < NN; c++)
+        if (s.ctype_old[c] != CELL_AIR && s.ctype[c] == CELL_AIR)
+            fresh.push_back(c);
+    bool err = false;
+    for (int c : fresh)
+        if (s.cnearest[c] < 0 || s.cnearest[c] >= s.nt)
+            err = true;
+    auto air2 = [&](i3 q) {
+        if (q.x < 0 || q.y < 0 || q.z < 0 || q.x >= N || q.y >= N || q.z >= N)
+            return false;
+        const int l = s.lin(q);
+        return s.ctype_old[l] == CELL_AIR && s.ctype[l] == CELL_AIR;
+    };
+    std::vector<float> &g0 = s.G(0), &g1 = s.G(-1);
+    for (int c : fresh)
+    {
+        if (err)
+        {
+            g0[c] = 1e10f;
+            continue;
+        }
+        const i3 co{c / (N * N), c / N % N, c % N};
+        const int tri = s.cnearest[c];
+        const float accs[2] = {acc_now[tri], acc_old[tri]};
+        std::vector<float> *lv[2] = {&g0, &g1};
+        const f3 xb = s.cpoint[c], xf = s.center_f(co);
+        const f3 xr = xb + 2.0f * (xf - xb);
+        const i3 base = base_coord(s, xr);
+        i3 nb[8];
+        float w[8];
+        bool any = false;
+        for (int dx = 0; dx < 2; dx++)
+            for (int dy = 0; dy < 2; dy++)
+                for (int dz = 0; dz < 2; dz++)
+                {
+                    const int k = dx * 4 + dy * 2 + dz;
+                    nb[k] = base + i3{dx, dy, dz};
+                    if (air2(nb[k]))
+                    {
+                        const float dist = length(s.center_f(nb[k]) - xr);
+                        w[k] = (float)(1.0 / (double)(dist * dist + 1e-8f));
+                        any = true;
+                    }
+                    else
+                        w[k] = 0;
+                }
+        if (!any)
+        {
+            float best = 1e10f, p1 = 0, p2 = 0;
+            for (int dx = -1; dx < 3; dx++)
+                for (int dy = -1; dy < 3; dy++)
+                    for (int dz = -1; dz < 3; dz++)
+                    {
+                        const i3 q = base + i3{dx, dy, dz};
+                        if (!air2(q))
+                            continue;
+                        const float dist = length(s.center_f(q) - xr);
+                        if (dist < best)
+                        {
+                            best = dist;
+                            p1 = g0[s.lin(q)];
+                            p2 = g1[s.lin(q)];
+                        }
+                    }
+            g0[c] = p1;
+            g1[c] = p2;
+            continue;
+        }
+        float sum = 0;
+        for (int k = 0; k < 8; k++)
+            sum += w[k];
+        for (int k = 0; k < 8; k++)
+            w[k] /= sum;
+        const float len = length(xf - xr);
+        for (int i = 0; i < 2; i++)
+        {
+            float pr = 0;
+            for (int k = 0; k < 8; k++)
+            {
+                // neighbours outside the grid carry weight 0 and are not read
+                const i3 q = nb[k];
+                const bool in = q.x >= 0 && q.y >= 0 && q.z >= 0 && q.x < N && q.y < N && q.z < N;
+                pr += w[k] * (in ? (*lv[i])[s.lin(q)] : 0.0f);
+            }
+            (*lv[i])[c] = pr - 1.225f * accs[i] * len;
+        }
+    }
+    return err ? -1 : (int)fresh.size();
+}
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -1943,6 +2315,71 @@ int orc_get_interp_weights(const orc_scene *s, float *out8)
 {
     std::memcpy(out8, s->interp.data(), s->interp.size() * sizeof(float));
     return 0;
+}
+
+int orc_classify(orc_scene *s, int thin_shell)
+{
+    if (!s->mesh_set)
+        return -1;
+    return classify(*s, thin_shell != 0);
+}
+
+int orc_get_cells(const orc_scene *s, int *type, int *ghost_idx, int *nearest, float *point3, float *reflect3,
+                  float *dist)
+{
+    if (!s->classified)
+        return -1;
+    const size_t NN = s->ctype.size();
+    if (type)
+        std::memcpy(type, s->ctype.data(), NN * sizeof(int));
+    if (ghost_idx)
+        std::memcpy(ghost_idx, s->cghost.data(), NN * sizeof(int));
+    if (nearest)
+        std::memcpy(nearest, s->cnearest.data(), NN * sizeof(int));
+    if (point3)
+        std::memcpy(point3, s->cpoint.data(), NN * sizeof(f3));
+    if (reflect3)
+        std::memcpy(reflect3, s->creflect.data(), NN * sizeof(f3));
+    if (dist)
+        std::memcpy(dist, s->cdist.data(), NN * sizeof(float));
+    return 0;
+}
+
+int orc_fill_fresh(orc_scene *s, const float *acc_now, const float *acc_old)
+{
+    if (!s->classified)
+        return -2;
+    return fill_fresh(*s, acc_now, acc_old);
+}
+
+int orc_set_solid_zero(orc_scene *s)
+{
+    // set_solid_cell_zero_kernel, ghost_cell.cu:547-555
+    if (!s->classified)
+        return -1;
+    std::vector<float> &g = s->G(0);
+    for (size_t c = 0; c < g.size(); c++)
+        if (s->ctype[c] == CELL_SOLID)
+            g[c] = 0;
+    return 0;
+}
+
+int orc_vertex_normals(const orc_scene *s, float *out3)
+{
+    if (!s->mesh_set)
+        return -1;
+    const std::vector<f3> vn = vertex_normals(*s);
+    std::memcpy(out3, vn.data(), vn.size() * sizeof(f3));
+    return 0;
+}
+
+void orc_closest_coeff(const float *p3, const float *tri9, float *out3)
+{
+    const f3 c = closest_coeff(f3{p3[0], p3[1], p3[2]}, f3{tri9[0], tri9[1], tri9[2]}, f3{tri9[3], tri9[4], tri9[5]},
+                               f3{tri9[6], tri9[7], tri9[8]});
+    out3[0] = c.x;
+    out3[1] = c.y;
+    out3[2] = c.z;
 }
 
 unsigned short orc_f2h(float f) { return f2h(f); }

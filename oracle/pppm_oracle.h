@@ -79,6 +79,21 @@ int orc_export_face_weights(const orc_scene *s, int *start, int *nbr, void *recs
 int orc_import_face_weights(orc_scene *s, const void *recs72);
 int orc_get_interp_weights(const orc_scene *s, float *out8);
 
+/* Cell classification and fresh cells of the ghost-cell solver (ghost_cell_util.cu:7-230,
+ * ghost_cell.cu:51-218, 547-560) on the scene's current mesh. Types: 0 AIR, 1 SOLID,
+ * 2 GHOST, 3 UNKNOWN (ghost_cell.h:11-17). orc_classify keeps the previous types for the
+ * fresh-cell test and returns the ghost-cell count. orc_fill_fresh fills the cells that
+ * turned AIR at levels t and t-1 from per-triangle accelerations (now, previous) and
+ * returns their number (-1: a fresh cell without a nearest triangle; 1e10 written). */
+int orc_classify(orc_scene *s, int thin_shell);
+int orc_get_cells(const orc_scene *s, int *type, int *ghost_idx, int *nearest, float *point3, float *reflect3,
+                  float *dist);
+int orc_fill_fresh(orc_scene *s, const float *acc_now, const float *acc_old);
+int orc_set_solid_zero(orc_scene *s);
+int orc_vertex_normals(const orc_scene *s, float *out3);
+/* barycentric weights of the point of triangle tri9 closest to p3 (helper_math.h:1603-1800) */
+void orc_closest_coeff(const float *p3, const float *tri9, float *out3);
+
 /* Primitives (for known-answer and closed-form tests) */
 unsigned short orc_f2h(float f);
 float orc_h2f(unsigned short h);

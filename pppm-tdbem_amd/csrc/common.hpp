@@ -50,6 +50,7 @@ struct f3
 __host__ __device__ inline f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 __host__ __device__ inline f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 __host__ __device__ inline f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__host__ __device__ inline f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
 __host__ __device__ inline f3 operator/(f3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
 __host__ __device__ inline float dotf(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __host__ __device__ inline f3 crossf(f3 a, f3 b)

@@ -125,6 +125,40 @@ hipError_t launch_rehash(const unsigned long long *okeys, const int *ovals, int 
                          int *nvals, int ncap, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long *a, long long n, unsigned long long v, hipStream_t s);
 
+// ---------------------------------------------------------------- cells.hip (classification)
+// cell types of the ghost-cell solver (ghost_cell.h:11-17)
+enum : int
+{
+    CELL_AIR = 0,
+    CELL_SOLID = 1,
+    CELL_GHOST = 2,
+    CELL_UNKNOWN = 3,
+};
+struct CellArgs
+{
+    int N;
+    float h;
+    float minx, miny, minz;
+    int thin_shell;
+};
+// vertex -> face CSR (cnt, fill: nv ints; start: nv+1; items: 3 nt)
+hipError_t launch_vertex_faces(const int *tris, int nt, int nv, int *cnt, int *start, int *fill, int *items,
+                               int *scan_tmp, hipStream_t s);
+hipError_t launch_vertex_normals(const float *verts, const int *tris, const TriGeom *geom, int nv, const int *start,
+                                 int *items, float *vnormal, hipStream_t s);
+// types, nearest triangle / point / mirror point / distance, ghost indices of all N^3 cells
+hipError_t launch_classify(const CellArgs &a, const float *verts, const int *tris, const int *cell_start,
+                           const int *cell_items, const float *vnormal, int *ctype, int *cnearest, float *cpoint,
+                           float *creflect, float *cdist, int *typ2, int *flag, int *scan, int *scan_tmp, int *cghost,
+                           hipStream_t s);
+hipError_t launch_fresh_flags(long long NN, int nt, const int *told, const int *tnew, const int *cnearest, int *flag,
+                              int *err, hipStream_t s);
+hipError_t launch_fresh_fill(const CellArgs &a, const StepState *st, const int *list, const int *n_ptr, int cap,
+                             const int *err, const int *told, const int *tnew, const int *cnearest,
+                             const float *cpoint, const float *acc_now, const float *acc_old, float *grid3,
+                             hipStream_t s);
+hipError_t launch_solid_zero(long long NN, const StepState *st, const int *ctype, float *grid3, hipStream_t s);
+
 // ---------------------------------------------------------------- step.hip
 struct StepArgs
 {

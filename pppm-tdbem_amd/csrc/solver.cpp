@@ -144,6 +144,15 @@ struct pppm_solver
 
     DBuf<float> hn, hd, far, nstage;
     DBuf<int> lcell, lc3;
+
+    // cell classification and fresh cells (GhostCellSolver cell_data / cell_data_old,
+    // ghost_cell.h:125-127), N^3 each
+    bool classified = false;
+    int ghost_count = 0;
+    DBuf<int> ctype, ctype_old, cghost, cnearest, ctyp2, cflag, cscan, fresh_list, fresh_err;
+    DBuf<float> cpoint, creflect, cdist, acc_stage;
+    DBuf<int> vf_cnt, vf_start, vf_fill, vf_items;
+    DBuf<float> vnormal;
     int st_h[ST_N] = {0};
     long long last_grid_recomputes = 0, last_face_recomputes = 0;
 
@@ -1282,6 +1291,127 @@ int pppm_get_b4(pppm_solver *s, int *cells, int *start, int *tris)
     if (tris)
         CK(hipMemcpyAsync(tris, s->b4_tris.p, (size_t)items * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     CK(hipStreamSynchronize(s->stream));
+    return PPPM_OK;
+}
+
+// ---------------------------------------------------------------- cell classification
+int pppm_classify(pppm_solver *s, int thin_shell, int *ghost_count)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    if (!s->mesh_set)
+        return fail(s, PPPM_ERR_STATE, "classify before set_mesh");
+    const int N = s->N, nt = s->nt, nv = s->nv;
+    const long long NN = (long long)N * N * N;
+    hipStream_t st = s->stream;
+    CK(s->ctype.ensure(NN));
+    CK(s->ctype_old.ensure(NN));
+    CK(s->cghost.ensure(NN));
+    CK(s->cnearest.ensure(NN));
+    CK(s->ctyp2.ensure(NN));
+    CK(s->cflag.ensure(NN));
+    CK(s->cscan.ensure(NN + 1));
+    CK(s->cpoint.ensure(3 * NN));
+    CK(s->creflect.ensure(3 * NN));
+    CK(s->cdist.ensure(NN));
+    CK(s->vf_cnt.ensure(nv));
+    CK(s->vf_fill.ensure(nv));
+    CK(s->vf_start.ensure(nv + 1));
+    CK(s->vf_items.ensure(3 * (size_t)nt));
+    CK(s->vnormal.ensure(3 * (size_t)nv));
+    CK(s->scan_tmp.ensure(scan_tmp_ints((int)std::max<long long>(NN, nv))));
+    // the previous labels are kept for the fresh-cell test (cell_data_old.assign(cell_data),
+    // ghost_cell.h:101); before the first classification every cell counts as AIR
+    if (s->classified)
+        CK(hipMemcpyAsync(s->ctype_old.p, s->ctype.p, NN * sizeof(int), hipMemcpyDeviceToDevice, st));
+    else
+        CK(launch_fill_int(s->ctype_old.p, NN, CELL_AIR, st));
+    CK(launch_vertex_faces(s->tris.p, nt, nv, s->vf_cnt.p, s->vf_start.p, s->vf_fill.p, s->vf_items.p,
+                           s->scan_tmp.p, st));
+    CK(launch_vertex_normals(s->verts.p, s->tris.p, s->geom.p, nv, s->vf_start.p, s->vf_items.p, s->vnormal.p, st));
+    const CellArgs a{N, s->h, s->minp[0], s->minp[1], s->minp[2], thin_shell ? 1 : 0};
+    CK(launch_classify(a, s->verts.p, s->tris.p, s->cell_start.p, s->cell_items.p, s->vnormal.p, s->ctype.p,
+                       s->cnearest.p, s->cpoint.p, s->creflect.p, s->cdist.p, s->ctyp2.p, s->cflag.p, s->cscan.p,
+                       s->scan_tmp.p, s->cghost.p, st));
+    int g = 0;
+    CK(hipMemcpyAsync(&g, s->cscan.p + NN, sizeof(int), hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    s->ghost_count = g;
+    s->classified = true;
+    if (ghost_count)
+        *ghost_count = g;
+    return PPPM_OK;
+}
+
+int pppm_fill_fresh(pppm_solver *s, const float *acc_now, const float *acc_old, int on_device, int *fresh_count)
+{
+    if (check_handle(s) || !acc_now || !acc_old)
+        return PPPM_ERR_ARG;
+    if (!s->classified)
+        return fail(s, PPPM_ERR_STATE, "fill_fresh before classify");
+    const int N = s->N, nt = s->nt;
+    const long long NN = (long long)N * N * N;
+    hipStream_t st = s->stream;
+    const float *an = acc_now, *ao = acc_old;
+    if (!on_device)
+    {
+        CK(s->acc_stage.ensure(2 * (size_t)nt));
+        RC(upload(s, s->acc_stage.p, acc_now, nt * sizeof(float), 0));
+        RC(upload(s, s->acc_stage.p + nt, acc_old, nt * sizeof(float), 0));
+        an = s->acc_stage.p;
+        ao = s->acc_stage.p + nt;
+    }
+    CK(s->fresh_list.ensure(NN));
+    CK(s->fresh_err.ensure(1));
+    CK(launch_fill_int(s->fresh_err.p, 1, 0, st));
+    CK(launch_fresh_flags(NN, nt, s->ctype_old.p, s->ctype.p, s->cnearest.p, s->cflag.p, s->fresh_err.p, st));
+    RC(scan(s, s->cflag.p, s->cscan.p, (int)NN));
+    CK(launch_scatter_selected(s->cflag.p, s->cscan.p, (int)NN, s->fresh_list.p, nullptr, st));
+    const CellArgs a{N, s->h, s->minp[0], s->minp[1], s->minp[2], 0};
+    CK(launch_fresh_fill(a, s->state.p, s->fresh_list.p, s->cscan.p + NN, (int)NN, s->fresh_err.p, s->ctype_old.p,
+                         s->ctype.p, s->cnearest.p, s->cpoint.p, an, ao, s->grid.p, st));
+    int cnt = 0, err = 0;
+    CK(hipMemcpyAsync(&cnt, s->cscan.p + NN, sizeof(int), hipMemcpyDeviceToHost, st));
+    CK(hipMemcpyAsync(&err, s->fresh_err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    if (fresh_count)
+        *fresh_count = err ? -1 : cnt;
+    return PPPM_OK;
+}
+
+int pppm_set_solid_zero(pppm_solver *s)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    if (!s->classified)
+        return fail(s, PPPM_ERR_STATE, "set_solid_zero before classify");
+    const long long NN = (long long)s->N * s->N * s->N;
+    CK(launch_solid_zero(NN, s->state.p, s->ctype.p, s->grid.p, s->stream));
+    return PPPM_OK;
+}
+
+int pppm_get_cells(pppm_solver *s, int *type, int *ghost_idx, int *nearest, float *point3, float *reflect3,
+                   float *dist)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    if (!s->classified)
+        return fail(s, PPPM_ERR_STATE, "get_cells before classify");
+    const size_t NN = (size_t)s->N * s->N * s->N;
+    hipStream_t st = s->stream;
+    if (type)
+        CK(hipMemcpyAsync(type, s->ctype.p, NN * sizeof(int), hipMemcpyDeviceToHost, st));
+    if (ghost_idx)
+        CK(hipMemcpyAsync(ghost_idx, s->cghost.p, NN * sizeof(int), hipMemcpyDeviceToHost, st));
+    if (nearest)
+        CK(hipMemcpyAsync(nearest, s->cnearest.p, NN * sizeof(int), hipMemcpyDeviceToHost, st));
+    if (point3)
+        CK(hipMemcpyAsync(point3, s->cpoint.p, 3 * NN * sizeof(float), hipMemcpyDeviceToHost, st));
+    if (reflect3)
+        CK(hipMemcpyAsync(reflect3, s->creflect.p, 3 * NN * sizeof(float), hipMemcpyDeviceToHost, st));
+    if (dist)
+        CK(hipMemcpyAsync(dist, s->cdist.p, NN * sizeof(float), hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
     return PPPM_OK;
 }
 

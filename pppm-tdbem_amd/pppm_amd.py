@@ -28,7 +28,8 @@ EXPORTS = [
     "pppm_time_idx", "pppm_grid_ptr", "pppm_read_cells", "pppm_history_ptrs", "pppm_triangles",
     "pppm_stats", "pppm_get_grid_weights", "pppm_set_grid_weights", "pppm_get_face_weights",
     "pppm_set_face_weights", "pppm_get_interp_weights", "pppm_get_c3", "pppm_get_b4",
-    "pppm_set_timing", "pppm_get_timing", "pppm_run_moving",
+    "pppm_set_timing", "pppm_get_timing", "pppm_run_moving", "pppm_classify", "pppm_fill_fresh",
+    "pppm_set_solid_zero", "pppm_get_cells",
 ]
 
 # LayerWeightHalf record (src/algorithm/bem.h:163-169): 16 x fp16, f32, 16 x fp16, f32
@@ -83,6 +84,10 @@ def load_library(path: str = LIB_PATH):
         "pppm_set_timing": ([vp, i], i),
         "pppm_get_timing": ([vp, vp, vp], i),
         "pppm_run_moving": ([vp, vp, ll, vp, ll, i, vp, i, vp, i], i),
+        "pppm_classify": ([vp, i, P(i)], i),
+        "pppm_fill_fresh": ([vp, vp, vp, i, P(i)], i),
+        "pppm_set_solid_zero": ([vp], i),
+        "pppm_get_cells": ([vp, vp, vp, vp, vp, vp, vp], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -350,6 +355,40 @@ class PPPMSolver:
         tris = np.zeros(max(int(start[-1]), 1), dtype=np.int32)
         self._check(self.lib.pppm_get_b4(self.h, None, None, tris.ctypes.data_as(ctypes.c_void_p)), "get_b4")
         return cells[:st["B4"]], start, tris[:int(start[-1])]
+
+    # ------------------------------------------------------------ classification / fresh cells
+    # GhostCellSolver::precompute_cell_data / fill_in_fresh_cell / set_solid_cell_zero
+    # (src/algorithm/ghost_cell.h:69-212)
+    AIR, SOLID, GHOST, UNKNOWN = 0, 1, 2, 3
+
+    def classify(self, thin_shell=False):
+        g = ctypes.c_int()
+        self._check(self.lib.pppm_classify(self.h, int(bool(thin_shell)), ctypes.byref(g)), "classify")
+        return g.value
+
+    def fill_fresh(self, acc_now, acc_old):
+        """Returns the number of fresh cells (-1: one had no nearest triangle)."""
+        if not hasattr(acc_now, "data_ptr"):
+            acc_now = np.ascontiguousarray(acc_now, dtype=np.float32)
+            acc_old = np.ascontiguousarray(acc_old, dtype=np.float32)
+        a, dev = _ptr(acc_now)
+        b, _ = _ptr(acc_old)
+        n = ctypes.c_int()
+        self._check(self.lib.pppm_fill_fresh(self.h, a, b, dev, ctypes.byref(n)), "fill_fresh")
+        return n.value
+
+    def set_solid_zero(self):
+        self._check(self.lib.pppm_set_solid_zero(self.h), "set_solid_zero")
+
+    def cells(self):
+        NN = self._res ** 3
+        out = {"type": np.zeros(NN, np.int32), "ghost_idx": np.zeros(NN, np.int32),
+               "nearest": np.zeros(NN, np.int32), "point": np.zeros((NN, 3), np.float32),
+               "reflect": np.zeros((NN, 3), np.float32), "dist": np.zeros(NN, np.float32)}
+        ptrs = [out[k].ctypes.data_as(ctypes.c_void_p) for k in ("type", "ghost_idx", "nearest", "point",
+                                                                  "reflect", "dist")]
+        self._check(self.lib.pppm_get_cells(self.h, *ptrs), "get_cells")
+        return out
 
 
 _hip = None
