@@ -97,6 +97,8 @@ int pppm_reset(pppm_solver *s);
 
 /* time_idx() (src/algorithm/pppm.h:71); t starts at -1 (src/algorithm/fdtd.h:55) */
 int pppm_time_idx(const pppm_solver *s, int *t);
+/* fdtd.t = t, as the demo does after a reset (experiments/demo/main.cu:255) */
+int pppm_set_time_idx(pppm_solver *s, int t);
 /* fdtd.grids[t + rel] device pointer (N^3 floats); get_grid() is rel = 0 (pppm.h:154) */
 int pppm_grid_ptr(pppm_solver *s, int rel, float **dev_ptr);
 /* copies grids[t] at n cells (xyz triples, host) to host out — the listener read

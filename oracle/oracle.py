@@ -46,6 +46,7 @@ def lib():
             "orc_solve_far_direct": ([vp], i),
             "orc_solve_near_direct": ([vp], i),
             "orc_time_idx": ([vp], i),
+            "orc_set_time_idx": ([vp, i], None),
             "orc_get_grid": ([vp, i, vp], i),
             "orc_set_grid": ([vp, i, vp], i),
             "orc_get_far": ([vp, vp], i),
@@ -155,6 +156,9 @@ class Oracle:
 
     def time_idx(self):
         return self.L.orc_time_idx(self.s)
+
+    def set_time_idx(self, t):
+        self.L.orc_set_time_idx(self.s, int(t))
 
     def grid(self, rel=0):
         out = np.empty(self.N ** 3, dtype=np.float32)

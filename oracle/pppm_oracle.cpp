@@ -2143,6 +2143,7 @@ int orc_solve_near_direct(orc_scene *s)
 }
 
 int orc_time_idx(const orc_scene *s) { return s->t; }
+void orc_set_time_idx(orc_scene *s, int t) { s->t = t; }
 
 int orc_get_grid(const orc_scene *s, int rel, float *out)
 {

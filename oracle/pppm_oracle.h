@@ -51,6 +51,7 @@ int orc_solve_far_direct(orc_scene *s);
 int orc_solve_near_direct(orc_scene *s);
 
 int orc_time_idx(const orc_scene *s);
+void orc_set_time_idx(orc_scene *s, int t); /* fdtd.t = t (experiments/demo/main.cu:255) */
 /* grids[t + rel] (CircularArray semantics, array.h:39) */
 int orc_get_grid(const orc_scene *s, int rel, float *out);
 int orc_set_grid(orc_scene *s, int rel, const float *in);

@@ -1049,6 +1049,17 @@ int pppm_time_idx(const pppm_solver *s, int *t)
     return PPPM_OK;
 }
 
+int pppm_set_time_idx(pppm_solver *s, int t)
+{
+    if (check_handle(s) || t < -1)
+        return PPPM_ERR_ARG;
+    StepState v{t, 0};
+    CK(hipMemcpyAsync(s->state.p, &v, sizeof(v), hipMemcpyHostToDevice, s->stream));
+    CK(hipStreamSynchronize(s->stream));
+    s->t_host = t;
+    return PPPM_OK;
+}
+
 int pppm_grid_ptr(pppm_solver *s, int rel, float **dev_ptr)
 {
     if (check_handle(s) || !dev_ptr)

@@ -29,7 +29,7 @@ EXPORTS = [
     "pppm_stats", "pppm_get_grid_weights", "pppm_set_grid_weights", "pppm_get_face_weights",
     "pppm_set_face_weights", "pppm_get_interp_weights", "pppm_get_c3", "pppm_get_b4",
     "pppm_set_timing", "pppm_get_timing", "pppm_run_moving", "pppm_classify", "pppm_fill_fresh",
-    "pppm_set_solid_zero", "pppm_get_cells",
+    "pppm_set_solid_zero", "pppm_get_cells", "pppm_set_time_idx",
 ]
 
 # LayerWeightHalf record (src/algorithm/bem.h:163-169): 16 x fp16, f32, 16 x fp16, f32
@@ -88,6 +88,7 @@ def load_library(path: str = LIB_PATH):
         "pppm_fill_fresh": ([vp, vp, vp, i, P(i)], i),
         "pppm_set_solid_zero": ([vp], i),
         "pppm_get_cells": ([vp, vp, vp, vp, vp, vp, vp], i),
+        "pppm_set_time_idx": ([vp, i], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -168,6 +169,9 @@ class PPPMSolver:
         t = ctypes.c_int()
         self._check(self.lib.pppm_time_idx(self.h, ctypes.byref(t)), "time_idx")
         return t.value
+
+    def set_time_idx(self, t):
+        self._check(self.lib.pppm_set_time_idx(self.h, int(t)), "set_time_idx")
 
     def set_mesh(self, verts, tris):
         if hasattr(verts, "data_ptr"):
