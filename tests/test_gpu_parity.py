@@ -222,12 +222,39 @@ def test_c2_moving_update_bitexact(amd, oracle_mod, scenes_mod):
     o.set_mesh(sc.motion(0), sc.tris)
     sg, so = [], []
     first_bad = None
+
+    def rec_diff(a, b):
+        if len(a) != len(b):
+            return -1
+        return int((a.view(np.uint8).reshape(len(a), -1) != b.view(np.uint8).reshape(len(b), -1)).any(1).sum())
+
+    first_weight_diff = None
     for i in range(steps):
         g.update_mesh(verts[i])
         o.update_mesh(sc.motion(i))
         st_g, st_o = g.stats(), o.stats()
         assert (st_g["C3"], st_g["P4"], st_g["grid_recomputes"], st_g["face_recomputes"]) == \
             (st_o["C3"], st_o["P4"], st_o["grid_recomputes"], st_o["face_recomputes"]), i
+        if first_weight_diff is None and i < 16:
+            fs_g, fn_g, fr_g = g.get_face_weights()
+            fs_o, fn_o, fr_o = o.face_weights()
+            gw = rec_diff(g.get_grid_weights(), o.grid_weights())
+            fw = rec_diff(fr_g, fr_o) if np.array_equal(fn_g, fn_o) else -2
+            iw = int(np.sum(g.get_interp_weights().view(np.uint32) != o.interp_weights().view(np.uint32)))
+            hg, dg = g.get_history()
+            ho, do = o.history()
+            rings = (int(np.sum(hg.view(np.uint32) != ho.view(np.uint32))),
+                     int(np.sum(dg.view(np.uint32) != do.view(np.uint32))))
+            print(f"update {i}: grid recomputes {st_g['grid_recomputes']} face recomputes {st_g['face_recomputes']}"
+                  f" | differing grid records {gw}, face records {fw}, interp {iw}, rings N/D {rings}", flush=True)
+            if gw or fw or iw or rings[0] or rings[1]:
+                first_weight_diff = i
+                bad = np.flatnonzero((fr_g.view(np.uint8).reshape(len(fr_g), -1) !=
+                                      fr_o.view(np.uint8).reshape(len(fr_o), -1)).any(1)) if fw > 0 else []
+                for r in bad[:4]:
+                    c = int(np.searchsorted(fs_g, r, side="right") - 1)
+                    print("  face record", r, "owner", c, "nbr", int(fn_g[r]), "gpu", fr_g[r], "oracle", fr_o[r],
+                          flush=True)
         g.update_grid_and_face(nm[i])
         o.step(sc.neumann[i])
         sg.append(g.read_cells([sc.listener])[0])
