@@ -78,6 +78,8 @@ def lib():
             "orc_set_solid_zero": ([vp], i),
             "orc_vertex_normals": ([vp, vp], i),
             "orc_closest_coeff": ([vp, vp, vp], None),
+            "orc_det_exp": ([d], d),
+            "orc_det_sincos": ([d, vp, vp], None),
         }
         for name, (a, r) in sig.items():
             fn = getattr(L, name)
@@ -349,3 +351,13 @@ def closest_coeff(p3, tri9):
     out = np.zeros(3, np.float32)
     lib().orc_closest_coeff(_p(p), _p(t), _p(out))
     return out
+
+
+def det_exp(x):
+    return float(lib().orc_det_exp(float(x)))
+
+
+def det_sincos(x):
+    s, c = ctypes.c_double(), ctypes.c_double()
+    lib().orc_det_sincos(float(x), ctypes.byref(s), ctypes.byref(c))
+    return s.value, c.value

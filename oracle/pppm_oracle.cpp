@@ -17,6 +17,8 @@
  */
 #include "pppm_oracle.h"
 
+#include "detmath.h"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -204,9 +206,12 @@ static inline float conv(const LWH &w, const float *nring, const float *dring, i
 // ------------------------------------------------------------------ quadrature, FP64 mode
 static inline void cexp_d(double re, double im, double &o_re, double &o_im)
 {
-    double e = std::exp(re);
-    o_re = e * std::cos(im);
-    o_im = e * std::sin(im);
+    // deterministic exp / sincos (detmath.h), the same operations as the device build
+    const double e = orc_detmath::det_exp(re);
+    double sn, cs;
+    orc_detmath::det_sincos(im, &sn, &cs);
+    o_re = e * cs;
+    o_im = e * sn;
 }
 
 // single_layer_potential / double_layer_potential, potential.h:22-47
@@ -2373,6 +2378,9 @@ int orc_vertex_normals(const orc_scene *s, float *out3)
     std::memcpy(out3, vn.data(), vn.size() * sizeof(f3));
     return 0;
 }
+
+double orc_det_exp(double x) { return orc_detmath::det_exp(x); }
+void orc_det_sincos(double x, double *s, double *c) { orc_detmath::det_sincos(x, s, c); }
 
 void orc_closest_coeff(const float *p3, const float *tri9, float *out3)
 {

@@ -96,6 +96,9 @@ int orc_vertex_normals(const orc_scene *s, float *out3);
 void orc_closest_coeff(const float *p3, const float *tri9, float *out3);
 
 /* Primitives (for known-answer and closed-form tests) */
+/* deterministic FP64 exp / sincos used by the FP64 quadrature (detmath.h) */
+double orc_det_exp(double x);
+void orc_det_sincos(double x, double *s, double *c);
 unsigned short orc_f2h(float f);
 float orc_h2f(unsigned short h);
 /* TDBEM::init (bem.h:264-277): lambda and the 16 complex wave numbers */

@@ -187,7 +187,9 @@ __global__ __launch_bounds__(B) void k_scatter_selected(const int *__restrict__ 
 }
 
 // one wave per segment: lane l = neighbour cell l of the width^3 block around the segment
-// cell (offset -1 .. width-2 per axis, x fastest); counts or fills in that order.
+// cell (offset -1 .. width-2 per axis, x fastest); counts or fills in that order. The grid
+// is bounded (the live count is only known on the device): waves stride over the live
+// segments, and the count variant also zeroes the counts of the capacity tail.
 template <int WIDTH, bool FILL>
 __global__ __launch_bounds__(B) void k_nbr_wave(const int *__restrict__ cells, const int *__restrict__ n_ptr, int cap,
                                                 const int *__restrict__ cell_start,
@@ -197,48 +199,47 @@ __global__ __launch_bounds__(B) void k_nbr_wave(const int *__restrict__ cells, c
 {
     constexpr int NB = WIDTH * WIDTH * WIDTH;
     static_assert(NB <= 64, "one lane per neighbour cell");
-    const int q = blockIdx.x * WPB + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (q >= cap)
-        return;
-    const int n = *n_ptr;
-    if (q >= n)
-    {
-        if (!FILL && lane == 0)
-            cnt[q] = 0;
-        return;
-    }
-    const int c = cells[q];
-    const int x = c / (N * N), y = c / N % N, z = c % N;
-    int s = 0, e = 0, dx = 0, dy = 0, dz = 0;
-    if (lane < NB)
-    {
-        dx = lane % WIDTH - 1;
-        dy = lane / WIDTH % WIDTH - 1;
-        dz = lane / (WIDTH * WIDTH) - 1;
-        const int nb = lin3(x + dx, y + dy, z + dz, N);
-        s = cell_start[nb];
-        e = cell_start[nb + 1];
-    }
-    const int len = e - s;
-    const int incl = wave_incl_scan(len);
+    const int n = min(*n_ptr, cap);
     if (!FILL)
+        for (long long q = n + (long long)blockIdx.x * B + threadIdx.x; q < cap; q += (long long)gridDim.x * B)
+            cnt[q] = 0;
+    const int stride = gridDim.x * WPB;
+    for (int q = blockIdx.x * WPB + (threadIdx.x >> 6); q < n; q += stride)
     {
-        if (lane == 63)
-            cnt[q] = incl;
-        return;
-    }
-    const int base = out_start[q] + incl - len;
-    if (lane == 63 && out_start[q] + incl > out_cap)
-        atomicOr(err_flag, ERR_CAPACITY);
-    if (out_start[q] + incl > out_cap)
-        return;
-    // 3x3x3 lists store tri*27 + weight offset (grid - tri_cell + 1, pppm_cache.h:36-40)
-    const int o = (1 - dx) + 3 * (1 - dy) + 9 * (1 - dz);
-    for (int k = 0; k < len; k++)
-    {
-        const int tri = cell_items[s + k];
-        out[base + k] = WIDTH == 3 ? tri * 27 + o : tri;
+        const int c = cells[q];
+        const int x = c / (N * N), y = c / N % N, z = c % N;
+        int s = 0, e = 0, dx = 0, dy = 0, dz = 0;
+        if (lane < NB)
+        {
+            dx = lane % WIDTH - 1;
+            dy = lane / WIDTH % WIDTH - 1;
+            dz = lane / (WIDTH * WIDTH) - 1;
+            const int nb = lin3(x + dx, y + dy, z + dz, N);
+            s = cell_start[nb];
+            e = cell_start[nb + 1];
+        }
+        const int len = e - s;
+        const int incl = wave_incl_scan(len);
+        if (!FILL)
+        {
+            if (lane == 63)
+                cnt[q] = incl;
+            continue;
+        }
+        const int base = out_start[q] + incl - len;
+        const bool fits = out_start[q] + incl <= out_cap;
+        if (lane == 63 && !fits)
+            atomicOr(err_flag, ERR_CAPACITY);
+        if (!fits)
+            continue;
+        // 3x3x3 lists store tri*27 + weight offset (grid - tri_cell + 1, pppm_cache.h:36-40)
+        const int o = (1 - dx) + 3 * (1 - dy) + 9 * (1 - dz);
+        for (int k = 0; k < len; k++)
+        {
+            const int tri = cell_items[s + k];
+            out[base + k] = WIDTH == 3 ? tri * 27 + o : tri;
+        }
     }
 }
 
@@ -336,6 +337,8 @@ __global__ void k_status(StatusSrc src, int *out)
 
 inline int nblk(long long n) { return n > 0 ? (int)((n + B - 1) / B) : 1; }
 inline int nblk_waves(long long waves) { return waves > 0 ? (int)((waves + WPB - 1) / WPB) : 1; }
+// list kernels whose live count is on the device: a bounded grid that strides
+inline int nblk_strided(long long waves) { return std::min(nblk_waves(waves), 1024); }
 }  // namespace
 
 hipError_t launch_fill_int(int *a, long long n, int value, hipStream_t s)
@@ -395,10 +398,10 @@ hipError_t launch_nbr_count(const int *cells, const int *n_ptr, int cap, const i
                             int *cnt, hipStream_t s)
 {
     if (width == 3)
-        k_nbr_wave<3, false><<<nblk_waves(cap), B, 0, s>>>(cells, n_ptr, cap, cell_start, nullptr, N, nullptr, 0,
+        k_nbr_wave<3, false><<<nblk_strided(cap), B, 0, s>>>(cells, n_ptr, cap, cell_start, nullptr, N, nullptr, 0,
                                                             nullptr, cnt, nullptr);
     else
-        k_nbr_wave<4, false><<<nblk_waves(cap), B, 0, s>>>(cells, n_ptr, cap, cell_start, nullptr, N, nullptr, 0,
+        k_nbr_wave<4, false><<<nblk_strided(cap), B, 0, s>>>(cells, n_ptr, cap, cell_start, nullptr, N, nullptr, 0,
                                                             nullptr, cnt, nullptr);
     return hipGetLastError();
 }
@@ -406,10 +409,10 @@ hipError_t launch_nbr_fill(const int *cells, const int *n_ptr, int cap, const in
                            int N, int width, const int *out_start, int out_cap, int *out, int *err_flag, hipStream_t s)
 {
     if (width == 3)
-        k_nbr_wave<3, true><<<nblk_waves(cap), B, 0, s>>>(cells, n_ptr, cap, cell_start, cell_items, N, out_start,
+        k_nbr_wave<3, true><<<nblk_strided(cap), B, 0, s>>>(cells, n_ptr, cap, cell_start, cell_items, N, out_start,
                                                            out_cap, out, nullptr, err_flag);
     else
-        k_nbr_wave<4, true><<<nblk_waves(cap), B, 0, s>>>(cells, n_ptr, cap, cell_start, cell_items, N, out_start,
+        k_nbr_wave<4, true><<<nblk_strided(cap), B, 0, s>>>(cells, n_ptr, cap, cell_start, cell_items, N, out_start,
                                                            out_cap, out, nullptr, err_flag);
     return hipGetLastError();
 }

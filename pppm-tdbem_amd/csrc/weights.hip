@@ -14,6 +14,7 @@
 // weight series is rounded to FP32 and quantised to FP16 with a max-abs FP32 scale,
 // exactly as the reference stores it. Storage: per record 16 x half2 {S_k, D_k}
 // (64 B, one half cache line) plus a float2 {scale_S, scale_D}.
+#include "detmath.hpp"
 #include "internal.hpp"
 
 #include <algorithm>
@@ -38,9 +39,9 @@ struct cdbl
 
 __device__ inline cdbl cexp_d(double re, double im)
 {
-    double e = exp(re);
+    const double e = det_exp(re);
     double sn, cs;
-    sincos(im, &sn, &cs);
+    det_sincos(im, &sn, &cs);
     return {e * cs, e * sn};
 }
 
