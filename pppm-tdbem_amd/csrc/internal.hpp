@@ -106,11 +106,12 @@ struct WeightArgs
 size_t grid_tmp_bytes(int max_dirty);
 size_t face_tmp_bytes(int max_dirty);
 hipError_t launch_grid_dirty(const TriGeom *geom, float *meta, int nt, float h, int first, int *flag, hipStream_t s);
-// records tri*27+o of the *n_ptr triangles in list (cap bounds *n_ptr)
+// records tri*27+o of the *n_ptr triangles in list (cap bounds *n_ptr). hint_dirty: host
+// estimate of *n_ptr that sizes the grid-stride launch (-1: cap); any value is correct.
 hipError_t launch_grid_weights(const float *verts, const int *tris, const TriGeom *geom, const int *list,
                                const int *n_ptr, int cap, const WeightArgs &a, const Tdbem &b, void *tmp,
                                uint32_t *wf, float2 *sf, uint32_t *wb, float2 *sb, float *meta, const int *err,
-                               hipStream_t s);
+                               long long hint_dirty, hipStream_t s);
 hipError_t launch_interp_weights(const float *verts, const int *tris, const TriGeom *geom, const WeightArgs &a,
                                  float *interp, const int *err, hipStream_t s);
 hipError_t launch_face_lookup(const TriGeom *geom, const int *tris, const int *owner, const int *face_nbr,
@@ -120,7 +121,7 @@ hipError_t launch_face_lookup(const TriGeom *geom, const int *tris, const int *o
 hipError_t launch_face_weights(const float *verts, const int *tris, const TriGeom *geom, const int *dirty_pairs,
                                const int *n_ptr, int cap, const int *owner, const int *face_nbr, const int *face_slot,
                                const float *interp, const WeightArgs &a, const Tdbem &b, void *tmp, uint32_t *fw,
-                               float2 *fsc, float2 *fmeta, const int *err, hipStream_t s);
+                               float2 *fsc, float2 *fmeta, const int *err, long long hint_dirty, hipStream_t s);
 hipError_t launch_rehash(const unsigned long long *okeys, const int *ovals, int ocap, unsigned long long *nkeys,
                          int *nvals, int ncap, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long *a, long long n, unsigned long long v, hipStream_t s);
@@ -168,18 +169,24 @@ struct StepArgs
     long long stride;   // Neumann row stride (floats)
     int n_listen;
 };
+// history windows of the gathers: wn/wd[tri*16 + k] = ring value at t - back - k, with t the
+// device time index when the kernel runs (back = 0 before the advance of a step, 1 after)
+hipError_t launch_windows(int nt, int back, const StepState *st, const float *hn, const float *hd, float *wn,
+                          float *wd, hipStream_t s);
 // wave maps: map[w] = first segment owned by gather wave w (launch_wave_map)
 hipError_t launch_far(const StepArgs &a, const StepState *st, const float *grid3, const int *c3_cells,
                       const int *c3_start, const int *c3_wave, int c3_waves, const int *c3_pairs, const uint32_t *wf,
-                      const float2 *sf, float *hn, const float *hd, float *far, const float *neumann, hipStream_t s);
+                      const float2 *sf, float *hn, const float *hd, const float *wn, const float *wd, float *far,
+                      const float *neumann, hipStream_t s);
 hipError_t launch_dirichlet(const StepArgs &a, const StepState *st, const int *face_start, const int *face_wave,
                             int face_waves, const int *face_nbr, const int *face_slot, const uint32_t *fw,
-                            const float2 *fsc, const float *hn, float *hd, const float *interp, const int *corner,
-                            const float *far, const TriGeom *geom, const int *err, hipStream_t s);
+                            const float2 *fsc, const float *hn, float *hd, const float *wn, const float *wd,
+                            const float *interp, const int *corner, const float *far, const TriGeom *geom,
+                            const int *err, hipStream_t s);
 hipError_t launch_near(const StepArgs &a, const StepState *st, float *grid3, const int *c3_cells,
                        const int *c3_start, const int *c3_wave, int c3_waves, const int *c3_pairs, const uint32_t *wb,
-                       const float2 *sb, const float *hn, const float *hd, const float *far, const int *listen_cell,
-                       const int *listen_c3, float *out, hipStream_t s);
+                       const float2 *sb, const float *hn, const float *hd, const float *wn, const float *wd,
+                       const float *far, const int *listen_cell, const int *listen_c3, float *out, hipStream_t s);
 hipError_t launch_listen_direct(const StepArgs &a, const StepState *st, const float *grid3, const int *listen_cell,
                                 float *out, hipStream_t s);
 hipError_t launch_set_neumann(const StepArgs &a, const StepState *st, float *hn, const float *neumann,

@@ -5,7 +5,7 @@
 // Reduce-then-scan in two launches and no inter-block waiting: the first kernel writes one
 // sum per 4096-item tile, the second has every tile add up the sums of the tiles before it
 // (at most a few hundred ints, one wave), scan its own items and, in the last tile, write
-// the total to out[n]. Arrays up to one tile take a single launch. Integer sums are exact,
+// the total to out[n]. Arrays up to 8192 items take a single launch. Integer sums are exact,
 // so the result does not depend on the order of additions. Only kernels are enqueued, so
 // the scan can be captured in a HIP graph.
 #include "internal.hpp"
@@ -16,8 +16,9 @@ namespace pppm_amd
 namespace
 {
 constexpr int SB = 256;        // threads per block
-constexpr int SI = 16;         // items per thread
+constexpr int SI = 16;         // items per thread (multi-tile scans)
 constexpr int SPB = SB * SI;   // items per tile
+constexpr int SI1 = 32;        // items per thread when one tile holds everything
 
 __device__ inline int wave_incl(int v)
 {
@@ -40,12 +41,13 @@ __device__ inline int wave_total(int v)
     return v;
 }
 
-__device__ inline void load_tile(const int *__restrict__ in, long long n, int v[SI], int &tsum)
+template <int IT>
+__device__ inline void load_tile(const int *__restrict__ in, long long n, int v[IT], int &tsum)
 {
-    const long long base = (long long)blockIdx.x * SPB + (long long)threadIdx.x * SI;
+    const long long base = (long long)blockIdx.x * SB * IT + (long long)threadIdx.x * IT;
     tsum = 0;
 #pragma unroll
-    for (int i = 0; i < SI; i++)
+    for (int i = 0; i < IT; i++)
     {
         const long long q = base + i;
         v[i] = q < n ? in[q] : 0;
@@ -77,13 +79,14 @@ __device__ inline int block_scan(int tsum, int &btotal)
 __global__ __launch_bounds__(SB) void k_tile_sums(const int *__restrict__ in, long long n, int *__restrict__ sums)
 {
     int v[SI], tsum, btotal;
-    load_tile(in, n, v, tsum);
+    load_tile<SI>(in, n, v, tsum);
     (void)block_scan(tsum, btotal);
     if (threadIdx.x == 0)
         sums[blockIdx.x] = btotal;
 }
 
-// sums == null: one tile, no offset
+// sums == null: one tile, no offset. IT items per thread.
+template <int IT>
 __global__ __launch_bounds__(SB) void k_tile_scan(const int *__restrict__ in, int *__restrict__ out, long long n,
                                                   const int *__restrict__ sums)
 {
@@ -101,14 +104,14 @@ __global__ __launch_bounds__(SB) void k_tile_scan(const int *__restrict__ in, in
         if (threadIdx.x == 0)
             s_off = acc;
     }
-    int v[SI], tsum, btotal;
-    load_tile(in, n, v, tsum);
+    int v[IT], tsum, btotal;
+    load_tile<IT>(in, n, v, tsum);
     int run = block_scan(tsum, btotal);  // its barrier also publishes s_off
     const int off = s_off;
     run += off;
-    const long long base = (long long)blockIdx.x * SPB + (long long)threadIdx.x * SI;
+    const long long base = (long long)blockIdx.x * SB * IT + (long long)threadIdx.x * IT;
 #pragma unroll
-    for (int i = 0; i < SI; i++)
+    for (int i = 0; i < IT; i++)
     {
         const long long q = base + i;
         if (q < n)
@@ -132,14 +135,14 @@ hipError_t scan_exclusive(const int *in, int *out, int n, int *tmp, hipStream_t 
 {
     if (n < 0)
         return hipErrorInvalidValue;
-    const long long nb = ((long long)n + SPB - 1) / SPB;
-    if (nb <= 1)
+    if (n <= SB * SI1)
     {
-        k_tile_scan<<<1, SB, 0, s>>>(in, out, n, nullptr);
+        k_tile_scan<SI1><<<1, SB, 0, s>>>(in, out, n, nullptr);
         return hipGetLastError();
     }
+    const long long nb = ((long long)n + SPB - 1) / SPB;
     k_tile_sums<<<(unsigned)nb, SB, 0, s>>>(in, n, tmp);
-    k_tile_scan<<<(unsigned)nb, SB, 0, s>>>(in, out, n, tmp);
+    k_tile_scan<SI><<<(unsigned)nb, SB, 0, s>>>(in, out, n, tmp);
     return hipGetLastError();
 }
 

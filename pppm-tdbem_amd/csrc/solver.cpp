@@ -143,6 +143,7 @@ struct pppm_solver
     DBuf<int> hvals, pool_counter;
 
     DBuf<float> hn, hd, far, nstage;
+    DBuf<float> wn, wd;  // per-step history windows, 16 floats per triangle and field
     DBuf<int> lcell, lc3;
 
     // cell classification and fresh cells (GhostCellSolver cell_data / cell_data_old,
@@ -456,8 +457,12 @@ int enqueue_rebuild(pppm_solver *s, bool first, const float *verts_series, long 
     CK(launch_grid_dirty(s->geom.p, s->gmeta.p, nt, s->h, first ? 1 : 0, s->gflag.p, st));
     RC(scan(s, s->gflag.p, s->gpos.p, nt));
     CK(launch_scatter_selected(s->gflag.p, s->gpos.p, nt, s->glist.p, nullptr, st));
+    // launch sizes follow the previous update's recompute counts (the kernels stride, so a
+    // larger count is still covered); a first build sizes for everything
+    const long long ghint = first ? -1 : 2 * s->last_grid_recomputes + 64;
+    const long long fhint = first ? -1 : 2 * s->last_face_recomputes + 64;
     CK(launch_grid_weights(s->verts.p, s->tris.p, s->geom.p, s->glist.p, s->gpos.p + nt, nt, wa, s->bem, s->gtmp.p,
-                           s->wf.p, s->sf.p, s->wb.p, s->sb.p, s->gmeta.p, err, st));
+                           s->wf.p, s->sf.p, s->wb.p, s->sb.p, s->gmeta.p, err, ghint, st));
 
     // interpolation weights, every update (update_interpolation_weight_kernel)
     CK(launch_interp_weights(s->verts.p, s->tris.p, s->geom.p, wa, s->interp.p, err, st));
@@ -471,7 +476,7 @@ int enqueue_rebuild(pppm_solver *s, bool first, const float *verts_series, long 
     CK(launch_scatter_selected(s->face_flag.p, s->face_pos.p, s->cap_p4, s->dirty_list.p, nullptr, st));
     CK(launch_face_weights(s->verts.p, s->tris.p, s->geom.p, s->dirty_list.p, s->face_pos.p + s->cap_p4, s->cap_p4,
                            s->face_owner.p, s->face_nbr.p, s->face_slot.p, s->interp.p, wa, s->bem, s->ftmp.p, s->fw.p,
-                           s->fsc.p, s->fmeta.p, err, st));
+                           s->fsc.p, s->fmeta.p, err, fhint, st));
 
     // status block
     StatusSrc src{};
@@ -575,6 +580,8 @@ int enqueue_step(pppm_solver *s, const float *neumann, int t0, long long stride,
     int sp = span_begin(s, 0);
     CK(launch_fdtd(s->grid.p, s->pmlbuf.p, fp, s->state.p, s->stream));
     span_end(s, sp);
+    if (s->mesh_set)
+        CK(launch_windows(s->nt, 0, s->state.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->stream));
     CK(launch_advance(s->state.p, s->stream));
     const StepArgs a = step_args(s, t0, stride, n_listen);
     if (!s->mesh_set)
@@ -585,16 +592,17 @@ int enqueue_step(pppm_solver *s, const float *neumann, int t0, long long stride,
     }
     sp = span_begin(s, 1);
     CK(launch_far(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_wave.p, s->c3_waves, s->c3_pairs.p,
-                  s->wf.p, s->sf.p, s->hn.p, s->hd.p, s->far.p, neumann, s->stream));
+                  s->wf.p, s->sf.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->far.p, neumann, s->stream));
     span_end(s, sp);
     sp = span_begin(s, 2);
     CK(launch_dirichlet(a, s->state.p, s->face_start.p, s->face_wave.p, s->face_waves, s->face_nbr.p, s->face_slot.p,
-                        s->fw.p, s->fsc.p, s->hn.p, s->hd.p, s->interp.p, s->corner.p, s->far.p, s->geom.p,
-                        s->errflag.p, s->stream));
+                        s->fw.p, s->fsc.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->interp.p, s->corner.p, s->far.p,
+                        s->geom.p, s->errflag.p, s->stream));
     span_end(s, sp);
     sp = span_begin(s, 3);
     CK(launch_near(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_wave.p, s->c3_waves, s->c3_pairs.p,
-                   s->wb.p, s->sb.p, s->hn.p, s->hd.p, s->far.p, s->lcell.p, s->lc3.p, out, s->stream));
+                   s->wb.p, s->sb.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->far.p, s->lcell.p, s->lc3.p, out,
+                   s->stream));
     span_end(s, sp);
     if (s->spans.size() > 4096)
         return collect_spans(s);
@@ -872,6 +880,8 @@ int pppm_set_mesh(pppm_solver *s, const float *verts, int nv, const int *tris, i
     // dirichlet / neumann histories reset (pppm.h:88-91)
     CK(s->hn.ensure((size_t)nt * RING));
     CK(s->hd.ensure((size_t)nt * RING));
+    CK(s->wn.ensure((size_t)nt * STEPS));
+    CK(s->wd.ensure((size_t)nt * STEPS));
     CK(s->nstage.ensure(nt));
     CK(hipMemsetAsync(s->hn.p, 0, (size_t)nt * RING * sizeof(float), s->stream));
     CK(hipMemsetAsync(s->hd.p, 0, (size_t)nt * RING * sizeof(float), s->stream));
@@ -974,8 +984,9 @@ int pppm_solve_far(pppm_solver *s)
     if (!s->mesh_set)
         return fail(s, PPPM_ERR_STATE, "solve_far before set_mesh");
     const StepArgs a = step_args(s, s->t_host, 0, 0);
+    CK(launch_windows(s->nt, 1, s->state.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->stream));
     CK(launch_far(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_wave.p, s->c3_waves, s->c3_pairs.p,
-                  s->wf.p, s->sf.p, s->hn.p, s->hd.p, s->far.p, nullptr, s->stream));
+                  s->wf.p, s->sf.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->far.p, nullptr, s->stream));
     return PPPM_OK;
 }
 
@@ -1003,9 +1014,10 @@ int pppm_update_dirichlet(pppm_solver *s)
     if (!s->mesh_set)
         return fail(s, PPPM_ERR_STATE, "update_dirichlet before set_mesh");
     const StepArgs a = step_args(s, s->t_host, 0, 0);
+    CK(launch_windows(s->nt, 1, s->state.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->stream));
     CK(launch_dirichlet(a, s->state.p, s->face_start.p, s->face_wave.p, s->face_waves, s->face_nbr.p, s->face_slot.p,
-                        s->fw.p, s->fsc.p, s->hn.p, s->hd.p, s->interp.p, s->corner.p, s->far.p, s->geom.p,
-                        s->errflag.p, s->stream));
+                        s->fw.p, s->fsc.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->interp.p, s->corner.p, s->far.p,
+                        s->geom.p, s->errflag.p, s->stream));
     return PPPM_OK;
 }
 
@@ -1016,8 +1028,10 @@ int pppm_solve_near(pppm_solver *s)
     if (!s->mesh_set)
         return fail(s, PPPM_ERR_STATE, "solve_near before set_mesh");
     const StepArgs a = step_args(s, s->t_host, 0, 0);
+    CK(launch_windows(s->nt, 1, s->state.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->stream));
     CK(launch_near(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_wave.p, s->c3_waves, s->c3_pairs.p,
-                   s->wb.p, s->sb.p, s->hn.p, s->hd.p, s->far.p, nullptr, nullptr, nullptr, s->stream));
+                   s->wb.p, s->sb.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->far.p, nullptr, nullptr, nullptr,
+                   s->stream));
     return PPPM_OK;
 }
 

@@ -17,7 +17,9 @@
 // segment boundaries.
 //
 // Weights: 64 B record per pair (16 x half2 {S_k, D_k}) + float2 scales. History: the
-// reference's 32-slot rings (slot = t mod 32), one 128 B line per triangle and field.
+// reference's 32-slot rings (slot = t mod 32), one 128 B line per triangle and field, plus
+// per-step windows of the 16 latest values in time order (one 64 B row per triangle and
+// field) from which the gathers read taps 1..15 as four 16 B loads.
 //
 // Time index: all kernels read t from device memory (StepState), so one captured HIP
 // graph replays any step.
@@ -39,32 +41,59 @@ __device__ inline void wave_sync()
 }
 
 // LayerWeightHalf::convolution (bem.h:203-213) for one pair, taps K0..15 in order.
-// DZERO0: the Dirichlet value of step t is read as 0 (pppm.cu:143).
+// Tap 0 reads the rings at slot t; taps 1..15 read the pair triangle's history windows
+// (values at t-1 .. t-15, contiguous, built once per step by k_windows) with four 16 B loads
+// per field. DZERO0: the Dirichlet value of step t is read as 0 (pppm.cu:143).
 template <int K0, bool DZERO0>
 __device__ inline float conv_pair(const uint32_t *__restrict__ w16, float2 sc, const float *__restrict__ nring,
-                                  const float *__restrict__ dring, int t, uint32_t *tap0 = nullptr)
+                                  const float *__restrict__ dring, const float *__restrict__ wn,
+                                  const float *__restrict__ wd, int t, uint32_t *tap0 = nullptr)
 {
     uint4 q[4];
 #pragma unroll
     for (int i = 0; i < 4; i++)
         q[i] = reinterpret_cast<const uint4 *>(w16)[i];
+    float4 hn4[4], hd4[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+    {
+        hn4[i] = reinterpret_cast<const float4 *>(wn)[i];
+        hd4[i] = reinterpret_cast<const float4 *>(wd)[i];
+    }
     const uint32_t *w = reinterpret_cast<const uint32_t *>(q);
+    const float *hnw = reinterpret_cast<const float *>(hn4);
+    const float *hdw = reinterpret_cast<const float *>(hd4);
     if (tap0)
         *tap0 = w[0];
     float r = 0.0f;
 #pragma unroll
     for (int k = K0; k < STEPS; k++)
     {
-        const int slot = (t - k) & (RING - 1);
         const float ws = f16_bits_to_f32(w[k]);
-        const float wd = f16_bits_to_f32(w[k] >> 16);
-        const float n = nring[slot];
-        const float d = (DZERO0 && k == 0) ? 0.0f : dring[slot];
+        const float wdk = f16_bits_to_f32(w[k] >> 16);
+        const float n = k == 0 ? nring[t & (RING - 1)] : hnw[k - 1];
+        const float d = k == 0 ? (DZERO0 ? 0.0f : dring[t & (RING - 1)]) : hdw[k - 1];
         const float a = -ws * sc.x * n;
-        const float b = wd * sc.y * d;
+        const float b = wdk * sc.y * d;
         r += a + b;
     }
     return r;
+}
+
+// History windows for the gathers of step t: row k of triangle i holds the ring value at
+// t - 1 - k, k = 0..15, so a gather reads tap k >= 1 at row k - 1. back = 0 when launched
+// before t is advanced (the time step), 1 after it (the phase entry points).
+__global__ __launch_bounds__(256) void k_windows(int nt, int back, const StepState *__restrict__ st,
+                                                 const float *__restrict__ hn, const float *__restrict__ hd,
+                                                 float *__restrict__ wn, float *__restrict__ wd)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nt * STEPS)
+        return;
+    const int tri = i / STEPS, k = i % STEPS;
+    const int slot = (st->t - back - k) & (RING - 1);
+    wn[i] = hn[tri * RING + slot];
+    wd[i] = hd[tri * RING + slot];
 }
 
 // far(c) = p(c) - sum_pairs W^fdtd (*) (N, D). W^fdtd has tap 0 == 0 (pppm_cache.cu:111),
@@ -76,6 +105,7 @@ __global__ __launch_bounds__(TB) void k_far(StepArgs a, const StepState *__restr
                                             int gather_waves, const int *__restrict__ c3_pairs,
                                             const uint32_t *__restrict__ wf, const float2 *__restrict__ sf,
                                             float *__restrict__ hn, const float *__restrict__ hd,
+                                            const float *__restrict__ wn, const float *__restrict__ wd,
                                             float *__restrict__ far, const float *__restrict__ neumann)
 {
     __shared__ float vals[WPB][WAVE];
@@ -100,7 +130,8 @@ __global__ __launch_bounds__(TB) void k_far(StepArgs a, const StepState *__restr
             {
                 const int rec = c3_pairs[p];
                 const int tri = rec / GC;
-                v = conv_pair<1, false>(wf + (size_t)rec * STEPS, sf[rec], hn + tri * RING, hd + tri * RING, t);
+                v = conv_pair<1, false>(wf + (size_t)rec * STEPS, sf[rec], hn + tri * RING, hd + tri * RING,
+                                       wn + tri * STEPS, wd + tri * STEPS, t);
             }
             vals[wid][lane] = v;
             wave_sync();
@@ -139,6 +170,7 @@ __global__ __launch_bounds__(TB) void k_dirichlet(StepArgs a, const StepState *_
                                                   const int *__restrict__ face_slot,
                                                   const uint32_t *__restrict__ fw, const float2 *__restrict__ fsc,
                                                   const float *__restrict__ hn, float *__restrict__ hd,
+                                                  const float *__restrict__ wn, const float *__restrict__ wd,
                                                   const float *__restrict__ interp, const int *__restrict__ corner,
                                                   const float *__restrict__ far, const TriGeom *__restrict__ geom,
                                                   const int *__restrict__ err)
@@ -169,7 +201,8 @@ __global__ __launch_bounds__(TB) void k_dirichlet(StepArgs a, const StepState *_
             const int ws = face_slot[p];
             const float2 sc = fsc[ws];
             uint32_t w0;
-            v = conv_pair<0, true>(fw + (size_t)ws * STEPS, sc, hn + j * RING, hd + j * RING, t, &w0);
+            v = conv_pair<0, true>(fw + (size_t)ws * STEPS, sc, hn + j * RING, hd + j * RING, wn + j * STEPS,
+                                  wd + j * STEPS, t, &w0);
             f = f16_bits_to_f32(w0 >> 16) * sc.y;  // face_factor = (float)D_0 * scale_D
         }
         vals[wid][lane] = v;
@@ -207,7 +240,8 @@ __global__ __launch_bounds__(TB) void k_near(StepArgs a, const StepState *__rest
                                              const int *__restrict__ c3_wave, int gather_waves,
                                              const int *__restrict__ c3_pairs, const uint32_t *__restrict__ wb,
                                              const float2 *__restrict__ sb, const float *__restrict__ hn,
-                                             const float *__restrict__ hd, const float *__restrict__ far,
+                                             const float *__restrict__ hd, const float *__restrict__ wn,
+                                             const float *__restrict__ wd, const float *__restrict__ far,
                                              const int *__restrict__ listen_cell, const int *__restrict__ listen_c3,
                                              float *__restrict__ out)
 {
@@ -235,7 +269,8 @@ __global__ __launch_bounds__(TB) void k_near(StepArgs a, const StepState *__rest
             {
                 const int rec = c3_pairs[p];
                 const int tri = rec / GC;
-                v = conv_pair<0, false>(wb + (size_t)rec * STEPS, sb[rec], hn + tri * RING, hd + tri * RING, t);
+                v = conv_pair<0, false>(wb + (size_t)rec * STEPS, sb[rec], hn + tri * RING, hd + tri * RING,
+                                       wn + tri * STEPS, wd + tri * STEPS, t);
             }
             vals[wid][lane] = v;
             wave_sync();
@@ -291,35 +326,45 @@ __global__ void k_set_neumann(StepArgs a, const StepState *__restrict__ st, floa
 inline int blocks_for_waves(long long waves) { return waves > 0 ? (int)((waves + WPB - 1) / WPB) : 1; }
 }  // namespace
 
+hipError_t launch_windows(int nt, int back, const StepState *st, const float *hn, const float *hd, float *wn,
+                          float *wd, hipStream_t s)
+{
+    const long long n = (long long)nt * STEPS;
+    k_windows<<<n > 0 ? (int)((n + 255) / 256) : 1, 256, 0, s>>>(nt, back, st, hn, hd, wn, wd);
+    return hipGetLastError();
+}
+
 hipError_t launch_far(const StepArgs &a, const StepState *st, const float *grid3, const int *c3_cells,
                       const int *c3_start, const int *c3_wave, int c3_waves, const int *c3_pairs, const uint32_t *wf,
-                      const float2 *sf, float *hn, const float *hd, float *far, const float *neumann, hipStream_t s)
+                      const float2 *sf, float *hn, const float *hd, const float *wn, const float *wd, float *far,
+                      const float *neumann, hipStream_t s)
 {
     const long long waves = (long long)c3_waves + (neumann ? (a.nt + WAVE - 1) / WAVE : 0);
     k_far<<<blocks_for_waves(waves), TB, 0, s>>>(a, st, grid3, c3_cells, c3_start, c3_wave, c3_waves, c3_pairs, wf,
-                                                  sf, hn, hd, far, neumann);
+                                                  sf, hn, hd, wn, wd, far, neumann);
     return hipGetLastError();
 }
 
 hipError_t launch_dirichlet(const StepArgs &a, const StepState *st, const int *face_start, const int *face_wave,
                             int face_waves, const int *face_nbr, const int *face_slot, const uint32_t *fw,
-                            const float2 *fsc, const float *hn, float *hd, const float *interp, const int *corner,
-                            const float *far, const TriGeom *geom, const int *err, hipStream_t s)
+                            const float2 *fsc, const float *hn, float *hd, const float *wn, const float *wd,
+                            const float *interp, const int *corner, const float *far, const TriGeom *geom,
+                            const int *err, hipStream_t s)
 {
     k_dirichlet<<<blocks_for_waves(face_waves), TB, 0, s>>>(a, st, face_start, face_wave, face_waves, face_nbr,
-                                                             face_slot, fw, fsc, hn, hd, interp, corner, far, geom,
-                                                             err);
+                                                             face_slot, fw, fsc, hn, hd, wn, wd, interp, corner, far,
+                                                             geom, err);
     return hipGetLastError();
 }
 
 hipError_t launch_near(const StepArgs &a, const StepState *st, float *grid3, const int *c3_cells,
                        const int *c3_start, const int *c3_wave, int c3_waves, const int *c3_pairs, const uint32_t *wb,
-                       const float2 *sb, const float *hn, const float *hd, const float *far, const int *listen_cell,
-                       const int *listen_c3, float *out, hipStream_t s)
+                       const float2 *sb, const float *hn, const float *hd, const float *wn, const float *wd,
+                       const float *far, const int *listen_cell, const int *listen_c3, float *out, hipStream_t s)
 {
     const long long waves = (long long)c3_waves + (out ? 1 : 0);
     k_near<<<blocks_for_waves(waves), TB, 0, s>>>(a, st, grid3, c3_cells, c3_start, c3_wave, c3_waves, c3_pairs, wb,
-                                                   sb, hn, hd, far, listen_cell, listen_c3, out);
+                                                   sb, hn, hd, wn, wd, far, listen_cell, listen_c3, out);
     return hipGetLastError();
 }
 

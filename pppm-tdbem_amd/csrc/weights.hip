@@ -668,6 +668,12 @@ __global__ __launch_bounds__(256) void k_fill_u64(unsigned long long *a, long lo
 inline int nb(long long n, int b) { return n > 0 ? (int)((n + b - 1) / b) : 1; }
 // grid-stride launches for work whose size lives in device memory: cap bounds the items
 inline int nb_capped(long long cap_items, int b) { return (int)std::min<long long>(nb(cap_items, b), 8192); }
+// grid-stride launch sized from a host estimate of the live count (the previous update's),
+// at least 64 blocks, at most the capacity-based size: any live count is still covered
+inline int nb_hint(long long hint_items, long long cap_items, int b)
+{
+    return std::max(std::min(nb(hint_items, b), nb_capped(cap_items, b)), std::min(64, nb_capped(cap_items, b)));
+}
 }  // namespace
 
 void tdbem_init(Tdbem &b, float dt, float c, float grid_size)
@@ -705,11 +711,12 @@ hipError_t launch_grid_dirty(const TriGeom *geom, float *meta, int nt, float h, 
 hipError_t launch_grid_weights(const float *verts, const int *tris, const TriGeom *geom, const int *list,
                                const int *n_ptr, int cap, const WeightArgs &a, const Tdbem &b, void *tmp,
                                uint32_t *wf, float2 *sf, uint32_t *wb, float2 *sb, float *meta, const int *err,
-                               hipStream_t s)
+                               long long hint_dirty, hipStream_t s)
 {
-    k_grid_integrals<<<nb_capped((long long)cap * GC * 9, B), B, 0, s>>>(verts, tris, geom, list, n_ptr, a, b,
+    const long long hint = hint_dirty < 0 ? cap : std::min<long long>(hint_dirty, cap);
+    k_grid_integrals<<<nb_hint(hint * GC * 9, (long long)cap * GC * 9, B), B, 0, s>>>(verts, tris, geom, list, n_ptr, a, b,
                                                                           (cdbl *)tmp, err);
-    k_grid_finish<<<nb_capped((long long)cap * GC, B), B, 0, s>>>(geom, list, n_ptr, b, (const cdbl *)tmp, wf, sf, wb,
+    k_grid_finish<<<nb_hint(hint * GC, (long long)cap * GC, B), B, 0, s>>>(geom, list, n_ptr, b, (const cdbl *)tmp, wf, sf, wb,
                                                                     sb, meta, err);
     return hipGetLastError();
 }
@@ -731,11 +738,12 @@ hipError_t launch_face_lookup(const TriGeom *geom, const int *tris, const int *o
 hipError_t launch_face_weights(const float *verts, const int *tris, const TriGeom *geom, const int *dirty_pairs,
                                const int *n_ptr, int cap, const int *owner, const int *face_nbr, const int *face_slot,
                                const float *interp, const WeightArgs &a, const Tdbem &b, void *tmp, uint32_t *fw,
-                               float2 *fsc, float2 *fmeta, const int *err, hipStream_t s)
+                               float2 *fsc, float2 *fmeta, const int *err, long long hint_dirty, hipStream_t s)
 {
-    k_face_integrals<<<nb_capped((long long)cap * 18, B), B, 0, s>>>(verts, tris, geom, dirty_pairs, n_ptr, owner,
+    const long long hint = hint_dirty < 0 ? cap : std::min<long long>(hint_dirty, cap);
+    k_face_integrals<<<nb_hint(hint * 18, (long long)cap * 18, B), B, 0, s>>>(verts, tris, geom, dirty_pairs, n_ptr, owner,
                                                                       face_nbr, interp, a, b, (cdbl *)tmp, err);
-    k_face_finish<<<nb_capped(cap, B), B, 0, s>>>(dirty_pairs, n_ptr, face_slot, geom, owner, face_nbr, b,
+    k_face_finish<<<nb_hint(hint, cap, B), B, 0, s>>>(dirty_pairs, n_ptr, face_slot, geom, owner, face_nbr, b,
                                                   (const cdbl *)tmp, fw, fsc, fmeta, err);
     return hipGetLastError();
 }
