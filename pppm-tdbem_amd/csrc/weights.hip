@@ -309,54 +309,131 @@ __global__ __launch_bounds__(256) void k_grid_dirty(const TriGeom *__restrict__ 
     flag[i] = d ? 1 : 0;
 }
 
-// update_grid_cache_kernel (pppm_cache.cu:64-123), integral part: one thread per
-// (dirty triangle, offset o, wave number k <= 8) evaluates the 8-target FDTD-stencil batch
-// and the single cell-centre target, both layers -> 4 complex values
-__global__ __launch_bounds__(B) void k_grid_integrals(const float *__restrict__ verts, const int *__restrict__ tris,
-                                                      const TriGeom *__restrict__ geom, const int *__restrict__ list,
-                                                      const int *__restrict__ n_ptr, WeightArgs a, Tdbem bem,
-                                                      cdbl *__restrict__ tmp, const int *__restrict__ err)
+// Both layer kernels of one (source point, target) pair: they share r and e^{ikr}, so the
+// values are those of two layer64 calls.
+__device__ inline void layer64_pair(d3 src, d3 trg, d3 n, double kr, double ki, cdbl &g0, cdbl &g1)
 {
-    const long long items = (long long)live_count(n_ptr, err) * GC * 9;
-    for (long long it = (long long)blockIdx.x * B + threadIdx.x; it < items; it += (long long)gridDim.x * B)
+    const d3 s0 = trg - src;
+    const double r2 = dotd(s0, s0);
+    if (r2 == 0)
+    {
+        g0 = g1 = {0, 0};
+        return;
+    }
+    const double r = sqrt(r2);
+    const double zr = -r * ki, zi = r * kr;
+    const cdbl e = cexp_d(zr, zi);
+    const double q0 = 4 * PI_R * r;
+    g0 = {e.re / q0, e.im / q0};
+    const d3 s1 = src - trg;  // = -s0 exactly, so dotd(s1, s1) == r2
+    const double q1 = 4 * PI_R * r2 * r;
+    const double ar = -e.re / q1, ai = -e.im / q1;
+    const double br = 1 - zr, bi = -zi;
+    const double pr = ar * br - ai * bi, pi = ar * bi + ai * br;
+    const double dn = dotd(s1, n);
+    g1 = {pr * dn, pi * dn};
+}
+
+// The target cells of one triangle's 27 stencil batches, as offsets from its cell: the
+// 3x3x3 block plus one more step along a single axis (81 cells); idx maps an offset in
+// [-2, 2]^3 to its slot.
+struct Targets81
+{
+    signed char off[81][3];
+    signed char idx[125];
+};
+constexpr Targets81 make_targets81()
+{
+    Targets81 t{};
+    for (int i = 0; i < 125; i++)
+        t.idx[i] = -1;
+    int n = 0;
+    for (int x = -2; x <= 2; x++)
+        for (int y = -2; y <= 2; y++)
+            for (int z = -2; z <= 2; z++)
+            {
+                const int ax = x < 0 ? -x : x, ay = y < 0 ? -y : y, az = z < 0 ? -z : z;
+                const int twos = (ax == 2) + (ay == 2) + (az == 2);
+                if (twos > 1)
+                    continue;
+                t.off[n][0] = (signed char)x;
+                t.off[n][1] = (signed char)y;
+                t.off[n][2] = (signed char)z;
+                t.idx[(x + 2) * 25 + (y + 2) * 5 + (z + 2)] = (signed char)n;
+                n++;
+            }
+    return t;
+}
+__constant__ Targets81 c_t81 = make_targets81();
+constexpr int GB = 128;  // threads per (triangle, k) block: 81 targets, then 27 batches
+
+// update_grid_cache_kernel (pppm_cache.cu:64-123), integral part. One block per (dirty
+// triangle, wave number k <= 8): the 81 distinct targets evaluate both layer kernels once
+// (LDS), then each of the 27 offsets sums its 8-target FDTD-stencil batch and takes its
+// cell-centre target, in the order of the reference's batch -> 4 complex values per (o, k).
+__global__ __launch_bounds__(GB) void k_grid_integrals(const float *__restrict__ verts, const int *__restrict__ tris,
+                                                       const TriGeom *__restrict__ geom, const int *__restrict__ list,
+                                                       const int *__restrict__ n_ptr, WeightArgs a, Tdbem bem,
+                                                       cdbl *__restrict__ tmp, const int *__restrict__ err)
+{
+    __shared__ cdbl gl[81][2];
+    const long long items = (long long)live_count(n_ptr, err) * 9;
+    const int j = threadIdx.x;
+    for (long long it = blockIdx.x; it < items; it += gridDim.x)
     {
         const int k = (int)(it % 9);
-        const int o = (int)((it / 9) % GC);
-        const int tri = list[it / (9 * GC)];
-        const TriGeom g = geom[tri];
+        const int li = (int)(it / 9);
+        const int tri = list[li];
+        const int3_ c = geom[tri].cell3;
         const TriD T = tri_d(verts, tris[3 * tri], tris[3 * tri + 1], tris[3 * tri + 2]);
-        const int3_ c = g.cell3;
-        const int3_ nb = {c.x + o % 3 - 1, c.y + o / 3 % 3 - 1, c.z + o / 9 - 1};
-        const float s = a.c * a.c * a.dt * a.dt / (a.h * a.h);
-        const int dx[8] = {0, -1, 1, 0, 0, 0, 0, 0}, dy[8] = {0, 0, 0, -1, 1, 0, 0, 0},
-                  dz[8] = {0, 0, 0, 0, 0, -1, 1, 0};
-        const float dsc[8] = {2, s, s, s, s, s, s, -6 * s};
-        d3 pts[8];
-        double scs[8];
-        int np = 0;
-        for (int q = 0; q < 8; q++)
-        {
-            const int x = nb.x + dx[q], y = nb.y + dy[q], z = nb.z + dz[q];
-            if (x == c.x && y == c.y && z == c.z)
-                continue;  // triangle_in_grid_cell (pppm_cache.cu:7-19)
-            pts[np] = cell_center_d(x, y, z, a);
-            scs[np] = dsc[q];
-            np++;
-        }
-        const bool self = (nb.x == c.x && nb.y == c.y && nb.z == c.z);
         const double kr = bem.wave_re[k], ki = bem.wave_im[k];
-        cdbl *o4 = tmp + it * 4;
-        o4[0] = f2p64(T, pts, scs, np, kr, ki, 0);
-        o4[1] = f2p64(T, pts, scs, np, kr, ki, 1);
-        if (!self)
+        const d3 p = l2g(TRI1_X, TRI1_X, T.v);
+        if (j < 81)
         {
-            const d3 pc = cell_center_d(nb.x, nb.y, nb.z, a);
-            const double one = 1.0;
-            o4[2] = f2p64(T, &pc, &one, 1, kr, ki, 0);
-            o4[3] = f2p64(T, &pc, &one, 1, kr, ki, 1);
+            const d3 pt = cell_center_d(c.x + c_t81.off[j][0], c.y + c_t81.off[j][1], c.z + c_t81.off[j][2], a);
+            layer64_pair(p, pt, T.n, kr, ki, gl[j][0], gl[j][1]);
         }
-        else
-            o4[2] = o4[3] = {0, 0};
+        __syncthreads();
+        if (j < GC)
+        {
+            const int o = j;
+            const int ox = o % 3 - 1, oy = o / 3 % 3 - 1, oz = o / 9 - 1;
+            const float s = a.c * a.c * a.dt * a.dt / (a.h * a.h);
+            const int dx[8] = {0, -1, 1, 0, 0, 0, 0, 0}, dy[8] = {0, 0, 0, -1, 1, 0, 0, 0},
+                      dz[8] = {0, 0, 0, 0, 0, -1, 1, 0};
+            const float dsc[8] = {2, s, s, s, s, s, s, -6 * s};
+            double sr0 = 0, si0 = 0, sr1 = 0, si1 = 0;
+            for (int q = 0; q < 8; q++)
+            {
+                const int ex = ox + dx[q], ey = oy + dy[q], ez = oz + dz[q];
+                if (ex == 0 && ey == 0 && ez == 0)
+                    continue;  // triangle_in_grid_cache: no target in the triangle's own cell (pppm_cache.cu:7-19)
+                const int slot = c_t81.idx[(ex + 2) * 25 + (ey + 2) * 5 + (ez + 2)];
+                const double sc = dsc[q];
+                sr0 += gl[slot][0].re * sc;
+                si0 += gl[slot][0].im * sc;
+                sr1 += gl[slot][1].re * sc;
+                si1 += gl[slot][1].im * sc;
+            }
+            const double f = 0.5 * TRI1_W * T.jac;
+            cdbl *o4 = tmp + (((long long)li * GC + o) * 9 + k) * 4;
+            o4[0] = {f * sr0, f * si0};
+            o4[1] = {f * sr1, f * si1};
+            if (!(ox == 0 && oy == 0 && oz == 0))
+            {
+                const int slot = c_t81.idx[(ox + 2) * 25 + (oy + 2) * 5 + (oz + 2)];
+                double cr0 = 0, ci0 = 0, cr1 = 0, ci1 = 0;
+                cr0 += gl[slot][0].re * 1.0;
+                ci0 += gl[slot][0].im * 1.0;
+                cr1 += gl[slot][1].re * 1.0;
+                ci1 += gl[slot][1].im * 1.0;
+                o4[2] = {f * cr0, f * ci0};
+                o4[3] = {f * cr1, f * ci1};
+            }
+            else
+                o4[2] = o4[3] = {0, 0};
+        }
+        __syncthreads();
     }
 }
 
@@ -714,8 +791,8 @@ hipError_t launch_grid_weights(const float *verts, const int *tris, const TriGeo
                                long long hint_dirty, hipStream_t s)
 {
     const long long hint = hint_dirty < 0 ? cap : std::min<long long>(hint_dirty, cap);
-    k_grid_integrals<<<nb_hint(hint * GC * 9, (long long)cap * GC * 9, B), B, 0, s>>>(verts, tris, geom, list, n_ptr, a, b,
-                                                                          (cdbl *)tmp, err);
+    const long long blocks = std::max<long long>(64, std::min<long long>(hint * 9, 16384));
+    k_grid_integrals<<<(unsigned)blocks, GB, 0, s>>>(verts, tris, geom, list, n_ptr, a, b, (cdbl *)tmp, err);
     k_grid_finish<<<nb_hint(hint * GC, (long long)cap * GC, B), B, 0, s>>>(geom, list, n_ptr, b, (const cdbl *)tmp, wf, sf, wb,
                                                                     sb, meta, err);
     return hipGetLastError();
