@@ -43,8 +43,35 @@ hipError_t launch_fdtd(float *grid3, float *pml3, const FdtdParams &p, const Ste
 // exclusive scan of n ints into out[0..n] (out[n] = total); tmp must hold scan_tmp_ints(n) ints
 size_t scan_tmp_ints(int n);
 hipError_t scan_exclusive(const int *in, int *out, int n, int *tmp, hipStream_t s);
+// one array of a batched scan; every output pointer may be null
+struct ScanJob
+{
+    const int *in;
+    int n;
+    int flag_mode;   // scan (in[q] != 0) instead of in[q]
+    int *pos;        // exclusive positions, n + 1 entries (pos[n] = total)
+    int *list;       // list[rank] = q for every selected q (flag mode)
+    int *index_map;  // index_map[q] = rank if selected, else -1 (flag mode)
+    int *total;      // receives the total
+    int tile0;       // set by scan_batch
+};
+constexpr int SCAN_MAX_JOBS = 4;
+struct ScanBatch
+{
+    ScanJob job[SCAN_MAX_JOBS];
+    int njobs;
+};
+// tmp must hold scan_batch_tmp_ints(sizes) ints
+size_t scan_batch_tmp_ints(const int *n, int njobs);
+hipError_t scan_batch(ScanBatch b, int *tmp, hipStream_t s);
 
 // ---------------------------------------------------------------- build.hip (binning, lists)
+struct WeightArgs
+{
+    int N, nt;
+    float h, c, dt;
+    float minx, miny, minz;
+};
 struct BuildArgs
 {
     int N, nt, nv;
@@ -54,31 +81,37 @@ struct BuildArgs
 hipError_t launch_fill_int(int *a, long long n, int value, hipStream_t s);
 hipError_t launch_copy_verts(const float *series, long long stride, int t0, const StepState *st, float *verts, int n,
                              hipStream_t s);
-hipError_t launch_tri_geometry(const float *verts, const int *tris, TriGeom *geom, const BuildArgs &a, int *err_flag,
-                               hipStream_t s);
-hipError_t launch_count_cells(const TriGeom *geom, int nt, int *cell_cnt, int *base_cnt, hipStream_t s);
-hipError_t launch_fill_cells(const TriGeom *geom, int nt, const int *cell_start, const int *base_start,
-                             int *cell_fill, int *base_fill, int *cell_items, int *base_items, int *cell_rank,
-                             int *base_rank, hipStream_t s);
+// counters of a mesh update, zero between updates (N^3 ints each)
+struct UpdateCounters
+{
+    int *cell_cnt, *base_cnt, *cell_fill, *mark;
+};
+struct ListArgs
+{
+    int N, nt;
+    const TriGeom *geom;
+    UpdateCounters cn;
+    int *cell_start, *cell_items, *cell_rank;
+    int *c3_cells, *c3_n, *c3_index, *c3_cnt, *c3_start, *c3_pairs, *c3_wave;
+    int cap_c3, c3_waves;
+    int *b4_cells, *b4_n, *b4_index, *b4_cnt, *b4_start, *b4_tris;
+    int cap_b4items;
+    int *face_cnt, *face_start, *face_nbr, *face_owner, *face_wave;
+    int cap_p4, face_waves;
+    int *corner;
+    int *err;
+};
+// in-cell order of the cell CSR (classification needs ascending triangle indices)
 hipError_t launch_sort_cells(const TriGeom *geom, int nt, const int *start, int *items, const int *rank,
                              int use_base, hipStream_t s);
-hipError_t launch_mark27(const TriGeom *geom, int nt, int N, int *flag, hipStream_t s);
-hipError_t launch_flag_base(const int *base_cnt, int N, int *flag, hipStream_t s);
-hipError_t launch_scatter_selected(const int *flag, const int *pos, int n, int *out, int *index_map, hipStream_t s);
-// width^3 neighbour blocks around compacted cells (count live ones, zero past *n_ptr; then fill)
-hipError_t launch_nbr_count(const int *cells, const int *n_ptr, int cap, const int *cell_start, int N, int width,
-                            int *cnt, hipStream_t s);
-hipError_t launch_nbr_fill(const int *cells, const int *n_ptr, int cap, const int *cell_start, const int *cell_items,
-                           int N, int width, const int *out_start, int out_cap, int *out, int *err_flag, hipStream_t s);
-hipError_t launch_face_counts(const TriGeom *geom, int nt, const int *b4_index, const int *b4_start, int *cnt,
-                              hipStream_t s);
-hipError_t launch_face_fill(const TriGeom *geom, int nt, const int *b4_index, const int *b4_start,
-                            const int *b4_tris, const int *face_start, int cap, int *face_nbr, int *face_owner,
-                            int *err_flag, hipStream_t s);
-hipError_t launch_flag_nonzero(const int *in, int n, int *flag01, hipStream_t s);
-hipError_t launch_corner_map(const TriGeom *geom, int nt, int N, const int *c3_index, int *corner, hipStream_t s);
-// n_ptr (device, may be null): live segment count; the map never extends past it
-hipError_t launch_wave_map(const int *start, int nseg, const int *n_ptr, int waves, int *map, hipStream_t s);
+// geometry, counts, marks, grid-cache dirty flags, interpolation weights; dst != null copies
+// the nv vertices of src into dst in the same launch
+hipError_t launch_tri_update(const float *src, float *dst, int nv, const int *tris, TriGeom *geom, const BuildArgs &ba,
+                             const WeightArgs &wa, const float *gmeta, int first, int *gflag, float *interp,
+                             const UpdateCounters &cn, int *err, hipStream_t s);
+hipError_t launch_lists_count(const ListArgs &a, hipStream_t s);
+hipError_t launch_lists_fill(const ListArgs &a, hipStream_t s);
+hipError_t launch_faces_fill(const ListArgs &a, hipStream_t s);
 hipError_t launch_advance(StepState *st, hipStream_t s);
 struct StatusSrc
 {
@@ -97,23 +130,14 @@ struct Tdbem
 };
 void tdbem_init(Tdbem &b, float dt, float c, float grid_size);
 
-struct WeightArgs
-{
-    int N, nt;
-    float h, c, dt;
-    float minx, miny, minz;
-};
 size_t grid_tmp_bytes(int max_dirty);
 size_t face_tmp_bytes(int max_dirty);
-hipError_t launch_grid_dirty(const TriGeom *geom, float *meta, int nt, float h, int first, int *flag, hipStream_t s);
 // records tri*27+o of the *n_ptr triangles in list (cap bounds *n_ptr). hint_dirty: host
 // estimate of *n_ptr that sizes the grid-stride launch (-1: cap); any value is correct.
 hipError_t launch_grid_weights(const float *verts, const int *tris, const TriGeom *geom, const int *list,
                                const int *n_ptr, int cap, const WeightArgs &a, const Tdbem &b, void *tmp,
                                uint32_t *wf, float2 *sf, uint32_t *wb, float2 *sb, float *meta, const int *err,
                                long long hint_dirty, hipStream_t s);
-hipError_t launch_interp_weights(const float *verts, const int *tris, const TriGeom *geom, const WeightArgs &a,
-                                 float *interp, const int *err, hipStream_t s);
 hipError_t launch_face_lookup(const TriGeom *geom, const int *tris, const int *owner, const int *face_nbr,
                               const int *n_ptr, int cap, unsigned long long *hkeys, int *hvals, int hcap,
                               int *pool_count, int pool_cap, float2 *fmeta, int *face_slot, int *dirty, int *err_flag,

@@ -1,14 +1,18 @@
-// Deterministic exclusive prefix sum of int arrays, used for the CSR layouts of the
-// triangle bins, the 3x3x3 / 4x4x4 lists and the compaction of non-empty cells (the
-// reference uses thrust::remove_if for the same compactions, src/core/array.h:77-115).
+// Deterministic exclusive prefix sums of int arrays, used for the CSR layouts of the
+// triangle bins and the 3x3x3 / 4x4x4 lists, and for the compaction of selected cells,
+// triangles and face pairs (the reference uses thrust::remove_if for the same compactions,
+// src/core/array.h:77-115).
 //
 // Reduce-then-scan in two launches and no inter-block waiting: the first kernel writes one
 // sum per 4096-item tile, the second has every tile add up the sums of the tiles before it
-// (at most a few hundred ints, one wave), scan its own items and, in the last tile, write
-// the total to out[n]. Arrays up to 8192 items take a single launch. Integer sums are exact,
-// so the result does not depend on the order of additions. Only kernels are enqueued, so
-// the scan can be captured in a HIP graph.
+// (one wave), scan its own items and, in the last tile, write the total. Several independent
+// arrays ("jobs") share the two launches; jobs of at most 8192 items take a single launch.
+// A job in flag mode scans (in != 0) and can write the compacted list of selected indices
+// and an index map (rank or -1) in the same pass. Integer sums are exact, so the result does
+// not depend on the order of additions. Only kernels are enqueued (graph-capturable).
 #include "internal.hpp"
+
+#include <algorithm>
 
 namespace pppm_amd
 {
@@ -41,17 +45,30 @@ __device__ inline int wave_total(int v)
     return v;
 }
 
-template <int IT>
-__device__ inline void load_tile(const int *__restrict__ in, long long n, int v[IT], int &tsum)
+// the job that owns global tile `tile` (jobs are laid out in tile order)
+__device__ inline const ScanJob &job_of(const ScanBatch &b, int tile)
 {
-    const long long base = (long long)blockIdx.x * SB * IT + (long long)threadIdx.x * IT;
+    int k = 0;
+    for (int q = 1; q < b.njobs; q++)
+        if (tile >= b.job[q].tile0)
+            k = q;
+    return b.job[k];
+}
+
+template <int IT>
+__device__ inline void load_tile(const ScanJob &J, int tile, int v[IT], int &tsum)
+{
+    const long long base = (long long)tile * SB * IT + (long long)threadIdx.x * IT;
     tsum = 0;
 #pragma unroll
     for (int i = 0; i < IT; i++)
     {
         const long long q = base + i;
-        v[i] = q < n ? in[q] : 0;
-        tsum += v[i];
+        int x = q < J.n ? J.in[q] : 0;
+        if (J.flag_mode)
+            x = x != 0 ? 1 : 0;
+        v[i] = x;
+        tsum += x;
     }
 }
 
@@ -76,50 +93,65 @@ __device__ inline int block_scan(int tsum, int &btotal)
     return woff + incl - tsum;
 }
 
-__global__ __launch_bounds__(SB) void k_tile_sums(const int *__restrict__ in, long long n, int *__restrict__ sums)
+__global__ __launch_bounds__(SB) void k_tile_sums(ScanBatch b, int *__restrict__ sums)
 {
+    const ScanJob &J = job_of(b, blockIdx.x);
     int v[SI], tsum, btotal;
-    load_tile<SI>(in, n, v, tsum);
+    load_tile<SI>(J, blockIdx.x - J.tile0, v, tsum);
     (void)block_scan(tsum, btotal);
     if (threadIdx.x == 0)
         sums[blockIdx.x] = btotal;
 }
 
-// sums == null: one tile, no offset. IT items per thread.
+// sums == null: every job is a single tile (one block per job). IT items per thread.
 template <int IT>
-__global__ __launch_bounds__(SB) void k_tile_scan(const int *__restrict__ in, int *__restrict__ out, long long n,
-                                                  const int *__restrict__ sums)
+__global__ __launch_bounds__(SB) void k_tile_scan(ScanBatch b, const int *__restrict__ sums)
 {
     __shared__ int s_off;
+    const ScanJob &J = sums ? job_of(b, blockIdx.x) : b.job[blockIdx.x];
+    const int tile = sums ? (int)blockIdx.x - J.tile0 : 0;
     if (threadIdx.x < 64)
     {
         int acc = 0;
         if (sums)
-            for (int b0 = 0; b0 < (int)blockIdx.x; b0 += 64)
+            for (int t0 = J.tile0; t0 < (int)blockIdx.x; t0 += 64)
             {
-                const int b = b0 + (int)threadIdx.x;
-                acc += b < (int)blockIdx.x ? sums[b] : 0;
+                const int t = t0 + (int)threadIdx.x;
+                acc += t < (int)blockIdx.x ? sums[t] : 0;
             }
         acc = wave_total(acc);
         if (threadIdx.x == 0)
             s_off = acc;
     }
     int v[IT], tsum, btotal;
-    load_tile<IT>(in, n, v, tsum);
+    load_tile<IT>(J, tile, v, tsum);
     int run = block_scan(tsum, btotal);  // its barrier also publishes s_off
     const int off = s_off;
     run += off;
-    const long long base = (long long)blockIdx.x * SB * IT + (long long)threadIdx.x * IT;
+    const long long base = (long long)tile * SB * IT + (long long)threadIdx.x * IT;
 #pragma unroll
     for (int i = 0; i < IT; i++)
     {
         const long long q = base + i;
-        if (q < n)
-            out[q] = run;
+        if (q < J.n)
+        {
+            if (J.pos)
+                J.pos[q] = run;
+            if (J.list && v[i])
+                J.list[run] = (int)q;
+            if (J.index_map)
+                J.index_map[q] = v[i] ? run : -1;
+        }
         run += v[i];
     }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
-        out[n] = off + btotal;
+    const long long tiles = ((long long)J.n + SB * IT - 1) / (SB * IT);
+    if (tile == (tiles > 0 ? tiles - 1 : 0) && threadIdx.x == 0)
+    {
+        if (J.pos)
+            J.pos[J.n] = off + btotal;
+        if (J.total)
+            *J.total = off + btotal;
+    }
 }
 
 }  // namespace
@@ -130,20 +162,51 @@ size_t scan_tmp_ints(int n)
     return (size_t)(nb > 1 ? nb : 1) + 16;
 }
 
+size_t scan_batch_tmp_ints(const int *n, int njobs)
+{
+    long long t = 0;
+    for (int k = 0; k < njobs; k++)
+        t += std::max<long long>(1, ((long long)n[k] + SPB - 1) / SPB);
+    return (size_t)t + 16;
+}
+
+// Jobs with at most SB * SI1 items each run in one launch (a block per job); otherwise a
+// tile-sums launch and a tile-scan launch cover every job's tiles.
+hipError_t scan_batch(ScanBatch b, int *tmp, hipStream_t s)
+{
+    if (b.njobs < 1 || b.njobs > SCAN_MAX_JOBS)
+        return hipErrorInvalidValue;
+    bool small = true;
+    int tiles = 0;
+    for (int k = 0; k < b.njobs; k++)
+    {
+        if (b.job[k].n < 0)
+            return hipErrorInvalidValue;
+        small = small && b.job[k].n <= SB * SI1;
+        b.job[k].tile0 = tiles;
+        tiles += (int)std::max<long long>(1, ((long long)b.job[k].n + SPB - 1) / SPB);
+    }
+    if (small)
+    {
+        k_tile_scan<SI1><<<b.njobs, SB, 0, s>>>(b, nullptr);
+        return hipGetLastError();
+    }
+    k_tile_sums<<<tiles, SB, 0, s>>>(b, tmp);
+    k_tile_scan<SI><<<tiles, SB, 0, s>>>(b, tmp);
+    return hipGetLastError();
+}
+
 // out must hold n+1 ints; tmp must hold scan_tmp_ints(n) ints.
 hipError_t scan_exclusive(const int *in, int *out, int n, int *tmp, hipStream_t s)
 {
     if (n < 0)
         return hipErrorInvalidValue;
-    if (n <= SB * SI1)
-    {
-        k_tile_scan<SI1><<<1, SB, 0, s>>>(in, out, n, nullptr);
-        return hipGetLastError();
-    }
-    const long long nb = ((long long)n + SPB - 1) / SPB;
-    k_tile_sums<<<(unsigned)nb, SB, 0, s>>>(in, n, tmp);
-    k_tile_scan<SI><<<(unsigned)nb, SB, 0, s>>>(in, out, n, tmp);
-    return hipGetLastError();
+    ScanBatch b{};
+    b.njobs = 1;
+    b.job[0].in = in;
+    b.job[0].n = n;
+    b.job[0].pos = out;
+    return scan_batch(b, tmp, s);
 }
 
 }  // namespace pppm_amd

@@ -92,6 +92,16 @@ enum
     ST_N = 8
 };
 
+// live counts written by the update's scans
+enum
+{
+    LV_C3 = 0,  // marked (3x3x3) cells
+    LV_B4 = 1,  // non-empty base cells
+    LV_G = 2,   // grid-cache triangles to recompute
+    LV_F = 3,   // face pairs to recompute
+    LV_N = 4
+};
+
 }  // namespace
 
 struct pppm_solver
@@ -121,14 +131,17 @@ struct pppm_solver
     int cap_c3 = 0, cap_b4items = 0, cap_p4 = 0;
     int c3_waves = 0, face_waves = 0;
 
-    DBuf<int> counters;  // [cell_cnt | base_cnt | cell_fill | base_fill], 4 x N^3
-    DBuf<int> cell_start, base_start, cell_items, base_items, cell_rank, base_rank;
-    DBuf<int> flag, c3_pos, b4_pos, scan_tmp;
+    // [cell_cnt | base_cnt | cell_fill | mark], 4 x N^3, all zero between updates
+    DBuf<int> counters;
+    bool counters_zero = false;
+    DBuf<int> cell_start, cell_items, cell_rank;
+    DBuf<int> scan_tmp;
+    DBuf<int> live;  // device-side live counts of a mesh update (LV_*)
     DBuf<int> c3_cells, c3_index, c3_cnt, c3_start, c3_pairs, c3_wave;
     DBuf<int> b4_cells, b4_index, b4_cnt, b4_start, b4_tris;
-    DBuf<int> face_cnt, face_start, face_nbr, face_slot, face_owner, face_dirty, face_flag, face_pos, dirty_list;
+    DBuf<int> face_cnt, face_start, face_nbr, face_slot, face_owner, face_flag, dirty_list;
     DBuf<int> face_wave, corner;
-    DBuf<int> gflag, gpos, glist;
+    DBuf<int> gflag, glist;
     DBuf<char> gtmp, ftmp;
     DBuf<int> errflag, status;
 
@@ -306,18 +319,15 @@ int allocate(pppm_solver *s)
     const int p3 = 27 * nt;
     s->c3_waves = p3 / WAVE + 1;
     s->face_waves = s->cap_p4 / WAVE + 1;
-    const int scan_n = std::max({NN, s->cap_c3, s->cap_p4, nt});
+    const int batch_n[4] = {NN, NN, NN, std::max({nt, s->cap_c3, s->cap_p4})};
+    if (s->counters.n < (size_t)4 * NN)
+        s->counters_zero = false;
     CK(s->counters.ensure((size_t)4 * NN));
     CK(s->cell_start.ensure(NN + 1));
-    CK(s->base_start.ensure(NN + 1));
     CK(s->cell_items.ensure(nt));
-    CK(s->base_items.ensure(nt));
     CK(s->cell_rank.ensure(nt));
-    CK(s->base_rank.ensure(nt));
-    CK(s->flag.ensure(NN));
-    CK(s->c3_pos.ensure(NN + 1));
-    CK(s->b4_pos.ensure(NN + 1));
-    CK(s->scan_tmp.ensure(scan_tmp_ints(scan_n)));
+    CK(s->scan_tmp.ensure(std::max(scan_batch_tmp_ints(batch_n, 4), scan_tmp_ints(NN))));
+    CK(s->live.ensure(LV_N));
     CK(s->c3_cells.ensure(s->cap_c3));
     CK(s->c3_index.ensure(NN));
     CK(s->c3_cnt.ensure(s->cap_c3));
@@ -335,14 +345,11 @@ int allocate(pppm_solver *s)
     CK(s->face_nbr.ensure(s->cap_p4));
     CK(s->face_slot.ensure(s->cap_p4));
     CK(s->face_owner.ensure(s->cap_p4));
-    CK(s->face_dirty.ensure(s->cap_p4));
     CK(s->face_flag.ensure(s->cap_p4));
-    CK(s->face_pos.ensure(s->cap_p4 + 1));
     CK(s->dirty_list.ensure(s->cap_p4));
     CK(s->face_wave.ensure(s->face_waves + 1));
     CK(s->corner.ensure((size_t)nt * 8));
     CK(s->gflag.ensure(nt));
-    CK(s->gpos.ensure(nt + 1));
     CK(s->glist.ensure(nt));
     CK(s->gtmp.ensure(grid_tmp_bytes(nt)));
     CK(s->ftmp.ensure(face_tmp_bytes(s->cap_p4)));
@@ -400,96 +407,125 @@ int ensure_pool(pppm_solver *s, int need)
 // ---------------------------------------------------------------- the update sequence
 // Geometry, binning, lists and weight caches for the vertex positions in s->verts (or, with
 // a series, row t+1-t0 of it). Asynchronous: only kernels are enqueued.
-int enqueue_rebuild(pppm_solver *s, bool first, const float *verts_series, long long vstride, int t0)
+int enqueue_rebuild(pppm_solver *s, bool first, const float *verts_series, long long vstride, int t0,
+                    const float *vsrc = nullptr)
 {
     const int N = s->N, nt = s->nt;
     const int NN = N * N * N;
     hipStream_t st = s->stream;
     int *err = s->errflag.p;
+    int *lv = s->live.p;
 
     if (verts_series)
         CK(launch_copy_verts(verts_series, vstride, t0, s->state.p, s->verts.p, s->nv * 3, st));
-    // triangle geometry (update_triangles_kernel, particle_grid.cu:40-56)
+    if (!s->counters_zero)
+        CK(launch_fill_int(s->counters.p, 4LL * NN, 0, st));
+    s->counters_zero = false;
+    const UpdateCounters cn{s->counters.p, s->counters.p + NN, s->counters.p + 2 * NN, s->counters.p + 3 * NN};
+
+    // geometry, counts, 3x3x3 marks, grid-cache dirty flags (GridCache::update_cache,
+    // pppm_cache.cu:125-134), interpolation weights (update_interpolation_weight_kernel)
     const BuildArgs ba{N, nt, s->nv, s->h, s->minp[0], s->minp[1], s->minp[2]};
-    CK(launch_tri_geometry(s->verts.p, s->tris.p, s->geom.p, ba, err, st));
-
-    // cell / base-cell CSR (grid_face_list / base_coord_face_list)
-    int *cell_cnt = s->counters.p, *base_cnt = cell_cnt + NN, *cell_fill = base_cnt + NN, *base_fill = cell_fill + NN;
-    CK(launch_fill_int(s->counters.p, 4LL * NN, 0, st));
-    CK(launch_count_cells(s->geom.p, nt, cell_cnt, base_cnt, st));
-    RC(scan(s, cell_cnt, s->cell_start.p, NN));
-    RC(scan(s, base_cnt, s->base_start.p, NN));
-    CK(launch_fill_cells(s->geom.p, nt, s->cell_start.p, s->base_start.p, cell_fill, base_fill, s->cell_items.p,
-                         s->base_items.p, s->cell_rank.p, s->base_rank.p, st));
-    CK(launch_sort_cells(s->geom.p, nt, s->cell_start.p, s->cell_items.p, s->cell_rank.p, 0, st));
-    CK(launch_sort_cells(s->geom.p, nt, s->base_start.p, s->base_items.p, s->base_rank.p, 1, st));
-
-    // C3 cells: within +-1 of a triangle cell, linear order, with their 3x3x3 pair lists
-    CK(launch_fill_int(s->flag.p, NN, 0, st));
-    CK(launch_mark27(s->geom.p, nt, N, s->flag.p, st));
-    RC(scan(s, s->flag.p, s->c3_pos.p, NN));
-    CK(launch_scatter_selected(s->flag.p, s->c3_pos.p, NN, s->c3_cells.p, s->c3_index.p, st));
-    CK(launch_nbr_count(s->c3_cells.p, s->c3_pos.p + NN, s->cap_c3, s->cell_start.p, N, 3, s->c3_cnt.p, st));
-    RC(scan(s, s->c3_cnt.p, s->c3_start.p, s->cap_c3));
-    CK(launch_nbr_fill(s->c3_cells.p, s->c3_pos.p + NN, s->cap_c3, s->cell_start.p, s->cell_items.p, N, 3,
-                       s->c3_start.p, 27 * nt, s->c3_pairs.p, err, st));
-    CK(launch_wave_map(s->c3_start.p, s->cap_c3, s->c3_pos.p + NN, s->c3_waves, s->c3_wave.p, st));
-
-    // non-empty base cells and their 4x4x4 lists
-    CK(launch_flag_base(base_cnt, N, s->flag.p, st));
-    RC(scan(s, s->flag.p, s->b4_pos.p, NN));
-    CK(launch_scatter_selected(s->flag.p, s->b4_pos.p, NN, s->b4_cells.p, s->b4_index.p, st));
-    CK(launch_nbr_count(s->b4_cells.p, s->b4_pos.p + NN, nt, s->cell_start.p, N, 4, s->b4_cnt.p, st));
-    RC(scan(s, s->b4_cnt.p, s->b4_start.p, nt));
-    CK(launch_nbr_fill(s->b4_cells.p, s->b4_pos.p + NN, nt, s->cell_start.p, s->cell_items.p, N, 4, s->b4_start.p,
-                       s->cap_b4items, s->b4_tris.p, err, st));
-
-    // per-triangle face lists (the 4x4x4 list of its base cell)
-    CK(launch_face_counts(s->geom.p, nt, s->b4_index.p, s->b4_start.p, s->face_cnt.p, st));
-    RC(scan(s, s->face_cnt.p, s->face_start.p, nt));
-    CK(launch_face_fill(s->geom.p, nt, s->b4_index.p, s->b4_start.p, s->b4_tris.p, s->face_start.p, s->cap_p4,
-                        s->face_nbr.p, s->face_owner.p, err, st));
-    CK(launch_wave_map(s->face_start.p, nt, nullptr, s->face_waves, s->face_wave.p, st));
-    CK(launch_corner_map(s->geom.p, nt, N, s->c3_index.p, s->corner.p, st));
-
-    // grid cache: dirty triangles only (GridCache::update_cache, pppm_cache.cu:125-134)
     const WeightArgs wa = weight_args(s);
-    CK(launch_grid_dirty(s->geom.p, s->gmeta.p, nt, s->h, first ? 1 : 0, s->gflag.p, st));
-    RC(scan(s, s->gflag.p, s->gpos.p, nt));
-    CK(launch_scatter_selected(s->gflag.p, s->gpos.p, nt, s->glist.p, nullptr, st));
-    // launch sizes follow the previous update's recompute counts (the kernels stride, so a
-    // larger count is still covered); a first build sizes for everything
+    const float *src = vsrc ? vsrc : s->verts.p;
+    CK(launch_tri_update(src, vsrc ? s->verts.p : nullptr, s->nv, s->tris.p, s->geom.p, ba, wa, s->gmeta.p,
+                         first ? 1 : 0, s->gflag.p, s->interp.p, cn, err, st));
+
+    // cell CSR offsets; compaction of non-empty base cells, marked cells, dirty triangles
+    {
+        ScanBatch b{};
+        b.njobs = 4;
+        b.job[0] = ScanJob{cn.cell_cnt, NN, 0, s->cell_start.p, nullptr, nullptr, nullptr, 0};
+        b.job[1] = ScanJob{cn.base_cnt, NN, 1, nullptr, s->b4_cells.p, s->b4_index.p, lv + LV_B4, 0};
+        b.job[2] = ScanJob{cn.mark, NN, 1, nullptr, s->c3_cells.p, s->c3_index.p, lv + LV_C3, 0};
+        b.job[3] = ScanJob{s->gflag.p, nt, 1, nullptr, s->glist.p, nullptr, lv + LV_G, 0};
+        CK(scan_batch(b, s->scan_tmp.p, st));
+    }
+
+    ListArgs la{};
+    la.N = N;
+    la.nt = nt;
+    la.geom = s->geom.p;
+    la.cn = cn;
+    la.cell_start = s->cell_start.p;
+    la.cell_items = s->cell_items.p;
+    la.cell_rank = s->cell_rank.p;
+    la.c3_cells = s->c3_cells.p;
+    la.c3_n = lv + LV_C3;
+    la.c3_index = s->c3_index.p;
+    la.c3_cnt = s->c3_cnt.p;
+    la.c3_start = s->c3_start.p;
+    la.c3_pairs = s->c3_pairs.p;
+    la.c3_wave = s->c3_wave.p;
+    la.cap_c3 = s->cap_c3;
+    la.c3_waves = s->c3_waves;
+    la.b4_cells = s->b4_cells.p;
+    la.b4_n = lv + LV_B4;
+    la.b4_index = s->b4_index.p;
+    la.b4_cnt = s->b4_cnt.p;
+    la.b4_start = s->b4_start.p;
+    la.b4_tris = s->b4_tris.p;
+    la.cap_b4items = s->cap_b4items;
+    la.face_cnt = s->face_cnt.p;
+    la.face_start = s->face_start.p;
+    la.face_nbr = s->face_nbr.p;
+    la.face_owner = s->face_owner.p;
+    la.face_wave = s->face_wave.p;
+    la.cap_p4 = s->cap_p4;
+    la.face_waves = s->face_waves;
+    la.corner = s->corner.p;
+    la.err = err;
+
+    // cell fill, 3x3x3 / 4x4x4 list lengths, their offsets, the lists themselves, face counts,
+    // corner map and the reset of the counters
+    CK(launch_lists_count(la, st));
+    {
+        ScanBatch b{};
+        b.njobs = 2;
+        b.job[0] = ScanJob{s->c3_cnt.p, s->cap_c3, 0, s->c3_start.p, nullptr, nullptr, nullptr, 0};
+        b.job[1] = ScanJob{s->b4_cnt.p, nt, 0, s->b4_start.p, nullptr, nullptr, nullptr, 0};
+        CK(scan_batch(b, s->scan_tmp.p, st));
+    }
+    CK(launch_lists_fill(la, st));
+    s->counters_zero = true;
+    // per-triangle face lists (the 4x4x4 list of its base cell)
+    RC(scan(s, s->face_cnt.p, s->face_start.p, nt));
+    CK(launch_faces_fill(la, st));
+
+    // grid cache: dirty triangles only. Launch sizes follow the previous update's recompute
+    // counts (the kernels stride, so a larger count is still covered); a first build sizes
+    // for everything.
     const long long ghint = first ? -1 : 2 * s->last_grid_recomputes + 64;
     const long long fhint = first ? -1 : 2 * s->last_face_recomputes + 64;
-    CK(launch_grid_weights(s->verts.p, s->tris.p, s->geom.p, s->glist.p, s->gpos.p + nt, nt, wa, s->bem, s->gtmp.p,
+    CK(launch_grid_weights(s->verts.p, s->tris.p, s->geom.p, s->glist.p, lv + LV_G, nt, wa, s->bem, s->gtmp.p,
                            s->wf.p, s->sf.p, s->wb.p, s->sb.p, s->gmeta.p, err, ghint, st));
-
-    // interpolation weights, every update (update_interpolation_weight_kernel)
-    CK(launch_interp_weights(s->verts.p, s->tris.p, s->geom.p, wa, s->interp.p, err, st));
 
     // face cache: persistent (c, j) slots, dirty pairs recomputed (FaceCache::update_cache)
     CK(launch_face_lookup(s->geom.p, s->tris.p, s->face_owner.p, s->face_nbr.p, s->face_start.p + nt, s->cap_p4,
                           s->hkeys.p, s->hvals.p, s->hcap, s->pool_counter.p, s->pool_cap, s->fmeta.p, s->face_slot.p,
-                          s->face_dirty.p, err, nt, st));
-    CK(launch_flag_nonzero(s->face_dirty.p, s->cap_p4, s->face_flag.p, st));
-    RC(scan(s, s->face_flag.p, s->face_pos.p, s->cap_p4));
-    CK(launch_scatter_selected(s->face_flag.p, s->face_pos.p, s->cap_p4, s->dirty_list.p, nullptr, st));
-    CK(launch_face_weights(s->verts.p, s->tris.p, s->geom.p, s->dirty_list.p, s->face_pos.p + s->cap_p4, s->cap_p4,
+                          s->face_flag.p, err, nt, st));
+    {
+        ScanBatch b{};
+        b.njobs = 1;
+        b.job[0] = ScanJob{s->face_flag.p, s->cap_p4, 1, nullptr, s->dirty_list.p, nullptr, lv + LV_F, 0};
+        CK(scan_batch(b, s->scan_tmp.p, st));
+    }
+    CK(launch_face_weights(s->verts.p, s->tris.p, s->geom.p, s->dirty_list.p, lv + LV_F, s->cap_p4,
                            s->face_owner.p, s->face_nbr.p, s->face_slot.p, s->interp.p, wa, s->bem, s->ftmp.p, s->fw.p,
                            s->fsc.p, s->fmeta.p, err, fhint, st));
 
     // status block
-    StatusSrc src{};
-    src.n = ST_N;
-    src.p[ST_ERR] = err;
-    src.p[ST_C3] = s->c3_pos.p + NN;
-    src.p[ST_B4] = s->b4_pos.p + NN;
-    src.p[ST_B4ITEMS] = s->b4_start.p + nt;
-    src.p[ST_P4] = s->face_start.p + nt;
-    src.p[ST_GDIRTY] = s->gpos.p + nt;
-    src.p[ST_FDIRTY] = s->face_pos.p + s->cap_p4;
-    src.p[ST_POOL] = s->pool_counter.p;
-    CK(launch_status(src, s->status.p, st));
+    StatusSrc stsrc{};
+    stsrc.n = ST_N;
+    stsrc.p[ST_ERR] = err;
+    stsrc.p[ST_C3] = lv + LV_C3;
+    stsrc.p[ST_B4] = lv + LV_B4;
+    stsrc.p[ST_B4ITEMS] = s->b4_start.p + nt;
+    stsrc.p[ST_P4] = s->face_start.p + nt;
+    stsrc.p[ST_GDIRTY] = lv + LV_G;
+    stsrc.p[ST_FDIRTY] = lv + LV_F;
+    stsrc.p[ST_POOL] = s->pool_counter.p;
+    CK(launch_status(stsrc, s->status.p, st));
     return PPPM_OK;
 }
 
@@ -543,7 +579,9 @@ int grow(pppm_solver *s)
 
 // synchronous update used by set_mesh / update_mesh: runs the sequence, retries once with
 // grown capacities if a list or the pool overflowed
-int rebuild_sync(pppm_solver *s, bool first)
+// vsrc (device, may be null): vertex positions to take instead of s->verts; the first
+// attempt copies them into s->verts
+int rebuild_sync(pppm_solver *s, bool first, const float *vsrc = nullptr)
 {
     for (int attempt = 0; attempt < 3; attempt++)
     {
@@ -555,7 +593,7 @@ int rebuild_sync(pppm_solver *s, bool first)
             CK(launch_fill_u64(s->hkeys.p, s->hcap, ~0ull, s->stream));
             CK(hipMemcpyAsync(s->pool_counter.p, &s->pool_count, sizeof(int), hipMemcpyHostToDevice, s->stream));
         }
-        RC(enqueue_rebuild(s, first, nullptr, 0, 0));
+        RC(enqueue_rebuild(s, first, nullptr, 0, 0, attempt == 0 ? vsrc : nullptr));
         RC(read_status(s));
         const int e = s->st_h[ST_ERR];
         if ((e & ~ERR_CAPACITY) != 0)
@@ -896,9 +934,10 @@ int pppm_update_mesh(pppm_solver *s, const float *verts, int on_device)
         return PPPM_ERR_ARG;
     if (!s->mesh_set)
         return fail(s, PPPM_ERR_STATE, "update_mesh before set_mesh");
-    RC(upload(s, s->verts.p, verts, (size_t)s->nv * 3 * sizeof(float), on_device));
+    if (!on_device)
+        RC(upload(s, s->verts.p, verts, (size_t)s->nv * 3 * sizeof(float), 0));
     const int sp = span_begin(s, 4);
-    const int rc = rebuild_sync(s, false);
+    const int rc = rebuild_sync(s, false, on_device ? verts : nullptr);
     span_end(s, sp);
     return rc;
 }
@@ -1354,6 +1393,9 @@ int pppm_classify(pppm_solver *s, int thin_shell, int *ghost_count)
     CK(launch_vertex_faces(s->tris.p, nt, nv, s->vf_cnt.p, s->vf_start.p, s->vf_fill.p, s->vf_items.p,
                            s->scan_tmp.p, st));
     CK(launch_vertex_normals(s->verts.p, s->tris.p, s->geom.p, nv, s->vf_start.p, s->vf_items.p, s->vnormal.p, st));
+    // the mesh update leaves each cell's triangles in atomic-slot order; the nearest-triangle
+    // search breaks ties by the first triangle in ascending order
+    CK(launch_sort_cells(s->geom.p, nt, s->cell_start.p, s->cell_items.p, s->cell_rank.p, 0, st));
     const CellArgs a{N, s->h, s->minp[0], s->minp[1], s->minp[2], thin_shell ? 1 : 0};
     CK(launch_classify(a, s->verts.p, s->tris.p, s->cell_start.p, s->cell_items.p, s->vnormal.p, s->ctype.p,
                        s->cnearest.p, s->cpoint.p, s->creflect.p, s->cdist.p, s->ctyp2.p, s->cflag.p, s->cscan.p,
@@ -1390,8 +1432,12 @@ int pppm_fill_fresh(pppm_solver *s, const float *acc_now, const float *acc_old, 
     CK(s->fresh_err.ensure(1));
     CK(launch_fill_int(s->fresh_err.p, 1, 0, st));
     CK(launch_fresh_flags(NN, nt, s->ctype_old.p, s->ctype.p, s->cnearest.p, s->cflag.p, s->fresh_err.p, st));
-    RC(scan(s, s->cflag.p, s->cscan.p, (int)NN));
-    CK(launch_scatter_selected(s->cflag.p, s->cscan.p, (int)NN, s->fresh_list.p, nullptr, st));
+    {
+        ScanBatch b{};
+        b.njobs = 1;
+        b.job[0] = ScanJob{s->cflag.p, (int)NN, 1, s->cscan.p, s->fresh_list.p, nullptr, nullptr, 0};
+        CK(scan_batch(b, s->scan_tmp.p, st));
+    }
     const CellArgs a{N, s->h, s->minp[0], s->minp[1], s->minp[2], 0};
     CK(launch_fresh_fill(a, s->state.p, s->fresh_list.p, s->cscan.p + NN, (int)NN, s->fresh_err.p, s->ctype_old.p,
                          s->ctype.p, s->cnearest.p, s->cpoint.p, an, ao, s->grid.p, st));

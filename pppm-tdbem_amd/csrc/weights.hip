@@ -16,6 +16,7 @@
 // (64 B, one half cache line) plus a float2 {scale_S, scale_D}.
 #include "detmath.hpp"
 #include "internal.hpp"
+#include "tri_device.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -29,8 +30,6 @@ constexpr int B = 128;
 
 __constant__ double c_gx[3] = {0.774596669241483, 0.000000000000000, -0.774596669241483};  // gauss/line.h:26-37
 __constant__ double c_gw[3] = {0.555555555555554, 0.888888888888889, 0.555555555555554};
-constexpr double TRI1_X = 0.3333333333333330;  // gauss/triangle.h:6-14
-constexpr double TRI1_W = 1.0;
 
 struct cdbl
 {
@@ -74,28 +73,6 @@ __device__ inline cdbl layer64(d3 src, d3 trg, d3 n, double kr, double ki, int t
     return {pr * dn, pi * dn};
 }
 
-struct TriD
-{
-    d3 v[3];
-    d3 n;
-    double jac;
-};
-
-__device__ inline f3 ld3(const float *v, int i) { return {v[3 * i], v[3 * i + 1], v[3 * i + 2]}; }
-
-__device__ inline TriD tri_d(const float *verts, int i0, int i1, int i2)
-{
-    TriD t;
-    t.v[0] = D(ld3(verts, i0));
-    t.v[1] = D(ld3(verts, i1));
-    t.v[2] = D(ld3(verts, i2));
-    d3 cr = crossd(t.v[1] - t.v[0], t.v[2] - t.v[0]);
-    t.jac = sqrt(dotd(cr, cr));
-    t.n = {cr.x / t.jac, cr.y / t.jac, cr.z / t.jac};
-    return t;
-}
-__device__ inline d3 l2g(double x1, double x2, const d3 *v) { return (1 - x1 - x2) * v[0] + x1 * v[1] + x2 * v[2]; }
-__device__ inline d3 l2g2(double x1, double x2, const d3 *v) { return (1 - x1) * v[0] + (x1 - x2) * v[1] + x2 * v[2]; }
 
 // potential_integrand over a batch of scaled targets (integrand.h:209-222)
 __device__ inline cdbl f2p64(const TriD &t, const d3 *pts, const double *scale, int npts, double kr, double ki,
@@ -271,14 +248,6 @@ __device__ inline void quantize_store(const float *s, const float *d, uint32_t *
     *sc = make_float2(ms, md);
 }
 
-__device__ inline d3 cell_center_d(int x, int y, int z, const WeightArgs &a)
-{
-    return {((double)x + 0.5) * a.h + a.minx, ((double)y + 0.5) * a.h + a.miny, ((double)z + 0.5) * a.h + a.minz};
-}
-__device__ inline f3 cell_center_f(int x, int y, int z, const WeightArgs &a)
-{
-    return f3{((float)x + 0.5f) * a.h, ((float)y + 0.5f) * a.h, ((float)z + 0.5f) * a.h} + f3{a.minx, a.miny, a.minz};
-}
 
 
 // A device-side count, or 0 when the mesh update has flagged an error (an invalid vertex
@@ -287,28 +256,6 @@ __device__ inline f3 cell_center_f(int x, int y, int z, const WeightArgs &a)
 __device__ inline int live_count(const int *n_ptr, const int *err) { return *err != 0 ? 0 : *n_ptr; }
 
 // ---------------------------------------------------------------- grid cache
-// GridCache::need_recompute (pppm_cache.h:64-69); meta = center(3) normal(3) area per triangle
-__global__ __launch_bounds__(256) void k_grid_dirty(const TriGeom *__restrict__ geom, const float *__restrict__ meta,
-                                                    int nt, float h, int first, int *flag)
-{
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= nt)
-        return;
-    if (first)
-    {
-        flag[i] = 1;
-        return;
-    }
-    const TriGeom g = geom[i];
-    const float *m = meta + 7 * i;
-    const f3 oc = {m[0], m[1], m[2]}, on = {m[3], m[4], m[5]};
-    const float oa = m[6];
-    const bool d = (double)lengthf(oc - g.center) > RECOMPUTE_THRESHOLD * (double)h ||
-                   (double)lengthf(on - g.normal) > RECOMPUTE_THRESHOLD ||
-                   (double)fabsf(oa - g.area) > RECOMPUTE_THRESHOLD * (double)g.area;
-    flag[i] = d ? 1 : 0;
-}
-
 // Both layer kernels of one (source point, target) pair: they share r and e^{ikr}, so the
 // values are those of two layer64 calls.
 __device__ inline void layer64_pair(d3 src, d3 trg, d3 n, double kr, double ki, cdbl &g0, cdbl &g1)
@@ -512,27 +459,6 @@ __global__ __launch_bounds__(B) void k_grid_finish(const TriGeom *__restrict__ g
 }
 
 // update_interpolation_weight_kernel (pppm_cache.cu:157-181), FP64 then rounded
-__global__ __launch_bounds__(256) void k_interp(const float *__restrict__ verts, const int *__restrict__ tris,
-                                                const TriGeom *__restrict__ geom, WeightArgs a,
-                                                float *__restrict__ interp, const int *__restrict__ err)
-{
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= a.nt || *err != 0)  // vertex indices are only trusted once validated
-        return;
-    const TriD T = tri_d(verts, tris[3 * i], tris[3 * i + 1], tris[3 * i + 2]);
-    const d3 p = l2g(TRI1_X, TRI1_X, T.v);
-    const int3_ b = geom[i].base3;
-    const d3 bc = cell_center_d(b.x, b.y, b.z, a);
-    const double h = a.h;
-    const double dx = (p.x - bc.x) / h, dy = (p.y - bc.y) / h, dz = (p.z - bc.z) / h;
-    const double w[8] = {(1 - dx) * (1 - dy) * (1 - dz), (1 - dx) * (1 - dy) * dz, (1 - dx) * dy * (1 - dz),
-                         (1 - dx) * dy * dz,             dx * (1 - dy) * (1 - dz), dx * (1 - dy) * dz,
-                         dx * dy * (1 - dz),             dx * dy * dz};
-    const double sc = 0.5 * TRI1_W * T.jac;
-    for (int q = 0; q < 8; q++)
-        interp[8 * i + q] = (float)(sc * w[q]);
-}
-
 // ---------------------------------------------------------------- face cache
 __device__ inline unsigned hash64(unsigned long long k, int cap)
 {
@@ -542,7 +468,8 @@ constexpr unsigned long long EMPTY_KEY = ~0ull;
 
 // compute_face_compute_list_kernel + get_cache_element_with_check (pppm_cache.cu:183-232,
 // pppm_cache.h:134-147): (c, j) -> persistent slot; dirty if new or moved past the thresholds.
-// Pairs past the live count write dirty = 0 (the flags are scanned over the capacity).
+// dirty[p] is 0 or 1; pairs past the live count write 0 (the flags are scanned over the
+// capacity).
 __global__ __launch_bounds__(256) void k_face_lookup(const TriGeom *__restrict__ geom, const int *__restrict__ tris,
                                                      const int *__restrict__ owner, const int *__restrict__ face_nbr,
                                                      const int *__restrict__ n_ptr, int cap,
@@ -624,9 +551,7 @@ __global__ __launch_bounds__(256) void k_face_lookup(const TriGeom *__restrict__
         // leaves the pair dirty for the next one
         if (fresh)
             fmeta[slot] = make_float2(INFINITY, INFINITY);
-        const int nn = common_vertices(tris[3 * j], tris[3 * j + 1], tris[3 * j + 2], tris[3 * c], tris[3 * c + 1],
-                                       tris[3 * c + 2]);
-        dirty[p] = 1 + (nn > 3 ? 3 : nn);
+        dirty[p] = 1;
     }
     else
         dirty[p] = 0;
@@ -780,11 +705,6 @@ void tdbem_init(Tdbem &b, float dt, float c, float grid_size)
 size_t grid_tmp_bytes(int max_dirty) { return (size_t)max_dirty * GC * 9 * 4 * sizeof(cdbl); }
 size_t face_tmp_bytes(int max_dirty) { return (size_t)max_dirty * 18 * sizeof(cdbl); }
 
-hipError_t launch_grid_dirty(const TriGeom *geom, float *meta, int nt, float h, int first, int *flag, hipStream_t s)
-{
-    k_grid_dirty<<<nb(nt, 256), 256, 0, s>>>(geom, meta, nt, h, first, flag);
-    return hipGetLastError();
-}
 hipError_t launch_grid_weights(const float *verts, const int *tris, const TriGeom *geom, const int *list,
                                const int *n_ptr, int cap, const WeightArgs &a, const Tdbem &b, void *tmp,
                                uint32_t *wf, float2 *sf, uint32_t *wb, float2 *sb, float *meta, const int *err,
@@ -795,12 +715,6 @@ hipError_t launch_grid_weights(const float *verts, const int *tris, const TriGeo
     k_grid_integrals<<<(unsigned)blocks, GB, 0, s>>>(verts, tris, geom, list, n_ptr, a, b, (cdbl *)tmp, err);
     k_grid_finish<<<nb_hint(hint * GC, (long long)cap * GC, B), B, 0, s>>>(geom, list, n_ptr, b, (const cdbl *)tmp, wf, sf, wb,
                                                                     sb, meta, err);
-    return hipGetLastError();
-}
-hipError_t launch_interp_weights(const float *verts, const int *tris, const TriGeom *geom, const WeightArgs &a,
-                                 float *interp, const int *err, hipStream_t s)
-{
-    k_interp<<<nb(a.nt, 256), 256, 0, s>>>(verts, tris, geom, a, interp, err);
     return hipGetLastError();
 }
 hipError_t launch_face_lookup(const TriGeom *geom, const int *tris, const int *owner, const int *face_nbr,
