@@ -75,6 +75,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--static-geometry", action="store_true",
+                    help="profiling variant: the moving workload's first frame held static, steps replayed "
+                         "through the captured graph (step kernels only, no mesh updates)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after the run, copy a buffer of this many MiB five times (a plain copy kernel of "
+                         "known size, to scale FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design note, "
@@ -94,6 +100,9 @@ def main():
     W = args.warmup
     total = W + K + args.timing_steps
     sc = scenes.c2_moving(steps=total) if args.workload == "c2_moving" else scenes.c1_static_sphere(steps=total)
+    if args.static_geometry:
+        sc.motion = None
+        sc.name = sc.name + "_static_geometry"
     moving = sc.motion is not None
     N = sc.res
 
@@ -137,6 +146,15 @@ def main():
         solver.set_timing(False)
         phases = solver.get_timing()
         stream.synchronize()
+
+    if args.calib_copy_mib > 0:
+        n = args.calib_copy_mib * (1 << 20) // 4
+        a = torch.ones(n, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     listener = float(solver.read_cells(lis)[0])
     ms_step = elapsed * 1e3 / K

@@ -16,12 +16,11 @@
 //   k_tri_update   geometry, cell / base-cell counts, 3x3x3 marks, grid-cache dirty flags,
 //                  interpolation weights (and the vertex copy)
 //   [scan batch]   cell CSR offsets; compaction of base cells, marked cells, dirty triangles
-//   k_lists_count  cell fill (atomic slots), 3x3x3 and 4x4x4 list lengths
-//   [scan batch]   3x3x3 and 4x4x4 list offsets
-//   k_lists_fill   3x3x3 and 4x4x4 lists (each cell's triangles ranked on the fly), face
-//                  counts, corner map, gather wave map, and the reset of the counters
-//   [scan]         face-list offsets
-//   k_faces_fill   per-triangle face lists and their gather wave map
+//   k_lists_count  cell fill (atomic slots), 3x3x3 list lengths of the marked cells, face-list
+//                  lengths (the 4x4x4 block around each triangle's base cell)
+//   [scan batch]   3x3x3 and face-list offsets
+//   k_lists_fill   3x3x3 lists and face lists (each cell's triangles sorted on the fly),
+//                  corner map, both gather wave maps, and the reset of the counters
 // The counters (cell and base counts, fill slots, marks) are zero when an update starts and
 // are cleared again by k_lists_fill for the cells this update touched, so no N^3 clear is
 // needed per update. Every launch has a host-known size (a capacity); the live counts stay
@@ -31,6 +30,7 @@
 #include "tri_device.hpp"
 
 #include <algorithm>
+#include <climits>
 
 namespace pppm_amd
 {
@@ -116,11 +116,23 @@ __device__ inline int nbr_range(int c, int N, const int *__restrict__ cell_start
     return e - s;
 }
 
-// width^3 list lengths of the live segment cells, waves striding over them (wave index wv of
-// nw); the capacity tail [n, cap) is zeroed by the range's threads (thread index tid of nth)
-template <int WIDTH>
+// The segments of a width^3 list: the listed cells (cells[q]), or, for TRI, the base cells of
+// the triangles (geom[q].base; the per-triangle face lists).
+template <bool TRI>
+__device__ inline int seg_cell(const int *__restrict__ cells, const TriGeom *__restrict__ geom, int q)
+{
+    if constexpr (TRI)
+        return geom[q].base;
+    else
+        return cells[q];
+}
+
+// width^3 list lengths of the live segments, waves striding over them (wave index wv of nw);
+// the capacity tail [n, cap) is zeroed by the range's threads (thread index tid of nth)
+template <int WIDTH, bool TRI>
 __device__ inline void nbr_count(int wv, int nw, long long tid, long long nth, const int *__restrict__ cells,
-                                 int n, int cap, const int *__restrict__ cell_start, int N, int *cnt)
+                                 const TriGeom *__restrict__ geom, int n, int cap,
+                                 const int *__restrict__ cell_start, int N, int *cnt)
 {
     const int lane = threadIdx.x & 63;
     for (long long q = n + tid; q < cap; q += nth)
@@ -128,7 +140,7 @@ __device__ inline void nbr_count(int wv, int nw, long long tid, long long nth, c
     for (int q = wv; q < n; q += nw)
     {
         int s, o;
-        const int len = nbr_range<WIDTH>(cells[q], N, cell_start, lane, s, o);
+        const int len = nbr_range<WIDTH>(seg_cell<TRI>(cells, geom, q), N, cell_start, lane, s, o);
         const int incl = wave_incl_scan(len);
         if (lane == 63)
             cnt[q] = incl;
@@ -136,17 +148,18 @@ __device__ inline void nbr_count(int wv, int nw, long long tid, long long nth, c
 }
 
 // fills the width^3 lists in lane order; the triangles of a neighbour cell were placed by
-// atomic slots, so each is written at its rank among them (ascending triangle index)
-template <int WIDTH>
-__device__ inline void nbr_fill(int wv, int nw, const int *__restrict__ cells, int n,
-                                const int *__restrict__ cell_start, const int *__restrict__ cell_items, int N,
-                                const int *__restrict__ out_start, int out_cap, int *out, int *err_flag)
+// atomic slots, so each is written at its rank among them (ascending triangle index). With
+// owner != null the segment index is written beside every entry.
+template <int WIDTH, bool TRI>
+__device__ inline void nbr_fill(int wv, int nw, const int *__restrict__ cells, const TriGeom *__restrict__ geom,
+                                int n, const int *__restrict__ cell_start, const int *__restrict__ cell_items, int N,
+                                const int *__restrict__ out_start, int out_cap, int *out, int *owner, int *err_flag)
 {
     const int lane = threadIdx.x & 63;
     for (int q = wv; q < n; q += nw)
     {
         int s, o;
-        const int len = nbr_range<WIDTH>(cells[q], N, cell_start, lane, s, o);
+        const int len = nbr_range<WIDTH>(seg_cell<TRI>(cells, geom, q), N, cell_start, lane, s, o);
         const int incl = wave_incl_scan(len);
         const int base = out_start[q] + incl - len;
         const bool fits = out_start[q] + incl <= out_cap;
@@ -154,14 +167,42 @@ __device__ inline void nbr_fill(int wv, int nw, const int *__restrict__ cells, i
             atomicOr(err_flag, ERR_CAPACITY);
         if (!fits)
             continue;
-        for (int k = 0; k < len; k++)
+        if (len <= 8)
         {
-            const int tri = cell_items[s + k];
-            int r = 0;
-            for (int m = 0; m < len; m++)
-                r += cell_items[s + m] < tri ? 1 : 0;
-            out[base + r] = WIDTH == 3 ? tri * 27 + o : tri;
+            // up to 8 triangles: loaded together, sorted in registers (odd-even transposition)
+            int v[8];
+#pragma unroll
+            for (int m = 0; m < 8; m++)
+                v[m] = m < len ? cell_items[s + m] : INT_MAX;
+#pragma unroll
+            for (int r = 0; r < 8; r++)
+#pragma unroll
+                for (int m = r & 1; m + 1 < 8; m += 2)
+                {
+                    const int lo = min(v[m], v[m + 1]), hi = max(v[m], v[m + 1]);
+                    v[m] = lo;
+                    v[m + 1] = hi;
+                }
+#pragma unroll
+            for (int m = 0; m < 8; m++)
+                if (m < len)
+                {
+                    out[base + m] = WIDTH == 3 ? v[m] * 27 + o : v[m];
+                    if (owner)
+                        owner[base + m] = q;
+                }
         }
+        else
+            for (int k = 0; k < len; k++)
+            {
+                const int tri = cell_items[s + k];
+                int r = 0;
+                for (int m = 0; m < len; m++)
+                    r += cell_items[s + m] < tri ? 1 : 0;
+                out[base + r] = WIDTH == 3 ? tri * 27 + o : tri;
+                if (owner)
+                    owner[base + r] = q;
+            }
     }
 }
 
@@ -212,7 +253,8 @@ __global__ __launch_bounds__(B) void k_tri_update(const float *__restrict__ src,
         interp_weights(src, tris, i, g.base3, wa, interp + 8 * i);
 }
 
-// block ranges: [0, r1) cell fill, [r1, r2) 3x3x3 lengths, [r2, grid) 4x4x4 lengths
+// block ranges: [0, r1) cell fill, [r1, r2) 3x3x3 list lengths of the marked cells,
+// [r2, grid) face-list lengths (the 4x4x4 block around each triangle's base cell)
 __global__ __launch_bounds__(B) void k_lists_count(ListArgs a, int r1, int r2)
 {
     const int b = blockIdx.x;
@@ -229,28 +271,29 @@ __global__ __launch_bounds__(B) void k_lists_count(ListArgs a, int r1, int r2)
     else if (b < r2)
     {
         const int k = b - r1, nb = r2 - r1;
-        nbr_count<3>(k * WPB + (threadIdx.x >> 6), nb * WPB, (long long)k * B + threadIdx.x, (long long)nb * B,
-                     a.c3_cells, min(*a.c3_n, a.cap_c3), a.cap_c3, a.cell_start, a.N, a.c3_cnt);
+        nbr_count<3, false>(k * WPB + (threadIdx.x >> 6), nb * WPB, (long long)k * B + threadIdx.x, (long long)nb * B,
+                     a.c3_cells, nullptr, min(*a.c3_n, a.cap_c3), a.cap_c3, a.cell_start, a.N, a.c3_cnt);
     }
     else
     {
         const int k = b - r2, nb = (int)gridDim.x - r2;
-        nbr_count<4>(k * WPB + (threadIdx.x >> 6), nb * WPB, (long long)k * B + threadIdx.x, (long long)nb * B,
-                     a.b4_cells, min(*a.b4_n, a.nt), a.nt, a.cell_start, a.N, a.b4_cnt);
+        nbr_count<4, true>(k * WPB + (threadIdx.x >> 6), nb * WPB, (long long)k * B + threadIdx.x, (long long)nb * B,
+                     nullptr, a.geom, a.nt, a.nt, a.cell_start, a.N, a.face_cnt);
     }
 }
 
-// block ranges: [0, r1) 3x3x3 lists, [r1, r2) 4x4x4 lists, [r2, r3) per triangle (face
-// counts, corner map, counter reset), [r3, grid) gather wave map of the 3x3x3 lists
-__global__ __launch_bounds__(B) void k_lists_fill(ListArgs a, int r1, int r2, int r3)
+// block ranges: [0, r1) 3x3x3 lists, [r1, r2) face lists with their owners, [r2, r3) per
+// triangle (corner map, counter reset), [r3, r4) gather wave map of the 3x3x3 lists,
+// [r4, grid) gather wave map of the face lists
+__global__ __launch_bounds__(B) void k_lists_fill(ListArgs a, int r1, int r2, int r3, int r4)
 {
     const int b = blockIdx.x;
     if (b < r1)
-        nbr_fill<3>(b * WPB + (threadIdx.x >> 6), r1 * WPB, a.c3_cells, min(*a.c3_n, a.cap_c3), a.cell_start,
-                    a.cell_items, a.N, a.c3_start, 27 * a.nt, a.c3_pairs, a.err);
+        nbr_fill<3, false>(b * WPB + (threadIdx.x >> 6), r1 * WPB, a.c3_cells, nullptr, min(*a.c3_n, a.cap_c3),
+                    a.cell_start, a.cell_items, a.N, a.c3_start, 27 * a.nt, a.c3_pairs, nullptr, a.err);
     else if (b < r2)
-        nbr_fill<4>((b - r1) * WPB + (threadIdx.x >> 6), (r2 - r1) * WPB, a.b4_cells, min(*a.b4_n, a.nt),
-                    a.cell_start, a.cell_items, a.N, a.b4_start, a.cap_b4items, a.b4_tris, a.err);
+        nbr_fill<4, true>((b - r1) * WPB + (threadIdx.x >> 6), (r2 - r1) * WPB, nullptr, a.geom, a.nt, a.cell_start,
+                    a.cell_items, a.N, a.face_start, a.cap_p4, a.face_nbr, a.face_owner, a.err);
     else if (b < r3)
     {
         const int i = (b - r2) * B + threadIdx.x;
@@ -258,8 +301,6 @@ __global__ __launch_bounds__(B) void k_lists_fill(ListArgs a, int r1, int r2, in
             return;
         const TriGeom g = a.geom[i];
         const int N = a.N;
-        const int bi = a.b4_index[g.base];
-        a.face_cnt[i] = a.b4_start[bi + 1] - a.b4_start[bi];
         for (int w = 0; w < 8; w++)
             a.corner[8 * i + w] = a.c3_index[lin3(g.base3.x + (w >> 2), g.base3.y + ((w >> 1) & 1),
                                                    g.base3.z + (w & 1), N)];
@@ -270,56 +311,35 @@ __global__ __launch_bounds__(B) void k_lists_fill(ListArgs a, int r1, int r2, in
         for (int o = 0; o < GC; o++)
             a.cn.mark[lin3(g.cell3.x + o % 3 - 1, g.cell3.y + o / 3 % 3 - 1, g.cell3.z + o / 9 - 1, N)] = 0;
     }
-    else
+    else if (b < r4)
     {
         const int w = (b - r3) * B + threadIdx.x;
         if (w <= a.c3_waves)
             wave_map(w, a.c3_start, min(*a.c3_n, a.cap_c3), a.c3_wave);
     }
-}
-
-// block ranges: [0, r1) one wave per triangle copies the 4x4x4 list of its base cell and the
-// owner index; [r1, grid) gather wave map of the face lists
-__global__ __launch_bounds__(B) void k_faces_fill(ListArgs a, int r1)
-{
-    const int b = blockIdx.x;
-    if (b >= r1)
+    else
     {
-        const int w = (b - r1) * B + threadIdx.x;
+        const int w = (b - r4) * B + threadIdx.x;
         if (w <= a.face_waves)
             wave_map(w, a.face_start, a.nt, a.face_wave);
-        return;
     }
-    const int i = b * WPB + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    // an earlier overflow (4x4x4 lists) leaves b4_tris incomplete: copy nothing
-    if (i >= a.nt || *a.err != 0)
-        return;
-    const int bi = a.b4_index[a.geom[i].base];
-    const int s = a.b4_start[bi], e = a.b4_start[bi + 1];
-    const int w = a.face_start[i];
-    if (w + (e - s) > a.cap_p4)
-    {
-        if (lane == 0)
-            atomicOr(a.err, ERR_CAPACITY);
-        return;
-    }
-    for (int k = lane; k < e - s; k += WAVE)
-    {
-        a.face_nbr[w + k] = a.b4_tris[s + k];
-        a.face_owner[w + k] = i;
-    }
+}
+
+// 4x4x4 lists of the non-empty base cells (for inspection; the time step reads the
+// per-triangle face lists): lengths, then, after their scan, the lists
+__global__ __launch_bounds__(B) void k_b4_lists(ListArgs a, int fill)
+{
+    const int nb = (int)gridDim.x;
+    const int n = min(*a.b4_n, a.nt);
+    if (!fill)
+        nbr_count<4, false>(blockIdx.x * WPB + (threadIdx.x >> 6), nb * WPB, (long long)blockIdx.x * B + threadIdx.x,
+                     (long long)nb * B, a.b4_cells, nullptr, n, a.nt, a.cell_start, a.N, a.b4_cnt);
+    else
+        nbr_fill<4, false>(blockIdx.x * WPB + (threadIdx.x >> 6), nb * WPB, a.b4_cells, nullptr, n, a.cell_start,
+                    a.cell_items, a.N, a.b4_start, a.cap_b4items, a.b4_tris, nullptr, a.err);
 }
 
 __global__ void k_advance(StepState *st) { st->t = st->t + 1; }
-
-// gathers scalar counters that live in device memory into one status block
-__global__ void k_status(StatusSrc src, int *out)
-{
-    const int i = threadIdx.x;
-    if (i < src.n)
-        out[i] = src.p[i] ? *src.p[i] : 0;
-}
 
 inline int nblk(long long n) { return n > 0 ? (int)((n + B - 1) / B) : 1; }
 inline int nblk_waves(long long waves) { return waves > 0 ? (int)((waves + WPB - 1) / WPB) : 1; }
@@ -362,24 +382,20 @@ hipError_t launch_lists_count(const ListArgs &a, hipStream_t s)
 }
 hipError_t launch_lists_fill(const ListArgs &a, hipStream_t s)
 {
-    const int n3 = nblk_strided(a.cap_c3), n4 = nblk_strided(a.nt), nt = nblk(a.nt), nm = nblk(a.c3_waves + 1LL);
-    k_lists_fill<<<n3 + n4 + nt + nm, B, 0, s>>>(a, n3, n3 + n4, n3 + n4 + nt);
+    const int n3 = nblk_strided(a.cap_c3), nf = nblk_strided(a.nt), nt = nblk(a.nt);
+    const int m3 = nblk(a.c3_waves + 1LL), mf = nblk(a.face_waves + 1LL);
+    const int r1 = n3, r2 = r1 + nf, r3 = r2 + nt, r4 = r3 + m3;
+    k_lists_fill<<<r4 + mf, B, 0, s>>>(a, r1, r2, r3, r4);
     return hipGetLastError();
 }
-hipError_t launch_faces_fill(const ListArgs &a, hipStream_t s)
+hipError_t launch_b4_lists(const ListArgs &a, int fill, hipStream_t s)
 {
-    const int r1 = nblk_waves(a.nt), nm = nblk(a.face_waves + 1LL);
-    k_faces_fill<<<r1 + nm, B, 0, s>>>(a, r1);
+    k_b4_lists<<<nblk_strided(a.nt), B, 0, s>>>(a, fill);
     return hipGetLastError();
 }
 hipError_t launch_advance(StepState *st, hipStream_t s)
 {
     k_advance<<<1, 1, 0, s>>>(st);
-    return hipGetLastError();
-}
-hipError_t launch_status(const StatusSrc &src, int *out, hipStream_t s)
-{
-    k_status<<<1, 64, 0, s>>>(src, out);
     return hipGetLastError();
 }
 

@@ -111,14 +111,9 @@ hipError_t launch_tri_update(const float *src, float *dst, int nv, const int *tr
                              const UpdateCounters &cn, int *err, hipStream_t s);
 hipError_t launch_lists_count(const ListArgs &a, hipStream_t s);
 hipError_t launch_lists_fill(const ListArgs &a, hipStream_t s);
-hipError_t launch_faces_fill(const ListArgs &a, hipStream_t s);
+// 4x4x4 lists of the non-empty base cells: fill = 0 counts (b4_cnt), fill = 1 writes b4_tris
+hipError_t launch_b4_lists(const ListArgs &a, int fill, hipStream_t s);
 hipError_t launch_advance(StepState *st, hipStream_t s);
-struct StatusSrc
-{
-    const int *p[16];
-    int n;
-};
-hipError_t launch_status(const StatusSrc &src, int *out, hipStream_t s);
 
 // ---------------------------------------------------------------- weights.hip (FP64 quadrature)
 struct Tdbem
@@ -126,22 +121,28 @@ struct Tdbem
     double lambda;
     double wave_re[STEPS], wave_im[STEPS];
     double tw_re[STEPS / 2], tw_im[STEPS / 2];  // exp(2 pi_ref i j / 16), j = 0..7 (bem.h:25)
+    double inv_pow[STEPS];                       // lambda^-n as the running quotient 1 / lambda / ... (bem.h:300-303)
     float threshold;
 };
 void tdbem_init(Tdbem &b, float dt, float c, float grid_size);
 
-size_t grid_tmp_bytes(int max_dirty);
 size_t face_tmp_bytes(int max_dirty);
 // records tri*27+o of the *n_ptr triangles in list (cap bounds *n_ptr). hint_dirty: host
 // estimate of *n_ptr that sizes the grid-stride launch (-1: cap); any value is correct.
 hipError_t launch_grid_weights(const float *verts, const int *tris, const TriGeom *geom, const int *list,
-                               const int *n_ptr, int cap, const WeightArgs &a, const Tdbem &b, void *tmp,
-                               uint32_t *wf, float2 *sf, uint32_t *wb, float2 *sb, float *meta, const int *err,
+                               const int *n_ptr, int cap, const WeightArgs &a, const Tdbem &b, uint32_t *wf,
+                               float2 *sf, uint32_t *wb, float2 *sb, float *meta, const int *err,
                                long long hint_dirty, hipStream_t s);
+// face lists and slots of the previous update (start == null: none); cur_start = this update's
+// face-list offsets
+struct PrevFaces
+{
+    const int *start, *nbr, *slot, *cur_start;
+};
 hipError_t launch_face_lookup(const TriGeom *geom, const int *tris, const int *owner, const int *face_nbr,
                               const int *n_ptr, int cap, unsigned long long *hkeys, int *hvals, int hcap,
                               int *pool_count, int pool_cap, float2 *fmeta, int *face_slot, int *dirty, int *err_flag,
-                              int nt, hipStream_t s);
+                              int nt, const PrevFaces &prev, hipStream_t s);
 hipError_t launch_face_weights(const float *verts, const int *tris, const TriGeom *geom, const int *dirty_pairs,
                                const int *n_ptr, int cap, const int *owner, const int *face_nbr, const int *face_slot,
                                const float *interp, const WeightArgs &a, const Tdbem &b, void *tmp, uint32_t *fw,

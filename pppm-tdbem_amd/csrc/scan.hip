@@ -55,22 +55,15 @@ __device__ inline const ScanJob &job_of(const ScanBatch &b, int tile)
     return b.job[k];
 }
 
-template <int IT>
-__device__ inline void load_tile(const ScanJob &J, int tile, int v[IT], int &tsum)
+__device__ inline int item_of(const ScanJob &J, long long q)
 {
-    const long long base = (long long)tile * SB * IT + (long long)threadIdx.x * IT;
-    tsum = 0;
-#pragma unroll
-    for (int i = 0; i < IT; i++)
-    {
-        const long long q = base + i;
-        int x = q < J.n ? J.in[q] : 0;
-        if (J.flag_mode)
-            x = x != 0 ? 1 : 0;
-        v[i] = x;
-        tsum += x;
-    }
+    const int x = q < J.n ? J.in[q] : 0;
+    return J.flag_mode ? (x != 0 ? 1 : 0) : x;
 }
+
+// LDS slot of tile item j: one pad word per 32, so the blocked reads (thread t reads items
+// t*IT .. t*IT+IT-1) hit distinct banks
+__device__ inline int pad(int j) { return j + (j >> 5); }
 
 // block-wide exclusive offset of this thread's items and the block total
 __device__ inline int block_scan(int tsum, int &btotal)
@@ -93,59 +86,89 @@ __device__ inline int block_scan(int tsum, int &btotal)
     return woff + incl - tsum;
 }
 
+// tile sums: striped (coalesced) loads, any order of integer addition
 __global__ __launch_bounds__(SB) void k_tile_sums(ScanBatch b, int *__restrict__ sums)
 {
     const ScanJob &J = job_of(b, blockIdx.x);
-    int v[SI], tsum, btotal;
-    load_tile<SI>(J, blockIdx.x - J.tile0, v, tsum);
+    const long long base = (long long)(blockIdx.x - J.tile0) * SPB;
+    int tsum = 0;
+#pragma unroll
+    for (int i = 0; i < SI; i++)
+        tsum += item_of(J, base + i * SB + threadIdx.x);
+    int btotal;
     (void)block_scan(tsum, btotal);
     if (threadIdx.x == 0)
         sums[blockIdx.x] = btotal;
 }
 
-// sums == null: every job is a single tile (one block per job). IT items per thread.
+// sums == null: every job is a single tile (one block per job). IT items per thread. Items
+// are loaded striped (coalesced) into LDS, scanned in blocked order, and the positions go
+// back through LDS for striped stores.
 template <int IT>
 __global__ __launch_bounds__(SB) void k_tile_scan(ScanBatch b, const int *__restrict__ sums)
 {
+    constexpr int T = SB * IT;
     __shared__ int s_off;
+    __shared__ int sv[T + T / 32];
     const ScanJob &J = sums ? job_of(b, blockIdx.x) : b.job[blockIdx.x];
     const int tile = sums ? (int)blockIdx.x - J.tile0 : 0;
-    if (threadIdx.x < 64)
+    const long long base = (long long)tile * T;
+    const int tid = threadIdx.x;
+    if (tid < 64)
     {
         int acc = 0;
         if (sums)
             for (int t0 = J.tile0; t0 < (int)blockIdx.x; t0 += 64)
             {
-                const int t = t0 + (int)threadIdx.x;
+                const int t = t0 + tid;
                 acc += t < (int)blockIdx.x ? sums[t] : 0;
             }
         acc = wave_total(acc);
-        if (threadIdx.x == 0)
+        if (tid == 0)
             s_off = acc;
     }
-    int v[IT], tsum, btotal;
-    load_tile<IT>(J, tile, v, tsum);
-    int run = block_scan(tsum, btotal);  // its barrier also publishes s_off
-    const int off = s_off;
-    run += off;
-    const long long base = (long long)tile * SB * IT + (long long)threadIdx.x * IT;
+#pragma unroll
+    for (int i = 0; i < IT; i++)
+        sv[pad(i * SB + tid)] = item_of(J, base + i * SB + tid);
+    __syncthreads();
+    int v[IT], tsum = 0;
 #pragma unroll
     for (int i = 0; i < IT; i++)
     {
-        const long long q = base + i;
-        if (q < J.n)
-        {
-            if (J.pos)
-                J.pos[q] = run;
-            if (J.list && v[i])
-                J.list[run] = (int)q;
-            if (J.index_map)
-                J.index_map[q] = v[i] ? run : -1;
-        }
+        v[i] = sv[pad(tid * IT + i)];
+        tsum += v[i];
+    }
+    int btotal;
+    int run = block_scan(tsum, btotal);  // its barrier also publishes s_off
+    const int off = s_off;
+    run += off;
+    // blocked pass: compaction lists, and (position << 1 | selected) back into LDS
+#pragma unroll
+    for (int i = 0; i < IT; i++)
+    {
+        const long long q = base + tid * IT + i;
+        if (J.list && v[i] && q < J.n)
+            J.list[run] = (int)q;
+        sv[pad(tid * IT + i)] = (run << 1) | (v[i] != 0 ? 1 : 0);
         run += v[i];
     }
-    const long long tiles = ((long long)J.n + SB * IT - 1) / (SB * IT);
-    if (tile == (tiles > 0 ? tiles - 1 : 0) && threadIdx.x == 0)
+    __syncthreads();
+    if (J.pos || J.index_map)
+#pragma unroll
+        for (int i = 0; i < IT; i++)
+        {
+            const long long q = base + i * SB + tid;
+            if (q < J.n)
+            {
+                const int e = sv[pad(i * SB + tid)];
+                if (J.pos)
+                    J.pos[q] = e >> 1;
+                if (J.index_map)
+                    J.index_map[q] = (e & 1) ? (e >> 1) : -1;
+            }
+        }
+    const long long tiles = ((long long)J.n + T - 1) / T;
+    if (tile == (tiles > 0 ? tiles - 1 : 0) && tid == 0)
     {
         if (J.pos)
             J.pos[J.n] = off + btotal;
@@ -180,7 +203,7 @@ hipError_t scan_batch(ScanBatch b, int *tmp, hipStream_t s)
     int tiles = 0;
     for (int k = 0; k < b.njobs; k++)
     {
-        if (b.job[k].n < 0)
+        if (b.job[k].n < 0 || b.job[k].n >= (1 << 30))  // positions are kept as (pos << 1 | bit)
             return hipErrorInvalidValue;
         small = small && b.job[k].n <= SB * SI1;
         b.job[k].tile0 = tiles;

@@ -13,6 +13,7 @@
 #include "pppm_hip.h"
 
 #include <algorithm>
+#include <utility>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -52,6 +53,11 @@ struct DBuf
         n = m;
         return hipSuccess;
     }
+    void swap(DBuf &o)
+    {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+    }
     void release()
     {
         if (p)
@@ -78,29 +84,20 @@ struct LWH72
 };
 static_assert(sizeof(LWH72) == 72, "LayerWeightHalf layout");
 
-// status block layout
+// device status block layout: error bits, live counts written by the update's scans, the
+// face-cache pool counter
 enum
 {
     ST_ERR = 0,
-    ST_C3 = 1,
-    ST_B4 = 2,
-    ST_B4ITEMS = 3,
-    ST_P4 = 4,
-    ST_GDIRTY = 5,
-    ST_FDIRTY = 6,
-    ST_POOL = 7,
+    ST_C3 = 1,      // marked (3x3x3) cells
+    ST_B4 = 2,      // non-empty base cells
+    ST_P4 = 3,      // face pairs
+    ST_GDIRTY = 4,  // grid-cache triangles to recompute
+    ST_FDIRTY = 5,  // face pairs to recompute
+    ST_POOL = 6,    // face-cache slots in use
     ST_N = 8
 };
 
-// live counts written by the update's scans
-enum
-{
-    LV_C3 = 0,  // marked (3x3x3) cells
-    LV_B4 = 1,  // non-empty base cells
-    LV_G = 2,   // grid-cache triangles to recompute
-    LV_F = 3,   // face pairs to recompute
-    LV_N = 4
-};
 
 }  // namespace
 
@@ -136,14 +133,20 @@ struct pppm_solver
     bool counters_zero = false;
     DBuf<int> cell_start, cell_items, cell_rank;
     DBuf<int> scan_tmp;
-    DBuf<int> live;  // device-side live counts of a mesh update (LV_*)
     DBuf<int> c3_cells, c3_index, c3_cnt, c3_start, c3_pairs, c3_wave;
     DBuf<int> b4_cells, b4_index, b4_cnt, b4_start, b4_tris;
     DBuf<int> face_cnt, face_start, face_nbr, face_slot, face_owner, face_flag, dirty_list;
+    // face lists and slots of the previous update (swapped with the current ones per update):
+    // the lookup takes a pair's slot from there when the pair sits at the same place
+    DBuf<int> prev_face_start, prev_face_nbr, prev_face_slot;
+    bool faces_valid = false;  // face_start / face_nbr / face_slot hold a completed update
     DBuf<int> face_wave, corner;
     DBuf<int> gflag, glist;
-    DBuf<char> gtmp, ftmp;
-    DBuf<int> errflag, status;
+    DBuf<char> ftmp;
+    // device status block (ST_*): error bits, the live counts the update's scans write, the
+    // face-cache pool counter; the host reads it with one copy per update
+    DBuf<int> status;
+    bool err_clean = false;  // status[ST_ERR] known to be 0
 
     DBuf<uint32_t> wf, wb;
     DBuf<float2> sf, sb;
@@ -153,7 +156,7 @@ struct pppm_solver
     DBuf<uint32_t> fw;
     DBuf<float2> fsc, fmeta;
     DBuf<unsigned long long> hkeys;
-    DBuf<int> hvals, pool_counter;
+    DBuf<int> hvals;
 
     DBuf<float> hn, hd, far, nstage;
     DBuf<float> wn, wd;  // per-step history windows, 16 floats per triangle and field
@@ -327,7 +330,6 @@ int allocate(pppm_solver *s)
     CK(s->cell_items.ensure(nt));
     CK(s->cell_rank.ensure(nt));
     CK(s->scan_tmp.ensure(std::max(scan_batch_tmp_ints(batch_n, 4), scan_tmp_ints(NN))));
-    CK(s->live.ensure(LV_N));
     CK(s->c3_cells.ensure(s->cap_c3));
     CK(s->c3_index.ensure(NN));
     CK(s->c3_cnt.ensure(s->cap_c3));
@@ -344,6 +346,10 @@ int allocate(pppm_solver *s)
     CK(s->face_start.ensure(nt + 1));
     CK(s->face_nbr.ensure(s->cap_p4));
     CK(s->face_slot.ensure(s->cap_p4));
+    CK(s->prev_face_start.ensure(nt + 1));
+    CK(s->prev_face_nbr.ensure(s->cap_p4));
+    CK(s->prev_face_slot.ensure(s->cap_p4));
+    s->faces_valid = false;  // buffers may have moved
     CK(s->face_owner.ensure(s->cap_p4));
     CK(s->face_flag.ensure(s->cap_p4));
     CK(s->dirty_list.ensure(s->cap_p4));
@@ -351,9 +357,7 @@ int allocate(pppm_solver *s)
     CK(s->corner.ensure((size_t)nt * 8));
     CK(s->gflag.ensure(nt));
     CK(s->glist.ensure(nt));
-    CK(s->gtmp.ensure(grid_tmp_bytes(nt)));
     CK(s->ftmp.ensure(face_tmp_bytes(s->cap_p4)));
-    CK(s->errflag.ensure(1));
     CK(s->status.ensure(16));
     return PPPM_OK;
 }
@@ -404,17 +408,56 @@ int ensure_pool(pppm_solver *s, int need)
     return PPPM_OK;
 }
 
+ListArgs list_args(pppm_solver *s, const UpdateCounters &cn)
+{
+    int *stp = s->status.p;
+    ListArgs la{};
+    la.N = s->N;
+    la.nt = s->nt;
+    la.geom = s->geom.p;
+    la.cn = cn;
+    la.cell_start = s->cell_start.p;
+    la.cell_items = s->cell_items.p;
+    la.cell_rank = s->cell_rank.p;
+    la.c3_cells = s->c3_cells.p;
+    la.c3_n = stp + ST_C3;
+    la.c3_index = s->c3_index.p;
+    la.c3_cnt = s->c3_cnt.p;
+    la.c3_start = s->c3_start.p;
+    la.c3_pairs = s->c3_pairs.p;
+    la.c3_wave = s->c3_wave.p;
+    la.cap_c3 = s->cap_c3;
+    la.c3_waves = s->c3_waves;
+    la.b4_cells = s->b4_cells.p;
+    la.b4_n = stp + ST_B4;
+    la.b4_index = s->b4_index.p;
+    la.b4_cnt = s->b4_cnt.p;
+    la.b4_start = s->b4_start.p;
+    la.b4_tris = s->b4_tris.p;
+    la.cap_b4items = s->cap_b4items;
+    la.face_cnt = s->face_cnt.p;
+    la.face_start = s->face_start.p;
+    la.face_nbr = s->face_nbr.p;
+    la.face_owner = s->face_owner.p;
+    la.face_wave = s->face_wave.p;
+    la.cap_p4 = s->cap_p4;
+    la.face_waves = s->face_waves;
+    la.corner = s->corner.p;
+    la.err = stp + ST_ERR;
+    return la;
+}
+
 // ---------------------------------------------------------------- the update sequence
 // Geometry, binning, lists and weight caches for the vertex positions in s->verts (or, with
 // a series, row t+1-t0 of it). Asynchronous: only kernels are enqueued.
 int enqueue_rebuild(pppm_solver *s, bool first, const float *verts_series, long long vstride, int t0,
-                    const float *vsrc = nullptr)
+                    const float *vsrc = nullptr, bool use_prev = false)
 {
     const int N = s->N, nt = s->nt;
     const int NN = N * N * N;
     hipStream_t st = s->stream;
-    int *err = s->errflag.p;
-    int *lv = s->live.p;
+    int *stp = s->status.p;
+    int *err = stp + ST_ERR;
 
     if (verts_series)
         CK(launch_copy_verts(verts_series, vstride, t0, s->state.p, s->verts.p, s->nv * 3, st));
@@ -436,96 +479,54 @@ int enqueue_rebuild(pppm_solver *s, bool first, const float *verts_series, long 
         ScanBatch b{};
         b.njobs = 4;
         b.job[0] = ScanJob{cn.cell_cnt, NN, 0, s->cell_start.p, nullptr, nullptr, nullptr, 0};
-        b.job[1] = ScanJob{cn.base_cnt, NN, 1, nullptr, s->b4_cells.p, s->b4_index.p, lv + LV_B4, 0};
-        b.job[2] = ScanJob{cn.mark, NN, 1, nullptr, s->c3_cells.p, s->c3_index.p, lv + LV_C3, 0};
-        b.job[3] = ScanJob{s->gflag.p, nt, 1, nullptr, s->glist.p, nullptr, lv + LV_G, 0};
+        b.job[1] = ScanJob{cn.base_cnt, NN, 1, nullptr, s->b4_cells.p, s->b4_index.p, stp + ST_B4, 0};
+        b.job[2] = ScanJob{cn.mark, NN, 1, nullptr, s->c3_cells.p, s->c3_index.p, stp + ST_C3, 0};
+        b.job[3] = ScanJob{s->gflag.p, nt, 1, nullptr, s->glist.p, nullptr, stp + ST_GDIRTY, 0};
         CK(scan_batch(b, s->scan_tmp.p, st));
     }
 
-    ListArgs la{};
-    la.N = N;
-    la.nt = nt;
-    la.geom = s->geom.p;
-    la.cn = cn;
-    la.cell_start = s->cell_start.p;
-    la.cell_items = s->cell_items.p;
-    la.cell_rank = s->cell_rank.p;
-    la.c3_cells = s->c3_cells.p;
-    la.c3_n = lv + LV_C3;
-    la.c3_index = s->c3_index.p;
-    la.c3_cnt = s->c3_cnt.p;
-    la.c3_start = s->c3_start.p;
-    la.c3_pairs = s->c3_pairs.p;
-    la.c3_wave = s->c3_wave.p;
-    la.cap_c3 = s->cap_c3;
-    la.c3_waves = s->c3_waves;
-    la.b4_cells = s->b4_cells.p;
-    la.b4_n = lv + LV_B4;
-    la.b4_index = s->b4_index.p;
-    la.b4_cnt = s->b4_cnt.p;
-    la.b4_start = s->b4_start.p;
-    la.b4_tris = s->b4_tris.p;
-    la.cap_b4items = s->cap_b4items;
-    la.face_cnt = s->face_cnt.p;
-    la.face_start = s->face_start.p;
-    la.face_nbr = s->face_nbr.p;
-    la.face_owner = s->face_owner.p;
-    la.face_wave = s->face_wave.p;
-    la.cap_p4 = s->cap_p4;
-    la.face_waves = s->face_waves;
-    la.corner = s->corner.p;
-    la.err = err;
+    const ListArgs la = list_args(s, cn);
 
-    // cell fill, 3x3x3 / 4x4x4 list lengths, their offsets, the lists themselves, face counts,
-    // corner map and the reset of the counters
+    // cell fill, 3x3x3 and face-list lengths (a triangle's face list is the 4x4x4 block around
+    // its base cell), their offsets, the lists themselves, corner map, gather wave maps and the
+    // reset of the counters
     CK(launch_lists_count(la, st));
     {
         ScanBatch b{};
         b.njobs = 2;
         b.job[0] = ScanJob{s->c3_cnt.p, s->cap_c3, 0, s->c3_start.p, nullptr, nullptr, nullptr, 0};
-        b.job[1] = ScanJob{s->b4_cnt.p, nt, 0, s->b4_start.p, nullptr, nullptr, nullptr, 0};
+        b.job[1] = ScanJob{s->face_cnt.p, nt, 0, s->face_start.p, nullptr, nullptr, stp + ST_P4, 0};
         CK(scan_batch(b, s->scan_tmp.p, st));
     }
     CK(launch_lists_fill(la, st));
     s->counters_zero = true;
-    // per-triangle face lists (the 4x4x4 list of its base cell)
-    RC(scan(s, s->face_cnt.p, s->face_start.p, nt));
-    CK(launch_faces_fill(la, st));
 
     // grid cache: dirty triangles only. Launch sizes follow the previous update's recompute
     // counts (the kernels stride, so a larger count is still covered); a first build sizes
     // for everything.
     const long long ghint = first ? -1 : 2 * s->last_grid_recomputes + 64;
     const long long fhint = first ? -1 : 2 * s->last_face_recomputes + 64;
-    CK(launch_grid_weights(s->verts.p, s->tris.p, s->geom.p, s->glist.p, lv + LV_G, nt, wa, s->bem, s->gtmp.p,
-                           s->wf.p, s->sf.p, s->wb.p, s->sb.p, s->gmeta.p, err, ghint, st));
+    CK(launch_grid_weights(s->verts.p, s->tris.p, s->geom.p, s->glist.p, stp + ST_GDIRTY, nt, wa, s->bem, s->wf.p,
+                           s->sf.p, s->wb.p, s->sb.p, s->gmeta.p, err, ghint, st));
 
     // face cache: persistent (c, j) slots, dirty pairs recomputed (FaceCache::update_cache)
     CK(launch_face_lookup(s->geom.p, s->tris.p, s->face_owner.p, s->face_nbr.p, s->face_start.p + nt, s->cap_p4,
-                          s->hkeys.p, s->hvals.p, s->hcap, s->pool_counter.p, s->pool_cap, s->fmeta.p, s->face_slot.p,
-                          s->face_flag.p, err, nt, st));
+                          s->hkeys.p, s->hvals.p, s->hcap, stp + ST_POOL, s->pool_cap, s->fmeta.p, s->face_slot.p,
+                          s->face_flag.p, err, nt,
+                          use_prev ? PrevFaces{s->prev_face_start.p, s->prev_face_nbr.p, s->prev_face_slot.p,
+                                               s->face_start.p}
+                                   : PrevFaces{nullptr, nullptr, nullptr, nullptr},
+                          st));
     {
         ScanBatch b{};
         b.njobs = 1;
-        b.job[0] = ScanJob{s->face_flag.p, s->cap_p4, 1, nullptr, s->dirty_list.p, nullptr, lv + LV_F, 0};
+        b.job[0] = ScanJob{s->face_flag.p, s->cap_p4, 1, nullptr, s->dirty_list.p, nullptr, stp + ST_FDIRTY, 0};
         CK(scan_batch(b, s->scan_tmp.p, st));
     }
-    CK(launch_face_weights(s->verts.p, s->tris.p, s->geom.p, s->dirty_list.p, lv + LV_F, s->cap_p4,
+    CK(launch_face_weights(s->verts.p, s->tris.p, s->geom.p, s->dirty_list.p, stp + ST_FDIRTY, s->cap_p4,
                            s->face_owner.p, s->face_nbr.p, s->face_slot.p, s->interp.p, wa, s->bem, s->ftmp.p, s->fw.p,
                            s->fsc.p, s->fmeta.p, err, fhint, st));
 
-    // status block
-    StatusSrc stsrc{};
-    stsrc.n = ST_N;
-    stsrc.p[ST_ERR] = err;
-    stsrc.p[ST_C3] = lv + LV_C3;
-    stsrc.p[ST_B4] = lv + LV_B4;
-    stsrc.p[ST_B4ITEMS] = s->b4_start.p + nt;
-    stsrc.p[ST_P4] = s->face_start.p + nt;
-    stsrc.p[ST_GDIRTY] = lv + LV_G;
-    stsrc.p[ST_FDIRTY] = lv + LV_F;
-    stsrc.p[ST_POOL] = s->pool_counter.p;
-    CK(launch_status(stsrc, s->status.p, st));
     return PPPM_OK;
 }
 
@@ -533,6 +534,7 @@ int read_status(pppm_solver *s)
 {
     CK(hipMemcpyAsync(s->st_h, s->status.p, sizeof(s->st_h), hipMemcpyDeviceToHost, s->stream));
     CK(hipStreamSynchronize(s->stream));
+    s->err_clean = s->st_h[ST_ERR] == 0;
     s->pool_count = s->st_h[ST_POOL];
     s->last_grid_recomputes = s->st_h[ST_GDIRTY];
     s->last_face_recomputes = s->st_h[ST_FDIRTY];
@@ -554,19 +556,13 @@ int status_error(pppm_solver *s)
 
 bool needs_growth(const pppm_solver *s)
 {
-    return s->st_h[ST_B4ITEMS] > s->cap_b4items || s->st_h[ST_P4] > s->cap_p4 ||
-           s->pool_count + s->cap_p4 > s->pool_cap;
+    return s->st_h[ST_P4] > s->cap_p4 || s->pool_count + s->cap_p4 > s->pool_cap;
 }
 
 // grows capacities to 1.25x the last observed needs (and the pool to hold one more update)
 int grow(pppm_solver *s)
 {
     bool realloc = false;
-    if (s->st_h[ST_B4ITEMS] > s->cap_b4items || s->cap_b4items == 0)
-    {
-        s->cap_b4items = std::max(s->st_h[ST_B4ITEMS], 1) * 5 / 4 + 1024;
-        realloc = true;
-    }
     if (s->st_h[ST_P4] > s->cap_p4 || s->cap_p4 == 0)
     {
         s->cap_p4 = std::max(s->st_h[ST_P4], 1) * 5 / 4 + 1024;
@@ -583,24 +579,41 @@ int grow(pppm_solver *s)
 // attempt copies them into s->verts
 int rebuild_sync(pppm_solver *s, bool first, const float *vsrc = nullptr)
 {
+    // the last update's face lists become the previous ones; this update writes the others
+    bool use_prev = !first && s->faces_valid;
+    if (use_prev)
+    {
+        s->face_start.swap(s->prev_face_start);
+        s->face_nbr.swap(s->prev_face_nbr);
+        s->face_slot.swap(s->prev_face_slot);
+    }
+    s->faces_valid = false;
     for (int attempt = 0; attempt < 3; attempt++)
     {
-        CK(launch_fill_int(s->errflag.p, 1, 0, s->stream));
+        // the error word is cleared only when the last status read did not show it clean (no
+        // kernel outside a mesh update sets it)
+        if (!s->err_clean)
+            CK(launch_fill_int(s->status.p + ST_ERR, 1, 0, s->stream));
+        s->err_clean = false;
         if (first)
         {
             // fresh caches (GridCache::init / FaceCache::init, pppm_cache.h:29-33, 115-124)
             s->pool_count = 0;
             CK(launch_fill_u64(s->hkeys.p, s->hcap, ~0ull, s->stream));
-            CK(hipMemcpyAsync(s->pool_counter.p, &s->pool_count, sizeof(int), hipMemcpyHostToDevice, s->stream));
+            CK(launch_fill_int(s->status.p + ST_POOL, 1, 0, s->stream));
         }
-        RC(enqueue_rebuild(s, first, nullptr, 0, 0, attempt == 0 ? vsrc : nullptr));
+        RC(enqueue_rebuild(s, first, nullptr, 0, 0, attempt == 0 ? vsrc : nullptr, use_prev));
         RC(read_status(s));
         const int e = s->st_h[ST_ERR];
         if ((e & ~ERR_CAPACITY) != 0)
             return status_error(s);
         if (!(e & ERR_CAPACITY) && !needs_growth(s))
+        {
+            s->faces_valid = true;
             return PPPM_OK;
+        }
         RC(grow(s));
+        use_prev = false;  // grow may have moved the previous lists
     }
     return status_error(s);
 }
@@ -635,7 +648,7 @@ int enqueue_step(pppm_solver *s, const float *neumann, int t0, long long stride,
     sp = span_begin(s, 2);
     CK(launch_dirichlet(a, s->state.p, s->face_start.p, s->face_wave.p, s->face_waves, s->face_nbr.p, s->face_slot.p,
                         s->fw.p, s->fsc.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->interp.p, s->corner.p, s->far.p,
-                        s->geom.p, s->errflag.p, s->stream));
+                        s->geom.p, s->status.p + ST_ERR, s->stream));
     span_end(s, sp);
     sp = span_begin(s, 3);
     CK(launch_near(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_wave.p, s->c3_waves, s->c3_pairs.p,
@@ -731,7 +744,9 @@ int run_impl(pppm_solver *s, const float *verts_series, long long vstride, const
     };
     if (verts_series)
     {
-        CK(launch_fill_int(s->errflag.p, 1, 0, s->stream));
+        CK(launch_fill_int(s->status.p + ST_ERR, 1, 0, s->stream));
+        s->err_clean = false;
+        s->faces_valid = false;
         RC(ensure_pool(s, s->pool_count + 2 * s->cap_p4));
     }
     int done = 0;
@@ -907,7 +922,6 @@ int pppm_set_mesh(pppm_solver *s, const float *verts, int nv, const int *tris, i
     CK(s->sb.ensure((size_t)nt * GC));
     CK(s->gmeta.ensure((size_t)nt * 7));
     CK(s->interp.ensure((size_t)nt * 8));
-    CK(s->pool_counter.ensure(1));
     // first guess for the 4x4x4 list capacities; rebuild_sync grows them if needed
     s->cap_b4items = std::max(s->cap_b4items, 16 * nt + 1024);
     s->cap_p4 = std::max(s->cap_p4, 16 * nt + 1024);
@@ -1056,7 +1070,7 @@ int pppm_update_dirichlet(pppm_solver *s)
     CK(launch_windows(s->nt, 1, s->state.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->stream));
     CK(launch_dirichlet(a, s->state.p, s->face_start.p, s->face_wave.p, s->face_waves, s->face_nbr.p, s->face_slot.p,
                         s->fw.p, s->fsc.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->interp.p, s->corner.p, s->far.p,
-                        s->geom.p, s->errflag.p, s->stream));
+                        s->geom.p, s->status.p + ST_ERR, s->stream));
     return PPPM_OK;
 }
 
@@ -1347,7 +1361,29 @@ int pppm_get_b4(pppm_solver *s, int *cells, int *start, int *tris)
     if (!s->mesh_set)
         return fail(s, PPPM_ERR_STATE, "no mesh");
     RC(read_status(s));
-    const int n = s->st_h[ST_B4], items = s->st_h[ST_B4ITEMS];
+    const int n = s->st_h[ST_B4], nt = s->nt;
+    // the time step reads per-triangle face lists; the base-cell lists are built on request
+    const int NN = s->N * s->N * s->N;
+    const UpdateCounters cn{s->counters.p, s->counters.p + NN, s->counters.p + 2 * NN, s->counters.p + 3 * NN};
+    ListArgs la = list_args(s, cn);
+    CK(launch_b4_lists(la, 0, s->stream));
+    {
+        ScanBatch b{};
+        b.njobs = 1;
+        b.job[0] = ScanJob{s->b4_cnt.p, nt, 0, s->b4_start.p, nullptr, nullptr, nullptr, 0};
+        CK(scan_batch(b, s->scan_tmp.p, s->stream));
+    }
+    int items = 0;
+    CK(hipMemcpyAsync(&items, s->b4_start.p + nt, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    CK(hipStreamSynchronize(s->stream));
+    if (items > s->cap_b4items)
+    {
+        s->cap_b4items = items + items / 4 + 1024;
+        CK(s->b4_tris.ensure(s->cap_b4items));
+        la.b4_tris = s->b4_tris.p;
+        la.cap_b4items = s->cap_b4items;
+    }
+    CK(launch_b4_lists(la, 1, s->stream));
     if (cells)
         CK(hipMemcpyAsync(cells, s->b4_cells.p, n * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     if (start)

@@ -80,6 +80,30 @@ __device__ inline float conv_pair(const uint32_t *__restrict__ w16, float2 sc, c
     return r;
 }
 
+// acc += row[lo .. hi) in index order, for a 64-entry LDS row: 16-byte LDS reads issued
+// together, every add predicated on the lane's range (outside it acc is kept), so a leader's
+// sequential sum does not wait on one LDS read per term.
+__device__ inline float seg_add(const float *row, int lo, int hi, float acc)
+{
+#pragma unroll
+    for (int c = 0; c < WAVE; c += 16)
+    {
+        float4 x[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            x[i] = reinterpret_cast<const float4 *>(row + c)[i];
+        const float *v = reinterpret_cast<const float *>(x);
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+        {
+            const int q = c + i;
+            const float sum = acc + v[i];
+            acc = (q >= lo && q < hi) ? sum : acc;
+        }
+    }
+    return acc;
+}
+
 // History windows for the gathers of step t: row k of triangle i holds the ring value at
 // t - 1 - k, k = 0..15, so a gather reads tap k >= 1 at row k - 1. back = 0 when launched
 // before t is advanced (the time step), 1 after it (the phase entry points).
@@ -108,7 +132,7 @@ __global__ __launch_bounds__(TB) void k_far(StepArgs a, const StepState *__restr
                                             const float *__restrict__ wn, const float *__restrict__ wd,
                                             float *__restrict__ far, const float *__restrict__ neumann)
 {
-    __shared__ float vals[WPB][WAVE];
+    __shared__ __align__(16) float vals[WPB][WAVE];
     const int t = st->t;
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wave = blockIdx.x * WPB + wid;
@@ -136,11 +160,7 @@ __global__ __launch_bounds__(TB) void k_far(StepArgs a, const StepState *__restr
             vals[wid][lane] = v;
             wave_sync();
             if (leader)
-            {
-                const int e = min(m1, base + WAVE);
-                for (int q = max(m0, base); q < e; q++)
-                    acc += vals[wid][q - base];
-            }
+                acc = seg_add(vals[wid], max(m0, base) - base, min(m1, base + WAVE) - base, acc);
             wave_sync();
         }
         if (leader)
@@ -175,8 +195,8 @@ __global__ __launch_bounds__(TB) void k_dirichlet(StepArgs a, const StepState *_
                                                   const float *__restrict__ far, const TriGeom *__restrict__ geom,
                                                   const int *__restrict__ err)
 {
-    __shared__ float vals[WPB][WAVE];
-    __shared__ float facs[WPB][WAVE];
+    __shared__ __align__(16) float vals[WPB][WAVE];
+    __shared__ __align__(16) float facs[WPB][WAVE];
     const int t = st->t;
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wave = blockIdx.x * WPB + wid;
@@ -210,12 +230,8 @@ __global__ __launch_bounds__(TB) void k_dirichlet(StepArgs a, const StepState *_
         wave_sync();
         if (leader)
         {
-            const int e = min(m1, base + WAVE);
-            for (int q = max(m0, base); q < e; q++)
-            {
-                acc += vals[wid][q - base];
-                fac += facs[wid][q - base];
-            }
+            acc = seg_add(vals[wid], max(m0, base) - base, min(m1, base + WAVE) - base, acc);
+            fac = seg_add(facs[wid], max(m0, base) - base, min(m1, base + WAVE) - base, fac);
         }
         wave_sync();
     }
@@ -245,7 +261,7 @@ __global__ __launch_bounds__(TB) void k_near(StepArgs a, const StepState *__rest
                                              const int *__restrict__ listen_cell, const int *__restrict__ listen_c3,
                                              float *__restrict__ out)
 {
-    __shared__ float vals[WPB][WAVE];
+    __shared__ __align__(16) float vals[WPB][WAVE];
     const int t = st->t;
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wave = blockIdx.x * WPB + wid;
@@ -275,11 +291,7 @@ __global__ __launch_bounds__(TB) void k_near(StepArgs a, const StepState *__rest
             vals[wid][lane] = v;
             wave_sync();
             if (leader)
-            {
-                const int e = min(m1, base + WAVE);
-                for (int q = max(m0, base); q < e; q++)
-                    acc += vals[wid][q - base];
-            }
+                acc = seg_add(vals[wid], max(m0, base) - base, min(m1, base + WAVE) - base, acc);
             wave_sync();
         }
         if (leader)

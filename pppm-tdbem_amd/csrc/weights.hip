@@ -74,22 +74,6 @@ __device__ inline cdbl layer64(d3 src, d3 trg, d3 n, double kr, double ki, int t
 }
 
 
-// potential_integrand over a batch of scaled targets (integrand.h:209-222)
-__device__ inline cdbl f2p64(const TriD &t, const d3 *pts, const double *scale, int npts, double kr, double ki,
-                             int type)
-{
-    d3 p = l2g(TRI1_X, TRI1_X, t.v);
-    double sr = 0, si = 0;
-    for (int i = 0; i < npts; i++)
-    {
-        cdbl g = layer64(p, pts[i], t.n, kr, ki, type);
-        sr += g.re * scale[i];
-        si += g.im * scale[i];
-    }
-    double f = 0.5 * TRI1_W * t.jac;
-    return {f * sr, f * si};
-}
-
 // singular_point (integrand.h:52-147); trial = tv, test = sv
 __device__ inline cdbl sing_point64(double xsi, double e1, double e2, double e3, double w, const d3 *tv,
                                     const d3 *sv, d3 tn, int nn, double kr, double ki, int type)
@@ -146,38 +130,44 @@ __device__ inline int common_vertices(int a0, int a1, int a2, int b0, int b1, in
            (a2 == b2);
 }
 
-// face2FaceIntegrand (integrand.h:228-259); the near-coincidence test stays in FP32
-__device__ inline cdbl f2f64(const float *verts, const int *src, const int *trg, double kr, double ki, int type,
-                             float threshold)
+// face2FaceIntegrand (integrand.h:228-259); the near-coincidence test stays in FP32. nn =
+// common vertices; SING selects the code path compiled: the singular rules (nn > 0) or the
+// regular 1-point rule (nn == 0), so the regular kernel carries none of the singular code's
+// registers.
+template <bool SING>
+__device__ inline cdbl f2f64(const float *verts, const int *src, const int *trg, int nn, double kr, double ki,
+                             int type, float threshold)
 {
     const f3 s0 = ld3(verts, src[0]), s1 = ld3(verts, src[1]), s2 = ld3(verts, src[2]);
     const f3 t0 = ld3(verts, trg[0]), t1 = ld3(verts, trg[1]), t2 = ld3(verts, trg[2]);
-    const int nn = common_vertices(src[0], src[1], src[2], trg[0], trg[1], trg[2]);
     const float dist = lengthf((s0 + s1 + s2) / 3.0f - (t0 + t1 + t2) / 3.0f);
     if (dist < threshold && nn == 0)
         return {0, 0};
     const TriD S = tri_d(verts, src[0], src[1], src[2]);
     const TriD T = tri_d(verts, trg[0], trg[1], trg[2]);
-    if (nn == 0)
+    if constexpr (!SING)
     {
         d3 a = l2g(TRI1_X, TRI1_X, S.v), b = l2g(TRI1_X, TRI1_X, T.v);
         cdbl g = layer64(a, b, S.n, kr, ki, type);
         double f = 0.25 * TRI1_W * TRI1_W * S.jac * T.jac;
         return {f * g.re, f * g.im};
     }
-    double rr = 0, ri = 0;
-    for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 3; b++)
-            for (int c = 0; c < 3; c++)
-                for (int d = 0; d < 3; d++)
-                {
-                    cdbl p = sing_point64(c_gx[a], c_gx[b], c_gx[c], c_gx[d], c_gw[a] * c_gw[b] * c_gw[c] * c_gw[d],
-                                          S.v, T.v, S.n, nn, kr, ki, type);
-                    rr += p.re;
-                    ri += p.im;
-                }
-    double f = S.jac * T.jac / 16;
-    return {rr * f, ri * f};
+    else
+    {
+        double rr = 0, ri = 0;
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                for (int c = 0; c < 3; c++)
+                    for (int d = 0; d < 3; d++)
+                    {
+                        cdbl p = sing_point64(c_gx[a], c_gx[b], c_gx[c], c_gx[d],
+                                              c_gw[a] * c_gw[b] * c_gw[c] * c_gw[d], S.v, T.v, S.n, nn, kr, ki, type);
+                        rr += p.re;
+                        ri += p.im;
+                    }
+        double f = S.jac * T.jac / 16;
+        return {rr * f, ri * f};
+    }
 }
 
 // Inverse transform with the reference's radix-2 decimation-in-time butterflies
@@ -217,13 +207,9 @@ __device__ inline void scaled_idft(const cdbl *v9, const Tdbem &b, double *out)
                 x[g + m + half] = {a.re - tr, a.im - ti};
             }
     }
-    double temp = 1.0;
 #pragma unroll
     for (int n = 0; n < STEPS; n++)
-    {
-        out[n] = x[n].re / STEPS * temp;
-        temp /= b.lambda;
-    }
+        out[n] = x[n].re / STEPS * b.inv_pow[n];
 }
 
 // LayerWeightHalf::set (bem.h:171-191) into the half2 {S, D} record layout
@@ -312,38 +298,51 @@ constexpr Targets81 make_targets81()
     return t;
 }
 __constant__ Targets81 c_t81 = make_targets81();
-constexpr int GB = 128;  // threads per (triangle, k) block: 81 targets, then 27 batches
+constexpr int GW = 256;  // threads per triangle block of the grid-cache weights
 
-// update_grid_cache_kernel (pppm_cache.cu:64-123), integral part. One block per (dirty
-// triangle, wave number k <= 8): the 81 distinct targets evaluate both layer kernels once
-// (LDS), then each of the 27 offsets sums its 8-target FDTD-stencil batch and takes its
-// cell-centre target, in the order of the reference's batch -> 4 complex values per (o, k).
-__global__ __launch_bounds__(GB) void k_grid_integrals(const float *__restrict__ verts, const int *__restrict__ tris,
-                                                       const TriGeom *__restrict__ geom, const int *__restrict__ list,
-                                                       const int *__restrict__ n_ptr, WeightArgs a, Tdbem bem,
-                                                       cdbl *__restrict__ tmp, const int *__restrict__ err)
+// update_grid_cache_kernel (pppm_cache.cu:64-123) for the dirty triangles, one block per
+// triangle, in four phases through LDS:
+//  1. both layer kernels at the 81 distinct target cells of the 27 stencil batches, for each
+//     wave number k <= 8 (one complex exponential per target, shared by the two layers);
+//  2. per (offset o, k): the 8-target FDTD-stencil batch, summed in the reference's target
+//     order, and the cell-centre target -> 4 complex values (single / double layer of each);
+//  3. per (o, value): the scaled inverse DFT to 16 real taps;
+//  4. per (o, record): W^fdtd = taps shifted by one minus the centre taps shifted by two
+//     (GridCache, pppm_cache.cu:99-116), W^bem = the centre taps; FP16 + scale each.
+// Every value is computed with the operations and order of a per-(o, k) loop, so the records
+// do not depend on how the work is spread over threads.
+__global__ __launch_bounds__(GW) void k_grid_weights(const float *__restrict__ verts, const int *__restrict__ tris,
+                                                     const TriGeom *__restrict__ geom, const int *__restrict__ list,
+                                                     const int *__restrict__ n_ptr, WeightArgs a, Tdbem bem,
+                                                     uint32_t *__restrict__ wf, float2 *__restrict__ sf,
+                                                     uint32_t *__restrict__ wb, float2 *__restrict__ sb,
+                                                     float *__restrict__ meta, const int *__restrict__ err)
 {
-    __shared__ cdbl gl[81][2];
-    const long long items = (long long)live_count(n_ptr, err) * 9;
-    const int j = threadIdx.x;
-    for (long long it = blockIdx.x; it < items; it += gridDim.x)
+    // phase 1 values, then (same memory) the 108 transforms of phase 3
+    __shared__ cdbl pool[GC * 4 * STEPS];
+    cdbl(*gl)[81][2] = reinterpret_cast<cdbl(*)[81][2]>(pool);
+    cdbl(*xs)[STEPS] = reinterpret_cast<cdbl(*)[STEPS]>(pool);
+    __shared__ cdbl bv[GC][9][4];
+    static_assert(sizeof(cdbl) * 9 * 81 * 2 <= sizeof(pool), "phase-1 values fit the transform buffer");
+    const int n = live_count(n_ptr, err);
+    const int tid = threadIdx.x;
+    for (int li = blockIdx.x; li < n; li += gridDim.x)
     {
-        const int k = (int)(it % 9);
-        const int li = (int)(it / 9);
         const int tri = list[li];
-        const int3_ c = geom[tri].cell3;
+        const TriGeom g = geom[tri];
+        const int3_ c = g.cell3;
         const TriD T = tri_d(verts, tris[3 * tri], tris[3 * tri + 1], tris[3 * tri + 2]);
-        const double kr = bem.wave_re[k], ki = bem.wave_im[k];
         const d3 p = l2g(TRI1_X, TRI1_X, T.v);
-        if (j < 81)
+        for (int e = tid; e < 9 * 81; e += GW)
         {
+            const int k = e / 81, j = e % 81;
             const d3 pt = cell_center_d(c.x + c_t81.off[j][0], c.y + c_t81.off[j][1], c.z + c_t81.off[j][2], a);
-            layer64_pair(p, pt, T.n, kr, ki, gl[j][0], gl[j][1]);
+            layer64_pair(p, pt, T.n, bem.wave_re[k], bem.wave_im[k], gl[k][j][0], gl[k][j][1]);
         }
         __syncthreads();
-        if (j < GC)
+        if (tid < GC * 9)
         {
-            const int o = j;
+            const int o = tid / 9, k = tid % 9;
             const int ox = o % 3 - 1, oy = o / 3 % 3 - 1, oz = o / 9 - 1;
             const float s = a.c * a.c * a.dt * a.dt / (a.h * a.h);
             const int dx[8] = {0, -1, 1, 0, 0, 0, 0, 0}, dy[8] = {0, 0, 0, -1, 1, 0, 0, 0},
@@ -357,108 +356,112 @@ __global__ __launch_bounds__(GB) void k_grid_integrals(const float *__restrict__
                     continue;  // triangle_in_grid_cache: no target in the triangle's own cell (pppm_cache.cu:7-19)
                 const int slot = c_t81.idx[(ex + 2) * 25 + (ey + 2) * 5 + (ez + 2)];
                 const double sc = dsc[q];
-                sr0 += gl[slot][0].re * sc;
-                si0 += gl[slot][0].im * sc;
-                sr1 += gl[slot][1].re * sc;
-                si1 += gl[slot][1].im * sc;
+                sr0 += gl[k][slot][0].re * sc;
+                si0 += gl[k][slot][0].im * sc;
+                sr1 += gl[k][slot][1].re * sc;
+                si1 += gl[k][slot][1].im * sc;
             }
             const double f = 0.5 * TRI1_W * T.jac;
-            cdbl *o4 = tmp + (((long long)li * GC + o) * 9 + k) * 4;
-            o4[0] = {f * sr0, f * si0};
-            o4[1] = {f * sr1, f * si1};
+            bv[o][k][0] = {f * sr0, f * si0};
+            bv[o][k][1] = {f * sr1, f * si1};
             if (!(ox == 0 && oy == 0 && oz == 0))
             {
                 const int slot = c_t81.idx[(ox + 2) * 25 + (oy + 2) * 5 + (oz + 2)];
                 double cr0 = 0, ci0 = 0, cr1 = 0, ci1 = 0;
-                cr0 += gl[slot][0].re * 1.0;
-                ci0 += gl[slot][0].im * 1.0;
-                cr1 += gl[slot][1].re * 1.0;
-                ci1 += gl[slot][1].im * 1.0;
-                o4[2] = {f * cr0, f * ci0};
-                o4[3] = {f * cr1, f * ci1};
+                cr0 += gl[k][slot][0].re * 1.0;
+                ci0 += gl[k][slot][0].im * 1.0;
+                cr1 += gl[k][slot][1].re * 1.0;
+                ci1 += gl[k][slot][1].im * 1.0;
+                bv[o][k][2] = {f * cr0, f * ci0};
+                bv[o][k][3] = {f * cr1, f * ci1};
             }
             else
-                o4[2] = o4[3] = {0, 0};
+                bv[o][k][2] = bv[o][k][3] = {0, 0};
+        }
+        __syncthreads();
+        // phase 3: scaled_idft of the 108 series (o, value) in LDS, butterfly by butterfly in
+        // the order-independent stages of the radix-2 transform (each butterfly of a stage
+        // touches its own two entries)
+        for (int e = tid; e < GC * 4 * STEPS; e += GW)
+        {
+            const int tr = e / STEPS, jj = e % STEPS, o = tr / 4, v = tr % 4;
+            const int k = bitrev4(jj);
+            xs[tr][jj] = k <= 8 ? bv[o][k][v] : cdbl{bv[o][16 - k][v].re, -bv[o][16 - k][v].im};
+        }
+        __syncthreads();
+#pragma unroll
+        for (int L = 2; L <= STEPS; L <<= 1)
+        {
+            const int half = L >> 1, tws = STEPS / L;
+            for (int e = tid; e < GC * 4 * (STEPS / 2); e += GW)
+            {
+                const int tr = e / (STEPS / 2), bf = e % (STEPS / 2);
+                const int g = (bf / half) * L, m = bf % half;
+                const double cr = bem.tw_re[m * tws], ci = bem.tw_im[m * tws];
+                const cdbl y = xs[tr][g + m + half];
+                const double tr_ = cr * y.re - ci * y.im;
+                const double ti_ = cr * y.im + ci * y.re;
+                const cdbl a0 = xs[tr][g + m];
+                xs[tr][g + m] = {a0.re + tr_, a0.im + ti_};
+                xs[tr][g + m + half] = {a0.re - tr_, a0.im - ti_};
+            }
+            __syncthreads();
+        }
+        for (int e = tid; e < GC * 4 * STEPS; e += GW)
+        {
+            const int tr = e / STEPS, n2 = e % STEPS, o = tr / 4, v = tr % 4;
+            // offset (0, 0, 0) has no centre target: its centre taps are exactly 0
+            xs[tr][n2].re = (v >= 2 && o == 13) ? 0.0 : xs[tr][n2].re / STEPS * bem.inv_pow[n2];
+        }
+        __syncthreads();
+        if (tid < GC * 2)
+        {
+            const int o = tid / 2, r = tid % 2;
+            const bool self = (o == 13);
+            const size_t rec = (size_t)tri * GC + o;
+            float fs[STEPS], fd[STEPS];
+            if (r == 0)
+            {
+                const cdbl *ws = xs[4 * o], *wd = xs[4 * o + 1], *gs = xs[4 * o + 2], *gd = xs[4 * o + 3];
+                for (int q = 0; q < STEPS; q++)
+                {
+                    double x = q >= 1 ? ws[q - 1].re : 0.0;
+                    double y = q >= 1 ? wd[q - 1].re : 0.0;
+                    if (!self && q >= 2)
+                    {
+                        x += gs[q - 2].re * -1.0;
+                        y += gd[q - 2].re * -1.0;
+                    }
+                    fs[q] = (float)x;
+                    fd[q] = (float)y;
+                }
+                quantize_store(fs, fd, wf + rec * STEPS, sf + rec);
+            }
+            else
+            {
+                for (int q = 0; q < STEPS; q++)
+                {
+                    fs[q] = (float)xs[4 * o + 2][q].re;
+                    fd[q] = (float)xs[4 * o + 3][q].re;
+                }
+                quantize_store(fs, fd, wb + rec * STEPS, sb + rec);
+            }
+            if (o == 0 && r == 0)
+            {
+                float *m = meta + 7 * tri;
+                m[0] = g.center.x;
+                m[1] = g.center.y;
+                m[2] = g.center.z;
+                m[3] = g.normal.x;
+                m[4] = g.normal.y;
+                m[5] = g.normal.z;
+                m[6] = g.area;
+            }
         }
         __syncthreads();
     }
 }
 
-// finish: scaled IDFTs, move(-1), add(G, -1, -2) (bem.h:73-80, 128-140), FP16 quantisation
-__global__ __launch_bounds__(B) void k_grid_finish(const TriGeom *__restrict__ geom, const int *__restrict__ list,
-                                                   const int *__restrict__ n_ptr, Tdbem bem,
-                                                   const cdbl *__restrict__ tmp, uint32_t *__restrict__ wf,
-                                                   float2 *__restrict__ sf, uint32_t *__restrict__ wb,
-                                                   float2 *__restrict__ sb, float *__restrict__ meta,
-                                                   const int *__restrict__ err)
-{
-    const long long items = (long long)live_count(n_ptr, err) * GC;
-    for (long long it = (long long)blockIdx.x * B + threadIdx.x; it < items; it += (long long)gridDim.x * B)
-    {
-        const int o = (int)(it % GC);
-        const int tri = list[it / GC];
-        const TriGeom g = geom[tri];
-        const int3_ c = g.cell3;
-        const bool self = (o == 13);  // offset (0, 0, 0): the triangle's own cell
-        (void)c;
-        const cdbl *src = tmp + it * 9 * 4;
-        cdbl v[9];
-        double ws[STEPS], wd[STEPS], gs[STEPS], gd[STEPS];
-#pragma unroll
-        for (int k = 0; k < 9; k++)
-            v[k] = src[4 * k + 0];
-        scaled_idft(v, bem, ws);
-#pragma unroll
-        for (int k = 0; k < 9; k++)
-            v[k] = src[4 * k + 1];
-        scaled_idft(v, bem, wd);
-        if (!self)
-        {
-#pragma unroll
-            for (int k = 0; k < 9; k++)
-                v[k] = src[4 * k + 2];
-            scaled_idft(v, bem, gs);
-#pragma unroll
-            for (int k = 0; k < 9; k++)
-                v[k] = src[4 * k + 3];
-            scaled_idft(v, bem, gd);
-        }
-        else
-            for (int q = 0; q < STEPS; q++)
-                gs[q] = gd[q] = 0;
-        float fs[STEPS], fd[STEPS], bs[STEPS], bd[STEPS];
-        for (int q = 0; q < STEPS; q++)
-        {
-            double x = q >= 1 ? ws[q - 1] : 0.0;
-            double y = q >= 1 ? wd[q - 1] : 0.0;
-            if (!self && q >= 2)
-            {
-                x += gs[q - 2] * -1.0;
-                y += gd[q - 2] * -1.0;
-            }
-            fs[q] = (float)x;
-            fd[q] = (float)y;
-            bs[q] = (float)gs[q];
-            bd[q] = (float)gd[q];
-        }
-        const size_t rec = (size_t)tri * GC + o;
-        quantize_store(fs, fd, wf + rec * STEPS, sf + rec);
-        quantize_store(bs, bd, wb + rec * STEPS, sb + rec);
-        if (o == 0)
-        {
-            float *m = meta + 7 * tri;
-            m[0] = g.center.x;
-            m[1] = g.center.y;
-            m[2] = g.center.z;
-            m[3] = g.normal.x;
-            m[4] = g.normal.y;
-            m[5] = g.normal.z;
-            m[6] = g.area;
-        }
-    }
-}
-
-// update_interpolation_weight_kernel (pppm_cache.cu:157-181), FP64 then rounded
 // ---------------------------------------------------------------- face cache
 __device__ inline unsigned hash64(unsigned long long k, int cap)
 {
@@ -475,7 +478,7 @@ __global__ __launch_bounds__(256) void k_face_lookup(const TriGeom *__restrict__
                                                      const int *__restrict__ n_ptr, int cap,
                                                      unsigned long long *hkeys, int *hvals, int hcap, int *pool_count,
                                                      int pool_cap, float2 *fmeta, int *face_slot, int *dirty,
-                                                     int *err_flag, int nt)
+                                                     int *err_flag, int nt, PrevFaces prev)
 {
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= cap)
@@ -494,20 +497,31 @@ __global__ __launch_bounds__(256) void k_face_lookup(const TriGeom *__restrict__
         face_slot[p] = -1;
         return;
     }
-    const unsigned long long key = ((unsigned long long)c << 32) | (unsigned)j;
-    unsigned h = hash64(key, hcap);
     int slot = -1;
     bool fresh = false;
-    for (int probe = 0; probe < hcap; probe++)
+    if (prev.start)
+    {
+        // the same (c, j) at the same place in c's list of the previous update: its slot,
+        // without a hash probe (a slot belongs to its key for the handle's life)
+        const int k = p - prev.cur_start[c];
+        const int o0 = prev.start[c], o1 = prev.start[c + 1];
+        if (k < o1 - o0 && prev.nbr[o0 + k] == j)
+            slot = prev.slot[o0 + k];
+        if (slot >= pool_cap)
+            slot = -1;
+    }
+    const unsigned long long key = ((unsigned long long)c << 32) | (unsigned)j;
+    unsigned h = hash64(key, hcap);
+    for (int probe = 0; slot < 0 && probe < hcap; probe++)
     {
         unsigned long long k = hkeys[h];
         if (k == EMPTY_KEY)
         {
-            unsigned long long prev = atomicCAS(&hkeys[h], EMPTY_KEY, key);
-            if (prev == EMPTY_KEY)
+            const unsigned long long was = atomicCAS(&hkeys[h], EMPTY_KEY, key);
+            if (was == EMPTY_KEY)
                 k = key, fresh = true;
             else
-                k = prev;
+                k = was;
         }
         if (k == key)
         {
@@ -559,6 +573,7 @@ __global__ __launch_bounds__(256) void k_face_lookup(const TriGeom *__restrict__
 
 // one thread per (dirty pair, k <= 8, layer): face-face integral plus the interpolated
 // far-field representation subtracted for corners farther than 1.5h (pppm_cache.cu:234-271)
+template <bool SING>
 __global__ __launch_bounds__(B) void k_face_integrals(const float *__restrict__ verts, const int *__restrict__ tris,
                                                       const TriGeom *__restrict__ geom,
                                                       const int *__restrict__ dirty_pairs,
@@ -575,13 +590,19 @@ __global__ __launch_bounds__(B) void k_face_integrals(const float *__restrict__ 
         const int c = owner[p], j = face_nbr[p];
         const int src[3] = {tris[3 * j], tris[3 * j + 1], tris[3 * j + 2]};
         const int trg[3] = {tris[3 * c], tris[3 * c + 1], tris[3 * c + 2]};
+        const int nn = common_vertices(src[0], src[1], src[2], trg[0], trg[1], trg[2]);
+        if (SING != (nn > 0))
+            continue;  // the other variant of this kernel computes this pair
         const double kr = bem.wave_re[k], ki = bem.wave_im[k];
-        cdbl v = f2f64(verts, src, trg, kr, ki, layer, bem.threshold);
+        cdbl v = f2f64<SING>(verts, src, trg, nn, kr, ki, layer, bem.threshold);
+        // the interpolated far field of the corners farther than 1.5h: potential_integrand
+        // (integrand.h:209-222) over the accepted corners in corner order, accumulated on the fly
         const TriGeom gc = geom[c], gj = geom[j];
-        d3 pts[8];
-        double scs[8];
-        int np = 0;
         const double lim = (double)a.h * 1.5;
+        const TriD J = tri_d(verts, src[0], src[1], src[2]);
+        const d3 pj = l2g(TRI1_X, TRI1_X, J.v);
+        double sr = 0, si = 0;
+        bool any = false;
         for (int w = 0; w < 8; w++)
         {
             const int x = gc.base3.x + (w >> 2), y = gc.base3.y + ((w >> 1) & 1), z = gc.base3.z + (w & 1);
@@ -589,17 +610,18 @@ __global__ __launch_bounds__(B) void k_face_integrals(const float *__restrict__ 
             if ((double)fabsf(ctr.x - gj.center.x) > lim || (double)fabsf(ctr.y - gj.center.y) > lim ||
                 (double)fabsf(ctr.z - gj.center.z) > lim)
             {
-                pts[np] = cell_center_d(x, y, z, a);
-                scs[np] = (double)(-interp[8 * c + w]);
-                np++;
+                const cdbl g = layer64(pj, cell_center_d(x, y, z, a), J.n, kr, ki, layer);
+                const double sc = (double)(-interp[8 * c + w]);
+                sr += g.re * sc;
+                si += g.im * sc;
+                any = true;
             }
         }
-        if (np > 0)
+        if (any)
         {
-            const TriD J = tri_d(verts, src[0], src[1], src[2]);
-            cdbl u = f2p64(J, pts, scs, np, kr, ki, layer);
-            v.re += u.re;
-            v.im += u.im;
+            const double f = 0.5 * TRI1_W * J.jac;
+            v.re += f * sr;
+            v.im += f * si;
         }
         tmp[i] = v;
     }
@@ -699,31 +721,34 @@ void tdbem_init(Tdbem &b, float dt, float c, float grid_size)
         b.tw_re[j] = std::cos(ang);
         b.tw_im[j] = std::sin(ang);
     }
+    double t = 1.0;
+    for (int n = 0; n < STEPS; n++)
+    {
+        b.inv_pow[n] = t;
+        t /= lam;
+    }
     b.threshold = (float)(grid_size * 0.1);
 }
 
-size_t grid_tmp_bytes(int max_dirty) { return (size_t)max_dirty * GC * 9 * 4 * sizeof(cdbl); }
 size_t face_tmp_bytes(int max_dirty) { return (size_t)max_dirty * 18 * sizeof(cdbl); }
 
 hipError_t launch_grid_weights(const float *verts, const int *tris, const TriGeom *geom, const int *list,
-                               const int *n_ptr, int cap, const WeightArgs &a, const Tdbem &b, void *tmp,
-                               uint32_t *wf, float2 *sf, uint32_t *wb, float2 *sb, float *meta, const int *err,
+                               const int *n_ptr, int cap, const WeightArgs &a, const Tdbem &b, uint32_t *wf,
+                               float2 *sf, uint32_t *wb, float2 *sb, float *meta, const int *err,
                                long long hint_dirty, hipStream_t s)
 {
     const long long hint = hint_dirty < 0 ? cap : std::min<long long>(hint_dirty, cap);
-    const long long blocks = std::max<long long>(64, std::min<long long>(hint * 9, 16384));
-    k_grid_integrals<<<(unsigned)blocks, GB, 0, s>>>(verts, tris, geom, list, n_ptr, a, b, (cdbl *)tmp, err);
-    k_grid_finish<<<nb_hint(hint * GC, (long long)cap * GC, B), B, 0, s>>>(geom, list, n_ptr, b, (const cdbl *)tmp, wf, sf, wb,
-                                                                    sb, meta, err);
+    const long long blocks = std::max<long long>(64, std::min<long long>(hint, 8192));
+    k_grid_weights<<<(unsigned)blocks, GW, 0, s>>>(verts, tris, geom, list, n_ptr, a, b, wf, sf, wb, sb, meta, err);
     return hipGetLastError();
 }
 hipError_t launch_face_lookup(const TriGeom *geom, const int *tris, const int *owner, const int *face_nbr,
                               const int *n_ptr, int cap, unsigned long long *hkeys, int *hvals, int hcap,
                               int *pool_count, int pool_cap, float2 *fmeta, int *face_slot, int *dirty, int *err_flag,
-                              int nt, hipStream_t s)
+                              int nt, const PrevFaces &prev, hipStream_t s)
 {
     k_face_lookup<<<nb(cap, 256), 256, 0, s>>>(geom, tris, owner, face_nbr, n_ptr, cap, hkeys, hvals, hcap,
-                                               pool_count, pool_cap, fmeta, face_slot, dirty, err_flag, nt);
+                                               pool_count, pool_cap, fmeta, face_slot, dirty, err_flag, nt, prev);
     return hipGetLastError();
 }
 hipError_t launch_face_weights(const float *verts, const int *tris, const TriGeom *geom, const int *dirty_pairs,
@@ -732,8 +757,11 @@ hipError_t launch_face_weights(const float *verts, const int *tris, const TriGeo
                                float2 *fsc, float2 *fmeta, const int *err, long long hint_dirty, hipStream_t s)
 {
     const long long hint = hint_dirty < 0 ? cap : std::min<long long>(hint_dirty, cap);
-    k_face_integrals<<<nb_hint(hint * 18, (long long)cap * 18, B), B, 0, s>>>(verts, tris, geom, dirty_pairs, n_ptr, owner,
-                                                                      face_nbr, interp, a, b, (cdbl *)tmp, err);
+    const int nbi = nb_hint(hint * 18, (long long)cap * 18, B);
+    k_face_integrals<false><<<nbi, B, 0, s>>>(verts, tris, geom, dirty_pairs, n_ptr, owner, face_nbr, interp, a, b,
+                                              (cdbl *)tmp, err);
+    k_face_integrals<true><<<nbi, B, 0, s>>>(verts, tris, geom, dirty_pairs, n_ptr, owner, face_nbr, interp, a, b,
+                                             (cdbl *)tmp, err);
     k_face_finish<<<nb_hint(hint, cap, B), B, 0, s>>>(dirty_pairs, n_ptr, face_slot, geom, owner, face_nbr, b,
                                                   (const cdbl *)tmp, fw, fsc, fmeta, err);
     return hipGetLastError();
