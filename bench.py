@@ -65,6 +65,58 @@ def cpu_baseline(sc, steps, threads):
     return sc.res ** 3 * steps / dt, dt
 
 
+CALIB_MIB = 256
+
+
+def measure_traffic(kernel, workload_args):
+    """HBM bytes per launch of `kernel` from two rocprofv3 counter passes over a short run of
+    this script (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same
+    counter reads for a plain 256 MiB device copy in the same run. Returns None if rocprofv3
+    is unavailable or a pass fails."""
+    import csv
+    import shutil
+    import tempfile
+    if shutil.which("rocprofv3") is None:
+        return None
+    env = dict(os.environ, TMPDIR="/tmp")
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="pppm_pmc_", dir="/tmp")
+        cmd = ["rocprofv3", "--pmc", counter, "--kernel-include-regex", kernel + "|copyBuffer",
+               "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable, os.path.abspath(__file__),
+               "--steps", "20", "--warmup", "2", "--timing-steps", "2", "--no-cpu", "--no-traffic",
+               "--calib-copy-mib", str(CALIB_MIB)] + workload_args
+        try:
+            subprocess.run(cmd, env=env, cwd=ROOT, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                           timeout=240, check=True)
+            kvals, cvals = [], []
+            for path, _, files in os.walk(d):
+                for f in files:
+                    if f.endswith("counter_collection.csv"):
+                        for r in csv.DictReader(open(os.path.join(path, f))):
+                            if r["Counter_Name"] != counter:
+                                continue
+                            v = float(r["Counter_Value"])
+                            if kernel in r["Kernel_Name"]:
+                                kvals.append(v)
+                            elif "copyBuffer" in r["Kernel_Name"]:
+                                cvals.append(v)
+            big = max(cvals) if cvals else 0.0
+            if not kvals or big <= 0:
+                return None
+            # the copy reads and writes CALIB_MIB MiB
+            factor = CALIB_MIB * 1024.0 / big
+            res[counter] = {"kib_per_launch": sum(kvals) / len(kvals), "scale": factor}
+        except (subprocess.SubprocessError, OSError, KeyError, ValueError):
+            return None
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+    f, w = res["FETCH_SIZE"], res["WRITE_SIZE"]
+    return {"bytes": (f["kib_per_launch"] * f["scale"] + w["kib_per_launch"] * w["scale"]) * 1024.0,
+            "fetch_kib": f["kib_per_launch"], "fetch_scale": f["scale"],
+            "write_kib": w["kib_per_launch"], "write_scale": w["scale"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +130,8 @@ def main():
     ap.add_argument("--static-geometry", action="store_true",
                     help="profiling variant: the moving workload's first frame held static, steps replayed "
                          "through the captured graph (step kernels only, no mesh updates)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) behind roofline.traffic")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB five times (a plain copy kernel of "
                          "known size, to scale FETCH_SIZE / WRITE_SIZE counters)")
@@ -204,6 +258,17 @@ def main():
         "phase_time_share": phase_share,
         "listener_final": listener,
     }
+    if not args.no_traffic:
+        kname = res["roofline"]["kernel"]
+        wl = ["--workload", args.workload] + (["--static-geometry"] if args.static_geometry else [])
+        tr = measure_traffic(kname, wl)
+        if tr is not None:
+            res["roofline"]["traffic"] = tr["bytes"]
+            res["roofline"]["traffic_detail"] = {
+                "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, 20-step run of this workload",
+                "fetch_kib_per_launch": tr["fetch_kib"], "fetch_scale": tr["fetch_scale"],
+                "write_kib_per_launch": tr["write_kib"], "write_scale": tr["write_scale"],
+                "calibration": f"{CALIB_MIB} MiB device copy"}
     if not args.no_cpu:
         cpu_rate, cpu_s = cpu_baseline(sc, args.cpu_steps, args.cpu_threads)
         res["cpu_baseline"] = {

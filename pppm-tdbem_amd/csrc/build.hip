@@ -207,9 +207,10 @@ __device__ inline void nbr_fill(int wv, int nw, const int *__restrict__ cells, c
 }
 
 // wave w of a segmented gather owns the segments whose first pair is in [64w, 64w+64):
-// map[w] = smallest s in [0, n] with start[s] >= 64 w, n = the live segment count. Searching
-// only the live prefix matters: capacity-sized CSR arrays carry a tail of empty segments
-// whose cell entries are not written, and a wave must never own them.
+// map[2w] = smallest s in [0, n] with start[s] >= 64 w, n = the live segment count, and
+// map[2w+1] = start[map[2w]] (the wave's first pair). Searching only the live prefix matters:
+// capacity-sized CSR arrays carry a tail of empty segments whose cell entries are not written,
+// and a wave must never own them.
 __device__ inline void wave_map(int w, const int *__restrict__ start, int n, int *map)
 {
     const int p = w * WAVE;
@@ -222,7 +223,8 @@ __device__ inline void wave_map(int w, const int *__restrict__ start, int n, int
         else
             lo = mid + 1;
     }
-    map[w] = lo;
+    map[2 * w] = lo;
+    map[2 * w + 1] = start[lo];
 }
 
 // geometry, counts, 3x3x3 marks, grid-cache dirty flags and interpolation weights of every

@@ -198,7 +198,7 @@ struct StepArgs
 // device time index when the kernel runs (back = 0 before the advance of a step, 1 after)
 hipError_t launch_windows(int nt, int back, const StepState *st, const float *hn, const float *hd, float *wn,
                           float *wd, hipStream_t s);
-// wave maps: map[w] = first segment owned by gather wave w (launch_wave_map)
+// wave maps: map[2w] = first segment owned by gather wave w, map[2w+1] = its first pair
 hipError_t launch_far(const StepArgs &a, const StepState *st, const float *grid3, const int *c3_cells,
                       const int *c3_start, const int *c3_wave, int c3_waves, const int *c3_pairs, const uint32_t *wf,
                       const float2 *sf, float *hn, const float *hd, const float *wn, const float *wd, float *far,

@@ -335,7 +335,7 @@ int allocate(pppm_solver *s)
     CK(s->c3_cnt.ensure(s->cap_c3));
     CK(s->c3_start.ensure(s->cap_c3 + 1));
     CK(s->c3_pairs.ensure(p3));
-    CK(s->c3_wave.ensure(s->c3_waves + 1));
+    CK(s->c3_wave.ensure(2 * (size_t)(s->c3_waves + 1)));
     CK(s->far.ensure(s->cap_c3));
     CK(s->b4_cells.ensure(nt));
     CK(s->b4_index.ensure(NN));
@@ -353,7 +353,7 @@ int allocate(pppm_solver *s)
     CK(s->face_owner.ensure(s->cap_p4));
     CK(s->face_flag.ensure(s->cap_p4));
     CK(s->dirty_list.ensure(s->cap_p4));
-    CK(s->face_wave.ensure(s->face_waves + 1));
+    CK(s->face_wave.ensure(2 * (size_t)(s->face_waves + 1)));
     CK(s->corner.ensure((size_t)nt * 8));
     CK(s->gflag.ensure(nt));
     CK(s->glist.ensure(nt));

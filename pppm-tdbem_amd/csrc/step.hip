@@ -9,7 +9,8 @@
 // Segmented gather with a fixed summation order. A "segment" is a C3 cell (far / near)
 // or a triangle (Dirichlet); its pairs are contiguous in a CSR list. Wave w owns the
 // segments whose first pair lies in [64w, 64w+64) — at most 64, one leader lane each;
-// the owned range comes from a wave map built with the lists. The wave walks its pair
+// the owned range comes from a wave map built with the lists, which also holds the first
+// pair of that range, so a wave starts with two 8-byte loads. The wave walks its pair
 // range in rounds of 64: lane i evaluates the 16-tap convolution of pair (round base + i)
 // sequentially (k = 0..15, as the reference's loop), parks the value in LDS, and every
 // leader adds the values of its own pairs in list order. Every sum therefore has the
@@ -80,30 +81,6 @@ __device__ inline float conv_pair(const uint32_t *__restrict__ w16, float2 sc, c
     return r;
 }
 
-// acc += row[lo .. hi) in index order, for a 64-entry LDS row: 16-byte LDS reads issued
-// together, every add predicated on the lane's range (outside it acc is kept), so a leader's
-// sequential sum does not wait on one LDS read per term.
-__device__ inline float seg_add(const float *row, int lo, int hi, float acc)
-{
-#pragma unroll
-    for (int c = 0; c < WAVE; c += 16)
-    {
-        float4 x[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-            x[i] = reinterpret_cast<const float4 *>(row + c)[i];
-        const float *v = reinterpret_cast<const float *>(x);
-#pragma unroll
-        for (int i = 0; i < 16; i++)
-        {
-            const int q = c + i;
-            const float sum = acc + v[i];
-            acc = (q >= lo && q < hi) ? sum : acc;
-        }
-    }
-    return acc;
-}
-
 // History windows for the gathers of step t: row k of triangle i holds the ring value at
 // t - 1 - k, k = 0..15, so a gather reads tap k >= 1 at row k - 1. back = 0 when launched
 // before t is advanced (the time step), 1 after it (the phase entry points).
@@ -132,27 +109,35 @@ __global__ __launch_bounds__(TB) void k_far(StepArgs a, const StepState *__restr
                                             const float *__restrict__ wn, const float *__restrict__ wd,
                                             float *__restrict__ far, const float *__restrict__ neumann)
 {
-    __shared__ __align__(16) float vals[WPB][WAVE];
+    __shared__ float vals[WPB][WAVE];
     const int t = st->t;
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wave = blockIdx.x * WPB + wid;
     if (wave < gather_waves)
     {
-        const int c0 = c3_wave[wave], c1 = c3_wave[wave + 1];
+        const int2 w0 = reinterpret_cast<const int2 *>(c3_wave)[wave];
+        const int2 w1 = reinterpret_cast<const int2 *>(c3_wave)[wave + 1];
+        const int c0 = w0.x, q0 = w0.y, c1 = w1.x, q1 = w1.y;
         if (c0 >= c1)
             return;
-        const int q0 = c3_start[c0], q1 = c3_start[c1];
+        const size_t nn = (size_t)a.N * a.N * a.N;
+        const float *g = grid3 + (size_t)(((t % 3) + 3) % 3) * nn;
         const int cs = c0 + lane;
         const bool leader = cs < c1;
+        // every index of the wave's first round is loaded at once, then what they point to
         const int m0 = leader ? c3_start[cs] : 0, m1 = leader ? c3_start[cs + 1] : 0;
+        const unsigned cell = leader ? (unsigned)c3_cells[cs] : ~0u;
+        int rec = q0 + lane < q1 ? c3_pairs[q0 + lane] : -1;
+        const float gv = cell < nn ? g[cell] : 0.0f;
         float acc = 0.0f;
         for (int base = q0; base < q1; base += WAVE)
         {
             const int p = base + lane;
+            if (base != q0)
+                rec = p < q1 ? c3_pairs[p] : -1;
             float v = 0.0f;
-            if (p < q1)
+            if (rec >= 0)
             {
-                const int rec = c3_pairs[p];
                 const int tri = rec / GC;
                 v = conv_pair<1, false>(wf + (size_t)rec * STEPS, sf[rec], hn + tri * RING, hd + tri * RING,
                                        wn + tri * STEPS, wd + tri * STEPS, t);
@@ -160,16 +145,15 @@ __global__ __launch_bounds__(TB) void k_far(StepArgs a, const StepState *__restr
             vals[wid][lane] = v;
             wave_sync();
             if (leader)
-                acc = seg_add(vals[wid], max(m0, base) - base, min(m1, base + WAVE) - base, acc);
+            {
+                const int e = min(m1, base + WAVE);
+                for (int q = max(m0, base); q < e; q++)
+                    acc += vals[wid][q - base];
+            }
             wave_sync();
         }
         if (leader)
-        {
-            const size_t nn = (size_t)a.N * a.N * a.N;
-            const float *g = grid3 + (size_t)(((t % 3) + 3) % 3) * nn;
-            const unsigned cell = (unsigned)c3_cells[cs];
-            far[cs] = cell < nn ? g[cell] - acc : 0.0f;
-        }
+            far[cs] = cell < nn ? gv - acc : 0.0f;
     }
     else
     {
@@ -195,57 +179,86 @@ __global__ __launch_bounds__(TB) void k_dirichlet(StepArgs a, const StepState *_
                                                   const float *__restrict__ far, const TriGeom *__restrict__ geom,
                                                   const int *__restrict__ err)
 {
-    __shared__ __align__(16) float vals[WPB][WAVE];
-    __shared__ __align__(16) float facs[WPB][WAVE];
+    __shared__ float vals[WPB][WAVE];
+    __shared__ float facs[WPB][WAVE];
     const int t = st->t;
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wave = blockIdx.x * WPB + wid;
     // after a flagged mesh update the face lists may be incomplete: leave D[t] untouched
     if (wave >= gather_waves || *err != 0)
         return;
-    const int c0 = face_wave[wave], c1 = face_wave[wave + 1];
+    const int2 w0 = reinterpret_cast<const int2 *>(face_wave)[wave];
+    const int2 w1 = reinterpret_cast<const int2 *>(face_wave)[wave + 1];
+    const int c0 = w0.x, q0 = w0.y, c1 = w1.x, q1 = w1.y;
     if (c0 >= c1)
         return;
-    const int q0 = face_start[c0], q1 = face_start[c1];
     const int cs = c0 + lane;
     const bool leader = cs < c1;
+    // every index of the wave's first round is loaded at once, then what they point to; the
+    // leaders' face far field (8-corner interpolation) does not depend on the gather
     const int m0 = leader ? face_start[cs] : 0, m1 = leader ? face_start[cs + 1] : 0;
+    int j = -1, ws = 0;
+    if (q0 + lane < q1)
+    {
+        j = face_nbr[q0 + lane];
+        ws = face_slot[q0 + lane];
+    }
+    float ff = 0.0f, area = 0.0f;
+    if (leader)
+    {
+        int ci[8];
+        float iw[8];
+#pragma unroll
+        for (int w = 0; w < 8; w++)
+        {
+            ci[w] = corner[8 * cs + w];  // corners of a base cell are C3 cells (>= 0)
+            iw[w] = interp[8 * cs + w];
+        }
+        area = geom[cs].area;
+#pragma unroll
+        for (int w = 0; w < 8; w++)
+            ff += iw[w] * (ci[w] >= 0 ? far[ci[w]] : 0.0f);
+    }
     float acc = 0.0f, fac = 0.0f;
     for (int base = q0; base < q1; base += WAVE)
     {
         const int p = base + lane;
-        float v = 0.0f, f = 0.0f;
-        if (p < q1)
+        if (base != q0)
         {
-            const int j = face_nbr[p];
-            const int ws = face_slot[p];
+            j = -1;
+            if (p < q1)
+            {
+                j = face_nbr[p];
+                ws = face_slot[p];
+            }
+        }
+        float v = 0.0f, f = 0.0f;
+        if (j >= 0)
+        {
             const float2 sc = fsc[ws];
-            uint32_t w0;
+            uint32_t tap0;
             v = conv_pair<0, true>(fw + (size_t)ws * STEPS, sc, hn + j * RING, hd + j * RING, wn + j * STEPS,
-                                  wd + j * STEPS, t, &w0);
-            f = f16_bits_to_f32(w0 >> 16) * sc.y;  // face_factor = (float)D_0 * scale_D
+                                  wd + j * STEPS, t, &tap0);
+            f = f16_bits_to_f32(tap0 >> 16) * sc.y;  // face_factor = (float)D_0 * scale_D
         }
         vals[wid][lane] = v;
         facs[wid][lane] = f;
         wave_sync();
         if (leader)
         {
-            acc = seg_add(vals[wid], max(m0, base) - base, min(m1, base + WAVE) - base, acc);
-            fac = seg_add(facs[wid], max(m0, base) - base, min(m1, base + WAVE) - base, fac);
+            const int e = min(m1, base + WAVE);
+            for (int q = max(m0, base); q < e; q++)
+            {
+                acc += vals[wid][q - base];
+                fac += facs[wid][q - base];
+            }
         }
         wave_sync();
     }
     if (leader)
     {
-        float ff = 0.0f;
-#pragma unroll
-        for (int w = 0; w < 8; w++)
-        {
-            const int ci = corner[8 * cs + w];  // corners of a base cell are C3 cells (>= 0)
-            ff += interp[8 * cs + w] * (ci >= 0 ? far[ci] : 0.0f);
-        }
         const float num = ff * a.rc + acc;
-        const double den = 0.5 * (double)geom[cs].area - (double)fac;
+        const double den = 0.5 * (double)area - (double)fac;
         hd[cs * RING + (t & (RING - 1))] = (float)((double)num / den);
     }
 }
@@ -261,7 +274,7 @@ __global__ __launch_bounds__(TB) void k_near(StepArgs a, const StepState *__rest
                                              const int *__restrict__ listen_cell, const int *__restrict__ listen_c3,
                                              float *__restrict__ out)
 {
-    __shared__ __align__(16) float vals[WPB][WAVE];
+    __shared__ float vals[WPB][WAVE];
     const int t = st->t;
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wave = blockIdx.x * WPB + wid;
@@ -269,21 +282,27 @@ __global__ __launch_bounds__(TB) void k_near(StepArgs a, const StepState *__rest
     float *g = grid3 + (size_t)(((t % 3) + 3) % 3) * nn;
     if (wave < gather_waves)
     {
-        const int c0 = c3_wave[wave], c1 = c3_wave[wave + 1];
+        const int2 w0 = reinterpret_cast<const int2 *>(c3_wave)[wave];
+        const int2 w1 = reinterpret_cast<const int2 *>(c3_wave)[wave + 1];
+        const int c0 = w0.x, q0 = w0.y, c1 = w1.x, q1 = w1.y;
         if (c0 >= c1)
             return;
-        const int q0 = c3_start[c0], q1 = c3_start[c1];
         const int cs = c0 + lane;
         const bool leader = cs < c1;
+        // every index of the wave's first round is loaded at once, then what they point to
         const int m0 = leader ? c3_start[cs] : 0, m1 = leader ? c3_start[cs + 1] : 0;
+        const unsigned cell = leader ? (unsigned)c3_cells[cs] : ~0u;
+        const float fv = leader ? far[cs] : 0.0f;
+        int rec = q0 + lane < q1 ? c3_pairs[q0 + lane] : -1;
         float acc = 0.0f;
         for (int base = q0; base < q1; base += WAVE)
         {
             const int p = base + lane;
+            if (base != q0)
+                rec = p < q1 ? c3_pairs[p] : -1;
             float v = 0.0f;
-            if (p < q1)
+            if (rec >= 0)
             {
-                const int rec = c3_pairs[p];
                 const int tri = rec / GC;
                 v = conv_pair<0, false>(wb + (size_t)rec * STEPS, sb[rec], hn + tri * RING, hd + tri * RING,
                                        wn + tri * STEPS, wd + tri * STEPS, t);
@@ -291,13 +310,16 @@ __global__ __launch_bounds__(TB) void k_near(StepArgs a, const StepState *__rest
             vals[wid][lane] = v;
             wave_sync();
             if (leader)
-                acc = seg_add(vals[wid], max(m0, base) - base, min(m1, base + WAVE) - base, acc);
+            {
+                const int e = min(m1, base + WAVE);
+                for (int q = max(m0, base); q < e; q++)
+                    acc += vals[wid][q - base];
+            }
             wave_sync();
         }
         if (leader)
         {
-            const float v = (float)((double)far[cs] + (double)acc * 0.98);
-            const unsigned cell = (unsigned)c3_cells[cs];
+            const float v = (float)((double)fv + (double)acc * 0.98);
             if (cell < nn)
                 g[cell] = v;
             if (out)

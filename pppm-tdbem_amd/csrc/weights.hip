@@ -298,7 +298,10 @@ constexpr Targets81 make_targets81()
     return t;
 }
 __constant__ Targets81 c_t81 = make_targets81();
-constexpr int GW = 256;  // threads per triangle block of the grid-cache weights
+// threads per triangle block of the grid-cache weights: one per phase-1 evaluation (729),
+// two rounds for the butterflies and the taps
+constexpr int GW = 768;
+static_assert(GW % STEPS == 0 && GW % WAVE == 0, "tap groups stay inside one wave");
 
 // update_grid_cache_kernel (pppm_cache.cu:64-123) for the dirty triangles, one block per
 // triangle, in four phases through LDS:
@@ -307,7 +310,7 @@ constexpr int GW = 256;  // threads per triangle block of the grid-cache weights
 //  2. per (offset o, k): the 8-target FDTD-stencil batch, summed in the reference's target
 //     order, and the cell-centre target -> 4 complex values (single / double layer of each);
 //  3. per (o, value): the scaled inverse DFT to 16 real taps;
-//  4. per (o, record): W^fdtd = taps shifted by one minus the centre taps shifted by two
+//  4. per (o, record, tap): W^fdtd = taps shifted by one minus the centre taps shifted by two
 //     (GridCache, pppm_cache.cu:99-116), W^bem = the centre taps; FP16 + scale each.
 // Every value is computed with the operations and order of a per-(o, k) loop, so the records
 // do not depend on how the work is spread over threads.
@@ -414,39 +417,47 @@ __global__ __launch_bounds__(GW) void k_grid_weights(const float *__restrict__ v
             xs[tr][n2].re = (v >= 2 && o == 13) ? 0.0 : xs[tr][n2].re / STEPS * bem.inv_pow[n2];
         }
         __syncthreads();
-        if (tid < GC * 2)
+        // phase 4: one lane per (o, record, tap); the max-abs scales of LayerWeightHalf::set
+        // are reduced over the 16 lanes of a record (exact, order-free)
+        for (int e = tid; e < GC * 2 * STEPS; e += GW)
         {
-            const int o = tid / 2, r = tid % 2;
+            const int rr = e / STEPS, q = e % STEPS, o = rr / 2, r = rr % 2;
             const bool self = (o == 13);
-            const size_t rec = (size_t)tri * GC + o;
-            float fs[STEPS], fd[STEPS];
+            float fsq, fdq;
             if (r == 0)
             {
-                const cdbl *ws = xs[4 * o], *wd = xs[4 * o + 1], *gs = xs[4 * o + 2], *gd = xs[4 * o + 3];
-                for (int q = 0; q < STEPS; q++)
+                double x = q >= 1 ? xs[4 * o][q - 1].re : 0.0;
+                double y = q >= 1 ? xs[4 * o + 1][q - 1].re : 0.0;
+                if (!self && q >= 2)
                 {
-                    double x = q >= 1 ? ws[q - 1].re : 0.0;
-                    double y = q >= 1 ? wd[q - 1].re : 0.0;
-                    if (!self && q >= 2)
-                    {
-                        x += gs[q - 2].re * -1.0;
-                        y += gd[q - 2].re * -1.0;
-                    }
-                    fs[q] = (float)x;
-                    fd[q] = (float)y;
+                    x += xs[4 * o + 2][q - 2].re * -1.0;
+                    y += xs[4 * o + 3][q - 2].re * -1.0;
                 }
-                quantize_store(fs, fd, wf + rec * STEPS, sf + rec);
+                fsq = (float)x;
+                fdq = (float)y;
             }
             else
             {
-                for (int q = 0; q < STEPS; q++)
-                {
-                    fs[q] = (float)xs[4 * o + 2][q].re;
-                    fd[q] = (float)xs[4 * o + 3][q].re;
-                }
-                quantize_store(fs, fd, wb + rec * STEPS, sb + rec);
+                fsq = (float)xs[4 * o + 2][q].re;
+                fdq = (float)xs[4 * o + 3][q].re;
             }
-            if (o == 0 && r == 0)
+            float ms = fabsf(fsq), md = fabsf(fdq);
+#pragma unroll
+            for (int off = STEPS / 2; off >= 1; off >>= 1)
+            {
+                ms = fmaxf(ms, __shfl_xor(ms, off, STEPS));
+                md = fmaxf(md, __shfl_xor(md, off, STEPS));
+            }
+            if (ms == 0)
+                ms = 1;
+            if (md == 0)
+                md = 1;
+            const size_t rec = (size_t)tri * GC + o;
+            uint32_t *w16 = (r == 0 ? wf : wb) + rec * STEPS;
+            w16[q] = f32_to_f16_bits(fsq / ms) | (f32_to_f16_bits(fdq / md) << 16);
+            if (q == 0)
+                (r == 0 ? sf : sb)[rec] = make_float2(ms, md);
+            if (e == 0)
             {
                 float *m = meta + 7 * tri;
                 m[0] = g.center.x;
@@ -571,10 +582,92 @@ __global__ __launch_bounds__(256) void k_face_lookup(const TriGeom *__restrict__
         dirty[p] = 0;
 }
 
-// one thread per (dirty pair, k <= 8, layer): face-face integral plus the interpolated
-// far-field representation subtracted for corners farther than 1.5h (pppm_cache.cu:234-271)
-template <bool SING>
-__global__ __launch_bounds__(B) void k_face_integrals(const float *__restrict__ verts, const int *__restrict__ tris,
+// Face-cache integrals (pppm_cache.cu:234-271) per (dirty pair, k <= 8, layer): the
+// face-face integral plus the interpolated far-field representation subtracted for corners
+// farther than 1.5h. Regular pairs (no common vertex) take 16 lanes per item: lane 0
+// evaluates the face-face term, lane 1 + w the far-field term of corner w, and lane 0 adds
+// the corner products in corner order, as potential_integrand does.
+__global__ __launch_bounds__(B) void k_face_regular(const float *__restrict__ verts, const int *__restrict__ tris,
+                                                    const TriGeom *__restrict__ geom,
+                                                    const int *__restrict__ dirty_pairs, const int *__restrict__ n_ptr,
+                                                    const int *__restrict__ owner, const int *__restrict__ face_nbr,
+                                                    const float *__restrict__ interp, WeightArgs a, Tdbem bem,
+                                                    cdbl *__restrict__ tmp, const int *__restrict__ err)
+{
+    constexpr int L = 16;
+    const long long lanes = (long long)live_count(n_ptr, err) * 18 * L;
+    // the stride is a multiple of 16, so the 16 lanes of an item stay together
+    for (long long e = (long long)blockIdx.x * B + threadIdx.x; e < lanes; e += (long long)gridDim.x * B)
+    {
+        const long long i = e / L;
+        const int sub = (int)(e % L);
+        const int d = (int)(i / 18), r = (int)(i % 18), layer = r / 9, k = r % 9;
+        const int p = dirty_pairs[d];
+        const int c = owner[p], j = face_nbr[p];
+        const int src[3] = {tris[3 * j], tris[3 * j + 1], tris[3 * j + 2]};
+        const int trg[3] = {tris[3 * c], tris[3 * c + 1], tris[3 * c + 2]};
+        const int nn = common_vertices(src[0], src[1], src[2], trg[0], trg[1], trg[2]);
+        const bool regular = nn == 0;  // the same for the 16 lanes of the item
+        const double kr = bem.wave_re[k], ki = bem.wave_im[k];
+        double pr = 0, pi = 0;
+        bool far_corner = false;
+        if (regular && sub == 0)
+        {
+            const cdbl v = f2f64<false>(verts, src, trg, nn, kr, ki, layer, bem.threshold);
+            pr = v.re;
+            pi = v.im;
+        }
+        else if (regular && sub <= 8)
+        {
+            const int w = sub - 1;
+            const TriGeom gc = geom[c], gj = geom[j];
+            const double lim = (double)a.h * 1.5;
+            const int x = gc.base3.x + (w >> 2), y = gc.base3.y + ((w >> 1) & 1), z = gc.base3.z + (w & 1);
+            const f3 ctr = cell_center_f(x, y, z, a);
+            if ((double)fabsf(ctr.x - gj.center.x) > lim || (double)fabsf(ctr.y - gj.center.y) > lim ||
+                (double)fabsf(ctr.z - gj.center.z) > lim)
+            {
+                const TriD J = tri_d(verts, src[0], src[1], src[2]);
+                const d3 pj = l2g(TRI1_X, TRI1_X, J.v);
+                const cdbl g = layer64(pj, cell_center_d(x, y, z, a), J.n, kr, ki, layer);
+                const double sc = (double)(-interp[8 * c + w]);
+                pr = g.re * sc;
+                pi = g.im * sc;
+                far_corner = true;
+            }
+        }
+        double sr = 0, si = 0;
+        bool any = false;
+#pragma unroll
+        for (int w = 0; w < 8; w++)
+        {
+            const double xr = __shfl(pr, w + 1, L), xi = __shfl(pi, w + 1, L);
+            const bool fw = __shfl(far_corner ? 1 : 0, w + 1, L) != 0;
+            if (fw)
+            {
+                sr += xr;
+                si += xi;
+                any = true;
+            }
+        }
+        if (regular && sub == 0)
+        {
+            cdbl v = {pr, pi};
+            if (any)
+            {
+                const TriD J = tri_d(verts, src[0], src[1], src[2]);
+                const double f = 0.5 * TRI1_W * J.jac;
+                v.re += f * sr;
+                v.im += f * si;
+            }
+            tmp[i] = v;
+        }
+    }
+}
+
+// Singular pairs (common vertices): one thread per (pair, k, layer), the 81-point rules in
+// sequence, then the corner terms.
+__global__ __launch_bounds__(B) void k_face_singular(const float *__restrict__ verts, const int *__restrict__ tris,
                                                       const TriGeom *__restrict__ geom,
                                                       const int *__restrict__ dirty_pairs,
                                                       const int *__restrict__ n_ptr, const int *__restrict__ owner,
@@ -591,10 +684,10 @@ __global__ __launch_bounds__(B) void k_face_integrals(const float *__restrict__ 
         const int src[3] = {tris[3 * j], tris[3 * j + 1], tris[3 * j + 2]};
         const int trg[3] = {tris[3 * c], tris[3 * c + 1], tris[3 * c + 2]};
         const int nn = common_vertices(src[0], src[1], src[2], trg[0], trg[1], trg[2]);
-        if (SING != (nn > 0))
-            continue;  // the other variant of this kernel computes this pair
+        if (nn == 0)
+            continue;  // k_face_regular computes this pair
         const double kr = bem.wave_re[k], ki = bem.wave_im[k];
-        cdbl v = f2f64<SING>(verts, src, trg, nn, kr, ki, layer, bem.threshold);
+        cdbl v = f2f64<true>(verts, src, trg, nn, kr, ki, layer, bem.threshold);
         // the interpolated far field of the corners farther than 1.5h: potential_integrand
         // (integrand.h:209-222) over the accepted corners in corner order, accumulated on the fly
         const TriGeom gc = geom[c], gj = geom[j];
@@ -757,11 +850,10 @@ hipError_t launch_face_weights(const float *verts, const int *tris, const TriGeo
                                float2 *fsc, float2 *fmeta, const int *err, long long hint_dirty, hipStream_t s)
 {
     const long long hint = hint_dirty < 0 ? cap : std::min<long long>(hint_dirty, cap);
-    const int nbi = nb_hint(hint * 18, (long long)cap * 18, B);
-    k_face_integrals<false><<<nbi, B, 0, s>>>(verts, tris, geom, dirty_pairs, n_ptr, owner, face_nbr, interp, a, b,
-                                              (cdbl *)tmp, err);
-    k_face_integrals<true><<<nbi, B, 0, s>>>(verts, tris, geom, dirty_pairs, n_ptr, owner, face_nbr, interp, a, b,
-                                             (cdbl *)tmp, err);
+    k_face_regular<<<nb_hint(hint * 18 * 16, (long long)cap * 18 * 16, B), B, 0, s>>>(
+        verts, tris, geom, dirty_pairs, n_ptr, owner, face_nbr, interp, a, b, (cdbl *)tmp, err);
+    k_face_singular<<<nb_hint(hint * 18, (long long)cap * 18, B), B, 0, s>>>(
+        verts, tris, geom, dirty_pairs, n_ptr, owner, face_nbr, interp, a, b, (cdbl *)tmp, err);
     k_face_finish<<<nb_hint(hint, cap, B), B, 0, s>>>(dirty_pairs, n_ptr, face_slot, geom, owner, face_nbr, b,
                                                   (const cdbl *)tmp, fw, fsc, fmeta, err);
     return hipGetLastError();

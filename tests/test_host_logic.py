@@ -2,8 +2,8 @@
 
 The far / near / Dirichlet kernels split a CSR list (segment s owns pairs
 start[s]..start[s+1]) over 64-lane waves: wave w owns the segments whose first pair lies in
-[64 w, 64 w + 64), found by k_wave_map as the smallest s with start[s] >= 64 w, and lane l
-of the wave leads segment map[w] + l. Lists are sized by capacity and only their first n
+[64 w, 64 w + 64), found by the wave map (wave_map in csrc/build.hip) as the smallest s with
+start[s] >= 64 w, and lane l of the wave leads segment map[w] + l. Lists are sized by capacity and only their first n
 segments are written (the rest are empty: start[s] == start[n] for s >= n). These tests
 restate that arithmetic and check that every live segment has exactly one leader and no
 leader falls on an unwritten tail segment.
@@ -14,7 +14,7 @@ WAVE = 64
 
 
 def wave_map(start, nseg, waves, n_live=None):
-    """k_wave_map: smallest s in [0, n] with start[s] >= 64 w (n = live count if given)."""
+    """The wave map: smallest s in [0, n] with start[s] >= 64 w (n = live count if given)."""
     n = nseg if n_live is None else min(n_live, nseg)
     out = np.empty(waves + 1, np.int64)
     for w in range(waves + 1):
