@@ -159,6 +159,35 @@ int pppm_set_solid_zero(pppm_solver *s);
 int pppm_get_cells(pppm_solver *s, int *type, int *ghost_idx, int *nearest, float *point3, float *reflect3,
                    float *dist);
 
+/* Modal sound source of a rigid body (RigidBody, src/utils/RigidBody.h:53-166) on the GPU: the
+ * surface modal matrix, the per-mode IIR oscillators and the surface accelerations that feed
+ * the solver's Neumann input. A handle owns its device data and runs on its stream.
+ * pppm_modal_create: eigvecs = the truncated eigenvector matrix, 3*nv rows (vertex dofs) x
+ *   nmodes columns, row-major (RigidBody::LoadEigen_, RigidBody.cu:141-188); eigvals its
+ *   eigenvalues; tris / normals the surface triangles and their unit normals. The IIR
+ *   coefficients come from ModalInfo::SetCoeffs (RigidBody.cu:11-27) with the material's
+ *   Rayleigh damping alpha, beta and timestep = 1 / sample_rate (RigidBody::InitIIR_); the
+ *   surface matrix is MatrixVertToTri (RigidBody.cu:206-230): per triangle and mode the mean of
+ *   the three vertex displacements dotted with the triangle normal. Oscillator state starts at 0.
+ * pppm_modal_run: for each of nsamples samples, the IIR step of every mode with the modal
+ *   forces of that sample (forces: nsamples x nmodes, row-major) and the surface accelerations
+ *   acc[s][t] = sum_i M(t, i) Q_i (CalculateIIR_ / update_surf_acc_kernel, RigidBody.cu:240-295),
+ *   written to accs (nsamples x nt, device memory). on_device: where forces live.
+ * pppm_modal_state / pppm_modal_set_state: the oscillators' (q1, q2, q3) per mode (nmodes x 3,
+ *   host memory), for RigidBody::SaveState / LoadState (RigidBody.cu:60-88).
+ * pppm_modal_surface_matrix: host copy of M (nt x nmodes, row-major).
+ * pppm_modal_coeffs: host copy of the IIR coefficients (nmodes x 3: coeff1, coeff2, coeff3). */
+typedef struct pppm_modal pppm_modal;
+int pppm_modal_create(pppm_modal **out, const float *eigvecs, int nv, int nmodes, const float *eigvals,
+                      const int *tris, int nt, const float *normals, float alpha, float beta, float sample_rate,
+                      void *hip_stream);
+int pppm_modal_destroy(pppm_modal *m);
+int pppm_modal_run(pppm_modal *m, const float *forces, int nsamples, float *accs, int on_device);
+int pppm_modal_state(pppm_modal *m, float *q3);
+int pppm_modal_set_state(pppm_modal *m, const float *q3);
+int pppm_modal_surface_matrix(pppm_modal *m, float *host_out);
+int pppm_modal_coeffs(pppm_modal *m, float *host_out);
+
 #ifdef __cplusplus
 }
 #endif

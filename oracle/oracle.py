@@ -80,6 +80,11 @@ def lib():
             "orc_closest_coeff": ([vp, vp, vp], None),
             "orc_det_exp": ([d], d),
             "orc_det_sincos": ([d, vp, vp], None),
+            "orc_modal_coeffs": ([vp, i, f, f, f, vp], None),
+            "orc_modal_surface_matrix": ([vp, i, i, vp, i, vp, vp], None),
+            "orc_modal_iir": ([vp, i, f, vp, vp, i, vp], None),
+            "orc_modal_accs": ([vp, i, i, vp, i, vp], None),
+            "orc_modal_forces": ([vp, i, vp, vp, vp, vp, i, f, vp], None),
         }
         for name, (a, r) in sig.items():
             fn = getattr(L, name)
@@ -350,6 +355,57 @@ def closest_coeff(p3, tri9):
     t = np.ascontiguousarray(tri9, np.float32).reshape(9)
     out = np.zeros(3, np.float32)
     lib().orc_closest_coeff(_p(p), _p(t), _p(out))
+    return out
+
+
+def modal_coeffs(eigvals, timestep, alpha, beta):
+    """ModalInfo::SetCoeffs for every mode -> (M, 3) float32 (coeff1, coeff2, coeff3)."""
+    ev = np.ascontiguousarray(eigvals, np.float32)
+    out = np.zeros((ev.size, 3), np.float32)
+    lib().orc_modal_coeffs(_p(ev), ev.size, ctypes.c_float(timestep), ctypes.c_float(alpha), ctypes.c_float(beta),
+                           _p(out))
+    return out
+
+
+def modal_surface_matrix(eig, tris, normals):
+    """MatrixVertToTri: eig (3 nv, M) float32, tris (T, 3), normals (T, 3) -> (T, M) float32."""
+    e = np.ascontiguousarray(eig, np.float32)
+    t = np.ascontiguousarray(tris, np.int32)
+    n = np.ascontiguousarray(normals, np.float32)
+    out = np.zeros((t.shape[0], e.shape[1]), np.float32)
+    lib().orc_modal_surface_matrix(_p(e), e.shape[0] // 3, e.shape[1], _p(t), t.shape[0], _p(n), _p(out))
+    return out
+
+
+def modal_iir(c123, timestep, state, forces):
+    """IIR recursion: state (M, 3) updated in place, forces (S, M) -> Q (S, M) float32."""
+    c = np.ascontiguousarray(c123, np.float32)
+    f = np.ascontiguousarray(forces, np.float32)
+    assert state.dtype == np.float32 and state.flags.c_contiguous
+    q = np.zeros_like(f)
+    lib().orc_modal_iir(_p(c), c.shape[0], ctypes.c_float(timestep), _p(state), _p(f), f.shape[0], _p(q))
+    return q
+
+
+def modal_accs(msurf, q):
+    """acc[s][t] = sum_i M[t][i] Q[s][i] in mode order -> (S, T) float32."""
+    m = np.ascontiguousarray(msurf, np.float32)
+    qq = np.ascontiguousarray(q, np.float32)
+    out = np.zeros((qq.shape[0], m.shape[0]), np.float32)
+    lib().orc_modal_accs(_p(m), m.shape[0], m.shape[1], _p(qq), qq.shape[0], _p(out))
+    return out
+
+
+def modal_forces(eig, imp_time, imp_vert, imp_vec, active, time):
+    """Modal forces of the active impulses at `time` -> (M,) float32."""
+    e = np.ascontiguousarray(eig, np.float32)
+    it = np.ascontiguousarray(imp_time, np.float32)
+    iv = np.ascontiguousarray(imp_vert, np.int32)
+    ivec = np.ascontiguousarray(imp_vec, np.float32)
+    act = np.ascontiguousarray(active, np.int32)
+    out = np.zeros(e.shape[1], np.float32)
+    lib().orc_modal_forces(_p(e), e.shape[1], _p(it), _p(iv), _p(ivec), _p(act), it.size, ctypes.c_float(time),
+                           _p(out))
     return out
 
 

@@ -23,7 +23,9 @@
 //                  corner map, both gather wave maps, and the reset of the counters
 // The counters (cell and base counts, fill slots, marks) are zero when an update starts and
 // are cleared again by k_lists_fill for the cells this update touched, so no N^3 clear is
-// needed per update. Every launch has a host-known size (a capacity); the live counts stay
+// needed per update. The face lookup also adds up the per-tile counts of its dirty flags (one
+// integer atomic per wave), so the face-pair scan is a single launch; k_tri_update clears them
+// for the next update. Every launch has a host-known size (a capacity); the live counts stay
 // in device memory and kernels stride over them, so an update needs no device-to-host round
 // trip and can be captured in a HIP graph.
 #include "internal.hpp"
@@ -210,21 +212,28 @@ __device__ inline void nbr_fill(int wv, int nw, const int *__restrict__ cells, c
 // map[2w] = smallest s in [0, n] with start[s] >= 64 w, n = the live segment count, and
 // map[2w+1] = start[map[2w]] (the wave's first pair). Searching only the live prefix matters:
 // capacity-sized CSR arrays carry a tail of empty segments whose cell entries are not written,
-// and a wave must never own them.
-__device__ inline void wave_map(int w, const int *__restrict__ start, int n, int *map)
+// and a wave must never own them. Written without a search: segment s (s <= n) is the answer
+// for exactly the waves with start[s-1] < 64 w <= start[s] (start[-1] = -1), and the waves
+// with 64 w > start[n] get n. Thread i handles segment i and wave i.
+__device__ inline void wave_map(int i, const int *__restrict__ start, int n, int waves, int *map)
 {
-    const int p = w * WAVE;
-    int lo = 0, hi = n;
-    while (lo < hi)
+    const int total = start[n];
+    if (i <= n)
     {
-        const int mid = (lo + hi) >> 1;
-        if (start[mid] >= p)
-            hi = mid;
-        else
-            lo = mid + 1;
+        const int si = start[i];
+        const int lo = i == 0 ? 0 : start[i - 1] / WAVE + 1;
+        const int hi = min(si / WAVE, waves);
+        for (int w = lo; w <= hi; w++)
+        {
+            map[2 * w] = i;
+            map[2 * w + 1] = si;
+        }
     }
-    map[2 * w] = lo;
-    map[2 * w + 1] = start[lo];
+    if (i <= waves && (long long)i * WAVE > total)
+    {
+        map[2 * i] = n;
+        map[2 * i + 1] = total;
+    }
 }
 
 // geometry, counts, 3x3x3 marks, grid-cache dirty flags and interpolation weights of every
@@ -238,6 +247,9 @@ __global__ __launch_bounds__(B) void k_tri_update(const float *__restrict__ src,
     const int i = blockIdx.x * B + threadIdx.x;
     if (dst && i < nv3)
         dst[i] = src[i];
+    // the face-pair scan of the previous update has consumed its tile sums
+    for (int q = i; q < cn.ts.n3; q += gridDim.x * B)
+        cn.ts.b3[q] = 0;
     if (i >= ba.nt)
         return;
     TriGeom g;
@@ -314,17 +326,9 @@ __global__ __launch_bounds__(B) void k_lists_fill(ListArgs a, int r1, int r2, in
             a.cn.mark[lin3(g.cell3.x + o % 3 - 1, g.cell3.y + o / 3 % 3 - 1, g.cell3.z + o / 9 - 1, N)] = 0;
     }
     else if (b < r4)
-    {
-        const int w = (b - r3) * B + threadIdx.x;
-        if (w <= a.c3_waves)
-            wave_map(w, a.c3_start, min(*a.c3_n, a.cap_c3), a.c3_wave);
-    }
+        wave_map((b - r3) * B + threadIdx.x, a.c3_start, min(*a.c3_n, a.cap_c3), a.c3_waves, a.c3_wave);
     else
-    {
-        const int w = (b - r4) * B + threadIdx.x;
-        if (w <= a.face_waves)
-            wave_map(w, a.face_start, a.nt, a.face_wave);
-    }
+        wave_map((b - r4) * B + threadIdx.x, a.face_start, a.nt, a.face_waves, a.face_wave);
 }
 
 // 4x4x4 lists of the non-empty base cells (for inspection; the time step reads the
@@ -385,7 +389,7 @@ hipError_t launch_lists_count(const ListArgs &a, hipStream_t s)
 hipError_t launch_lists_fill(const ListArgs &a, hipStream_t s)
 {
     const int n3 = nblk_strided(a.cap_c3), nf = nblk_strided(a.nt), nt = nblk(a.nt);
-    const int m3 = nblk(a.c3_waves + 1LL), mf = nblk(a.face_waves + 1LL);
+    const int m3 = nblk(std::max(a.c3_waves, a.cap_c3) + 1LL), mf = nblk(std::max(a.face_waves, a.nt) + 1LL);
     const int r1 = n3, r2 = r1 + nf, r3 = r2 + nt, r4 = r3 + m3;
     k_lists_fill<<<r4 + mf, B, 0, s>>>(a, r1, r2, r3, r4);
     return hipGetLastError();

@@ -61,9 +61,16 @@ struct ScanBatch
     ScanJob job[SCAN_MAX_JOBS];
     int njobs;
 };
+constexpr int SCAN_TILE = 4096;  // items per tile of a multi-tile scan
 // tmp must hold scan_batch_tmp_ints(sizes) ints
 size_t scan_batch_tmp_ints(const int *n, int njobs);
-hipError_t scan_batch(ScanBatch b, int *tmp, hipStream_t s);
+// sets job[k].tile0 (the tiles of all jobs laid out in job order); returns the tile count
+int scan_batch_layout(ScanBatch &b);
+// true when every job fits one tile of the single-launch scan (no tile sums are read)
+bool scan_batch_single(const ScanBatch &b);
+// presums (may be null): per-tile sums the producers of the inputs already accumulated, in the
+// layout of scan_batch_layout (tile of item q of job k = tile0 + q / SCAN_TILE); null: computed
+hipError_t scan_batch(ScanBatch b, int *tmp, hipStream_t s, const int *presums = nullptr);
 
 // ---------------------------------------------------------------- build.hip (binning, lists)
 struct WeightArgs
@@ -81,10 +88,18 @@ struct BuildArgs
 hipError_t launch_fill_int(int *a, long long n, int value, hipStream_t s);
 hipError_t launch_copy_verts(const float *series, long long stride, int t0, const StepState *st, float *verts, int n,
                              hipStream_t s);
+// per-tile counts of the dirty face pairs, accumulated by the face lookup for the face-pair scan
+// (layout of scan_batch_layout; zero between updates: k_tri_update clears them)
+struct TileSums
+{
+    int *b3;
+    int n3;
+};
 // counters of a mesh update, zero between updates (N^3 ints each)
 struct UpdateCounters
 {
     int *cell_cnt, *base_cnt, *cell_fill, *mark;
+    TileSums ts;
 };
 struct ListArgs
 {
@@ -139,10 +154,11 @@ struct PrevFaces
 {
     const int *start, *nbr, *slot, *cur_start;
 };
+// tsum (may be null): per-tile counts of the dirty flags (tile = pair / SCAN_TILE), accumulated
 hipError_t launch_face_lookup(const TriGeom *geom, const int *tris, const int *owner, const int *face_nbr,
                               const int *n_ptr, int cap, unsigned long long *hkeys, int *hvals, int hcap,
                               int *pool_count, int pool_cap, float2 *fmeta, int *face_slot, int *dirty, int *err_flag,
-                              int nt, const PrevFaces &prev, hipStream_t s);
+                              int nt, const PrevFaces &prev, int *tsum, hipStream_t s);
 hipError_t launch_face_weights(const float *verts, const int *tris, const TriGeom *geom, const int *dirty_pairs,
                                const int *n_ptr, int cap, const int *owner, const int *face_nbr, const int *face_slot,
                                const float *interp, const WeightArgs &a, const Tdbem &b, void *tmp, uint32_t *fw,

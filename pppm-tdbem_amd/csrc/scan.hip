@@ -22,6 +22,7 @@ namespace
 constexpr int SB = 256;        // threads per block
 constexpr int SI = 16;         // items per thread (multi-tile scans)
 constexpr int SPB = SB * SI;   // items per tile
+static_assert(SPB == SCAN_TILE, "tile size shared with the producers of tile sums");
 constexpr int SI1 = 32;        // items per thread when one tile holds everything
 
 __device__ inline int wave_incl(int v)
@@ -195,27 +196,44 @@ size_t scan_batch_tmp_ints(const int *n, int njobs)
 
 // Jobs with at most SB * SI1 items each run in one launch (a block per job); otherwise a
 // tile-sums launch and a tile-scan launch cover every job's tiles.
-hipError_t scan_batch(ScanBatch b, int *tmp, hipStream_t s)
+int scan_batch_layout(ScanBatch &b)
 {
-    if (b.njobs < 1 || b.njobs > SCAN_MAX_JOBS)
-        return hipErrorInvalidValue;
-    bool small = true;
     int tiles = 0;
     for (int k = 0; k < b.njobs; k++)
     {
-        if (b.job[k].n < 0 || b.job[k].n >= (1 << 30))  // positions are kept as (pos << 1 | bit)
-            return hipErrorInvalidValue;
-        small = small && b.job[k].n <= SB * SI1;
         b.job[k].tile0 = tiles;
         tiles += (int)std::max<long long>(1, ((long long)b.job[k].n + SPB - 1) / SPB);
     }
-    if (small)
+    return tiles;
+}
+
+bool scan_batch_single(const ScanBatch &b)
+{
+    for (int k = 0; k < b.njobs; k++)
+        if (b.job[k].n > SB * SI1)
+            return false;
+    return true;
+}
+
+hipError_t scan_batch(ScanBatch b, int *tmp, hipStream_t s, const int *presums)
+{
+    if (b.njobs < 1 || b.njobs > SCAN_MAX_JOBS)
+        return hipErrorInvalidValue;
+    for (int k = 0; k < b.njobs; k++)
+        if (b.job[k].n < 0 || b.job[k].n >= (1 << 30))  // positions are kept as (pos << 1 | bit)
+            return hipErrorInvalidValue;
+    const int tiles = scan_batch_layout(b);
+    if (scan_batch_single(b))
     {
         k_tile_scan<SI1><<<b.njobs, SB, 0, s>>>(b, nullptr);
         return hipGetLastError();
     }
-    k_tile_sums<<<tiles, SB, 0, s>>>(b, tmp);
-    k_tile_scan<SI><<<tiles, SB, 0, s>>>(b, tmp);
+    if (!presums)
+    {
+        k_tile_sums<<<tiles, SB, 0, s>>>(b, tmp);
+        presums = tmp;
+    }
+    k_tile_scan<SI><<<tiles, SB, 0, s>>>(b, presums);
     return hipGetLastError();
 }
 
@@ -229,7 +247,7 @@ hipError_t scan_exclusive(const int *in, int *out, int n, int *tmp, hipStream_t 
     b.job[0].in = in;
     b.job[0].n = n;
     b.job[0].pos = out;
-    return scan_batch(b, tmp, s);
+    return scan_batch(b, tmp, s, nullptr);
 }
 
 }  // namespace pppm_amd

@@ -130,6 +130,7 @@ struct pppm_solver
 
     // [cell_cnt | base_cnt | cell_fill | mark], 4 x N^3, all zero between updates
     DBuf<int> counters;
+    DBuf<int> tsums;  // per-tile counts of the dirty face pairs, zero between updates
     bool counters_zero = false;
     DBuf<int> cell_start, cell_items, cell_rank;
     DBuf<int> scan_tmp;
@@ -461,10 +462,36 @@ int enqueue_rebuild(pppm_solver *s, bool first, const float *verts_series, long 
 
     if (verts_series)
         CK(launch_copy_verts(verts_series, vstride, t0, s->state.p, s->verts.p, s->nv * 3, st));
+    UpdateCounters cn{s->counters.p, s->counters.p + NN, s->counters.p + 2 * NN, s->counters.p + 3 * NN, {}};
+
+    // the update's three scan batches (capacity-sized, so their tile layouts are host-known);
+    // the face lookup accumulates the tile sums of the third
+    ScanBatch b1{}, b2{}, b3{};
+    b1.njobs = 4;
+    b1.job[0] = ScanJob{cn.cell_cnt, NN, 0, s->cell_start.p, nullptr, nullptr, nullptr, 0};
+    b1.job[1] = ScanJob{cn.base_cnt, NN, 1, nullptr, s->b4_cells.p, s->b4_index.p, stp + ST_B4, 0};
+    b1.job[2] = ScanJob{cn.mark, NN, 1, nullptr, s->c3_cells.p, s->c3_index.p, stp + ST_C3, 0};
+    b1.job[3] = ScanJob{s->gflag.p, nt, 1, nullptr, s->glist.p, nullptr, stp + ST_GDIRTY, 0};
+    b2.njobs = 2;
+    b2.job[0] = ScanJob{s->c3_cnt.p, s->cap_c3, 0, s->c3_start.p, nullptr, nullptr, nullptr, 0};
+    b2.job[1] = ScanJob{s->face_cnt.p, nt, 0, s->face_start.p, nullptr, nullptr, stp + ST_P4, 0};
+    b3.njobs = 1;
+    b3.job[0] = ScanJob{s->face_flag.p, s->cap_p4, 1, nullptr, s->dirty_list.p, nullptr, stp + ST_FDIRTY, 0};
+    TileSums &ts = cn.ts;
+    ts.n3 = scan_batch_layout(b3);
+    const size_t nsum = (size_t)ts.n3;
+    if (s->tsums.n < nsum)
+    {
+        CK(s->tsums.ensure(nsum));
+        s->counters_zero = false;
+    }
+    ts.b3 = s->tsums.p;
     if (!s->counters_zero)
+    {
         CK(launch_fill_int(s->counters.p, 4LL * NN, 0, st));
+        CK(launch_fill_int(s->tsums.p, (long long)nsum, 0, st));
+    }
     s->counters_zero = false;
-    const UpdateCounters cn{s->counters.p, s->counters.p + NN, s->counters.p + 2 * NN, s->counters.p + 3 * NN};
 
     // geometry, counts, 3x3x3 marks, grid-cache dirty flags (GridCache::update_cache,
     // pppm_cache.cu:125-134), interpolation weights (update_interpolation_weight_kernel)
@@ -475,15 +502,7 @@ int enqueue_rebuild(pppm_solver *s, bool first, const float *verts_series, long 
                          first ? 1 : 0, s->gflag.p, s->interp.p, cn, err, st));
 
     // cell CSR offsets; compaction of non-empty base cells, marked cells, dirty triangles
-    {
-        ScanBatch b{};
-        b.njobs = 4;
-        b.job[0] = ScanJob{cn.cell_cnt, NN, 0, s->cell_start.p, nullptr, nullptr, nullptr, 0};
-        b.job[1] = ScanJob{cn.base_cnt, NN, 1, nullptr, s->b4_cells.p, s->b4_index.p, stp + ST_B4, 0};
-        b.job[2] = ScanJob{cn.mark, NN, 1, nullptr, s->c3_cells.p, s->c3_index.p, stp + ST_C3, 0};
-        b.job[3] = ScanJob{s->gflag.p, nt, 1, nullptr, s->glist.p, nullptr, stp + ST_GDIRTY, 0};
-        CK(scan_batch(b, s->scan_tmp.p, st));
-    }
+    CK(scan_batch(b1, s->scan_tmp.p, st));
 
     const ListArgs la = list_args(s, cn);
 
@@ -491,13 +510,7 @@ int enqueue_rebuild(pppm_solver *s, bool first, const float *verts_series, long 
     // its base cell), their offsets, the lists themselves, corner map, gather wave maps and the
     // reset of the counters
     CK(launch_lists_count(la, st));
-    {
-        ScanBatch b{};
-        b.njobs = 2;
-        b.job[0] = ScanJob{s->c3_cnt.p, s->cap_c3, 0, s->c3_start.p, nullptr, nullptr, nullptr, 0};
-        b.job[1] = ScanJob{s->face_cnt.p, nt, 0, s->face_start.p, nullptr, nullptr, stp + ST_P4, 0};
-        CK(scan_batch(b, s->scan_tmp.p, st));
-    }
+    CK(scan_batch(b2, s->scan_tmp.p, st));
     CK(launch_lists_fill(la, st));
     s->counters_zero = true;
 
@@ -516,13 +529,8 @@ int enqueue_rebuild(pppm_solver *s, bool first, const float *verts_series, long 
                           use_prev ? PrevFaces{s->prev_face_start.p, s->prev_face_nbr.p, s->prev_face_slot.p,
                                                s->face_start.p}
                                    : PrevFaces{nullptr, nullptr, nullptr, nullptr},
-                          st));
-    {
-        ScanBatch b{};
-        b.njobs = 1;
-        b.job[0] = ScanJob{s->face_flag.p, s->cap_p4, 1, nullptr, s->dirty_list.p, nullptr, stp + ST_FDIRTY, 0};
-        CK(scan_batch(b, s->scan_tmp.p, st));
-    }
+                          ts.b3, st));
+    CK(scan_batch(b3, s->scan_tmp.p, st, ts.b3));
     CK(launch_face_weights(s->verts.p, s->tris.p, s->geom.p, s->dirty_list.p, stp + ST_FDIRTY, s->cap_p4,
                            s->face_owner.p, s->face_nbr.p, s->face_slot.p, s->interp.p, wa, s->bem, s->ftmp.p, s->fw.p,
                            s->fsc.p, s->fmeta.p, err, fhint, st));
@@ -1364,7 +1372,7 @@ int pppm_get_b4(pppm_solver *s, int *cells, int *start, int *tris)
     const int n = s->st_h[ST_B4], nt = s->nt;
     // the time step reads per-triangle face lists; the base-cell lists are built on request
     const int NN = s->N * s->N * s->N;
-    const UpdateCounters cn{s->counters.p, s->counters.p + NN, s->counters.p + 2 * NN, s->counters.p + 3 * NN};
+    const UpdateCounters cn{s->counters.p, s->counters.p + NN, s->counters.p + 2 * NN, s->counters.p + 3 * NN, {}};
     ListArgs la = list_args(s, cn);
     CK(launch_b4_lists(la, 0, s->stream));
     {

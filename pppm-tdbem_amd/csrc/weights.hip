@@ -327,8 +327,31 @@ __global__ __launch_bounds__(GW) void k_grid_weights(const float *__restrict__ v
     cdbl(*xs)[STEPS] = reinterpret_cast<cdbl(*)[STEPS]>(pool);
     __shared__ cdbl bv[GC][9][4];
     static_assert(sizeof(cdbl) * 9 * 81 * 2 <= sizeof(pool), "phase-1 values fit the transform buffer");
+    // per-block copies of the target table, the wave numbers, twiddles and lambda powers
+    __shared__ int s_off[81];
+    __shared__ signed char s_idx[125];
+    __shared__ double s_wr[9], s_wi[9], s_twr[STEPS / 2], s_twi[STEPS / 2], s_inv[STEPS];
     const int n = live_count(n_ptr, err);
     const int tid = threadIdx.x;
+    if ((int)blockIdx.x >= n)
+        return;  // the launch is sized from an estimate; blocks without a triangle do nothing
+    if (tid < 81)
+        s_off[tid] = (c_t81.off[tid][0] + 2) * 25 + (c_t81.off[tid][1] + 2) * 5 + (c_t81.off[tid][2] + 2);
+    if (tid < 125)
+        s_idx[tid] = c_t81.idx[tid];
+    if (tid < 9)
+    {
+        s_wr[tid] = bem.wave_re[tid];
+        s_wi[tid] = bem.wave_im[tid];
+    }
+    if (tid < STEPS / 2)
+    {
+        s_twr[tid] = bem.tw_re[tid];
+        s_twi[tid] = bem.tw_im[tid];
+    }
+    if (tid < STEPS)
+        s_inv[tid] = bem.inv_pow[tid];
+    __syncthreads();
     for (int li = blockIdx.x; li < n; li += gridDim.x)
     {
         const int tri = list[li];
@@ -339,8 +362,10 @@ __global__ __launch_bounds__(GW) void k_grid_weights(const float *__restrict__ v
         for (int e = tid; e < 9 * 81; e += GW)
         {
             const int k = e / 81, j = e % 81;
-            const d3 pt = cell_center_d(c.x + c_t81.off[j][0], c.y + c_t81.off[j][1], c.z + c_t81.off[j][2], a);
-            layer64_pair(p, pt, T.n, bem.wave_re[k], bem.wave_im[k], gl[k][j][0], gl[k][j][1]);
+            const int q = s_off[j];
+            const int ox = q / 25 - 2, oy = q / 5 % 5 - 2, oz = q % 5 - 2;
+            const d3 pt = cell_center_d(c.x + ox, c.y + oy, c.z + oz, a);
+            layer64_pair(p, pt, T.n, s_wr[k], s_wi[k], gl[k][j][0], gl[k][j][1]);
         }
         __syncthreads();
         if (tid < GC * 9)
@@ -357,7 +382,7 @@ __global__ __launch_bounds__(GW) void k_grid_weights(const float *__restrict__ v
                 const int ex = ox + dx[q], ey = oy + dy[q], ez = oz + dz[q];
                 if (ex == 0 && ey == 0 && ez == 0)
                     continue;  // triangle_in_grid_cache: no target in the triangle's own cell (pppm_cache.cu:7-19)
-                const int slot = c_t81.idx[(ex + 2) * 25 + (ey + 2) * 5 + (ez + 2)];
+                const int slot = s_idx[(ex + 2) * 25 + (ey + 2) * 5 + (ez + 2)];
                 const double sc = dsc[q];
                 sr0 += gl[k][slot][0].re * sc;
                 si0 += gl[k][slot][0].im * sc;
@@ -369,7 +394,7 @@ __global__ __launch_bounds__(GW) void k_grid_weights(const float *__restrict__ v
             bv[o][k][1] = {f * sr1, f * si1};
             if (!(ox == 0 && oy == 0 && oz == 0))
             {
-                const int slot = c_t81.idx[(ox + 2) * 25 + (oy + 2) * 5 + (oz + 2)];
+                const int slot = s_idx[(ox + 2) * 25 + (oy + 2) * 5 + (oz + 2)];
                 double cr0 = 0, ci0 = 0, cr1 = 0, ci1 = 0;
                 cr0 += gl[k][slot][0].re * 1.0;
                 ci0 += gl[k][slot][0].im * 1.0;
@@ -382,39 +407,30 @@ __global__ __launch_bounds__(GW) void k_grid_weights(const float *__restrict__ v
                 bv[o][k][2] = bv[o][k][3] = {0, 0};
         }
         __syncthreads();
-        // phase 3: scaled_idft of the 108 series (o, value) in LDS, butterfly by butterfly in
-        // the order-independent stages of the radix-2 transform (each butterfly of a stage
-        // touches its own two entries)
+        // phase 3: scaled_idft of the 108 series (o, value), 16 lanes per series, one entry of
+        // the bit-reversed input per lane; each radix-2 stage exchanges butterfly partners with
+        // a lane shuffle, and both partners form the twiddled term from the same operands, so
+        // every entry gets the operations of the sequential butterflies
         for (int e = tid; e < GC * 4 * STEPS; e += GW)
         {
             const int tr = e / STEPS, jj = e % STEPS, o = tr / 4, v = tr % 4;
             const int k = bitrev4(jj);
-            xs[tr][jj] = k <= 8 ? bv[o][k][v] : cdbl{bv[o][16 - k][v].re, -bv[o][16 - k][v].im};
-        }
-        __syncthreads();
+            cdbl x = k <= 8 ? bv[o][k][v] : cdbl{bv[o][16 - k][v].re, -bv[o][16 - k][v].im};
 #pragma unroll
-        for (int L = 2; L <= STEPS; L <<= 1)
-        {
-            const int half = L >> 1, tws = STEPS / L;
-            for (int e = tid; e < GC * 4 * (STEPS / 2); e += GW)
+            for (int L = 2; L <= STEPS; L <<= 1)
             {
-                const int tr = e / (STEPS / 2), bf = e % (STEPS / 2);
-                const int g = (bf / half) * L, m = bf % half;
-                const double cr = bem.tw_re[m * tws], ci = bem.tw_im[m * tws];
-                const cdbl y = xs[tr][g + m + half];
+                const int half = L >> 1, tws = STEPS / L;
+                const bool lower = (jj & half) == 0;
+                const cdbl other = {__shfl_xor(x.re, half, STEPS), __shfl_xor(x.im, half, STEPS)};
+                const cdbl y = lower ? other : x, a0 = lower ? x : other;
+                const int m = jj & (half - 1);
+                const double cr = s_twr[m * tws], ci = s_twi[m * tws];
                 const double tr_ = cr * y.re - ci * y.im;
                 const double ti_ = cr * y.im + ci * y.re;
-                const cdbl a0 = xs[tr][g + m];
-                xs[tr][g + m] = {a0.re + tr_, a0.im + ti_};
-                xs[tr][g + m + half] = {a0.re - tr_, a0.im - ti_};
+                x = lower ? cdbl{a0.re + tr_, a0.im + ti_} : cdbl{a0.re - tr_, a0.im - ti_};
             }
-            __syncthreads();
-        }
-        for (int e = tid; e < GC * 4 * STEPS; e += GW)
-        {
-            const int tr = e / STEPS, n2 = e % STEPS, o = tr / 4, v = tr % 4;
             // offset (0, 0, 0) has no centre target: its centre taps are exactly 0
-            xs[tr][n2].re = (v >= 2 && o == 13) ? 0.0 : xs[tr][n2].re / STEPS * bem.inv_pow[n2];
+            xs[tr][jj].re = (v >= 2 && o == 13) ? 0.0 : x.re / STEPS * s_inv[jj];
         }
         __syncthreads();
         // phase 4: one lane per (o, record, tap); the max-abs scales of LayerWeightHalf::set
@@ -484,21 +500,17 @@ constexpr unsigned long long EMPTY_KEY = ~0ull;
 // pppm_cache.h:134-147): (c, j) -> persistent slot; dirty if new or moved past the thresholds.
 // dirty[p] is 0 or 1; pairs past the live count write 0 (the flags are scanned over the
 // capacity).
-__global__ __launch_bounds__(256) void k_face_lookup(const TriGeom *__restrict__ geom, const int *__restrict__ tris,
-                                                     const int *__restrict__ owner, const int *__restrict__ face_nbr,
-                                                     const int *__restrict__ n_ptr, int cap,
-                                                     unsigned long long *hkeys, int *hvals, int hcap, int *pool_count,
-                                                     int pool_cap, float2 *fmeta, int *face_slot, int *dirty,
-                                                     int *err_flag, int nt, PrevFaces prev)
+__device__ inline int lookup_pair(int p, const TriGeom *__restrict__ geom, const int *__restrict__ owner,
+                                  const int *__restrict__ face_nbr, const int *__restrict__ n_ptr,
+                                  unsigned long long *hkeys, int *hvals, int hcap, int *pool_count, int pool_cap,
+                                  float2 *fmeta, int *face_slot, int *dirty, int *err_flag, int nt,
+                                  const PrevFaces &prev)
 {
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= cap)
-        return;
     // lists past an overflow (or an invalid mesh) were not written: touch nothing
     if (p >= live_count(n_ptr, err_flag))
     {
         dirty[p] = 0;
-        return;
+        return 0;
     }
     const int c = owner[p], j = face_nbr[p];
     if ((unsigned)c >= (unsigned)nt || (unsigned)j >= (unsigned)nt)
@@ -506,7 +518,7 @@ __global__ __launch_bounds__(256) void k_face_lookup(const TriGeom *__restrict__
         atomicOr(err_flag, ERR_CAPACITY);
         dirty[p] = 0;
         face_slot[p] = -1;
-        return;
+        return 0;
     }
     int slot = -1;
     bool fresh = false;
@@ -557,7 +569,7 @@ __global__ __launch_bounds__(256) void k_face_lookup(const TriGeom *__restrict__
     if (slot < 0)
     {
         dirty[p] = 0;
-        return;
+        return 0;
     }
     const TriGeom gc = geom[c], gj = geom[j];
     const float normal_angle = dotf(gj.normal, gc.normal);
@@ -577,9 +589,27 @@ __global__ __launch_bounds__(256) void k_face_lookup(const TriGeom *__restrict__
         if (fresh)
             fmeta[slot] = make_float2(INFINITY, INFINITY);
         dirty[p] = 1;
+        return 1;
     }
-    else
-        dirty[p] = 0;
+    dirty[p] = 0;
+    return 0;
+}
+
+__global__ __launch_bounds__(256) void k_face_lookup(const TriGeom *__restrict__ geom, const int *__restrict__ owner,
+                                                     const int *__restrict__ face_nbr, const int *__restrict__ n_ptr,
+                                                     int cap, unsigned long long *hkeys, int *hvals, int hcap,
+                                                     int *pool_count, int pool_cap, float2 *fmeta, int *face_slot,
+                                                     int *dirty, int *err_flag, int nt, PrevFaces prev, int *tsum)
+{
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    int f = 0;
+    if (p < cap)
+        f = lookup_pair(p, geom, owner, face_nbr, n_ptr, hkeys, hvals, hcap, pool_count, pool_cap, fmeta, face_slot,
+                        dirty, err_flag, nt, prev);
+    // one atomic per wave for the scan's tile sums (64 pairs never straddle a tile)
+    const unsigned long long bal = __ballot(f);
+    if (tsum && (threadIdx.x & 63) == 0 && bal)
+        atomicAdd(&tsum[(p & ~63) / SCAN_TILE], (int)__popcll(bal));
 }
 
 // Face-cache integrals (pppm_cache.cu:234-271) per (dirty pair, k <= 8, layer): the
@@ -831,17 +861,19 @@ hipError_t launch_grid_weights(const float *verts, const int *tris, const TriGeo
                                long long hint_dirty, hipStream_t s)
 {
     const long long hint = hint_dirty < 0 ? cap : std::min<long long>(hint_dirty, cap);
-    const long long blocks = std::max<long long>(64, std::min<long long>(hint, 8192));
+    // blocks stride over the dirty triangles; 1024 blocks keep every CU busy (one 768-thread
+    // block is resident per CU at a time)
+    const long long blocks = std::max<long long>(64, std::min<long long>(hint, 1024));
     k_grid_weights<<<(unsigned)blocks, GW, 0, s>>>(verts, tris, geom, list, n_ptr, a, b, wf, sf, wb, sb, meta, err);
     return hipGetLastError();
 }
 hipError_t launch_face_lookup(const TriGeom *geom, const int *tris, const int *owner, const int *face_nbr,
                               const int *n_ptr, int cap, unsigned long long *hkeys, int *hvals, int hcap,
                               int *pool_count, int pool_cap, float2 *fmeta, int *face_slot, int *dirty, int *err_flag,
-                              int nt, const PrevFaces &prev, hipStream_t s)
+                              int nt, const PrevFaces &prev, int *tsum, hipStream_t s)
 {
-    k_face_lookup<<<nb(cap, 256), 256, 0, s>>>(geom, tris, owner, face_nbr, n_ptr, cap, hkeys, hvals, hcap,
-                                               pool_count, pool_cap, fmeta, face_slot, dirty, err_flag, nt, prev);
+    k_face_lookup<<<nb(cap, 256), 256, 0, s>>>(geom, owner, face_nbr, n_ptr, cap, hkeys, hvals, hcap, pool_count,
+                                               pool_cap, fmeta, face_slot, dirty, err_flag, nt, prev, tsum);
     return hipGetLastError();
 }
 hipError_t launch_face_weights(const float *verts, const int *tris, const TriGeom *geom, const int *dirty_pairs,

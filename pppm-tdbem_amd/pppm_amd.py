@@ -30,6 +30,8 @@ EXPORTS = [
     "pppm_set_face_weights", "pppm_get_interp_weights", "pppm_get_c3", "pppm_get_b4",
     "pppm_set_timing", "pppm_get_timing", "pppm_run_moving", "pppm_classify", "pppm_fill_fresh",
     "pppm_set_solid_zero", "pppm_get_cells", "pppm_set_time_idx",
+    "pppm_modal_create", "pppm_modal_destroy", "pppm_modal_run", "pppm_modal_state", "pppm_modal_set_state",
+    "pppm_modal_surface_matrix", "pppm_modal_coeffs",
 ]
 
 # LayerWeightHalf record (src/algorithm/bem.h:163-169): 16 x fp16, f32, 16 x fp16, f32
@@ -89,6 +91,13 @@ def load_library(path: str = LIB_PATH):
         "pppm_set_solid_zero": ([vp], i),
         "pppm_get_cells": ([vp, vp, vp, vp, vp, vp, vp], i),
         "pppm_set_time_idx": ([vp, i], i),
+        "pppm_modal_create": ([P(vp), vp, i, i, vp, vp, i, vp, f, f, f, vp], i),
+        "pppm_modal_destroy": ([vp], i),
+        "pppm_modal_run": ([vp, vp, i, vp, i], i),
+        "pppm_modal_state": ([vp, vp], i),
+        "pppm_modal_set_state": ([vp, vp], i),
+        "pppm_modal_surface_matrix": ([vp, vp], i),
+        "pppm_modal_coeffs": ([vp, vp], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -405,3 +414,60 @@ def _hip_runtime():
         _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
         _hip.hipMemcpy.restype = ctypes.c_int
     return _hip
+
+
+class ModalSource:
+    """Modal sound source of a rigid body on the GPU (RigidBody, src/utils/RigidBody.h:53-166):
+    eig (3 nv, M) eigenvector matrix, eigvals (M,), the surface triangles and unit normals, the
+    material's Rayleigh damping (alpha, beta) and the audio sample rate."""
+
+    def __init__(self, eig, eigvals, tris, normals, alpha, beta, sample_rate, stream=None):
+        self.lib = load_library()
+        e = np.ascontiguousarray(eig, np.float32)
+        ev = np.ascontiguousarray(eigvals, np.float32)
+        t = np.ascontiguousarray(tris, np.int32)
+        n = np.ascontiguousarray(normals, np.float32)
+        self.nt, self.nmodes = t.shape[0], e.shape[1]
+        self.h = ctypes.c_void_p()
+        st = ctypes.c_void_p(stream) if stream else None
+        rc = self.lib.pppm_modal_create(ctypes.byref(self.h), e.ctypes.data_as(ctypes.c_void_p), e.shape[0] // 3,
+                                        self.nmodes, ev.ctypes.data_as(ctypes.c_void_p),
+                                        t.ctypes.data_as(ctypes.c_void_p), self.nt, n.ctypes.data_as(ctypes.c_void_p),
+                                        float(alpha), float(beta), float(sample_rate), st)
+        if rc != 0:
+            raise PPPMError(f"pppm_modal_create failed ({rc})")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.pppm_modal_destroy(self.h)
+            self.h = None
+
+    def run(self, forces, accs):
+        """forces (S, M) host array or device tensor; accs (S, T) float32 device tensor."""
+        fp, fdev = _ptr(forces)
+        ap, adev = _ptr(accs)
+        if not adev:
+            raise PPPMError("accs must be device memory")
+        rc = self.lib.pppm_modal_run(self.h, fp, int(forces.shape[0]), ap, fdev)
+        if rc != 0:
+            raise PPPMError(f"pppm_modal_run failed ({rc})")
+
+    def _get(self, fn, shape):
+        out = np.zeros(shape, np.float32)
+        if fn(self.h, out.ctypes.data_as(ctypes.c_void_p)) != 0:
+            raise PPPMError("modal query failed")
+        return out
+
+    def state(self):
+        return self._get(self.lib.pppm_modal_state, (self.nmodes, 3))
+
+    def set_state(self, q3):
+        q = np.ascontiguousarray(q3, np.float32)
+        if self.lib.pppm_modal_set_state(self.h, q.ctypes.data_as(ctypes.c_void_p)) != 0:
+            raise PPPMError("pppm_modal_set_state failed")
+
+    def surface_matrix(self):
+        return self._get(self.lib.pppm_modal_surface_matrix, (self.nt, self.nmodes))
+
+    def coeffs(self):
+        return self._get(self.lib.pppm_modal_coeffs, (self.nmodes, 3))

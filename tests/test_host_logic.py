@@ -81,3 +81,41 @@ def test_pairs_are_covered_once():
     for w in range(waves):
         seen[start[wmap[w]]:start[wmap[w + 1]]] += 1
     assert np.all(seen == 1)
+
+
+def wave_map_scatter(start, n, waves):
+    """The wave map as the device writes it, without a search: thread i stores segment i for the
+    waves with start[i-1] < 64 w <= start[i], and n for the waves with 64 w > start[n]; each
+    entry also carries the wave's first pair start[map[w]]."""
+    out = np.full((waves + 1, 2), -7, np.int64)
+    written = np.zeros(waves + 1, np.int64)
+    total = int(start[n])
+    for i in range(max(n, waves) + 1):
+        if i <= n:
+            lo = 0 if i == 0 else int(start[i - 1]) // WAVE + 1
+            hi = min(int(start[i]) // WAVE, waves)
+            for w in range(lo, hi + 1):
+                out[w] = (i, start[i])
+                written[w] += 1
+        if i <= waves and i * WAVE > total:
+            out[i] = (n, total)
+            written[i] += 1
+    assert np.all(written == 1), "every wave entry is written exactly once"
+    return out
+
+
+def test_scatter_wave_map_equals_search():
+    rng = np.random.default_rng(17)
+    cases = [(1, 1, 0), (5, 64, 0), (100, 1000, 0), (2000, 2500, 0), (64, 64, 0), (65, 200, 0)]
+    for n_live, cap, _ in cases:
+        for max_len in (3, 40, 200):
+            lens = np.zeros(cap, np.int64)
+            lens[:n_live] = rng.integers(0, max_len, n_live)   # empty segments included
+            start = np.zeros(cap + 1, np.int64)
+            start[1:] = np.cumsum(lens)
+            total = int(start[n_live])
+            for waves in (total // WAVE + 1, total // WAVE + 37):
+                ref = wave_map(start, cap, waves, n_live)
+                got = wave_map_scatter(start, n_live, waves)
+                assert np.array_equal(got[:, 0], ref), (n_live, cap, max_len, waves)
+                assert np.array_equal(got[:, 1], start[ref])
