@@ -68,8 +68,9 @@ def cpu_baseline(sc, steps, threads):
 CALIB_MIB = 256
 
 
-def measure_traffic(kernel, workload_args):
-    """HBM bytes per launch of `kernel` from two rocprofv3 counter passes over a short run of
+def measure_traffic(kernels, workload_args):
+    """HBM bytes per step of the kernels named in `kernels` (one launch each per step; their
+    per-dispatch averages are added) from two rocprofv3 counter passes over a short run of
     this script (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same
     counter reads for a plain 256 MiB device copy in the same run. Returns None if rocprofv3
     is unavailable or a pass fails."""
@@ -82,14 +83,14 @@ def measure_traffic(kernel, workload_args):
     res = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix="pppm_pmc_", dir="/tmp")
-        cmd = ["rocprofv3", "--pmc", counter, "--kernel-include-regex", kernel + "|copyBuffer",
+        cmd = ["rocprofv3", "--pmc", counter, "--kernel-include-regex", "|".join(kernels) + "|copyBuffer",
                "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable, os.path.abspath(__file__),
                "--steps", "20", "--warmup", "2", "--timing-steps", "2", "--no-cpu", "--no-traffic",
                "--calib-copy-mib", str(CALIB_MIB)] + workload_args
         try:
             subprocess.run(cmd, env=env, cwd=ROOT, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
                            timeout=240, check=True)
-            kvals, cvals = [], []
+            kvals, cvals = {k: [] for k in kernels}, []
             for path, _, files in os.walk(d):
                 for f in files:
                     if f.endswith("counter_collection.csv"):
@@ -97,16 +98,18 @@ def measure_traffic(kernel, workload_args):
                             if r["Counter_Name"] != counter:
                                 continue
                             v = float(r["Counter_Value"])
-                            if kernel in r["Kernel_Name"]:
-                                kvals.append(v)
-                            elif "copyBuffer" in r["Kernel_Name"]:
+                            if "copyBuffer" in r["Kernel_Name"]:
                                 cvals.append(v)
+                                continue
+                            for k in kernels:
+                                if k + "(" in r["Kernel_Name"] or r["Kernel_Name"].endswith(k):
+                                    kvals[k].append(v)
             big = max(cvals) if cvals else 0.0
-            if not kvals or big <= 0:
+            if any(not v for v in kvals.values()) or big <= 0:
                 return None
             # the copy reads and writes CALIB_MIB MiB
             factor = CALIB_MIB * 1024.0 / big
-            res[counter] = {"kib_per_launch": sum(kvals) / len(kvals), "scale": factor}
+            res[counter] = {"kib_per_launch": sum(sum(v) / len(v) for v in kvals.values()), "scale": factor}
         except (subprocess.SubprocessError, OSError, KeyError, ValueError):
             return None
         finally:
@@ -117,12 +120,67 @@ def measure_traffic(kernel, workload_args):
             "write_kib": w["kib_per_launch"], "write_scale": w["scale"]}
 
 
+def modal_bench(nt=20480, nmodes=200, nsamples=4096, calls=5, cpu=True):
+    """Modal sound source (surface accelerations of a rigid body) on one MI355X: nt surface
+    triangles (icosphere level 5 size), nmodes modes, blocks of nsamples audio samples per call.
+    Prints one JSON line; the step-path metric is the default run."""
+    import torch
+    import pppm_amd
+    rng = np.random.default_rng(11)
+    nv = nt // 2 + 2
+    tris = rng.integers(0, nv, (nt, 3)).astype(np.int32)
+    n = rng.standard_normal((nt, 3))
+    n = (n / np.linalg.norm(n, axis=1, keepdims=True)).astype(np.float32)
+    eig = (rng.standard_normal((3 * nv, nmodes)) * 0.1).astype(np.float32)
+    eigvals = ((2 * np.pi * np.sort(rng.uniform(100.0, 15000.0, nmodes))) ** 2).astype(np.float32)
+    ms = pppm_amd.ModalSource(eig, eigvals, tris, n, 30.0, 1e-6, 44100.0)
+    forces = torch.from_numpy((rng.standard_normal((nsamples, nmodes)) * 1e3).astype(np.float32)).cuda()
+    accs = torch.empty((nsamples, nt), dtype=torch.float32, device="cuda")
+    ms.run(forces, accs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        ms.run(forces, accs)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / calls
+    flops = 2.0 * nt * nmodes * nsamples
+    bytes_alg = 4.0 * (nt * nmodes + nsamples * nmodes + nsamples * nt)
+    res = {"metric": "modal surface accelerations/s", "value": nt * nsamples / el, "unit": "triangle-samples/s",
+           "n_gpus": 1, "ms_per_call": el * 1e3, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "modal_source", "triangles": nt, "modes": nmodes, "samples_per_call": nsamples},
+           "roofline": {"bound": "valu-fp32", "achieved": flops / el / 1e12, "unit": "TFLOP/s",
+                        "peak": 157.3, "frac": flops / el / 1e12 / 157.3,
+                        "note": "whole call (oscillators + accelerations); sequential FP32 sums in mode order, "
+                                "one fused multiply-add per term, no packed FP32 (ceiling 78.6 TFLOP/s); "
+                                "algorithmic bytes %.0f MB -> %.0f GB/s" % (bytes_alg / 1e6, bytes_alg / el / 1e9)}}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        c = oracle.modal_coeffs(eigvals, 1.0 / 44100.0, 30.0, 1e-6)
+        m = oracle.modal_surface_matrix(eig, tris, n)
+        state = np.zeros((nmodes, 3), np.float32)
+        f_all = forces.cpu().numpy()
+        ns, dt = 0, 0.0
+        while ns < nsamples and dt < 10.0:  # about 10 s of CPU work, in blocks of 256 samples
+            f = f_all[ns:ns + 256]
+            t0 = time.perf_counter()
+            q = oracle.modal_iir(c, 1.0 / 44100.0, state, f)
+            oracle.modal_accs(m, q)
+            dt += time.perf_counter() - t0
+            ns += f.shape[0]
+        res["cpu_baseline"] = {"value": nt * ns / dt, "unit": "triangle-samples/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle IIR + accelerations, {ns} samples, {dt:.2f} s"}
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's 4000)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="c2_moving", choices=["c2_moving", "c1_static_sphere"])
+    ap.add_argument("--workload", default="c2_moving", choices=["c2_moving", "c1_static_sphere", "large_static_sphere"],
+                    help="c2_moving: the metric's workload; c1_static_sphere: 32^3 static sphere; "
+                         "large_static_sphere: one 512^3 grid around the C5 object (HBM-resident levels)")
     ap.add_argument("--timing-steps", type=int, default=200, help="extra steps timed per kernel with HIP events")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
@@ -135,7 +193,12 @@ def main():
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB five times (a plain copy kernel of "
                          "known size, to scale FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--modal", action="store_true",
+                    help="measure the modal sound source instead (its own JSON line)")
     args = ap.parse_args()
+    if args.modal:
+        modal_bench(cpu=not args.no_cpu)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design note, "
                                    "not an implemented path", "n_gpus_requested": args.gpus}))
@@ -149,11 +212,12 @@ def main():
         print(json.dumps({"error": "no GPU"}))
         sys.exit(1)
 
-    default_steps = 4000 if args.workload == "c2_moving" else 1000
+    default_steps = {"c2_moving": 4000, "c1_static_sphere": 1000, "large_static_sphere": 500}[args.workload]
     K = args.steps or default_steps
     W = args.warmup
     total = W + K + args.timing_steps
-    sc = scenes.c2_moving(steps=total) if args.workload == "c2_moving" else scenes.c1_static_sphere(steps=total)
+    sc = {"c2_moving": scenes.c2_moving, "c1_static_sphere": scenes.c1_static_sphere,
+          "large_static_sphere": scenes.large_static_sphere}[args.workload](steps=total)
     if args.static_geometry:
         sc.motion = None
         sc.name = sc.name + "_static_geometry"
@@ -244,7 +308,9 @@ def main():
         },
         "hbm_gbps_effective_step_kernels": step_bytes / (sum(avg[k] for k in step_kernels) * 1e-3) / 1e9,
         "roofline": {
-            "kernel": {"fdtd": "k_fdtd_abc", "far": "k_far", "dirichlet": "k_dirichlet", "near": "k_near"}[dom],
+            # from an interior edge of 128 cells the FDTD step is two launches (marched interior, ABC shell)
+            "kernel": {"fdtd": "k_fdtd_march + k_fdtd_abc" if N - 4 >= 128 else "k_fdtd_abc", "far": "k_far",
+                       "dirichlet": "k_dirichlet", "near": "k_near"}[dom],
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBPS,
@@ -261,7 +327,7 @@ def main():
     if not args.no_traffic:
         kname = res["roofline"]["kernel"]
         wl = ["--workload", args.workload] + (["--static-geometry"] if args.static_geometry else [])
-        tr = measure_traffic(kname, wl)
+        tr = measure_traffic(kname.split(" + "), wl)
         if tr is not None:
             res["roofline"]["traffic"] = tr["bytes"]
             res["roofline"]["traffic_detail"] = {

@@ -159,6 +159,47 @@ int pppm_set_solid_zero(pppm_solver *s);
 int pppm_get_cells(pppm_solver *s, int *type, int *ghost_idx, int *nearest, float *point3, float *reflect3,
                    float *dist);
 
+/* Ghost-cell boundary solve of the comparison solver (GhostCellSolver,
+ * src/algorithm/ghost_cell.h:72-193), on the handle's grid and mesh. After pppm_classify:
+ * pppm_ghost_setup: ghost list; per ghost the 8x8 stencil system of the trilinear-plus basis
+ *   with the Neumann row at the nearest surface point (construct_phi_matrix_kernel,
+ *   ghost_cell.cu:254-300), its SVD, condition number max S / min S and the stencil weights
+ *   inv(Phi)^T phi(reflect) (precompute_p_weight_kernel, ghost_cell.cu:302-350,
+ *   inverse_from_svd_kernel, svd.cu:129-148); rows with a condition number above cond_threshold
+ *   use the first-order rule. cond_threshold = 0: first order everywhere and no linear solve
+ *   (the reference's thin-shell mode, ghost_cell.cu:490-495). second_order (optional): the
+ *   number of second-order rows. Synchronous. PPPM_ERR_OUT_OF_GRID if a stencil leaves the grid.
+ * pppm_ghost_set_neumann: previous Neumann data <- current, current <- neumann (nt floats,
+ *   set_boundary_condition, ghost_cell.h:118-123).
+ * pppm_ghost_solve: right-hand side from the pressure at the current level and the Neumann
+ *   data, the sparse solve, and the solution written into the ghost cells (solve_ghost_cell,
+ *   ghost_cell.cu:522-540; BiCGSTAB_Solver::solve, sparse.cu:512-545). iterations (optional,
+ *   synchronises): the solver's iteration count.
+ * pppm_ghost_step: GhostCellSolver::update (ghost_cell.h:131-141): FDTD step, set_neumann,
+ *   solve, SOLID cells to zero.
+ * pppm_ghost_update_mesh: GhostCellSolver::update_mesh (ghost_cell.h:96-112): new vertices,
+ *   classification (thin-shell when the threshold is 0), fresh cells from the current and
+ *   previous Neumann data, ghost setup with the same threshold. fresh_count as pppm_fill_fresh.
+ * pppm_ghost_get: host copies (any pointer may be NULL) of the ghost cells' linear indices,
+ *   order (1 = second order), weights (8 per ghost), condition numbers, last right-hand side,
+ *   last solution, and the sparse rows as 9 (column, value) slots per ghost (column -1: empty;
+ *   slot 8: the unit diagonal). */
+int pppm_ghost_setup(pppm_solver *s, float cond_threshold, int *second_order);
+int pppm_ghost_set_neumann(pppm_solver *s, const float *neumann, int on_device);
+int pppm_ghost_solve(pppm_solver *s, int *iterations);
+int pppm_ghost_step(pppm_solver *s, const float *neumann, int on_device);
+int pppm_ghost_update_mesh(pppm_solver *s, const float *verts, int on_device, int *fresh_count);
+int pppm_ghost_get(pppm_solver *s, int *cells, int *order, float *weights8, float *cond, float *rhs, float *x,
+                   int *cols9, float *vals9);
+
+/* pppm_ffat_collect: far-field (FFAT) map accumulation at the current time level
+ * (collect_ffat_map, experiments/static/main.cu:60-80 and experiments/acoustic/main.cu:46-74):
+ * map_dev (device memory, 6 x 2w x 2w floats, row-major) gets map(b, x, y) += |p(cell)| with
+ * face b = 2 i + (sign > 0) of axis i and cell = c + sign w e_i + (x - w) e_{i+1} + (y - w) e_{i+2}
+ * (axes cyclic, c = (N/2, N/2, N/2)). Requires 1 <= w and the cube inside the grid
+ * (w <= N/2 - 1 for even N); the reference does not check. Asynchronous on the handle's stream. */
+int pppm_ffat_collect(pppm_solver *s, int w, float *map_dev);
+
 /* Modal sound source of a rigid body (RigidBody, src/utils/RigidBody.h:53-166) on the GPU: the
  * surface modal matrix, the per-mode IIR oscillators and the surface accelerations that feed
  * the solver's Neumann input. A handle owns its device data and runs on its stream.
@@ -169,9 +210,11 @@ int pppm_get_cells(pppm_solver *s, int *type, int *ghost_idx, int *nearest, floa
  *   Rayleigh damping alpha, beta and timestep = 1 / sample_rate (RigidBody::InitIIR_); the
  *   surface matrix is MatrixVertToTri (RigidBody.cu:206-230): per triangle and mode the mean of
  *   the three vertex displacements dotted with the triangle normal. Oscillator state starts at 0.
- * pppm_modal_run: for each of nsamples samples, the IIR step of every mode with the modal
- *   forces of that sample (forces: nsamples x nmodes, row-major) and the surface accelerations
- *   acc[s][t] = sum_i M(t, i) Q_i (CalculateIIR_ / update_surf_acc_kernel, RigidBody.cu:240-295),
+ *   nt <= 65535 * 256.
+ * pppm_modal_run: for each of nsamples samples (0 <= nsamples <= 2^30), the IIR step of every
+ *   mode with the modal forces of that sample (forces: nsamples x nmodes, row-major) and the
+ *   surface accelerations acc[s][t] = sum_i M(t, i) Q_i, summed in mode order with one fused
+ *   multiply-add per term (CalculateIIR_ / update_surf_acc_kernel, RigidBody.cu:240-295),
  *   written to accs (nsamples x nt, device memory). on_device: where forces live.
  * pppm_modal_state / pppm_modal_set_state: the oscillators' (q1, q2, q3) per mode (nmodes x 3,
  *   host memory), for RigidBody::SaveState / LoadState (RigidBody.cu:60-88).

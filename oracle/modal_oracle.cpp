@@ -7,7 +7,10 @@
 //   orc_modal_accs            update_surf_acc_kernel          src/utils/RigidBody.cu:240-250
 //   orc_modal_forces          CalculateIIR_, impulse part     src/utils/RigidBody.cu:257-281,
 //                             Impulse::amp / dead             src/utils/RigidBody.h:19-31
-// FP32 throughout, in the reference's operation order (built with -ffp-contract=off).
+// FP32 throughout, in the reference's operation order (built with -ffp-contract=off). The
+// reference's host code (coefficients, oscillators, forces) is uncontracted; its device sum
+// `acc += M(t, i) * Q[i]` is contracted by its device compiler into a fused multiply-add, which
+// orc_modal_accs states with std::fma.
 #include <cmath>
 #include <vector>
 
@@ -82,7 +85,7 @@ void orc_modal_accs(const float *msurf, int nt, int nmodes, const float *q, int 
         {
             float a = 0;
             for (int i = 0; i < nmodes; i++)
-                a += msurf[(size_t)t * nmodes + i] * q[(size_t)s * nmodes + i];
+                a = std::fma(msurf[(size_t)t * nmodes + i], q[(size_t)s * nmodes + i], a);
             acc[(size_t)s * nt + t] = a;
         }
 }

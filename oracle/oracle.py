@@ -84,6 +84,7 @@ def lib():
             "orc_modal_surface_matrix": ([vp, i, i, vp, i, vp, vp], None),
             "orc_modal_iir": ([vp, i, f, vp, vp, i, vp], None),
             "orc_modal_accs": ([vp, i, i, vp, i, vp], None),
+            "orc_ffat_collect": ([vp, i, i, vp], None),
             "orc_modal_forces": ([vp, i, vp, vp, vp, vp, i, f, vp], None),
         }
         for name, (a, r) in sig.items():
@@ -394,6 +395,15 @@ def modal_accs(msurf, q):
     out = np.zeros((qq.shape[0], m.shape[0]), np.float32)
     lib().orc_modal_accs(_p(m), m.shape[0], m.shape[1], _p(qq), qq.shape[0], _p(out))
     return out
+
+
+def ffat_collect(grid, w, fmap):
+    """collect_ffat_map: fmap (6, 2w, 2w) float32 += |p| on the six faces of the cube of half
+    width w around the grid centre (in place)."""
+    g = np.ascontiguousarray(grid, np.float32)
+    assert fmap.dtype == np.float32 and fmap.flags.c_contiguous and fmap.shape == (6, 2 * w, 2 * w)
+    lib().orc_ffat_collect(_p(g), g.shape[0], int(w), _p(fmap))
+    return fmap
 
 
 def modal_forces(eig, imp_time, imp_vert, imp_vec, active, time):

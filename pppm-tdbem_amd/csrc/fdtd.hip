@@ -10,9 +10,13 @@
 // blocks solve the 4x4 ABC system for the index-1 / N-2 shell. Interior and shell
 // write disjoint cells of level t+1; the shell also writes the ghost layer of level t,
 // which no thread of this launch reads except the writing thread itself (masked
-// terms of its own system), so the fusion is race-free. t is read from device memory
-// (so the launch can be replayed from a graph); a one-thread kernel advances it after.
+// terms of its own system), so the fusion is race-free. From an interior edge of 128 cells
+// (grids beyond the Infinity Cache) the interior is instead marched along x by its own
+// launch before the shell launch (k_fdtd_march). t is read from device memory (so the
+// launches can be replayed from a graph); a one-thread kernel advances it after.
 #include "internal.hpp"
+
+#include <algorithm>
 
 namespace pppm_amd
 {
@@ -21,6 +25,8 @@ namespace
 {
 
 constexpr int BLK = 256;
+// interior edge (N - 4) from which the interior is marched along x instead of one thread per cell
+constexpr int kMarchMin = 128;
 
 __device__ inline float *level(float *g3, int t, int rel, size_t nn)
 {
@@ -120,6 +126,89 @@ __device__ inline void abc_cell(float *g, const float *gm, float *gp, int N, int
             g[me + judge[d] * strd[d]] = r[d];
 }
 
+// Leapfrog update of one interior cell from its six neighbours, in the reference's FP64
+// operation order: sum x-, x+, y-, y+, z-, z+, then -6 * centre; / h^2; 2 p + coef lap - p_prev
+__device__ __forceinline__ float leapfrog(float xm, float xp, float ym, float yp, float zm, float zp, float gc,
+                                          float gmv, const FdtdParams &p)
+{
+    double sum = 0;
+    sum += xm;
+    sum += xp;
+    sum += ym;
+    sum += yp;
+    sum += zm;
+    sum += zp;
+    sum -= 6 * gc;
+    const double lap = sum / p.hh;
+    double v = 2 * gc;
+    v += p.coef * lap;
+    v += -gmv;
+    return (float)v;
+}
+
+// Interior of a large grid: a block owns MTY x 64 (y, z) columns of one chunk of x planes and
+// marches them along x, so the x-neighbours come from registers and the y/z neighbours of the
+// current plane from cache: each level-t value is fetched from HBM about once instead of once
+// per plane that reads it. A tile's 64 z-columns start on a 256-byte boundary (z = 64 tz + lane,
+// cells outside 2..N-3 idle) so a wave's row is two whole cache lines; the 16 rows of a block
+// are marched by its 16 waves together, so only the two rows past the tile come from another
+// block. Distinct noalias parameters let the compiler issue the loads of the next planes ahead
+// of this plane's store.
+constexpr int MTY = 16;
+constexpr int MBLK = 64 * MTY;
+__device__ __forceinline__ void march_columns(const float *__restrict__ g, const float *__restrict__ gm,
+                                              float *__restrict__ gp, int N, int x0, int x1, int y, int z,
+                                              const FdtdParams &p)
+{
+    const int NN2 = N * N;
+    int me = lin3(x0, y, z, N);
+    float gxm = g[me - NN2], gc = g[me], gxp = g[me + NN2];
+#pragma unroll 4
+    for (int x = x0; x < x1; x++)
+    {
+        const float gym = g[me - N], gyp = g[me + N], gzm = g[me - 1], gzp = g[me + 1];
+        const float gnx = g[me + 2 * NN2];  // x + 2 <= N - 1
+        const float gmv = gm[me];
+        gp[me] = leapfrog(gxm, gxp, gym, gyp, gzm, gzp, gc, gmv, p);
+        gxm = gc;
+        gc = gxp;
+        gxp = gnx;
+        me += NN2;
+    }
+}
+
+// Marched-interior layout: tiles of MTY x 64 columns times chunks of xch planes, tile index z
+// fastest, then y, then the x chunk; n_tiles of them, launched as n_blocks (a multiple of 8).
+struct MarchShape
+{
+    int ntz, nty, xch, n_tiles, n_blocks;
+};
+
+__global__ __launch_bounds__(MBLK) void k_fdtd_march(float *__restrict__ grid3, FdtdParams p,
+                                                     const StepState *__restrict__ st, MarchShape ms)
+{
+    const int N = p.N;
+    const size_t nn = (size_t)N * N * N;
+    const int t = st->t;
+    // blocks go to the 8 XCDs in turn; give each XCD a contiguous run of tiles so that
+    // neighbouring tiles (and their shared halo rows) meet in one L2
+    const int b = blockIdx.x, per = ms.n_blocks >> 3;
+    const int tile = (b & 7) * per + (b >> 3);
+    if (tile >= ms.n_tiles)
+        return;
+    const int tz = tile % ms.ntz, r = tile / ms.ntz;
+    const int ty = r % ms.nty, xc = r / ms.nty;
+    const int z = tz * 64 + (int)(threadIdx.x & 63);
+    const int y = 2 + ty * MTY + (int)(threadIdx.x >> 6);
+    if (y > N - 3 || z < 2 || z > N - 3)
+        return;
+    const int x0 = 2 + xc * ms.xch;
+    const int x1 = min(x0 + ms.xch, N - 2);
+    march_columns(level(grid3, t, 0, nn), level(grid3, t, -1, nn), level(grid3, t, 1, nn), N, x0, x1, y, z, p);
+}
+
+// One thread per interior cell (blocks [0, nb_inner)) and the ABC shell (the remaining
+// blocks); nb_inner = 0 when the interior is marched by k_fdtd_march instead.
 __global__ __launch_bounds__(BLK) void k_fdtd_abc(float *__restrict__ grid3, FdtdParams p, const StepState *__restrict__ st,
                                                   int nb_inner)
 {
@@ -140,26 +229,14 @@ __global__ __launch_bounds__(BLK) void k_fdtd_abc(float *__restrict__ grid3, Fdt
             const int y = 2 + rem / M, z = 2 + rem % M;
             const int me = lin3(x, y, z, N);
             // laplacian + update in FP64, sum order x-, x+, y-, y+, z-, z+, -6*centre (fdtd.cu:7-33)
-            double sum = 0;
-            sum += g[me - N * N];
-            sum += g[me + N * N];
-            sum += g[me - N];
-            sum += g[me + N];
-            sum += g[me - 1];
-            sum += g[me + 1];
-            const float gc = g[me];
-            sum -= 6 * gc;
-            const double lap = sum / p.hh;
-            double v = 2 * gc;
-            v += p.coef * lap;
-            v += -gm[me];
-            gp[me] = (float)v;
+            gp[me] = leapfrog(g[me - N * N], g[me + N * N], g[me - N], g[me + N], g[me - 1], g[me + 1], g[me], gm[me],
+                              p);
         }
     }
     else
     {
         const int S = N - 2;
-        long long idx = (long long)(blockIdx.x - nb_inner) * BLK + threadIdx.x;
+        long long idx = (long long)((int)blockIdx.x - nb_inner) * BLK + threadIdx.x;
         const long long fx = 2LL * S * S, fy = 2LL * (S - 2) * S, fz = 2LL * (S - 2) * (S - 2);
         int i = -1, j = 0, k = 0;
         if (idx < fx)
@@ -281,7 +358,21 @@ hipError_t launch_fdtd(float *grid3, float *pml3, const FdtdParams &p, const Ste
     }
     const long long M = N - 4, S = N - 2;
     const long long inner = M * M * M, shell = S * S * S - inner;
-    const int nb_inner = (int)((inner + BLK - 1) / BLK);
+    int nb_inner = (int)((inner + BLK - 1) / BLK);
+    if (M >= kMarchMin)
+    {
+        // about 2048 tiles or more; chunks of at least 16 planes (each chunk re-reads 2 planes)
+        MarchShape ms{};
+        ms.ntz = (N - 3) / 64 + 1;  // z tiles [64 tz, 64 tz + 64) up to z = N - 3
+        ms.nty = (int)((M + MTY - 1) / MTY);
+        const long long cols = (long long)ms.ntz * ms.nty;
+        const long long nxc = std::max<long long>(1, (2048 + cols - 1) / cols);
+        ms.xch = (int)std::max<long long>(16, (M + nxc - 1) / nxc);
+        ms.n_tiles = (int)(cols * ((M + ms.xch - 1) / ms.xch));
+        ms.n_blocks = (ms.n_tiles + 7) / 8 * 8;
+        k_fdtd_march<<<ms.n_blocks, MBLK, 0, s>>>(grid3, p, st, ms);
+        nb_inner = 0;
+    }
     const int nb_shell = (int)((shell + BLK - 1) / BLK);
     k_fdtd_abc<<<nb_inner + nb_shell, BLK, 0, s>>>(grid3, p, st, nb_inner);
     return hipGetLastError();

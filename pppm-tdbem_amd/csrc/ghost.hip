@@ -1,0 +1,530 @@
+// Ghost-cell boundary solve of the comparison solver (gfx950): per ghost cell an 8x8 stencil
+// system, a sparse system coupling neighbouring ghost cells, solved every step.
+//
+// Reference behaviour: construct_ghost_cell_list, construct_phi_matrix_kernel,
+// precompute_p_weight_kernel, construct_equation_kernel, update_ghost_cell_kernel,
+// GhostCellSolver::precompute_ghost_matrix / solve_ghost_cell (src/algorithm/ghost_cell.cu:
+// 220-545); the SVD pseudo-inverse of cusolver_svd / inverse_from_svd_kernel
+// (src/algorithm/svd.cu:36-157); the BiCGSTAB solve of BiCGSTAB_Solver::solve
+// (src/algorithm/sparse.cu:137-320, 512-545).
+//
+// Design:
+//  - the stencil matrices (FP32 entries exactly as the reference builds them) are decomposed
+//    in FP64 by one-sided Jacobi, eight lanes per matrix (lane r holds row r of the matrix and
+//    of the accumulated rotation), with the column dot products reduced by lane shuffles; the
+//    stencil weights p_weight = inv(Phi)^T phi(r) come out of the same factors;
+//  - a ghost row has at most 9 entries (itself and the 8 stencil cells), so the sparse matrix
+//    is stored as 9 (column, value) slots per row, no sort or compaction needed;
+//  - the per-step system is solved by BiCGSTAB in FP64 inside one work-group (LDS tree
+//    reductions in a fixed order: deterministic), on the right-hand side scaled to unit RMS
+//    as the reference does. The reference preconditions with ILU(0) and stops at a relative
+//    residual of 1e-6 in FP32; here the rows are already unit-diagonal, no preconditioner is
+//    applied and the solve runs to a relative residual of 1e-10 in FP64, so the result is the
+//    system's solution to well within the reference's own tolerance.
+#include "internal.hpp"
+
+#include <cmath>
+
+namespace pppm_amd
+{
+
+namespace
+{
+constexpr int GB = 256;
+constexpr int SOLVE_THREADS = 1024;
+constexpr int GHOST_MAX_ITER = 300;
+constexpr double GHOST_TOL = 1e-10;  // relative residual of the scaled system
+
+__device__ inline int3 cell_of(int lin, int N) { return make_int3(lin / (N * N), (lin / N) % N, lin % N); }
+
+__device__ inline f3 center_of(int x, int y, int z, const CellArgs &a)
+{
+    return f3{((float)x + 0.5f) * a.h, ((float)y + 0.5f) * a.h, ((float)z + 0.5f) * a.h} + f3{a.minx, a.miny, a.minz};
+}
+
+// get_base_coord_for_reflect (ghost_cell.cu:20-26): int3((r - min) / h - 0.5), truncation
+__device__ inline int3 base_for_reflect(f3 r, const CellArgs &a)
+{
+    const f3 q = (r - f3{a.minx, a.miny, a.minz}) / a.h;
+    return make_int3((int)(q.x - 0.5f), (int)(q.y - 0.5f), (int)(q.z - 0.5f));
+}
+
+// get_normalized_coord (ghost_cell.cu:12-17): (p - centre(base)) / h * 2 - 1, per component
+__device__ inline f3 normalized(f3 p, int3 b, const CellArgs &a)
+{
+    const f3 c = center_of(b.x, b.y, b.z, a);
+    const f3 d = (p - c) / a.h;
+    return f3{d.x * 2 + -1.0f, d.y * 2 + -1.0f, d.z * 2 + -1.0f};
+}
+
+__device__ inline float dotf(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+__device__ inline bool inside(int3 c, int N) { return c.x >= 0 && c.y >= 0 && c.z >= 0 && c.x < N && c.y < N && c.z < N; }
+
+// construct_ghost_cell_list (ghost_cell.cu:220-234): cell index of each ghost, by its ghost index
+__global__ void k_ghost_list(long long NN, const int *__restrict__ ctype, const int *__restrict__ cghost,
+                             int *__restrict__ gcell)
+{
+    const long long c = (long long)blockIdx.x * GB + threadIdx.x;
+    if (c < NN && ctype[c] == CELL_GHOST)
+        gcell[cghost[c]] = (int)c;
+}
+
+// Row i of the stencil matrix of a ghost (construct_phi_matrix_kernel, ghost_cell.cu:254-300)
+__device__ inline void phi_row(int i, int3 base, int3 gc, f3 normal, f3 nearest, const CellArgs &a, float row[8])
+{
+    const int3 d = make_int3(i % 2, (i / 2) % 2, i / 4);
+    const int3 nb = make_int3(base.x + d.x, base.y + d.y, base.z + d.z);
+    if (nb.x == gc.x && nb.y == gc.y && nb.z == gc.z)
+    {
+        // the ghost itself: the normal derivative of the basis at the nearest surface point
+        const f3 c = normalized(nearest, base, a);
+        row[0] = dotf(normal, f3{c.y * c.z, c.x * c.z, c.x * c.y});
+        row[1] = dotf(normal, f3{c.y, c.x, 0});
+        row[2] = dotf(normal, f3{0, c.z, c.y});
+        row[3] = dotf(normal, f3{c.z, 0, c.x});
+        row[4] = dotf(normal, f3{1, 0, 0});
+        row[5] = dotf(normal, f3{0, 1, 0});
+        row[6] = dotf(normal, f3{0, 0, 1});
+        row[7] = 0;
+    }
+    else
+    {
+        // basis [xyz, xy, yz, xz, x, y, z, 1] at the corner (2d - 1) of [-1, 1]^3
+        const float cx = (float)(d.x * 2 - 1), cy = (float)(d.y * 2 - 1), cz = (float)(d.z * 2 - 1);
+        row[0] = cx * cy * cz;
+        row[1] = cx * cy;
+        row[2] = cy * cz;
+        row[3] = cx * cz;
+        row[4] = cx;
+        row[5] = cy;
+        row[6] = cz;
+        row[7] = 1;
+    }
+}
+
+__device__ inline double sum8(double v)
+{
+    v += __shfl_xor(v, 1, 8);
+    v += __shfl_xor(v, 2, 8);
+    v += __shfl_xor(v, 4, 8);
+    return v;
+}
+
+// Stencil weights of every ghost: eight lanes per ghost. The SVD of Phi = U S V^T comes from
+// one-sided Jacobi rotations applied to Phi's columns (and accumulated in V); the weights
+// p = inv(Phi)^T phi(r) = U f(S) V^T phi(r) with f(s) = 1/s for s > 1e-6 and s otherwise
+// (inverse_from_svd_kernel, svd.cu:129-148). Condition number max S / min S above the
+// threshold, or a NaN weight, selects the first-order rule (precompute_p_weight_kernel,
+// ghost_cell.cu:302-350).
+__global__ __launch_bounds__(GB) void k_ghost_weights(CellArgs a, int n, const int *__restrict__ gcell,
+                                                      const int *__restrict__ cnearest,
+                                                      const float *__restrict__ cpoint,
+                                                      const float *__restrict__ creflect, float threshold,
+                                                      int *__restrict__ order, float *__restrict__ pw,
+                                                      float *__restrict__ cond, int *__restrict__ err)
+{
+    const int gid = (int)(((long long)blockIdx.x * GB + threadIdx.x) >> 3);
+    const int r = threadIdx.x & 7;
+    const bool live = gid < n;
+    const int g = live ? gid : n - 1;  // whole groups of eight stay together through the shuffles
+    const int c = gcell[g];
+    const int3 gc = cell_of(c, a.N);
+    const f3 nearest{cpoint[3 * (size_t)c], cpoint[3 * (size_t)c + 1], cpoint[3 * (size_t)c + 2]};
+    const f3 reflect{creflect[3 * (size_t)c], creflect[3 * (size_t)c + 1], creflect[3 * (size_t)c + 2]};
+    const f3 dn = reflect - nearest;
+    const float inv_len = 1.0f / sqrtf(dotf(dn, dn));
+    const f3 normal = dn * inv_len;
+    const int3 base = base_for_reflect(reflect, a);
+    if (live && r == 0 && (!inside(base, a.N) || !inside(make_int3(base.x + 1, base.y + 1, base.z + 1), a.N)))
+        atomicOr(err, ERR_OUT_OF_GRID);
+    if (live && r == 0 && cnearest[c] < 0)
+        atomicOr(err, ERR_BAD_INDEX);  // a ghost without a nearest triangle has no boundary data
+    float rowf[8];
+    phi_row(r, base, gc, normal, nearest, a, rowf);
+    double A[8], V[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+    {
+        A[j] = (double)rowf[j];
+        V[j] = (j == r) ? 1.0 : 0.0;
+    }
+    // cyclic one-sided Jacobi; all eight lanes of a group take the same decisions
+    for (int sweep = 0; sweep < 30; sweep++)
+    {
+        bool rotated = false;
+#pragma unroll
+        for (int p = 0; p < 7; p++)
+        {
+#pragma unroll
+            for (int q = p + 1; q < 8; q++)
+            {
+                const double al = sum8(A[p] * A[p]);
+                const double be = sum8(A[q] * A[q]);
+                const double ga = sum8(A[p] * A[q]);
+                if (fabs(ga) > 1e-15 * sqrt(al * be) && ga != 0.0)
+                {
+                    rotated = true;
+                    const double zeta = (be - al) / (2.0 * ga);
+                    const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                    const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+                    const double ap = A[p], aq = A[q];
+                    A[p] = cs * ap - sn * aq;
+                    A[q] = sn * ap + cs * aq;
+                    const double vp = V[p], vq = V[q];
+                    V[p] = cs * vp - sn * vq;
+                    V[q] = sn * vp + cs * vq;
+                }
+            }
+        }
+        // every lane of the group computed the same sums, so `rotated` is group-uniform; the
+        // loop ends for the whole wave once no group rotates
+        if (!__any(rotated))
+            break;
+    }
+    double sig[8];
+    float smax = 0.0f, smin = 3.402823466e+38f;
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+    {
+        sig[j] = sqrt(sum8(A[j] * A[j]));
+        const float sf = (float)sig[j];
+        smax = sf > smax ? sf : smax;
+        smin = sf < smin ? sf : smin;
+    }
+    const float cn = smax / smin;
+    // phi(r) of the reflected point in the stencil's coordinates
+    const f3 rn = normalized(reflect, base, a);
+    const float phr[8] = {rn.x * rn.y * rn.z, rn.x * rn.y, rn.y * rn.z, rn.x * rn.z, rn.x, rn.y, rn.z, 1.0f};
+    // c_j = V[:, j] . phi(r); weight of row r = sum_j U[r][j] f(s_j) c_j, U[r][j] = A[r][j] / s_j
+    double w = 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+    {
+        const double cj = sum8(V[j] * (double)phr[r]);
+        if (sig[j] > 0.0)
+        {
+            const double f = sig[j] > 1e-6 ? 1.0 / sig[j] : sig[j];
+            w += (A[j] / sig[j]) * f * cj;
+        }
+    }
+    const float wf = (float)w;
+    const bool bad = cn > threshold;
+    const int nan_here = (wf != wf) ? 1 : 0;
+    int nan_any = nan_here;
+    nan_any |= __shfl_xor(nan_any, 1, 8);
+    nan_any |= __shfl_xor(nan_any, 2, 8);
+    nan_any |= __shfl_xor(nan_any, 4, 8);
+    if (!live)
+        return;
+    const bool second = !bad && !nan_any;
+    pw[(size_t)gid * 8 + r] = second ? wf : 0.0f;
+    if (r == 0)
+    {
+        order[gid] = second ? 1 : 0;
+        cond[gid] = cn;
+    }
+}
+
+// Matrix rows (construct_equation_kernel<true, false>, ghost_cell.cu:352-470, after
+// COOMatrix::eliminate_zeros): slot i < 8 holds (ghost index, -p_i) of stencil cell i when that
+// cell is another ghost and the entry is non-zero, slot 8 the unit diagonal; empty slots have
+// column -1. First-order rows hold only the diagonal.
+__global__ void k_ghost_matrix(CellArgs a, int n, const int *__restrict__ gcell, const float *__restrict__ creflect,
+                               const int *__restrict__ ctype, const int *__restrict__ cghost,
+                               const int *__restrict__ order, const float *__restrict__ pw, int *__restrict__ mcol,
+                               float *__restrict__ mval)
+{
+    const int g = blockIdx.x * GB + threadIdx.x;
+    if (g >= n)
+        return;
+    int *col = mcol + (size_t)g * 9;
+    float *val = mval + (size_t)g * 9;
+    for (int i = 0; i < 8; i++)
+    {
+        col[i] = -1;
+        val[i] = 0.0f;
+    }
+    col[8] = g;
+    val[8] = 1.0f;
+    if (!order[g])
+        return;
+    const int c = gcell[g];
+    const f3 reflect{creflect[3 * (size_t)c], creflect[3 * (size_t)c + 1], creflect[3 * (size_t)c + 2]};
+    const int3 base = base_for_reflect(reflect, a);
+    for (int i = 0; i < 8; i++)
+    {
+        const int3 nb = make_int3(base.x + i % 2, base.y + (i / 2) % 2, base.z + i / 4);
+        if (!inside(nb, a.N))
+            continue;
+        const int l = lin3(nb.x, nb.y, nb.z, a.N);
+        if (l == c || ctype[l] != CELL_GHOST)
+            continue;
+        const float v = -pw[(size_t)g * 8 + i];
+        if (v != 0.0f)
+        {
+            col[i] = cghost[l];
+            val[i] = v;
+        }
+    }
+}
+
+// Right-hand side (construct_equation_kernel<false, true>, ghost_cell.cu:352-470) from the
+// pressure at level t and the Neumann data per triangle; FP32 in the reference's order
+__global__ void k_ghost_rhs(CellArgs a, int n, const StepState *__restrict__ st, const float *__restrict__ grid3,
+                            const int *__restrict__ gcell, const float *__restrict__ creflect,
+                            const float *__restrict__ cdist, const int *__restrict__ cnearest,
+                            const int *__restrict__ ctype, const int *__restrict__ cghost,
+                            const int *__restrict__ order, const float *__restrict__ pw,
+                            const float *__restrict__ neu, float *__restrict__ b)
+{
+    const int g = blockIdx.x * GB + threadIdx.x;
+    if (g >= n)
+        return;
+    const int N = a.N;
+    const size_t nn = (size_t)N * N * N;
+    const int t = st->t;
+    const float *p = grid3 + (size_t)(((t % 3) + 3) % 3) * nn;
+    const int c = gcell[g];
+    const int tri = cnearest[c];
+    const float nv = tri >= 0 ? neu[tri] : 0.0f;  // set-up reports such a ghost as an error
+    if (!order[g])
+    {
+        // first order: mean over the AIR face neighbours of p_n - h a_n
+        const int3 gc = cell_of(c, N);
+        const int3 d6[6] = {make_int3(1, 0, 0),  make_int3(-1, 0, 0), make_int3(0, 1, 0),
+                            make_int3(0, -1, 0), make_int3(0, 0, 1),  make_int3(0, 0, -1)};
+        float acc = 0.0f;
+        int cnt = 0;
+        for (int i = 0; i < 6; i++)
+        {
+            const int3 q = make_int3(gc.x + d6[i].x, gc.y + d6[i].y, gc.z + d6[i].z);
+            if (!inside(q, N))
+                continue;
+            const int l = lin3(q.x, q.y, q.z, N);
+            if (ctype[l] == CELL_AIR)
+            {
+                acc += -nv * a.h + p[l];
+                cnt++;
+            }
+        }
+        b[g] = acc / (float)cnt;
+        return;
+    }
+    const f3 reflect{creflect[3 * (size_t)c], creflect[3 * (size_t)c + 1], creflect[3 * (size_t)c + 2]};
+    const int3 base = base_for_reflect(reflect, a);
+    float bv = 0.0f;
+    bv += -nv * cdist[c] * 2;
+    for (int i = 0; i < 8; i++)
+    {
+        const int3 nb = make_int3(base.x + i % 2, base.y + (i / 2) % 2, base.z + i / 4);
+        if (!inside(nb, N))
+            continue;
+        const int l = lin3(nb.x, nb.y, nb.z, N);
+        const float w = pw[(size_t)g * 8 + i];
+        if (cghost[l] == g && ctype[l] == CELL_GHOST)
+        {
+            const float scale = a.h / 2;  // stencil coordinates span [-1, 1]: d/dn scales by h / 2
+            bv += w * (scale * nv);
+        }
+        else if (ctype[l] != CELL_GHOST)
+            bv += w * p[l];
+    }
+    b[g] = bv;
+}
+
+// one-block reduction in a fixed tree order
+__device__ inline double block_sum(double v, double *red)
+{
+    const int tid = threadIdx.x;
+    red[tid] = v;
+    __syncthreads();
+    for (int s = SOLVE_THREADS / 2; s > 0; s >>= 1)
+    {
+        if (tid < s)
+            red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ inline void spmv(int n, const int *__restrict__ mcol, const float *__restrict__ mval, const double *x,
+                            double *y)
+{
+    for (int i = threadIdx.x; i < n; i += SOLVE_THREADS)
+    {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+        {
+            const int cidx = mcol[(size_t)i * 9 + k];
+            if (cidx >= 0)
+                acc += (double)mval[(size_t)i * 9 + k] * x[cidx];
+        }
+        y[i] = acc;
+    }
+}
+
+// BiCGSTAB on A x = b in one work-group. work holds 6 vectors of n doubles (r0, r, p, v, s,
+// t) and x (n doubles). Writes x as FP32 into xout and the iteration count into info[0].
+__global__ __launch_bounds__(SOLVE_THREADS) void k_ghost_bicgstab(int n, const int *__restrict__ mcol,
+                                                                  const float *__restrict__ mval,
+                                                                  const float *__restrict__ b, double *work,
+                                                                  float *__restrict__ xout, int max_iter, double tol,
+                                                                  int *__restrict__ info)
+{
+    __shared__ double red[SOLVE_THREADS];
+    double *r0 = work, *r = work + n, *pv = work + 2 * (size_t)n, *v = work + 3 * (size_t)n;
+    double *sv = work + 4 * (size_t)n, *tv = work + 5 * (size_t)n, *x = work + 6 * (size_t)n;
+    const int tid = threadIdx.x;
+    // scale b to unit RMS (BiCGSTAB_Solver::solve, sparse.cu:512-545)
+    double ss = 0.0;
+    for (int i = tid; i < n; i += SOLVE_THREADS)
+        ss += (double)b[i] * (double)b[i];
+    const double bn = sqrt(block_sum(ss, red) / (double)n);
+    if (bn == 0.0)
+    {
+        for (int i = tid; i < n; i += SOLVE_THREADS)
+            xout[i] = 0.0f;
+        if (tid == 0)
+            info[0] = 0;
+        return;
+    }
+    double rr = 0.0;
+    for (int i = tid; i < n; i += SOLVE_THREADS)
+    {
+        const double bi = (double)b[i] / bn;
+        r0[i] = bi;  // x0 = 0, so r0 = b
+        r[i] = bi;
+        x[i] = 0.0;
+        rr += bi * bi;
+    }
+    const double thr = tol * sqrt(block_sum(rr, red));
+    double rho_prev = 1.0, alpha = 1.0, omega = 1.0;
+    int it = 0;
+    for (it = 1; it <= max_iter; it++)
+    {
+        double lr = 0.0;
+        for (int i = tid; i < n; i += SOLVE_THREADS)
+            lr += r0[i] * r[i];
+        const double rho = block_sum(lr, red);
+        if (rho == 0.0)
+            break;
+        const double beta = (rho / rho_prev) * (alpha / omega);
+        for (int i = tid; i < n; i += SOLVE_THREADS)
+            pv[i] = it == 1 ? r[i] : r[i] + beta * (pv[i] - omega * v[i]);
+        __syncthreads();
+        spmv(n, mcol, mval, pv, v);
+        double lv = 0.0;
+        for (int i = tid; i < n; i += SOLVE_THREADS)
+            lv += r0[i] * v[i];
+        const double den = block_sum(lv, red);
+        if (den == 0.0)
+            break;
+        alpha = rho / den;
+        double ls = 0.0;
+        for (int i = tid; i < n; i += SOLVE_THREADS)
+        {
+            sv[i] = r[i] - alpha * v[i];
+            ls += sv[i] * sv[i];
+        }
+        const double ns = sqrt(block_sum(ls, red));
+        if (ns < thr)
+        {
+            for (int i = tid; i < n; i += SOLVE_THREADS)
+                x[i] += alpha * pv[i];
+            break;
+        }
+        spmv(n, mcol, mval, sv, tv);
+        double lts = 0.0, ltt = 0.0;
+        for (int i = tid; i < n; i += SOLVE_THREADS)
+        {
+            lts += tv[i] * sv[i];
+            ltt += tv[i] * tv[i];
+        }
+        const double ts = block_sum(lts, red), tt = block_sum(ltt, red);
+        omega = tt != 0.0 ? ts / tt : 0.0;
+        double lrr = 0.0;
+        for (int i = tid; i < n; i += SOLVE_THREADS)
+        {
+            x[i] += alpha * pv[i] + omega * sv[i];
+            r[i] = sv[i] - omega * tv[i];
+            lrr += r[i] * r[i];
+        }
+        const double nr = sqrt(block_sum(lrr, red));
+        if (nr < thr || omega == 0.0)
+            break;
+        rho_prev = rho;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += SOLVE_THREADS)
+        xout[i] = (float)(x[i] * bn);
+    if (tid == 0)
+        info[0] = it > max_iter ? max_iter : it;
+}
+
+// update_ghost_cell_kernel (ghost_cell.cu:510-520): the solution into the ghost cells at level t
+__global__ void k_ghost_write(int n, int N, const StepState *__restrict__ st, const int *__restrict__ gcell,
+                              const float *__restrict__ x, float *__restrict__ grid3)
+{
+    const int g = blockIdx.x * GB + threadIdx.x;
+    if (g >= n)
+        return;
+    const size_t nn = (size_t)N * N * N;
+    const int t = st->t;
+    grid3[(size_t)(((t % 3) + 3) % 3) * nn + gcell[g]] = x[g];
+}
+
+inline unsigned gblocks(long long n) { return (unsigned)(n > 0 ? (n + GB - 1) / GB : 1); }
+}  // namespace
+
+hipError_t launch_ghost_setup(const CellArgs &a, int n, const int *ctype, const int *cghost, const int *cnearest,
+                              const float *cpoint, const float *creflect, float threshold, int *gcell, int *order,
+                              float *pw, float *cond, int *mcol, float *mval, int *err, hipStream_t s)
+{
+    const long long NN = (long long)a.N * a.N * a.N;
+    k_ghost_list<<<gblocks(NN), GB, 0, s>>>(NN, ctype, cghost, gcell);
+    if (n <= 0)
+        return hipGetLastError();
+    if (threshold > 0.0f)
+        k_ghost_weights<<<gblocks(8LL * n), GB, 0, s>>>(a, n, gcell, cnearest, cpoint, creflect, threshold, order,
+                                                        pw, cond, err);
+    else
+    {
+        // a zero threshold means the first-order rule everywhere and no SVD
+        // (GhostCellSolver::precompute_ghost_matrix, ghost_cell.cu:490-495)
+        hipError_t e = launch_fill_int(order, n, 0, s);
+        if (e == hipSuccess)
+            e = launch_fill_int((int *)pw, 8LL * n, 0, s);
+        if (e == hipSuccess)
+            e = launch_fill_int((int *)cond, n, 0, s);
+        if (e != hipSuccess)
+            return e;
+    }
+    k_ghost_matrix<<<gblocks(n), GB, 0, s>>>(a, n, gcell, creflect, ctype, cghost, order, pw, mcol, mval);
+    return hipGetLastError();
+}
+
+hipError_t launch_ghost_solve(const CellArgs &a, int n, const StepState *st, float *grid3, const int *gcell,
+                              const float *creflect, const float *cdist, const int *cnearest, const int *ctype,
+                              const int *cghost, const int *order, const float *pw, const float *neu,
+                              const int *mcol, const float *mval, bool linear, float *b, float *x, double *work,
+                              int *info, hipStream_t s)
+{
+    if (n <= 0)
+        return hipSuccess;
+    k_ghost_rhs<<<gblocks(n), GB, 0, s>>>(a, n, st, grid3, gcell, creflect, cdist, cnearest, ctype, cghost, order,
+                                           pw, neu, b);
+    const float *sol = b;
+    if (linear)
+    {
+        k_ghost_bicgstab<<<1, SOLVE_THREADS, 0, s>>>(n, mcol, mval, b, work, x, GHOST_MAX_ITER, GHOST_TOL, info);
+        sol = x;
+    }
+    k_ghost_write<<<gblocks(n), GB, 0, s>>>(n, a.N, st, gcell, sol, grid3);
+    return hipGetLastError();
+}
+
+}  // namespace pppm_amd

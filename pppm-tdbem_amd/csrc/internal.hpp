@@ -201,6 +201,19 @@ hipError_t launch_fresh_fill(const CellArgs &a, const StepState *st, const int *
                              hipStream_t s);
 hipError_t launch_solid_zero(long long NN, const StepState *st, const int *ctype, float *grid3, hipStream_t s);
 
+// ---------------------------------------------------------------- ghost.hip
+// ghost list, stencil weights (order 1 = second order), 9-slot sparse rows of the ghost system
+hipError_t launch_ghost_setup(const CellArgs &a, int n, const int *ctype, const int *cghost, const int *cnearest,
+                              const float *cpoint, const float *creflect, float threshold, int *gcell, int *order,
+                              float *pw, float *cond, int *mcol, float *mval, int *err, hipStream_t s);
+// right-hand side at level t, the linear solve (if `linear`) and the ghost values into level t;
+// work: 7 n doubles, info[0]: iterations of the solve
+hipError_t launch_ghost_solve(const CellArgs &a, int n, const StepState *st, float *grid3, const int *gcell,
+                              const float *creflect, const float *cdist, const int *cnearest, const int *ctype,
+                              const int *cghost, const int *order, const float *pw, const float *neu,
+                              const int *mcol, const float *mval, bool linear, float *b, float *x, double *work,
+                              int *info, hipStream_t s);
+
 // ---------------------------------------------------------------- step.hip
 struct StepArgs
 {
@@ -228,6 +241,8 @@ hipError_t launch_near(const StepArgs &a, const StepState *st, float *grid3, con
                        const int *c3_start, const int *c3_wave, int c3_waves, const int *c3_pairs, const uint32_t *wb,
                        const float2 *sb, const float *hn, const float *hd, const float *wn, const float *wd,
                        const float *far, const int *listen_cell, const int *listen_c3, float *out, hipStream_t s);
+// FFAT map accumulation at the current time level: map is 6 x 2w x 2w, 1 <= w <= N/2 - 1
+hipError_t launch_ffat(const StepState *st, const float *grid3, int N, int w, float *map, hipStream_t s);
 hipError_t launch_listen_direct(const StepArgs &a, const StepState *st, const float *grid3, const int *listen_cell,
                                 float *out, hipStream_t s);
 hipError_t launch_set_neumann(const StepArgs &a, const StepState *st, float *hn, const float *neumann,

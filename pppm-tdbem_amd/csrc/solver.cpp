@@ -171,6 +171,16 @@ struct pppm_solver
     DBuf<float> cpoint, creflect, cdist, acc_stage;
     DBuf<int> vf_cnt, vf_start, vf_fill, vf_items;
     DBuf<float> vnormal;
+
+    // ghost-cell boundary solve (GhostCellSolver, ghost_cell.h:72-193): ghost list, stencil
+    // weights and order, 9-slot sparse rows, right-hand side and solution, the Neumann data of
+    // the current and the previous step, FP64 solver work vectors
+    bool ghost_ready = false, ghost_linear = false;
+    float ghost_threshold = 25.0f;
+    int gh_n = 0, gh_nt = 0, gh_iters = 0;
+    DBuf<int> gh_cell, gh_order, gh_mcol, gh_info, gh_err;
+    DBuf<float> gh_pw, gh_cond, gh_mval, gh_b, gh_x, gh_neu, gh_neu_old;
+    DBuf<double> gh_work;
     int st_h[ST_N] = {0};
     long long last_grid_recomputes = 0, last_face_recomputes = 0;
 
@@ -1163,6 +1173,17 @@ int pppm_read_cells(pppm_solver *s, const int *xyz, int n, float *host_out)
     return PPPM_OK;
 }
 
+int pppm_ffat_collect(pppm_solver *s, int w, float *map_dev)
+{
+    if (check_handle(s) || !map_dev)
+        return PPPM_ERR_ARG;
+    const int c = s->N / 2;
+    if (w < 1 || c - w < 0 || c + w > s->N - 1)
+        return fail(s, PPPM_ERR_ARG, "ffat_collect: the cube of half width w must lie inside the grid");
+    CK(launch_ffat(s->state.p, s->grid.p, s->N, w, map_dev, s->stream));
+    return PPPM_OK;
+}
+
 int pppm_history_ptrs(pppm_solver *s, float **neumann, float **dirichlet)
 {
     if (check_handle(s))
@@ -1502,6 +1523,157 @@ int pppm_set_solid_zero(pppm_solver *s)
         return fail(s, PPPM_ERR_STATE, "set_solid_zero before classify");
     const long long NN = (long long)s->N * s->N * s->N;
     CK(launch_solid_zero(NN, s->state.p, s->ctype.p, s->grid.p, s->stream));
+    return PPPM_OK;
+}
+
+int pppm_ghost_setup(pppm_solver *s, float cond_threshold, int *second_order)
+{
+    if (check_handle(s) || !(cond_threshold >= 0.0f))
+        return PPPM_ERR_ARG;
+    if (!s->classified)
+        return fail(s, PPPM_ERR_STATE, "ghost_setup before classify");
+    const int n = s->ghost_count, nt = s->nt;
+    hipStream_t st = s->stream;
+    CK(s->gh_cell.ensure(n));
+    CK(s->gh_order.ensure(n));
+    CK(s->gh_pw.ensure(8 * (size_t)n));
+    CK(s->gh_cond.ensure(n));
+    CK(s->gh_mcol.ensure(9 * (size_t)n));
+    CK(s->gh_mval.ensure(9 * (size_t)n));
+    CK(s->gh_b.ensure(n));
+    CK(s->gh_x.ensure(n));
+    CK(s->gh_work.ensure(7 * (size_t)n));
+    CK(s->gh_info.ensure(1));
+    CK(s->gh_err.ensure(1));
+    if (s->gh_nt != nt)
+    {
+        // the Neumann data start at zero (GhostCellSolver::set_mesh, ghost_cell.h:88-93)
+        CK(s->gh_neu.ensure(nt));
+        CK(s->gh_neu_old.ensure(nt));
+        CK(launch_fill_int((int *)s->gh_neu.p, nt, 0, st));
+        CK(launch_fill_int((int *)s->gh_neu_old.p, nt, 0, st));
+        s->gh_nt = nt;
+    }
+    CK(launch_fill_int(s->gh_err.p, 1, 0, st));
+    CK(launch_fill_int(s->gh_info.p, 1, 0, st));
+    const CellArgs a{s->N, s->h, s->minp[0], s->minp[1], s->minp[2], 0};
+    CK(launch_ghost_setup(a, n, s->ctype.p, s->cghost.p, s->cnearest.p, s->cpoint.p, s->creflect.p, cond_threshold,
+                          s->gh_cell.p, s->gh_order.p, s->gh_pw.p, s->gh_cond.p, s->gh_mcol.p, s->gh_mval.p,
+                          s->gh_err.p, st));
+    int err = 0;
+    std::vector<int> order(n > 0 ? n : 1);
+    CK(hipMemcpyAsync(&err, s->gh_err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    if (n > 0)
+        CK(hipMemcpyAsync(order.data(), s->gh_order.p, n * sizeof(int), hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    s->ghost_ready = false;
+    if (err & ERR_OUT_OF_GRID)
+        return fail(s, PPPM_ERR_OUT_OF_GRID, "ghost_setup: a ghost cell's stencil leaves the grid");
+    if (err)
+        return fail(s, PPPM_ERR_STATE, "ghost_setup: a ghost cell has no nearest triangle");
+    s->gh_n = n;
+    s->ghost_threshold = cond_threshold;
+    s->ghost_linear = cond_threshold > 0.0f;
+    s->ghost_ready = true;
+    if (second_order)
+    {
+        int c = 0;
+        for (int i = 0; i < n; i++)
+            c += order[i];
+        *second_order = c;
+    }
+    return PPPM_OK;
+}
+
+int pppm_ghost_set_neumann(pppm_solver *s, const float *neumann, int on_device)
+{
+    if (check_handle(s) || !neumann)
+        return PPPM_ERR_ARG;
+    if (!s->ghost_ready)
+        return fail(s, PPPM_ERR_STATE, "ghost_set_neumann before ghost_setup");
+    CK(hipMemcpyAsync(s->gh_neu_old.p, s->gh_neu.p, s->nt * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
+    RC(upload(s, s->gh_neu.p, neumann, s->nt * sizeof(float), on_device));
+    return PPPM_OK;
+}
+
+int pppm_ghost_solve(pppm_solver *s, int *iterations)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    if (!s->ghost_ready)
+        return fail(s, PPPM_ERR_STATE, "ghost_solve before ghost_setup");
+    const CellArgs a{s->N, s->h, s->minp[0], s->minp[1], s->minp[2], 0};
+    CK(launch_ghost_solve(a, s->gh_n, s->state.p, s->grid.p, s->gh_cell.p, s->creflect.p, s->cdist.p, s->cnearest.p,
+                          s->ctype.p, s->cghost.p, s->gh_order.p, s->gh_pw.p, s->gh_neu.p, s->gh_mcol.p,
+                          s->gh_mval.p, s->ghost_linear, s->gh_b.p, s->gh_x.p, s->gh_work.p, s->gh_info.p,
+                          s->stream));
+    if (iterations)
+    {
+        int it = 0;
+        if (s->ghost_linear && s->gh_n > 0)
+        {
+            CK(hipMemcpyAsync(&it, s->gh_info.p, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+            CK(hipStreamSynchronize(s->stream));
+        }
+        *iterations = it;
+    }
+    return PPPM_OK;
+}
+
+int pppm_ghost_step(pppm_solver *s, const float *neumann, int on_device)
+{
+    if (check_handle(s) || !neumann)
+        return PPPM_ERR_ARG;
+    if (!s->ghost_ready)
+        return fail(s, PPPM_ERR_STATE, "ghost_step before ghost_setup");
+    RC(pppm_fdtd_step(s));
+    RC(pppm_ghost_set_neumann(s, neumann, on_device));
+    RC(pppm_ghost_solve(s, nullptr));
+    return pppm_set_solid_zero(s);
+}
+
+int pppm_ghost_update_mesh(pppm_solver *s, const float *verts, int on_device, int *fresh_count)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    if (!s->ghost_ready)
+        return fail(s, PPPM_ERR_STATE, "ghost_update_mesh before ghost_setup");
+    RC(pppm_update_mesh(s, verts, on_device));
+    RC(pppm_classify(s, s->ghost_threshold == 0.0f ? 1 : 0, nullptr));
+    RC(pppm_fill_fresh(s, s->gh_neu.p, s->gh_neu_old.p, 1, fresh_count));
+    return pppm_ghost_setup(s, s->ghost_threshold, nullptr);
+}
+
+int pppm_ghost_get(pppm_solver *s, int *cells, int *order, float *weights8, float *cond, float *rhs, float *x,
+                   int *cols9, float *vals9)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    if (!s->ghost_ready)
+        return fail(s, PPPM_ERR_STATE, "ghost_get before ghost_setup");
+    const size_t n = s->gh_n;
+    hipStream_t st = s->stream;
+    if (n > 0)
+    {
+        if (cells)
+            CK(hipMemcpyAsync(cells, s->gh_cell.p, n * sizeof(int), hipMemcpyDeviceToHost, st));
+        if (order)
+            CK(hipMemcpyAsync(order, s->gh_order.p, n * sizeof(int), hipMemcpyDeviceToHost, st));
+        if (weights8)
+            CK(hipMemcpyAsync(weights8, s->gh_pw.p, 8 * n * sizeof(float), hipMemcpyDeviceToHost, st));
+        if (cond)
+            CK(hipMemcpyAsync(cond, s->gh_cond.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
+        if (rhs)
+            CK(hipMemcpyAsync(rhs, s->gh_b.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
+        if (x)
+            CK(hipMemcpyAsync(x, s->ghost_linear ? s->gh_x.p : s->gh_b.p, n * sizeof(float), hipMemcpyDeviceToHost,
+                              st));
+        if (cols9)
+            CK(hipMemcpyAsync(cols9, s->gh_mcol.p, 9 * n * sizeof(int), hipMemcpyDeviceToHost, st));
+        if (vals9)
+            CK(hipMemcpyAsync(vals9, s->gh_mval.p, 9 * n * sizeof(float), hipMemcpyDeviceToHost, st));
+    }
+    CK(hipStreamSynchronize(st));
     return PPPM_OK;
 }
 

@@ -29,7 +29,9 @@ EXPORTS = [
     "pppm_stats", "pppm_get_grid_weights", "pppm_set_grid_weights", "pppm_get_face_weights",
     "pppm_set_face_weights", "pppm_get_interp_weights", "pppm_get_c3", "pppm_get_b4",
     "pppm_set_timing", "pppm_get_timing", "pppm_run_moving", "pppm_classify", "pppm_fill_fresh",
-    "pppm_set_solid_zero", "pppm_get_cells", "pppm_set_time_idx",
+    "pppm_set_solid_zero", "pppm_get_cells", "pppm_set_time_idx", "pppm_ffat_collect",
+    "pppm_ghost_setup", "pppm_ghost_set_neumann", "pppm_ghost_solve", "pppm_ghost_step", "pppm_ghost_update_mesh",
+    "pppm_ghost_get",
     "pppm_modal_create", "pppm_modal_destroy", "pppm_modal_run", "pppm_modal_state", "pppm_modal_set_state",
     "pppm_modal_surface_matrix", "pppm_modal_coeffs",
 ]
@@ -73,6 +75,7 @@ def load_library(path: str = LIB_PATH):
         "pppm_time_idx": ([vp, P(i)], i),
         "pppm_grid_ptr": ([vp, i, P(vp)], i),
         "pppm_read_cells": ([vp, vp, i, vp], i),
+        "pppm_ffat_collect": ([vp, i, vp], i),
         "pppm_history_ptrs": ([vp, P(vp), P(vp)], i),
         "pppm_triangles": ([vp, vp], i),
         "pppm_stats": ([vp, vp], i),
@@ -91,6 +94,12 @@ def load_library(path: str = LIB_PATH):
         "pppm_set_solid_zero": ([vp], i),
         "pppm_get_cells": ([vp, vp, vp, vp, vp, vp, vp], i),
         "pppm_set_time_idx": ([vp, i], i),
+        "pppm_ghost_setup": ([vp, f, P(i)], i),
+        "pppm_ghost_set_neumann": ([vp, vp, i], i),
+        "pppm_ghost_solve": ([vp, P(i)], i),
+        "pppm_ghost_step": ([vp, vp, i], i),
+        "pppm_ghost_update_mesh": ([vp, vp, i, P(i)], i),
+        "pppm_ghost_get": ([vp, vp, vp, vp, vp, vp, vp, vp, vp], i),
         "pppm_modal_create": ([P(vp), vp, i, i, vp, vp, i, vp, f, f, f, vp], i),
         "pppm_modal_destroy": ([vp], i),
         "pppm_modal_run": ([vp, vp, i, vp, i], i),
@@ -310,6 +319,16 @@ class PPPMSolver:
                                              out.ctypes.data_as(ctypes.c_void_p)), "read_cells")
         return out
 
+    def ffat_collect(self, w, fmap):
+        """FFAT map accumulation at the current time level (collect_ffat_map,
+        experiments/static/main.cu:60-80): fmap is a float32 device tensor of shape (6, 2w, 2w),
+        updated in place on the solver's stream."""
+        import torch
+        if not (isinstance(fmap, torch.Tensor) and fmap.is_cuda and fmap.dtype == torch.float32
+                and fmap.is_contiguous() and tuple(fmap.shape) == (6, 2 * w, 2 * w)):
+            raise ValueError("fmap must be a contiguous float32 device tensor of shape (6, 2w, 2w)")
+        self._check(self.lib.pppm_ffat_collect(self.h, int(w), ctypes.c_void_p(fmap.data_ptr())), "ffat_collect")
+
     def get_triangles(self):
         out = np.zeros((self.nt, 13), dtype=np.float32)
         self._check(self.lib.pppm_triangles(self.h, out.ctypes.data_as(ctypes.c_void_p)), "triangles")
@@ -392,6 +411,53 @@ class PPPMSolver:
 
     def set_solid_zero(self):
         self._check(self.lib.pppm_set_solid_zero(self.h), "set_solid_zero")
+
+    # ghost-cell boundary solve (GhostCellSolver, src/algorithm/ghost_cell.h:72-193)
+    def ghost_setup(self, cond_threshold=25.0):
+        """Stencil weights and sparse rows for the current classification; returns the number of
+        second-order rows."""
+        c = ctypes.c_int()
+        self._check(self.lib.pppm_ghost_setup(self.h, float(cond_threshold), ctypes.byref(c)), "ghost_setup")
+        return c.value
+
+    def ghost_set_neumann(self, neumann):
+        if not hasattr(neumann, "data_ptr"):
+            neumann = np.ascontiguousarray(neumann, dtype=np.float32)
+        p, dev = _ptr(neumann)
+        self._check(self.lib.pppm_ghost_set_neumann(self.h, p, dev), "ghost_set_neumann")
+
+    def ghost_solve(self):
+        """Right-hand side, solve and ghost values at the current level; returns the iterations."""
+        it = ctypes.c_int()
+        self._check(self.lib.pppm_ghost_solve(self.h, ctypes.byref(it)), "ghost_solve")
+        return it.value
+
+    def ghost_step(self, neumann):
+        """GhostCellSolver::update: FDTD step, Neumann data, ghost solve, SOLID cells to zero."""
+        if not hasattr(neumann, "data_ptr"):
+            neumann = np.ascontiguousarray(neumann, dtype=np.float32)
+        p, dev = _ptr(neumann)
+        self._check(self.lib.pppm_ghost_step(self.h, p, dev), "ghost_step")
+
+    def ghost_update_mesh(self, verts):
+        """GhostCellSolver::update_mesh; returns the fresh-cell count."""
+        if not hasattr(verts, "data_ptr"):
+            verts = np.ascontiguousarray(verts, dtype=np.float32)
+        p, dev = _ptr(verts)
+        n = ctypes.c_int()
+        self._check(self.lib.pppm_ghost_update_mesh(self.h, p, dev, ctypes.byref(n)), "ghost_update_mesh")
+        return n.value
+
+    def ghost_get(self, n):
+        """Host copies of the ghost data (n = the classification's ghost count)."""
+        out = {"cells": np.zeros(n, np.int32), "order": np.zeros(n, np.int32),
+               "weights": np.zeros((n, 8), np.float32), "cond": np.zeros(n, np.float32),
+               "rhs": np.zeros(n, np.float32), "x": np.zeros(n, np.float32),
+               "cols": np.zeros((n, 9), np.int32), "vals": np.zeros((n, 9), np.float32)}
+        ptrs = [out[k].ctypes.data_as(ctypes.c_void_p) for k in ("cells", "order", "weights", "cond", "rhs", "x",
+                                                                  "cols", "vals")]
+        self._check(self.lib.pppm_ghost_get(self.h, *ptrs), "ghost_get")
+        return out
 
     def cells(self):
         NN = self._res ** 3

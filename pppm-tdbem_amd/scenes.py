@@ -201,6 +201,27 @@ def c2_moving(steps: int = 4000, c: float = 343.0, res: int = 64, seed: int = SE
                  motion=verts_at, info={"n_cube_v": n_cube_v, "axis": axis.tolist()})
 
 
+def large_static_sphere(steps: int = 500, c: float = 343.0, res: int = 512, seed: int = SEED) -> Scene:
+    """One large grid: the C5 scene's object (icosphere level 5, radius 0.05 m, each vertex's
+    radius scaled by 1 + U[-0.05, 0.05]) at the C5 cell size h = radius / 32 (128^3 over four
+    radii), in a res^3 domain (512^3 = 16 radii wide, 1.6 GB of pressure levels), driven by the
+    C1 pulse on every triangle. The FDTD's N^3 stream dominates the step's bytes."""
+    rng = np.random.default_rng(seed + 5)
+    radius = 0.05
+    h = np.float32(4 * radius / 128)
+    length = float(h) * res
+    dt = np.float32(float(h) / (math.sqrt(3.0) * c))
+    V, F = icosphere(5, radius)
+    V = (V.astype(np.float64) * (1.0 + rng.uniform(-0.05, 0.05, (V.shape[0], 1)))).astype(np.float32)
+    mn = (-length / 2, -length / 2, -length / 2)
+    t = np.arange(steps) * float(dt)
+    g = gaussian_sine(t).astype(np.float32)
+    neumann = np.repeat(g[:, None], F.shape[0], axis=1)
+    lis = tuple(int(0.875 * res) for _ in range(3))
+    return Scene("large_static_sphere", res, float(h), float(dt), mn, c, V, F, steps, lis, neumann,
+                 info={"radius": radius, "signal": g})
+
+
 def rel_l2(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)

@@ -29,10 +29,13 @@ def c1_short(scenes_mod):
     return scenes_mod.c1_static_sphere(steps=1000)
 
 
-def test_fdtd_abc_bitexact(amd, oracle_mod):
-    """FDTD::step with the ABC (fdtd.cu:20-150): random levels t-1, t; 4 steps; bit-exact."""
+@pytest.mark.parametrize("N", [20, 140, 200])
+def test_fdtd_abc_bitexact(amd, oracle_mod, N):
+    """FDTD::step with the ABC (fdtd.cu:20-150): random levels t-1, t; 4 steps; bit-exact.
+    N = 140 and 200 take the marched-interior path (partial tiles in z and in the last x chunk)."""
+    print(f"fdtd abc: N={N}", flush=True)
     rng = np.random.default_rng(1)
-    N, h = 20, 0.01
+    h = 0.01
     dt = h / (np.sqrt(3) * 343 * 1.01)
     g = amd.PPPMSolver(N, h, dt, (0, 0, 0), 0, 0.83, 343.0)
     o = oracle_mod.Oracle(N, h, dt, (0, 0, 0), 343.0, 0.83, oracle_mod.FP64)

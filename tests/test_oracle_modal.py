@@ -128,3 +128,16 @@ def test_force_of_an_impulse_at_its_peak(mesh):
     np.testing.assert_allclose(f, ref, rtol=1e-5, atol=1e-5)
     off = oracle_mod.modal_forces(eig, [t0], [13], vec, [0], float(t0 + tau / 2))
     np.testing.assert_array_equal(off, 0.0)
+
+
+def test_accumulation_is_fused_per_term():
+    """Q = (-(1 + 2^-11), 1 + 2^-12), M = (1, 1 + 2^-12): the exact second product is
+    1 + 2^-11 + 2^-24, so a fused multiply-add leaves 2^-24 while a rounded product (a tie, to
+    even) cancels to 0 — the device compiler's contraction of `acc += M * Q` is the former."""
+    m = np.array([[1.0, 1.0 + 2.0 ** -12]], np.float32)
+    q = np.array([[-(1.0 + 2.0 ** -11), 1.0 + 2.0 ** -12]], np.float32)
+    acc = oracle_mod.modal_accs(m, q)
+    assert acc[0, 0] == np.float32(2.0 ** -24)
+    # in mode order: the same two terms the other way round start from the inexact product
+    acc_rev = oracle_mod.modal_accs(m[:, ::-1].copy(), q[:, ::-1].copy())
+    assert acc_rev[0, 0] == np.float32(0.0)
