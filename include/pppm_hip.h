@@ -200,6 +200,16 @@ int pppm_ghost_get(pppm_solver *s, int *cells, int *order, float *weights8, floa
  * (w <= N/2 - 1 for even N); the reference does not check. Asynchronous on the handle's stream. */
 int pppm_ffat_collect(pppm_solver *s, int w, float *map_dev);
 
+/* pppm_slice_image: the experiments' picture of one grid plane at the current time level
+ * (save_grid, src/utils/visualize.h:120-132, through CudaRender::setData / add_mesh_to_images,
+ * src/ui/window.cu:28-330, and write_to_png, src/utils/array_writer.h:68-105): plane `index`
+ * normal to `axis` (0: x, 1: y, 2: z; the reference's slice vector (0.5, 0, 0) is axis 0, index
+ * N/2), upsampled to W = N * (1080 / N) pixels with bilinear interpolation, the jet colour map
+ * over [-max_value, max_value] at half brightness, cell borders and the mesh's cut drawn in
+ * white. rgba_host: W * W * 4 bytes, PNG row order (top row first), or NULL to query W only.
+ * width (optional): W. Synchronous. */
+int pppm_slice_image(pppm_solver *s, int axis, int index, float max_value, unsigned char *rgba_host, int *width);
+
 /* Modal sound source of a rigid body (RigidBody, src/utils/RigidBody.h:53-166) on the GPU: the
  * surface modal matrix, the per-mode IIR oscillators and the surface accelerations that feed
  * the solver's Neumann input. A handle owns its device data and runs on its stream.

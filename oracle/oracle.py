@@ -85,6 +85,7 @@ def lib():
             "orc_modal_iir": ([vp, i, f, vp, vp, i, vp], None),
             "orc_modal_accs": ([vp, i, i, vp, i, vp], None),
             "orc_ffat_collect": ([vp, i, i, vp], None),
+            "orc_slice_image": ([vp, i, f, vp, i, i, f, vp, vp, i, vp], None),
             "orc_modal_forces": ([vp, i, vp, vp, vp, vp, i, f, vp], None),
         }
         for name, (a, r) in sig.items():
@@ -404,6 +405,25 @@ def ffat_collect(grid, w, fmap):
     assert fmap.dtype == np.float32 and fmap.flags.c_contiguous and fmap.shape == (6, 2 * w, 2 * w)
     lib().orc_ffat_collect(_p(g), g.shape[0], int(w), _p(fmap))
     return fmap
+
+
+def slice_image(grid, h, min_pos, axis, index, vmax, verts=None, tris=None):
+    """save_grid's picture of plane `index` normal to `axis`: (W, W, 4) uint8 in PNG row order."""
+    g = np.ascontiguousarray(grid, np.float32)
+    N = g.shape[0]
+    W = N * (1080 // N)
+    out = np.zeros((W, W, 4), np.uint8)
+    mn = np.ascontiguousarray(min_pos, np.float32)
+    if verts is None:
+        v = np.zeros((1, 3), np.float32)
+        t = np.zeros((1, 3), np.int32)
+        nt = 0
+    else:
+        v = np.ascontiguousarray(verts, np.float32)
+        t = np.ascontiguousarray(tris, np.int32)
+        nt = t.shape[0]
+    lib().orc_slice_image(_p(g), N, float(h), _p(mn), int(axis), int(index), float(vmax), _p(v), _p(t), nt, _p(out))
+    return out
 
 
 def modal_forces(eig, imp_time, imp_vert, imp_vec, active, time):

@@ -146,38 +146,59 @@ __device__ __forceinline__ float leapfrog(float xm, float xp, float ym, float yp
     return (float)v;
 }
 
-// Interior of a large grid: a block owns MTY x 64 (y, z) columns of one chunk of x planes and
+// Interior of a large grid: a block owns MTY x MTZ (y, z) columns of one chunk of x planes and
 // marches them along x, so the x-neighbours come from registers and the y/z neighbours of the
 // current plane from cache: each level-t value is fetched from HBM about once instead of once
-// per plane that reads it. A tile's 64 z-columns start on a 256-byte boundary (z = 64 tz + lane,
-// cells outside 2..N-3 idle) so a wave's row is two whole cache lines; the 16 rows of a block
-// are marched by its 16 waves together, so only the two rows past the tile come from another
-// block. Distinct noalias parameters let the compiler issue the loads of the next planes ahead
+// per plane that reads it. A thread holds MCPT columns 64 apart, so a wave covers MTZ z-values
+// of one row from a 256-byte boundary (cells outside 2..N-3 idle, reading a live cell); the 16
+// rows of a block are marched by its 16 waves together, so only the two rows past the tile and
+// the two cache lines past its ends come from other blocks. Distinct noalias parameters let the compiler issue the loads of the next planes ahead
 // of this plane's store.
 constexpr int MTY = 16;
 constexpr int MBLK = 64 * MTY;
+constexpr int MCPT = 4;            // z columns per thread, 64 apart: a tile is 16 rows x 256 z
+constexpr int MTZ = 64 * MCPT;
 __device__ __forceinline__ void march_columns(const float *__restrict__ g, const float *__restrict__ gm,
-                                              float *__restrict__ gp, int N, int x0, int x1, int y, int z,
+                                              float *__restrict__ gp, int N, int x0, int x1, int y, int z0,
                                               const FdtdParams &p)
 {
     const int NN2 = N * N;
-    int me = lin3(x0, y, z, N);
-    float gxm = g[me - NN2], gc = g[me], gxp = g[me + NN2];
-#pragma unroll 4
+    bool on[MCPT];
+    float gxm[MCPT], gc[MCPT], gxp[MCPT];
+    int me = lin3(x0, y, 0, N);
+#pragma unroll
+    for (int c = 0; c < MCPT; c++)
+    {
+        const int z = z0 + 64 * c;
+        on[c] = z >= 2 && z <= N - 3;
+        const int zz = on[c] ? z : 2;
+        gxm[c] = g[me + zz - NN2];
+        gc[c] = g[me + zz];
+        gxp[c] = g[me + zz + NN2];
+    }
+#pragma unroll 2
     for (int x = x0; x < x1; x++)
     {
-        const float gym = g[me - N], gyp = g[me + N], gzm = g[me - 1], gzp = g[me + 1];
-        const float gnx = g[me + 2 * NN2];  // x + 2 <= N - 1
-        const float gmv = gm[me];
-        gp[me] = leapfrog(gxm, gxp, gym, gyp, gzm, gzp, gc, gmv, p);
-        gxm = gc;
-        gc = gxp;
-        gxp = gnx;
+#pragma unroll
+        for (int c = 0; c < MCPT; c++)
+        {
+            const int z = on[c] ? z0 + 64 * c : 2;
+            const int q = me + z;
+            const float gym = g[q - N], gyp = g[q + N], gzm = g[q - 1], gzp = g[q + 1];
+            const float gnx = g[q + 2 * NN2];  // x + 2 <= N - 1
+            const float gmv = gm[q];
+            const float v = leapfrog(gxm[c], gxp[c], gym, gyp, gzm, gzp, gc[c], gmv, p);
+            if (on[c])
+                gp[q] = v;
+            gxm[c] = gc[c];
+            gc[c] = gxp[c];
+            gxp[c] = gnx;
+        }
         me += NN2;
     }
 }
 
-// Marched-interior layout: tiles of MTY x 64 columns times chunks of xch planes, tile index z
+// Marched-interior layout: tiles of MTY x MTZ columns times chunks of xch planes, tile index z
 // fastest, then y, then the x chunk; n_tiles of them, launched as n_blocks (a multiple of 8).
 struct MarchShape
 {
@@ -198,13 +219,13 @@ __global__ __launch_bounds__(MBLK) void k_fdtd_march(float *__restrict__ grid3, 
         return;
     const int tz = tile % ms.ntz, r = tile / ms.ntz;
     const int ty = r % ms.nty, xc = r / ms.nty;
-    const int z = tz * 64 + (int)(threadIdx.x & 63);
+    const int z0 = tz * MTZ + (int)(threadIdx.x & 63);
     const int y = 2 + ty * MTY + (int)(threadIdx.x >> 6);
-    if (y > N - 3 || z < 2 || z > N - 3)
+    if (y > N - 3 || z0 > N - 3)
         return;
     const int x0 = 2 + xc * ms.xch;
     const int x1 = min(x0 + ms.xch, N - 2);
-    march_columns(level(grid3, t, 0, nn), level(grid3, t, -1, nn), level(grid3, t, 1, nn), N, x0, x1, y, z, p);
+    march_columns(level(grid3, t, 0, nn), level(grid3, t, -1, nn), level(grid3, t, 1, nn), N, x0, x1, y, z0, p);
 }
 
 // One thread per interior cell (blocks [0, nb_inner)) and the ABC shell (the remaining
@@ -236,10 +257,30 @@ __global__ __launch_bounds__(BLK) void k_fdtd_abc(float *__restrict__ grid3, Fdt
     else
     {
         const int S = N - 2;
-        long long idx = (long long)((int)blockIdx.x - nb_inner) * BLK + threadIdx.x;
         const long long fx = 2LL * S * S, fy = 2LL * (S - 2) * S, fz = 2LL * (S - 2) * (S - 2);
+        // blocks [nb_inner, nb_inner + nb_fxy): the x and y faces (rows along z, coalesced);
+        // the rest: the z faces, one cell per row, so their blocks are handed to the XCDs in
+        // contiguous runs of x planes and a plane's rows meet its neighbours' in one L2
+        const int b = (int)blockIdx.x - nb_inner;
+        const int nb_fxy = (int)((fx + fy + BLK - 1) / BLK);
+        long long idx;
+        if (b < nb_fxy)
+        {
+            idx = (long long)b * BLK + threadIdx.x;
+            if (idx >= fx + fy)
+                idx = -1;
+        }
+        else
+        {
+            const int bz = b - nb_fxy, per = (int)((fz + 8LL * BLK - 1) / (8LL * BLK));
+            idx = fx + fy + ((long long)(bz & 7) * per + (bz >> 3)) * BLK + threadIdx.x;
+        }
         int i = -1, j = 0, k = 0;
-        if (idx < fx)
+        if (idx >= fx + fy + fz)
+            idx = -1;
+        if (idx < 0)
+            ;
+        else if (idx < fx)
         {
             int side = (int)(idx / ((long long)S * S));
             int r = (int)(idx % ((long long)S * S));
@@ -357,13 +398,13 @@ hipError_t launch_fdtd(float *grid3, float *pml3, const FdtdParams &p, const Ste
         return hipGetLastError();
     }
     const long long M = N - 4, S = N - 2;
-    const long long inner = M * M * M, shell = S * S * S - inner;
+    const long long inner = M * M * M;
     int nb_inner = (int)((inner + BLK - 1) / BLK);
     if (M >= kMarchMin)
     {
         // about 2048 tiles or more; chunks of at least 16 planes (each chunk re-reads 2 planes)
         MarchShape ms{};
-        ms.ntz = (N - 3) / 64 + 1;  // z tiles [64 tz, 64 tz + 64) up to z = N - 3
+        ms.ntz = (N - 3) / MTZ + 1;  // z tiles [MTZ tz, MTZ tz + MTZ) up to z = N - 3
         ms.nty = (int)((M + MTY - 1) / MTY);
         const long long cols = (long long)ms.ntz * ms.nty;
         const long long nxc = std::max<long long>(1, (2048 + cols - 1) / cols);
@@ -373,7 +414,8 @@ hipError_t launch_fdtd(float *grid3, float *pml3, const FdtdParams &p, const Ste
         k_fdtd_march<<<ms.n_blocks, MBLK, 0, s>>>(grid3, p, st, ms);
         nb_inner = 0;
     }
-    const int nb_shell = (int)((shell + BLK - 1) / BLK);
+    const long long fxy = 2 * S * S + 2 * (S - 2) * S, fz = 2 * (S - 2) * (S - 2);
+    const int nb_shell = (int)((fxy + BLK - 1) / BLK) + 8 * (int)((fz + 8LL * BLK - 1) / (8LL * BLK));
     k_fdtd_abc<<<nb_inner + nb_shell, BLK, 0, s>>>(grid3, p, st, nb_inner);
     return hipGetLastError();
 }

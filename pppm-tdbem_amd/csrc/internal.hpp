@@ -241,6 +241,12 @@ hipError_t launch_near(const StepArgs &a, const StepState *st, float *grid3, con
                        const int *c3_start, const int *c3_wave, int c3_waves, const int *c3_pairs, const uint32_t *wb,
                        const float2 *sb, const float *hn, const float *hd, const float *wn, const float *wd,
                        const float *far, const int *listen_cell, const int *listen_c3, float *out, hipStream_t s);
+// colour image of grid plane `index` normal to `axis` (0: x, 1: y, 2: z) at the current level
+// with the mesh's cut drawn over it: W x W RGBA in PNG row order, W = N * (1080 / N);
+// cover / count: W x W ints of scratch
+hipError_t launch_slice_image(int N, float h, const float *mn3, int axis, int index, float vmax,
+                              const StepState *st, const float *grid3, const float *verts, const int *tris, int nt,
+                              int *cover, int *count, unsigned char *rgba, hipStream_t s);
 // FFAT map accumulation at the current time level: map is 6 x 2w x 2w, 1 <= w <= N/2 - 1
 hipError_t launch_ffat(const StepState *st, const float *grid3, int N, int w, float *map, hipStream_t s);
 hipError_t launch_listen_direct(const StepArgs &a, const StepState *st, const float *grid3, const int *listen_cell,

@@ -181,6 +181,10 @@ struct pppm_solver
     DBuf<int> gh_cell, gh_order, gh_mcol, gh_info, gh_err;
     DBuf<float> gh_pw, gh_cond, gh_mval, gh_b, gh_x, gh_neu, gh_neu_old;
     DBuf<double> gh_work;
+
+    // slice images: per-pixel mesh-line coverage and counts, the RGBA image
+    DBuf<int> img_cover, img_count;
+    DBuf<unsigned char> img_rgba;
     int st_h[ST_N] = {0};
     long long last_grid_recomputes = 0, last_face_recomputes = 0;
 
@@ -1169,6 +1173,29 @@ int pppm_read_cells(pppm_solver *s, const int *xyz, int n, float *host_out)
             return fail(s, PPPM_ERR_ARG, "read_cells: cell outside the grid");
         CK(hipMemcpyAsync(host_out + i, g + lin3(x, y, z, N), sizeof(float), hipMemcpyDeviceToHost, s->stream));
     }
+    CK(hipStreamSynchronize(s->stream));
+    return PPPM_OK;
+}
+
+int pppm_slice_image(pppm_solver *s, int axis, int index, float max_value, unsigned char *rgba_host, int *width)
+{
+    if (check_handle(s) || axis < 0 || axis > 2 || index < 0 || index >= s->N || !(max_value > 0.0f))
+        return PPPM_ERR_ARG;
+    if (s->N > 1080)
+        return fail(s, PPPM_ERR_ARG, "slice_image: the image is 1080 pixels wide; the grid is wider");
+    const int W = s->N * (1080 / s->N);
+    if (width)
+        *width = W;
+    if (!rgba_host)
+        return PPPM_OK;
+    const size_t WW = (size_t)W * W;
+    CK(s->img_cover.ensure(WW));
+    CK(s->img_count.ensure(WW));
+    CK(s->img_rgba.ensure(4 * WW));
+    CK(launch_slice_image(s->N, s->h, s->minp, axis, index, max_value, s->state.p, s->grid.p,
+                          s->mesh_set ? s->verts.p : nullptr, s->mesh_set ? s->tris.p : nullptr,
+                          s->mesh_set ? s->nt : 0, s->img_cover.p, s->img_count.p, s->img_rgba.p, s->stream));
+    CK(hipMemcpyAsync(rgba_host, s->img_rgba.p, 4 * WW, hipMemcpyDeviceToHost, s->stream));
     CK(hipStreamSynchronize(s->stream));
     return PPPM_OK;
 }

@@ -31,7 +31,7 @@ EXPORTS = [
     "pppm_set_timing", "pppm_get_timing", "pppm_run_moving", "pppm_classify", "pppm_fill_fresh",
     "pppm_set_solid_zero", "pppm_get_cells", "pppm_set_time_idx", "pppm_ffat_collect",
     "pppm_ghost_setup", "pppm_ghost_set_neumann", "pppm_ghost_solve", "pppm_ghost_step", "pppm_ghost_update_mesh",
-    "pppm_ghost_get",
+    "pppm_ghost_get", "pppm_slice_image",
     "pppm_modal_create", "pppm_modal_destroy", "pppm_modal_run", "pppm_modal_state", "pppm_modal_set_state",
     "pppm_modal_surface_matrix", "pppm_modal_coeffs",
 ]
@@ -100,6 +100,7 @@ def load_library(path: str = LIB_PATH):
         "pppm_ghost_step": ([vp, vp, i], i),
         "pppm_ghost_update_mesh": ([vp, vp, i, P(i)], i),
         "pppm_ghost_get": ([vp, vp, vp, vp, vp, vp, vp, vp, vp], i),
+        "pppm_slice_image": ([vp, i, i, f, vp, P(i)], i),
         "pppm_modal_create": ([P(vp), vp, i, i, vp, vp, i, vp, f, f, f, vp], i),
         "pppm_modal_destroy": ([vp], i),
         "pppm_modal_run": ([vp, vp, i, vp, i], i),
@@ -329,6 +330,24 @@ class PPPMSolver:
             raise ValueError("fmap must be a contiguous float32 device tensor of shape (6, 2w, 2w)")
         self._check(self.lib.pppm_ffat_collect(self.h, int(w), ctypes.c_void_p(fmap.data_ptr())), "ffat_collect")
 
+    def slice_image(self, axis, index, max_value=1.0):
+        """save_grid's picture of plane `index` normal to `axis` (0: x, 1: y, 2: z) at the
+        current level: (W, W, 4) uint8 RGBA in PNG row order (src/utils/visualize.h:120-132)."""
+        w = ctypes.c_int()
+        self._check(self.lib.pppm_slice_image(self.h, int(axis), int(index), float(max_value), None,
+                                              ctypes.byref(w)), "slice_image")
+        out = np.zeros((w.value, w.value, 4), np.uint8)
+        self._check(self.lib.pppm_slice_image(self.h, int(axis), int(index), float(max_value),
+                                              out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(w)), "slice_image")
+        return out
+
+    def save_grid(self, filename, max_value=1.0, slice_=(0.0, 0.5, 0.0)):
+        """save_grid (src/utils/visualize.h:120-132): the plane int(N * slice_[a]) of the first
+        axis a with a non-zero slice component, written as a PNG file."""
+        a = next(i for i in range(3) if slice_[i] != 0)
+        idx = int(np.float32(self._res) * np.float32(slice_[a]))
+        write_png(filename, self.slice_image(a, idx, max_value))
+
     def get_triangles(self):
         out = np.zeros((self.nt, 13), dtype=np.float32)
         self._check(self.lib.pppm_triangles(self.h, out.ctypes.data_as(ctypes.c_void_p)), "triangles")
@@ -468,6 +487,46 @@ class PPPMSolver:
                                                                   "reflect", "dist")]
         self._check(self.lib.pppm_get_cells(self.h, *ptrs), "get_cells")
         return out
+
+
+def write_png(path, rgba):
+    """RGBA8 image (H, W, 4), top row first, as a PNG file (write_to_png,
+    src/utils/array_writer.h:68-92): 8-bit RGBA, no interlace, deflate via zlib."""
+    import struct
+    import zlib
+    img = np.ascontiguousarray(rgba, np.uint8)
+    h, w = img.shape[0], img.shape[1]
+    raw = b"".join(b"\x00" + img[r].tobytes() for r in range(h))
+
+    def chunk(tag, data):
+        return struct.pack(">I", len(data)) + tag + data + struct.pack(">I", zlib.crc32(tag + data) & 0xFFFFFFFF)
+
+    with open(path, "wb") as fh:
+        fh.write(b"\x89PNG\r\n\x1a\n")
+        fh.write(chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 6, 0, 0, 0)))
+        fh.write(chunk(b"IDAT", zlib.compress(raw, 6)))
+        fh.write(chunk(b"IEND", b""))
+
+
+def read_png(path):
+    """Inverse of write_png for 8-bit RGBA images with filter type 0 rows."""
+    import struct
+    import zlib
+    data = open(path, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, idat, w, h = 8, b"", 0, 0
+    while pos < len(data):
+        n = struct.unpack(">I", data[pos:pos + 4])[0]
+        tag = data[pos + 4:pos + 8]
+        body = data[pos + 8:pos + 8 + n]
+        if tag == b"IHDR":
+            w, h = struct.unpack(">II", body[:8])
+        elif tag == b"IDAT":
+            idat += body
+        pos += 12 + n
+    raw = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(h, 1 + 4 * w)
+    assert np.all(raw[:, 0] == 0)
+    return raw[:, 1:].reshape(h, w, 4).copy()
 
 
 _hip = None

@@ -40,3 +40,27 @@ def test_ffat_width_outside_grid_is_rejected(amd):
     m = torch.zeros((6, 32, 32), dtype=torch.float32, device="cuda")
     with pytest.raises(amd.PPPMError):
         g.ffat_collect(16, m)
+
+
+@pytest.mark.parametrize("axis,index", [(0, 16), (1, 16), (2, 11), (0, 1)])
+def test_slice_image_bitexact(amd, oracle_mod, scenes_mod, axis, index):
+    """save_grid's picture (colour map, interpolation, cell borders, the mesh's cut) of a plane
+    of the C1 scene after 30 steps, equal to the oracle's byte for byte; written and read back
+    as a PNG file."""
+    import os
+    import tempfile
+    sc = scenes_mod.c1_static_sphere(steps=30)
+    print(f"slice image: axis {axis}, index {index}", flush=True)
+    g = amd.PPPMSolver(sc.res, sc.dl, sc.dt, sc.min_pos, 0, 0.83, sc.c)
+    g.set_mesh(sc.verts, sc.tris)
+    for i in range(sc.steps):
+        g.update_grid_and_face(sc.neumann[i])
+    vmax = float(np.abs(g.get_grid(0)).max()) * 0.5
+    img = g.slice_image(axis, index, vmax)
+    ref = oracle_mod.slice_image(g.get_grid(0), sc.dl, sc.min_pos, axis, index, vmax, sc.verts, sc.tris)
+    diff = int(np.count_nonzero(np.any(img != ref, axis=2)))
+    print(f"pixels differing: {diff} of {img.shape[0] * img.shape[1]}", flush=True)
+    assert diff == 0
+    path = os.path.join(tempfile.mkdtemp(), "slice.png")
+    amd.write_png(path, img)
+    assert np.array_equal(amd.read_png(path), img)
