@@ -61,7 +61,7 @@ struct ScanBatch
     ScanJob job[SCAN_MAX_JOBS];
     int njobs;
 };
-constexpr int SCAN_TILE = 4096;  // items per tile of a multi-tile scan
+constexpr int SCAN_TILE = 1024;  // items per tile of a multi-tile scan
 // tmp must hold scan_batch_tmp_ints(sizes) ints
 size_t scan_batch_tmp_ints(const int *n, int njobs);
 // sets job[k].tile0 (the tiles of all jobs laid out in job order); returns the tile count

@@ -4,7 +4,7 @@
 // src/core/array.h:77-115).
 //
 // Reduce-then-scan in two launches and no inter-block waiting: the first kernel writes one
-// sum per 4096-item tile, the second has every tile add up the sums of the tiles before it
+// sum per 1024-item tile, the second has every tile add up the sums of the tiles before it
 // (one wave), scan its own items and, in the last tile, write the total. Several independent
 // arrays ("jobs") share the two launches; jobs of at most 8192 items take a single launch.
 // A job in flag mode scans (in != 0) and can write the compacted list of selected indices
@@ -20,7 +20,7 @@ namespace pppm_amd
 namespace
 {
 constexpr int SB = 256;        // threads per block
-constexpr int SI = 16;         // items per thread (multi-tile scans)
+constexpr int SI = 4;          // items per thread (multi-tile scans)
 constexpr int SPB = SB * SI;   // items per tile
 static_assert(SPB == SCAN_TILE, "tile size shared with the producers of tile sums");
 constexpr int SI1 = 32;        // items per thread when one tile holds everything
