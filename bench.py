@@ -173,6 +173,51 @@ def modal_bench(nt=20480, nmodes=200, nsamples=4096, calls=5, cpu=True):
     print(json.dumps(res))
 
 
+def ghost_bench(res=64, steps=500, warmup=20, cpu=True):
+    """Ghost-cell comparison solver (GhostCellSolver::update: FDTD step, boundary data, ghost
+    solve, SOLID cells to zero) on one MI355X: the C1 sphere on a res^3 grid, condition
+    threshold 25. Prints one JSON line; the step-path metric is the default run."""
+    import torch
+    import pppm_amd
+    import scenes
+    sc = scenes.c1_static_sphere(steps=warmup + steps, res=res)
+    g = pppm_amd.PPPMSolver(sc.res, sc.dl, sc.dt, sc.min_pos, 0, 0.83, sc.c)
+    g.set_mesh(sc.verts, sc.tris)
+    n = g.classify(False)
+    second = g.ghost_setup(25.0)
+    neu = torch.from_numpy(sc.neumann).cuda()
+    for i in range(warmup):
+        g.ghost_step(neu[i])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(warmup, warmup + steps):
+        g.ghost_step(neu[i])
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    it = g.ghost_solve()
+    res_ = {"metric": "ghost-cell solver cell-updates/s", "value": res ** 3 * steps / el, "unit": "cell-updates/s",
+            "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": el * 1e3 / steps, "higher_is_better": True,
+            "dtype": "f32 (solve f64)", "data": "synthetic",
+            "config": {"workload": f"c1_static_sphere_{res}_ghost_cell", "grid": f"{res}^3", "ghost_cells": n,
+                       "second_order_rows": second, "solver_iterations_last_step": it}}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        import ghost_oracle
+        o = oracle.Oracle(sc.res, sc.dl, sc.dt, sc.min_pos, sc.c, 0.83, oracle.FP64)
+        cells = g.cells()
+        st = ghost_oracle.ghost_setup(cells, sc.res, sc.dl, sc.min_pos, 25.0)
+        k, dt = 0, 0.0
+        while k < steps and dt < 10.0:
+            t1 = time.perf_counter()
+            ghost_oracle.ghost_step(o, st, cells, sc.neumann[k])
+            dt += time.perf_counter() - t1
+            k += 1
+        res_["cpu_baseline"] = {"value": res ** 3 * k / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+                                "sample": f"oracle FDTD + numpy ghost solve (dense LU), {k} steps, {dt:.1f} s"}
+    print(json.dumps(res_))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,9 +240,14 @@ def main():
                          "known size, to scale FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--modal", action="store_true",
                     help="measure the modal sound source instead (its own JSON line)")
+    ap.add_argument("--ghost", action="store_true",
+                    help="measure the ghost-cell comparison solver instead (its own JSON line)")
     args = ap.parse_args()
     if args.modal:
         modal_bench(cpu=not args.no_cpu)
+        return
+    if args.ghost:
+        ghost_bench(cpu=not args.no_cpu)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design note, "

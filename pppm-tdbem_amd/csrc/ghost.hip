@@ -465,6 +465,183 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_ghost_bicgstab(int n, const i
         info[0] = it > max_iter ? max_iter : it;
 }
 
+// The same BiCGSTAB for systems of at most SOLVE_THREADS * RPT rows: every thread keeps its RPT
+// rows of r0, r, p, v, s, t and x in registers, the two vectors a matrix product reads (p, s)
+// are shared through LDS, and each dot product is one wave butterfly plus an in-order sum of
+// the 16 wave totals (two barriers). Same operations and order of the scalar recurrences as
+// k_ghost_bicgstab; the sums group differently, so the two agree to rounding.
+constexpr int RPT = 3;
+constexpr int NWAVES = SOLVE_THREADS / 64;
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// sums of a and b over the block; red holds 2 * NWAVES doubles
+__device__ inline void block_sum2(double &a, double &b, double *red)
+{
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+    {
+        red[w] = a;
+        red[NWAVES + w] = b;
+    }
+    __syncthreads();
+    double sa = 0.0, sb = 0.0;
+#pragma unroll
+    for (int k = 0; k < NWAVES; k++)
+    {
+        sa += red[k];
+        sb += red[NWAVES + k];
+    }
+    __syncthreads();
+    a = sa;
+    b = sb;
+}
+
+__global__ __launch_bounds__(SOLVE_THREADS) void k_ghost_bicgstab_small(int n, const int *__restrict__ mcol,
+                                                                        const float *__restrict__ mval,
+                                                                        const float *__restrict__ b,
+                                                                        float *__restrict__ xout, int max_iter,
+                                                                        double tol, int *__restrict__ info)
+{
+    __shared__ double sp[SOLVE_THREADS * RPT], ssv[SOLVE_THREADS * RPT];
+    __shared__ double red[2 * NWAVES];
+    const int tid = threadIdx.x;
+    double r0[RPT], r[RPT], pv[RPT], v[RPT], sv[RPT], tv[RPT], x[RPT], bb[RPT];
+    bool own[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; k++)
+    {
+        const int i = tid + k * SOLVE_THREADS;
+        own[k] = i < n;
+        bb[k] = own[k] ? (double)b[i] : 0.0;
+        pv[k] = v[k] = sv[k] = tv[k] = x[k] = 0.0;
+    }
+    auto spmv = [&](const double *in, double *out) {
+#pragma unroll
+        for (int k = 0; k < RPT; k++)
+        {
+            const int i = tid + k * SOLVE_THREADS;
+            double acc = 0.0;
+            if (own[k])
+#pragma unroll
+                for (int q = 0; q < 9; q++)
+                {
+                    const int c = mcol[(size_t)i * 9 + q];
+                    if (c >= 0)
+                        acc += (double)mval[(size_t)i * 9 + q] * in[c];
+                }
+            out[k] = acc;
+        }
+    };
+    double ss = 0.0, dummy = 0.0;
+#pragma unroll
+    for (int k = 0; k < RPT; k++)
+        ss += bb[k] * bb[k];
+    block_sum2(ss, dummy, red);
+    const double bn = sqrt(ss / (double)n);
+    if (bn == 0.0)
+    {
+#pragma unroll
+        for (int k = 0; k < RPT; k++)
+            if (own[k])
+                xout[tid + k * SOLVE_THREADS] = 0.0f;
+        if (tid == 0)
+            info[0] = 0;
+        return;
+    }
+    double rr = 0.0;
+#pragma unroll
+    for (int k = 0; k < RPT; k++)
+    {
+        r0[k] = bb[k] / bn;  // x0 = 0, so r0 = b
+        r[k] = r0[k];
+        rr += r0[k] * r0[k];
+    }
+    block_sum2(rr, dummy, red);
+    const double thr = tol * sqrt(rr);
+    double rho_prev = 1.0, alpha = 1.0, omega = 1.0;
+    int it = 0;
+    for (it = 1; it <= max_iter; it++)
+    {
+        double rho = 0.0;
+#pragma unroll
+        for (int k = 0; k < RPT; k++)
+            rho += r0[k] * r[k];
+        block_sum2(rho, dummy, red);
+        if (rho == 0.0)
+            break;
+        const double beta = (rho / rho_prev) * (alpha / omega);
+#pragma unroll
+        for (int k = 0; k < RPT; k++)
+        {
+            pv[k] = it == 1 ? r[k] : r[k] + beta * (pv[k] - omega * v[k]);
+            sp[tid + k * SOLVE_THREADS] = pv[k];
+        }
+        __syncthreads();
+        spmv(sp, v);
+        double den = 0.0;
+#pragma unroll
+        for (int k = 0; k < RPT; k++)
+            den += r0[k] * v[k];
+        block_sum2(den, dummy, red);
+        if (den == 0.0)
+            break;
+        alpha = rho / den;
+        double ls = 0.0;
+#pragma unroll
+        for (int k = 0; k < RPT; k++)
+        {
+            sv[k] = r[k] - alpha * v[k];
+            ssv[tid + k * SOLVE_THREADS] = sv[k];
+            ls += sv[k] * sv[k];
+        }
+        block_sum2(ls, dummy, red);  // its barriers also publish s
+        if (sqrt(ls) < thr)
+        {
+#pragma unroll
+            for (int k = 0; k < RPT; k++)
+                x[k] += alpha * pv[k];
+            break;
+        }
+        spmv(ssv, tv);
+        double ts = 0.0, tt = 0.0;
+#pragma unroll
+        for (int k = 0; k < RPT; k++)
+        {
+            ts += tv[k] * sv[k];
+            tt += tv[k] * tv[k];
+        }
+        block_sum2(ts, tt, red);
+        omega = tt != 0.0 ? ts / tt : 0.0;
+        double lrr = 0.0;
+#pragma unroll
+        for (int k = 0; k < RPT; k++)
+        {
+            x[k] += alpha * pv[k] + omega * sv[k];
+            r[k] = sv[k] - omega * tv[k];
+            lrr += r[k] * r[k];
+        }
+        block_sum2(lrr, dummy, red);
+        if (sqrt(lrr) < thr || omega == 0.0)
+            break;
+        rho_prev = rho;
+    }
+#pragma unroll
+    for (int k = 0; k < RPT; k++)
+        if (own[k])
+            xout[tid + k * SOLVE_THREADS] = (float)(x[k] * bn);
+    if (tid == 0)
+        info[0] = it > max_iter ? max_iter : it;
+}
+
 // update_ghost_cell_kernel (ghost_cell.cu:510-520): the solution into the ghost cells at level t
 __global__ void k_ghost_write(int n, int N, const StepState *__restrict__ st, const int *__restrict__ gcell,
                               const float *__restrict__ x, float *__restrict__ grid3)
@@ -520,7 +697,10 @@ hipError_t launch_ghost_solve(const CellArgs &a, int n, const StepState *st, flo
     const float *sol = b;
     if (linear)
     {
-        k_ghost_bicgstab<<<1, SOLVE_THREADS, 0, s>>>(n, mcol, mval, b, work, x, GHOST_MAX_ITER, GHOST_TOL, info);
+        if (n <= SOLVE_THREADS * RPT)
+            k_ghost_bicgstab_small<<<1, SOLVE_THREADS, 0, s>>>(n, mcol, mval, b, x, GHOST_MAX_ITER, GHOST_TOL, info);
+        else
+            k_ghost_bicgstab<<<1, SOLVE_THREADS, 0, s>>>(n, mcol, mval, b, work, x, GHOST_MAX_ITER, GHOST_TOL, info);
         sol = x;
     }
     k_ghost_write<<<gblocks(n), GB, 0, s>>>(n, a.N, st, gcell, sol, grid3);

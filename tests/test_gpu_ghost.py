@@ -51,8 +51,10 @@ def test_ghost_setup_matches_oracle(amd, scenes_mod):
     _close(d["vals"], st["vals"], 1e-6, "matrix values")
 
 
-def test_ghost_solve_matches_oracle(amd, scenes_mod):
-    sc = scenes_mod.c1_static_sphere(steps=2)
+@pytest.mark.parametrize("res", [32, 96])
+def test_ghost_solve_matches_oracle(amd, scenes_mod, res):
+    """res 32: the register/LDS solver (at most 3072 rows); res 96: the global-memory one."""
+    sc = scenes_mod.c1_static_sphere(steps=2, res=res)
     g, n, _, cells, st = _setup_pair(amd, sc, 25.0)
     rng = np.random.default_rng(4)
     p = rng.standard_normal(sc.res ** 3).astype(np.float32)
