@@ -290,10 +290,9 @@ def main():
     def run_steps(i0, n, use_graph=True):
         if moving:
             # the demo loop's per-sample UpdateSound: update_mesh, then update_grid_and_face
-            # (experiments/demo/main.cu:170-205); rows are already in HBM
+            # (experiments/demo/main.cu:170-205), as one call; rows are already in HBM
             for i in range(i0, i0 + n):
-                solver.update_mesh(verts_all[i])
-                solver.update_grid_and_face(neumann[i])
+                solver.update_mesh_and_step(verts_all[i], neumann[i])
         else:
             solver.run(neumann[i0:i0 + n], n, lis, out[i0:i0 + n], use_graph=use_graph)
 

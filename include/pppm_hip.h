@@ -67,6 +67,10 @@ int pppm_update_mesh(pppm_solver *s, const float *verts, int on_device);
 /* PPPMSolver::update_grid_and_face / update_step (src/algorithm/pppm.h:137-152):
  * FDTD step, far field, Neumann input (nt floats), Dirichlet update, near field. */
 int pppm_step(pppm_solver *s, const float *neumann, int on_device);
+/* pppm_update_mesh_and_step: pppm_update_mesh then pppm_step in one call: the demo's per-sample
+ * UpdateSound with a moving mesh (experiments/demo/main.cu:170-205). Same inputs and errors as
+ * the two calls; verts and neumann are both host or both device memory (on_device). */
+int pppm_update_mesh_and_step(pppm_solver *s, const float *verts, const float *neumann, int on_device);
 /* Runs `steps` consecutive update_steps. neumann_series is device memory holding
  * steps x stride floats (row i = Neumann input of step i, nt used). After each step
  * the pressure at the n_listen cells (xyz triples, host array) is written to

@@ -185,7 +185,18 @@ struct pppm_solver
     // slice images: per-pixel mesh-line coverage and counts, the RGBA image
     DBuf<int> img_cover, img_count;
     DBuf<unsigned char> img_rgba;
-    int st_h[ST_N] = {0};
+    // host copy of the status block, in pinned memory so that the per-update read is a direct
+    // DMA rather than a staged copy
+    struct PinnedStatus
+    {
+        int *p = nullptr;
+        ~PinnedStatus()
+        {
+            if (p)
+                (void)hipHostFree(p);
+        }
+    } st_pin;
+    int *st_h = nullptr;
     long long last_grid_recomputes = 0, last_face_recomputes = 0;
 
     // per-phase GPU timing with HIP events on the handle's stream (the reference's
@@ -554,7 +565,7 @@ int enqueue_rebuild(pppm_solver *s, bool first, const float *verts_series, long 
 
 int read_status(pppm_solver *s)
 {
-    CK(hipMemcpyAsync(s->st_h, s->status.p, sizeof(s->st_h), hipMemcpyDeviceToHost, s->stream));
+    CK(hipMemcpyAsync(s->st_h, s->status.p, ST_N * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     CK(hipStreamSynchronize(s->stream));
     s->err_clean = s->st_h[ST_ERR] == 0;
     s->pool_count = s->st_h[ST_POOL];
@@ -874,6 +885,14 @@ int pppm_create(pppm_solver **out, int res, float dl, float dt, const float *min
         delete s;
         return PPPM_ERR_HIP;
     }
+    if (hipHostMalloc((void **)&s->st_pin.p, ST_N * sizeof(int), hipHostMallocDefault) != hipSuccess)
+    {
+        (void)hipStreamDestroy(s->stream);
+        delete s;
+        return PPPM_ERR_HIP;
+    }
+    s->st_h = s->st_pin.p;
+    std::memset(s->st_h, 0, ST_N * sizeof(int));
     s->own_stream = true;
     const size_t NN = (size_t)res * res * res;
     if (s->grid.ensure(3 * NN) != hipSuccess || s->state.ensure(1) != hipSuccess ||
@@ -1039,6 +1058,14 @@ int pppm_run_moving(pppm_solver *s, const float *verts_series, long long vstride
         return fail(s, PPPM_ERR_ARG, "run_moving: vertex series required (stride >= 3 nv)");
     return run_impl(s, verts_series, vstride, neumann_series, stride, steps, listen_xyz, n_listen, out_dev,
                     use_graph);
+}
+
+int pppm_update_mesh_and_step(pppm_solver *s, const float *verts, const float *neumann, int on_device)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    RC(pppm_update_mesh(s, verts, on_device));
+    return pppm_step(s, neumann, on_device);
 }
 
 int pppm_fdtd_step(pppm_solver *s)

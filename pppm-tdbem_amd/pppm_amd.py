@@ -23,7 +23,7 @@ PPPM_FLAG_HONOR_PML = 1
 
 EXPORTS = [
     "pppm_create", "pppm_destroy", "pppm_set_stream", "pppm_last_error", "pppm_set_mesh",
-    "pppm_update_mesh", "pppm_step", "pppm_run", "pppm_fdtd_step", "pppm_solve_far",
+    "pppm_update_mesh", "pppm_step", "pppm_update_mesh_and_step", "pppm_run", "pppm_fdtd_step", "pppm_solve_far",
     "pppm_set_neumann", "pppm_update_dirichlet", "pppm_solve_near", "pppm_reset",
     "pppm_time_idx", "pppm_grid_ptr", "pppm_read_cells", "pppm_history_ptrs", "pppm_triangles",
     "pppm_stats", "pppm_get_grid_weights", "pppm_set_grid_weights", "pppm_get_face_weights",
@@ -65,6 +65,7 @@ def load_library(path: str = LIB_PATH):
         "pppm_set_mesh": ([vp, vp, i, vp, i, i], i),
         "pppm_update_mesh": ([vp, vp, i], i),
         "pppm_step": ([vp, vp, i], i),
+        "pppm_update_mesh_and_step": ([vp, vp, vp, i], i),
         "pppm_run": ([vp, vp, ll, i, vp, i, vp, i], i),
         "pppm_fdtd_step": ([vp], i),
         "pppm_solve_far": ([vp], i),
@@ -213,6 +214,16 @@ class PPPMSolver:
             verts = np.ascontiguousarray(verts, dtype=np.float32)
         v, dv = _ptr(verts)
         self._check(self.lib.pppm_update_mesh(self.h, v, dv), "update_mesh")
+
+    def update_mesh_and_step(self, verts, neumann):
+        """update_mesh(verts) then update_grid_and_face(neumann) in one call; both numpy arrays
+        or both device tensors."""
+        if not hasattr(verts, "data_ptr"):
+            verts = np.ascontiguousarray(verts, dtype=np.float32)
+            neumann = np.ascontiguousarray(neumann, dtype=np.float32)
+        pv, dev = _ptr(verts)
+        pn, _ = _ptr(neumann)
+        self._check(self.lib.pppm_update_mesh_and_step(self.h, pv, pn, dev), "update_mesh_and_step")
 
     def update_grid_and_face(self, neumann):
         if not hasattr(neumann, "data_ptr"):

@@ -338,3 +338,26 @@ def test_error_paths(amd):
     v = np.array([[0, 0, 0], [0.001, 0, 0], [0, 0.001, 0]], np.float32)  # cell 0: outside [2, N-3]
     with pytest.raises(amd.PPPMError):
         g.set_mesh(v, np.array([[0, 1, 2]], np.int32))
+
+
+def test_update_mesh_and_step_equals_two_calls(amd, scenes_mod):
+    """pppm_update_mesh_and_step is update_mesh followed by update_grid_and_face: same grids bit for bit
+    over 12 moving steps (host inputs and device inputs)."""
+    import torch
+    sc = scenes_mod.c2_moving(steps=12)
+    print("update_mesh_and_step vs update_mesh + step: moving workload, 12 steps", flush=True)
+    a = amd.PPPMSolver(sc.res, sc.dl, sc.dt, sc.min_pos, 0, 0.83, sc.c)
+    b = amd.PPPMSolver(sc.res, sc.dl, sc.dt, sc.min_pos, 0, 0.83, sc.c)
+    a.set_mesh(sc.motion(0), sc.tris)
+    b.set_mesh(sc.motion(0), sc.tris)
+    for i in range(sc.steps):
+        v, n = sc.motion(i), sc.neumann[i]
+        if i % 2:
+            a.update_mesh_and_step(torch.from_numpy(v).cuda(), torch.from_numpy(np.ascontiguousarray(n)).cuda())
+        else:
+            a.update_mesh_and_step(v, n)
+        b.update_mesh(v)
+        b.update_grid_and_face(n)
+    torch.cuda.synchronize()
+    for rel in (0, -1):
+        assert np.array_equal(a.get_grid(rel).view(np.uint32), b.get_grid(rel).view(np.uint32))
