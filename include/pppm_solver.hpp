@@ -83,6 +83,13 @@ class PPPMSolver
     void update_grid_and_face(const std::vector<float> &neumann) { update_grid_and_face(neumann.data()); }
     void update_step(const std::vector<float> &neumann) { update_grid_and_face(neumann); }
 
+    // update_mesh then update_grid_and_face in one call: the demo's per-sample UpdateSound with
+    // a moving mesh (experiments/demo/main.cu:170-205)
+    void update_mesh_and_step(const float3_ *verts, const float *neumann, bool on_device = false)
+    {
+        check(pppm_update_mesh_and_step(h_, &verts->x, neumann, on_device), "update_mesh_and_step");
+    }
+
     // the phases, as PPPMTest.cu and FaceCacheTest.cu call them (pppm.h:127-135)
     void fdtd_step() { check(pppm_fdtd_step(h_), "fdtd.step"); }
     void solve_fdtd_far() { check(pppm_solve_far(h_), "solve_fdtd_far"); }
