@@ -340,6 +340,25 @@ def test_error_paths(amd):
         g.set_mesh(v, np.array([[0, 1, 2]], np.int32))
 
 
+def test_error_paths_of_the_extra_entry_points(amd):
+    """Argument and state checks of the ghost-cell, image and modal entry points (host side)."""
+    g = amd.PPPMSolver(16, 0.01, 1e-5, (0, 0, 0), 0, 0.83, 343.0)
+    with pytest.raises(amd.PPPMError):
+        g.ghost_setup(25.0)  # before classify
+    with pytest.raises(amd.PPPMError):
+        g.ghost_solve()  # before ghost_setup
+    with pytest.raises(amd.PPPMError):
+        g.ghost_step(np.zeros(1, np.float32))
+    for axis, index, vmax in ((3, 4, 1.0), (0, 16, 1.0), (1, -1, 1.0), (2, 4, 0.0)):
+        with pytest.raises(amd.PPPMError):
+            g.slice_image(axis, index, vmax)
+    assert g.slice_image(2, 8, 1.0).shape == (16 * (1080 // 16),) * 2 + (4,)
+    eig = np.zeros((9, 2), np.float32)
+    with pytest.raises(Exception):
+        amd.ModalSource(eig, np.ones(2, np.float32), np.array([[0, 1, 7]], np.int32),
+                        np.array([[0, 0, 1]], np.float32), 1.0, 0.0, 44100.0)  # vertex 7 of 3
+
+
 def test_update_mesh_and_step_equals_two_calls(amd, scenes_mod):
     """pppm_update_mesh_and_step is update_mesh followed by update_grid_and_face: same grids bit for bit
     over 12 moving steps (host inputs and device inputs)."""
