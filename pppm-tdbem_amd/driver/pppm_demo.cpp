@@ -1,11 +1,14 @@
 // pppm_demo — the reference's demo driver (experiments/demo/main.cu:38-275) on the MI355X
 // library: reads the same scene config, moves Audio objects along their motion, feeds their
 // surface accelerations, steps the PPPM solver once per audio sample, applies the demo's
-// chunked restart with a cross-fade, and writes result.txt in the same layout.
+// chunked restart with a cross-fade, and writes result.txt in the same layout. Modal objects
+// (RigidBody, src/utils/RigidBody.cu) run their oscillators through the library's modal source
+// at the solver's frame rate, with the contact impulses as modal forces.
 //
 //   pppm_demo <scene.cfg>                    run; writes <outputDir>/result.txt
 //   pppm_demo --setup <scene.cfg>            print the derived grid / timing parameters
 //   pppm_demo --calls <scene.cfg> <out.bin>  record every solver call of a run without a GPU
+//   pppm_demo --trace <scene.cfg> <out.bin>  run (writes result.txt) and record the calls too
 //
 // The call record (for checking the loop against another solver) is a sequence of records:
 //   int32 op; op 0 set_mesh: int32 nv, nt, float32 verts[3 nv], int32 tris[3 nt]
@@ -20,6 +23,9 @@
 #include <string>
 #include <vector>
 
+#include <hip/hip_runtime_api.h>
+
+#include "pppm_hip.h"
 #include "pppm_solver.hpp"
 #include "scene.hpp"
 
@@ -129,10 +135,158 @@ F3 closest_point(F3 p, F3 v0, F3 v1, F3 v2)
     return (v0 + r.s * e0) + r.t * e1;
 }
 
+// A rigid body with modal sound (RigidBody, src/utils/RigidBody.h:53-166): per audio sample one
+// animation frame at most, the contact impulses active at the sample's time as modal forces
+// (CalculateIIR_, RigidBody.cu:254-295, summed on the host as the reference does), and the
+// oscillators and surface accelerations of that sample on the GPU (pppm_modal_run).
+class ModalObject : public Object
+{
+  public:
+    explicit ModalObject(const std::string &dir, const std::string &material = "plastic")
+        : d(load_rigid_body(dir))
+    {
+        material_damping(material, alpha, beta);
+        verts = d.verts;
+        tris = d.surf.tris;
+        if (tris.empty() || d.eigvals.empty())
+            throw std::runtime_error(dir + ": a modal object needs a closed surface and at least one mode");
+    }
+    ~ModalObject() override
+    {
+        if (m)
+            pppm_modal_destroy(m);
+        if (accs_dev)
+            (void)hipFree(accs_dev);
+    }
+    // RigidBody::set_sample_rate (RigidBody.h:64-70) / InitIIR_ (RigidBody.cu:190-205)
+    void set_sample_rate(float rate)
+    {
+        timestep = 1.0f / rate;
+        if (m)
+            pppm_modal_destroy(m);
+        m = nullptr;
+        if (pppm_modal_create(&m, d.eigvecs.data(), (int)d.verts.size(), (int)d.eigvals.size(), d.eigvals.data(),
+                              &tris[0].x, (int)tris.size(), &d.surf.normals[0].x, alpha, beta, rate,
+                              nullptr) != PPPM_OK)
+            throw std::runtime_error(name + ": pppm_modal_create failed");
+        if (!accs_dev && hipMalloc((void **)&accs_dev, tris.size() * sizeof(float)) != hipSuccess)
+            throw std::runtime_error(name + ": hipMalloc failed");
+        if (hipMemset(accs_dev, 0, tris.size() * sizeof(float)) != hipSuccess)
+            throw std::runtime_error(name + ": hipMemset failed");
+        time = 0;
+        anim = 0;
+    }
+    float time_step() const override { return timestep; }
+    bool update_until(float t) override
+    {
+        while (time < t)
+            audio_step();
+        return moved;
+    }
+    void submit_accelerations(float *out) override
+    {
+        if (hipMemcpy(out, accs_dev, tris.size() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+            throw std::runtime_error(name + ": hipMemcpy failed");
+    }
+    // get_bbox (RigidBody.cu:310-338): the frames from the one before the first impulse on
+    BBox bbox() override
+    {
+        BBox b;
+        size_t f = 0;
+        if (!d.impulses.empty() && d.times.size() > 1)
+        {
+            f = 1;
+            while (f < d.times.size() && d.times[f] < d.impulses[0].time)
+                f++;
+            f--;
+        }
+        for (; f < d.times.size(); f++)
+            for (auto &v : d.verts)
+                b.add(rotate(d.rots[f], v) + d.trans[f]);
+        if (d.times.empty())
+            for (auto &v : verts)
+                b.add(v);
+        return b;
+    }
+    // move_to_first_impulse (RigidBody.cu:355-362)
+    void move_to_first_impulse()
+    {
+        if (d.impulses.empty())
+            return;
+        while (time + timestep < d.impulses[0].time)
+            audio_step();
+    }
+    void save(ObjectState &s) override
+    {
+        s.verts = verts;
+        s.anim = anim;
+        s.time = time;
+        s.imp = imp;
+        s.q3.resize(3 * d.eigvals.size());
+        s.accs.resize(tris.size());
+        if (pppm_modal_state(m, s.q3.data()) != PPPM_OK)
+            throw std::runtime_error(name + ": pppm_modal_state failed");
+        submit_accelerations(s.accs.data());
+    }
+    void load(const ObjectState &s) override
+    {
+        verts = s.verts;
+        anim = s.anim;
+        time = s.time;
+        imp = s.imp;
+        if (pppm_modal_set_state(m, s.q3.data()) != PPPM_OK ||
+            hipMemcpy(accs_dev, s.accs.data(), tris.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            throw std::runtime_error(name + ": restoring the modal state failed");
+        // the active-impulse list is not part of the saved state (RigidBodyState, RigidBody.h:47-56)
+    }
+    float time = 0;
+
+  private:
+    RigidBodyData d;
+    float alpha = 0, beta = 0, timestep = 0;
+    pppm_modal *m = nullptr;
+    float *accs_dev = nullptr;
+    int anim = 0, imp = 0;
+    bool moved = false;
+    std::vector<Impulse> active;
+    std::vector<float> force;
+
+    // audio_step (RigidBody.cu:340-353) and CalculateIIR_'s impulse part (RigidBody.cu:257-281)
+    void audio_step()
+    {
+        moved = false;
+        if (anim < (int)d.times.size() && d.times[anim] <= time)
+        {
+            for (size_t i = 0; i < verts.size(); i++)
+                verts[i] = rotate(d.rots[anim], d.verts[i]) + d.trans[anim];
+            anim++;
+            moved = true;
+        }
+        time += timestep;
+        while (imp < (int)d.impulses.size() && time >= d.impulses[imp].time)
+            active.push_back(d.impulses[imp++]);
+        while (!active.empty() && time - active[0].time > 0.0005f)  // Impulse::dead, RigidBody.h:30
+            active.erase(active.begin());
+        const int M = (int)d.eigvals.size();
+        force.assign(M, 0.0f);
+        for (int i = 0; i < M; i++)
+            for (auto &a : active)
+            {
+                const float amp = (float)std::sin(M_PI * (time - a.time) / 0.0005f);  // Impulse::amp
+                const F3 I = a.vec * amp;
+                const size_t r = 3 * (size_t)a.vertex;
+                force[i] += d.eigvecs[r * M + i] * I.x + d.eigvecs[(r + 1) * M + i] * I.y +
+                            d.eigvecs[(r + 2) * M + i] * I.z;
+            }
+        if (pppm_modal_run(m, force.data(), 1, accs_dev, 0) != PPPM_OK)
+            throw std::runtime_error(name + ": pppm_modal_run failed");
+    }
+};
+
 // grid and timing derived from the scene (experiments/demo/main.cu:94-147)
 struct Setup
 {
-    int res = 0, frame_rate = 0, frame_num = 0, chunk = 0, back = 0;
+    int res = 0, frame_rate = 0, frame_num = 0, chunk = 0, back = 0, mute = 0;
     float grid_size = 0, dt = 0, reflect = 0, start_time = 0, max_time = 0;
     F3 min_pos{};
     I3 check{};
@@ -153,7 +307,19 @@ Setup derive(const SceneConfig &cfg, ObjectCollection &col)
     // 1.01 x the CFL rate at the reference's speed of sound (AIR_WAVE_SPEED, macro.h:82)
     s.frame_rate = (int)(1.01f / (s.grid_size / std::sqrt(3) / 340.3f));
     s.dt = 1.0f / s.frame_rate;
-    s.start_time = 0.0f + cfg.num("startTime");
+    // modal objects run at the frame rate and skip ahead to their first contact; the samples
+    // before it are silent (main.cu:104-124)
+    float curr = 0.0f;
+    for (auto &o : col.objects)
+        if (auto *rb = dynamic_cast<ModalObject *>(o.get()))
+        {
+            rb->set_sample_rate((float)s.frame_rate);
+            rb->move_to_first_impulse();
+            curr = std::max(curr, rb->time);
+        }
+    s.mute = (int)(curr * s.frame_rate);
+    col.update_mesh();
+    s.start_time = curr + cfg.num("startTime");
     s.max_time = cfg.num("maxTime") + s.start_time;
     s.frame_num = (int)((s.max_time - s.start_time) / s.dt);
     s.check = {(int)(cfg.num("checkCoordX") * s.res), (int)(cfg.num("checkCoordY") * s.res),
@@ -248,10 +414,44 @@ struct RecordSink : Sink
     int time_idx() override { return t; }
 };
 
+// the GPU run with every call also recorded
+struct TeeSink : Sink
+{
+    Sink &a, &b;
+    TeeSink(Sink &gpu, Sink &rec) : a(gpu), b(rec) {}
+    void set_mesh(const std::vector<F3> &v, const std::vector<I3> &t) override
+    {
+        a.set_mesh(v, t);
+        b.set_mesh(v, t);
+    }
+    void update_mesh(const std::vector<F3> &v) override
+    {
+        a.update_mesh(v);
+        b.update_mesh(v);
+    }
+    void step(const std::vector<float> &n) override
+    {
+        a.step(n);
+        b.step(n);
+    }
+    float read(int idx, float w, I3 c) override
+    {
+        b.read(idx, w, c);
+        return a.read(idx, w, c);
+    }
+    void reset(int t) override
+    {
+        a.reset(t);
+        b.reset(t);
+    }
+    int time_idx() override { return a.time_idx(); }
+};
+
 // AudioAndManualTest's loop (experiments/demo/main.cu:166-272)
 std::vector<float> run(const Setup &s, ObjectCollection &col, Sink &sink)
 {
-    std::vector<float> result((size_t)s.frame_num + 2, 0.0f);
+    std::vector<float> result((size_t)s.mute + s.frame_num + 2, 0.0f);
+    const size_t mute = (size_t)s.mute;
     result[0] = (float)s.frame_rate;
     float curr = s.start_time;
     bool mesh_set = false;
@@ -277,7 +477,7 @@ std::vector<float> run(const Setup &s, ObjectCollection &col, Sink &sink)
     // CheckNaN looks at result[progress + 1] (main.cu:158-165), which is the sample just
     // written inside the restart blocks but the next, unwritten one on the plain path
     auto nan_at = [&](int progress) {
-        const size_t k = (size_t)progress + 1;
+        const size_t k = mute + progress + 1;
         return k < result.size() && std::isnan(result[k]);
     };
     int progress = 0;
@@ -299,8 +499,8 @@ std::vector<float> run(const Setup &s, ObjectCollection &col, Sink &sink)
             {
                 update_sound(true);
                 const float factor = (float)(j - i) / (float)(end - i);
-                const float p = sink.read(j + 1, 1 - factor, s.check);
-                result[(size_t)j + 1] = p * (1 - factor);
+                const float p = sink.read((int)mute + j + 1, 1 - factor, s.check);
+                result[mute + j + 1] = p * (1 - factor);
                 ok = !nan_at(progress);
             }
             if (!ok)
@@ -313,7 +513,7 @@ std::vector<float> run(const Setup &s, ObjectCollection &col, Sink &sink)
             for (int j = i; j < end && ok; j++)
             {
                 update_sound(false);
-                result[(size_t)j + 1] += sink.read(j + 1, -1.0f, s.check);
+                result[mute + j + 1] += sink.read((int)mute + j + 1, -1.0f, s.check);
                 ok = !nan_at(progress);
                 if (ok)
                     progress++;
@@ -323,7 +523,7 @@ std::vector<float> run(const Setup &s, ObjectCollection &col, Sink &sink)
             continue;
         }
         update_sound(false);
-        result[(size_t)i + 1] = sink.read(i + 1, 1.0f, s.check);
+        result[mute + i + 1] = sink.read((int)mute + i + 1, 1.0f, s.check);
         progress++;
         if (nan_at(progress))
             break;
@@ -370,22 +570,29 @@ int main(int argc, char **argv)
             mode = "setup", cfg_path = argv[2];
         else if (argc >= 4 && !std::strcmp(argv[1], "--calls"))
             mode = "calls", cfg_path = argv[2], out = argv[3];
+        else if (argc >= 4 && !std::strcmp(argv[1], "--trace"))
+            mode = "trace", cfg_path = argv[2], out = argv[3];
         else if (argc == 2)
             cfg_path = argv[1];
         else
         {
-            std::fprintf(stderr, "usage: pppm_demo <scene.cfg> | --setup <scene.cfg> | --calls <scene.cfg> <out>\n");
+            std::fprintf(stderr, "usage: pppm_demo <scene.cfg> | --setup <scene.cfg> | --calls <scene.cfg> <out> | "
+                                 "--trace <scene.cfg> <out>\n");
             return 2;
         }
         const SceneConfig cfg = SceneConfig::load(cfg_path);
-        ObjectCollection col(cfg.dirs.at("inputDir"), object_list(cfg));
+        // Modal objects (material "plastic", main.cu:62-66) need the GPU: not in the recording mode
+        ObjectCollection::ModalFactory modal;
+        if (mode != "calls")
+            modal = [](const std::string &dir) { return std::unique_ptr<Object>(new ModalObject(dir)); };
+        ObjectCollection col(cfg.dirs.at("inputDir"), object_list(cfg), modal);
         const Setup s = derive(cfg, col);
         if (mode == "setup")
         {
             std::printf("res %d\ngrid_size %.9g\nmin_pos %.9g %.9g %.9g\nframe_rate %d\ndt %.9g\nframe_num %d\n"
-                        "check %d %d %d\nchunk %d %d\nvertices %zu\ntriangles %zu\n",
+                        "check %d %d %d\nchunk %d %d\nvertices %zu\ntriangles %zu\nmute %d\n",
                         s.res, s.grid_size, s.min_pos.x, s.min_pos.y, s.min_pos.z, s.frame_rate, s.dt, s.frame_num,
-                        s.check.x, s.check.y, s.check.z, s.chunk, s.back, col.verts.size(), col.tris.size());
+                        s.check.x, s.check.y, s.check.z, s.chunk, s.back, col.verts.size(), col.tris.size(), s.mute);
             return 0;
         }
         if (mode == "calls")
@@ -395,7 +602,15 @@ int main(int argc, char **argv)
             return 0;
         }
         GpuSink gpu(s);
-        const std::vector<float> r = run(s, col, gpu);
+        std::vector<float> r;
+        if (mode == "trace")
+        {
+            RecordSink rec(out);
+            TeeSink tee(gpu, rec);
+            r = run(s, col, tee);
+        }
+        else
+            r = run(s, col, gpu);
         const std::string odir = cfg.dirs.at("outputDir");
         std::filesystem::create_directories(odir);
         write_result(odir + "/result.txt", r);

@@ -6,20 +6,26 @@
 //   Mesh::loadOBJ, bbox       src/utils/objIO.cu:94-108,169-220
 //   ManualObject, AudioObject src/utils/Object.h:73-323, src/utils/Object.cu:6-270
 //   ObjectCollection          src/utils/ObjectCollection.h, ObjectCollection.cu:22-160
+//   RigidBody data files      src/utils/RigidBody.cu:90-188 (motion, contact impulses, tet mesh,
+//                             eigen pairs), FindAllSurfaces src/utils/Object.cu:72-142,
+//                             MaterialParameters src/utils/material.h:34-96
 // Mesh repair (fix_mesh: a pymesh container) is not run; meshes are used as given, and the
 // selected-surface map that repair would have built is computed against the same mesh.
 #pragma once
 
 #include <algorithm>
+#include <array>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <fstream>
 #include <map>
 #include <memory>
+#include <functional>
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 namespace pppm_demo
@@ -40,6 +46,8 @@ struct I3
 inline F3 operator+(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 inline F3 operator-(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 inline F3 operator*(float s, F3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline F3 operator*(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline F3 operator/(F3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
 inline float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 inline F3 cross(F3 a, F3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 inline float length(F3 a) { return std::sqrt(dot(a, a)); }
@@ -201,7 +209,212 @@ struct ObjectState
 {
     std::vector<F3> verts;
     int anim = -1, acc = 0;
+    // rigid bodies (RigidBodyState, RigidBody.h:47-56): clock, next impulse, oscillators
+    float time = 0;
+    int imp = 0;
+    std::vector<float> q3, accs;
 };
+
+// motion.txt: "t tx ty tz qx qy qz qw" per line (Object::LoadMotion_, Object.cu:195-222)
+inline void load_motion_file(const std::string &path, std::vector<float> &times, std::vector<F3> &trans,
+                             std::vector<F4> &rots)
+{
+    std::ifstream in(path);
+    if (!in.good())
+        return;
+    std::string line;
+    while (std::getline(in, line))
+    {
+        std::istringstream ss(line);
+        float t;
+        F3 p;
+        F4 q;
+        if (!(ss >> t >> p.x >> p.y >> p.z >> q.x >> q.y >> q.z >> q.w))
+            continue;
+        times.push_back(t);
+        trans.push_back(p);
+        rots.push_back(q);
+    }
+}
+
+// Rayleigh damping (alpha, beta) of the reference's materials (material.h:34-96)
+inline void material_damping(const std::string &name, float &alpha, float &beta)
+{
+    static const std::map<std::string, std::pair<float, float>> table = {
+        {"ceramic", {40.0f, 1e-7f}}, {"plastic", {30.0f, 1e-6f}}, {"steel", {5.0f, 3e-8f}},
+        {"mdf", {35.0f, 5e-6f}},     {"wood", {60.0f, 2e-6f}}};
+    auto it = table.find(name);
+    alpha = it == table.end() ? 0.0f : it->second.first;
+    beta = it == table.end() ? 0.0f : it->second.second;
+}
+
+// Boundary triangles of a tetrahedral mesh, oriented outwards (FindAllSurfaces,
+// Object.cu:72-142): faces used by exactly one tetrahedron, in the iteration order of a hash
+// map keyed by the face's vertex triple with the reference's hash ((x << 20) + (y << 10) + z),
+// flipped when the tetrahedron's fourth vertex lies on the normal's side.
+struct TetSurface
+{
+    std::vector<I3> tris;
+    std::vector<F3> normals;
+};
+inline TetSurface find_all_surfaces(const std::vector<std::array<int, 4>> &tets, const std::vector<F3> &v)
+{
+    struct Key
+    {
+        int x, y, z;
+        bool operator==(const Key &o) const { return x == o.x && y == o.y && z == o.z; }
+    };
+    struct KeyHash
+    {
+        int operator()(const Key &k) const { return (k.x << 20) + (k.y << 10) + k.z; }
+    };
+    struct Info
+    {
+        int cnt = 0, tet = 0, except = 0;
+    };
+    std::unordered_map<Key, Info, KeyHash> faces;
+    for (int i = 0; i < (int)tets.size(); i++)
+        for (int i0 = 0; i0 < 4; i0++)
+        {
+            int k[3], n = 0;
+            for (int j = 0; j < 4; j++)
+                if (j != i0)
+                    k[n++] = tets[i][j];
+            Info &f = faces[Key{k[0], k[1], k[2]}];
+            f.cnt++;
+            f.tet = i;
+            f.except = i0;
+        }
+    TetSurface out;
+    for (auto &kv : faces)
+    {
+        if (kv.second.cnt != 1)
+            continue;
+        I3 t{kv.first.x, kv.first.y, kv.first.z};
+        const F3 c = (v[t.x] + v[t.y] + v[t.z]) / 3;
+        const F3 ex = v[tets[kv.second.tet][kv.second.except]] - c;
+        const F3 e1 = v[t.y] - v[t.x], e2 = v[t.z] - v[t.x];
+        const F3 cr = cross(e1, e2);
+        F3 nrm = cr * (1.0f / std::sqrt(dot(cr, cr)));
+        if (dot(nrm, ex) > 0)
+        {
+            std::swap(t.y, t.z);
+            nrm = F3{-nrm.x, -nrm.y, -nrm.z};
+        }
+        out.tris.push_back(t);
+        out.normals.push_back(nrm);
+    }
+    return out;
+}
+
+// A rigid body's data directory (RigidBody::load_data, RigidBody.cu:90-188): motion.txt,
+// contact.txt ("t vertex nx ny nz magnitude"), vertices.txt ("x y z"), tets.txt (4 vertex
+// indices, sorted per tetrahedron), eigenvalues.txt, eigenvectors.txt (3 nv rows of all
+// eigenvalues' columns); modes are kept up to the last eigenvalue below max_frequency Hz.
+struct Impulse
+{
+    float time;
+    int vertex;
+    F3 vec;
+};
+struct RigidBodyData
+{
+    std::vector<F3> verts;
+    TetSurface surf;
+    std::vector<float> times;
+    std::vector<F3> trans;
+    std::vector<F4> rots;
+    std::vector<Impulse> impulses;
+    std::vector<float> eigvals;  // M
+    std::vector<float> eigvecs;  // 3 nv x M, row-major
+};
+inline RigidBodyData load_rigid_body(const std::string &dir, float max_frequency = 15000.0f)
+{
+    RigidBodyData d;
+    load_motion_file(dir + "/motion.txt", d.times, d.trans, d.rots);
+    auto open = [&](const std::string &f) {
+        std::ifstream in(dir + "/" + f);
+        if (!in.good())
+            throw std::runtime_error("cannot read " + dir + "/" + f);
+        return in;
+    };
+    std::string line;
+    {
+        std::ifstream in = open("contact.txt");
+        while (std::getline(in, line))
+        {
+            if (line.empty())
+                continue;
+            std::istringstream ss(line);
+            float t, vid, mag;
+            F3 nrm;
+            ss >> t >> vid >> nrm.x >> nrm.y >> nrm.z >> mag;
+            d.impulses.push_back({t, (int)(vid + 0.5f), nrm * mag});  // F2I (macro.h:81)
+        }
+    }
+    {
+        std::ifstream in = open("vertices.txt");
+        while (std::getline(in, line))
+        {
+            if (line.empty())
+                continue;
+            std::istringstream ss(line);
+            F3 p;
+            ss >> p.x >> p.y >> p.z;
+            d.verts.push_back(p);
+        }
+    }
+    std::vector<std::array<int, 4>> tets;
+    {
+        std::ifstream in = open("tets.txt");
+        while (std::getline(in, line))
+        {
+            if (line.empty())
+                continue;
+            std::istringstream ss(line);
+            float a[4];
+            ss >> a[0] >> a[1] >> a[2] >> a[3];
+            std::array<int, 4> t{(int)(a[0] + 0.5f), (int)(a[1] + 0.5f), (int)(a[2] + 0.5f), (int)(a[3] + 0.5f)};
+            std::sort(t.begin(), t.end());
+            for (int k : t)
+                if (k < 0 || k >= (int)d.verts.size())
+                    throw std::runtime_error(dir + "/tets.txt: vertex index out of range");
+            tets.push_back(t);
+        }
+    }
+    d.surf = find_all_surfaces(tets, d.verts);
+    std::vector<float> all;
+    int modes = 0;
+    {
+        std::ifstream in = open("eigenvalues.txt");
+        while (std::getline(in, line))
+        {
+            if (line.empty())
+                continue;
+            std::istringstream ss(line);
+            float val;
+            ss >> val;
+            all.push_back(val);
+            if (std::sqrt(val) / (2 * M_PI) < max_frequency)
+                modes = (int)all.size();
+        }
+    }
+    d.eigvals.assign(all.begin(), all.begin() + modes);
+    const size_t rows = 3 * d.verts.size();
+    d.eigvecs.assign(rows * modes, 0.0f);
+    {
+        std::ifstream in = open("eigenvectors.txt");
+        for (size_t i = 0; i < rows; i++)
+            for (size_t j = 0; j < all.size(); j++)
+            {
+                float val = 0;
+                in >> val;
+                if ((int)j < modes)
+                    d.eigvecs[i * modes + j] = val;
+            }
+    }
+    return d;
+}
 
 class Object
 {
@@ -331,25 +544,7 @@ class AudioObject : public Object
                 verts[i] = rotate(rots[anim], base[i]) + trans[anim];
         return moved;
     }
-    void load_motion(const std::string &path)
-    {
-        std::ifstream in(path);
-        if (!in.good())
-            return;
-        std::string line;
-        while (std::getline(in, line))
-        {
-            std::istringstream ss(line);
-            float t;
-            F3 p;
-            F4 q;
-            if (!(ss >> t >> p.x >> p.y >> p.z >> q.x >> q.y >> q.z >> q.w))
-                continue;
-            times.push_back(t);
-            trans.push_back(p);
-            rots.push_back(q);
-        }
-    }
+    void load_motion(const std::string &path) { load_motion_file(path, times, trans, rots); }
     // LoadAccs_ (Object.cu:228-247): a value counts only if the stream is still good after
     // reading it, so a last value not followed by whitespace is dropped
     void load_accs(const std::string &path)
@@ -417,7 +612,9 @@ class AudioObject : public Object
 class ObjectCollection
 {
   public:
-    ObjectCollection(const std::string &dir, const std::vector<std::pair<std::string, std::string>> &named)
+    using ModalFactory = std::function<std::unique_ptr<Object>(const std::string &dir)>;
+    ObjectCollection(const std::string &dir, const std::vector<std::pair<std::string, std::string>> &named,
+                     const ModalFactory &modal = nullptr)
     {
         size_t nv = 0, nt = 0;
         for (auto &[name, type] : named)
@@ -427,8 +624,12 @@ class ObjectCollection
                 o = std::make_unique<ManualObject>(dir + "/" + name);
             else if (type == "Audio")
                 o = std::make_unique<AudioObject>(dir + "/" + name);
+            else if (type == "Modal" && modal)
+                o = modal(dir + "/" + name);
+            else if (type == "Modal")
+                throw std::runtime_error("Modal objects run their oscillators on the GPU: not available in this mode");
             else
-                throw std::runtime_error("unsupported sound type: " + type + " (Audio and Manual are supported)");
+                throw std::runtime_error("unsupported sound type: " + type + " (Audio, Manual and Modal are supported)");
             o->name = name;
             vert_off.push_back(nv);
             tri_off.push_back(nt);
