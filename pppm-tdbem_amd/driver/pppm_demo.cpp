@@ -353,11 +353,25 @@ struct GpuSink : Sink
         solver.set_mesh(reinterpret_cast<const pppm_amd::float3_ *>(v.data()), (int)v.size(),
                         reinterpret_cast<const pppm_amd::int3_ *>(t.data()), (int)t.size());
     }
+    // a mesh update is held back and issued with the next step as one call (update_mesh, then
+    // update_grid_and_face: the same two operations without a host round trip between them)
     void update_mesh(const std::vector<F3> &v) override
     {
-        solver.update_mesh(reinterpret_cast<const pppm_amd::float3_ *>(v.data()));
+        pending = v;
+        has_pending = true;
     }
-    void step(const std::vector<float> &n) override { solver.update_grid_and_face(n); }
+    void step(const std::vector<float> &n) override
+    {
+        if (has_pending)
+        {
+            solver.update_mesh_and_step(reinterpret_cast<const pppm_amd::float3_ *>(pending.data()), n.data());
+            has_pending = false;
+        }
+        else
+            solver.update_grid_and_face(n);
+    }
+    std::vector<F3> pending;
+    bool has_pending = false;
     float read(int, float, I3 c) override { return solver.read_cell(c.x, c.y, c.z); }
     void reset(int t) override
     {

@@ -1,5 +1,6 @@
 // Scene layer of the demo driver: the reference's config layout, OBJ meshes, Manual and Audio
-// objects and their concatenation, on the host (C++17, no device code). The numerics that
+// objects and their concatenation, the rigid bodies' data files, on the host (C++17, no device
+// code). The numerics that
 // decide which mesh and which accelerations a step sees follow the reference's float
 // operations:
 //   SimpleJsonReader          src/utils/simple_json_reader.h:13-86
