@@ -222,6 +222,28 @@ def large_static_sphere(steps: int = 500, c: float = 343.0, res: int = 512, seed
                  info={"radius": radius, "signal": g})
 
 
+def c3_scene(res: int, level: int, seed: int, steps: int = 2000, c: float = 343.0) -> Scene:
+    """One scene of the batched configuration: an icosphere of the given level, scaled
+    anisotropically by U[0.5, 1.5]^3 (radius 0.05 m before scaling), a res^3 grid over twice its
+    bounding box, and a per-scene Gaussian-modulated sine on every triangle (centre frequency
+    U[500, 2000] Hz, t0 = 3 sigma with sigma = 1 ms)."""
+    rng = np.random.default_rng(SEED + 100 + seed)
+    V, F = icosphere(level, 0.05)
+    V = (V.astype(np.float64) * rng.uniform(0.5, 1.5, 3)).astype(np.float32)
+    lo, hi = V.min(axis=0), V.max(axis=0)
+    length = 2.0 * float(np.max(hi - lo))
+    h = np.float32(length / res)
+    dt = np.float32(float(h) / (math.sqrt(3.0) * c))
+    ctr = (lo + hi) / 2.0
+    mn = tuple(float(x) for x in (ctr - length / 2))
+    t = np.arange(steps) * float(dt)
+    g = gaussian_sine(t, f0=float(rng.uniform(500.0, 2000.0)), t0=3e-3, sigma=1e-3).astype(np.float32)
+    neumann = np.repeat(g[:, None], F.shape[0], axis=1)
+    lis = tuple(int(0.875 * res) for _ in range(3))
+    return Scene(f"c3_scene_{res}_L{level}", res, float(h), float(dt), mn, c, V, F, steps, lis, neumann,
+                 info={"signal": g})
+
+
 def rel_l2(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
