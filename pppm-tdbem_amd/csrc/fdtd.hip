@@ -257,7 +257,9 @@ __global__ __launch_bounds__(BLK) void k_fdtd_abc(float *__restrict__ grid3, Fdt
     else
     {
         const int S = N - 2;
-        const long long fx = 2LL * S * S, fy = 2LL * (S - 2) * S, fz = 2LL * (S - 2) * (S - 2);
+        // z faces in 16 x 16 tiles of (x, y), padded: a block's cells share their neighbour rows
+        const int zt = (S - 2 + 15) / 16;
+        const long long fx = 2LL * S * S, fy = 2LL * (S - 2) * S, fz = 2LL * zt * zt * 256;
         // blocks [nb_inner, nb_inner + nb_fxy): the x and y faces (rows along z, coalesced);
         // the rest: the z faces, one cell per row, so their blocks are handed to the XCDs in
         // contiguous runs of x planes and a plane's rows meet its neighbours' in one L2
@@ -298,11 +300,14 @@ __global__ __launch_bounds__(BLK) void k_fdtd_abc(float *__restrict__ grid3, Fdt
         }
         else if ((idx -= fy) < fz)
         {
-            int side = (int)(idx / ((long long)(S - 2) * (S - 2)));
-            int r = (int)(idx % ((long long)(S - 2) * (S - 2)));
+            const int side = (int)(idx / ((long long)zt * zt * 256));
+            const int r = (int)(idx % ((long long)zt * zt * 256));
+            const int tile = r >> 8, w = r & 255;
             k = side ? N - 2 : 1;
-            i = 2 + r / (S - 2);
-            j = 2 + r % (S - 2);
+            i = 2 + (tile / zt) * 16 + (w >> 4);
+            j = 2 + (tile % zt) * 16 + (w & 15);
+            if (i > N - 3 || j > N - 3)
+                i = -1;
         }
         if (i >= 0)
             abc_cell(g, gm, gp, N, i, j, k, p);
@@ -414,7 +419,8 @@ hipError_t launch_fdtd(float *grid3, float *pml3, const FdtdParams &p, const Ste
         k_fdtd_march<<<ms.n_blocks, MBLK, 0, s>>>(grid3, p, st, ms);
         nb_inner = 0;
     }
-    const long long fxy = 2 * S * S + 2 * (S - 2) * S, fz = 2 * (S - 2) * (S - 2);
+    const long long zt = (S - 2 + 15) / 16;
+    const long long fxy = 2 * S * S + 2 * (S - 2) * S, fz = 2 * zt * zt * 256;
     const int nb_shell = (int)((fxy + BLK - 1) / BLK) + 8 * (int)((fz + 8LL * BLK - 1) / (8LL * BLK));
     k_fdtd_abc<<<nb_inner + nb_shell, BLK, 0, s>>>(grid3, p, st, nb_inner);
     return hipGetLastError();

@@ -1645,7 +1645,8 @@ int pppm_ghost_set_neumann(pppm_solver *s, const float *neumann, int on_device)
         return PPPM_ERR_ARG;
     if (!s->ghost_ready)
         return fail(s, PPPM_ERR_STATE, "ghost_set_neumann before ghost_setup");
-    CK(hipMemcpyAsync(s->gh_neu_old.p, s->gh_neu.p, s->nt * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
+    // previous <- current by swapping the two buffers, then the new data into current
+    s->gh_neu.swap(s->gh_neu_old);
     RC(upload(s, s->gh_neu.p, neumann, s->nt * sizeof(float), on_device));
     return PPPM_OK;
 }
