@@ -109,7 +109,7 @@ struct ListArgs
     int *cell_start, *cell_items, *cell_rank;
     int *c3_cells, *c3_n, *c3_index, *c3_cnt, *c3_start, *c3_pairs, *c3_wave;
     int cap_c3, c3_waves;
-    int *b4_cells, *b4_n, *b4_index, *b4_cnt, *b4_start, *b4_tris;
+    int *b4_cells, *b4_n, *b4_cnt, *b4_start, *b4_tris;
     int cap_b4items;
     int *face_cnt, *face_start, *face_nbr, *face_owner, *face_wave;
     int cap_p4, face_waves;

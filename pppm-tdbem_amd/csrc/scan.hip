@@ -3,10 +3,12 @@
 // triangles and face pairs (the reference uses thrust::remove_if for the same compactions,
 // src/core/array.h:77-115).
 //
-// Reduce-then-scan in two launches and no inter-block waiting: the first kernel writes one
-// sum per 1024-item tile, the second has every tile add up the sums of the tiles before it
-// (one wave), scan its own items and, in the last tile, write the total. Several independent
-// arrays ("jobs") share the two launches; jobs of at most 8192 items take a single launch.
+// Reduce-then-scan with no inter-block waiting: the first kernel writes one sum per 1024-item
+// tile, the second has every tile add up the sums of the tiles before it (one wave), scan its
+// own items and, in the last tile, write the total. Past 512 tiles in a job that per-tile sum
+// grows with the square of the tile count, so the tile offsets come instead from a scan of the
+// tile sums (the same routine, one level down). Several independent arrays ("jobs") share the
+// launches; jobs of at most 8192 items take a single launch.
 // A job in flag mode scans (in != 0) and can write the compacted list of selected indices
 // and an index map (rank or -1) in the same pass. Integer sums are exact, so the result does
 // not depend on the order of additions. Only kernels are enqueued (graph-capturable).
@@ -24,6 +26,7 @@ constexpr int SI = 4;          // items per thread (multi-tile scans)
 constexpr int SPB = SB * SI;   // items per tile
 static_assert(SPB == SCAN_TILE, "tile size shared with the producers of tile sums");
 constexpr int SI1 = 32;        // items per thread when one tile holds everything
+constexpr int kDirectTiles = 512;  // most tiles per job whose offsets the tiles sum themselves
 
 __device__ inline int wave_incl(int v)
 {
@@ -46,15 +49,17 @@ __device__ inline int wave_total(int v)
     return v;
 }
 
-// the job that owns global tile `tile` (jobs are laid out in tile order)
-__device__ inline const ScanJob &job_of(const ScanBatch &b, int tile)
+// index of the job that owns global tile `tile` (jobs are laid out in tile order)
+__device__ inline int job_index(const ScanBatch &b, int tile)
 {
     int k = 0;
     for (int q = 1; q < b.njobs; q++)
         if (tile >= b.job[q].tile0)
             k = q;
-    return b.job[k];
+    return k;
 }
+
+__device__ inline const ScanJob &job_of(const ScanBatch &b, int tile) { return b.job[job_index(b, tile)]; }
 
 __device__ inline int item_of(const ScanJob &J, long long q)
 {
@@ -102,11 +107,13 @@ __global__ __launch_bounds__(SB) void k_tile_sums(ScanBatch b, int *__restrict__
         sums[blockIdx.x] = btotal;
 }
 
-// sums == null: every job is a single tile (one block per job). IT items per thread. Items
-// are loaded striped (coalesced) into LDS, scanned in blocked order, and the positions go
-// back through LDS for striped stores.
+// sums == null: every job is a single tile (one block per job). offs != null: the exclusive
+// offset of tile t of job k within its job is offs[t + k]; otherwise it is the sum of sums[]
+// over the job's earlier tiles. IT items per thread. Items are loaded striped (coalesced)
+// into LDS, scanned in blocked order, and the positions go back through LDS for striped stores.
 template <int IT>
-__global__ __launch_bounds__(SB) void k_tile_scan(ScanBatch b, const int *__restrict__ sums)
+__global__ __launch_bounds__(SB) void k_tile_scan(ScanBatch b, const int *__restrict__ sums,
+                                                  const int *__restrict__ offs)
 {
     constexpr int T = SB * IT;
     __shared__ int s_off;
@@ -115,7 +122,12 @@ __global__ __launch_bounds__(SB) void k_tile_scan(ScanBatch b, const int *__rest
     const int tile = sums ? (int)blockIdx.x - J.tile0 : 0;
     const long long base = (long long)tile * T;
     const int tid = threadIdx.x;
-    if (tid < 64)
+    if (offs)
+    {
+        if (tid == 0)
+            s_off = offs[blockIdx.x + job_index(b, blockIdx.x)];
+    }
+    else if (tid < 64)
     {
         int acc = 0;
         if (sums)
@@ -180,18 +192,40 @@ __global__ __launch_bounds__(SB) void k_tile_scan(ScanBatch b, const int *__rest
 
 }  // namespace
 
+namespace
+{
+long long tiles_of(long long n) { return std::max<long long>(1, (n + SPB - 1) / SPB); }
+
+// ints of tmp a batch with these job lengths uses: the tile sums, and past kDirectTiles tiles in
+// a job, the tile offsets and the tmp of their own scan
+size_t tmp_ints(const long long *n, int njobs)
+{
+    long long t = 0, most = 0, inner[SCAN_MAX_JOBS];
+    for (int k = 0; k < njobs; k++)
+    {
+        inner[k] = tiles_of(n[k]);
+        t += inner[k];
+        most = std::max(most, inner[k]);
+    }
+    size_t need = (size_t)t + 16;
+    if (most > kDirectTiles)
+        need += (size_t)t + njobs + 16 + tmp_ints(inner, njobs);
+    return need;
+}
+}  // namespace
+
 size_t scan_tmp_ints(int n)
 {
-    const long long nb = ((long long)n + SPB - 1) / SPB;
-    return (size_t)(nb > 1 ? nb : 1) + 16;
+    const long long nn = n;
+    return tmp_ints(&nn, 1);
 }
 
 size_t scan_batch_tmp_ints(const int *n, int njobs)
 {
-    long long t = 0;
-    for (int k = 0; k < njobs; k++)
-        t += std::max<long long>(1, ((long long)n[k] + SPB - 1) / SPB);
-    return (size_t)t + 16;
+    long long nn[SCAN_MAX_JOBS];
+    for (int k = 0; k < njobs && k < SCAN_MAX_JOBS; k++)
+        nn[k] = n[k];
+    return tmp_ints(nn, std::min(njobs, SCAN_MAX_JOBS));
 }
 
 // Jobs with at most SB * SI1 items each run in one launch (a block per job); otherwise a
@@ -225,7 +259,7 @@ hipError_t scan_batch(ScanBatch b, int *tmp, hipStream_t s, const int *presums)
     const int tiles = scan_batch_layout(b);
     if (scan_batch_single(b))
     {
-        k_tile_scan<SI1><<<b.njobs, SB, 0, s>>>(b, nullptr);
+        k_tile_scan<SI1><<<b.njobs, SB, 0, s>>>(b, nullptr, nullptr);
         return hipGetLastError();
     }
     if (!presums)
@@ -233,7 +267,24 @@ hipError_t scan_batch(ScanBatch b, int *tmp, hipStream_t s, const int *presums)
         k_tile_sums<<<tiles, SB, 0, s>>>(b, tmp);
         presums = tmp;
     }
-    k_tile_scan<SI><<<tiles, SB, 0, s>>>(b, presums);
+    long long most = 0;
+    for (int k = 0; k < b.njobs; k++)
+        most = std::max(most, tiles_of(b.job[k].n));
+    int *offs = nullptr;
+    if (most > kDirectTiles)
+    {
+        // job k's tile offsets (and its total) land at offs + tile0 + k .. + tiles
+        offs = tmp + tiles + 16;
+        ScanBatch inner{};
+        inner.njobs = b.njobs;
+        for (int k = 0; k < b.njobs; k++)
+            inner.job[k] = ScanJob{presums + b.job[k].tile0, (int)tiles_of(b.job[k].n), 0, offs + b.job[k].tile0 + k,
+                                   nullptr, nullptr, nullptr, 0};
+        const hipError_t e = scan_batch(inner, offs + tiles + b.njobs + 16, s, nullptr);
+        if (e != hipSuccess)
+            return e;
+    }
+    k_tile_scan<SI><<<tiles, SB, 0, s>>>(b, presums, offs);
     return hipGetLastError();
 }
 

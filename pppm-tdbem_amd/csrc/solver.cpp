@@ -135,7 +135,7 @@ struct pppm_solver
     DBuf<int> cell_start, cell_items, cell_rank;
     DBuf<int> scan_tmp;
     DBuf<int> c3_cells, c3_index, c3_cnt, c3_start, c3_pairs, c3_wave;
-    DBuf<int> b4_cells, b4_index, b4_cnt, b4_start, b4_tris;
+    DBuf<int> b4_cells, b4_cnt, b4_start, b4_tris;
     DBuf<int> face_cnt, face_start, face_nbr, face_slot, face_owner, face_flag, dirty_list;
     // face lists and slots of the previous update (swapped with the current ones per update):
     // the lookup takes a pair's slot from there when the pair sits at the same place
@@ -364,7 +364,6 @@ int allocate(pppm_solver *s)
     CK(s->c3_wave.ensure(2 * (size_t)(s->c3_waves + 1)));
     CK(s->far.ensure(s->cap_c3));
     CK(s->b4_cells.ensure(nt));
-    CK(s->b4_index.ensure(NN));
     CK(s->b4_cnt.ensure(nt));
     CK(s->b4_start.ensure(nt + 1));
     CK(s->b4_tris.ensure(s->cap_b4items));
@@ -456,7 +455,6 @@ ListArgs list_args(pppm_solver *s, const UpdateCounters &cn)
     la.c3_waves = s->c3_waves;
     la.b4_cells = s->b4_cells.p;
     la.b4_n = stp + ST_B4;
-    la.b4_index = s->b4_index.p;
     la.b4_cnt = s->b4_cnt.p;
     la.b4_start = s->b4_start.p;
     la.b4_tris = s->b4_tris.p;
@@ -494,7 +492,7 @@ int enqueue_rebuild(pppm_solver *s, bool first, const float *verts_series, long 
     ScanBatch b1{}, b2{}, b3{};
     b1.njobs = 4;
     b1.job[0] = ScanJob{cn.cell_cnt, NN, 0, s->cell_start.p, nullptr, nullptr, nullptr, 0};
-    b1.job[1] = ScanJob{cn.base_cnt, NN, 1, nullptr, s->b4_cells.p, s->b4_index.p, stp + ST_B4, 0};
+    b1.job[1] = ScanJob{cn.base_cnt, NN, 1, nullptr, s->b4_cells.p, nullptr, stp + ST_B4, 0};
     b1.job[2] = ScanJob{cn.mark, NN, 1, nullptr, s->c3_cells.p, s->c3_index.p, stp + ST_C3, 0};
     b1.job[3] = ScanJob{s->gflag.p, nt, 1, nullptr, s->glist.p, nullptr, stp + ST_GDIRTY, 0};
     b2.njobs = 2;

@@ -382,16 +382,18 @@ def test_update_mesh_and_step_equals_two_calls(amd, scenes_mod):
         assert np.array_equal(a.get_grid(rel).view(np.uint32), b.get_grid(rel).view(np.uint32))
 
 
-@pytest.mark.parametrize("case", ["c3_24_L1", "c3_48_L2", "c3_96_L4", "c5_128_L5"])
+@pytest.mark.parametrize("case", ["c3_24_L1", "c3_48_L2", "c3_96_L4", "c5_128_L5", "c5_256_L5"])
 def test_config_parity(amd, oracle_mod, scenes_mod, case):
     """Scenes of the batched configuration (N 24..96, icosphere levels 1..4, anisotropic
-    scaling, per-scene pulse) and one scene of the large configuration (128^3, level 5, vertex
-    radii jittered by 5 %): GPU-built weights and 30 steps equal the FP64 oracle bit for bit."""
+    scaling, per-scene pulse) and scenes of the large configuration (level 5, vertex radii
+    jittered by 5 %; 128^3 over 30 steps, 256^3 over 4 steps, whose 16M-cell scans take the
+    two-level tile offsets): GPU-built weights and the steps equal the FP64 oracle bit for bit."""
     if case.startswith("c3"):
         _, n, lv = case.split("_")
         sc = scenes_mod.c3_scene(int(n), int(lv[1:]), seed=int(n), steps=30)
     else:
-        sc = scenes_mod.large_static_sphere(steps=30, res=128)
+        res = int(case.split("_")[1])
+        sc = scenes_mod.large_static_sphere(steps=30 if res == 128 else 4, res=res)
     print(f"config parity: {case}, {sc.res}^3, T={sc.tris.shape[0]}", flush=True)
     g, o = _pair(amd, oracle_mod, sc)
     assert np.array_equal(g.get_grid_weights().view(np.uint8), o.grid_weights().view(np.uint8))
