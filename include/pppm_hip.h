@@ -33,7 +33,8 @@ enum pppm_status
     PPPM_ERR_ARG = -1,         /* bad argument */
     PPPM_ERR_HIP = -2,         /* HIP runtime error */
     PPPM_ERR_OUT_OF_GRID = -3, /* a triangle lies outside cells [2, N-3] */
-    PPPM_ERR_STATE = -4,       /* call order (e.g. step before set_mesh) */
+    PPPM_ERR_STATE = -4,       /* call order (e.g. step before set_mesh, or after a failed
+                                  update_mesh until an update succeeds) */
     PPPM_ERR_CAPACITY = -5     /* an internal capacity was exceeded */
 };
 

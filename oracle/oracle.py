@@ -74,6 +74,7 @@ def lib():
             "orc_singular_fp64": ([vp, vp, vp, d, d, i, i, vp], None),
             "orc_classify": ([vp, i], i),
             "orc_get_cells": ([vp, vp, vp, vp, vp, vp, vp], i),
+            "orc_set_cells": ([vp, vp, vp, vp, vp], i),
             "orc_fill_fresh": ([vp, vp, vp], i),
             "orc_set_solid_zero": ([vp], i),
             "orc_vertex_normals": ([vp, vp], i),
@@ -267,6 +268,14 @@ class Oracle:
                                                  ("type", "ghost_idx", "nearest", "point", "reflect", "dist"))),
                  "get_cells")
         return out
+
+    def set_cells(self, ctype, ctype_old, nearest, point3):
+        """Test hook: cell types now / before, nearest triangles and points (N^3 entries)."""
+        a = np.ascontiguousarray(ctype, dtype=np.int32)
+        b = np.ascontiguousarray(ctype_old, dtype=np.int32)
+        n = np.ascontiguousarray(nearest, dtype=np.int32)
+        p = np.ascontiguousarray(point3, dtype=np.float32)
+        self._ok(self.L.orc_set_cells(self.s, _p(a), _p(b), _p(n), _p(p)), "set_cells")
 
     def fill_fresh(self, acc_now, acc_old):
         a = np.ascontiguousarray(acc_now, np.float32)

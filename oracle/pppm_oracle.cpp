@@ -2351,6 +2351,25 @@ int orc_get_cells(const orc_scene *s, int *type, int *ghost_idx, int *nearest, f
     return 0;
 }
 
+int orc_set_cells(orc_scene *s, const int *type, const int *type_old, const int *nearest, const float *point3)
+{
+    // test hook: cell data as classify / the previous classify would leave it, so that the
+    // fresh-cell fill can be checked on a hand-built stencil
+    if (!s->mesh_set)
+        return -1;
+    const size_t NN = (size_t)s->N * s->N * s->N;
+    s->ctype.assign(type, type + NN);
+    s->ctype_old.assign(type_old, type_old + NN);
+    s->cnearest.assign(nearest, nearest + NN);
+    s->cpoint.resize(NN);
+    std::memcpy(s->cpoint.data(), point3, NN * sizeof(f3));
+    s->cghost.assign(NN, -1);
+    s->creflect.assign(NN, f3{0, 0, 0});
+    s->cdist.assign(NN, 0.0f);
+    s->classified = true;
+    return 0;
+}
+
 int orc_fill_fresh(orc_scene *s, const float *acc_now, const float *acc_old)
 {
     if (!s->classified)

@@ -89,6 +89,9 @@ int orc_get_interp_weights(const orc_scene *s, float *out8);
 int orc_classify(orc_scene *s, int thin_shell);
 int orc_get_cells(const orc_scene *s, int *type, int *ghost_idx, int *nearest, float *point3, float *reflect3,
                   float *dist);
+/* test hook: sets the current and previous cell types, nearest triangles and nearest surface
+ * points (N^3 entries each) directly, as if classify had produced them */
+int orc_set_cells(orc_scene *s, const int *type, const int *type_old, const int *nearest, const float *point3);
 int orc_fill_fresh(orc_scene *s, const float *acc_now, const float *acc_old);
 int orc_set_solid_zero(orc_scene *s);
 int orc_vertex_normals(const orc_scene *s, float *out3);

@@ -120,6 +120,9 @@ struct pppm_solver
 
     int nv = 0, nt = 0;
     bool mesh_set = false;
+    // the last mesh update failed (out of grid, bad index, capacity): the lists and weights no
+    // longer describe the mesh, so the step refuses to run until an update succeeds
+    bool mesh_stale = false;
     DBuf<float> verts;
     DBuf<int> tris;
     DBuf<TriGeom> geom;
@@ -608,7 +611,7 @@ int grow(pppm_solver *s)
 // grown capacities if a list or the pool overflowed
 // vsrc (device, may be null): vertex positions to take instead of s->verts; the first
 // attempt copies them into s->verts
-int rebuild_sync(pppm_solver *s, bool first, const float *vsrc = nullptr)
+int rebuild_attempts(pppm_solver *s, bool first, const float *vsrc)
 {
     // the last update's face lists become the previous ones; this update writes the others
     bool use_prev = !first && s->faces_valid;
@@ -647,6 +650,24 @@ int rebuild_sync(pppm_solver *s, bool first, const float *vsrc = nullptr)
         use_prev = false;  // grow may have moved the previous lists
     }
     return status_error(s);
+}
+
+int rebuild_sync(pppm_solver *s, bool first, const float *vsrc = nullptr)
+{
+    const int rc = rebuild_attempts(s, first, vsrc);
+    s->mesh_stale = rc != PPPM_OK;
+    return rc;
+}
+
+// the step and its phases need a mesh whose last update succeeded
+int check_ready(pppm_solver *s, const char *what)
+{
+    if (!s->mesh_set)
+        return fail(s, PPPM_ERR_STATE, std::string(what) + " before set_mesh");
+    if (s->mesh_stale)
+        return fail(s, PPPM_ERR_STATE,
+                    std::string(what) + ": the last mesh update failed; update_mesh with a valid mesh first");
+    return PPPM_OK;
 }
 
 int upload(pppm_solver *s, void *dst, const void *src, size_t bytes, int on_device)
@@ -731,6 +752,8 @@ int run_impl(pppm_solver *s, const float *verts_series, long long vstride, const
         return fail(s, PPPM_ERR_ARG, "run: neumann_series required");
     if (verts_series && !s->mesh_set)
         return fail(s, PPPM_ERR_STATE, "run: moving mesh before set_mesh");
+    if (s->mesh_set && !verts_series)
+        RC(check_ready(s, "run"));
     if (steps == 0)
         return PPPM_OK;
     // a listener's C3 index can change with the lists: moving runs read the grid directly
@@ -1025,8 +1048,7 @@ int pppm_step(pppm_solver *s, const float *neumann, int on_device)
 {
     if (check_handle(s))
         return PPPM_ERR_ARG;
-    if (!s->mesh_set)
-        return fail(s, PPPM_ERR_STATE, "step before set_mesh");
+    RC(check_ready(s, "step"));
     const float *nd = neumann;
     if (neumann && !on_device)
     {
@@ -1081,8 +1103,7 @@ int pppm_solve_far(pppm_solver *s)
 {
     if (check_handle(s))
         return PPPM_ERR_ARG;
-    if (!s->mesh_set)
-        return fail(s, PPPM_ERR_STATE, "solve_far before set_mesh");
+    RC(check_ready(s, "solve_far"));
     const StepArgs a = step_args(s, s->t_host, 0, 0);
     CK(launch_windows(s->nt, 1, s->state.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->stream));
     CK(launch_far(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_wave.p, s->c3_waves, s->c3_pairs.p,
@@ -1094,8 +1115,9 @@ int pppm_set_neumann(pppm_solver *s, const float *neumann, int on_device)
 {
     if (check_handle(s))
         return PPPM_ERR_ARG;
-    if (!s->mesh_set || !neumann)
-        return fail(s, PPPM_ERR_STATE, "set_neumann before set_mesh");
+    if (!neumann)
+        return fail(s, PPPM_ERR_ARG, "set_neumann: null input");
+    RC(check_ready(s, "set_neumann"));
     const float *nd = neumann;
     if (!on_device)
     {
@@ -1111,8 +1133,7 @@ int pppm_update_dirichlet(pppm_solver *s)
 {
     if (check_handle(s))
         return PPPM_ERR_ARG;
-    if (!s->mesh_set)
-        return fail(s, PPPM_ERR_STATE, "update_dirichlet before set_mesh");
+    RC(check_ready(s, "update_dirichlet"));
     const StepArgs a = step_args(s, s->t_host, 0, 0);
     CK(launch_windows(s->nt, 1, s->state.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->stream));
     CK(launch_dirichlet(a, s->state.p, s->face_start.p, s->face_wave.p, s->face_waves, s->face_nbr.p, s->face_slot.p,
@@ -1125,8 +1146,7 @@ int pppm_solve_near(pppm_solver *s)
 {
     if (check_handle(s))
         return PPPM_ERR_ARG;
-    if (!s->mesh_set)
-        return fail(s, PPPM_ERR_STATE, "solve_near before set_mesh");
+    RC(check_ready(s, "solve_near"));
     const StepArgs a = step_args(s, s->t_host, 0, 0);
     CK(launch_windows(s->nt, 1, s->state.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->stream));
     CK(launch_near(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_wave.p, s->c3_waves, s->c3_pairs.p,

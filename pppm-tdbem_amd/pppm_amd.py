@@ -118,16 +118,54 @@ def load_library(path: str = LIB_PATH):
     return lib
 
 
-def _ptr(a):
-    """(pointer, on_device) for a numpy array or a torch tensor."""
+_NP = {"f32": np.float32, "i32": np.int32}
+
+
+class _Arg:
+    """A validated array argument: `ptr` for the C call, `dev` (1: device memory), and `keep`,
+    the host array the pointer refers to (held for the duration of the call)."""
+
+    __slots__ = ("ptr", "dev", "keep")
+
+    def __init__(self, ptr, dev, keep):
+        self.ptr, self.dev, self.keep = ptr, dev, keep
+
+
+def _arg(a, name, count, kind="f32", where="any"):
+    """Checks an array argument before it reaches the C ABI: element type (`kind`: f32 / i32),
+    at least `count` elements, and its memory (`where`: "device", "host" or "any"). Host data
+    may be any array-like (converted to a contiguous numpy array of the right type); a torch
+    tensor must already have the right dtype and be contiguous. Raises PPPMError, so a short or
+    host buffer never reaches a kernel as a device pointer."""
     if a is None:
-        return None, 0
+        raise PPPMError(f"{name} is required")
     if hasattr(a, "data_ptr"):  # torch tensor
+        import torch
+        want = torch.float32 if kind == "f32" else torch.int32
+        if a.dtype != want:
+            raise PPPMError(f"{name} must be {want}, got {a.dtype}")
         if not a.is_contiguous():
-            raise PPPMError("tensor must be contiguous")
-        return ctypes.c_void_p(a.data_ptr()), int(a.is_cuda)
-    arr = np.ascontiguousarray(a)
-    return arr.ctypes.data_as(ctypes.c_void_p), 0
+            raise PPPMError(f"{name} must be contiguous")
+        dev = int(a.is_cuda)
+        if a.numel() < count:
+            raise PPPMError(f"{name} holds {a.numel()} elements, needs {count}")
+        keep = a
+        ptr = ctypes.c_void_p(a.data_ptr())
+    else:
+        arr = np.asarray(a)
+        if arr.dtype != _NP[kind]:
+            if kind == "i32" and not np.issubdtype(arr.dtype, np.integer):
+                raise PPPMError(f"{name} must be an integer array")
+            arr = arr.astype(_NP[kind])
+        arr = np.ascontiguousarray(arr)
+        if arr.size < count:
+            raise PPPMError(f"{name} holds {arr.size} elements, needs {count}")
+        dev, keep, ptr = 0, arr, arr.ctypes.data_as(ctypes.c_void_p)
+    if where == "device" and not dev:
+        raise PPPMError(f"{name} must be device memory (a CUDA tensor)")
+    if where == "host" and dev:
+        raise PPPMError(f"{name} must be host memory")
+    return _Arg(ptr, dev, keep)
 
 
 class PPPMSolver:
@@ -194,63 +232,62 @@ class PPPMSolver:
         self._check(self.lib.pppm_set_time_idx(self.h, int(t)), "set_time_idx")
 
     def set_mesh(self, verts, tris):
-        if hasattr(verts, "data_ptr"):
-            nv, nt = verts.shape[0], tris.shape[0]
-            v, dv = _ptr(verts)
-            t, dt_ = _ptr(tris)
-            if dv != dt_:
-                raise PPPMError("verts and tris must both be host or both device")
-        else:
-            verts = np.ascontiguousarray(verts, dtype=np.float32)
-            tris = np.ascontiguousarray(tris, dtype=np.int32)
-            nv, nt = verts.shape[0], tris.shape[0]
-            v, dv = _ptr(verts)
-            t, _ = _ptr(tris)
-        self._check(self.lib.pppm_set_mesh(self.h, v, nv, t, nt, dv), "set_mesh")
+        nv = int(np.shape(verts)[0]) if np.ndim(verts) == 2 else int(np.size(verts)) // 3
+        nt = int(np.shape(tris)[0]) if np.ndim(tris) == 2 else int(np.size(tris)) // 3
+        v = _arg(verts, "verts", 3 * nv, "f32")
+        t = _arg(tris, "tris", 3 * nt, "i32")
+        if v.dev != t.dev:
+            raise PPPMError("verts and tris must both be host or both device")
+        self._check(self.lib.pppm_set_mesh(self.h, v.ptr, nv, t.ptr, nt, v.dev), "set_mesh")
         self.nv, self.nt = nv, nt
 
     def update_mesh(self, verts):
-        if not hasattr(verts, "data_ptr"):
-            verts = np.ascontiguousarray(verts, dtype=np.float32)
-        v, dv = _ptr(verts)
-        self._check(self.lib.pppm_update_mesh(self.h, v, dv), "update_mesh")
+        v = _arg(verts, "verts", 3 * self.nv, "f32")
+        self._check(self.lib.pppm_update_mesh(self.h, v.ptr, v.dev), "update_mesh")
 
     def update_mesh_and_step(self, verts, neumann):
         """update_mesh(verts) then update_grid_and_face(neumann) in one call; both numpy arrays
         or both device tensors."""
-        if not hasattr(verts, "data_ptr"):
-            verts = np.ascontiguousarray(verts, dtype=np.float32)
-            neumann = np.ascontiguousarray(neumann, dtype=np.float32)
-        pv, dev = _ptr(verts)
-        pn, _ = _ptr(neumann)
-        self._check(self.lib.pppm_update_mesh_and_step(self.h, pv, pn, dev), "update_mesh_and_step")
+        v = _arg(verts, "verts", 3 * self.nv, "f32")
+        n = _arg(neumann, "neumann", self.nt, "f32")
+        if v.dev != n.dev:
+            raise PPPMError("verts and neumann must both be host or both device")
+        self._check(self.lib.pppm_update_mesh_and_step(self.h, v.ptr, n.ptr, v.dev), "update_mesh_and_step")
 
     def update_grid_and_face(self, neumann):
-        if not hasattr(neumann, "data_ptr"):
-            neumann = np.ascontiguousarray(neumann, dtype=np.float32)
-        p, d = _ptr(neumann)
-        self._check(self.lib.pppm_step(self.h, p, d), "step")
+        n = _arg(neumann, "neumann", self.nt, "f32")
+        self._check(self.lib.pppm_step(self.h, n.ptr, n.dev), "step")
 
     update_step = update_grid_and_face
 
+    def _listeners(self, listeners, out_dev, steps):
+        lx = np.ascontiguousarray(np.asarray(listeners, dtype=np.int32).reshape(-1, 3))
+        if lx.shape[0] and (lx.min() < 0 or lx.max() >= self._res):
+            raise PPPMError("listener outside the grid")
+        out = _arg(out_dev, "out", int(steps) * lx.shape[0], "f32", "device") if lx.shape[0] else None
+        return lx, out
+
     def run(self, neumann_series_dev, steps, listeners, out_dev, use_graph=True, stride=None):
         """steps x stride device Neumann series -> out_dev[steps, n_listen] (device)."""
-        lx = np.ascontiguousarray(np.asarray(listeners, dtype=np.int32).reshape(-1, 3))
-        stride = self.nt if stride is None else stride
-        np_, _ = _ptr(neumann_series_dev)
-        op, _ = _ptr(out_dev)
-        self._check(self.lib.pppm_run(self.h, np_, int(stride), int(steps), lx.ctypes.data_as(ctypes.c_void_p),
-                                      lx.shape[0], op, int(use_graph)), "run")
+        stride = self.nt if stride is None else int(stride)
+        if stride < self.nt or steps < 0:
+            raise PPPMError("run: stride must be >= the triangle count and steps >= 0")
+        lx, out = self._listeners(listeners, out_dev, steps)
+        need = (int(steps) - 1) * stride + self.nt if steps > 0 else 0
+        n = _arg(neumann_series_dev, "neumann_series", need, "f32", "device")
+        self._check(self.lib.pppm_run(self.h, n.ptr, stride, int(steps), lx.ctypes.data_as(ctypes.c_void_p),
+                                      lx.shape[0], out.ptr if out else None, int(use_graph)), "run")
 
     def run_moving(self, verts_series_dev, neumann_series_dev, steps, listeners, out_dev, use_graph=True):
         """Moving mesh: before step i the vertices become verts_series_dev[i] (device, steps x nv x 3)."""
-        lx = np.ascontiguousarray(np.asarray(listeners, dtype=np.int32).reshape(-1, 3))
-        vp_, _ = _ptr(verts_series_dev)
-        np_, _ = _ptr(neumann_series_dev)
-        op, _ = _ptr(out_dev)
-        self._check(self.lib.pppm_run_moving(self.h, vp_, int(self.nv * 3), np_, int(self.nt), int(steps),
-                                             lx.ctypes.data_as(ctypes.c_void_p), lx.shape[0], op, int(use_graph)),
-                    "run_moving")
+        if steps < 0:
+            raise PPPMError("run_moving: steps >= 0")
+        lx, out = self._listeners(listeners, out_dev, steps)
+        v = _arg(verts_series_dev, "verts_series", int(steps) * 3 * self.nv, "f32", "device")
+        n = _arg(neumann_series_dev, "neumann_series", int(steps) * self.nt, "f32", "device")
+        self._check(self.lib.pppm_run_moving(self.h, v.ptr, int(self.nv * 3), n.ptr, int(self.nt), int(steps),
+                                             lx.ctypes.data_as(ctypes.c_void_p), lx.shape[0],
+                                             out.ptr if out else None, int(use_graph)), "run_moving")
 
     def fdtd_step(self):
         self._check(self.lib.pppm_fdtd_step(self.h), "fdtd_step")
@@ -259,10 +296,8 @@ class PPPMSolver:
         self._check(self.lib.pppm_solve_far(self.h), "solve_far")
 
     def set_neumann_condition(self, neumann):
-        if not hasattr(neumann, "data_ptr"):
-            neumann = np.ascontiguousarray(neumann, dtype=np.float32)
-        p, d = _ptr(neumann)
-        self._check(self.lib.pppm_set_neumann(self.h, p, d), "set_neumann")
+        n = _arg(neumann, "neumann", self.nt, "f32")
+        self._check(self.lib.pppm_set_neumann(self.h, n.ptr, n.dev), "set_neumann")
 
     def update_dirichlet(self):
         self._check(self.lib.pppm_update_dirichlet(self.h), "update_dirichlet")
@@ -430,13 +465,12 @@ class PPPMSolver:
 
     def fill_fresh(self, acc_now, acc_old):
         """Returns the number of fresh cells (-1: one had no nearest triangle)."""
-        if not hasattr(acc_now, "data_ptr"):
-            acc_now = np.ascontiguousarray(acc_now, dtype=np.float32)
-            acc_old = np.ascontiguousarray(acc_old, dtype=np.float32)
-        a, dev = _ptr(acc_now)
-        b, _ = _ptr(acc_old)
+        a = _arg(acc_now, "acc_now", self.nt, "f32")
+        b = _arg(acc_old, "acc_old", self.nt, "f32")
+        if a.dev != b.dev:
+            raise PPPMError("acc_now and acc_old must both be host or both device")
         n = ctypes.c_int()
-        self._check(self.lib.pppm_fill_fresh(self.h, a, b, dev, ctypes.byref(n)), "fill_fresh")
+        self._check(self.lib.pppm_fill_fresh(self.h, a.ptr, b.ptr, a.dev, ctypes.byref(n)), "fill_fresh")
         return n.value
 
     def set_solid_zero(self):
@@ -451,10 +485,8 @@ class PPPMSolver:
         return c.value
 
     def ghost_set_neumann(self, neumann):
-        if not hasattr(neumann, "data_ptr"):
-            neumann = np.ascontiguousarray(neumann, dtype=np.float32)
-        p, dev = _ptr(neumann)
-        self._check(self.lib.pppm_ghost_set_neumann(self.h, p, dev), "ghost_set_neumann")
+        n = _arg(neumann, "neumann", self.nt, "f32")
+        self._check(self.lib.pppm_ghost_set_neumann(self.h, n.ptr, n.dev), "ghost_set_neumann")
 
     def ghost_solve(self):
         """Right-hand side, solve and ghost values at the current level; returns the iterations."""
@@ -464,18 +496,14 @@ class PPPMSolver:
 
     def ghost_step(self, neumann):
         """GhostCellSolver::update: FDTD step, Neumann data, ghost solve, SOLID cells to zero."""
-        if not hasattr(neumann, "data_ptr"):
-            neumann = np.ascontiguousarray(neumann, dtype=np.float32)
-        p, dev = _ptr(neumann)
-        self._check(self.lib.pppm_ghost_step(self.h, p, dev), "ghost_step")
+        n = _arg(neumann, "neumann", self.nt, "f32")
+        self._check(self.lib.pppm_ghost_step(self.h, n.ptr, n.dev), "ghost_step")
 
     def ghost_update_mesh(self, verts):
         """GhostCellSolver::update_mesh; returns the fresh-cell count."""
-        if not hasattr(verts, "data_ptr"):
-            verts = np.ascontiguousarray(verts, dtype=np.float32)
-        p, dev = _ptr(verts)
+        v = _arg(verts, "verts", 3 * self.nv, "f32")
         n = ctypes.c_int()
-        self._check(self.lib.pppm_ghost_update_mesh(self.h, p, dev, ctypes.byref(n)), "ghost_update_mesh")
+        self._check(self.lib.pppm_ghost_update_mesh(self.h, v.ptr, v.dev, ctypes.byref(n)), "ghost_update_mesh")
         return n.value
 
     def ghost_get(self, n):
@@ -580,11 +608,10 @@ class ModalSource:
 
     def run(self, forces, accs):
         """forces (S, M) host array or device tensor; accs (S, T) float32 device tensor."""
-        fp, fdev = _ptr(forces)
-        ap, adev = _ptr(accs)
-        if not adev:
-            raise PPPMError("accs must be device memory")
-        rc = self.lib.pppm_modal_run(self.h, fp, int(forces.shape[0]), ap, fdev)
+        ns = int(forces.shape[0])
+        f = _arg(forces, "forces", ns * self.nmodes, "f32")
+        a = _arg(accs, "accs", ns * self.nt, "f32", "device")
+        rc = self.lib.pppm_modal_run(self.h, f.ptr, ns, a.ptr, f.dev)
         if rc != 0:
             raise PPPMError(f"pppm_modal_run failed ({rc})")
 

@@ -340,6 +340,34 @@ def test_error_paths(amd):
         g.set_mesh(v, np.array([[0, 1, 2]], np.int32))
 
 
+def test_failed_update_blocks_the_step_until_a_valid_update(amd, scenes_mod):
+    """After an update_mesh that fails (every triangle outside cells [2, N-3]) the lists and
+    weights no longer describe the mesh: the step and its phases return PPPM_ERR_STATE until an
+    update succeeds; then the handle steps exactly as one that never saw the failure."""
+    sc = scenes_mod.c1_static_sphere(steps=6)
+    print("failed update then valid update: C1 sphere, 6 steps", flush=True)
+    a = amd.PPPMSolver(sc.res, sc.dl, sc.dt, sc.min_pos, 0, 0.83, sc.c)
+    b = amd.PPPMSolver(sc.res, sc.dl, sc.dt, sc.min_pos, 0, 0.83, sc.c)
+    for g in (a, b):
+        g.set_mesh(sc.verts, sc.tris)
+        g.update_grid_and_face(sc.neumann[0])
+    bad = sc.verts.copy()
+    bad[:, 0] += np.float32(1.0)
+    with pytest.raises(amd.PPPMError):
+        a.update_mesh(bad)
+    for call in (lambda: a.update_grid_and_face(sc.neumann[1]), a.solve_fdtd_far, a.update_dirichlet,
+                 a.solve_fdtd_near, lambda: a.set_neumann_condition(sc.neumann[1])):
+        with pytest.raises(amd.PPPMError, match="last mesh update failed"):
+            call()
+    a.update_mesh(sc.verts)
+    b.update_mesh(sc.verts)
+    for i in range(1, sc.steps):
+        a.update_grid_and_face(sc.neumann[i])
+        b.update_grid_and_face(sc.neumann[i])
+    for rel in (0, -1):
+        assert np.array_equal(a.get_grid(rel).view(np.uint32), b.get_grid(rel).view(np.uint32))
+
+
 def test_error_paths_of_the_extra_entry_points(amd):
     """Argument and state checks of the ghost-cell, image and modal entry points (host side)."""
     g = amd.PPPMSolver(16, 0.01, 1e-5, (0, 0, 0), 0, 0.83, 343.0)

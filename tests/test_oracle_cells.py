@@ -168,3 +168,68 @@ def test_set_solid_zero(oracle_mod, scenes_mod):
     t = o.cells()["type"]
     g = o.grid(0).ravel()
     assert np.all(g[t == SOLID] == 0) and np.all(g[t != SOLID] == 1)
+
+
+def _stencil_scene(oracle_mod, air_cells, fresh=(4, 5, 5), xr_cells=(8.75, 5.5, 5.5)):
+    """16^3 grid (h = 5 cm, origin 0), every cell SOLID before and after except the given cells,
+    AIR before and after, and one fresh cell (SOLID before, AIR after) whose nearest surface
+    point is placed so that its mirror point 2 xf - xb lands at xr_cells (in cell units)."""
+    N, h = 16, 0.05
+    o = oracle_mod.Oracle(N, h, h / (math.sqrt(3) * 343.0), (0, 0, 0), 343.0, 0.83, oracle_mod.FP64)
+    tri = np.array([[12.1, 12.2, 12.3], [12.6, 12.2, 12.3], [12.1, 12.7, 12.3]], np.float32) * h
+    o.set_mesh(tri, np.array([[0, 1, 2]], np.int32))
+    NN = N ** 3
+    lin = lambda p: (p[0] * N + p[1]) * N + p[2]  # noqa: E731
+    now = np.full(NN, SOLID, np.int32)
+    old = np.full(NN, SOLID, np.int32)
+    for p in air_cells:
+        now[lin(p)] = old[lin(p)] = AIR
+    now[lin(fresh)] = AIR
+    xf = (np.array(fresh) + 0.5) * h
+    xb = 2 * xf - np.array(xr_cells) * h
+    point = np.zeros((NN, 3), np.float32)
+    point[lin(fresh)] = xb
+    nearest = np.zeros(NN, np.int32)
+    o.set_cells(now, old, nearest, point)
+    rng = np.random.default_rng(31)
+    g0, g1 = rng.standard_normal(NN).astype(np.float32), rng.standard_normal(NN).astype(np.float32)
+    o.set_grid(0, g0)
+    o.set_grid(-1, g1)
+    return o, g0, g1, lin, np.linalg.norm(xf - np.array(xr_cells) * h)
+
+
+def test_fresh_fill_weights_are_inverse_distance_squared(oracle_mod):
+    """Shepard interpolation with power 2 (ghost_cell.cu:120-131): the mirror point sits a
+    quarter of the way from the AIR cell (8,5,5) to the AIR cell (9,5,5), the other six cells
+    of its 2x2x2 block are not AIR, so the weights are 1/(h/4)^2 : 1/(3h/4)^2 = 9 : 1
+    (inverse distance would give 3 : 1), each with the reference's EPS = 1e-8 added to the
+    squared distance. The acceleration term is -1.225 a |xf - xr|."""
+    A, B = (8, 5, 5), (9, 5, 5)
+    o, g0, g1, lin, L = _stencil_scene(oracle_mod, [A, B])
+    acc_now, acc_old = np.float32([0.7]), np.float32([-0.4])
+    assert o.fill_fresh(acc_now, acc_old) == 1
+    c = lin((4, 5, 5))
+    n0, n1 = o.grid(0).ravel(), o.grid(-1).ravel()
+    h = 0.05
+    wa, wb = 1 / ((h / 4) ** 2 + 1e-8), 1 / ((3 * h / 4) ** 2 + 1e-8)
+    assert wa / wb == pytest.approx(9.0, rel=2e-4)
+    for got, g, acc in ((n0[c], g0, acc_now[0]), (n1[c], g1, acc_old[0])):
+        want = (wa * g[lin(A)] + wb * g[lin(B)]) / (wa + wb) - 1.225 * acc * L
+        assert got == pytest.approx(want, rel=1e-5, abs=1e-6)
+    # nothing else changes
+    mask = np.ones(len(n0), bool)
+    mask[c] = False
+    assert np.array_equal(n0[mask], g0[mask]) and np.array_equal(n1[mask], g1[mask])
+
+
+def test_fresh_fill_falls_back_to_the_nearest_air_cell(oracle_mod):
+    """No AIR cell in the 2x2x2 block of the mirror point: the value (at t and t-1) is copied
+    from the nearest cell that is AIR before and after within the 4x4x4 block around the base
+    cell, without the acceleration term (ghost_cell.cu:136-160). Of (10,5,5) at 1.25 h and
+    (7,6,7) at sqrt(2.5625 + 1 + 2.25) h, the first is nearer."""
+    near, far = (10, 5, 5), (7, 6, 7)
+    o, g0, g1, lin, _ = _stencil_scene(oracle_mod, [near, far])
+    assert o.fill_fresh(np.float32([5.0]), np.float32([5.0])) == 1
+    c = lin((4, 5, 5))
+    assert o.grid(0).ravel()[c] == g0[lin(near)]
+    assert o.grid(-1).ravel()[c] == g1[lin(near)]
