@@ -81,6 +81,20 @@ int pppm_update_mesh_and_step(pppm_solver *s, const float *verts, const float *n
 int pppm_run(pppm_solver *s, const float *neumann_series, long long stride, int steps, const int *listen_xyz,
              int n_listen, float *out_dev, int use_graph);
 
+/* pppm_run_noalloc: pppm_run without the graph and without any device allocation — no
+ * counterpart in the reference. For callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc. Same arguments and
+ * results as pppm_run(..., use_graph = 0) on a static mesh (bit for bit); the steps are
+ * enqueued asynchronously on the handle's stream (synchronise it before reading out_dev).
+ * The gathers' per-step buffers (history windows, far field, listener table) live in one
+ * file-scope device array of 16 MiB per GPU, held until the process ends; a mesh needs
+ * 512 + 2 * ceil256(64 nt) + 4 * min(27 nt, N^3) bytes of it (about 70 000 triangles at most),
+ * PPPM_ERR_CAPACITY beyond. Every call on a GPU shares that GPU's array: calls that overlap on
+ * one GPU (two handles on concurrent streams) race; consecutive calls are independent (each
+ * writes what it reads). Timing spans (pppm_set_timing) are not recorded. */
+int pppm_run_noalloc(pppm_solver *s, const float *neumann_series, long long stride, int steps, const int *listen_xyz,
+                     int n_listen, float *out_dev);
+
 /* pppm_run for a moving mesh: before step i the vertices are replaced by row i of
  * verts_series (device memory, steps x vstride floats, vstride >= 3 nv) and the mesh is
  * updated as by pppm_update_mesh — the demo loop's UpdateSound (experiments/demo/main.cu:
@@ -89,6 +103,35 @@ int pppm_run(pppm_solver *s, const float *neumann_series, long long stride, int 
  * ignored for moving meshes. */
 int pppm_run_moving(pppm_solver *s, const float *verts_series, long long vstride, const float *neumann_series,
                     long long stride, int steps, const int *listen_xyz, int n_listen, float *out_dev, int use_graph);
+
+/* Batched scenes: independent scenes stepped together, each kernel of the time step one
+ * launch over every scene (no counterpart in the reference, which steps one PPPMSolver at a
+ * time: src/algorithm/pppm.h:137-145 per scene). scenes[i] are handles with their meshes set
+ * (static meshes, no PML); the batch references them, so they must outlive it, and its runs
+ * advance their time indices as pppm_run would. Each scene's results equal its own pppm_run
+ * bit for bit.
+ * pppm_batch_run: per scene i, neumann_series[i] (device, steps x strides[i] floats, row k =
+ *   the Neumann input of step k, strides[i] >= nt_i) and one listener cell listen_xyz[3i..3i+2];
+ *   out_dev (device, steps x n_scenes floats) gets out_dev[k * n_scenes + i] = scene i's listener
+ *   after step k. use_graph != 0: 32 batch steps per captured HIP graph launch. Synchronous.
+ * pppm_batch_autotune: times every FDTD interior variant (0: one thread per cell; 1..4:
+ *   LDS-staged 2.5D tiles of 8x64, 16x32, 4x64, 8x32 (y, z) columns) for each size class of grid
+ *   edge N present in the batch (classes [0,40) [40,56) [56,72) [72,88) [88,112) [112,160)
+ *   [160,...)) over `reps` repetitions, keeps the fastest per class, and (ms_out optional,
+ *   7 x 5 floats) reports the milliseconds per repetition [class][variant] (untimed entries are
+ *   left as they were). The FDTD launches it times do not advance t and rewrite the same values,
+ *   so the scenes' state is unchanged.
+ * pppm_batch_variants / pppm_batch_set_variants: the FDTD variant of each scene (n_scenes ints). */
+typedef struct pppm_batch pppm_batch;
+int pppm_batch_create(pppm_batch **out, pppm_solver *const *scenes, int n_scenes);
+int pppm_batch_destroy(pppm_batch *b);
+const char *pppm_batch_last_error(const pppm_batch *b);
+int pppm_batch_set_stream(pppm_batch *b, void *hip_stream);
+int pppm_batch_run(pppm_batch *b, const float *const *neumann_series, const long long *strides, int steps,
+                   const int *listen_xyz, float *out_dev, int use_graph);
+int pppm_batch_autotune(pppm_batch *b, int reps, float *ms_out);
+int pppm_batch_variants(pppm_batch *b, int *variants);
+int pppm_batch_set_variants(pppm_batch *b, const int *variants);
 
 /* Phase entry points used by the reference's tests (src/algorithm/pppm.h:127-135,
  * src/algorithm/fdtd.h:73-87). */

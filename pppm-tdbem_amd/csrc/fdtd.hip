@@ -229,20 +229,20 @@ __global__ __launch_bounds__(MBLK) void k_fdtd_march(float *__restrict__ grid3, 
 }
 
 // One thread per interior cell (blocks [0, nb_inner)) and the ABC shell (the remaining
-// blocks); nb_inner = 0 when the interior is marched by k_fdtd_march instead.
-__global__ __launch_bounds__(BLK) void k_fdtd_abc(float *__restrict__ grid3, FdtdParams p, const StepState *__restrict__ st,
-                                                  int nb_inner)
+// blocks); nb_inner = 0 when the interior is marched by another kernel. blk: the block's index
+// within this scene's launch (or its share of a batched launch).
+__device__ __forceinline__ void fdtd_block(float *__restrict__ grid3, const FdtdParams &p, int t, int blk,
+                                           int nb_inner)
 {
     const int N = p.N;
     const size_t nn = (size_t)N * N * N;
-    const int t = st->t;
     float *g = level(grid3, t, 0, nn);
     const float *gm = level(grid3, t, -1, nn);
     float *gp = level(grid3, t, 1, nn);
-    if ((int)blockIdx.x < nb_inner)
+    if (blk < nb_inner)
     {
         const int M = N - 4;
-        const long long idx = (long long)blockIdx.x * BLK + threadIdx.x;
+        const long long idx = (long long)blk * BLK + threadIdx.x;
         if (idx < (long long)M * M * M)
         {
             const int x = 2 + (int)(idx / (M * M));
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(BLK) void k_fdtd_abc(float *__restrict__ grid3, Fdt
         // blocks [nb_inner, nb_inner + nb_fxy): the x and y faces (rows along z, coalesced);
         // the rest: the z faces, one cell per row, so their blocks are handed to the XCDs in
         // contiguous runs of x planes and a plane's rows meet its neighbours' in one L2
-        const int b = (int)blockIdx.x - nb_inner;
+        const int b = blk - nb_inner;
         const int nb_fxy = (int)((fx + fy + BLK - 1) / BLK);
         long long idx;
         if (b < nb_fxy)
@@ -312,6 +312,96 @@ __global__ __launch_bounds__(BLK) void k_fdtd_abc(float *__restrict__ grid3, Fdt
         if (i >= 0)
             abc_cell(g, gm, gp, N, i, j, k, p);
     }
+}
+
+__global__ __launch_bounds__(BLK) void k_fdtd_abc(float *__restrict__ grid3, FdtdParams p, const StepState *__restrict__ st,
+                                                  int nb_inner)
+{
+    fdtd_block(grid3, p, st->t, (int)blockIdx.x, nb_inner);
+}
+
+// Interior of a small grid as LDS-staged 2.5D tiles: a block of TY x TZ threads owns TY x TZ
+// (y, z) columns of one chunk of x planes and marches them along x. Per plane the block stages
+// the level-t values of its tile plus a one-cell halo in y and z in LDS (double buffered), so
+// each level-t value comes from memory about once ((TY+2)(TZ+2) / (TY TZ) with the halo); the
+// x-neighbours stay in registers. Threads outside 2..N-3 only help with the staging. The
+// update is the same leapfrog on the same seven values, so the results equal the one thread
+// per cell kernel's bit for bit.
+template <int TY, int TZ>
+__device__ __forceinline__ void tile_march(float *__restrict__ grid3, const FdtdParams &p, int t, int tile, int xch,
+                                           float *__restrict__ lds)
+{
+    constexpr int LY = TY + 2, LZ = TZ + 2, LN = LY * LZ, NT = TY * TZ;
+    const int N = p.N, M = N - 4;
+    const size_t nn = (size_t)N * N * N;
+    const float *__restrict__ g = level(grid3, t, 0, nn);
+    const float *__restrict__ gm = level(grid3, t, -1, nn);
+    float *__restrict__ gp = level(grid3, t, 1, nn);
+    const int ntz = (M + TZ - 1) / TZ, nty = (M + TY - 1) / TY;
+    const int tz = tile % ntz, r = tile / ntz;
+    const int ty = r % nty, xc = r / nty;
+    const int y0 = 2 + ty * TY, z0 = 2 + tz * TZ;
+    const int x0 = 2 + xc * xch, x1 = min(x0 + xch, N - 2);
+    const int tid = threadIdx.x;
+    const int ly = tid / TZ, lz = tid % TZ;
+    const int y = y0 + ly, z = z0 + lz;
+    const bool on = y <= N - 3 && z <= N - 3;
+    const int me_l = (ly + 1) * LZ + lz + 1;
+    // stage plane x (tile plus halo, clamped into the grid) into buf
+    auto stage = [&](int x, float *buf) {
+        const float *pl = g + (size_t)x * N * N;
+        for (int i = tid; i < LN; i += NT)
+        {
+            const int gy = min(max(y0 - 1 + i / LZ, 0), N - 1), gz = min(max(z0 - 1 + i % LZ, 0), N - 1);
+            buf[i] = pl[gy * N + gz];
+        }
+    };
+    float *cur = lds, *nxt = lds + LN;
+    stage(x0, cur);
+    const int yy = on ? y : 2, zz = on ? z : 2;
+    float gxm = g[lin3(x0 - 1, yy, zz, N)];
+    __syncthreads();
+    float gc = cur[me_l];
+    for (int x = x0; x < x1; x++)
+    {
+        stage(x + 1, nxt);  // x + 1 <= N - 2: inside the grid
+        const int q = lin3(x, yy, zz, N);
+        const float gmv = gm[q];
+        __syncthreads();
+        const float gxp = nxt[me_l];
+        const float v = leapfrog(gxm, gxp, cur[me_l - LZ], cur[me_l + LZ], cur[me_l - 1], cur[me_l + 1], gc, gmv, p);
+        if (on)
+            gp[q] = v;
+        gxm = gc;
+        gc = gxp;
+        __syncthreads();  // every thread is done with cur before it is staged again
+        float *tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+    }
+}
+
+template <int TY, int TZ>
+__global__ __launch_bounds__(TY *TZ) void k_batch_tile(const FdtdScene *__restrict__ ds, BatchMap m)
+{
+    __shared__ float lds[2 * (TY + 2) * (TZ + 2)];
+    const int v = (int)(blockIdx.x & 7) * m.per + (int)(blockIdx.x >> 3);
+    if (v >= m.total)
+        return;
+    const int sc = m.scene_of[v];
+    const FdtdScene &d = ds[sc];
+    tile_march<TY, TZ>(d.grid3, d.p, d.st->t, v - m.first[sc], d.xch, lds);
+}
+
+// the ABC shell of every scene, plus the interior of the scenes stepped one thread per cell
+__global__ __launch_bounds__(BLK) void k_batch_fdtd(const FdtdScene *__restrict__ ds, BatchMap m)
+{
+    const int v = (int)(blockIdx.x & 7) * m.per + (int)(blockIdx.x >> 3);
+    if (v >= m.total)
+        return;
+    const int sc = m.scene_of[v];
+    const FdtdScene &d = ds[sc];
+    fdtd_block(d.grid3, d.p, d.st->t, v - m.first[sc], d.nb_inner);
 }
 
 // get_damp (fdtd.cu:162-169)
@@ -391,6 +481,62 @@ __global__ __launch_bounds__(BLK) void k_fdtd_pml(float *__restrict__ grid3, flo
 }
 
 }  // namespace
+
+// (TY, TZ) of the tile variants (FdtdVariant::Tile*)
+static void tile_shape(int variant, int &ty, int &tz)
+{
+    switch (variant)
+    {
+    case 1: ty = 8, tz = 64; break;
+    case 2: ty = 16, tz = 32; break;
+    case 3: ty = 4, tz = 64; break;
+    default: ty = 8, tz = 32; break;
+    }
+}
+
+int fdtd_shell_blocks(int N)
+{
+    const long long S = N - 2;
+    const long long zt = (S - 2 + 15) / 16;
+    const long long fxy = 2 * S * S + 2 * (S - 2) * S, fz = 2 * zt * zt * 256;
+    return (int)((fxy + BLK - 1) / BLK) + 8 * (int)((fz + 8LL * BLK - 1) / (8LL * BLK));
+}
+
+int fdtd_inner_blocks(int N)
+{
+    const long long M = N - 4;
+    return (int)((M * M * M + BLK - 1) / BLK);
+}
+
+int fdtd_tile_blocks(int N, int variant, int &xch)
+{
+    int ty, tz;
+    tile_shape(variant, ty, tz);
+    const int M = N - 4;
+    const long long cols = (long long)((M + ty - 1) / ty) * ((M + tz - 1) / tz);
+    // chunks of at least 8 planes, enough tiles that a scene alone spreads over the CUs
+    const long long nxc = std::max<long long>(1, std::min<long long>((M + 7) / 8, (512 + cols - 1) / cols));
+    xch = (int)((M + nxc - 1) / nxc);
+    return (int)(cols * ((M + xch - 1) / xch));
+}
+
+hipError_t launch_batch_fdtd(const FdtdScene *ds, const BatchMap &m, hipStream_t s)
+{
+    k_batch_fdtd<<<8 * m.per, BLK, 0, s>>>(ds, m);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_tile(int variant, const FdtdScene *ds, const BatchMap &m, hipStream_t s)
+{
+    switch (variant)
+    {
+    case 1: k_batch_tile<8, 64><<<8 * m.per, 8 * 64, 0, s>>>(ds, m); break;
+    case 2: k_batch_tile<16, 32><<<8 * m.per, 16 * 32, 0, s>>>(ds, m); break;
+    case 3: k_batch_tile<4, 64><<<8 * m.per, 4 * 64, 0, s>>>(ds, m); break;
+    default: k_batch_tile<8, 32><<<8 * m.per, 8 * 32, 0, s>>>(ds, m); break;
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_fdtd(float *grid3, float *pml3, const FdtdParams &p, const StepState *st, hipStream_t s)
 {

@@ -38,6 +38,27 @@ struct FdtdParams
 
 // ---------------------------------------------------------------- fdtd.hip
 hipError_t launch_fdtd(float *grid3, float *pml3, const FdtdParams &p, const StepState *st, hipStream_t s);
+// One scene of a batched FDTD step. nb_inner: blocks of its one-thread-per-cell interior in the
+// batched shell launch (0: the interior is marched by a tile launch); xch: x planes per tile.
+struct FdtdScene
+{
+    FdtdParams p;
+    float *grid3;
+    const StepState *st;
+    int nb_inner, xch;
+};
+// FDTD interior kernels a scene can use: 0 one thread per cell, 1..4 LDS-staged 2.5D tiles of
+// 8 x 64, 16 x 32, 4 x 64 and 8 x 32 (y, z) columns
+constexpr int kFdtdVariants = 5;
+int fdtd_shell_blocks(int N);
+int fdtd_inner_blocks(int N);
+// tiles of an N^3 grid's interior for a tile variant; sets the x planes per tile
+int fdtd_tile_blocks(int N, int variant, int &xch);
+struct BatchMap;
+// shell of every scene plus the one-thread-per-cell interiors (nb_inner > 0)
+hipError_t launch_batch_fdtd(const FdtdScene *ds, const BatchMap &m, hipStream_t s);
+// interiors of the scenes using tile variant `variant` (1..4)
+hipError_t launch_batch_tile(int variant, const FdtdScene *ds, const BatchMap &m, hipStream_t s);
 
 // ---------------------------------------------------------------- scan.hip
 // exclusive scan of n ints into out[0..n] (out[n] = total); tmp must hold scan_tmp_ints(n) ints
@@ -214,6 +235,12 @@ hipError_t launch_ghost_solve(const CellArgs &a, int n, const StepState *st, flo
                               const int *mcol, const float *mval, bool linear, float *b, float *x, double *work,
                               int *info, hipStream_t s);
 
+// ---------------------------------------------------------------- noalloc.hip
+// bytes of the per-device scratch array of the allocation-free entry point
+constexpr size_t kScratchBytes = 16u << 20;
+// device address of that array on the current device (looked up once per device)
+hipError_t scratch_ptr(void **out);
+
 // ---------------------------------------------------------------- step.hip
 struct StepArgs
 {
@@ -223,24 +250,72 @@ struct StepArgs
     long long stride;   // Neumann row stride (floats)
     int n_listen;
 };
+// Everything the step kernels of one scene read and write: its sizes, lists, weight caches,
+// histories and per-step buffers. Passed by value to the single-scene kernels; the batched
+// kernels read an array of them from device memory.
+struct SceneStep
+{
+    StepArgs a;
+    StepState *st;
+    float *grid3;
+    // C3 gathers (far field, near field): cells, their pair lists, wave maps
+    const int *c3_cells, *c3_start, *c3_wave, *c3_pairs;
+    int c3_waves;
+    // face gather (Dirichlet update)
+    int face_waves;
+    const int *face_start, *face_wave, *face_nbr, *face_slot;
+    const uint32_t *wf, *wb, *fw;  // 16 x half2 {S_k, D_k} per record
+    const float2 *sf, *sb, *fsc;   // scales
+    const float *interp;
+    const int *corner;
+    const TriGeom *geom;
+    const int *err;
+    // histories and per-step buffers
+    float *hn, *hd, *wn, *wd, *far;
+    const float *neumann;  // Neumann series (row t - t0), null: the ring slot is written elsewhere
+    // listener recording: out[(t - t0) * out_stride + l] for l < a.n_listen (null: none)
+    const int *lcell, *lc3;
+    float *out;
+    long long out_stride;
+};
 // history windows of the gathers: wn/wd[tri*16 + k] = ring value at t - back - k, with t the
 // device time index when the kernel runs (back = 0 before the advance of a step, 1 after)
-hipError_t launch_windows(int nt, int back, const StepState *st, const float *hn, const float *hd, float *wn,
-                          float *wd, hipStream_t s);
-// wave maps: map[2w] = first segment owned by gather wave w, map[2w+1] = its first pair
-hipError_t launch_far(const StepArgs &a, const StepState *st, const float *grid3, const int *c3_cells,
-                      const int *c3_start, const int *c3_wave, int c3_waves, const int *c3_pairs, const uint32_t *wf,
-                      const float2 *sf, float *hn, const float *hd, const float *wn, const float *wd, float *far,
-                      const float *neumann, hipStream_t s);
-hipError_t launch_dirichlet(const StepArgs &a, const StepState *st, const int *face_start, const int *face_wave,
-                            int face_waves, const int *face_nbr, const int *face_slot, const uint32_t *fw,
-                            const float2 *fsc, const float *hn, float *hd, const float *wn, const float *wd,
-                            const float *interp, const int *corner, const float *far, const TriGeom *geom,
-                            const int *err, hipStream_t s);
-hipError_t launch_near(const StepArgs &a, const StepState *st, float *grid3, const int *c3_cells,
-                       const int *c3_start, const int *c3_wave, int c3_waves, const int *c3_pairs, const uint32_t *wb,
-                       const float2 *sb, const float *hn, const float *hd, const float *wn, const float *wd,
-                       const float *far, const int *listen_cell, const int *listen_c3, float *out, hipStream_t s);
+hipError_t launch_windows(const SceneStep &d, int back, hipStream_t s);
+hipError_t launch_far(const SceneStep &d, hipStream_t s);
+hipError_t launch_dirichlet(const SceneStep &d, hipStream_t s);
+hipError_t launch_near(const SceneStep &d, hipStream_t s);
+hipError_t launch_listen_direct(const SceneStep &d, hipStream_t s);
+hipError_t launch_set_neumann(const SceneStep &d, hipStream_t s);
+// listener table on the device: cell[l] = linear index of xyz[l], c3[l] = c3_index[cell[l]]
+// (-1: not a corrected cell); at most 64 listeners, passed by value
+struct ListenTable
+{
+    int n;
+    int xyz[3 * 64];
+};
+hipError_t launch_listen_setup(const ListenTable &lt, int N, const int *c3_index, int *cell, int *c3,
+                               hipStream_t s);
+
+// Batched step kernels: one launch covers the blocks of every scene of a batch. scene_of[v] is
+// the scene of virtual block v, first[scene] its first virtual block; total virtual blocks,
+// per = ceil(total / 8) (the launch has 8 * per blocks, dealt to the XCDs in contiguous runs).
+struct BatchMap
+{
+    const int *scene_of;
+    const int *first;
+    int total, per;
+};
+enum class BatchKernel
+{
+    Windows,
+    Far,
+    Dirichlet,
+    Near
+};
+// blocks of one scene in a batched launch of kernel k
+int batch_blocks(const SceneStep &d, BatchKernel k);
+hipError_t launch_batch(BatchKernel k, const SceneStep *descs, const BatchMap &m, hipStream_t s);
+hipError_t launch_batch_advance(const SceneStep *descs, int n, hipStream_t s);
 // colour image of grid plane `index` normal to `axis` (0: x, 1: y, 2: z) at the current level
 // with the mesh's cut drawn over it: W x W RGBA in PNG row order, W = N * (1080 / N);
 // cover / count: W x W ints of scratch
@@ -249,9 +324,5 @@ hipError_t launch_slice_image(int N, float h, const float *mn3, int axis, int in
                               int *cover, int *count, unsigned char *rgba, hipStream_t s);
 // FFAT map accumulation at the current time level: map is 6 x 2w x 2w, 1 <= w <= N/2 - 1
 hipError_t launch_ffat(const StepState *st, const float *grid3, int N, int w, float *map, hipStream_t s);
-hipError_t launch_listen_direct(const StepArgs &a, const StepState *st, const float *grid3, const int *listen_cell,
-                                float *out, hipStream_t s);
-hipError_t launch_set_neumann(const StepArgs &a, const StepState *st, float *hn, const float *neumann,
-                              hipStream_t s);
 
 }  // namespace pppm_amd

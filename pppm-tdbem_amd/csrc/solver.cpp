@@ -19,61 +19,12 @@
 #include <string>
 #include <vector>
 
-#include "internal.hpp"
+#include "solver_state.hpp"
 
 using namespace pppm_amd;
 
 namespace
 {
-
-// owning device buffer; grows on demand, contents are not preserved by ensure()
-template <class T>
-struct DBuf
-{
-    T *p = nullptr;
-    size_t n = 0;
-    DBuf() = default;
-    DBuf(const DBuf &) = delete;
-    DBuf &operator=(const DBuf &) = delete;
-    ~DBuf() { release(); }
-    hipError_t ensure(size_t m)
-    {
-        if (m == 0)
-            m = 1;
-        if (p && m <= n)
-            return hipSuccess;
-        release();
-        hipError_t e = hipMalloc((void **)&p, m * sizeof(T));
-        if (e != hipSuccess)
-        {
-            p = nullptr;
-            n = 0;
-            return e;
-        }
-        n = m;
-        return hipSuccess;
-    }
-    void swap(DBuf &o)
-    {
-        std::swap(p, o.p);
-        std::swap(n, o.n);
-    }
-    void release()
-    {
-        if (p)
-            (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    void take(DBuf &o)
-    {
-        release();
-        p = o.p;
-        n = o.n;
-        o.p = nullptr;
-        o.n = 0;
-    }
-};
 
 struct LWH72
 {
@@ -84,137 +35,7 @@ struct LWH72
 };
 static_assert(sizeof(LWH72) == 72, "LayerWeightHalf layout");
 
-// device status block layout: error bits, live counts written by the update's scans, the
-// face-cache pool counter
-enum
-{
-    ST_ERR = 0,
-    ST_C3 = 1,      // marked (3x3x3) cells
-    ST_B4 = 2,      // non-empty base cells
-    ST_P4 = 3,      // face pairs
-    ST_GDIRTY = 4,  // grid-cache triangles to recompute
-    ST_FDIRTY = 5,  // face pairs to recompute
-    ST_POOL = 6,    // face-cache slots in use
-    ST_N = 8
-};
-
-
 }  // namespace
-
-struct pppm_solver
-{
-    int N = 0;
-    float h = 0, dt = 0, c = 0, rc = 0;
-    float minp[3] = {0, 0, 0};
-    int pml_width = 0;
-    unsigned flags = 0;
-    bool use_pml = false;
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    std::string err;
-    int t_host = -1;
-    Tdbem bem;
-
-    DBuf<float> grid, pmlbuf;
-    DBuf<StepState> state;
-
-    int nv = 0, nt = 0;
-    bool mesh_set = false;
-    // the last mesh update failed (out of grid, bad index, capacity): the lists and weights no
-    // longer describe the mesh, so the step refuses to run until an update succeeds
-    bool mesh_stale = false;
-    DBuf<float> verts;
-    DBuf<int> tris;
-    DBuf<TriGeom> geom;
-
-    // capacities (host-known) and derived launch sizes
-    int cap_c3 = 0, cap_b4items = 0, cap_p4 = 0;
-    int c3_waves = 0, face_waves = 0;
-
-    // [cell_cnt | base_cnt | cell_fill | mark], 4 x N^3, all zero between updates
-    DBuf<int> counters;
-    DBuf<int> tsums;  // per-tile counts of the dirty face pairs, zero between updates
-    bool counters_zero = false;
-    DBuf<int> cell_start, cell_items, cell_rank;
-    DBuf<int> scan_tmp;
-    DBuf<int> c3_cells, c3_index, c3_cnt, c3_start, c3_pairs, c3_wave;
-    DBuf<int> b4_cells, b4_cnt, b4_start, b4_tris;
-    DBuf<int> face_cnt, face_start, face_nbr, face_slot, face_owner, face_flag, dirty_list;
-    // face lists and slots of the previous update (swapped with the current ones per update):
-    // the lookup takes a pair's slot from there when the pair sits at the same place
-    DBuf<int> prev_face_start, prev_face_nbr, prev_face_slot;
-    bool faces_valid = false;  // face_start / face_nbr / face_slot hold a completed update
-    DBuf<int> face_wave, corner;
-    DBuf<int> gflag, glist;
-    DBuf<char> ftmp;
-    // device status block (ST_*): error bits, the live counts the update's scans write, the
-    // face-cache pool counter; the host reads it with one copy per update
-    DBuf<int> status;
-    bool err_clean = false;  // status[ST_ERR] known to be 0
-
-    DBuf<uint32_t> wf, wb;
-    DBuf<float2> sf, sb;
-    DBuf<float> gmeta, interp;
-
-    int pool_cap = 0, hcap = 0, pool_count = 0;
-    DBuf<uint32_t> fw;
-    DBuf<float2> fsc, fmeta;
-    DBuf<unsigned long long> hkeys;
-    DBuf<int> hvals;
-
-    DBuf<float> hn, hd, far, nstage;
-    DBuf<float> wn, wd;  // per-step history windows, 16 floats per triangle and field
-    DBuf<int> lcell, lc3;
-
-    // cell classification and fresh cells (GhostCellSolver cell_data / cell_data_old,
-    // ghost_cell.h:125-127), N^3 each
-    bool classified = false;
-    int ghost_count = 0;
-    DBuf<int> ctype, ctype_old, cghost, cnearest, ctyp2, cflag, cscan, fresh_list, fresh_err;
-    DBuf<float> cpoint, creflect, cdist, acc_stage;
-    DBuf<int> vf_cnt, vf_start, vf_fill, vf_items;
-    DBuf<float> vnormal;
-
-    // ghost-cell boundary solve (GhostCellSolver, ghost_cell.h:72-193): ghost list, stencil
-    // weights and order, 9-slot sparse rows, right-hand side and solution, the Neumann data of
-    // the current and the previous step, FP64 solver work vectors
-    bool ghost_ready = false, ghost_linear = false;
-    float ghost_threshold = 25.0f;
-    int gh_n = 0, gh_nt = 0, gh_iters = 0;
-    DBuf<int> gh_cell, gh_order, gh_mcol, gh_info, gh_err;
-    DBuf<float> gh_pw, gh_cond, gh_mval, gh_b, gh_x, gh_neu, gh_neu_old;
-    DBuf<double> gh_work;
-
-    // slice images: per-pixel mesh-line coverage and counts, the RGBA image
-    DBuf<int> img_cover, img_count;
-    DBuf<unsigned char> img_rgba;
-    // host copy of the status block, in pinned memory so that the per-update read is a direct
-    // DMA rather than a staged copy
-    struct PinnedStatus
-    {
-        int *p = nullptr;
-        ~PinnedStatus()
-        {
-            if (p)
-                (void)hipHostFree(p);
-        }
-    } st_pin;
-    int *st_h = nullptr;
-    long long last_grid_recomputes = 0, last_face_recomputes = 0;
-
-    // per-phase GPU timing with HIP events on the handle's stream (the reference's
-    // log_time / LOG_TIME per phase, src/core/macro.h:35-48)
-    bool timing = false;
-    struct Span
-    {
-        int phase;
-        hipEvent_t a, b;
-    };
-    std::vector<Span> spans;
-    std::vector<hipEvent_t> pool;
-    double phase_ms[8] = {0};
-    long long phase_n[8] = {0};
-};
 
 namespace
 {
@@ -676,38 +497,88 @@ int upload(pppm_solver *s, void *dst, const void *src, size_t bytes, int on_devi
     return PPPM_OK;
 }
 
-// one update_grid_and_face on the handle's stream; neumann is device memory (row t - t0)
-int enqueue_step(pppm_solver *s, const float *neumann, int t0, long long stride, int n_listen, float *out)
+// per-step work buffers of the gathers: the handle's own, or (pppm_run_noalloc) slices of the
+// per-device scratch array
+struct StepBufs
+{
+    float *wn, *wd, *far;
+    const int *lcell, *lc3;
+};
+
+StepBufs own_bufs(pppm_solver *s) { return StepBufs{s->wn.p, s->wd.p, s->far.p, s->lcell.p, s->lc3.p}; }
+
+// the step kernels' view of the handle (neumann: series rows t - t0 or null; out: listener
+// samples, n_listen per step, or null)
+SceneStep scene_step(const pppm_solver *s, const StepBufs &b, const float *neumann, int t0, long long stride,
+                     int n_listen, float *out)
+{
+    SceneStep d{};
+    d.a = step_args(s, t0, stride, n_listen);
+    d.st = s->state.p;
+    d.grid3 = s->grid.p;
+    d.c3_cells = s->c3_cells.p;
+    d.c3_start = s->c3_start.p;
+    d.c3_wave = s->c3_wave.p;
+    d.c3_pairs = s->c3_pairs.p;
+    d.c3_waves = s->c3_waves;
+    d.face_waves = s->face_waves;
+    d.face_start = s->face_start.p;
+    d.face_wave = s->face_wave.p;
+    d.face_nbr = s->face_nbr.p;
+    d.face_slot = s->face_slot.p;
+    d.wf = s->wf.p;
+    d.wb = s->wb.p;
+    d.fw = s->fw.p;
+    d.sf = s->sf.p;
+    d.sb = s->sb.p;
+    d.fsc = s->fsc.p;
+    d.interp = s->interp.p;
+    d.corner = s->corner.p;
+    d.geom = s->geom.p;
+    d.err = s->status.p + ST_ERR;
+    d.hn = s->hn.p;
+    d.hd = s->hd.p;
+    d.wn = b.wn;
+    d.wd = b.wd;
+    d.far = b.far;
+    d.neumann = neumann;
+    d.lcell = b.lcell;
+    d.lc3 = b.lc3;
+    d.out = out;
+    d.out_stride = n_listen;
+    return d;
+}
+
+// one update_grid_and_face on the handle's stream; neumann is device memory (row t - t0).
+// timed: phase spans when timing is on (events are created on first use, so the
+// allocation-free path passes false)
+int enqueue_step(pppm_solver *s, const StepBufs &b, const float *neumann, int t0, long long stride, int n_listen,
+                 float *out, bool timed = true)
 {
     const FdtdParams fp = fdtd_params(s);
-    int sp = span_begin(s, 0);
+    const SceneStep d = scene_step(s, b, neumann, t0, stride, n_listen, out);
+    int sp = timed ? span_begin(s, 0) : -1;
     CK(launch_fdtd(s->grid.p, s->pmlbuf.p, fp, s->state.p, s->stream));
     span_end(s, sp);
     if (s->mesh_set)
-        CK(launch_windows(s->nt, 0, s->state.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->stream));
+        CK(launch_windows(d, 0, s->stream));
     CK(launch_advance(s->state.p, s->stream));
-    const StepArgs a = step_args(s, t0, stride, n_listen);
     if (!s->mesh_set)
     {
         if (out)
-            CK(launch_listen_direct(a, s->state.p, s->grid.p, s->lcell.p, out, s->stream));
+            CK(launch_listen_direct(d, s->stream));
         return PPPM_OK;
     }
-    sp = span_begin(s, 1);
-    CK(launch_far(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_wave.p, s->c3_waves, s->c3_pairs.p,
-                  s->wf.p, s->sf.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->far.p, neumann, s->stream));
+    sp = timed ? span_begin(s, 1) : -1;
+    CK(launch_far(d, s->stream));
     span_end(s, sp);
-    sp = span_begin(s, 2);
-    CK(launch_dirichlet(a, s->state.p, s->face_start.p, s->face_wave.p, s->face_waves, s->face_nbr.p, s->face_slot.p,
-                        s->fw.p, s->fsc.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->interp.p, s->corner.p, s->far.p,
-                        s->geom.p, s->status.p + ST_ERR, s->stream));
+    sp = timed ? span_begin(s, 2) : -1;
+    CK(launch_dirichlet(d, s->stream));
     span_end(s, sp);
-    sp = span_begin(s, 3);
-    CK(launch_near(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_wave.p, s->c3_waves, s->c3_pairs.p,
-                   s->wb.p, s->sb.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->far.p, s->lcell.p, s->lc3.p, out,
-                   s->stream));
+    sp = timed ? span_begin(s, 3) : -1;
+    CK(launch_near(d, s->stream));
     span_end(s, sp);
-    if (s->spans.size() > 4096)
+    if (timed && s->spans.size() > 4096)
         return collect_spans(s);
     return PPPM_OK;
 }
@@ -786,14 +657,11 @@ int run_impl(pppm_solver *s, const float *verts_series, long long vstride, const
             span_end(s, sp);
         }
         if (!verts_series)
-            return enqueue_step(s, neumann_series, t0, stride, n_listen, out);
+            return enqueue_step(s, own_bufs(s), neumann_series, t0, stride, n_listen, out);
         // with moving lists the listener is read by its own launch after the near field
-        RC(enqueue_step(s, neumann_series, t0, stride, n_listen, nullptr));
+        RC(enqueue_step(s, own_bufs(s), neumann_series, t0, stride, n_listen, nullptr));
         if (out)
-        {
-            const StepArgs a = step_args(s, t0, stride, n_listen);
-            CK(launch_listen_direct(a, s->state.p, s->grid.p, s->lcell.p, out, s->stream));
-        }
+            CK(launch_listen_direct(scene_step(s, own_bufs(s), nullptr, t0, stride, n_listen, out), s->stream));
         return PPPM_OK;
     };
     if (verts_series)
@@ -876,6 +744,23 @@ int run_impl(pppm_solver *s, const float *verts_series, long long vstride, const
 }
 
 }  // namespace
+
+namespace pppm_amd
+{
+int solver_fail(pppm_solver *s, int code, const std::string &msg) { return fail(s, code, msg); }
+int solver_check_ready(pppm_solver *s, const char *what) { return check_ready(s, what); }
+int solver_set_listeners(pppm_solver *s, const int *listen_xyz, int n_listen)
+{
+    return set_listeners(s, listen_xyz, n_listen);
+}
+FdtdParams solver_fdtd_params(const pppm_solver *s) { return fdtd_params(s); }
+SceneStep solver_scene_step(const pppm_solver *s, const float *neumann, int t0, long long stride, int n_listen,
+                            float *out)
+{
+    return scene_step(s, StepBufs{s->wn.p, s->wd.p, s->far.p, s->lcell.p, s->lc3.p}, neumann, t0, stride, n_listen,
+                      out);
+}
+}  // namespace pppm_amd
 
 // ============================================================================ C ABI
 extern "C" {
@@ -1056,7 +941,7 @@ int pppm_step(pppm_solver *s, const float *neumann, int on_device)
         nd = s->nstage.p;
     }
     const int t_new = s->t_host + 1;
-    RC(enqueue_step(s, nd, t_new, 0, 0, nullptr));
+    RC(enqueue_step(s, own_bufs(s), nd, t_new, 0, 0, nullptr));
     s->t_host = t_new;
     return PPPM_OK;
 }
@@ -1067,6 +952,57 @@ int pppm_run(pppm_solver *s, const float *neumann_series, long long stride, int 
     if (check_handle(s))
         return PPPM_ERR_ARG;
     return run_impl(s, nullptr, 0, neumann_series, stride, steps, listen_xyz, n_listen, out_dev, use_graph);
+}
+
+int pppm_run_noalloc(pppm_solver *s, const float *neumann_series, long long stride, int steps, const int *listen_xyz,
+                     int n_listen, float *out_dev)
+{
+    if (check_handle(s))
+        return PPPM_ERR_ARG;
+    RC(check_ready(s, "run_noalloc"));
+    if (steps < 0 || !neumann_series || stride < s->nt || n_listen < 0 || n_listen > 64 ||
+        (n_listen > 0 && (!listen_xyz || !out_dev)))
+        return fail(s, PPPM_ERR_ARG, "run_noalloc: bad arguments (at most 64 listeners, stride >= nt)");
+    ListenTable lt{};
+    lt.n = n_listen;
+    for (int l = 0; l < n_listen; l++)
+    {
+        for (int d = 0; d < 3; d++)
+        {
+            const int v = listen_xyz[3 * l + d];
+            if (v < 0 || v >= s->N)
+                return fail(s, PPPM_ERR_ARG, "run_noalloc: listener outside the grid");
+            lt.xyz[3 * l + d] = v;
+        }
+    }
+    // scratch layout: listener cells and C3 indices (2 x 64 ints), the two history windows
+    // (16 floats per triangle each), the far field of the C3 cells (capacity-sized)
+    const size_t win = (size_t)s->nt * STEPS * sizeof(float);
+    const size_t off_wn = 512, off_wd = off_wn + ((win + 255) & ~(size_t)255);
+    const size_t off_far = off_wd + ((win + 255) & ~(size_t)255);
+    const size_t need = off_far + (size_t)s->cap_c3 * sizeof(float);
+    if (need > kScratchBytes)
+        return fail(s, PPPM_ERR_CAPACITY,
+                    "run_noalloc: the mesh needs " + std::to_string(need) + " bytes of scratch, the maximum is " +
+                        std::to_string(kScratchBytes) + " (use pppm_run)");
+    if (steps == 0)
+        return PPPM_OK;
+    void *base = nullptr;
+    CK(scratch_ptr(&base));
+    unsigned char *sb = (unsigned char *)base;
+    int *lcell = (int *)sb, *lc3 = (int *)sb + 64;
+    const StepBufs b{(float *)(sb + off_wn), (float *)(sb + off_wd), (float *)(sb + off_far), lcell, lc3};
+    // every scratch entry a kernel reads is written earlier in the same call: the listener
+    // table here, the windows by each step's window copy, the far field of the live C3 cells by
+    // each step's far-field gather (the Dirichlet update and the near field read only those)
+    if (n_listen > 0)
+        CK(launch_listen_setup(lt, s->N, s->c3_index.p, lcell, lc3, s->stream));
+    const int t0 = s->t_host + 1;
+    float *out = n_listen > 0 ? out_dev : nullptr;
+    for (int i = 0; i < steps; i++)
+        RC(enqueue_step(s, b, neumann_series, t0, stride, n_listen, out, false));
+    s->t_host += steps;
+    return PPPM_OK;
 }
 
 int pppm_run_moving(pppm_solver *s, const float *verts_series, long long vstride, const float *neumann_series,
@@ -1104,10 +1040,9 @@ int pppm_solve_far(pppm_solver *s)
     if (check_handle(s))
         return PPPM_ERR_ARG;
     RC(check_ready(s, "solve_far"));
-    const StepArgs a = step_args(s, s->t_host, 0, 0);
-    CK(launch_windows(s->nt, 1, s->state.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->stream));
-    CK(launch_far(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_wave.p, s->c3_waves, s->c3_pairs.p,
-                  s->wf.p, s->sf.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->far.p, nullptr, s->stream));
+    const SceneStep d = scene_step(s, own_bufs(s), nullptr, s->t_host, 0, 0, nullptr);
+    CK(launch_windows(d, 1, s->stream));
+    CK(launch_far(d, s->stream));
     return PPPM_OK;
 }
 
@@ -1124,8 +1059,7 @@ int pppm_set_neumann(pppm_solver *s, const float *neumann, int on_device)
         RC(upload(s, s->nstage.p, neumann, (size_t)s->nt * sizeof(float), 0));
         nd = s->nstage.p;
     }
-    const StepArgs a = step_args(s, s->t_host, 0, 0);
-    CK(launch_set_neumann(a, s->state.p, s->hn.p, nd, s->stream));
+    CK(launch_set_neumann(scene_step(s, own_bufs(s), nd, s->t_host, 0, 0, nullptr), s->stream));
     return PPPM_OK;
 }
 
@@ -1134,11 +1068,9 @@ int pppm_update_dirichlet(pppm_solver *s)
     if (check_handle(s))
         return PPPM_ERR_ARG;
     RC(check_ready(s, "update_dirichlet"));
-    const StepArgs a = step_args(s, s->t_host, 0, 0);
-    CK(launch_windows(s->nt, 1, s->state.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->stream));
-    CK(launch_dirichlet(a, s->state.p, s->face_start.p, s->face_wave.p, s->face_waves, s->face_nbr.p, s->face_slot.p,
-                        s->fw.p, s->fsc.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->interp.p, s->corner.p, s->far.p,
-                        s->geom.p, s->status.p + ST_ERR, s->stream));
+    const SceneStep d = scene_step(s, own_bufs(s), nullptr, s->t_host, 0, 0, nullptr);
+    CK(launch_windows(d, 1, s->stream));
+    CK(launch_dirichlet(d, s->stream));
     return PPPM_OK;
 }
 
@@ -1147,11 +1079,9 @@ int pppm_solve_near(pppm_solver *s)
     if (check_handle(s))
         return PPPM_ERR_ARG;
     RC(check_ready(s, "solve_near"));
-    const StepArgs a = step_args(s, s->t_host, 0, 0);
-    CK(launch_windows(s->nt, 1, s->state.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->stream));
-    CK(launch_near(a, s->state.p, s->grid.p, s->c3_cells.p, s->c3_start.p, s->c3_wave.p, s->c3_waves, s->c3_pairs.p,
-                   s->wb.p, s->sb.p, s->hn.p, s->hd.p, s->wn.p, s->wd.p, s->far.p, nullptr, nullptr, nullptr,
-                   s->stream));
+    const SceneStep d = scene_step(s, own_bufs(s), nullptr, s->t_host, 0, 0, nullptr);
+    CK(launch_windows(d, 1, s->stream));
+    CK(launch_near(d, s->stream));
     return PPPM_OK;
 }
 

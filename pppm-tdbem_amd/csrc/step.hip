@@ -93,111 +93,85 @@ __device__ inline float conv_pair(const uint32_t *__restrict__ w16, float2 sc, c
 // History windows for the gathers of step t: row k of triangle i holds the ring value at
 // t - 1 - k, k = 0..15, so a gather reads tap k >= 1 at row k - 1. back = 0 when launched
 // before t is advanced (the time step), 1 after it (the phase entry points).
-__global__ __launch_bounds__(256) void k_windows(int nt, int back, const StepState *__restrict__ st,
-                                                 const float *__restrict__ hn, const float *__restrict__ hd,
-                                                 float *__restrict__ wn, float *__restrict__ wd)
+__device__ __forceinline__ void windows_item(const SceneStep &d, int back, int i)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= nt * STEPS)
+    if (i >= d.a.nt * STEPS)
         return;
     const int tri = i / STEPS, k = i % STEPS;
-    const int slot = (st->t - back - k) & (RING - 1);
-    wn[i] = hn[tri * RING + slot];
-    wd[i] = hd[tri * RING + slot];
+    const int slot = (d.st->t - back - k) & (RING - 1);
+    d.wn[i] = d.hn[tri * RING + slot];
+    d.wd[i] = d.hd[tri * RING + slot];
 }
 
 // far(c) = p(c) - sum_pairs W^fdtd (*) (N, D). W^fdtd has tap 0 == 0 (pppm_cache.cu:111),
 // so taps 1..15 only (adding the zero tap leaves the sum bit-identical); the Neumann slot
 // of step t is therefore free to be written by the extra waves of this launch.
-__global__ __launch_bounds__(TB) void k_far(StepArgs a, const StepState *__restrict__ st,
-                                            const float *__restrict__ grid3, const int *__restrict__ c3_cells,
-                                            const int *__restrict__ c3_start, const int *__restrict__ c3_wave,
-                                            int gather_waves, const int *__restrict__ c3_pairs,
-                                            const uint32_t *__restrict__ wf, const float2 *__restrict__ sf,
-                                            float *__restrict__ hn, const float *__restrict__ hd,
-                                            const float *__restrict__ wn, const float *__restrict__ wd,
-                                            float *__restrict__ far, const float *__restrict__ neumann, int xper)
+__device__ __forceinline__ void far_wave(const SceneStep &d, int wave, int lane, float *__restrict__ vals)
 {
-    __shared__ float vals[WPB][WAVE];
-    const int t = st->t;
-    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int wave = xcd_wave(xper, wid);
-    if (wave < gather_waves)
+    const int t = d.st->t;
+    if (wave < d.c3_waves)
     {
-        const int2 w0 = reinterpret_cast<const int2 *>(c3_wave)[wave];
-        const int2 w1 = reinterpret_cast<const int2 *>(c3_wave)[wave + 1];
+        const int2 w0 = reinterpret_cast<const int2 *>(d.c3_wave)[wave];
+        const int2 w1 = reinterpret_cast<const int2 *>(d.c3_wave)[wave + 1];
         const int c0 = w0.x, q0 = w0.y, c1 = w1.x, q1 = w1.y;
         if (c0 >= c1)
             return;
-        const size_t nn = (size_t)a.N * a.N * a.N;
-        const float *g = grid3 + (size_t)(((t % 3) + 3) % 3) * nn;
+        const size_t nn = (size_t)d.a.N * d.a.N * d.a.N;
+        const float *g = d.grid3 + (size_t)(((t % 3) + 3) % 3) * nn;
         const int cs = c0 + lane;
         const bool leader = cs < c1;
         // every index of the wave's first round is loaded at once, then what they point to
-        const int m0 = leader ? c3_start[cs] : 0, m1 = leader ? c3_start[cs + 1] : 0;
-        const unsigned cell = leader ? (unsigned)c3_cells[cs] : ~0u;
-        int rec = q0 + lane < q1 ? c3_pairs[q0 + lane] : -1;
+        const int m0 = leader ? d.c3_start[cs] : 0, m1 = leader ? d.c3_start[cs + 1] : 0;
+        const unsigned cell = leader ? (unsigned)d.c3_cells[cs] : ~0u;
+        int rec = q0 + lane < q1 ? d.c3_pairs[q0 + lane] : -1;
         const float gv = cell < nn ? g[cell] : 0.0f;
         float acc = 0.0f;
         for (int base = q0; base < q1; base += WAVE)
         {
             const int p = base + lane;
             if (base != q0)
-                rec = p < q1 ? c3_pairs[p] : -1;
+                rec = p < q1 ? d.c3_pairs[p] : -1;
             float v = 0.0f;
             if (rec >= 0)
             {
                 const int tri = rec / GC;
-                v = conv_pair<1, false>(wf + (size_t)rec * STEPS, sf[rec], hn + tri * RING, hd + tri * RING,
-                                       wn + tri * STEPS, wd + tri * STEPS, t);
+                v = conv_pair<1, false>(d.wf + (size_t)rec * STEPS, d.sf[rec], d.hn + tri * RING, d.hd + tri * RING,
+                                       d.wn + tri * STEPS, d.wd + tri * STEPS, t);
             }
-            vals[wid][lane] = v;
+            vals[lane] = v;
             wave_sync();
             if (leader)
             {
                 const int e = min(m1, base + WAVE);
                 for (int q = max(m0, base); q < e; q++)
-                    acc += vals[wid][q - base];
+                    acc += vals[q - base];
             }
             wave_sync();
         }
         if (leader)
-            far[cs] = cell < nn ? gv - acc : 0.0f;
+            d.far[cs] = cell < nn ? gv - acc : 0.0f;
     }
     else
     {
         // Neumann input of step t into the ring (set_neumann_condition_kernel)
-        const int i = (wave - gather_waves) * WAVE + lane;
-        if (neumann && i < a.nt)
-            hn[i * RING + (t & (RING - 1))] = neumann[(size_t)(t - a.t0) * a.stride + i];
+        const int i = (wave - d.c3_waves) * WAVE + lane;
+        if (d.neumann && i < d.a.nt)
+            d.hn[i * RING + (t & (RING - 1))] = d.neumann[(size_t)(t - d.a.t0) * d.a.stride + i];
     }
 }
 
 // Face far field (8-corner interpolation, pppm.cu:116-144), face near field against the
 // 4x4x4 list with D[t] = 0 and its tap-0 factors (pppm.cu:146-170), then
 // D_c[t] = (far*rc + near) / (0.5*area - sum_j D_0(c, j)) (pppm.cu:172-193).
-__global__ __launch_bounds__(TB) void k_dirichlet(StepArgs a, const StepState *__restrict__ st,
-                                                  const int *__restrict__ face_start,
-                                                  const int *__restrict__ face_wave, int gather_waves,
-                                                  const int *__restrict__ face_nbr,
-                                                  const int *__restrict__ face_slot,
-                                                  const uint32_t *__restrict__ fw, const float2 *__restrict__ fsc,
-                                                  const float *__restrict__ hn, float *__restrict__ hd,
-                                                  const float *__restrict__ wn, const float *__restrict__ wd,
-                                                  const float *__restrict__ interp, const int *__restrict__ corner,
-                                                  const float *__restrict__ far, const TriGeom *__restrict__ geom,
-                                                  const int *__restrict__ err)
+__device__ __forceinline__ void dirichlet_wave(const SceneStep &d, int wave, int lane, float *__restrict__ vals,
+                                               float *__restrict__ facs)
 {
-    __shared__ float vals[WPB][WAVE];
-    __shared__ float facs[WPB][WAVE];
-    const int t = st->t;
-    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int wave = blockIdx.x * WPB + wid;
+    const int t = d.st->t;
     // after a flagged mesh update the face lists may be incomplete: leave D[t] untouched
-    if (wave >= gather_waves || *err != 0)
+    if (wave >= d.face_waves || *d.err != 0)
         return;
-    const int2 w0 = reinterpret_cast<const int2 *>(face_wave)[wave];
-    const int2 w1 = reinterpret_cast<const int2 *>(face_wave)[wave + 1];
+    const int2 w0 = reinterpret_cast<const int2 *>(d.face_wave)[wave];
+    const int2 w1 = reinterpret_cast<const int2 *>(d.face_wave)[wave + 1];
     const int c0 = w0.x, q0 = w0.y, c1 = w1.x, q1 = w1.y;
     if (c0 >= c1)
         return;
@@ -205,12 +179,12 @@ __global__ __launch_bounds__(TB) void k_dirichlet(StepArgs a, const StepState *_
     const bool leader = cs < c1;
     // every index of the wave's first round is loaded at once, then what they point to; the
     // leaders' face far field (8-corner interpolation) does not depend on the gather
-    const int m0 = leader ? face_start[cs] : 0, m1 = leader ? face_start[cs + 1] : 0;
+    const int m0 = leader ? d.face_start[cs] : 0, m1 = leader ? d.face_start[cs + 1] : 0;
     int j = -1, ws = 0;
     if (q0 + lane < q1)
     {
-        j = face_nbr[q0 + lane];
-        ws = face_slot[q0 + lane];
+        j = d.face_nbr[q0 + lane];
+        ws = d.face_slot[q0 + lane];
     }
     float ff = 0.0f, area = 0.0f;
     if (leader)
@@ -220,13 +194,13 @@ __global__ __launch_bounds__(TB) void k_dirichlet(StepArgs a, const StepState *_
 #pragma unroll
         for (int w = 0; w < 8; w++)
         {
-            ci[w] = corner[8 * cs + w];  // corners of a base cell are C3 cells (>= 0)
-            iw[w] = interp[8 * cs + w];
+            ci[w] = d.corner[8 * cs + w];  // corners of a base cell are C3 cells (>= 0)
+            iw[w] = d.interp[8 * cs + w];
         }
-        area = geom[cs].area;
+        area = d.geom[cs].area;
 #pragma unroll
         for (int w = 0; w < 8; w++)
-            ff += iw[w] * (ci[w] >= 0 ? far[ci[w]] : 0.0f);
+            ff += iw[w] * (ci[w] >= 0 ? d.far[ci[w]] : 0.0f);
     }
     float acc = 0.0f, fac = 0.0f;
     for (int base = q0; base < q1; base += WAVE)
@@ -237,92 +211,81 @@ __global__ __launch_bounds__(TB) void k_dirichlet(StepArgs a, const StepState *_
             j = -1;
             if (p < q1)
             {
-                j = face_nbr[p];
-                ws = face_slot[p];
+                j = d.face_nbr[p];
+                ws = d.face_slot[p];
             }
         }
         float v = 0.0f, f = 0.0f;
         if (j >= 0)
         {
-            const float2 sc = fsc[ws];
+            const float2 sc = d.fsc[ws];
             uint32_t tap0;
-            v = conv_pair<0, true>(fw + (size_t)ws * STEPS, sc, hn + j * RING, hd + j * RING, wn + j * STEPS,
-                                  wd + j * STEPS, t, &tap0);
+            v = conv_pair<0, true>(d.fw + (size_t)ws * STEPS, sc, d.hn + j * RING, d.hd + j * RING, d.wn + j * STEPS,
+                                  d.wd + j * STEPS, t, &tap0);
             f = f16_bits_to_f32(tap0 >> 16) * sc.y;  // face_factor = (float)D_0 * scale_D
         }
-        vals[wid][lane] = v;
-        facs[wid][lane] = f;
+        vals[lane] = v;
+        facs[lane] = f;
         wave_sync();
         if (leader)
         {
             const int e = min(m1, base + WAVE);
             for (int q = max(m0, base); q < e; q++)
             {
-                acc += vals[wid][q - base];
-                fac += facs[wid][q - base];
+                acc += vals[q - base];
+                fac += facs[q - base];
             }
         }
         wave_sync();
     }
     if (leader)
     {
-        const float num = ff * a.rc + acc;
+        const float num = ff * d.a.rc + acc;
         const double den = 0.5 * (double)area - (double)fac;
-        hd[cs * RING + (t & (RING - 1))] = (float)((double)num / den);
+        d.hd[cs * RING + (t & (RING - 1))] = (float)((double)num / den);
     }
 }
 
 // p(c) = far + 0.98 * sum W^bem (*) (N, D) over all 16 taps; records listener samples.
-__global__ __launch_bounds__(TB) void k_near(StepArgs a, const StepState *__restrict__ st, float *__restrict__ grid3,
-                                             const int *__restrict__ c3_cells, const int *__restrict__ c3_start,
-                                             const int *__restrict__ c3_wave, int gather_waves,
-                                             const int *__restrict__ c3_pairs, const uint32_t *__restrict__ wb,
-                                             const float2 *__restrict__ sb, const float *__restrict__ hn,
-                                             const float *__restrict__ hd, const float *__restrict__ wn,
-                                             const float *__restrict__ wd, const float *__restrict__ far,
-                                             const int *__restrict__ listen_cell, const int *__restrict__ listen_c3,
-                                             float *__restrict__ out, int xper)
+__device__ __forceinline__ void near_wave(const SceneStep &d, int wave, int lane, float *__restrict__ vals)
 {
-    __shared__ float vals[WPB][WAVE];
-    const int t = st->t;
-    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int wave = xcd_wave(xper, wid);
-    const size_t nn = (size_t)a.N * a.N * a.N;
-    float *g = grid3 + (size_t)(((t % 3) + 3) % 3) * nn;
-    if (wave < gather_waves)
+    const int t = d.st->t;
+    const size_t nn = (size_t)d.a.N * d.a.N * d.a.N;
+    float *g = d.grid3 + (size_t)(((t % 3) + 3) % 3) * nn;
+    if (wave < d.c3_waves)
     {
-        const int2 w0 = reinterpret_cast<const int2 *>(c3_wave)[wave];
-        const int2 w1 = reinterpret_cast<const int2 *>(c3_wave)[wave + 1];
+        const int2 w0 = reinterpret_cast<const int2 *>(d.c3_wave)[wave];
+        const int2 w1 = reinterpret_cast<const int2 *>(d.c3_wave)[wave + 1];
         const int c0 = w0.x, q0 = w0.y, c1 = w1.x, q1 = w1.y;
         if (c0 >= c1)
             return;
         const int cs = c0 + lane;
         const bool leader = cs < c1;
         // every index of the wave's first round is loaded at once, then what they point to
-        const int m0 = leader ? c3_start[cs] : 0, m1 = leader ? c3_start[cs + 1] : 0;
-        const unsigned cell = leader ? (unsigned)c3_cells[cs] : ~0u;
-        const float fv = leader ? far[cs] : 0.0f;
-        int rec = q0 + lane < q1 ? c3_pairs[q0 + lane] : -1;
+        const int m0 = leader ? d.c3_start[cs] : 0, m1 = leader ? d.c3_start[cs + 1] : 0;
+        const unsigned cell = leader ? (unsigned)d.c3_cells[cs] : ~0u;
+        const float fv = leader ? d.far[cs] : 0.0f;
+        int rec = q0 + lane < q1 ? d.c3_pairs[q0 + lane] : -1;
         float acc = 0.0f;
         for (int base = q0; base < q1; base += WAVE)
         {
             const int p = base + lane;
             if (base != q0)
-                rec = p < q1 ? c3_pairs[p] : -1;
+                rec = p < q1 ? d.c3_pairs[p] : -1;
             float v = 0.0f;
             if (rec >= 0)
             {
                 const int tri = rec / GC;
-                v = conv_pair<0, false>(wb + (size_t)rec * STEPS, sb[rec], hn + tri * RING, hd + tri * RING,
-                                       wn + tri * STEPS, wd + tri * STEPS, t);
+                v = conv_pair<0, false>(d.wb + (size_t)rec * STEPS, d.sb[rec], d.hn + tri * RING, d.hd + tri * RING,
+                                       d.wn + tri * STEPS, d.wd + tri * STEPS, t);
             }
-            vals[wid][lane] = v;
+            vals[lane] = v;
             wave_sync();
             if (leader)
             {
                 const int e = min(m1, base + WAVE);
                 for (int q = max(m0, base); q < e; q++)
-                    acc += vals[wid][q - base];
+                    acc += vals[q - base];
             }
             wave_sync();
         }
@@ -331,105 +294,239 @@ __global__ __launch_bounds__(TB) void k_near(StepArgs a, const StepState *__rest
             const float v = (float)((double)fv + (double)acc * 0.98);
             if (cell < nn)
                 g[cell] = v;
-            if (out)
-                for (int l = 0; l < a.n_listen; l++)
-                    if (listen_c3[l] == cs)
-                        out[(size_t)(t - a.t0) * a.n_listen + l] = v;
+            if (d.out)
+                for (int l = 0; l < d.a.n_listen; l++)
+                    if (d.lc3[l] == cs)
+                        d.out[(size_t)(t - d.a.t0) * d.out_stride + l] = v;
         }
     }
-    else if (out && wave == gather_waves)
+    else if (d.out && wave == d.c3_waves)
     {
         // listeners outside the corrected region read the grid directly (no wave writes them)
-        if (lane < a.n_listen && listen_c3[lane] < 0)
-            out[(size_t)(t - a.t0) * a.n_listen + lane] = g[listen_cell[lane]];
+        if (lane < d.a.n_listen && d.lc3[lane] < 0)
+            d.out[(size_t)(t - d.a.t0) * d.out_stride + lane] = g[d.lcell[lane]];
     }
 }
 
-__global__ void k_listen_direct(StepArgs a, const StepState *__restrict__ st, const float *__restrict__ grid3,
-                                const int *__restrict__ listen_cell, float *__restrict__ out)
+// ---------------------------------------------------------------- single-scene kernels
+__global__ __launch_bounds__(256) void k_windows(SceneStep d, int back)
 {
-    const int t = st->t;
-    const int l = threadIdx.x;
-    if (l >= a.n_listen)
-        return;
-    const size_t nn = (size_t)a.N * a.N * a.N;
-    const float *g = grid3 + (size_t)(((t % 3) + 3) % 3) * nn;
-    out[(size_t)(t - a.t0) * a.n_listen + l] = g[listen_cell[l]];
+    windows_item(d, back, blockIdx.x * 256 + threadIdx.x);
 }
 
-__global__ void k_set_neumann(StepArgs a, const StepState *__restrict__ st, float *__restrict__ hn,
-                              const float *__restrict__ neumann)
+__global__ __launch_bounds__(TB) void k_far(SceneStep d, int xper)
 {
-    const int t = st->t;
+    __shared__ float vals[WPB][WAVE];
+    const int wid = threadIdx.x >> 6;
+    far_wave(d, xcd_wave(xper, wid), threadIdx.x & 63, vals[wid]);
+}
+
+__global__ __launch_bounds__(TB) void k_dirichlet(SceneStep d)
+{
+    __shared__ float vals[WPB][WAVE];
+    __shared__ float facs[WPB][WAVE];
+    const int wid = threadIdx.x >> 6;
+    dirichlet_wave(d, blockIdx.x * WPB + wid, threadIdx.x & 63, vals[wid], facs[wid]);
+}
+
+__global__ __launch_bounds__(TB) void k_near(SceneStep d, int xper)
+{
+    __shared__ float vals[WPB][WAVE];
+    const int wid = threadIdx.x >> 6;
+    near_wave(d, xcd_wave(xper, wid), threadIdx.x & 63, vals[wid]);
+}
+
+__global__ void k_listen_direct(SceneStep d)
+{
+    const int t = d.st->t;
+    const int l = threadIdx.x;
+    if (l >= d.a.n_listen)
+        return;
+    const size_t nn = (size_t)d.a.N * d.a.N * d.a.N;
+    const float *g = d.grid3 + (size_t)(((t % 3) + 3) % 3) * nn;
+    d.out[(size_t)(t - d.a.t0) * d.out_stride + l] = g[d.lcell[l]];
+}
+
+__global__ void k_listen_setup(ListenTable lt, int N, const int *__restrict__ c3_index, int *__restrict__ cell,
+                               int *__restrict__ c3)
+{
+    const int l = threadIdx.x;
+    if (l >= lt.n)
+        return;
+    const int q = lin3(lt.xyz[3 * l], lt.xyz[3 * l + 1], lt.xyz[3 * l + 2], N);
+    cell[l] = q;
+    c3[l] = c3_index[q];
+}
+
+__global__ void k_set_neumann(SceneStep d)
+{
+    const int t = d.st->t;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < a.nt)
-        hn[i * RING + (t & (RING - 1))] = neumann[(size_t)(t - a.t0) * a.stride + i];
+    if (i < d.a.nt)
+        d.hn[i * RING + (t & (RING - 1))] = d.neumann[(size_t)(t - d.a.t0) * d.a.stride + i];
+}
+
+// ---------------------------------------------------------------- batched kernels
+// A batched launch covers the blocks of every scene: virtual block v belongs to scene
+// m.scene_of[v] as its local block v - m.first[scene]. Blocks go to the 8 XCDs in turn and
+// block b takes virtual block (b % 8) * m.per + b / 8, so each XCD works through one
+// contiguous run of scenes (their lists, windows and grids stay in that XCD's L2).
+__device__ __forceinline__ bool batch_block(const BatchMap &m, int &scene, int &local)
+{
+    const int v = (int)(blockIdx.x & 7) * m.per + (int)(blockIdx.x >> 3);
+    if (v >= m.total)
+        return false;
+    scene = m.scene_of[v];
+    local = v - m.first[scene];
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_batch_windows(const SceneStep *__restrict__ ds, BatchMap m)
+{
+    int sc, lb;
+    if (batch_block(m, sc, lb))
+        windows_item(ds[sc], 0, lb * 256 + threadIdx.x);
+}
+
+__global__ __launch_bounds__(TB) void k_batch_far(const SceneStep *__restrict__ ds, BatchMap m)
+{
+    __shared__ float vals[WPB][WAVE];
+    int sc, lb;
+    if (!batch_block(m, sc, lb))
+        return;
+    const int wid = threadIdx.x >> 6;
+    far_wave(ds[sc], lb * WPB + wid, threadIdx.x & 63, vals[wid]);
+}
+
+__global__ __launch_bounds__(TB) void k_batch_dirichlet(const SceneStep *__restrict__ ds, BatchMap m)
+{
+    __shared__ float vals[WPB][WAVE];
+    __shared__ float facs[WPB][WAVE];
+    int sc, lb;
+    if (!batch_block(m, sc, lb))
+        return;
+    const int wid = threadIdx.x >> 6;
+    dirichlet_wave(ds[sc], lb * WPB + wid, threadIdx.x & 63, vals[wid], facs[wid]);
+}
+
+__global__ __launch_bounds__(TB) void k_batch_near(const SceneStep *__restrict__ ds, BatchMap m)
+{
+    __shared__ float vals[WPB][WAVE];
+    int sc, lb;
+    if (!batch_block(m, sc, lb))
+        return;
+    const int wid = threadIdx.x >> 6;
+    near_wave(ds[sc], lb * WPB + wid, threadIdx.x & 63, vals[wid]);
+}
+
+__global__ void k_batch_advance(const SceneStep *__restrict__ ds, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        ds[i].st->t = ds[i].st->t + 1;
 }
 
 inline int blocks_for_waves(long long waves) { return waves > 0 ? (int)((waves + WPB - 1) / WPB) : 1; }
 
 // blocks per XCD of a gather launch over `waves` waves (the launch has 8 times as many)
 inline int xcd_per(long long waves) { return (blocks_for_waves(waves) + 7) / 8; }
+
+long long far_waves(const SceneStep &d) { return (long long)d.c3_waves + (d.neumann ? (d.a.nt + WAVE - 1) / WAVE : 0); }
+long long near_waves(const SceneStep &d) { return (long long)d.c3_waves + (d.out ? 1 : 0); }
 }  // namespace
 
-hipError_t launch_windows(int nt, int back, const StepState *st, const float *hn, const float *hd, float *wn,
-                          float *wd, hipStream_t s)
+int batch_blocks(const SceneStep &d, BatchKernel k)
 {
-    const long long n = (long long)nt * STEPS;
-    k_windows<<<n > 0 ? (int)((n + 255) / 256) : 1, 256, 0, s>>>(nt, back, st, hn, hd, wn, wd);
+    switch (k)
+    {
+    case BatchKernel::Windows:
+    {
+        const long long n = (long long)d.a.nt * STEPS;
+        return n > 0 ? (int)((n + 255) / 256) : 1;
+    }
+    case BatchKernel::Far:
+        return blocks_for_waves(far_waves(d));
+    case BatchKernel::Dirichlet:
+        return blocks_for_waves(d.face_waves);
+    case BatchKernel::Near:
+        return blocks_for_waves(near_waves(d));
+    }
+    return 1;
+}
+
+hipError_t launch_windows(const SceneStep &d, int back, hipStream_t s)
+{
+    const long long n = (long long)d.a.nt * STEPS;
+    k_windows<<<n > 0 ? (int)((n + 255) / 256) : 1, 256, 0, s>>>(d, back);
     return hipGetLastError();
 }
 
-hipError_t launch_far(const StepArgs &a, const StepState *st, const float *grid3, const int *c3_cells,
-                      const int *c3_start, const int *c3_wave, int c3_waves, const int *c3_pairs, const uint32_t *wf,
-                      const float2 *sf, float *hn, const float *hd, const float *wn, const float *wd, float *far,
-                      const float *neumann, hipStream_t s)
+hipError_t launch_far(const SceneStep &d, hipStream_t s)
 {
-    const long long waves = (long long)c3_waves + (neumann ? (a.nt + WAVE - 1) / WAVE : 0);
-    const int per = xcd_per(waves);
-    k_far<<<8 * per, TB, 0, s>>>(a, st, grid3, c3_cells, c3_start, c3_wave, c3_waves, c3_pairs, wf, sf, hn, hd, wn, wd,
-                                 far, neumann, per);
+    const int per = xcd_per(far_waves(d));
+    k_far<<<8 * per, TB, 0, s>>>(d, per);
     return hipGetLastError();
 }
 
-hipError_t launch_dirichlet(const StepArgs &a, const StepState *st, const int *face_start, const int *face_wave,
-                            int face_waves, const int *face_nbr, const int *face_slot, const uint32_t *fw,
-                            const float2 *fsc, const float *hn, float *hd, const float *wn, const float *wd,
-                            const float *interp, const int *corner, const float *far, const TriGeom *geom,
-                            const int *err, hipStream_t s)
+hipError_t launch_dirichlet(const SceneStep &d, hipStream_t s)
 {
     // plain block order: the heavy face lists of finely meshed objects sit together in the wave
     // range, and round-robin spreads them over the XCDs (the contiguous mapping of the C3 gathers
     // cut this kernel's HBM traffic from 47.6 to 39.7 MB but raised its time from 19.1 to 21.6 us)
-    k_dirichlet<<<blocks_for_waves(face_waves), TB, 0, s>>>(a, st, face_start, face_wave, face_waves, face_nbr,
-                                                             face_slot, fw, fsc, hn, hd, wn, wd, interp, corner, far,
-                                                             geom, err);
+    k_dirichlet<<<blocks_for_waves(d.face_waves), TB, 0, s>>>(d);
     return hipGetLastError();
 }
 
-hipError_t launch_near(const StepArgs &a, const StepState *st, float *grid3, const int *c3_cells,
-                       const int *c3_start, const int *c3_wave, int c3_waves, const int *c3_pairs, const uint32_t *wb,
-                       const float2 *sb, const float *hn, const float *hd, const float *wn, const float *wd,
-                       const float *far, const int *listen_cell, const int *listen_c3, float *out, hipStream_t s)
+hipError_t launch_near(const SceneStep &d, hipStream_t s)
 {
-    const long long waves = (long long)c3_waves + (out ? 1 : 0);
-    const int per = xcd_per(waves);
-    k_near<<<8 * per, TB, 0, s>>>(a, st, grid3, c3_cells, c3_start, c3_wave, c3_waves, c3_pairs, wb, sb, hn, hd, wn,
-                                  wd, far, listen_cell, listen_c3, out, per);
+    const int per = xcd_per(near_waves(d));
+    k_near<<<8 * per, TB, 0, s>>>(d, per);
     return hipGetLastError();
 }
 
-hipError_t launch_listen_direct(const StepArgs &a, const StepState *st, const float *grid3, const int *listen_cell,
-                                float *out, hipStream_t s)
+hipError_t launch_listen_direct(const SceneStep &d, hipStream_t s)
 {
-    k_listen_direct<<<1, 64, 0, s>>>(a, st, grid3, listen_cell, out);
+    k_listen_direct<<<1, 64, 0, s>>>(d);
     return hipGetLastError();
 }
 
-hipError_t launch_set_neumann(const StepArgs &a, const StepState *st, float *hn, const float *neumann,
-                              hipStream_t s)
+hipError_t launch_listen_setup(const ListenTable &lt, int N, const int *c3_index, int *cell, int *c3,
+                               hipStream_t s)
 {
-    k_set_neumann<<<(a.nt + 255) / 256 > 0 ? (a.nt + 255) / 256 : 1, 256, 0, s>>>(a, st, hn, neumann);
+    k_listen_setup<<<1, 64, 0, s>>>(lt, N, c3_index, cell, c3);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_neumann(const SceneStep &d, hipStream_t s)
+{
+    k_set_neumann<<<(d.a.nt + 255) / 256 > 0 ? (d.a.nt + 255) / 256 : 1, 256, 0, s>>>(d);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch(BatchKernel k, const SceneStep *descs, const BatchMap &m, hipStream_t s)
+{
+    const int grid = 8 * m.per;
+    switch (k)
+    {
+    case BatchKernel::Windows:
+        k_batch_windows<<<grid, 256, 0, s>>>(descs, m);
+        break;
+    case BatchKernel::Far:
+        k_batch_far<<<grid, TB, 0, s>>>(descs, m);
+        break;
+    case BatchKernel::Dirichlet:
+        k_batch_dirichlet<<<grid, TB, 0, s>>>(descs, m);
+        break;
+    case BatchKernel::Near:
+        k_batch_near<<<grid, TB, 0, s>>>(descs, m);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_advance(const SceneStep *descs, int n, hipStream_t s)
+{
+    k_batch_advance<<<(n + 255) / 256, 256, 0, s>>>(descs, n);
     return hipGetLastError();
 }
 

@@ -23,7 +23,8 @@ PPPM_FLAG_HONOR_PML = 1
 
 EXPORTS = [
     "pppm_create", "pppm_destroy", "pppm_set_stream", "pppm_last_error", "pppm_set_mesh",
-    "pppm_update_mesh", "pppm_step", "pppm_update_mesh_and_step", "pppm_run", "pppm_fdtd_step", "pppm_solve_far",
+    "pppm_update_mesh", "pppm_step", "pppm_update_mesh_and_step", "pppm_run", "pppm_run_noalloc", "pppm_fdtd_step",
+    "pppm_solve_far",
     "pppm_set_neumann", "pppm_update_dirichlet", "pppm_solve_near", "pppm_reset",
     "pppm_time_idx", "pppm_grid_ptr", "pppm_read_cells", "pppm_history_ptrs", "pppm_triangles",
     "pppm_stats", "pppm_get_grid_weights", "pppm_set_grid_weights", "pppm_get_face_weights",
@@ -34,6 +35,8 @@ EXPORTS = [
     "pppm_ghost_get", "pppm_slice_image",
     "pppm_modal_create", "pppm_modal_destroy", "pppm_modal_run", "pppm_modal_state", "pppm_modal_set_state",
     "pppm_modal_surface_matrix", "pppm_modal_coeffs",
+    "pppm_batch_create", "pppm_batch_destroy", "pppm_batch_last_error", "pppm_batch_set_stream", "pppm_batch_run",
+    "pppm_batch_autotune", "pppm_batch_variants", "pppm_batch_set_variants",
 ]
 
 # LayerWeightHalf record (src/algorithm/bem.h:163-169): 16 x fp16, f32, 16 x fp16, f32
@@ -67,6 +70,7 @@ def load_library(path: str = LIB_PATH):
         "pppm_step": ([vp, vp, i], i),
         "pppm_update_mesh_and_step": ([vp, vp, vp, i], i),
         "pppm_run": ([vp, vp, ll, i, vp, i, vp, i], i),
+        "pppm_run_noalloc": ([vp, vp, ll, i, vp, i, vp], i),
         "pppm_fdtd_step": ([vp], i),
         "pppm_solve_far": ([vp], i),
         "pppm_set_neumann": ([vp, vp, i], i),
@@ -109,6 +113,14 @@ def load_library(path: str = LIB_PATH):
         "pppm_modal_set_state": ([vp, vp], i),
         "pppm_modal_surface_matrix": ([vp, vp], i),
         "pppm_modal_coeffs": ([vp, vp], i),
+        "pppm_batch_create": ([P(vp), vp, i], i),
+        "pppm_batch_destroy": ([vp], i),
+        "pppm_batch_last_error": ([vp], ctypes.c_char_p),
+        "pppm_batch_set_stream": ([vp, vp], i),
+        "pppm_batch_run": ([vp, vp, vp, i, vp, vp, i], i),
+        "pppm_batch_autotune": ([vp, i, vp], i),
+        "pppm_batch_variants": ([vp, vp], i),
+        "pppm_batch_set_variants": ([vp, vp], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -277,6 +289,19 @@ class PPPMSolver:
         n = _arg(neumann_series_dev, "neumann_series", need, "f32", "device")
         self._check(self.lib.pppm_run(self.h, n.ptr, stride, int(steps), lx.ctypes.data_as(ctypes.c_void_p),
                                       lx.shape[0], out.ptr if out else None, int(use_graph)), "run")
+
+    def run_noalloc(self, neumann_series_dev, steps, listeners, out_dev, stride=None):
+        """run() without a graph and without device allocation (pppm_run_noalloc): the gathers'
+        per-step buffers come from the library's per-device scratch array. Asynchronous on the
+        solver's stream."""
+        stride = self.nt if stride is None else int(stride)
+        if stride < self.nt or steps < 0:
+            raise PPPMError("run_noalloc: stride must be >= the triangle count and steps >= 0")
+        lx, out = self._listeners(listeners, out_dev, steps)
+        need = (int(steps) - 1) * stride + self.nt if steps > 0 else 0
+        n = _arg(neumann_series_dev, "neumann_series", need, "f32", "device")
+        self._check(self.lib.pppm_run_noalloc(self.h, n.ptr, stride, int(steps), lx.ctypes.data_as(ctypes.c_void_p),
+                                              lx.shape[0], out.ptr if out else None), "run_noalloc")
 
     def run_moving(self, verts_series_dev, neumann_series_dev, steps, listeners, out_dev, use_graph=True):
         """Moving mesh: before step i the vertices become verts_series_dev[i] (device, steps x nv x 3)."""
@@ -526,6 +551,77 @@ class PPPMSolver:
                                                                   "reflect", "dist")]
         self._check(self.lib.pppm_get_cells(self.h, *ptrs), "get_cells")
         return out
+
+
+class PPPMBatch:
+    """Independent scenes stepped together (pppm_batch_*): every kernel of the time step is one
+    launch over all scenes. `scenes` are PPPMSolver handles with their meshes set; the batch
+    keeps references to them."""
+
+    SIZE_CLASSES = (0, 40, 56, 72, 88, 112, 160)
+    VARIANTS = ("cell", "tile8x64", "tile16x32", "tile4x64", "tile8x32")
+
+    def __init__(self, scenes):
+        self.lib = load_library()
+        self.scenes = list(scenes)
+        arr = (ctypes.c_void_p * len(self.scenes))(*[s.h.value for s in self.scenes])
+        self.h = ctypes.c_void_p()
+        rc = self.lib.pppm_batch_create(ctypes.byref(self.h), arr, len(self.scenes))
+        if rc != 0:
+            raise PPPMError(f"pppm_batch_create failed ({rc})")
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.pppm_batch_last_error(self.h)
+            raise PPPMError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.lib.pppm_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def run(self, neumann_series, steps, listeners, out_dev, use_graph=True, strides=None):
+        """neumann_series: one device tensor per scene (steps x stride_i); listeners: one (x, y, z)
+        per scene; out_dev: device tensor of steps x n_scenes floats."""
+        n = len(self.scenes)
+        if len(neumann_series) != n or len(listeners) != n:
+            raise PPPMError("one Neumann series and one listener per scene")
+        strides = [s.nt for s in self.scenes] if strides is None else [int(x) for x in strides]
+        args = []
+        for i, (sc, ns, st) in enumerate(zip(self.scenes, neumann_series, strides)):
+            if st < sc.nt:
+                raise PPPMError(f"scene {i}: stride < triangle count")
+            args.append(_arg(ns, f"neumann_series[{i}]", (int(steps) - 1) * st + sc.nt if steps > 0 else 0,
+                             "f32", "device"))
+        lx = np.ascontiguousarray(np.asarray(listeners, dtype=np.int32).reshape(n, 3))
+        for i, sc in enumerate(self.scenes):
+            if lx[i].min() < 0 or lx[i].max() >= sc.res():
+                raise PPPMError(f"scene {i}: listener outside the grid")
+        out = _arg(out_dev, "out", int(steps) * n, "f32", "device")
+        ptrs = (ctypes.c_void_p * n)(*[a.ptr.value for a in args])
+        strd = (ctypes.c_longlong * n)(*strides)
+        self._check(self.lib.pppm_batch_run(self.h, ptrs, strd, int(steps), lx.ctypes.data_as(ctypes.c_void_p),
+                                            out.ptr, int(use_graph)), "batch_run")
+
+    def autotune(self, reps=20):
+        """Picks the fastest FDTD interior variant per size class; returns {class_lo: {variant: ms}}."""
+        ms = np.full(len(self.SIZE_CLASSES) * len(self.VARIANTS), np.nan, np.float32)
+        self._check(self.lib.pppm_batch_autotune(self.h, int(reps), ms.ctypes.data_as(ctypes.c_void_p)), "autotune")
+        ms = ms.reshape(len(self.SIZE_CLASSES), len(self.VARIANTS))
+        return {lo: {v: float(ms[c, k]) for k, v in enumerate(self.VARIANTS) if np.isfinite(ms[c, k])}
+                for c, lo in enumerate(self.SIZE_CLASSES) if np.isfinite(ms[c]).any()}
+
+    def variants(self):
+        out = np.zeros(len(self.scenes), np.int32)
+        self._check(self.lib.pppm_batch_variants(self.h, out.ctypes.data_as(ctypes.c_void_p)), "variants")
+        return out
+
+    def set_variants(self, v):
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.int32), (len(self.scenes),)))
+        self._check(self.lib.pppm_batch_set_variants(self.h, a.ctypes.data_as(ctypes.c_void_p)), "set_variants")
 
 
 def write_png(path, rgba):
