@@ -133,6 +133,30 @@ int pppm_batch_autotune(pppm_batch *b, int reps, float *ms_out);
 int pppm_batch_variants(pppm_batch *b, int *variants);
 int pppm_batch_set_variants(pppm_batch *b, const int *variants);
 
+/* Adjoint: gradients of a listener loss with respect to the per-triangle Neumann inputs (no
+ * counterpart in the reference). The step (src/algorithm/pppm.h:137-145) is linear in the grid
+ * levels, the history rings and the inputs, so its adjoint — the transposed step run backwards,
+ * ABC shell included (src/algorithm/fdtd.cu:20-150, pppm.cu:8-204) — needs no forward state,
+ * only the cotangents of the listener samples. Static mesh: the mesh must not change between
+ * the forward run and the adjoint (PPPM_ERR_STATE otherwise); the PML is not differentiated.
+ * pppm_adjoint_reset: zero adjoint state; the next reverse step is the forward step t_last
+ *   (the last step of the forward run, whose time index was t_last after it).
+ * pppm_adjoint_run: reverses `steps` steps, t = T, T-1, ..., T-steps+1 (T the current reverse
+ *   index). gout_dev (device, steps x n_listen): row r = the cotangent of the listener samples
+ *   of forward step T-steps+1+r (the cells listen_xyz, host, n_listen <= 64); grad_dev (device,
+ *   steps x grad_stride, grad_stride >= nt): row r = the gradient with respect to the Neumann
+ *   input of step T-steps+1+r. Consecutive calls continue backwards, so a run can be reversed
+ *   segment by segment. use_graph != 0 replays 32 reverse steps per captured graph. Synchronous.
+ * pppm_state_bytes / pppm_save_state / pppm_load_state: a checkpoint of the forward state
+ *   (grid levels t and t-1 with the ghost layer, both history rings, t) in caller-owned device
+ *   memory; loading one restores the solver to that step (same grid and mesh only). */
+int pppm_adjoint_reset(pppm_solver *s, int t_last);
+int pppm_adjoint_run(pppm_solver *s, const float *gout_dev, int steps, const int *listen_xyz, int n_listen,
+                     float *grad_dev, long long grad_stride, int use_graph);
+int pppm_state_bytes(pppm_solver *s, long long *bytes);
+int pppm_save_state(pppm_solver *s, void *dst_dev);
+int pppm_load_state(pppm_solver *s, const void *src_dev);
+
 /* Phase entry points used by the reference's tests (src/algorithm/pppm.h:127-135,
  * src/algorithm/fdtd.h:73-87). */
 int pppm_fdtd_step(pppm_solver *s);                              /* FDTD::step */

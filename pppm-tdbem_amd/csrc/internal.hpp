@@ -235,6 +235,41 @@ hipError_t launch_ghost_solve(const CellArgs &a, int n, const StepState *st, flo
                               const int *mcol, const float *mval, bool linear, float *b, float *x, double *work,
                               int *info, hipStream_t s);
 
+// ---------------------------------------------------------------- adjoint.hip
+// reverse time index of the adjoint (the forward step being reversed), in device memory
+struct AdjState
+{
+    int t;
+    int pad;
+};
+// Everything one reverse step reads and writes: the forward's mesh data (static), the
+// transposed lists, the adjoint state, the listener cotangents in and the gradient out.
+struct AdjScene
+{
+    FdtdParams fp;
+    int N, nt, n_c3;
+    AdjState *ast;
+    float *lam3;   // adjoint grid levels (3 x N^3), slot = t mod 3 as the forward's
+    float4 *bb;    // per shell cell: A^{-T} of its cotangents (N^3, shell entries used)
+    float *ln, *ld;                  // adjoint history rings (nt x 32)
+    float *unear, *lfar, *v;         // per C3 cell, per C3 cell, per triangle
+    const int *c3_cells, *tc3;       // C3 cell -> grid index; (tri, o) -> C3 cell
+    const uint32_t *wf, *wb, *fw;    // the forward's weight records
+    const float2 *sf, *sb, *fsc;
+    const int *ft_start, *ft_c, *ft_slot;  // per triangle j: face pairs (c, j) and their slots
+    const int *ct_start, *ct_c;            // per C3 cell: triangles with it as a base-cell corner
+    const float *ct_w;                     // ... and rc x their interpolation weight
+    const double *den;                     // A_c / 2 - sum_j D_0(c, j)
+    const int *lcell, *lc3;
+    int n_listen;
+    const float *gout;   // listener cotangents, row t - t_first, n_listen per row (null: none)
+    float *grad;         // gradient of the Neumann inputs, row t - t_first
+    long long grad_stride;
+    int t_first;
+};
+// one reverse step (t = ast->t, then ast->t - 1)
+hipError_t enqueue_adjoint_step(const AdjScene &d, hipStream_t s);
+
 // ---------------------------------------------------------------- noalloc.hip
 // bytes of the per-device scratch array of the allocation-free entry point
 constexpr size_t kScratchBytes = 16u << 20;

@@ -477,6 +477,8 @@ int rebuild_sync(pppm_solver *s, bool first, const float *vsrc = nullptr)
 {
     const int rc = rebuild_attempts(s, first, vsrc);
     s->mesh_stale = rc != PPPM_OK;
+    if (rc == PPPM_OK)
+        s->mesh_version++;
     return rc;
 }
 
@@ -754,6 +756,7 @@ int solver_set_listeners(pppm_solver *s, const int *listen_xyz, int n_listen)
     return set_listeners(s, listen_xyz, n_listen);
 }
 FdtdParams solver_fdtd_params(const pppm_solver *s) { return fdtd_params(s); }
+int solver_read_status(pppm_solver *s) { return read_status(s); }
 SceneStep solver_scene_step(const pppm_solver *s, const float *neumann, int t0, long long stride, int n_listen,
                             float *out)
 {

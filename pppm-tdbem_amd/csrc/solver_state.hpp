@@ -185,6 +185,20 @@ struct pppm_solver
     int *st_h = nullptr;
     long long last_grid_recomputes = 0, last_face_recomputes = 0;
 
+    // mesh version: counts successful mesh updates (the adjoint's transposed lists belong to one)
+    long long mesh_version = 0;
+    // adjoint (adjoint.cpp): transposed lists of the mesh version adj_version, the adjoint state
+    // (grid levels, shell solutions, history rings, per-step buffers) and its reverse time index
+    long long adj_version = -1;
+    int adj_t = -1, adj_n_c3 = 0;
+    DBuf<int> adj_tc3, adj_ft_start, adj_ft_c, adj_ft_slot, adj_ct_start, adj_ct_c;
+    DBuf<float> adj_ct_w;
+    DBuf<double> adj_den;
+    DBuf<float> adj_lam3, adj_ln, adj_ld, adj_unear, adj_lfar, adj_v;
+    DBuf<float4> adj_bb;
+    DBuf<pppm_amd::AdjState> adj_state;
+    DBuf<unsigned char> state_trailer;
+
     // per-phase GPU timing with HIP events on the handle's stream (the reference's
     // log_time / LOG_TIME per phase, src/core/macro.h:35-48)
     bool timing = false;
@@ -206,6 +220,8 @@ namespace pppm_amd
 int solver_fail(pppm_solver *s, int code, const std::string &msg);
 int solver_check_ready(pppm_solver *s, const char *what);
 int solver_set_listeners(pppm_solver *s, const int *listen_xyz, int n_listen);
+// reads the status block (live counts, pool count) into s->st_h
+int solver_read_status(pppm_solver *s);
 FdtdParams solver_fdtd_params(const pppm_solver *s);
 // the step kernels' view of the handle with its own per-step buffers
 SceneStep solver_scene_step(const pppm_solver *s, const float *neumann, int t0, long long stride, int n_listen,

@@ -37,6 +37,7 @@ EXPORTS = [
     "pppm_modal_surface_matrix", "pppm_modal_coeffs",
     "pppm_batch_create", "pppm_batch_destroy", "pppm_batch_last_error", "pppm_batch_set_stream", "pppm_batch_run",
     "pppm_batch_autotune", "pppm_batch_variants", "pppm_batch_set_variants",
+    "pppm_adjoint_reset", "pppm_adjoint_run", "pppm_state_bytes", "pppm_save_state", "pppm_load_state",
 ]
 
 # LayerWeightHalf record (src/algorithm/bem.h:163-169): 16 x fp16, f32, 16 x fp16, f32
@@ -121,6 +122,11 @@ def load_library(path: str = LIB_PATH):
         "pppm_batch_autotune": ([vp, i, vp], i),
         "pppm_batch_variants": ([vp, vp], i),
         "pppm_batch_set_variants": ([vp, vp], i),
+        "pppm_adjoint_reset": ([vp, i], i),
+        "pppm_adjoint_run": ([vp, vp, i, vp, i, vp, ll, i], i),
+        "pppm_state_bytes": ([vp, P(ll)], i),
+        "pppm_save_state": ([vp, vp], i),
+        "pppm_load_state": ([vp, vp], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -130,7 +136,7 @@ def load_library(path: str = LIB_PATH):
     return lib
 
 
-_NP = {"f32": np.float32, "i32": np.int32}
+_NP = {"f32": np.float32, "i32": np.int32, "u8": np.uint8}
 
 
 class _Arg:
@@ -153,7 +159,7 @@ def _arg(a, name, count, kind="f32", where="any"):
         raise PPPMError(f"{name} is required")
     if hasattr(a, "data_ptr"):  # torch tensor
         import torch
-        want = torch.float32 if kind == "f32" else torch.int32
+        want = {"f32": torch.float32, "i32": torch.int32, "u8": torch.uint8}[kind]
         if a.dtype != want:
             raise PPPMError(f"{name} must be {want}, got {a.dtype}")
         if not a.is_contiguous():
@@ -313,6 +319,44 @@ class PPPMSolver:
         self._check(self.lib.pppm_run_moving(self.h, v.ptr, int(self.nv * 3), n.ptr, int(self.nt), int(steps),
                                              lx.ctypes.data_as(ctypes.c_void_p), lx.shape[0],
                                              out.ptr if out else None, int(use_graph)), "run_moving")
+
+    # ------------------------------------------------------------------ adjoint and checkpoints
+    def adjoint_reset(self, t_last):
+        """Zero adjoint state; the next reverse step is forward step t_last."""
+        self._check(self.lib.pppm_adjoint_reset(self.h, int(t_last)), "adjoint_reset")
+
+    def adjoint_run(self, gout_dev, steps, listeners, grad_dev, use_graph=True, grad_stride=None):
+        """Reverses `steps` steps: gout_dev (device, steps x n_listen, forward time order) ->
+        grad_dev (device, steps x grad_stride: the gradient of each step's Neumann input)."""
+        lx = np.ascontiguousarray(np.asarray(listeners, dtype=np.int32).reshape(-1, 3))
+        if lx.shape[0] and (lx.min() < 0 or lx.max() >= self._res):
+            raise PPPMError("listener outside the grid")
+        gs = self.nt if grad_stride is None else int(grad_stride)
+        if gs < self.nt or steps < 0:
+            raise PPPMError("adjoint_run: grad_stride >= triangle count, steps >= 0")
+        go = _arg(gout_dev, "gout", int(steps) * lx.shape[0], "f32", "device") if lx.shape[0] else None
+        gr = _arg(grad_dev, "grad", (int(steps) - 1) * gs + self.nt if steps > 0 else 0, "f32", "device")
+        self._check(self.lib.pppm_adjoint_run(self.h, go.ptr if go else None, int(steps),
+                                              lx.ctypes.data_as(ctypes.c_void_p), lx.shape[0], gr.ptr, gs,
+                                              int(use_graph)), "adjoint_run")
+
+    def state_bytes(self):
+        n = ctypes.c_longlong()
+        self._check(self.lib.pppm_state_bytes(self.h, ctypes.byref(n)), "state_bytes")
+        return n.value
+
+    def save_state(self, dst_dev):
+        """Checkpoint of the forward state into a device tensor of state_bytes() bytes."""
+        d = _arg(dst_dev, "state", 0, "u8", "device")
+        if dst_dev.numel() * dst_dev.element_size() < self.state_bytes():
+            raise PPPMError("state buffer too small")
+        self._check(self.lib.pppm_save_state(self.h, d.ptr), "save_state")
+
+    def load_state(self, src_dev):
+        d = _arg(src_dev, "state", 0, "u8", "device")
+        if src_dev.numel() * src_dev.element_size() < self.state_bytes():
+            raise PPPMError("state buffer too small")
+        self._check(self.lib.pppm_load_state(self.h, d.ptr), "load_state")
 
     def fdtd_step(self):
         self._check(self.lib.pppm_fdtd_step(self.h), "fdtd_step")
