@@ -121,7 +121,11 @@ int pppm_run_moving(pppm_solver *s, const float *verts_series, long long vstride
  *   7 x 5 floats) reports the milliseconds per repetition [class][variant] (untimed entries are
  *   left as they were). The FDTD launches it times do not advance t and rewrite the same values,
  *   so the scenes' state is unchanged.
- * pppm_batch_variants / pppm_batch_set_variants: the FDTD variant of each scene (n_scenes ints). */
+ * pppm_batch_variants / pppm_batch_set_variants: the FDTD variant of each scene (n_scenes ints).
+ * pppm_batch_set_timing / pppm_batch_get_timing: HIP-event timing of each phase of eagerly
+ *   launched batch steps (a timed run does not use the graph): 0 FDTD (shell and interiors),
+ *   1 history windows and time advance, 2 far field, 3 Dirichlet update, 4 near field;
+ *   milliseconds and launch counts accumulated since the last get (which resets them). */
 typedef struct pppm_batch pppm_batch;
 int pppm_batch_create(pppm_batch **out, pppm_solver *const *scenes, int n_scenes);
 int pppm_batch_destroy(pppm_batch *b);
@@ -132,6 +136,8 @@ int pppm_batch_run(pppm_batch *b, const float *const *neumann_series, const long
 int pppm_batch_autotune(pppm_batch *b, int reps, float *ms_out);
 int pppm_batch_variants(pppm_batch *b, int *variants);
 int pppm_batch_set_variants(pppm_batch *b, const int *variants);
+int pppm_batch_set_timing(pppm_batch *b, int enable);
+int pppm_batch_get_timing(pppm_batch *b, double *ms8, long long *count8);
 
 /* Adjoint: gradients of a listener loss with respect to the per-triangle Neumann inputs (no
  * counterpart in the reference). The step (src/algorithm/pppm.h:137-145) is linear in the grid

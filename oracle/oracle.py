@@ -75,6 +75,9 @@ def lib():
             "orc_classify": ([vp, i], i),
             "orc_get_cells": ([vp, vp, vp, vp, vp, vp, vp], i),
             "orc_set_cells": ([vp, vp, vp, vp, vp], i),
+            "orc_linear_forward": ([vp, vp, i, vp, i, vp], i),
+            "orc_linear_adjoint": ([vp, vp, i, vp, i, vp], i),
+            "orc_linear_fd": ([vp, vp, vp, i, vp, i, i, vp, vp, d, vp], i),
             "orc_fill_fresh": ([vp, vp, vp], i),
             "orc_set_solid_zero": ([vp], i),
             "orc_vertex_normals": ([vp, vp], i),
@@ -268,6 +271,40 @@ class Oracle:
                                                  ("type", "ghost_idx", "nearest", "point", "reflect", "dist"))),
                  "get_cells")
         return out
+
+    # the step as a linear map in FP64 (adjoint_oracle.cpp)
+    def _cells(self, listeners):
+        lx = np.asarray(listeners, dtype=np.int64).reshape(-1, 3)
+        return np.ascontiguousarray((lx[:, 0] * self.N + lx[:, 1]) * self.N + lx[:, 2], dtype=np.int32)
+
+    def linear_forward(self, g, listeners):
+        """FP64 forward from the zero state: g (T, nt) -> listener samples (T, n_listen)."""
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        lc = self._cells(listeners)
+        out = np.zeros((g.shape[0], len(lc)), np.float64)
+        self._ok(self.L.orc_linear_forward(self.s, _p(g), g.shape[0], _p(lc), len(lc), _p(out)), "linear_forward")
+        return out
+
+    def linear_adjoint(self, gout, listeners):
+        """FP64 adjoint: listener cotangents (T, n_listen) -> gradient of the inputs (T, nt)."""
+        gout = np.ascontiguousarray(gout, dtype=np.float64)
+        lc = self._cells(listeners)
+        grad = np.zeros((gout.shape[0], self.nt), np.float64)
+        self._ok(self.L.orc_linear_adjoint(self.s, _p(gout), gout.shape[0], _p(lc), len(lc), _p(grad)),
+                 "linear_adjoint")
+        return grad
+
+    def linear_fd(self, g, target, listeners, steps, tris, eps=1e-3):
+        """Central differences of sum (out - target)^2 at the entries (steps[e], tris[e])."""
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        target = np.ascontiguousarray(target, dtype=np.float64)
+        lc = self._cells(listeners)
+        st = np.ascontiguousarray(steps, dtype=np.int32)
+        tr = np.ascontiguousarray(tris, dtype=np.int32)
+        fd = np.zeros(len(st), np.float64)
+        self._ok(self.L.orc_linear_fd(self.s, _p(g), _p(target), g.shape[0], _p(lc), len(lc), len(st), _p(st),
+                                      _p(tr), float(eps), _p(fd)), "linear_fd")
+        return fd
 
     def set_cells(self, ctype, ctype_old, nearest, point3):
         """Test hook: cell types now / before, nearest triangles and points (N^3 entries)."""

@@ -18,6 +18,7 @@
 #include "pppm_oracle.h"
 
 #include "detmath.h"
+#include "linear_model.h"
 
 #include <algorithm>
 #include <cmath>
@@ -2349,6 +2350,79 @@ int orc_get_cells(const orc_scene *s, int *type, int *ghost_idx, int *nearest, f
     if (dist)
         std::memcpy(dist, s->cdist.data(), NN * sizeof(float));
     return 0;
+}
+
+// The scene's step as flat arrays (weights decoded as the step reads them: (float)h * scale)
+bool orc_linear_model(const orc_scene *s, LinearModel &m)
+{
+    if (!s->mesh_set || s->pml > 0)
+        return false;
+    const int N = s->N, nt = s->nt;
+    m.N = N;
+    m.nt = nt;
+    m.rc = s->rc;
+    const double c = s->c, dt = s->dt, h = s->h;
+    m.coef = c * c * dt * dt;
+    m.hh = h * h;
+    const float fc = s->c, fdt = s->dt, fh = s->h;
+    m.acoef = fc * fc * fdt * fdt / (fh * fh);
+    m.coef1 = 1 / (fdt * fdt);
+    m.coef2 = fc / (2 * fh * fdt);
+    m.coef3 = -(fc * fc) / (2 * fh * fh);
+    m.c3_cells = s->c3_cells;
+    m.c3_start = s->c3_start;
+    m.c3_pairs = s->c3_pairs;
+    const size_t R = (size_t)nt * 27;
+    m.wfS.resize(R * STEPS);
+    m.wfD.resize(R * STEPS);
+    m.wbS.resize(R * STEPS);
+    m.wbD.resize(R * STEPS);
+    for (int i = 0; i < nt; i++)
+        for (int o = 0; o < 27; o++)
+        {
+            const LWH &f = s->gcache[i].fdtd[o], &b = s->gcache[i].bem[o];
+            const size_t r = ((size_t)i * 27 + o) * STEPS;
+            for (int k = 0; k < STEPS; k++)
+            {
+                m.wfS[r + k] = h2f(f.s[k]) * f.ss;
+                m.wfD[r + k] = h2f(f.d[k]) * f.sd;
+                m.wbS[r + k] = h2f(b.s[k]) * b.ss;
+                m.wbD[r + k] = h2f(b.d[k]) * b.sd;
+            }
+        }
+    std::vector<int> c3_of(s->grid[0].size(), -1);
+    for (size_t q = 0; q < s->c3_cells.size(); q++)
+        c3_of[s->c3_cells[q]] = (int)q;
+    m.f_start.assign(nt + 1, 0);
+    m.f_nbr.clear();
+    m.fS.clear();
+    m.fD.clear();
+    m.den.assign(nt, 0.0);
+    m.corner.assign((size_t)nt * 8, -1);
+    m.interp.assign(s->interp.begin(), s->interp.end());
+    for (int ci = 0; ci < nt; ci++)
+    {
+        const int b = s->b4_of_tri[ci];
+        double fac = 0;
+        for (int q = s->b4_start[b]; q < s->b4_start[b + 1]; q++)
+        {
+            const int ji = s->b4_tris[q];
+            const LWH &w = s->fcache.at((long long)ci * nt + ji).w;
+            m.f_nbr.push_back(ji);
+            for (int k = 0; k < STEPS; k++)
+            {
+                m.fS.push_back(h2f(w.s[k]) * w.ss);
+                m.fD.push_back(h2f(w.d[k]) * w.sd);
+            }
+            fac += (double)(h2f(w.d[0]) * w.sd);
+        }
+        m.f_start[ci + 1] = (int)m.f_nbr.size();
+        m.den[ci] = 0.5 * (double)s->tris[ci].area - fac;
+        const i3 bc = s->tris[ci].base;
+        for (int w = 0; w < 8; w++)
+            m.corner[ci * 8 + w] = c3_of[s->lin(bc.x + (w >> 2), bc.y + ((w >> 1) & 1), bc.z + (w & 1))];
+    }
+    return true;
 }
 
 int orc_set_cells(orc_scene *s, const int *type, const int *type_old, const int *nearest, const float *point3)

@@ -98,6 +98,16 @@ int orc_vertex_normals(const orc_scene *s, float *out3);
 /* barycentric weights of the point of triangle tri9 closest to p3 (helper_math.h:1603-1800) */
 void orc_closest_coeff(const float *p3, const float *tri9, float *out3);
 
+/* The step as a linear map, in FP64 (adjoint_oracle.cpp), on the scene's current mesh and
+ * caches (ABC only): forward of T steps from the zero state with Neumann inputs g (T x nt),
+ * listener samples out (T x nl) at the grid cells lcell; the adjoint of that map
+ * (grad = d<gout, out>/dg); central differences of J = sum (out - target)^2 at n entries
+ * (step[e], tri[e]) with step eps, each J a full forward run. Return 0, or -1 without a mesh. */
+int orc_linear_forward(const orc_scene *s, const double *g, int T, const int *lcell, int nl, double *out);
+int orc_linear_adjoint(const orc_scene *s, const double *gout, int T, const int *lcell, int nl, double *grad);
+int orc_linear_fd(const orc_scene *s, const double *g, const double *target, int T, const int *lcell, int nl,
+                  int n_entries, const int *step, const int *tri, double eps, double *fd);
+
 /* Primitives (for known-answer and closed-form tests) */
 /* deterministic FP64 exp / sincos used by the FP64 quadrature (detmath.h) */
 double orc_det_exp(double x);

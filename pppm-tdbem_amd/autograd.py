@@ -1,0 +1,104 @@
+"""Differentiable listener signals of the PPPM step (torch.autograd.Function over the C ABI).
+
+The step (update_grid_and_face, src/algorithm/pppm.h:137-145) is linear in the grid levels, the
+history rings and the per-triangle Neumann inputs g (the surface accelerations), so the
+gradient of any loss of the listener samples is the adjoint (pppm_adjoint_run: the transposed
+step run backwards) applied to the samples' cotangents; it reads no forward state.
+
+ListenerSignal: out = F(g), the listener samples of a run from the zero state; backward is one
+adjoint sweep. Memory: the samples (T x n_listen).
+
+ListenerLoss: J = sum (F(g) - target)^2 with sqrt(T) checkpointing: the forward keeps
+ceil(T / S) solver states (S = ceil(sqrt T) steps apart) and not the samples; the backward walks
+the segments from the last, restores each segment's checkpoint, recomputes that segment's
+samples (and so its residuals) and reverses it with the adjoint, whose state carries from one
+segment to the one before. The recomputation costs one extra forward run; the gradient equals
+ListenerSignal's bit for bit.
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+
+from pppm_amd import PPPMError
+
+
+def _check_inputs(g, solver):
+    if not (isinstance(g, torch.Tensor) and g.is_cuda and g.dtype == torch.float32 and g.dim() == 2):
+        raise PPPMError("g must be a (steps, triangles) float32 device tensor")
+    if g.shape[1] != solver.nt:
+        raise PPPMError(f"g has {g.shape[1]} columns, the mesh {solver.nt} triangles")
+
+
+class ListenerSignal(torch.autograd.Function):
+    """out[t, l] = pressure at listeners[l] after step t, from the zero state (solver.reset())."""
+
+    @staticmethod
+    def forward(ctx, g, solver, listeners, use_graph=True):
+        _check_inputs(g, solver)
+        T = g.shape[0]
+        solver.reset()
+        out = torch.zeros(T, len(listeners), device=g.device, dtype=torch.float32)
+        solver.run(g.contiguous(), T, listeners, out, use_graph=use_graph)
+        ctx.solver, ctx.listeners, ctx.T, ctx.use_graph = solver, listeners, T, use_graph
+        return out
+
+    @staticmethod
+    def backward(ctx, gout):
+        s = ctx.solver
+        grad = torch.zeros(ctx.T, s.nt, device=gout.device, dtype=torch.float32)
+        s.adjoint_reset(ctx.T - 1)
+        s.adjoint_run(gout.contiguous().float(), ctx.T, ctx.listeners, grad, use_graph=ctx.use_graph)
+        return grad, None, None, None
+
+
+class ListenerLoss(torch.autograd.Function):
+    """J = sum_{t, l} (out[t, l] - target[t, l])^2 (FP64 accumulation, returned as float32) with
+    sqrt(T) checkpoints; `segment` overrides the checkpoint spacing."""
+
+    @staticmethod
+    def forward(ctx, g, target, solver, listeners, segment=None, use_graph=True):
+        _check_inputs(g, solver)
+        T, L = g.shape[0], len(listeners)
+        if tuple(target.shape) != (T, L):
+            raise PPPMError("target must be (steps, listeners)")
+        S = int(segment) if segment else max(1, math.ceil(math.sqrt(T)))
+        nseg = (T + S - 1) // S
+        ck = torch.empty(nseg, solver.state_bytes(), dtype=torch.uint8, device=g.device)
+        out = torch.zeros(S, L, device=g.device, dtype=torch.float32)
+        J = torch.zeros((), device=g.device, dtype=torch.float64)
+        solver.reset()
+        for k in range(nseg):
+            a, b = k * S, min(T, (k + 1) * S)
+            solver.save_state(ck[k])
+            solver.run(g[a:b], b - a, listeners, out[:b - a], use_graph=use_graph)
+            J += ((out[:b - a].double() - target[a:b].double()) ** 2).sum()
+        ctx.save_for_backward(g, target)
+        ctx.solver, ctx.listeners, ctx.S, ctx.nseg, ctx.ck, ctx.use_graph = solver, listeners, S, nseg, ck, use_graph
+        ctx.checkpoint_bytes = ck.numel()
+        return J.float()
+
+    @staticmethod
+    def backward(ctx, gJ):
+        g, target = ctx.saved_tensors
+        s, S, L = ctx.solver, ctx.S, len(ctx.listeners)
+        T = g.shape[0]
+        grad = torch.zeros(T, s.nt, device=g.device, dtype=torch.float32)
+        out = torch.zeros(S, L, device=g.device, dtype=torch.float32)
+        s.adjoint_reset(T - 1)
+        for k in reversed(range(ctx.nseg)):
+            a, b = k * S, min(T, (k + 1) * S)
+            s.load_state(ctx.ck[k])
+            s.run(g[a:b], b - a, ctx.listeners, out[:b - a], use_graph=ctx.use_graph)   # recompute
+            gout = (2.0 * gJ.float()) * (out[:b - a] - target[a:b].float())
+            s.adjoint_run(gout.contiguous(), b - a, ctx.listeners, grad[a:b], use_graph=ctx.use_graph)
+        return grad, None, None, None, None, None
+
+
+def listener_signal(g, solver, listeners, use_graph=True):
+    return ListenerSignal.apply(g, solver, listeners, use_graph)
+
+
+def listener_loss(g, target, solver, listeners, segment=None, use_graph=True):
+    return ListenerLoss.apply(g, target, solver, listeners, segment, use_graph)

@@ -137,7 +137,8 @@ int adjoint_setup(pppm_solver *s)
     std::vector<int> ct_start(n_c3 + 1, 0), ct_c((size_t)8 * nt);
     std::vector<float> ct_w((size_t)8 * nt);
     for (int i = 0; i < 8 * nt; i++)
-        ct_start[corner[i] + 1]++;
+        if (corner[i] >= 0 && corner[i] < n_c3)  // corners of a base cell are C3 cells
+            ct_start[corner[i] + 1]++;
     for (int x = 0; x < n_c3; x++)
         ct_start[x + 1] += ct_start[x];
     {
@@ -145,6 +146,8 @@ int adjoint_setup(pppm_solver *s)
         for (int c = 0; c < nt; c++)
             for (int w = 0; w < 8; w++)
             {
+                if (corner[8 * c + w] < 0 || corner[8 * c + w] >= n_c3)
+                    continue;
                 const int at = fill[corner[8 * c + w]]++;
                 ct_c[at] = c;
                 ct_w[at] = s->rc * interp[8 * c + w];

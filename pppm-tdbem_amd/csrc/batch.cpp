@@ -53,7 +53,13 @@ struct pppm_batch
     DBuf<SceneStep> descs;
     DBuf<FdtdScene> fdescs;
     Map m_fdtd, m_tile[kFdtdVariants], m_win, m_far, m_dir, m_near;
-    bool descs_ready = false;
+    // per-phase HIP event timing of eager batch steps: 0 FDTD (shell and interiors), 1 history
+    // windows and time advance, 2 far field, 3 Dirichlet update, 4 near field
+    bool timing = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> spans;
+    std::vector<hipEvent_t> pool;
+    double phase_ms[8] = {0};
+    long long phase_n[8] = {0};
 };
 
 namespace
@@ -147,15 +153,76 @@ int enqueue_fdtd(pppm_batch *b)
     return PPPM_OK;
 }
 
+hipEvent_t take_event(pppm_batch *b)
+{
+    if (!b->pool.empty())
+    {
+        hipEvent_t e = b->pool.back();
+        b->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    return hipEventCreate(&e) == hipSuccess ? e : nullptr;
+}
+
+int span_begin(pppm_batch *b, int phase)
+{
+    if (!b->timing)
+        return -1;
+    hipEvent_t a = take_event(b), c = take_event(b);
+    if (!a || !c || hipEventRecord(a, b->stream) != hipSuccess)
+        return -1;
+    b->spans.push_back({phase, {a, c}});
+    return (int)b->spans.size() - 1;
+}
+
+void span_end(pppm_batch *b, int i)
+{
+    if (i >= 0)
+        (void)hipEventRecord(b->spans[i].second.second, b->stream);
+}
+
+int collect_spans(pppm_batch *b)
+{
+    if (b->spans.empty())
+        return PPPM_OK;
+    BCK(hipStreamSynchronize(b->stream));
+    for (auto &sp : b->spans)
+    {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, sp.second.first, sp.second.second) == hipSuccess)
+        {
+            b->phase_ms[sp.first] += ms;
+            b->phase_n[sp.first]++;
+        }
+        b->pool.push_back(sp.second.first);
+        b->pool.push_back(sp.second.second);
+    }
+    b->spans.clear();
+    return PPPM_OK;
+}
+
 int enqueue_batch_step(pppm_batch *b)
 {
+    int sp = span_begin(b, 0);
     BRC(enqueue_fdtd(b));
+    span_end(b, sp);
     const int n = (int)b->sc.size();
+    sp = span_begin(b, 1);
     BCK(launch_batch(BatchKernel::Windows, b->descs.p, b->m_win.view(), b->stream));
     BCK(launch_batch_advance(b->descs.p, n, b->stream));
+    span_end(b, sp);
+    sp = span_begin(b, 2);
     BCK(launch_batch(BatchKernel::Far, b->descs.p, b->m_far.view(), b->stream));
+    span_end(b, sp);
+    sp = span_begin(b, 3);
     BCK(launch_batch(BatchKernel::Dirichlet, b->descs.p, b->m_dir.view(), b->stream));
+    span_end(b, sp);
+    sp = span_begin(b, 4);
     BCK(launch_batch(BatchKernel::Near, b->descs.p, b->m_near.view(), b->stream));
+    span_end(b, sp);
+    if (b->spans.size() > 4096)
+        return collect_spans(b);
     return PPPM_OK;
 }
 
@@ -207,6 +274,13 @@ int pppm_batch_destroy(pppm_batch *b)
         return PPPM_ERR_ARG;
     if (b->stream)
         (void)hipStreamSynchronize(b->stream);
+    for (auto &sp : b->spans)
+    {
+        (void)hipEventDestroy(sp.second.first);
+        (void)hipEventDestroy(sp.second.second);
+    }
+    for (hipEvent_t e : b->pool)
+        (void)hipEventDestroy(e);
     if (b->own_stream && b->stream)
         (void)hipStreamDestroy(b->stream);
     delete b;
@@ -270,7 +344,7 @@ int pppm_batch_run(pppm_batch *b, const float *const *neumann_series, const long
     BRC(build_fdtd(b));
 
     int done = 0;
-    if (use_graph)
+    if (use_graph && !b->timing)
     {
         const int chunk = std::min(steps, 32);
         hipGraph_t g = nullptr;
@@ -305,6 +379,32 @@ int pppm_batch_run(pppm_batch *b, const float *const *neumann_series, const long
     BCK(hipStreamSynchronize(b->stream));
     for (pppm_solver *s : b->sc)
         s->t_host += steps;
+    return PPPM_OK;
+}
+
+int pppm_batch_set_timing(pppm_batch *b, int enable)
+{
+    if (!b)
+        return PPPM_ERR_ARG;
+    BRC(collect_spans(b));
+    b->timing = enable != 0;
+    return PPPM_OK;
+}
+
+int pppm_batch_get_timing(pppm_batch *b, double *ms8, long long *count8)
+{
+    if (!b)
+        return PPPM_ERR_ARG;
+    BRC(collect_spans(b));
+    for (int i = 0; i < 8; i++)
+    {
+        if (ms8)
+            ms8[i] = b->phase_ms[i];
+        if (count8)
+            count8[i] = b->phase_n[i];
+        b->phase_ms[i] = 0;
+        b->phase_n[i] = 0;
+    }
     return PPPM_OK;
 }
 

@@ -36,7 +36,8 @@ EXPORTS = [
     "pppm_modal_create", "pppm_modal_destroy", "pppm_modal_run", "pppm_modal_state", "pppm_modal_set_state",
     "pppm_modal_surface_matrix", "pppm_modal_coeffs",
     "pppm_batch_create", "pppm_batch_destroy", "pppm_batch_last_error", "pppm_batch_set_stream", "pppm_batch_run",
-    "pppm_batch_autotune", "pppm_batch_variants", "pppm_batch_set_variants",
+    "pppm_batch_autotune", "pppm_batch_variants", "pppm_batch_set_variants", "pppm_batch_set_timing",
+    "pppm_batch_get_timing",
     "pppm_adjoint_reset", "pppm_adjoint_run", "pppm_state_bytes", "pppm_save_state", "pppm_load_state",
 ]
 
@@ -122,6 +123,8 @@ def load_library(path: str = LIB_PATH):
         "pppm_batch_autotune": ([vp, i, vp], i),
         "pppm_batch_variants": ([vp, vp], i),
         "pppm_batch_set_variants": ([vp, vp], i),
+        "pppm_batch_set_timing": ([vp, i], i),
+        "pppm_batch_get_timing": ([vp, vp, vp], i),
         "pppm_adjoint_reset": ([vp, i], i),
         "pppm_adjoint_run": ([vp, vp, i, vp, i, vp, ll, i], i),
         "pppm_state_bytes": ([vp, P(ll)], i),
@@ -657,6 +660,19 @@ class PPPMBatch:
         ms = ms.reshape(len(self.SIZE_CLASSES), len(self.VARIANTS))
         return {lo: {v: float(ms[c, k]) for k, v in enumerate(self.VARIANTS) if np.isfinite(ms[c, k])}
                 for c, lo in enumerate(self.SIZE_CLASSES) if np.isfinite(ms[c]).any()}
+
+    PHASES = ("fdtd", "windows", "far", "dirichlet", "near")
+
+    def set_timing(self, enable=True):
+        self._check(self.lib.pppm_batch_set_timing(self.h, int(enable)), "batch_set_timing")
+
+    def get_timing(self):
+        """{phase: (total_ms, launches)} of eager batch steps since the last call (HIP events)."""
+        ms = np.zeros(8, np.float64)
+        n = np.zeros(8, np.int64)
+        self._check(self.lib.pppm_batch_get_timing(self.h, ms.ctypes.data_as(ctypes.c_void_p),
+                                                   n.ctypes.data_as(ctypes.c_void_p)), "batch_get_timing")
+        return {name: (float(ms[i]), int(n[i])) for i, name in enumerate(self.PHASES)}
 
     def variants(self):
         out = np.zeros(len(self.scenes), np.int32)

@@ -244,6 +244,26 @@ def c3_scene(res: int, level: int, seed: int, steps: int = 2000, c: float = 343.
                  info={"signal": g})
 
 
+def c4_adjoint(steps: int = 2000, c: float = 343.0, res: int = 48, seed: int = SEED) -> Scene:
+    """The adjoint configuration: icosphere L3 (1280 triangles), radius 0.05 m, centred in a res^3
+    grid over twice its diameter, listener at relative (0.875, 0.875, 0.875). `neumann` holds the
+    inputs the gradient is taken at (seeded band-limited noise, 0-4 kHz, RMS 0.5); info["target"]
+    holds the seeded per-triangle accelerations (band-limited noise, unit RMS) whose listener
+    signal is the loss's target."""
+    rng = np.random.default_rng(seed + 4)
+    radius = 0.05
+    length = 2.0 * (2 * radius)
+    h = np.float32(length / res)
+    dt = np.float32(float(h) / (math.sqrt(3.0) * c))
+    V, F = icosphere(3, radius)
+    mn = (-length / 2, -length / 2, -length / 2)
+    target_acc = bandlimited_noise(rng, F.shape[0], steps, float(dt)).T.astype(np.float32)
+    g0 = (0.5 * bandlimited_noise(rng, F.shape[0], steps, float(dt))).T.astype(np.float32)
+    lis = tuple(int(0.875 * res) for _ in range(3))
+    return Scene("c4_adjoint", res, float(h), float(dt), mn, c, V, F, steps, lis, np.ascontiguousarray(g0),
+                 info={"radius": radius, "target_acc": np.ascontiguousarray(target_acc)})
+
+
 def rel_l2(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
