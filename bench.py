@@ -65,6 +65,23 @@ def cpu_baseline(sc, steps, threads):
     return sc.res ** 3 * steps / dt, dt
 
 
+def host_info():
+    """nproc and the CPU model of the host the baseline ran on."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count()
+    return {"nproc": nproc, "cpu_model": model}
+
+
 CALIB_MIB = 256
 
 
@@ -86,6 +103,7 @@ def measure_traffic(kernels, workload_args):
         cmd = ["rocprofv3", "--pmc", counter, "--kernel-include-regex", "|".join(kernels) + "|copyBuffer",
                "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable, os.path.abspath(__file__),
                "--steps", "20", "--warmup", "2", "--timing-steps", "2", "--no-cpu", "--no-traffic",
+               "--batch-steps", "0", "--adjoint-steps", "0",
                "--calib-copy-mib", str(CALIB_MIB)] + workload_args
         try:
             subprocess.run(cmd, env=env, cwd=ROOT, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
@@ -118,6 +136,161 @@ def measure_traffic(kernels, workload_args):
     return {"bytes": (f["kib_per_launch"] * f["scale"] + w["kib_per_launch"] * w["scale"]) * 1024.0,
             "fetch_kib": f["kib_per_launch"], "fetch_scale": f["scale"],
             "write_kib": w["kib_per_launch"], "write_scale": w["scale"]}
+
+
+def batch_bench(n_scenes=64, steps=200, warmup=20, timing_steps=20):
+    """Batched forward: the 64 uneven scenes (grid edges 24..96, icosphere levels 1..4) stepped
+    together by pppm_batch_run (one launch per kernel over every scene, 32 batch steps per graph
+    launch, FDTD interior variants picked by the autotuner), against the same scenes run one
+    after another through pppm_run (graph-replayed too). Returns the JSON object of the leg."""
+    import torch
+    import pppm_amd
+    import scenes
+
+    sizes = scenes.c3_batch_sizes(n_scenes)
+    total = warmup + 2 * steps + timing_steps
+    scs = scenes.c3_batch(n_scenes, steps=total)
+    t_set = time.perf_counter()
+    hs = []
+    for sc in scs:
+        g = pppm_amd.PPPMSolver(sc.res, sc.dl, sc.dt, sc.min_pos, 0, 0.83, sc.c)
+        g.set_mesh(sc.verts, sc.tris)
+        hs.append(g)
+    torch.cuda.synchronize()
+    t_set = time.perf_counter() - t_set
+    ns = [torch.from_numpy(sc.neumann).cuda() for sc in scs]
+    lis = [sc.listener for sc in scs]
+    cells = float(sum(sc.res ** 3 for sc in scs))
+    st = [g.stats() for g in hs]
+    b = pppm_amd.PPPMBatch(hs)
+    t_tune = time.perf_counter()
+    table = b.autotune(reps=5)
+    t_tune = time.perf_counter() - t_tune
+    var = b.variants()
+    out = torch.zeros(total, n_scenes, device="cuda")
+    i0 = 0
+
+    def rows(a, n):
+        return [x[a:a + n] for x in ns]
+
+    b.run(rows(i0, warmup), warmup, lis, out[i0:i0 + warmup])
+    i0 += warmup
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run(rows(i0, steps), steps, lis, out[i0:i0 + steps])
+    torch.cuda.synchronize()
+    el_b = time.perf_counter() - t0
+    i0 += steps
+    # per-phase HIP event timing over eager batch steps
+    b.set_timing(True)
+    b.run(rows(i0, timing_steps), timing_steps, lis, out[i0:i0 + timing_steps], use_graph=False)
+    b.set_timing(False)
+    ph = b.get_timing()
+    i0 += timing_steps
+    # the same scenes one after another (pppm_run, graph-replayed), same number of steps
+    o1 = torch.zeros(steps, 1, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for g, x, l in zip(hs, ns, lis):
+        g.run(x[i0:i0 + steps], steps, [l], o1)
+    torch.cuda.synchronize()
+    el_s = time.perf_counter() - t0
+
+    by = {"fdtd": 0.0, "windows": 0.0, "far": 0.0, "dirichlet": 0.0, "near": 0.0}
+    for sc, s_ in zip(scs, st):
+        a = algorithmic_bytes(sc.res, s_)
+        for k in ("fdtd", "far", "dirichlet", "near"):
+            by[k] += a[k]
+        by["windows"] += 256.0 * s_["T"]  # two 16-value ring windows read and written per triangle
+    avg = {k: (v[0] / v[1] if v[1] else None) for k, v in ph.items()}
+    step_kernels = ["fdtd", "far", "dirichlet", "near"]
+    dom = max(step_kernels, key=lambda k: avg[k] or 0.0)
+    ach = by[dom] / (avg[dom] * 1e-3) / 1e9
+    step_bytes = sum(by[k] for k in step_kernels)
+    kname = {"fdtd": "k_batch_fdtd + k_batch_tile", "far": "k_batch_far", "dirichlet": "k_batch_dirichlet",
+             "near": "k_batch_near"}[dom]
+    return {
+        "metric": "batched grid cell-updates/s",
+        "value": cells * steps / el_b,
+        "unit": "cell-updates/s",
+        "steps": steps,
+        "ms_per_batch_step": el_b * 1e3 / steps,
+        "sequential_value": cells * steps / el_s,
+        "sequential_ms_per_step_all_scenes": el_s * 1e3 / steps,
+        "speedup_vs_sequential": el_s / el_b,
+        "config": {"workload": f"c3_batch_{n_scenes}_scenes", "scenes": n_scenes, "grid_edges": [r for r, _ in sizes],
+                   "levels": [v for _, v in sizes], "cells_per_step": int(cells),
+                   "triangles": int(sum(s_["T"] for s_ in st)), "p3_pairs": int(sum(s_["P3"] for s_ in st)),
+                   "p4_pairs": int(sum(s_["P4"] for s_ in st)), "c3_cells": int(sum(s_["C3"] for s_ in st))},
+        "algorithmic_bytes_per_batch_step": step_bytes,
+        "hbm_gbps_effective_step": step_bytes / (el_b / steps) / 1e9,
+        "hbm_frac_effective_step": step_bytes / (el_b / steps) / 1e9 / HBM_PEAK_GBPS,
+        "roofline": {"kernel": kname, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBPS, "bytes_per_launch": by[dom], "avg_launch_us": avg[dom] * 1e3},
+        "phase_avg_us": {k: (v * 1e3 if v else None) for k, v in avg.items()},
+        "phase_gbps": {k: (by[k] / (avg[k] * 1e-3) / 1e9 if avg[k] else None) for k in by},
+        "autotune_ms": {str(k): v for k, v in table.items()},
+        "variants_used": {pppm_amd.PPPMBatch.VARIANTS[v]: int((var == v).sum()) for v in sorted(set(var.tolist()))},
+        "setup_s": {"set_mesh_all_scenes": t_set, "autotune": t_tune},
+    }
+
+
+def adjoint_bench(steps=2000):
+    """Adjoint: the 48^3 / icosphere L3 scene, `steps` steps. Times a forward run (pppm_run),
+    the adjoint sweep over the same steps (pppm_adjoint_run), and the sqrt(T)-checkpointed
+    listener loss with its backward (forward + recompute + adjoint). Returns the leg's object."""
+    import torch
+    import pppm_amd
+    import scenes
+    import autograd as pad
+
+    sc = scenes.c4_adjoint(steps=steps)
+    s = pppm_amd.PPPMSolver(sc.res, sc.dl, sc.dt, sc.min_pos, 0, 0.83, sc.c)
+    s.set_mesh(sc.verts, sc.tris)
+    g = torch.from_numpy(sc.neumann).cuda()
+    lis = [sc.listener]
+    out = torch.zeros(steps, 1, device="cuda")
+    gout = torch.randn(steps, 1, device="cuda")
+    grad = torch.zeros(steps, len(sc.tris), device="cuda")
+    # warm-up of both directions (set-up of the transposed lists happens in the first reset)
+    s.reset()
+    s.run(g[:64], 64, lis, out[:64])
+    s.adjoint_reset(63)
+    s.adjoint_run(gout[:64], 64, lis, grad[:64])
+    torch.cuda.synchronize()
+    s.reset()
+    t0 = time.perf_counter()
+    s.run(g, steps, lis, out)
+    torch.cuda.synchronize()
+    el_f = time.perf_counter() - t0
+    s.adjoint_reset(steps - 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.adjoint_run(gout, steps, lis, grad)
+    torch.cuda.synchronize()
+    el_a = time.perf_counter() - t0
+    target = torch.zeros(steps, 1, device="cuda")
+    gg = g.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J = pad.listener_loss(gg, target, s, lis)
+    J.backward()
+    torch.cuda.synchronize()
+    el_l = time.perf_counter() - t0
+    N = sc.res
+    return {
+        "metric": "adjoint grid cell-updates/s",
+        "value": N ** 3 * steps / el_a,
+        "unit": "cell-updates/s",
+        "steps": steps,
+        "forward_ms_per_step": el_f * 1e3 / steps,
+        "adjoint_ms_per_step": el_a * 1e3 / steps,
+        "adjoint_over_forward": el_a / el_f,
+        "checkpointed_loss_and_gradient_over_forward": el_l / el_f,
+        "checkpoint_segments": int(math.ceil(steps / math.ceil(math.sqrt(steps)))),
+        "config": {"workload": "c4_adjoint", "grid": f"{N}^3", "triangles": len(sc.tris),
+                   "gradient_shape": [steps, len(sc.tris)]},
+    }
 
 
 def modal_bench(nt=20480, nmodes=200, nsamples=4096, calls=5, cpu=True):
@@ -242,6 +415,10 @@ def main():
                     help="measure the modal sound source instead (its own JSON line)")
     ap.add_argument("--ghost", action="store_true",
                     help="measure the ghost-cell comparison solver instead (its own JSON line)")
+    ap.add_argument("--batch-steps", type=int, default=200,
+                    help="timed batch steps of the 64-scene batched leg (0: leg skipped)")
+    ap.add_argument("--adjoint-steps", type=int, default=2000,
+                    help="steps of the adjoint leg (48^3, icosphere L3; 0: leg skipped)")
     args = ap.parse_args()
     if args.modal:
         modal_bench(cpu=not args.no_cpu)
@@ -384,8 +561,14 @@ def main():
                 "fetch_kib_per_launch": tr["fetch_kib"], "fetch_scale": tr["fetch_scale"],
                 "write_kib_per_launch": tr["write_kib"], "write_scale": tr["write_scale"],
                 "calibration": f"{CALIB_MIB} MiB device copy"}
+    if args.batch_steps > 0:
+        res["batched"] = batch_bench(steps=args.batch_steps)
+    if args.adjoint_steps > 0:
+        res["adjoint"] = adjoint_bench(steps=args.adjoint_steps)
     if not args.no_cpu:
         cpu_rate, cpu_s = cpu_baseline(sc, args.cpu_steps, args.cpu_threads)
+        one_steps = max(1, args.cpu_steps // 4)
+        one_rate, one_s = cpu_baseline(sc, one_steps, 1)
         res["cpu_baseline"] = {
             "value": cpu_rate,
             "unit": "cell-updates/s",
@@ -393,6 +576,9 @@ def main():
             "kind": "port",
             "sample": f"CPU oracle (OpenMP, FP64 weights), first {args.cpu_steps} steps of {sc.name} incl. "
                       f"update_mesh, after an untimed set_mesh; {cpu_s:.1f} s",
+            "value_1_thread": one_rate,
+            "sample_1_thread": f"first {one_steps} steps, 1 thread; {one_s:.1f} s",
+            "host": host_info(),
         }
     print(json.dumps(res))
 

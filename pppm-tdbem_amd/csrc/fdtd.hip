@@ -230,7 +230,9 @@ __global__ __launch_bounds__(MBLK) void k_fdtd_march(float *__restrict__ grid3, 
 
 // One thread per interior cell (blocks [0, nb_inner)) and the ABC shell (the remaining
 // blocks); nb_inner = 0 when the interior is marched by another kernel. blk: the block's index
-// within this scene's launch (or its share of a batched launch).
+// within this scene's launch (or its share of a batched launch). Interior blocks cover 256
+// consecutive cells of one x plane's (y, z) interior (a plane takes ceil((N-4)^2 / 256) blocks),
+// so every index is 32-bit and the plane index is one wave-uniform division.
 __device__ __forceinline__ void fdtd_block(float *__restrict__ grid3, const FdtdParams &p, int t, int blk,
                                            int nb_inner)
 {
@@ -241,13 +243,14 @@ __device__ __forceinline__ void fdtd_block(float *__restrict__ grid3, const Fdtd
     float *gp = level(grid3, t, 1, nn);
     if (blk < nb_inner)
     {
-        const int M = N - 4;
-        const long long idx = (long long)blk * BLK + threadIdx.x;
-        if (idx < (long long)M * M * M)
+        const unsigned M = N - 4, MM = M * M;
+        const unsigned bpp = (MM + BLK - 1) / BLK;
+        const unsigned xb = (unsigned)blk / bpp;
+        const unsigned q = ((unsigned)blk - xb * bpp) * BLK + threadIdx.x;
+        if (q < MM)
         {
-            const int x = 2 + (int)(idx / (M * M));
-            const int rem = (int)(idx % (M * M));
-            const int y = 2 + rem / M, z = 2 + rem % M;
+            const unsigned yq = q / M;
+            const int x = 2 + (int)xb, y = 2 + (int)yq, z = 2 + (int)(q - yq * M);
             const int me = lin3(x, y, z, N);
             // laplacian + update in FP64, sum order x-, x+, y-, y+, z-, z+, -6*centre (fdtd.cu:7-33)
             gp[me] = leapfrog(g[me - N * N], g[me + N * N], g[me - N], g[me + N], g[me - 1], g[me + 1], g[me], gm[me],
@@ -259,23 +262,23 @@ __device__ __forceinline__ void fdtd_block(float *__restrict__ grid3, const Fdtd
         const int S = N - 2;
         // z faces in 16 x 16 tiles of (x, y), padded: a block's cells share their neighbour rows
         const int zt = (S - 2 + 15) / 16;
-        const long long fx = 2LL * S * S, fy = 2LL * (S - 2) * S, fz = 2LL * zt * zt * 256;
+        const int fx = 2 * S * S, fy = 2 * (S - 2) * S, fz = 2 * zt * zt * 256;
         // blocks [nb_inner, nb_inner + nb_fxy): the x and y faces (rows along z, coalesced);
         // the rest: the z faces, one cell per row, so their blocks are handed to the XCDs in
         // contiguous runs of x planes and a plane's rows meet its neighbours' in one L2
         const int b = blk - nb_inner;
-        const int nb_fxy = (int)((fx + fy + BLK - 1) / BLK);
-        long long idx;
+        const int nb_fxy = (fx + fy + BLK - 1) / BLK;
+        int idx;
         if (b < nb_fxy)
         {
-            idx = (long long)b * BLK + threadIdx.x;
+            idx = b * BLK + (int)threadIdx.x;
             if (idx >= fx + fy)
                 idx = -1;
         }
         else
         {
-            const int bz = b - nb_fxy, per = (int)((fz + 8LL * BLK - 1) / (8LL * BLK));
-            idx = fx + fy + ((long long)(bz & 7) * per + (bz >> 3)) * BLK + threadIdx.x;
+            const int bz = b - nb_fxy, per = (fz + 8 * BLK - 1) / (8 * BLK);
+            idx = fx + fy + ((bz & 7) * per + (bz >> 3)) * BLK + (int)threadIdx.x;
         }
         int i = -1, j = 0, k = 0;
         if (idx >= fx + fy + fz)
@@ -284,24 +287,24 @@ __device__ __forceinline__ void fdtd_block(float *__restrict__ grid3, const Fdtd
             ;
         else if (idx < fx)
         {
-            int side = (int)(idx / ((long long)S * S));
-            int r = (int)(idx % ((long long)S * S));
+            const int side = idx / (S * S);
+            const int r = idx - side * (S * S);
             i = side ? N - 2 : 1;
             j = 1 + r / S;
             k = 1 + r % S;
         }
         else if ((idx -= fx) < fy)
         {
-            int side = (int)(idx / ((long long)(S - 2) * S));
-            int r = (int)(idx % ((long long)(S - 2) * S));
+            const int side = idx / ((S - 2) * S);
+            const int r = idx - side * ((S - 2) * S);
             j = side ? N - 2 : 1;
             i = 2 + r / S;
             k = 1 + r % S;
         }
         else if ((idx -= fy) < fz)
         {
-            const int side = (int)(idx / ((long long)zt * zt * 256));
-            const int r = (int)(idx % ((long long)zt * zt * 256));
+            const int side = idx / (zt * zt * 256);
+            const int r = idx - side * (zt * zt * 256);
             const int tile = r >> 8, w = r & 255;
             k = side ? N - 2 : 1;
             i = 2 + (tile / zt) * 16 + (w >> 4);
@@ -505,7 +508,7 @@ int fdtd_shell_blocks(int N)
 int fdtd_inner_blocks(int N)
 {
     const long long M = N - 4;
-    return (int)((M * M * M + BLK - 1) / BLK);
+    return (int)(M * ((M * M + BLK - 1) / BLK));
 }
 
 int fdtd_tile_blocks(int N, int variant, int &xch)
@@ -549,8 +552,7 @@ hipError_t launch_fdtd(float *grid3, float *pml3, const FdtdParams &p, const Ste
         return hipGetLastError();
     }
     const long long M = N - 4, S = N - 2;
-    const long long inner = M * M * M;
-    int nb_inner = (int)((inner + BLK - 1) / BLK);
+    int nb_inner = fdtd_inner_blocks(N);
     if (M >= kMarchMin)
     {
         // about 2048 tiles or more; chunks of at least 16 planes (each chunk re-reads 2 planes)

@@ -50,6 +50,31 @@ __device__ inline void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// acc plus the values vals[lo], ..., vals[hi - 1] of one round (64 values parked by the wave's
+// lanes, 16-byte aligned), added one at a time in that order: the sequential loop's sum. Every
+// lane sweeps all 64 positions with 16-byte LDS reads (the same addresses in every lane, so
+// broadcasts) and keeps acc where a position is outside its range, so a round costs four
+// waits on LDS instead of one per value, and lanes with several short ranges (or none) run
+// in step. Positions outside [lo, hi) leave acc bit for bit as it was (a select, not + 0).
+__device__ __forceinline__ float round_sum(float acc, const float *__restrict__ vals, int lo, int hi)
+{
+#pragma unroll
+    for (int c = 0; c < WAVE; c += 16)
+    {
+        float v[16];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            *reinterpret_cast<float4 *>(&v[4 * i]) = reinterpret_cast<const float4 *>(vals + c)[i];
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+        {
+            const float sum = acc + v[i];
+            acc = (c + i >= lo && c + i < hi) ? sum : acc;
+        }
+    }
+    return acc;
+}
+
 // LayerWeightHalf::convolution (bem.h:203-213) for one pair, taps K0..15 in order.
 // Tap 0 reads the rings at slot t; taps 1..15 read the pair triangle's history windows
 // (values at t-1 .. t-15, contiguous, built once per step by k_windows) with four 16 B loads
@@ -140,12 +165,7 @@ __device__ __forceinline__ void far_wave(const SceneStep &d, int wave, int lane,
             }
             vals[lane] = v;
             wave_sync();
-            if (leader)
-            {
-                const int e = min(m1, base + WAVE);
-                for (int q = max(m0, base); q < e; q++)
-                    acc += vals[q - base];
-            }
+            acc = round_sum(acc, vals, max(m0, base) - base, min(m1, base + WAVE) - base);
             wave_sync();
         }
         if (leader)
@@ -227,14 +247,10 @@ __device__ __forceinline__ void dirichlet_wave(const SceneStep &d, int wave, int
         vals[lane] = v;
         facs[lane] = f;
         wave_sync();
-        if (leader)
         {
-            const int e = min(m1, base + WAVE);
-            for (int q = max(m0, base); q < e; q++)
-            {
-                acc += vals[q - base];
-                fac += facs[q - base];
-            }
+            const int lo = max(m0, base) - base, hi = min(m1, base + WAVE) - base;
+            acc = round_sum(acc, vals, lo, hi);
+            fac = round_sum(fac, facs, lo, hi);
         }
         wave_sync();
     }
@@ -281,12 +297,7 @@ __device__ __forceinline__ void near_wave(const SceneStep &d, int wave, int lane
             }
             vals[lane] = v;
             wave_sync();
-            if (leader)
-            {
-                const int e = min(m1, base + WAVE);
-                for (int q = max(m0, base); q < e; q++)
-                    acc += vals[q - base];
-            }
+            acc = round_sum(acc, vals, max(m0, base) - base, min(m1, base + WAVE) - base);
             wave_sync();
         }
         if (leader)
@@ -316,22 +327,22 @@ __global__ __launch_bounds__(256) void k_windows(SceneStep d, int back)
 
 __global__ __launch_bounds__(TB) void k_far(SceneStep d, int xper)
 {
-    __shared__ float vals[WPB][WAVE];
+    __shared__ __align__(16) float vals[WPB][WAVE];
     const int wid = threadIdx.x >> 6;
     far_wave(d, xcd_wave(xper, wid), threadIdx.x & 63, vals[wid]);
 }
 
 __global__ __launch_bounds__(TB) void k_dirichlet(SceneStep d)
 {
-    __shared__ float vals[WPB][WAVE];
-    __shared__ float facs[WPB][WAVE];
+    __shared__ __align__(16) float vals[WPB][WAVE];
+    __shared__ __align__(16) float facs[WPB][WAVE];
     const int wid = threadIdx.x >> 6;
     dirichlet_wave(d, blockIdx.x * WPB + wid, threadIdx.x & 63, vals[wid], facs[wid]);
 }
 
 __global__ __launch_bounds__(TB) void k_near(SceneStep d, int xper)
 {
-    __shared__ float vals[WPB][WAVE];
+    __shared__ __align__(16) float vals[WPB][WAVE];
     const int wid = threadIdx.x >> 6;
     near_wave(d, xcd_wave(xper, wid), threadIdx.x & 63, vals[wid]);
 }
@@ -390,7 +401,7 @@ __global__ __launch_bounds__(256) void k_batch_windows(const SceneStep *__restri
 
 __global__ __launch_bounds__(TB) void k_batch_far(const SceneStep *__restrict__ ds, BatchMap m)
 {
-    __shared__ float vals[WPB][WAVE];
+    __shared__ __align__(16) float vals[WPB][WAVE];
     int sc, lb;
     if (!batch_block(m, sc, lb))
         return;
@@ -400,8 +411,8 @@ __global__ __launch_bounds__(TB) void k_batch_far(const SceneStep *__restrict__ 
 
 __global__ __launch_bounds__(TB) void k_batch_dirichlet(const SceneStep *__restrict__ ds, BatchMap m)
 {
-    __shared__ float vals[WPB][WAVE];
-    __shared__ float facs[WPB][WAVE];
+    __shared__ __align__(16) float vals[WPB][WAVE];
+    __shared__ __align__(16) float facs[WPB][WAVE];
     int sc, lb;
     if (!batch_block(m, sc, lb))
         return;
@@ -411,7 +422,7 @@ __global__ __launch_bounds__(TB) void k_batch_dirichlet(const SceneStep *__restr
 
 __global__ __launch_bounds__(TB) void k_batch_near(const SceneStep *__restrict__ ds, BatchMap m)
 {
-    __shared__ float vals[WPB][WAVE];
+    __shared__ __align__(16) float vals[WPB][WAVE];
     int sc, lb;
     if (!batch_block(m, sc, lb))
         return;

@@ -507,22 +507,20 @@ __device__ inline int lookup_pair(int p, const TriGeom *__restrict__ geom, const
                                   const PrevFaces &prev)
 {
     // lists past an overflow (or an invalid mesh) were not written: touch nothing
-    if (p >= live_count(n_ptr, err_flag))
+    const bool live = p < live_count(n_ptr, err_flag);
+    int c = -1, j = -1;
+    bool ok = false;
+    if (live)
     {
-        dirty[p] = 0;
-        return 0;
+        c = owner[p];
+        j = face_nbr[p];
+        ok = (unsigned)c < (unsigned)nt && (unsigned)j < (unsigned)nt;
+        if (!ok)
+            atomicOr(err_flag, ERR_CAPACITY);
     }
-    const int c = owner[p], j = face_nbr[p];
-    if ((unsigned)c >= (unsigned)nt || (unsigned)j >= (unsigned)nt)
-    {
-        atomicOr(err_flag, ERR_CAPACITY);
-        dirty[p] = 0;
-        face_slot[p] = -1;
-        return 0;
-    }
-    int slot = -1;
+    int slot = -1, hpos = -1;  // hpos: hash entry of a key that needs a new slot
     bool fresh = false;
-    if (prev.start)
+    if (ok && prev.start)
     {
         // the same (c, j) at the same place in c's list of the previous update: its slot,
         // without a hash probe (a slot belongs to its key for the handle's life)
@@ -535,7 +533,7 @@ __device__ inline int lookup_pair(int p, const TriGeom *__restrict__ geom, const
     }
     const unsigned long long key = ((unsigned long long)c << 32) | (unsigned)j;
     unsigned h = hash64(key, hcap);
-    for (int probe = 0; slot < 0 && probe < hcap; probe++)
+    for (int probe = 0; ok && slot < 0 && probe < hcap; probe++)
     {
         unsigned long long k = hkeys[h];
         if (k == EMPTY_KEY)
@@ -552,18 +550,46 @@ __device__ inline int lookup_pair(int p, const TriGeom *__restrict__ geom, const
                 slot = hvals[h];
             if (slot < 0)  // new key, or a key left without a slot by an earlier overflow
             {
-                slot = atomicAdd(pool_count, 1);
-                if (slot >= pool_cap)
-                {
-                    atomicOr(err_flag, ERR_CAPACITY);
-                    slot = -1;
-                }
-                hvals[h] = slot;
+                hpos = (int)h;
                 fresh = true;
             }
             break;
         }
         h = (h + 1) & (unsigned)(hcap - 1);
+    }
+    // new slots: one atomic per wave, handed out in lane order, so the 64 B records of the
+    // wave's consecutive pairs are contiguous and the gathers that walk the lists read whole
+    // cache lines (no key is looked up twice in one update, so its entry gets its slot here)
+    const unsigned long long need = __ballot(hpos >= 0);
+    if (need)
+    {
+        const int lane = (int)(threadIdx.x & 63);
+        const int leader = __builtin_ctzll(need);
+        int base = 0;
+        if (lane == leader)
+            base = atomicAdd(pool_count, (int)__popcll(need));
+        base = __shfl(base, leader);
+        if (hpos >= 0)
+        {
+            slot = base + (int)__popcll(need & ((1ull << lane) - 1ull));
+            if (slot >= pool_cap)
+            {
+                atomicOr(err_flag, ERR_CAPACITY);
+                slot = -1;
+            }
+            hvals[hpos] = slot;
+        }
+    }
+    if (!live)
+    {
+        dirty[p] = 0;
+        return 0;
+    }
+    if (!ok)
+    {
+        dirty[p] = 0;
+        face_slot[p] = -1;
+        return 0;
     }
     face_slot[p] = slot;
     if (slot < 0)

@@ -244,6 +244,21 @@ def c3_scene(res: int, level: int, seed: int, steps: int = 2000, c: float = 343.
                  info={"signal": g})
 
 
+def c3_batch_sizes(n_scenes: int = 64, seed: int = SEED):
+    """(grid edge, icosphere level) of each scene of the batched configuration: edges drawn
+    uniformly from 24..96, levels from 1..4 (seeded)."""
+    rng = np.random.default_rng(seed + 3)
+    res = rng.integers(24, 97, n_scenes)
+    lv = rng.integers(1, 5, n_scenes)
+    return [(int(r), int(v)) for r, v in zip(res, lv)]
+
+
+def c3_batch(n_scenes: int = 64, steps: int = 2000, seed: int = SEED):
+    """The batched configuration: n_scenes independent scenes of uneven sizes (c3_scene each,
+    scene i seeded with i)."""
+    return [c3_scene(r, v, seed=i, steps=steps) for i, (r, v) in enumerate(c3_batch_sizes(n_scenes, seed))]
+
+
 def c4_adjoint(steps: int = 2000, c: float = 343.0, res: int = 48, seed: int = SEED) -> Scene:
     """The adjoint configuration: icosphere L3 (1280 triangles), radius 0.05 m, centred in a res^3
     grid over twice its diameter, listener at relative (0.875, 0.875, 0.875). `neumann` holds the

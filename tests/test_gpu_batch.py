@@ -12,7 +12,12 @@ CASES = [(24, 1), (37, 2), (48, 2), (61, 3), (80, 3), (96, 4)]
 
 
 def _scenes(scenes_mod, steps, cases=CASES):
-    return [scenes_mod.c3_scene(n, lv, seed=n, steps=steps) for n, lv in cases]
+    scs = [scenes_mod.c3_scene(n, lv, seed=n, steps=steps) for n, lv in cases]
+    # listener two cells past the object's largest half-extent along x (the scenes' own corner
+    # listener is more than 48 stencil steps from the surface on the larger grids)
+    for sc in scs:
+        sc.listener = (3 * sc.res // 4 + 2, sc.res // 2, sc.res // 2)
+    return scs
 
 
 def _single(amd, sc, steps):
@@ -59,7 +64,7 @@ def test_batch_equals_single_runs(amd, scenes_mod, variant):
         ref, g0, g1 = _single(amd, sc, steps)
         assert _same(out[:, i], ref), (i, sc.name)
         assert _same(hs[i].get_grid(0), g0) and _same(hs[i].get_grid(-1), g1), (i, sc.name)
-        assert np.abs(ref).max() > 0
+        assert np.abs(ref).max() > 0 and np.abs(g0).max() > 0, (i, sc.name)
 
 
 def test_batch_mixed_variants_eager_and_autotuned(amd, scenes_mod):
