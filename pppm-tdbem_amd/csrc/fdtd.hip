@@ -228,11 +228,16 @@ __global__ __launch_bounds__(MBLK) void k_fdtd_march(float *__restrict__ grid3, 
     march_columns(level(grid3, t, 0, nn), level(grid3, t, -1, nn), level(grid3, t, 1, nn), N, x0, x1, y, z0, p);
 }
 
-// One thread per interior cell (blocks [0, nb_inner)) and the ABC shell (the remaining
-// blocks); nb_inner = 0 when the interior is marched by another kernel. blk: the block's index
-// within this scene's launch (or its share of a batched launch). Interior blocks cover 256
-// consecutive cells of one x plane's (y, z) interior (a plane takes ceil((N-4)^2 / 256) blocks),
-// so every index is 32-bit and the plane index is one wave-uniform division.
+// Interior cells per thread of the thread-per-column path: XC consecutive x planes
+constexpr int XC = 4;
+
+// Interior columns (blocks [0, nb_inner)) and the ABC shell (the remaining blocks); nb_inner = 0
+// when the interior is marched by another kernel. blk: the block's index within this scene's
+// launch (or its share of a batched launch). An interior block covers 256 consecutive (y, z)
+// cells of one chunk of XC x planes (a chunk takes ceil((N-4)^2 / 256) blocks; every index is
+// 32-bit); each thread loads the XC + 2 values of its column, the y / z neighbours and the
+// previous level of its XC cells before it writes any of them, so the loads of XC cells are in
+// flight together.
 __device__ __forceinline__ void fdtd_block(float *__restrict__ grid3, const FdtdParams &p, int t, int blk,
                                            int nb_inner)
 {
@@ -250,11 +255,28 @@ __device__ __forceinline__ void fdtd_block(float *__restrict__ grid3, const Fdtd
         if (q < MM)
         {
             const unsigned yq = q / M;
-            const int x = 2 + (int)xb, y = 2 + (int)yq, z = 2 + (int)(q - yq * M);
-            const int me = lin3(x, y, z, N);
-            // laplacian + update in FP64, sum order x-, x+, y-, y+, z-, z+, -6*centre (fdtd.cu:7-33)
-            gp[me] = leapfrog(g[me - N * N], g[me + N * N], g[me - N], g[me + N], g[me - 1], g[me + 1], g[me], gm[me],
-                              p);
+            const int y = 2 + (int)yq, z = 2 + (int)(q - yq * M), x0 = 2 + (int)xb * XC;
+            const int NN2 = N * N;
+            const int c0 = lin3(x0, y, z, N);
+            float col[XC + 2], ym[XC], yp[XC], zm[XC], zp[XC], pm[XC];
+#pragma unroll
+            for (int i = 0; i < XC + 2; i++)
+                col[i] = g[c0 + (min(x0 - 1 + i, N - 2) - x0) * NN2];  // planes past N - 2 are not used
+#pragma unroll
+            for (int i = 0; i < XC; i++)
+            {
+                const int me = c0 + (min(x0 + i, N - 3) - x0) * NN2;
+                ym[i] = g[me - N];
+                yp[i] = g[me + N];
+                zm[i] = g[me - 1];
+                zp[i] = g[me + 1];
+                pm[i] = gm[me];
+            }
+#pragma unroll
+            for (int i = 0; i < XC; i++)
+                if (x0 + i <= N - 3)
+                    // laplacian + update in FP64, sum order x-, x+, y-, y+, z-, z+, -6*centre (fdtd.cu:7-33)
+                    gp[c0 + i * NN2] = leapfrog(col[i], col[i + 2], ym[i], yp[i], zm[i], zp[i], col[i + 1], pm[i], p);
         }
     }
     else
@@ -508,7 +530,7 @@ int fdtd_shell_blocks(int N)
 int fdtd_inner_blocks(int N)
 {
     const long long M = N - 4;
-    return (int)(M * ((M * M + BLK - 1) / BLK));
+    return (int)(((M + XC - 1) / XC) * ((M * M + BLK - 1) / BLK));
 }
 
 int fdtd_tile_blocks(int N, int variant, int &xch)

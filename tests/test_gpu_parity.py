@@ -29,10 +29,12 @@ def c1_short(scenes_mod):
     return scenes_mod.c1_static_sphere(steps=1000)
 
 
-@pytest.mark.parametrize("N", [20, 140, 200])
+@pytest.mark.parametrize("N", [20, 23, 57, 140, 200])
 def test_fdtd_abc_bitexact(amd, oracle_mod, N):
     """FDTD::step with the ABC (fdtd.cu:20-150): random levels t-1, t; 4 steps; bit-exact.
-    N = 140 and 200 take the marched-interior path (partial tiles in z and in the last x chunk)."""
+    N = 23 and 57: interior edges that are not a multiple of the 4-plane column chunks (a partial
+    last chunk). N = 140 and 200 take the marched-interior path (partial tiles in z and in the
+    last x chunk)."""
     print(f"fdtd abc: N={N}", flush=True)
     rng = np.random.default_rng(1)
     h = 0.01
