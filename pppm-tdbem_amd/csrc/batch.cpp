@@ -315,11 +315,14 @@ int pppm_batch_run(pppm_batch *b, const float *const *neumann_series, const long
             return bfail(b, PPPM_ERR_ARG, "batch_run: scene " + std::to_string(i) + ": Neumann series missing or stride < nt");
     if (steps == 0)
         return PPPM_OK;
-    // per-scene set-up: listener tables, pending work of the scenes' own streams
+    // per-scene set-up: list-ordered weight records, listener tables, pending work of the
+    // scenes' own streams
     std::vector<SceneStep> hd(n);
     for (int i = 0; i < n; i++)
     {
         pppm_solver *s = b->sc[i];
+        if (solver_ensure_ordered(s))
+            return bfail(b, PPPM_ERR_HIP, "batch_run: scene " + std::to_string(i) + ": " + s->err);
         const int rc = solver_set_listeners(s, listen_xyz + 3 * i, 1);
         if (rc)
             return bfail(b, rc, "batch_run: scene " + std::to_string(i) + ": " + s->err);

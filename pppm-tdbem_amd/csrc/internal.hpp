@@ -301,6 +301,10 @@ struct SceneStep
     const int *face_start, *face_wave, *face_nbr, *face_slot;
     const uint32_t *wf, *wb, *fw;  // 16 x half2 {S_k, D_k} per record
     const float2 *sf, *sb, *fsc;   // scales
+    // the same records in list order (record of C3 pair p / face pair p at p), or null: the
+    // gathers then read wf / wb at c3_pairs[p] and fw at face_slot[p]
+    const uint32_t *owf, *owb, *ofw;
+    const float2 *osf, *osb, *ofsc;
     const float *interp;
     const int *corner;
     const TriGeom *geom;
@@ -330,6 +334,12 @@ struct ListenTable
 };
 hipError_t launch_listen_setup(const ListenTable &lt, int N, const int *c3_index, int *cell, int *c3,
                                hipStream_t s);
+// list-ordered copies of the weight records: o*[p] = *[c3_pairs[p]] for the n3 C3 pairs and
+// ofw[p] = fw[face_slot[p]] for the n4 face pairs (slots below pool_cap)
+hipError_t launch_order_records(const int *c3_pairs, int n3, const uint32_t *wf, const float2 *sf, const uint32_t *wb,
+                                const float2 *sb, uint32_t *owf, float2 *osf, uint32_t *owb, float2 *osb,
+                                const int *face_slot, int n4, int pool_cap, const uint32_t *fw, const float2 *fsc,
+                                uint32_t *ofw, float2 *ofsc, hipStream_t s);
 
 // Batched step kernels: one launch covers the blocks of every scene of a batch. scene_of[v] is
 // the scene of virtual block v, first[scene] its first virtual block; total virtual blocks,

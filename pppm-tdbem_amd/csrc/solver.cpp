@@ -534,6 +534,15 @@ SceneStep scene_step(const pppm_solver *s, const StepBufs &b, const float *neuma
     d.sf = s->sf.p;
     d.sb = s->sb.p;
     d.fsc = s->fsc.p;
+    if (s->ord_version == s->mesh_version)
+    {
+        d.owf = s->owf.p;
+        d.owb = s->owb.p;
+        d.ofw = s->ofw.p;
+        d.osf = s->osf.p;
+        d.osb = s->osb.p;
+        d.ofsc = s->ofsc.p;
+    }
     d.interp = s->interp.p;
     d.corner = s->corner.p;
     d.geom = s->geom.p;
@@ -587,6 +596,27 @@ int enqueue_step(pppm_solver *s, const StepBufs &b, const float *neumann, int t0
 
 int check_handle(pppm_solver *s) { return s ? PPPM_OK : PPPM_ERR_ARG; }
 
+// the weight records of the current mesh version in gather-list order (for static meshes; a
+// stale copy is never used: scene_step compares the versions)
+int ensure_ordered(pppm_solver *s)
+{
+    if (!s->mesh_set || s->mesh_stale || s->ord_version == s->mesh_version)
+        return PPPM_OK;
+    RC(read_status(s));
+    const int n3 = GC * s->nt, n4 = s->st_h[ST_P4];
+    CK(s->owf.ensure((size_t)n3 * STEPS));
+    CK(s->owb.ensure((size_t)n3 * STEPS));
+    CK(s->osf.ensure(n3));
+    CK(s->osb.ensure(n3));
+    CK(s->ofw.ensure((size_t)std::max(n4, 1) * STEPS));
+    CK(s->ofsc.ensure(std::max(n4, 1)));
+    CK(launch_order_records(s->c3_pairs.p, n3, s->wf.p, s->sf.p, s->wb.p, s->sb.p, s->owf.p, s->osf.p, s->owb.p,
+                            s->osb.p, s->face_slot.p, n4, s->pool_cap, s->fw.p, s->fsc.p, s->ofw.p, s->ofsc.p,
+                            s->stream));
+    s->ord_version = s->mesh_version;
+    return PPPM_OK;
+}
+
 int set_listeners(pppm_solver *s, const int *listen_xyz, int n_listen)
 {
     const int N = s->N;
@@ -629,6 +659,8 @@ int run_impl(pppm_solver *s, const float *verts_series, long long vstride, const
         RC(check_ready(s, "run"));
     if (steps == 0)
         return PPPM_OK;
+    if (!verts_series)
+        RC(ensure_ordered(s));
     // a listener's C3 index can change with the lists: moving runs read the grid directly
     RC(set_listeners(s, listen_xyz, n_listen));
     if (verts_series && n_listen > 0)
@@ -757,6 +789,7 @@ int solver_set_listeners(pppm_solver *s, const int *listen_xyz, int n_listen)
 }
 FdtdParams solver_fdtd_params(const pppm_solver *s) { return fdtd_params(s); }
 int solver_read_status(pppm_solver *s) { return read_status(s); }
+int solver_ensure_ordered(pppm_solver *s) { return ensure_ordered(s); }
 SceneStep solver_scene_step(const pppm_solver *s, const float *neumann, int t0, long long stride, int n_listen,
                             float *out)
 {
@@ -1281,6 +1314,8 @@ int pppm_get_grid_weights(pppm_solver *s, void *host_recs72)
 
 int pppm_set_grid_weights(pppm_solver *s, const void *host_recs72)
 {
+    if (s)
+        s->ord_version = -1;  // the list-ordered copies no longer match
     if (check_handle(s) || !host_recs72)
         return PPPM_ERR_ARG;
     if (!s->mesh_set)
@@ -1337,6 +1372,8 @@ int pppm_get_face_weights(pppm_solver *s, int *host_start, int *host_nbr, void *
 
 int pppm_set_face_weights(pppm_solver *s, const void *host_recs72)
 {
+    if (s)
+        s->ord_version = -1;  // the list-ordered copies no longer match
     if (check_handle(s) || !host_recs72)
         return PPPM_ERR_ARG;
     if (!s->mesh_set)

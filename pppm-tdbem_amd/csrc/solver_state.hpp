@@ -145,6 +145,14 @@ struct pppm_solver
     DBuf<unsigned long long> hkeys;
     DBuf<int> hvals;
 
+    // the step's weight records in gather-list order (C3 pair p, face pair p) for a static mesh:
+    // built once per mesh version (ord_version) by the static-mesh entry points, so the gathers
+    // fetch a pair's record with its index instead of after it; moving meshes read through
+    // c3_pairs / face_slot
+    long long ord_version = -1;
+    DBuf<uint32_t> owf, owb, ofw;
+    DBuf<float2> osf, osb, ofsc;
+
     DBuf<float> hn, hd, far, nstage;
     DBuf<float> wn, wd;  // per-step history windows, 16 floats per triangle and field
     DBuf<int> lcell, lc3;
@@ -223,6 +231,8 @@ int solver_set_listeners(pppm_solver *s, const int *listen_xyz, int n_listen);
 // reads the status block (live counts, pool count) into s->st_h
 int solver_read_status(pppm_solver *s);
 FdtdParams solver_fdtd_params(const pppm_solver *s);
+// builds the list-ordered weight records for the current mesh version if they are stale
+int solver_ensure_ordered(pppm_solver *s);
 // the step kernels' view of the handle with its own per-step buffers
 SceneStep solver_scene_step(const pppm_solver *s, const float *neumann, int t0, long long stride, int n_listen,
                             float *out);

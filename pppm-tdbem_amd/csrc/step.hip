@@ -58,6 +58,24 @@ __device__ inline void wave_sync()
 // in step. Positions outside [lo, hi) leave acc bit for bit as it was (a select, not + 0).
 __device__ __forceinline__ float round_sum(float acc, const float *__restrict__ vals, int lo, int hi)
 {
+    if (lo <= 0 && hi >= WAVE)
+    {
+        // the whole round belongs to this lane's segment (the usual case of the long face lists)
+#pragma unroll
+        for (int c = 0; c < WAVE; c += 16)
+        {
+            float v[16];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                *reinterpret_cast<float4 *>(&v[4 * i]) = reinterpret_cast<const float4 *>(vals + c)[i];
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                acc += v[i];
+        }
+        return acc;
+    }
+    if (lo >= hi)
+        return acc;
 #pragma unroll
     for (int c = 0; c < WAVE; c += 16)
     {
@@ -75,14 +93,14 @@ __device__ __forceinline__ float round_sum(float acc, const float *__restrict__ 
     return acc;
 }
 
-// LayerWeightHalf::convolution (bem.h:203-213) for one pair, taps K0..15 in order.
-// Tap 0 reads the rings at slot t; taps 1..15 read the pair triangle's history windows
-// (values at t-1 .. t-15, contiguous, built once per step by k_windows) with four 16 B loads
-// per field. DZERO0: the Dirichlet value of step t is read as 0 (pppm.cu:143).
+// LayerWeightHalf::convolution (bem.h:203-213) for one pair, taps K0..15 in order, from the
+// pair triangle's history windows (four 16 B loads per field): rows 0..14 hold the values at
+// t-1 .. t-15 (taps 1..15), row 15 the value at t (tap 0; written into the window by the kernel
+// that produces it, see windows_item). DZERO0: the Dirichlet value of step t is read as 0
+// (pppm.cu:143).
 template <int K0, bool DZERO0>
-__device__ inline float conv_pair(const uint32_t *__restrict__ w16, float2 sc, const float *__restrict__ nring,
-                                  const float *__restrict__ dring, const float *__restrict__ wn,
-                                  const float *__restrict__ wd, int t, uint32_t *tap0 = nullptr)
+__device__ inline float conv_pair(const uint32_t *__restrict__ w16, float2 sc, const float *__restrict__ wn,
+                                  const float *__restrict__ wd, uint32_t *tap0 = nullptr)
 {
     uint4 q[4];
 #pragma unroll
@@ -106,8 +124,8 @@ __device__ inline float conv_pair(const uint32_t *__restrict__ w16, float2 sc, c
     {
         const float ws = f16_bits_to_f32(w[k]);
         const float wdk = f16_bits_to_f32(w[k] >> 16);
-        const float n = k == 0 ? nring[t & (RING - 1)] : hnw[k - 1];
-        const float d = k == 0 ? (DZERO0 ? 0.0f : dring[t & (RING - 1)]) : hdw[k - 1];
+        const float n = k == 0 ? hnw[STEPS - 1] : hnw[k - 1];
+        const float d = k == 0 ? (DZERO0 ? 0.0f : hdw[STEPS - 1]) : hdw[k - 1];
         const float a = -ws * sc.x * n;
         const float b = wdk * sc.y * d;
         r += a + b;
@@ -115,15 +133,18 @@ __device__ inline float conv_pair(const uint32_t *__restrict__ w16, float2 sc, c
     return r;
 }
 
-// History windows for the gathers of step t: row k of triangle i holds the ring value at
-// t - 1 - k, k = 0..15, so a gather reads tap k >= 1 at row k - 1. back = 0 when launched
-// before t is advanced (the time step), 1 after it (the phase entry points).
+// History windows for the gathers of step t: row k < 15 of triangle i holds the ring value at
+// t - 1 - k, so a gather reads tap k >= 1 at row k - 1; row 15 holds the value at t (tap 0).
+// back = 0 when launched before t is advanced (the time step: the values at t do not exist yet;
+// the far-field launch writes the Neumann value and the Dirichlet update the Dirichlet value
+// into row 15 as into the ring), 1 after it (the phase entry points: the ring holds them).
 __device__ __forceinline__ void windows_item(const SceneStep &d, int back, int i)
 {
     if (i >= d.a.nt * STEPS)
         return;
     const int tri = i / STEPS, k = i % STEPS;
-    const int slot = (d.st->t - back - k) & (RING - 1);
+    const int tn = d.st->t - back + 1;  // t of the step the gathers serve
+    const int slot = (k == STEPS - 1 ? tn : tn - 1 - k) & (RING - 1);
     d.wn[i] = d.hn[tri * RING + slot];
     d.wd[i] = d.hd[tri * RING + slot];
 }
@@ -157,11 +178,12 @@ __device__ __forceinline__ void far_wave(const SceneStep &d, int wave, int lane,
             if (base != q0)
                 rec = p < q1 ? d.c3_pairs[p] : -1;
             float v = 0.0f;
-            if (rec >= 0)
+            if (p < q1)  // rec >= 0 exactly then; the list-ordered record does not wait for rec
             {
                 const int tri = rec / GC;
-                v = conv_pair<1, false>(d.wf + (size_t)rec * STEPS, d.sf[rec], d.hn + tri * RING, d.hd + tri * RING,
-                                       d.wn + tri * STEPS, d.wd + tri * STEPS, t);
+                const uint32_t *w = d.owf ? d.owf + (size_t)p * STEPS : d.wf + (size_t)rec * STEPS;
+                const float2 sc = d.owf ? d.osf[p] : d.sf[rec];
+                v = conv_pair<1, false>(w, sc, d.wn + tri * STEPS, d.wd + tri * STEPS);
             }
             vals[lane] = v;
             wave_sync();
@@ -176,7 +198,11 @@ __device__ __forceinline__ void far_wave(const SceneStep &d, int wave, int lane,
         // Neumann input of step t into the ring (set_neumann_condition_kernel)
         const int i = (wave - d.c3_waves) * WAVE + lane;
         if (d.neumann && i < d.a.nt)
-            d.hn[i * RING + (t & (RING - 1))] = d.neumann[(size_t)(t - d.a.t0) * d.a.stride + i];
+        {
+            const float v = d.neumann[(size_t)(t - d.a.t0) * d.a.stride + i];
+            d.hn[i * RING + (t & (RING - 1))] = v;
+            d.wn[i * STEPS + STEPS - 1] = v;  // tap 0 of the Dirichlet update and the near field
+        }
     }
 }
 
@@ -236,12 +262,12 @@ __device__ __forceinline__ void dirichlet_wave(const SceneStep &d, int wave, int
             }
         }
         float v = 0.0f, f = 0.0f;
-        if (j >= 0)
+        if (p < q1)  // j >= 0 exactly then; the list-ordered record does not wait for the slot
         {
-            const float2 sc = d.fsc[ws];
+            const float2 sc = d.ofw ? d.ofsc[p] : d.fsc[ws];
+            const uint32_t *w = d.ofw ? d.ofw + (size_t)p * STEPS : d.fw + (size_t)ws * STEPS;
             uint32_t tap0;
-            v = conv_pair<0, true>(d.fw + (size_t)ws * STEPS, sc, d.hn + j * RING, d.hd + j * RING, d.wn + j * STEPS,
-                                  d.wd + j * STEPS, t, &tap0);
+            v = conv_pair<0, true>(w, sc, d.wn + j * STEPS, d.wd + j * STEPS, &tap0);
             f = f16_bits_to_f32(tap0 >> 16) * sc.y;  // face_factor = (float)D_0 * scale_D
         }
         vals[lane] = v;
@@ -258,7 +284,9 @@ __device__ __forceinline__ void dirichlet_wave(const SceneStep &d, int wave, int
     {
         const float num = ff * d.a.rc + acc;
         const double den = 0.5 * (double)area - (double)fac;
-        d.hd[cs * RING + (t & (RING - 1))] = (float)((double)num / den);
+        const float dv = (float)((double)num / den);
+        d.hd[cs * RING + (t & (RING - 1))] = dv;
+        d.wd[cs * STEPS + STEPS - 1] = dv;  // tap 0 of the near field
     }
 }
 
@@ -289,11 +317,12 @@ __device__ __forceinline__ void near_wave(const SceneStep &d, int wave, int lane
             if (base != q0)
                 rec = p < q1 ? d.c3_pairs[p] : -1;
             float v = 0.0f;
-            if (rec >= 0)
+            if (p < q1)  // rec >= 0 exactly then; the list-ordered record does not wait for rec
             {
                 const int tri = rec / GC;
-                v = conv_pair<0, false>(d.wb + (size_t)rec * STEPS, d.sb[rec], d.hn + tri * RING, d.hd + tri * RING,
-                                       d.wn + tri * STEPS, d.wd + tri * STEPS, t);
+                const uint32_t *w = d.owb ? d.owb + (size_t)p * STEPS : d.wb + (size_t)rec * STEPS;
+                const float2 sc = d.owb ? d.osb[p] : d.sb[rec];
+                v = conv_pair<0, false>(w, sc, d.wn + tri * STEPS, d.wd + tri * STEPS);
             }
             vals[lane] = v;
             wave_sync();
@@ -356,6 +385,43 @@ __global__ void k_listen_direct(SceneStep d)
     const size_t nn = (size_t)d.a.N * d.a.N * d.a.N;
     const float *g = d.grid3 + (size_t)(((t % 3) + 3) % 3) * nn;
     d.out[(size_t)(t - d.a.t0) * d.out_stride + l] = g[d.lcell[l]];
+}
+
+__global__ __launch_bounds__(256) void k_order_records(const int *__restrict__ c3_pairs, int n3,
+                                                       const uint4 *__restrict__ wf, const float2 *__restrict__ sf,
+                                                       const uint4 *__restrict__ wb, const float2 *__restrict__ sb,
+                                                       uint4 *__restrict__ owf, float2 *__restrict__ osf,
+                                                       uint4 *__restrict__ owb, float2 *__restrict__ osb,
+                                                       const int *__restrict__ face_slot, int n4, int pool_cap,
+                                                       const uint4 *__restrict__ fw, const float2 *__restrict__ fsc,
+                                                       uint4 *__restrict__ ofw, float2 *__restrict__ ofsc)
+{
+    // one thread per 16-byte quarter of a record: consecutive threads write consecutive bytes
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < 4LL * n3)
+    {
+        const int pr = (int)(i >> 2), q = (int)(i & 3);
+        const int r = c3_pairs[pr];
+        const bool ok = (unsigned)r < (unsigned)n3;  // the C3 pairs are a permutation of [0, n3)
+        owf[4 * (size_t)pr + q] = ok ? wf[4 * (size_t)r + q] : make_uint4(0, 0, 0, 0);
+        owb[4 * (size_t)pr + q] = ok ? wb[4 * (size_t)r + q] : make_uint4(0, 0, 0, 0);
+        if (q == 0)
+        {
+            osf[pr] = ok ? sf[r] : make_float2(0, 0);
+            osb[pr] = ok ? sb[r] : make_float2(0, 0);
+        }
+        return;
+    }
+    i -= 4LL * n3;
+    if (i < 4LL * n4)
+    {
+        const int pr = (int)(i >> 2), q = (int)(i & 3);
+        const int sl = face_slot[pr];
+        const bool ok = (unsigned)sl < (unsigned)pool_cap;
+        ofw[4 * (size_t)pr + q] = ok ? fw[4 * (size_t)sl + q] : make_uint4(0, 0, 0, 0);
+        if (q == 0)
+            ofsc[pr] = ok ? fsc[sl] : make_float2(0, 0);
+    }
 }
 
 __global__ void k_listen_setup(ListenTable lt, int N, const int *__restrict__ c3_index, int *__restrict__ cell,
@@ -505,6 +571,20 @@ hipError_t launch_listen_setup(const ListenTable &lt, int N, const int *c3_index
                                hipStream_t s)
 {
     k_listen_setup<<<1, 64, 0, s>>>(lt, N, c3_index, cell, c3);
+    return hipGetLastError();
+}
+
+hipError_t launch_order_records(const int *c3_pairs, int n3, const uint32_t *wf, const float2 *sf, const uint32_t *wb,
+                                const float2 *sb, uint32_t *owf, float2 *osf, uint32_t *owb, float2 *osb,
+                                const int *face_slot, int n4, int pool_cap, const uint32_t *fw, const float2 *fsc,
+                                uint32_t *ofw, float2 *ofsc, hipStream_t s)
+{
+    const long long n = 4LL * n3 + 4LL * n4;
+    if (n <= 0)
+        return hipSuccess;
+    k_order_records<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(
+        c3_pairs, n3, (const uint4 *)wf, sf, (const uint4 *)wb, sb, (uint4 *)owf, osf, (uint4 *)owb, osb, face_slot, n4,
+        pool_cap, (const uint4 *)fw, fsc, (uint4 *)ofw, ofsc);
     return hipGetLastError();
 }
 
