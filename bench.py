@@ -26,6 +26,15 @@ PKG = os.path.join(ROOT, "pppm-tdbem_amd")
 sys.path.insert(0, PKG)
 
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64
+
+# FP64 operations (each +, -, x, /, sqrt counted once) k_grid_weights spends on one dirty
+# triangle, counted from its code: 729 layer evaluations (81 targets x 9 wave numbers) of about
+# 114 each (distance and its square root, the deterministic complex exponential: exp 33 (range
+# reduction + degree-13 Horner) + sin/cos 42, two complex quotients, the double-layer factor),
+# 243 stencil batch sums of about 78, the 108 scaled 16-point inverse transforms (4 radix-2 stages
+# of 8 per entry, then the scaling: about 59 000) and the tap shifts (about 900).
+GRID_WEIGHT_FLOPS_PER_TRIANGLE = 729 * 114 + 243 * 78 + 108 * 16 * (4 * 8 + 2) + 27 * 16 * 2
 
 
 def algorithmic_bytes(N, st):
@@ -489,6 +498,7 @@ def main():
         run_steps(W + K, args.timing_steps, use_graph=False)
         solver.set_timing(False)
         phases = solver.get_timing()
+        work = solver.get_update_work()
         stream.synchronize()
 
     if args.calib_copy_mib > 0:
@@ -509,7 +519,34 @@ def main():
     step_kernels = ["fdtd", "far", "dirichlet", "near"]
     dom = max(step_kernels, key=lambda k: avg[k] or 0.0)
     achieved = bytes_[dom] / (avg[dom] * 1e-3) / 1e9
-    phase_share = {k: round(phases[k][0] / max(sum(p[0] for p in phases.values()), 1e-12), 4) for k in phases}
+    # shares of the timed step: the update's own phases (grid weights, face cache, lists) are
+    # parts of update_mesh, so the step total is fdtd + far + dirichlet + near + update_mesh
+    top = ["fdtd", "far", "dirichlet", "near", "update_mesh"]
+    step_total = max(sum(phases[k][0] for k in top), 1e-12)
+    phase_share = {k: round(phases[k][0] / step_total, 4) for k in phases}
+    # per-kernel rooflines of every phase above 10 % of the step: HBM for the step kernels
+    # (algorithmic bytes), FP64 for the grid-cache recompute (operations per dirty triangle)
+    kernels = {}
+    for k, kn in (("fdtd", "k_fdtd_abc"), ("far", "k_far"), ("dirichlet", "k_dirichlet"), ("near", "k_near")):
+        if avg[k]:
+            a = bytes_[k] / (avg[k] * 1e-3) / 1e9
+            kernels[kn] = {"share_of_step": phase_share[k], "avg_us": avg[k] * 1e3, "bound": "hbm",
+                           "bytes_per_launch": bytes_[k], "achieved": a, "unit": "GB/s", "frac": a / HBM_PEAK_GBPS}
+    if avg.get("grid_weights") and work["updates"]:
+        tri = work["grid_triangles"] / work["updates"]
+        fl = tri * GRID_WEIGHT_FLOPS_PER_TRIANGLE
+        a = fl / (avg["grid_weights"] * 1e-3) / 1e12
+        kernels["k_grid_weights"] = {"share_of_step": phase_share["grid_weights"], "avg_us": avg["grid_weights"] * 1e3,
+                                     "bound": "fp64", "dirty_triangles_per_update": tri,
+                                     "fp64_ops_per_triangle": GRID_WEIGHT_FLOPS_PER_TRIANGLE, "achieved": a,
+                                     "unit": "TFLOP/s", "peak": FP64_PEAK_TFLOPS, "frac": a / FP64_PEAK_TFLOPS}
+    for k, kn in (("face_cache", "face-cache update (k_face_lookup, scan, k_face_regular/_singular/_finish)"),
+                  ("lists", "geometry, binning and lists (k_tri_update, scans, k_lists_count/_fill)")):
+        if avg.get(k):
+            kernels[kn] = {"share_of_step": phase_share[k], "avg_us": avg[k] * 1e3}
+    if work["updates"]:
+        kernels["update_work"] = {"grid_triangles_per_update": work["grid_triangles"] / work["updates"],
+                                  "face_pairs_per_update": work["face_pairs"] / work["updates"]}
 
     res = {
         "metric": "grid cell-updates/s",
@@ -546,8 +583,10 @@ def main():
             "bytes_per_launch": bytes_[dom],
             "avg_launch_us": avg[dom] * 1e3,
         },
+        "hbm_gbps_effective_step": step_bytes / (ms_step * 1e-3) / 1e9,
         "phase_avg_us": {k: (v * 1e3 if v else None) for k, v in avg.items()},
         "phase_time_share": phase_share,
+        "kernels": kernels,
         "listener_final": listener,
     }
     if not args.no_traffic:

@@ -195,10 +195,17 @@ int pppm_stats(pppm_solver *s, long long *out8);
 /* Per-phase GPU timing with HIP events on the handle's stream, the counterpart of the
  * reference's log_time / LOG_TIME phase timers (src/core/macro.h:35-48, pppm.cu:45-50).
  * Phases: 0 FDTD, 1 far field (+ Neumann input), 2 Dirichlet update, 3 near field,
- * 4 update_mesh. Only eagerly launched work is timed (graph replays are not).
- * pppm_get_timing returns accumulated milliseconds and launch counts, then resets them. */
+ * 4 update_mesh, and within it 5 grid-cache recompute (k_grid_weights), 6 face-cache update
+ * (lookup, dirty-pair scan, integrals), 7 geometry, binning and lists. Only eagerly launched
+ * work is timed (graph replays are not).
+ * pppm_get_timing returns accumulated milliseconds and launch counts, then resets them.
+ * pppm_get_update_work: the mesh updates completed while timing was on, with the grid-cache
+ * triangles and face-cache pairs they recomputed (out3 = updates, triangles, pairs), then
+ * resets them. No counterpart in the reference (its log_time, src/core/macro.h:35-48, times
+ * phases only). */
 int pppm_set_timing(pppm_solver *s, int enable);
 int pppm_get_timing(pppm_solver *s, double *ms8, long long *count8);
+int pppm_get_update_work(pppm_solver *s, long long *out3);
 
 /* Cache inspection / loading in the reference's LayerWeightHalf layout (72 B per
  * record, src/algorithm/bem.h:163-169). Grid cache: records (tri*27+o)*2 + {0: fdtd, 1: bem}

@@ -345,6 +345,7 @@ int enqueue_rebuild(pppm_solver *s, bool first, const float *verts_series, long 
     const BuildArgs ba{N, nt, s->nv, s->h, s->minp[0], s->minp[1], s->minp[2]};
     const WeightArgs wa = weight_args(s);
     const float *src = vsrc ? vsrc : s->verts.p;
+    int sp = span_begin(s, 7);
     CK(launch_tri_update(src, vsrc ? s->verts.p : nullptr, s->nv, s->tris.p, s->geom.p, ba, wa, s->gmeta.p,
                          first ? 1 : 0, s->gflag.p, s->interp.p, cn, err, st));
 
@@ -360,14 +361,18 @@ int enqueue_rebuild(pppm_solver *s, bool first, const float *verts_series, long 
     CK(scan_batch(b2, s->scan_tmp.p, st));
     CK(launch_lists_fill(la, st));
     s->counters_zero = true;
+    span_end(s, sp);
 
     // grid cache: dirty triangles only. Launch sizes follow the previous update's recompute
     // counts (the kernels stride, so a larger count is still covered); a first build sizes
     // for everything.
     const long long ghint = first ? -1 : 2 * s->last_grid_recomputes + 64;
     const long long fhint = first ? -1 : 2 * s->last_face_recomputes + 64;
+    sp = span_begin(s, 5);
     CK(launch_grid_weights(s->verts.p, s->tris.p, s->geom.p, s->glist.p, stp + ST_GDIRTY, nt, wa, s->bem, s->wf.p,
                            s->sf.p, s->wb.p, s->sb.p, s->gmeta.p, err, ghint, st));
+    span_end(s, sp);
+    sp = span_begin(s, 6);
 
     // face cache: persistent (c, j) slots, dirty pairs recomputed (FaceCache::update_cache)
     CK(launch_face_lookup(s->geom.p, s->tris.p, s->face_owner.p, s->face_nbr.p, s->face_start.p + nt, s->cap_p4,
@@ -381,6 +386,7 @@ int enqueue_rebuild(pppm_solver *s, bool first, const float *verts_series, long 
     CK(launch_face_weights(s->verts.p, s->tris.p, s->geom.p, s->dirty_list.p, stp + ST_FDIRTY, s->cap_p4,
                            s->face_owner.p, s->face_nbr.p, s->face_slot.p, s->interp.p, wa, s->bem, s->ftmp.p, s->fw.p,
                            s->fsc.p, s->fmeta.p, err, fhint, st));
+    span_end(s, sp);
 
     return PPPM_OK;
 }
@@ -465,6 +471,12 @@ int rebuild_attempts(pppm_solver *s, bool first, const float *vsrc)
         if (!(e & ERR_CAPACITY) && !needs_growth(s))
         {
             s->faces_valid = true;
+            if (s->timing)
+            {
+                s->work_updates++;
+                s->work_grid += s->st_h[ST_GDIRTY];
+                s->work_face += s->st_h[ST_FDIRTY];
+            }
             return PPPM_OK;
         }
         RC(grow(s));
@@ -962,6 +974,17 @@ int pppm_get_timing(pppm_solver *s, double *ms8, long long *count8)
         s->phase_ms[i] = 0;
         s->phase_n[i] = 0;
     }
+    return PPPM_OK;
+}
+
+int pppm_get_update_work(pppm_solver *s, long long *out3)
+{
+    if (check_handle(s) || !out3)
+        return PPPM_ERR_ARG;
+    out3[0] = s->work_updates;
+    out3[1] = s->work_grid;
+    out3[2] = s->work_face;
+    s->work_updates = s->work_grid = s->work_face = 0;
     return PPPM_OK;
 }
 

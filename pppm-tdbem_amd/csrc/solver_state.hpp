@@ -219,6 +219,9 @@ struct pppm_solver
     std::vector<hipEvent_t> pool;
     double phase_ms[8] = {0};
     long long phase_n[8] = {0};
+    // work of the mesh updates completed while timing is on: updates, grid-cache triangles and
+    // face-cache pairs recomputed
+    long long work_updates = 0, work_grid = 0, work_face = 0;
 };
 
 

@@ -29,7 +29,7 @@ EXPORTS = [
     "pppm_time_idx", "pppm_grid_ptr", "pppm_read_cells", "pppm_history_ptrs", "pppm_triangles",
     "pppm_stats", "pppm_get_grid_weights", "pppm_set_grid_weights", "pppm_get_face_weights",
     "pppm_set_face_weights", "pppm_get_interp_weights", "pppm_get_c3", "pppm_get_b4",
-    "pppm_set_timing", "pppm_get_timing", "pppm_run_moving", "pppm_classify", "pppm_fill_fresh",
+    "pppm_set_timing", "pppm_get_timing", "pppm_get_update_work", "pppm_run_moving", "pppm_classify", "pppm_fill_fresh",
     "pppm_set_solid_zero", "pppm_get_cells", "pppm_set_time_idx", "pppm_ffat_collect",
     "pppm_ghost_setup", "pppm_ghost_set_neumann", "pppm_ghost_solve", "pppm_ghost_step", "pppm_ghost_update_mesh",
     "pppm_ghost_get", "pppm_slice_image",
@@ -95,6 +95,7 @@ def load_library(path: str = LIB_PATH):
         "pppm_get_b4": ([vp, vp, vp, vp], i),
         "pppm_set_timing": ([vp, i], i),
         "pppm_get_timing": ([vp, vp, vp], i),
+        "pppm_get_update_work": ([vp, vp], i),
         "pppm_run_moving": ([vp, vp, ll, vp, ll, i, vp, i, vp, i], i),
         "pppm_classify": ([vp, i, P(i)], i),
         "pppm_fill_fresh": ([vp, vp, vp, i, P(i)], i),
@@ -377,7 +378,7 @@ class PPPMSolver:
     def solve_fdtd_near(self):
         self._check(self.lib.pppm_solve_near(self.h), "solve_near")
 
-    PHASES = ("fdtd", "far", "dirichlet", "near", "update_mesh")
+    PHASES = ("fdtd", "far", "dirichlet", "near", "update_mesh", "grid_weights", "face_cache", "lists")
 
     def set_timing(self, enable=True):
         self._check(self.lib.pppm_set_timing(self.h, int(enable)), "set_timing")
@@ -389,6 +390,13 @@ class PPPMSolver:
         self._check(self.lib.pppm_get_timing(self.h, ms.ctypes.data_as(ctypes.c_void_p),
                                              n.ctypes.data_as(ctypes.c_void_p)), "get_timing")
         return {name: (float(ms[i]), int(n[i])) for i, name in enumerate(self.PHASES)}
+
+    def get_update_work(self):
+        """{updates, grid_triangles, face_pairs}: mesh updates completed while timing was on and
+        the cache entries they recomputed, since the last call."""
+        out = np.zeros(3, np.int64)
+        self._check(self.lib.pppm_get_update_work(self.h, out.ctypes.data_as(ctypes.c_void_p)), "get_update_work")
+        return {"updates": int(out[0]), "grid_triangles": int(out[1]), "face_pairs": int(out[2])}
 
     def reset(self):
         self._check(self.lib.pppm_reset(self.h), "reset")
