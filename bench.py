@@ -302,6 +302,145 @@ def adjoint_bench(steps=2000):
     }
 
 
+def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11):
+    """Batched adjoint at scale: n_scenes scenes of res^3 with a radially perturbed icosphere L5
+    (20480 triangles) and 8 listeners each, `steps` steps, source reconstruction by gradient
+    descent on the per-triangle accelerations: J_i = sum (F_i(g_i) - target_i)^2, target_i the
+    listener signals of seeded accelerations, g_i from zero. Each iteration: the batched,
+    sqrt(T)-checkpointed loss and its gradient (autograd.BatchListenerLoss: batched forward,
+    batched recompute per segment, adjoint per scene), then an exact line search along -grad
+    (J is quadratic: alpha_i = |grad_i|^2 / (2 |F_i grad_i|^2), one batched forward).
+    Spot checks on scene 0 at full length: the adjoint identity <y, F g> = <F^T y, g> for random
+    g, y, and the directional derivative (J(g + d) - J(g - d)) / 2 against <grad J, d>, which J
+    being quadratic makes exact up to rounding. Returns the leg's JSON object."""
+    import torch
+    import pppm_amd
+    import scenes
+    import autograd as pad
+
+    t_start = time.perf_counter()
+    free0, total = torch.cuda.mem_get_info()
+    scs = [scenes.c5_scene(i, steps=steps, res=res) for i in range(n_scenes)]
+    t_gen = time.perf_counter() - t_start
+    t0 = time.perf_counter()
+    sv = []
+    for sc in scs:
+        g = pppm_amd.PPPMSolver(sc.res, sc.dl, sc.dt, sc.min_pos, 0, 0.83, sc.c)
+        g.set_mesh(sc.verts, sc.tris)
+        sv.append(g)
+    torch.cuda.synchronize()
+    t_setmesh = time.perf_counter() - t0
+    batch = pppm_amd.PPPMBatch(sv)
+    lis = [sc.info["listeners"] for sc in scs]
+    L, nt = len(lis[0]), len(scs[0].tris)
+    dev = "cuda"
+    # targets: listener signals of the seeded accelerations
+    true = [torch.from_numpy(sc.info["target_acc"]).to(dev) for sc in scs]
+    for sc in scs:
+        sc.info["target_acc"] = None
+    out = torch.zeros(steps, n_scenes, L, device=dev)
+    for s_ in sv:
+        s_.reset()
+    batch.run(true, steps, lis, out)
+    targets = [out[:, i].clone() for i in range(n_scenes)]
+    # plain batched forward and per-scene adjoint sweeps over the whole length (no checkpoints)
+    for s_ in sv:
+        s_.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch.run(true, steps, lis, out)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    fwd_out0 = out[:, 0].clone()
+    gen = torch.Generator(device=dev).manual_seed(seed_dir)
+    y = [torch.randn(steps, L, device=dev, generator=gen) for _ in range(n_scenes)]
+    grads = [torch.zeros(steps, nt, device=dev) for _ in range(n_scenes)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i, (s_, yy, gg) in enumerate(zip(sv, y, grads)):
+        s_.adjoint_reset(steps - 1)
+        s_.adjoint_run(yy, steps, lis[i], gg)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    # adjoint identity on scene 0: <y, F g> = <F^T y, g> with g = the seeded accelerations
+    lhs = float((y[0].double() * fwd_out0.double()).sum())
+    rhs = float((grads[0].double() * true[0].double()).sum())
+    dot_rel = abs(lhs - rhs) / max(abs(lhs), 1e-300)
+    del true, grads, y
+    # gradient descent
+    gs = [torch.zeros(steps, nt, device=dev) for _ in range(n_scenes)]
+    hist, t_loss, t_back, t_ls = [], 0.0, 0.0, 0.0
+    ck_bytes = 0
+    torch.cuda.reset_peak_memory_stats()
+    for it in range(iters):
+        xs = [g.detach().requires_grad_(True) for g in gs]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        J = pad.batch_listener_loss(batch, sv, xs, targets, lis)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        J.sum().backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        t_loss += t1 - t0
+        t_back += t2 - t1
+        ck_bytes = sum(sv_.state_bytes() for sv_ in sv) * math.ceil(steps / math.ceil(math.sqrt(steps)))
+        hist.append(J.detach().double().cpu().numpy())
+        gr = [x.grad.detach() for x in xs]
+        # exact line search along -grad: alpha_i = |grad_i|^2 / (2 |F_i grad_i|^2)
+        for s_ in sv:
+            s_.reset()
+        batch.run(gr, steps, lis, out)
+        torch.cuda.synchronize()
+        t_ls += time.perf_counter() - t2
+        for i in range(n_scenes):
+            den = 2.0 * float((out[:, i].double() ** 2).sum())
+            alpha = float((gr[i].double() ** 2).sum()) / den if den > 0 else 0.0
+            gs[i] = (xs[i].detach() - alpha * gr[i]).contiguous()
+        del xs, gr
+    # directional derivative on scene 0 at the last iterate before its update
+    d = torch.randn(steps, nt, device=dev, generator=gen)
+    g_prev = gs[0]  # any point works for a quadratic; take the current iterate
+
+    def J0(g):
+        o = torch.zeros(steps, L, device=dev)
+        sv[0].reset()
+        sv[0].run(g.contiguous(), steps, lis[0], o)
+        return float(((o.double() - targets[0].double()) ** 2).sum())
+
+    xs0 = g_prev.detach().requires_grad_(True)
+    Jc = pad.listener_loss(xs0, targets[0], sv[0], lis[0])
+    Jc.backward()
+    fd = (J0(g_prev + d) - J0(g_prev - d)) / 2.0
+    an = float((xs0.grad.double() * d.double()).sum())
+    dir_rel = abs(fd - an) / max(abs(an), 1e-300)
+    free1, _ = torch.cuda.mem_get_info()
+    hist = np.array(hist)
+    cells = n_scenes * res ** 3 * steps
+    per_it = (t_loss + t_back + t_ls) / iters
+    res_ = {
+        "metric": "batched adjoint: source reconstruction by gradient descent",
+        "config": {"workload": f"c5_batched_adjoint_{n_scenes}x{res}^3", "scenes": n_scenes, "grid": f"{res}^3",
+                   "triangles_per_scene": nt, "listeners_per_scene": L, "steps": steps, "iterations": iters},
+        "forward_cell_updates_per_s": cells / t_fwd,
+        "adjoint_cell_updates_per_s": cells / t_adj,
+        "forward_s": t_fwd, "adjoint_s": t_adj, "adjoint_over_forward": t_adj / t_fwd,
+        "iteration_s": per_it,
+        "iteration_breakdown_s": {"checkpointed_forward": t_loss / iters, "recompute_and_adjoint": t_back / iters,
+                                  "line_search_forward": t_ls / iters},
+        "checkpoint_recompute_overhead": (t_loss + t_back) / iters / (t_fwd + t_adj) - 1.0,
+        "checkpoint_bytes": ck_bytes,
+        "memory_gb": {"device_used": (total - free1) / 1e9, "device_used_before": (total - free0) / 1e9,
+                      "torch_peak_during_descent": torch.cuda.max_memory_allocated() / 1e9},
+        "loss_relative_to_initial": (hist[:, :] / hist[0]).mean(axis=1).tolist(),
+        "loss_per_scene_first_last": [hist[0].tolist()[:4], hist[-1].tolist()[:4]],
+        "spot_checks": {"adjoint_identity_rel": dot_rel, "directional_derivative_rel": dir_rel},
+        "setup_s": {"scene_generation": t_gen, "set_mesh_all": t_setmesh},
+        "projected_s_for_50_iterations": 50 * per_it,
+    }
+    return res_
+
+
 def modal_bench(nt=20480, nmodes=200, nsamples=4096, calls=5, cpu=True):
     """Modal sound source (surface accelerations of a rigid body) on one MI355X: nt surface
     triangles (icosphere level 5 size), nmodes modes, blocks of nsamples audio samples per call.
@@ -405,7 +544,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's 4000)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="c2_moving", choices=["c2_moving", "c1_static_sphere", "large_static_sphere"],
+    ap.add_argument("--c5-scenes", type=int, default=16)
+    ap.add_argument("--c5-steps", type=int, default=8192)
+    ap.add_argument("--c5-iters", type=int, default=3)
+    ap.add_argument("--c5-res", type=int, default=128)
+    ap.add_argument("--workload", default="c2_moving",
+                    choices=["c2_moving", "c1_static_sphere", "large_static_sphere", "c5_batched_adjoint"],
                     help="c2_moving: the metric's workload; c1_static_sphere: 32^3 static sphere; "
                          "large_static_sphere: one 512^3 grid around the C5 object (HBM-resident levels)")
     ap.add_argument("--timing-steps", type=int, default=200, help="extra steps timed per kernel with HIP events")
@@ -434,6 +578,9 @@ def main():
         return
     if args.ghost:
         ghost_bench(cpu=not args.no_cpu)
+        return
+    if args.workload == "c5_batched_adjoint":
+        print(json.dumps(c5_bench(args.c5_scenes, args.c5_steps, args.c5_iters, args.c5_res)))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design note, "

@@ -111,9 +111,10 @@ int pppm_run_moving(pppm_solver *s, const float *verts_series, long long vstride
  * advance their time indices as pppm_run would. Each scene's results equal its own pppm_run
  * bit for bit.
  * pppm_batch_run: per scene i, neumann_series[i] (device, steps x strides[i] floats, row k =
- *   the Neumann input of step k, strides[i] >= nt_i) and one listener cell listen_xyz[3i..3i+2];
- *   out_dev (device, steps x n_scenes floats) gets out_dev[k * n_scenes + i] = scene i's listener
- *   after step k. use_graph != 0: 32 batch steps per captured HIP graph launch. Synchronous.
+ *   the Neumann input of step k, strides[i] >= nt_i) and n_listen listener cells (1 to 64)
+ *   listen_xyz[3 (i n_listen + l) ..]; out_dev (device, steps x n_scenes x n_listen floats) gets
+ *   out_dev[(k n_scenes + i) n_listen + l] = scene i's listener l after step k. use_graph != 0:
+ *   32 batch steps per captured HIP graph launch. Synchronous.
  * pppm_batch_autotune: times every FDTD interior variant (0: one thread per cell; 1..4:
  *   LDS-staged 2.5D tiles of 8x64, 16x32, 4x64, 8x32 (y, z) columns) for each size class of grid
  *   edge N present in the batch (classes [0,40) [40,56) [56,72) [72,88) [88,112) [112,160)
@@ -132,7 +133,7 @@ int pppm_batch_destroy(pppm_batch *b);
 const char *pppm_batch_last_error(const pppm_batch *b);
 int pppm_batch_set_stream(pppm_batch *b, void *hip_stream);
 int pppm_batch_run(pppm_batch *b, const float *const *neumann_series, const long long *strides, int steps,
-                   const int *listen_xyz, float *out_dev, int use_graph);
+                   int n_listen, const int *listen_xyz, float *out_dev, int use_graph);
 int pppm_batch_autotune(pppm_batch *b, int reps, float *ms_out);
 int pppm_batch_variants(pppm_batch *b, int *variants);
 int pppm_batch_set_variants(pppm_batch *b, const int *variants);

@@ -102,3 +102,69 @@ def listener_signal(g, solver, listeners, use_graph=True):
 
 def listener_loss(g, target, solver, listeners, segment=None, use_graph=True):
     return ListenerLoss.apply(g, target, solver, listeners, segment, use_graph)
+
+
+class BatchListenerLoss(torch.autograd.Function):
+    """J_i = sum_{t, l} (out_i[t, l] - target_i[t, l])^2 for every scene i of a batch, with the
+    same sqrt(T) checkpointing as ListenerLoss: the forward runs all scenes together
+    (pppm_batch_run) in segments of S steps from the zero state and keeps each scene's state at
+    the start of every segment; the backward walks the segments from the last, restores every
+    scene's checkpoint, recomputes the segment's samples with one batched run and reverses it
+    scene by scene with the adjoint. Returns J as an (n_scenes,) FP64-accumulated float32 tensor;
+    the gradient of scene i depends only on its own inputs (the scenes are independent).
+    Inputs: batch (PPPMBatch over `solvers`), targets (one (T, L) tensor per scene), listeners
+    (L cells per scene), then the per-scene (T, nt_i) inputs g_i."""
+
+    @staticmethod
+    def forward(ctx, batch, solvers, targets, listeners, segment, use_graph, *gs):
+        n = len(solvers)
+        for g, s in zip(gs, solvers):
+            _check_inputs(g, s)
+        T, L = gs[0].shape[0], len(listeners[0])
+        if any(g.shape[0] != T for g in gs) or any(len(x) != L for x in listeners):
+            raise PPPMError("every scene needs the same number of steps and of listeners")
+        if any(tuple(t.shape) != (T, L) for t in targets):
+            raise PPPMError("targets must be (steps, listeners) per scene")
+        S = int(segment) if segment else max(1, math.ceil(math.sqrt(T)))
+        nseg = (T + S - 1) // S
+        dev = gs[0].device
+        ck = [torch.empty(nseg, s.state_bytes(), dtype=torch.uint8, device=dev) for s in solvers]
+        out = torch.zeros(S, n, L, device=dev, dtype=torch.float32)
+        J = torch.zeros(n, device=dev, dtype=torch.float64)
+        for s in solvers:
+            s.reset()
+        for k in range(nseg):
+            a, b = k * S, min(T, (k + 1) * S)
+            for i, s in enumerate(solvers):
+                s.save_state(ck[i][k])
+            batch.run([g[a:b] for g in gs], b - a, listeners, out[:b - a], use_graph=use_graph)
+            for i in range(n):
+                J[i] += ((out[:b - a, i].double() - targets[i][a:b].double()) ** 2).sum()
+        ctx.save_for_backward(*gs)
+        ctx.batch, ctx.solvers, ctx.targets, ctx.listeners = batch, solvers, targets, listeners
+        ctx.S, ctx.nseg, ctx.ck, ctx.use_graph = S, nseg, ck, use_graph
+        ctx.checkpoint_bytes = sum(c.numel() for c in ck)
+        return J.float()
+
+    @staticmethod
+    def backward(ctx, gJ):
+        gs = ctx.saved_tensors
+        solvers, S, n = ctx.solvers, ctx.S, len(ctx.solvers)
+        T, L = gs[0].shape[0], len(ctx.listeners[0])
+        grads = [torch.zeros(T, s.nt, device=g.device, dtype=torch.float32) for g, s in zip(gs, solvers)]
+        out = torch.zeros(S, n, L, device=gs[0].device, dtype=torch.float32)
+        for s in solvers:
+            s.adjoint_reset(T - 1)
+        for k in reversed(range(ctx.nseg)):
+            a, b = k * S, min(T, (k + 1) * S)
+            for i, s in enumerate(solvers):
+                s.load_state(ctx.ck[i][k])
+            ctx.batch.run([g[a:b] for g in gs], b - a, ctx.listeners, out[:b - a], use_graph=ctx.use_graph)  # recompute
+            for i, s in enumerate(solvers):
+                gout = (2.0 * gJ[i].float()) * (out[:b - a, i] - ctx.targets[i][a:b].float())
+                s.adjoint_run(gout.contiguous(), b - a, ctx.listeners[i], grads[i][a:b], use_graph=ctx.use_graph)
+        return (None, None, None, None, None, None) + tuple(grads)
+
+
+def batch_listener_loss(batch, solvers, gs, targets, listeners, segment=None, use_graph=True):
+    return BatchListenerLoss.apply(batch, solvers, targets, listeners, segment, use_graph, *gs)

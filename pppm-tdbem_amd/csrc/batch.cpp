@@ -302,13 +302,13 @@ int pppm_batch_set_stream(pppm_batch *b, void *hip_stream)
 }
 
 int pppm_batch_run(pppm_batch *b, const float *const *neumann_series, const long long *strides, int steps,
-                   const int *listen_xyz, float *out_dev, int use_graph)
+                   int n_listen, const int *listen_xyz, float *out_dev, int use_graph)
 {
     if (!b)
         return PPPM_ERR_ARG;
     const int n = (int)b->sc.size();
-    if (steps < 0 || !neumann_series || !strides || !listen_xyz || !out_dev)
-        return bfail(b, PPPM_ERR_ARG, "batch_run: bad arguments");
+    if (steps < 0 || !neumann_series || !strides || !listen_xyz || !out_dev || n_listen < 1 || n_listen > 64)
+        return bfail(b, PPPM_ERR_ARG, "batch_run: bad arguments (1 to 64 listeners per scene)");
     BRC(check_scenes(b));
     for (int i = 0; i < n; i++)
         if (!neumann_series[i] || strides[i] < b->sc[i]->nt)
@@ -323,12 +323,13 @@ int pppm_batch_run(pppm_batch *b, const float *const *neumann_series, const long
         pppm_solver *s = b->sc[i];
         if (solver_ensure_ordered(s))
             return bfail(b, PPPM_ERR_HIP, "batch_run: scene " + std::to_string(i) + ": " + s->err);
-        const int rc = solver_set_listeners(s, listen_xyz + 3 * i, 1);
+        const int rc = solver_set_listeners(s, listen_xyz + 3 * (size_t)i * n_listen, n_listen);
         if (rc)
             return bfail(b, rc, "batch_run: scene " + std::to_string(i) + ": " + s->err);
         BCK(hipStreamSynchronize(s->stream));
-        hd[i] = solver_scene_step(s, neumann_series[i], s->t_host + 1, strides[i], 1, out_dev + i);
-        hd[i].out_stride = n;
+        hd[i] = solver_scene_step(s, neumann_series[i], s->t_host + 1, strides[i], n_listen,
+                                  out_dev + (size_t)i * n_listen);
+        hd[i].out_stride = (long long)n * n_listen;
     }
     BCK(b->descs.ensure(n));
     BCK(hipMemcpyAsync(b->descs.p, hd.data(), n * sizeof(SceneStep), hipMemcpyHostToDevice, b->stream));

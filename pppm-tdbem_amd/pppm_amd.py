@@ -120,7 +120,7 @@ def load_library(path: str = LIB_PATH):
         "pppm_batch_destroy": ([vp], i),
         "pppm_batch_last_error": ([vp], ctypes.c_char_p),
         "pppm_batch_set_stream": ([vp, vp], i),
-        "pppm_batch_run": ([vp, vp, vp, i, vp, vp, i], i),
+        "pppm_batch_run": ([vp, vp, vp, i, i, vp, vp, i], i),
         "pppm_batch_autotune": ([vp, i, vp], i),
         "pppm_batch_variants": ([vp, vp], i),
         "pppm_batch_set_variants": ([vp, vp], i),
@@ -640,10 +640,11 @@ class PPPMBatch:
 
     def run(self, neumann_series, steps, listeners, out_dev, use_graph=True, strides=None):
         """neumann_series: one device tensor per scene (steps x stride_i); listeners: one (x, y, z)
-        per scene; out_dev: device tensor of steps x n_scenes floats."""
+        per scene, or for every scene the same number L of them (a list of L cells per scene);
+        out_dev: device tensor of steps x n_scenes (x L) floats, row k = the listeners after step k."""
         n = len(self.scenes)
         if len(neumann_series) != n or len(listeners) != n:
-            raise PPPMError("one Neumann series and one listener per scene")
+            raise PPPMError("one Neumann series and one listener set per scene")
         strides = [s.nt for s in self.scenes] if strides is None else [int(x) for x in strides]
         args = []
         for i, (sc, ns, st) in enumerate(zip(self.scenes, neumann_series, strides)):
@@ -651,14 +652,20 @@ class PPPMBatch:
                 raise PPPMError(f"scene {i}: stride < triangle count")
             args.append(_arg(ns, f"neumann_series[{i}]", (int(steps) - 1) * st + sc.nt if steps > 0 else 0,
                              "f32", "device"))
-        lx = np.ascontiguousarray(np.asarray(listeners, dtype=np.int32).reshape(n, 3))
+        lx = np.asarray(listeners, dtype=np.int32)
+        if lx.ndim == 2:
+            lx = lx[:, None, :]
+        if lx.ndim != 3 or lx.shape[0] != n or lx.shape[2] != 3 or not 1 <= lx.shape[1] <= 64:
+            raise PPPMError("listeners: (x, y, z) per scene, or the same number (1 to 64) of cells per scene")
+        L = lx.shape[1]
+        lx = np.ascontiguousarray(lx)
         for i, sc in enumerate(self.scenes):
             if lx[i].min() < 0 or lx[i].max() >= sc.res():
                 raise PPPMError(f"scene {i}: listener outside the grid")
-        out = _arg(out_dev, "out", int(steps) * n, "f32", "device")
+        out = _arg(out_dev, "out", int(steps) * n * L, "f32", "device")
         ptrs = (ctypes.c_void_p * n)(*[a.ptr.value for a in args])
         strd = (ctypes.c_longlong * n)(*strides)
-        self._check(self.lib.pppm_batch_run(self.h, ptrs, strd, int(steps), lx.ctypes.data_as(ctypes.c_void_p),
+        self._check(self.lib.pppm_batch_run(self.h, ptrs, strd, int(steps), int(L), lx.ctypes.data_as(ctypes.c_void_p),
                                             out.ptr, int(use_graph)), "batch_run")
 
     def autotune(self, reps=20):

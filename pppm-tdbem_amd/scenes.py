@@ -279,6 +279,34 @@ def c4_adjoint(steps: int = 2000, c: float = 343.0, res: int = 48, seed: int = S
                  info={"radius": radius, "target_acc": np.ascontiguousarray(target_acc)})
 
 
+def c5_scene(index: int, steps: int = 8192, c: float = 343.0, res: int = 128, seed: int = SEED,
+             n_listen: int = 8) -> Scene:
+    """One scene of the batched adjoint configuration: icosphere L5 (10242 vertices, 20480
+    triangles) of radius 0.05 m with each vertex's radius scaled by 1 + U[-0.05, 0.05] (seeded per
+    scene), centred in a res^3 grid over four radii (h = radius / 32 at 128^3), n_listen listeners
+    at seeded positions 1.3 to 1.9 radii from the centre. info["target_acc"]: the seeded
+    per-triangle accelerations (band-limited noise, 0-4 kHz, unit RMS) whose listener signals
+    are the reconstruction target; `neumann` is None (the unknowns start at zero)."""
+    rng = np.random.default_rng(seed + 500 + index)
+    radius = 0.05
+    length = 4 * radius
+    h = np.float32(length / res)
+    dt = np.float32(float(h) / (math.sqrt(3.0) * c))
+    V, F = icosphere(5, radius)
+    V = (V.astype(np.float64) * (1.0 + rng.uniform(-0.05, 0.05, (V.shape[0], 1)))).astype(np.float32)
+    mn = (-length / 2, -length / 2, -length / 2)
+    lis = []
+    while len(lis) < n_listen:
+        d = rng.standard_normal(3)
+        d *= rng.uniform(1.3, 1.9) * radius / np.linalg.norm(d)
+        cell = tuple(int(np.floor((x - m) / float(h))) for x, m in zip(d, mn))
+        if all(4 <= q <= res - 5 for q in cell) and cell not in lis:
+            lis.append(cell)
+    target_acc = bandlimited_noise(rng, F.shape[0], steps, float(dt)).T.astype(np.float32)
+    return Scene(f"c5_scene_{index}", res, float(h), float(dt), mn, c, V, F, steps, lis[0], None,
+                 info={"listeners": lis, "target_acc": np.ascontiguousarray(target_acc), "radius": radius})
+
+
 def rel_l2(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)

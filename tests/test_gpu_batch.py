@@ -89,3 +89,29 @@ def test_batch_mixed_variants_eager_and_autotuned(amd, scenes_mod):
     for i, sc in enumerate(scs):
         ref, _, _ = _single(amd, sc, 2 * steps)
         assert _same(out2[:, i], ref[steps:]), (i, sc.name)
+
+
+def test_batch_several_listeners_per_scene(amd, scenes_mod):
+    """Three listeners per scene (one near the object, one in the corrected region's corner case
+    of the scene's own corner, one on the axis): each scene's samples equal its single run's."""
+    import torch
+    steps = 40
+    scs = _scenes(scenes_mod, steps, CASES[:4])
+    lis = [[sc.listener, (sc.res // 2, sc.res // 2, 3 * sc.res // 4 + 1), (2, 3, 4)] for sc in scs]
+    print(f"batch, 3 listeners per scene: {len(scs)} scenes, {steps} steps", flush=True)
+    hs = []
+    for sc in scs:
+        g = amd.PPPMSolver(sc.res, sc.dl, sc.dt, sc.min_pos, 0, 0.83, sc.c)
+        g.set_mesh(sc.verts, sc.tris)
+        hs.append(g)
+    b = amd.PPPMBatch(hs)
+    out = torch.zeros(steps, len(scs), 3, device="cuda")
+    b.run([torch.from_numpy(sc.neumann[:steps]).cuda() for sc in scs], steps, lis, out)
+    out = out.cpu().numpy()
+    for i, sc in enumerate(scs):
+        g = amd.PPPMSolver(sc.res, sc.dl, sc.dt, sc.min_pos, 0, 0.83, sc.c)
+        g.set_mesh(sc.verts, sc.tris)
+        ref = torch.zeros(steps, 3, device="cuda")
+        g.run(torch.from_numpy(sc.neumann[:steps]).cuda(), steps, lis[i], ref)
+        assert _same(out[:, i, :], ref.cpu().numpy()), (i, sc.name)
+    assert np.abs(out[:, :, :2]).max() > 0

@@ -1,0 +1,63 @@
+"""The batched, sqrt(T)-checkpointed listener loss (autograd.BatchListenerLoss: batched forward
+and recompute, adjoint per scene) against the single-scene checkpointed loss of each scene
+(autograd.ListenerLoss): losses and gradients bit for bit, several listeners per scene; and the
+adjoint identity <y, F g> = <F^T y, g> on a scene of the batched-adjoint configuration's kind
+(radially perturbed icosphere, 8 listeners)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batch_loss_equals_single_scene_losses(amd, scenes_mod):
+    import torch
+    import autograd as pad
+    T = 90
+    cases = [(24, 1), (37, 2), (30, 2)]
+    scs = [scenes_mod.c3_scene(n, lv, seed=n, steps=T) for n, lv in cases]
+    lis = [[(3 * sc.res // 4 + 2, sc.res // 2, sc.res // 2), (sc.res // 2, 2 + sc.res // 8, sc.res // 2)]
+           for sc in scs]
+    print(f"batched checkpointed loss: {len(scs)} scenes, {T} steps, 2 listeners", flush=True)
+    rng = np.random.default_rng(3)
+    targets = [torch.from_numpy(rng.standard_normal((T, 2)).astype(np.float32) * 1e-4).cuda() for _ in scs]
+
+    def solver(sc):
+        g = amd.PPPMSolver(sc.res, sc.dl, sc.dt, sc.min_pos, 0, 0.83, sc.c)
+        g.set_mesh(sc.verts, sc.tris)
+        return g
+
+    sv = [solver(sc) for sc in scs]
+    b = amd.PPPMBatch(sv)
+    xs = [torch.from_numpy(sc.neumann).cuda().requires_grad_(True) for sc in scs]
+    J = pad.batch_listener_loss(b, sv, xs, targets, lis, segment=13)
+    J.sum().backward()
+    for i, sc in enumerate(scs):
+        s1 = solver(sc)
+        x1 = torch.from_numpy(sc.neumann).cuda().requires_grad_(True)
+        J1 = pad.listener_loss(x1, targets[i], s1, lis[i], segment=13)
+        J1.backward()
+        assert abs(float(J1) - float(J[i])) <= 1e-6 * abs(float(J1)), i
+        assert torch.equal(x1.grad, xs[i].grad), i
+        assert float(x1.grad.abs().max()) > 0
+
+
+def test_adjoint_identity_on_a_perturbed_l5_scene(amd, scenes_mod):
+    import torch
+    T = 160
+    sc = scenes_mod.c5_scene(0, steps=T, res=64)
+    lis = sc.info["listeners"]
+    print(f"adjoint identity: {sc.res}^3, {len(sc.tris)} triangles, {len(lis)} listeners, {T} steps", flush=True)
+    s = amd.PPPMSolver(sc.res, sc.dl, sc.dt, sc.min_pos, 0, 0.83, sc.c)
+    s.set_mesh(sc.verts, sc.tris)
+    g = torch.from_numpy(sc.info["target_acc"]).cuda()
+    out = torch.zeros(T, len(lis), device="cuda")
+    s.run(g, T, lis, out)
+    y = torch.from_numpy(np.random.default_rng(4).standard_normal((T, len(lis))).astype(np.float32)).cuda()
+    grad = torch.zeros(T, len(sc.tris), device="cuda")
+    s.adjoint_reset(T - 1)
+    s.adjoint_run(y, T, lis, grad)
+    lhs = float((y.double() * out.double()).sum())
+    rhs = float((grad.double() * g.double()).sum())
+    rel = abs(lhs - rhs) / abs(lhs)
+    print(f"  <y, F g> = {lhs:.6e}, <F^T y, g> = {rhs:.6e}, relative difference {rel:.2e}")
+    assert abs(lhs) > 0 and rel < 1e-3
