@@ -112,7 +112,7 @@ def measure_traffic(kernels, workload_args):
         cmd = ["rocprofv3", "--pmc", counter, "--kernel-include-regex", "|".join(kernels) + "|copyBuffer",
                "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable, os.path.abspath(__file__),
                "--steps", "20", "--warmup", "2", "--timing-steps", "2", "--no-cpu", "--no-traffic",
-               "--batch-steps", "0", "--adjoint-steps", "0",
+               "--batch-steps", "0", "--adjoint-steps", "0", "--batched-adjoint-steps", "0",
                "--calib-copy-mib", str(CALIB_MIB)] + workload_args
         try:
             subprocess.run(cmd, env=env, cwd=ROOT, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
@@ -353,8 +353,11 @@ def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11):
     t_fwd = time.perf_counter() - t0
     fwd_out0 = out[:, 0].clone()
     gen = torch.Generator(device=dev).manual_seed(seed_dir)
-    y = [torch.randn(steps, L, device=dev, generator=gen) for _ in range(n_scenes)]
+    # cotangents y_i = F_i g_i (so <y, F g> = |F g|^2 has no cancellation)
+    y = [out[:, i].clone() for i in range(n_scenes)]
     grads = [torch.zeros(steps, nt, device=dev) for _ in range(n_scenes)]
+    for s_ in sv:  # the transposed lists are built once per mesh, outside the timing
+        s_.adjoint_reset(steps - 1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i, (s_, yy, gg) in enumerate(zip(sv, y, grads)):
@@ -398,9 +401,9 @@ def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11):
             alpha = float((gr[i].double() ** 2).sum()) / den if den > 0 else 0.0
             gs[i] = (xs[i].detach() - alpha * gr[i]).contiguous()
         del xs, gr
-    # directional derivative on scene 0 at the last iterate before its update
-    d = torch.randn(steps, nt, device=dev, generator=gen)
-    g_prev = gs[0]  # any point works for a quadratic; take the current iterate
+    # directional derivative on scene 0 at the current iterate (any point works for a quadratic),
+    # along a random direction with a component along the gradient (no cancellation in <grad, d>)
+    g_prev = gs[0]
 
     def J0(g):
         o = torch.zeros(steps, L, device=dev)
@@ -411,6 +414,9 @@ def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11):
     xs0 = g_prev.detach().requires_grad_(True)
     Jc = pad.listener_loss(xs0, targets[0], sv[0], lis[0])
     Jc.backward()
+    gn = xs0.grad.detach()
+    d = torch.randn(steps, nt, device=dev, generator=gen) * float(gn.abs().mean()) - gn
+    d = d / float(d.abs().max())
     fd = (J0(g_prev + d) - J0(g_prev - d)) / 2.0
     an = float((xs0.grad.double() * d.double()).sum())
     dir_rel = abs(fd - an) / max(abs(an), 1e-300)
@@ -572,6 +578,9 @@ def main():
                     help="timed batch steps of the 64-scene batched leg (0: leg skipped)")
     ap.add_argument("--adjoint-steps", type=int, default=2000,
                     help="steps of the adjoint leg (48^3, icosphere L3; 0: leg skipped)")
+    ap.add_argument("--batched-adjoint-steps", type=int, default=256,
+                    help="steps of the reduced batched-adjoint leg (4 x 64^3, perturbed icosphere L5, "
+                         "3 gradient-descent iterations; 0: leg skipped)")
     args = ap.parse_args()
     if args.modal:
         modal_bench(cpu=not args.no_cpu)
@@ -623,9 +632,10 @@ def main():
     def run_steps(i0, n, use_graph=True):
         if moving:
             # the demo loop's per-sample UpdateSound: update_mesh, then update_grid_and_face
-            # (experiments/demo/main.cu:170-205), as one call; rows are already in HBM
-            for i in range(i0, i0 + n):
-                solver.update_mesh_and_step(verts_all[i], neumann[i])
+            # (experiments/demo/main.cu:170-205), for every step of the window in one call
+            # (pppm_run_moving: the update's status is read while the step runs); the vertex and
+            # Neumann rows are already in HBM
+            solver.run_moving(verts_all[i0:i0 + n], neumann[i0:i0 + n], n, lis, out[i0:i0 + n])
         else:
             solver.run(neumann[i0:i0 + n], n, lis, out[i0:i0 + n], use_graph=use_graph)
 
@@ -751,6 +761,10 @@ def main():
         res["batched"] = batch_bench(steps=args.batch_steps)
     if args.adjoint_steps > 0:
         res["adjoint"] = adjoint_bench(steps=args.adjoint_steps)
+    if args.batched_adjoint_steps > 0:
+        # the batched adjoint at a reduced size (the full 16 x 128^3 x 8192-step workload runs with
+        # --workload c5_batched_adjoint)
+        res["batched_adjoint"] = c5_bench(n_scenes=4, steps=args.batched_adjoint_steps, iters=3, res=64)
     if not args.no_cpu:
         cpu_rate, cpu_s = cpu_baseline(sc, args.cpu_steps, args.cpu_threads)
         one_steps = max(1, args.cpu_steps // 4)

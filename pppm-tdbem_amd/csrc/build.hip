@@ -345,7 +345,13 @@ __global__ __launch_bounds__(B) void k_b4_lists(ListArgs a, int fill)
                     a.cell_items, a.N, a.b4_start, a.cap_b4items, a.b4_tris, nullptr, a.err);
 }
 
-__global__ void k_advance(StepState *st) { st->t = st->t + 1; }
+// skip (may be null): a mesh update's error word; a step behind a failed update does nothing
+__global__ void k_advance(StepState *st, const int *skip)
+{
+    if (skip && *skip)
+        return;
+    st->t = st->t + 1;
+}
 
 inline int nblk(long long n) { return n > 0 ? (int)((n + B - 1) / B) : 1; }
 inline int nblk_waves(long long waves) { return waves > 0 ? (int)((waves + WPB - 1) / WPB) : 1; }
@@ -399,9 +405,9 @@ hipError_t launch_b4_lists(const ListArgs &a, int fill, hipStream_t s)
     k_b4_lists<<<nblk_strided(a.nt), B, 0, s>>>(a, fill);
     return hipGetLastError();
 }
-hipError_t launch_advance(StepState *st, hipStream_t s)
+hipError_t launch_advance(StepState *st, hipStream_t s, const int *skip)
 {
-    k_advance<<<1, 1, 0, s>>>(st);
+    k_advance<<<1, 1, 0, s>>>(st, skip);
     return hipGetLastError();
 }
 

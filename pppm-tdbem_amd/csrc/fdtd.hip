@@ -206,8 +206,11 @@ struct MarchShape
 };
 
 __global__ __launch_bounds__(MBLK) void k_fdtd_march(float *__restrict__ grid3, FdtdParams p,
-                                                     const StepState *__restrict__ st, MarchShape ms)
+                                                     const StepState *__restrict__ st, MarchShape ms,
+                                                     const int *__restrict__ skip)
 {
+    if (skip && *skip)
+        return;
     const int N = p.N;
     const size_t nn = (size_t)N * N * N;
     const int t = st->t;
@@ -340,8 +343,10 @@ __device__ __forceinline__ void fdtd_block(float *__restrict__ grid3, const Fdtd
 }
 
 __global__ __launch_bounds__(BLK) void k_fdtd_abc(float *__restrict__ grid3, FdtdParams p, const StepState *__restrict__ st,
-                                                  int nb_inner)
+                                                  int nb_inner, const int *__restrict__ skip)
 {
+    if (skip && *skip)
+        return;
     fdtd_block(grid3, p, st->t, (int)blockIdx.x, nb_inner);
 }
 
@@ -440,8 +445,10 @@ __device__ inline float pml_damp(const FdtdParams &p, int x, int y, int z)
 
 // pml_step_kernel (fdtd.cu:179-232); PML state: 6 floats per cell (v1[3], v2[3]) per level
 __global__ __launch_bounds__(BLK) void k_fdtd_pml(float *__restrict__ grid3, float *__restrict__ pml3, FdtdParams p,
-                                                  const StepState *__restrict__ st)
+                                                  const StepState *__restrict__ st, const int *__restrict__ skip)
 {
+    if (skip && *skip)
+        return;
     const int N = p.N;
     const size_t nn = (size_t)N * N * N;
     const int t = st->t;
@@ -563,14 +570,15 @@ hipError_t launch_batch_tile(int variant, const FdtdScene *ds, const BatchMap &m
     return hipGetLastError();
 }
 
-hipError_t launch_fdtd(float *grid3, float *pml3, const FdtdParams &p, const StepState *st, hipStream_t s)
+hipError_t launch_fdtd(float *grid3, float *pml3, const FdtdParams &p, const StepState *st, hipStream_t s,
+                       const int *skip)
 {
     const int N = p.N;
     if (p.pml > 0)
     {
         long long cells = (long long)(N - 2) * (N - 2) * (N - 2);
         int nb = (int)((cells + BLK - 1) / BLK);
-        k_fdtd_pml<<<nb, BLK, 0, s>>>(grid3, pml3, p, st);
+        k_fdtd_pml<<<nb, BLK, 0, s>>>(grid3, pml3, p, st, skip);
         return hipGetLastError();
     }
     const long long M = N - 4, S = N - 2;
@@ -586,13 +594,13 @@ hipError_t launch_fdtd(float *grid3, float *pml3, const FdtdParams &p, const Ste
         ms.xch = (int)std::max<long long>(16, (M + nxc - 1) / nxc);
         ms.n_tiles = (int)(cols * ((M + ms.xch - 1) / ms.xch));
         ms.n_blocks = (ms.n_tiles + 7) / 8 * 8;
-        k_fdtd_march<<<ms.n_blocks, MBLK, 0, s>>>(grid3, p, st, ms);
+        k_fdtd_march<<<ms.n_blocks, MBLK, 0, s>>>(grid3, p, st, ms, skip);
         nb_inner = 0;
     }
     const long long zt = (S - 2 + 15) / 16;
     const long long fxy = 2 * S * S + 2 * (S - 2) * S, fz = 2 * zt * zt * 256;
     const int nb_shell = (int)((fxy + BLK - 1) / BLK) + 8 * (int)((fz + 8LL * BLK - 1) / (8LL * BLK));
-    k_fdtd_abc<<<nb_inner + nb_shell, BLK, 0, s>>>(grid3, p, st, nb_inner);
+    k_fdtd_abc<<<nb_inner + nb_shell, BLK, 0, s>>>(grid3, p, st, nb_inner, skip);
     return hipGetLastError();
 }
 

@@ -37,7 +37,9 @@ struct FdtdParams
 };
 
 // ---------------------------------------------------------------- fdtd.hip
-hipError_t launch_fdtd(float *grid3, float *pml3, const FdtdParams &p, const StepState *st, hipStream_t s);
+// skip (may be null): as for launch_advance
+hipError_t launch_fdtd(float *grid3, float *pml3, const FdtdParams &p, const StepState *st, hipStream_t s,
+                       const int *skip = nullptr);
 // One scene of a batched FDTD step. nb_inner: blocks of its one-thread-per-cell interior in the
 // batched shell launch (0: the interior is marched by a tile launch); xch: x planes per tile.
 struct FdtdScene
@@ -149,7 +151,9 @@ hipError_t launch_lists_count(const ListArgs &a, hipStream_t s);
 hipError_t launch_lists_fill(const ListArgs &a, hipStream_t s);
 // 4x4x4 lists of the non-empty base cells: fill = 0 counts (b4_cnt), fill = 1 writes b4_tris
 hipError_t launch_b4_lists(const ListArgs &a, int fill, hipStream_t s);
-hipError_t launch_advance(StepState *st, hipStream_t s);
+// skip (may be null): the error word of a mesh update the step was enqueued behind; if it is set
+// when the kernel runs, the kernel does nothing (the step is run again after the update is)
+hipError_t launch_advance(StepState *st, hipStream_t s, const int *skip = nullptr);
 
 // ---------------------------------------------------------------- weights.hip (FP64 quadrature)
 struct Tdbem

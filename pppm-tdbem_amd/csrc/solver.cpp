@@ -434,14 +434,32 @@ int grow(pppm_solver *s)
     return ensure_pool(s, s->pool_count + 2 * s->cap_p4);
 }
 
-// synchronous update used by set_mesh / update_mesh: runs the sequence, retries once with
-// grown capacities if a list or the pool overflowed
-// vsrc (device, may be null): vertex positions to take instead of s->verts; the first
-// attempt copies them into s->verts
-int rebuild_attempts(pppm_solver *s, bool first, const float *vsrc)
+// status bookkeeping after the status block has arrived in st_h
+void take_status(pppm_solver *s)
 {
-    // the last update's face lists become the previous ones; this update writes the others
-    bool use_prev = !first && s->faces_valid;
+    s->err_clean = s->st_h[ST_ERR] == 0;
+    s->pool_count = s->st_h[ST_POOL];
+    s->last_grid_recomputes = s->st_h[ST_GDIRTY];
+    s->last_face_recomputes = s->st_h[ST_FDIRTY];
+}
+
+// a completed update: the face lists are valid, and its work counts while timing is on
+void update_done(pppm_solver *s)
+{
+    s->faces_valid = true;
+    if (s->timing)
+    {
+        s->work_updates++;
+        s->work_grid += s->st_h[ST_GDIRTY];
+        s->work_face += s->st_h[ST_FDIRTY];
+    }
+}
+
+// the last update's face lists become the previous ones (the lookup reuses their slots) and this
+// update writes the others; returns whether the previous lists are valid
+bool begin_update(pppm_solver *s, bool first)
+{
+    const bool use_prev = !first && s->faces_valid;
     if (use_prev)
     {
         s->face_start.swap(s->prev_face_start);
@@ -449,40 +467,55 @@ int rebuild_attempts(pppm_solver *s, bool first, const float *vsrc)
         s->face_slot.swap(s->prev_face_slot);
     }
     s->faces_valid = false;
-    for (int attempt = 0; attempt < 3; attempt++)
+    return use_prev;
+}
+
+// enqueues one attempt of the update sequence (attempt 0 takes the vertices vsrc if given)
+int enqueue_attempt(pppm_solver *s, bool first, const float *vsrc, bool use_prev)
+{
+    // the error word is cleared only when the last status read did not show it clean (no
+    // kernel outside a mesh update sets it)
+    if (!s->err_clean)
+        CK(launch_fill_int(s->status.p + ST_ERR, 1, 0, s->stream));
+    s->err_clean = false;
+    if (first)
     {
-        // the error word is cleared only when the last status read did not show it clean (no
-        // kernel outside a mesh update sets it)
-        if (!s->err_clean)
-            CK(launch_fill_int(s->status.p + ST_ERR, 1, 0, s->stream));
-        s->err_clean = false;
-        if (first)
-        {
-            // fresh caches (GridCache::init / FaceCache::init, pppm_cache.h:29-33, 115-124)
-            s->pool_count = 0;
-            CK(launch_fill_u64(s->hkeys.p, s->hcap, ~0ull, s->stream));
-            CK(launch_fill_int(s->status.p + ST_POOL, 1, 0, s->stream));
-        }
-        RC(enqueue_rebuild(s, first, nullptr, 0, 0, attempt == 0 ? vsrc : nullptr, use_prev));
+        // fresh caches (GridCache::init / FaceCache::init, pppm_cache.h:29-33, 115-124)
+        s->pool_count = 0;
+        CK(launch_fill_u64(s->hkeys.p, s->hcap, ~0ull, s->stream));
+        CK(launch_fill_int(s->status.p + ST_POOL, 1, 0, s->stream));
+    }
+    return enqueue_rebuild(s, first, nullptr, 0, 0, vsrc, use_prev);
+}
+
+// synchronous update attempts a0..2 (used by set_mesh / update_mesh): runs the sequence, reads
+// the status, retries with grown capacities if a list or the pool overflowed
+int rebuild_attempts_from(pppm_solver *s, bool first, const float *vsrc, int a0, bool use_prev)
+{
+    for (int attempt = a0; attempt < 3; attempt++)
+    {
+        RC(enqueue_attempt(s, first, attempt == 0 ? vsrc : nullptr, use_prev));
         RC(read_status(s));
         const int e = s->st_h[ST_ERR];
         if ((e & ~ERR_CAPACITY) != 0)
             return status_error(s);
         if (!(e & ERR_CAPACITY) && !needs_growth(s))
         {
-            s->faces_valid = true;
-            if (s->timing)
-            {
-                s->work_updates++;
-                s->work_grid += s->st_h[ST_GDIRTY];
-                s->work_face += s->st_h[ST_FDIRTY];
-            }
+            update_done(s);
             return PPPM_OK;
         }
         RC(grow(s));
         use_prev = false;  // grow may have moved the previous lists
     }
     return status_error(s);
+}
+
+// vsrc (device, may be null): vertex positions to take instead of s->verts; the first
+// attempt copies them into s->verts
+int rebuild_attempts(pppm_solver *s, bool first, const float *vsrc)
+{
+    const bool use_prev = begin_update(s, first);
+    return rebuild_attempts_from(s, first, vsrc, 0, use_prev);
 }
 
 int rebuild_sync(pppm_solver *s, bool first, const float *vsrc = nullptr)
@@ -580,12 +613,14 @@ int enqueue_step(pppm_solver *s, const StepBufs &b, const float *neumann, int t0
 {
     const FdtdParams fp = fdtd_params(s);
     const SceneStep d = scene_step(s, b, neumann, t0, stride, n_listen, out);
+    // a step enqueued behind a mesh update does nothing if that update raises an error bit
+    const int *skip = s->mesh_set ? d.err : nullptr;
     int sp = timed ? span_begin(s, 0) : -1;
-    CK(launch_fdtd(s->grid.p, s->pmlbuf.p, fp, s->state.p, s->stream));
+    CK(launch_fdtd(s->grid.p, s->pmlbuf.p, fp, s->state.p, s->stream, skip));
     span_end(s, sp);
     if (s->mesh_set)
         CK(launch_windows(d, 0, s->stream));
-    CK(launch_advance(s->state.p, s->stream));
+    CK(launch_advance(s->state.p, s->stream, skip));
     if (!s->mesh_set)
     {
         if (out)
@@ -607,6 +642,63 @@ int enqueue_step(pppm_solver *s, const StepBufs &b, const float *neumann, int t0
 }
 
 int check_handle(pppm_solver *s) { return s ? PPPM_OK : PPPM_ERR_ARG; }
+
+// update_mesh followed by update_grid_and_face (the demo loop's UpdateSound) with the status read
+// overlapped with the step: the update, a copy of its status block and the step are enqueued
+// together; the step's kernels do nothing if the update raised an error bit, and the host waits
+// only for the status. Then, as the synchronous update: an invalid mesh returns its error (the
+// step did nothing); an overflowed list or pool grows and the update runs again, followed by the
+// step again; a pool short of headroom for the next update grows. The results are those of
+// pppm_update_mesh + pppm_step. vsrc: device vertices (null: s->verts already holds them);
+// neumann: device rows (row t - t0, stride; null: the ring slot is written elsewhere); out:
+// listener samples by a direct read (may be null). The caller advances t_host when it returns OK.
+int update_and_step(pppm_solver *s, const float *vsrc, const float *neumann, int t0, long long stride, int n_listen,
+                    float *out)
+{
+    if (!s->ev_status.e)
+        CK(hipEventCreateWithFlags(&s->ev_status.e, hipEventDisableTiming));
+    const bool use_prev = begin_update(s, false);
+    int sp = span_begin(s, 4);
+    RC(enqueue_attempt(s, false, vsrc, use_prev));
+    CK(hipMemcpyAsync(s->st_h, s->status.p, ST_N * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    CK(hipEventRecord(s->ev_status.e, s->stream));
+    span_end(s, sp);
+    auto step = [&]() -> int {
+        RC(enqueue_step(s, own_bufs(s), neumann, t0, stride, n_listen, nullptr));
+        if (out)
+            CK(launch_listen_direct(scene_step(s, own_bufs(s), nullptr, t0, stride, n_listen, out), s->stream));
+        return PPPM_OK;
+    };
+    RC(step());
+    CK(hipEventSynchronize(s->ev_status.e));
+    take_status(s);
+    const int e = s->st_h[ST_ERR];
+    if (e == 0)
+    {
+        update_done(s);
+        s->mesh_stale = false;
+        s->mesh_version++;
+        if (needs_growth(s))
+        {
+            CK(hipStreamSynchronize(s->stream));  // the step's kernels use the buffers grow may move
+            RC(grow(s));
+        }
+        return PPPM_OK;
+    }
+    CK(hipStreamSynchronize(s->stream));  // the step's kernels are done (they did nothing)
+    if ((e & ~ERR_CAPACITY) != 0)
+    {
+        s->mesh_stale = true;
+        return status_error(s);
+    }
+    RC(grow(s));
+    const int rc = rebuild_attempts_from(s, false, nullptr, 1, false);
+    s->mesh_stale = rc != PPPM_OK;
+    if (rc)
+        return rc;
+    s->mesh_version++;
+    return step();
+}
 
 // the weight records of the current mesh version in gather-list order (for static meshes; a
 // stale copy is never used: scene_step compares the versions)
@@ -690,16 +782,13 @@ int run_impl(pppm_solver *s, const float *verts_series, long long vstride, const
     if (verts_series && !kAsyncMovingStep)
         use_graph = 0;
     auto one_step = [&](int i) -> int {
+        if (verts_series && !kAsyncMovingStep)
+            // the synchronous update with its status read overlapped with the step
+            return update_and_step(s, verts_series + (size_t)i * vstride, neumann_series, t0, stride, n_listen, out);
         if (verts_series)
         {
             const int sp = span_begin(s, 4);
-            if (kAsyncMovingStep)
-                RC(enqueue_rebuild(s, false, verts_series, vstride, t0));
-            else
-            {
-                RC(upload(s, s->verts.p, verts_series + (size_t)i * vstride, (size_t)s->nv * 3 * sizeof(float), 1));
-                RC(rebuild_sync(s, false));
-            }
+            RC(enqueue_rebuild(s, false, verts_series, vstride, t0));
             span_end(s, sp);
         }
         if (!verts_series)
@@ -722,8 +811,13 @@ int run_impl(pppm_solver *s, const float *verts_series, long long vstride, const
     {
         for (; done < steps; done++)
         {
-            RC(one_step(done));
-            if (verts_series && (done % 64 == 63 || done == steps - 1))
+            const int rc = one_step(done);
+            if (rc)
+            {
+                s->t_host += done;  // the steps before this one ran
+                return rc;
+            }
+            if (verts_series && kAsyncMovingStep && (done % 64 == 63 || done == steps - 1))
             {
                 RC(read_status(s));
                 RC(status_error(s));
@@ -1079,8 +1173,23 @@ int pppm_update_mesh_and_step(pppm_solver *s, const float *verts, const float *n
 {
     if (check_handle(s))
         return PPPM_ERR_ARG;
-    RC(pppm_update_mesh(s, verts, on_device));
-    return pppm_step(s, neumann, on_device);
+    if (!s->mesh_set)
+        return fail(s, PPPM_ERR_STATE, "update_mesh before set_mesh");
+    if (!verts)
+        return fail(s, PPPM_ERR_ARG, "update_mesh_and_step: null vertices");
+    const float *nd = neumann;
+    if (!on_device)
+    {
+        RC(upload(s, s->verts.p, verts, (size_t)s->nv * 3 * sizeof(float), 0));
+        if (neumann)
+        {
+            RC(upload(s, s->nstage.p, neumann, (size_t)s->nt * sizeof(float), 0));
+            nd = s->nstage.p;
+        }
+    }
+    RC(update_and_step(s, on_device ? verts : nullptr, nd, s->t_host + 1, 0, 0, nullptr));
+    s->t_host++;
+    return PPPM_OK;
 }
 
 int pppm_fdtd_step(pppm_solver *s)

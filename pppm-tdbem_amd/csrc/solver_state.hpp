@@ -191,6 +191,16 @@ struct pppm_solver
         }
     } st_pin;
     int *st_h = nullptr;
+    // recorded behind the status copy of an update whose step is enqueued before the status is read
+    struct StatusEvent
+    {
+        hipEvent_t e = nullptr;
+        ~StatusEvent()
+        {
+            if (e)
+                (void)hipEventDestroy(e);
+        }
+    } ev_status;
     long long last_grid_recomputes = 0, last_face_recomputes = 0;
 
     // mesh version: counts successful mesh updates (the adjoint's transposed lists belong to one)

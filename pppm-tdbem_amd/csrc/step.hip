@@ -140,7 +140,7 @@ __device__ inline float conv_pair(const uint32_t *__restrict__ w16, float2 sc, c
 // into row 15 as into the ring), 1 after it (the phase entry points: the ring holds them).
 __device__ __forceinline__ void windows_item(const SceneStep &d, int back, int i)
 {
-    if (i >= d.a.nt * STEPS)
+    if (i >= d.a.nt * STEPS || (d.err && *d.err))
         return;
     const int tri = i / STEPS, k = i % STEPS;
     const int tn = d.st->t - back + 1;  // t of the step the gathers serve
@@ -154,6 +154,8 @@ __device__ __forceinline__ void windows_item(const SceneStep &d, int back, int i
 // of step t is therefore free to be written by the extra waves of this launch.
 __device__ __forceinline__ void far_wave(const SceneStep &d, int wave, int lane, float *__restrict__ vals)
 {
+    if (d.err && *d.err)  // behind a mesh update that failed: the step is run again after the update is
+        return;
     const int t = d.st->t;
     if (wave < d.c3_waves)
     {
@@ -293,6 +295,8 @@ __device__ __forceinline__ void dirichlet_wave(const SceneStep &d, int wave, int
 // p(c) = far + 0.98 * sum W^bem (*) (N, D) over all 16 taps; records listener samples.
 __device__ __forceinline__ void near_wave(const SceneStep &d, int wave, int lane, float *__restrict__ vals)
 {
+    if (d.err && *d.err)
+        return;
     const int t = d.st->t;
     const size_t nn = (size_t)d.a.N * d.a.N * d.a.N;
     float *g = d.grid3 + (size_t)(((t % 3) + 3) % 3) * nn;
@@ -380,7 +384,7 @@ __global__ void k_listen_direct(SceneStep d)
 {
     const int t = d.st->t;
     const int l = threadIdx.x;
-    if (l >= d.a.n_listen)
+    if (l >= d.a.n_listen || (d.err && *d.err))
         return;
     const size_t nn = (size_t)d.a.N * d.a.N * d.a.N;
     const float *g = d.grid3 + (size_t)(((t % 3) + 3) % 3) * nn;
