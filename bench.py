@@ -318,9 +318,15 @@ def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11):
     import scenes
     import autograd as pad
 
+    def note(msg):  # progress on stderr (the JSON line stays alone on stdout)
+        print(f"[c5 {time.perf_counter() - t_start:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
     t_start = time.perf_counter()
     free0, total = torch.cuda.mem_get_info()
-    scs = [scenes.c5_scene(i, steps=steps, res=res) for i in range(n_scenes)]
+    scs = []
+    for i in range(n_scenes):
+        scs.append(scenes.c5_scene(i, steps=steps, res=res))
+        note(f"scene {i + 1}/{n_scenes} generated")
     t_gen = time.perf_counter() - t_start
     t0 = time.perf_counter()
     sv = []
@@ -330,6 +336,7 @@ def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11):
         sv.append(g)
     torch.cuda.synchronize()
     t_setmesh = time.perf_counter() - t0
+    note(f"set_mesh of {n_scenes} scenes: {t_setmesh:.1f} s")
     batch = pppm_amd.PPPMBatch(sv)
     lis = [sc.info["listeners"] for sc in scs]
     L, nt = len(lis[0]), len(scs[0].tris)
@@ -343,6 +350,7 @@ def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11):
         s_.reset()
     batch.run(true, steps, lis, out)
     targets = [out[:, i].clone() for i in range(n_scenes)]
+    note("targets computed")
     # plain batched forward and per-scene adjoint sweeps over the whole length (no checkpoints)
     for s_ in sv:
         s_.reset()
@@ -351,6 +359,7 @@ def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11):
     batch.run(true, steps, lis, out)
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
+    note(f"batched forward: {t_fwd:.1f} s")
     fwd_out0 = out[:, 0].clone()
     gen = torch.Generator(device=dev).manual_seed(seed_dir)
     # cotangents y_i = F_i g_i (so <y, F g> = |F g|^2 has no cancellation)
@@ -365,6 +374,7 @@ def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11):
         s_.adjoint_run(yy, steps, lis[i], gg)
     torch.cuda.synchronize()
     t_adj = time.perf_counter() - t0
+    note(f"adjoint sweeps: {t_adj:.1f} s")
     # adjoint identity on scene 0: <y, F g> = <F^T y, g> with g = the seeded accelerations
     lhs = float((y[0].double() * fwd_out0.double()).sum())
     rhs = float((grads[0].double() * true[0].double()).sum())
@@ -389,6 +399,8 @@ def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11):
         t_back += t2 - t1
         ck_bytes = sum(sv_.state_bytes() for sv_ in sv) * math.ceil(steps / math.ceil(math.sqrt(steps)))
         hist.append(J.detach().double().cpu().numpy())
+        note(f"iteration {it + 1}/{iters}: loss / initial {float((hist[-1] / hist[0]).mean()):.4f}; "
+             f"forward {t1 - t0:.1f} s, recompute + adjoint {t2 - t1:.1f} s")
         gr = [x.grad.detach() for x in xs]
         # exact line search along -grad: alpha_i = |grad_i|^2 / (2 |F_i grad_i|^2)
         for s_ in sv:

@@ -655,13 +655,14 @@ int check_handle(pppm_solver *s) { return s ? PPPM_OK : PPPM_ERR_ARG; }
 int update_and_step(pppm_solver *s, const float *vsrc, const float *neumann, int t0, long long stride, int n_listen,
                     float *out)
 {
-    if (!s->ev_status.e)
-        CK(hipEventCreateWithFlags(&s->ev_status.e, hipEventDisableTiming));
+    auto &c = s->stc;
+    if (!c.ev_status)
+        CK(hipEventCreateWithFlags(&c.ev_status, hipEventDisableTiming));
     const bool use_prev = begin_update(s, false);
     int sp = span_begin(s, 4);
     RC(enqueue_attempt(s, false, vsrc, use_prev));
     CK(hipMemcpyAsync(s->st_h, s->status.p, ST_N * sizeof(int), hipMemcpyDeviceToHost, s->stream));
-    CK(hipEventRecord(s->ev_status.e, s->stream));
+    CK(hipEventRecord(c.ev_status, s->stream));
     span_end(s, sp);
     auto step = [&]() -> int {
         RC(enqueue_step(s, own_bufs(s), neumann, t0, stride, n_listen, nullptr));
@@ -670,7 +671,7 @@ int update_and_step(pppm_solver *s, const float *vsrc, const float *neumann, int
         return PPPM_OK;
     };
     RC(step());
-    CK(hipEventSynchronize(s->ev_status.e));
+    CK(hipEventSynchronize(c.ev_status));
     take_status(s);
     const int e = s->st_h[ST_ERR];
     if (e == 0)

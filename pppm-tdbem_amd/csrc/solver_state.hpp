@@ -191,16 +191,18 @@ struct pppm_solver
         }
     } st_pin;
     int *st_h = nullptr;
-    // recorded behind the status copy of an update whose step is enqueued before the status is read
-    struct StatusEvent
+    // update_and_step: recorded behind the status copy of an update whose step is enqueued before
+    // the status is read (a copy on a second stream waiting for the update was slower: the
+    // step's kernels queued behind the cross-stream wait)
+    struct StatusCopy
     {
-        hipEvent_t e = nullptr;
-        ~StatusEvent()
+        hipEvent_t ev_status = nullptr;
+        ~StatusCopy()
         {
-            if (e)
-                (void)hipEventDestroy(e);
+            if (ev_status)
+                (void)hipEventDestroy(ev_status);
         }
-    } ev_status;
+    } stc;
     long long last_grid_recomputes = 0, last_face_recomputes = 0;
 
     // mesh version: counts successful mesh updates (the adjoint's transposed lists belong to one)

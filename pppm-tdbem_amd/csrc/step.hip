@@ -177,8 +177,8 @@ __device__ __forceinline__ void far_wave(const SceneStep &d, int wave, int lane,
         for (int base = q0; base < q1; base += WAVE)
         {
             const int p = base + lane;
-            if (base != q0)
-                rec = p < q1 ? d.c3_pairs[p] : -1;
+            // the next round's index is loaded now, beside this round's records
+            const int rec_next = p + WAVE < q1 ? d.c3_pairs[p + WAVE] : -1;
             float v = 0.0f;
             if (p < q1)  // rec >= 0 exactly then; the list-ordered record does not wait for rec
             {
@@ -191,6 +191,7 @@ __device__ __forceinline__ void far_wave(const SceneStep &d, int wave, int lane,
             wave_sync();
             acc = round_sum(acc, vals, max(m0, base) - base, min(m1, base + WAVE) - base);
             wave_sync();
+            rec = rec_next;
         }
         if (leader)
             d.far[cs] = cell < nn ? gv - acc : 0.0f;
@@ -254,14 +255,12 @@ __device__ __forceinline__ void dirichlet_wave(const SceneStep &d, int wave, int
     for (int base = q0; base < q1; base += WAVE)
     {
         const int p = base + lane;
-        if (base != q0)
+        // the next round's indices are loaded now, beside this round's records
+        int j_next = -1, ws_next = 0;
+        if (p + WAVE < q1)
         {
-            j = -1;
-            if (p < q1)
-            {
-                j = d.face_nbr[p];
-                ws = d.face_slot[p];
-            }
+            j_next = d.face_nbr[p + WAVE];
+            ws_next = d.face_slot[p + WAVE];
         }
         float v = 0.0f, f = 0.0f;
         if (p < q1)  // j >= 0 exactly then; the list-ordered record does not wait for the slot
@@ -281,6 +280,8 @@ __device__ __forceinline__ void dirichlet_wave(const SceneStep &d, int wave, int
             fac = round_sum(fac, facs, lo, hi);
         }
         wave_sync();
+        j = j_next;
+        ws = ws_next;
     }
     if (leader)
     {
@@ -318,8 +319,8 @@ __device__ __forceinline__ void near_wave(const SceneStep &d, int wave, int lane
         for (int base = q0; base < q1; base += WAVE)
         {
             const int p = base + lane;
-            if (base != q0)
-                rec = p < q1 ? d.c3_pairs[p] : -1;
+            // the next round's index is loaded now, beside this round's records
+            const int rec_next = p + WAVE < q1 ? d.c3_pairs[p + WAVE] : -1;
             float v = 0.0f;
             if (p < q1)  // rec >= 0 exactly then; the list-ordered record does not wait for rec
             {
@@ -332,6 +333,7 @@ __device__ __forceinline__ void near_wave(const SceneStep &d, int wave, int lane
             wave_sync();
             acc = round_sum(acc, vals, max(m0, base) - base, min(m1, base + WAVE) - base);
             wave_sync();
+            rec = rec_next;
         }
         if (leader)
         {

@@ -283,17 +283,14 @@ def test_c2_moving_update_bitexact(amd, oracle_mod, scenes_mod):
         assert np.array_equal(a, b)
 
 
-RUN_MOVING_OFF = (
-    "run_moving took a device memory fault before the wave-map fix (the last gather wave owned "
-    "the unwritten capacity tail of the C3 list and used its entries as grid indices); the "
-    "moving step is verified through update_mesh + step first and this case stays off the device "
-    "until then")
+# pppm_run_moving performs, before each step, the synchronous update of pppm_update_mesh (its
+# status read overlapped with the step); the graph flag is ignored for moving meshes (the
+# graph-captured moving step stays off, see the design notes on the earlier device fault).
 
 
-@pytest.mark.skip(reason=RUN_MOVING_OFF)
 def test_c2_moving_run_bitexact(amd, oracle_mod, scenes_mod):
-    """Config 2 (moving sphere + static cube, 64^3), first 40 steps: the graph-captured moving
-    run (update_mesh + step per step, all on the device) vs the oracle, bit-exact."""
+    """Config 2 (moving sphere + static cube, 64^3), first 40 steps: pppm_run_moving (update_mesh
+    + step per step in one call) vs the oracle, bit-exact."""
     import torch
     steps = 40
     sc = scenes_mod.c2_moving(steps=steps)
@@ -316,7 +313,6 @@ def test_c2_moving_run_bitexact(amd, oracle_mod, scenes_mod):
     assert st_g["C3"] == st_o["C3"] and st_g["P4"] == st_o["P4"]
 
 
-@pytest.mark.skip(reason=RUN_MOVING_OFF)
 def test_run_moving_graph_equals_eager(amd, scenes_mod):
     import torch
     steps = 24
