@@ -342,12 +342,31 @@ __device__ __forceinline__ void fdtd_block(float *__restrict__ grid3, const Fdtd
     }
 }
 
+// The FDTD of one scene (interior blocks, then the ABC shell) and, with a fuse, the history
+// windows of the step's gathers (the last blocks; they read only the rings, which no FDTD block
+// writes).
 __global__ __launch_bounds__(BLK) void k_fdtd_abc(float *__restrict__ grid3, FdtdParams p, const StepState *__restrict__ st,
-                                                  int nb_inner, const int *__restrict__ skip)
+                                                  int nb_inner, int nb_fdtd, const int *__restrict__ skip, StepFuse f)
 {
     if (skip && *skip)
         return;
-    fdtd_block(grid3, p, st->t, (int)blockIdx.x, nb_inner);
+    const int t = st->t;
+    const int blk = (int)blockIdx.x;
+    if (blk < nb_fdtd)
+        fdtd_block(grid3, p, t, blk, nb_inner);
+    else
+    {
+        // windows for the step that follows (t + 1): row k < 15 = ring value at t - k, row 15 =
+        // ring slot t + 1 (overwritten by the far-field and Dirichlet launches before use)
+        const int i = (blk - nb_fdtd) * BLK + (int)threadIdx.x;
+        if (i < f.nt * STEPS)
+        {
+            const int tri = i / STEPS, k = i % STEPS;
+            const int slot = (k == STEPS - 1 ? t + 1 : t - k) & (RING - 1);
+            f.wn[i] = f.hn[tri * RING + slot];
+            f.wd[i] = f.hd[tri * RING + slot];
+        }
+    }
 }
 
 // Interior of a small grid as LDS-staged 2.5D tiles: a block of TY x TZ threads owns TY x TZ
@@ -571,7 +590,7 @@ hipError_t launch_batch_tile(int variant, const FdtdScene *ds, const BatchMap &m
 }
 
 hipError_t launch_fdtd(float *grid3, float *pml3, const FdtdParams &p, const StepState *st, hipStream_t s,
-                       const int *skip)
+                       const int *skip, const StepFuse *fuse)
 {
     const int N = p.N;
     if (p.pml > 0)
@@ -600,7 +619,14 @@ hipError_t launch_fdtd(float *grid3, float *pml3, const FdtdParams &p, const Ste
     const long long zt = (S - 2 + 15) / 16;
     const long long fxy = 2 * S * S + 2 * (S - 2) * S, fz = 2 * zt * zt * 256;
     const int nb_shell = (int)((fxy + BLK - 1) / BLK) + 8 * (int)((fz + 8LL * BLK - 1) / (8LL * BLK));
-    k_fdtd_abc<<<nb_inner + nb_shell, BLK, 0, s>>>(grid3, p, st, nb_inner, skip);
+    StepFuse f{};
+    int nb_win = 0;
+    if (fuse && fuse->wn)
+    {
+        f = *fuse;
+        nb_win = (int)(((long long)f.nt * STEPS + BLK - 1) / BLK);
+    }
+    k_fdtd_abc<<<nb_inner + nb_shell + nb_win, BLK, 0, s>>>(grid3, p, st, nb_inner, nb_inner + nb_shell, skip, f);
     return hipGetLastError();
 }
 

@@ -37,9 +37,19 @@ struct FdtdParams
 };
 
 // ---------------------------------------------------------------- fdtd.hip
+// What the ABC-shell launch of a time step can do besides the FDTD: the gathers' history
+// windows for the step, in extra blocks (wn / wd as in step.hip's windows_item with back = 0).
+// wn == nullptr: nothing. (Advancing t in the same launch, by the last block to finish, cost more
+// than the one-thread kernel it saves: every block counted itself done on one atomic.)
+struct StepFuse
+{
+    const float *hn, *hd;
+    float *wn, *wd;
+    int nt;
+};
 // skip (may be null): as for launch_advance
 hipError_t launch_fdtd(float *grid3, float *pml3, const FdtdParams &p, const StepState *st, hipStream_t s,
-                       const int *skip = nullptr);
+                       const int *skip = nullptr, const StepFuse *fuse = nullptr);
 // One scene of a batched FDTD step. nb_inner: blocks of its one-thread-per-cell interior in the
 // batched shell launch (0: the interior is marched by a tile launch); xch: x planes per tile.
 struct FdtdScene

@@ -616,10 +616,20 @@ int enqueue_step(pppm_solver *s, const StepBufs &b, const float *neumann, int t0
     // a step enqueued behind a mesh update does nothing if that update raises an error bit
     const int *skip = s->mesh_set ? d.err : nullptr;
     int sp = timed ? span_begin(s, 0) : -1;
-    CK(launch_fdtd(s->grid.p, s->pmlbuf.p, fp, s->state.p, s->stream, skip));
-    span_end(s, sp);
-    if (s->mesh_set)
-        CK(launch_windows(d, 0, s->stream));
+    if (s->mesh_set && !s->use_pml)
+    {
+        // the ABC-shell launch also builds the gathers' history windows
+        const StepFuse fuse{s->hn.p, s->hd.p, d.wn, d.wd, s->nt};
+        CK(launch_fdtd(s->grid.p, s->pmlbuf.p, fp, s->state.p, s->stream, skip, &fuse));
+        span_end(s, sp);
+    }
+    else
+    {
+        CK(launch_fdtd(s->grid.p, s->pmlbuf.p, fp, s->state.p, s->stream, skip));
+        span_end(s, sp);
+        if (s->mesh_set)
+            CK(launch_windows(d, 0, s->stream));
+    }
     CK(launch_advance(s->state.p, s->stream, skip));
     if (!s->mesh_set)
     {
