@@ -296,8 +296,23 @@ ListArgs list_args(pppm_solver *s, const UpdateCounters &cn)
 }
 
 // ---------------------------------------------------------------- the update sequence
+// The grid-cache recompute goes to the update's second stream unless phases are being timed
+// (spans on the handle's stream) or the stream is being captured into a graph.
+bool fork_update(const pppm_solver *s)
+{
+    if (s->timing || !s->uf.aux)
+        return false;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s->stream, &cs) != hipSuccess)
+        return false;
+    return cs == hipStreamCaptureStatusNone;
+}
+
 // Geometry, binning, lists and weight caches for the vertex positions in s->verts (or, with
-// a series, row t+1-t0 of it). Asynchronous: only kernels are enqueued.
+// a series, row t+1-t0 of it). Asynchronous: only kernels are enqueued. The grid-cache
+// recompute needs only the first scan batch (its dirty-triangle list); forked, it runs on the
+// second stream while the lists and the face cache are built, and the handle's stream waits
+// for it at the end (the step kernels read both caches).
 int enqueue_rebuild(pppm_solver *s, bool first, const float *verts_series, long long vstride, int t0,
                     const float *vsrc = nullptr, bool use_prev = false)
 {
@@ -352,6 +367,24 @@ int enqueue_rebuild(pppm_solver *s, bool first, const float *verts_series, long 
     // cell CSR offsets; compaction of non-empty base cells, marked cells, dirty triangles
     CK(scan_batch(b1, s->scan_tmp.p, st));
 
+    // grid cache: dirty triangles only. Launch sizes follow the previous update's recompute
+    // counts (the kernels stride, so a larger count is still covered); a first build sizes
+    // for everything.
+    const long long ghint = first ? -1 : 2 * s->last_grid_recomputes + 64;
+    const long long fhint = first ? -1 : 2 * s->last_face_recomputes + 64;
+    auto grid_weights = [&](hipStream_t gs) {
+        return launch_grid_weights(s->verts.p, s->tris.p, s->geom.p, s->glist.p, stp + ST_GDIRTY, nt, wa, s->bem,
+                                   s->wf.p, s->sf.p, s->wb.p, s->sb.p, s->gmeta.p, err, ghint, gs);
+    };
+    const bool fork = fork_update(s);
+    if (fork)
+    {
+        CK(hipEventRecord(s->uf.fork, st));
+        CK(hipStreamWaitEvent(s->uf.aux, s->uf.fork, 0));
+        CK(grid_weights(s->uf.aux));
+        CK(hipEventRecord(s->uf.join, s->uf.aux));
+    }
+
     const ListArgs la = list_args(s, cn);
 
     // cell fill, 3x3x3 and face-list lengths (a triangle's face list is the 4x4x4 block around
@@ -363,15 +396,12 @@ int enqueue_rebuild(pppm_solver *s, bool first, const float *verts_series, long 
     s->counters_zero = true;
     span_end(s, sp);
 
-    // grid cache: dirty triangles only. Launch sizes follow the previous update's recompute
-    // counts (the kernels stride, so a larger count is still covered); a first build sizes
-    // for everything.
-    const long long ghint = first ? -1 : 2 * s->last_grid_recomputes + 64;
-    const long long fhint = first ? -1 : 2 * s->last_face_recomputes + 64;
-    sp = span_begin(s, 5);
-    CK(launch_grid_weights(s->verts.p, s->tris.p, s->geom.p, s->glist.p, stp + ST_GDIRTY, nt, wa, s->bem, s->wf.p,
-                           s->sf.p, s->wb.p, s->sb.p, s->gmeta.p, err, ghint, st));
-    span_end(s, sp);
+    if (!fork)
+    {
+        sp = span_begin(s, 5);
+        CK(grid_weights(st));
+        span_end(s, sp);
+    }
     sp = span_begin(s, 6);
 
     // face cache: persistent (c, j) slots, dirty pairs recomputed (FaceCache::update_cache)
@@ -387,6 +417,8 @@ int enqueue_rebuild(pppm_solver *s, bool first, const float *verts_series, long 
                            s->face_owner.p, s->face_nbr.p, s->face_slot.p, s->interp.p, wa, s->bem, s->ftmp.p, s->fw.p,
                            s->fsc.p, s->fmeta.p, err, fhint, st));
     span_end(s, sp);
+    if (fork)
+        CK(hipStreamWaitEvent(st, s->uf.join, 0));
 
     return PPPM_OK;
 }
@@ -953,6 +985,13 @@ int pppm_create(pppm_solver **out, int res, float dl, float dt, const float *min
     s->st_h = s->st_pin.p;
     std::memset(s->st_h, 0, ST_N * sizeof(int));
     s->own_stream = true;
+    if (hipStreamCreateWithFlags(&s->uf.aux, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&s->uf.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s->uf.join, hipEventDisableTiming) != hipSuccess)
+    {
+        pppm_destroy(s);
+        return PPPM_ERR_HIP;
+    }
     const size_t NN = (size_t)res * res * res;
     if (s->grid.ensure(3 * NN) != hipSuccess || s->state.ensure(1) != hipSuccess ||
         (s->use_pml && s->pmlbuf.ensure(3 * NN * 6) != hipSuccess))

@@ -204,6 +204,24 @@ struct pppm_solver
         }
     } stc;
     long long last_grid_recomputes = 0, last_face_recomputes = 0;
+    // second stream of a mesh update: the grid-cache recompute runs on it beside the lists and
+    // the face cache (forked after the first scan batch, joined before the update ends)
+    struct UpdateFork
+    {
+        hipStream_t aux = nullptr;
+        hipEvent_t fork = nullptr, join = nullptr;
+        ~UpdateFork()
+        {
+            if (aux)
+                (void)hipStreamSynchronize(aux);
+            if (fork)
+                (void)hipEventDestroy(fork);
+            if (join)
+                (void)hipEventDestroy(join);
+            if (aux)
+                (void)hipStreamDestroy(aux);
+        }
+    } uf;
 
     // mesh version: counts successful mesh updates (the adjoint's transposed lists belong to one)
     long long mesh_version = 0;
