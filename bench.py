@@ -302,14 +302,17 @@ def adjoint_bench(steps=2000):
     }
 
 
-def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11):
+def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11, step_rule="exact"):
     """Batched adjoint at scale: n_scenes scenes of res^3 with a radially perturbed icosphere L5
     (20480 triangles) and 8 listeners each, `steps` steps, source reconstruction by gradient
     descent on the per-triangle accelerations: J_i = sum (F_i(g_i) - target_i)^2, target_i the
     listener signals of seeded accelerations, g_i from zero. Each iteration: the batched,
     sqrt(T)-checkpointed loss and its gradient (autograd.BatchListenerLoss: batched forward,
-    batched recompute per segment, adjoint per scene), then an exact line search along -grad
-    (J is quadratic: alpha_i = |grad_i|^2 / (2 |F_i grad_i|^2), one batched forward).
+    batched recompute and batched adjoint per segment), then a step along -grad: step_rule
+    "exact" takes the exact line search (J is quadratic: alpha_i = |grad_i|^2 / (2 |F_i grad_i|^2),
+    one batched forward); "bb" takes it on the first iteration and the Barzilai-Borwein step
+    alpha_i = s.s / s.y after it (s, y: the changes of the iterate and of the gradient over the
+    last iteration; s.y = 2 |F s|^2 > 0 for this J), which needs no extra forward.
     Spot checks on scene 0 at full length: the adjoint identity <y, F g> = <F^T y, g> for random
     g, y, and the directional derivative (J(g + d) - J(g - d)) / 2 against <grad J, d>, which J
     being quadratic makes exact up to rounding. Returns the leg's JSON object."""
@@ -351,7 +354,7 @@ def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11):
     batch.run(true, steps, lis, out)
     targets = [out[:, i].clone() for i in range(n_scenes)]
     note("targets computed")
-    # plain batched forward and per-scene adjoint sweeps over the whole length (no checkpoints)
+    # plain batched forward and batched adjoint sweep over the whole length (no checkpoints)
     for s_ in sv:
         s_.reset()
     torch.cuda.synchronize()
@@ -365,16 +368,16 @@ def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11):
     # cotangents y_i = F_i g_i (so <y, F g> = |F g|^2 has no cancellation)
     y = [out[:, i].clone() for i in range(n_scenes)]
     grads = [torch.zeros(steps, nt, device=dev) for _ in range(n_scenes)]
-    for s_ in sv:  # the transposed lists are built once per mesh, outside the timing
-        s_.adjoint_reset(steps - 1)
+    batch.adjoint_reset(steps - 1)  # the transposed lists are built once per mesh, outside the timing
+    yb = torch.stack(y, 1).contiguous()  # (steps, n_scenes, L): the layout of the batched forward's output
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for i, (s_, yy, gg) in enumerate(zip(sv, y, grads)):
-        s_.adjoint_reset(steps - 1)
-        s_.adjoint_run(yy, steps, lis[i], gg)
+    batch.adjoint_reset(steps - 1)
+    batch.adjoint_run(yb, steps, lis, grads)
     torch.cuda.synchronize()
     t_adj = time.perf_counter() - t0
-    note(f"adjoint sweeps: {t_adj:.1f} s")
+    del yb
+    note(f"batched adjoint sweep: {t_adj:.1f} s")
     # adjoint identity on scene 0: <y, F g> = <F^T y, g> with g = the seeded accelerations
     lhs = float((y[0].double() * fwd_out0.double()).sum())
     rhs = float((grads[0].double() * true[0].double()).sum())
@@ -382,6 +385,7 @@ def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11):
     del true, grads, y
     # gradient descent
     gs = [torch.zeros(steps, nt, device=dev) for _ in range(n_scenes)]
+    prev_x = prev_g = None
     hist, t_loss, t_back, t_ls = [], 0.0, 0.0, 0.0
     ck_bytes = 0
     torch.cuda.reset_peak_memory_stats()
@@ -402,16 +406,29 @@ def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11):
         note(f"iteration {it + 1}/{iters}: loss / initial {float((hist[-1] / hist[0]).mean()):.4f}; "
              f"forward {t1 - t0:.1f} s, recompute + adjoint {t2 - t1:.1f} s")
         gr = [x.grad.detach() for x in xs]
-        # exact line search along -grad: alpha_i = |grad_i|^2 / (2 |F_i grad_i|^2)
-        for s_ in sv:
-            s_.reset()
-        batch.run(gr, steps, lis, out)
+        alphas = []
+        if step_rule == "bb" and prev_x is not None:
+            # Barzilai-Borwein: alpha_i = s.s / s.y
+            for i in range(n_scenes):
+                sd = (xs[i].detach() - prev_x[i]).double()
+                yd = (gr[i] - prev_g[i]).double()
+                sy = float((sd * yd).sum())
+                alphas.append(float((sd * sd).sum()) / sy if sy > 0 else 0.0)
+                del sd, yd
+        else:
+            # exact line search along -grad: alpha_i = |grad_i|^2 / (2 |F_i grad_i|^2)
+            for s_ in sv:
+                s_.reset()
+            batch.run(gr, steps, lis, out)
+            for i in range(n_scenes):
+                den = 2.0 * float((out[:, i].double() ** 2).sum())
+                alphas.append(float((gr[i].double() ** 2).sum()) / den if den > 0 else 0.0)
         torch.cuda.synchronize()
         t_ls += time.perf_counter() - t2
+        if step_rule == "bb":
+            prev_x, prev_g = [x.detach() for x in xs], gr
         for i in range(n_scenes):
-            den = 2.0 * float((out[:, i].double() ** 2).sum())
-            alpha = float((gr[i].double() ** 2).sum()) / den if den > 0 else 0.0
-            gs[i] = (xs[i].detach() - alpha * gr[i]).contiguous()
+            gs[i] = (xs[i].detach() - alphas[i] * gr[i]).contiguous()
         del xs, gr
     # directional derivative on scene 0 at the current iterate (any point works for a quadratic),
     # along a random direction with a component along the gradient (no cancellation in <grad, d>)
@@ -439,13 +456,14 @@ def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11):
     res_ = {
         "metric": "batched adjoint: source reconstruction by gradient descent",
         "config": {"workload": f"c5_batched_adjoint_{n_scenes}x{res}^3", "scenes": n_scenes, "grid": f"{res}^3",
-                   "triangles_per_scene": nt, "listeners_per_scene": L, "steps": steps, "iterations": iters},
+                   "triangles_per_scene": nt, "listeners_per_scene": L, "steps": steps, "iterations": iters,
+                   "step_rule": step_rule},
         "forward_cell_updates_per_s": cells / t_fwd,
         "adjoint_cell_updates_per_s": cells / t_adj,
         "forward_s": t_fwd, "adjoint_s": t_adj, "adjoint_over_forward": t_adj / t_fwd,
         "iteration_s": per_it,
         "iteration_breakdown_s": {"checkpointed_forward": t_loss / iters, "recompute_and_adjoint": t_back / iters,
-                                  "line_search_forward": t_ls / iters},
+                                  "step_size": t_ls / iters},
         "checkpoint_recompute_overhead": (t_loss + t_back) / iters / (t_fwd + t_adj) - 1.0,
         "checkpoint_bytes": ck_bytes,
         "memory_gb": {"device_used": (total - free1) / 1e9, "device_used_before": (total - free0) / 1e9,
@@ -566,6 +584,9 @@ def main():
     ap.add_argument("--c5-steps", type=int, default=8192)
     ap.add_argument("--c5-iters", type=int, default=3)
     ap.add_argument("--c5-res", type=int, default=128)
+    ap.add_argument("--c5-step", default="exact", choices=["exact", "bb"],
+                    help="gradient-descent step: exact line search (one batched forward per iteration) or "
+                         "Barzilai-Borwein after the first iteration")
     ap.add_argument("--workload", default="c2_moving",
                     choices=["c2_moving", "c1_static_sphere", "large_static_sphere", "c5_batched_adjoint"],
                     help="c2_moving: the metric's workload; c1_static_sphere: 32^3 static sphere; "
@@ -601,7 +622,7 @@ def main():
         ghost_bench(cpu=not args.no_cpu)
         return
     if args.workload == "c5_batched_adjoint":
-        print(json.dumps(c5_bench(args.c5_scenes, args.c5_steps, args.c5_iters, args.c5_res)))
+        print(json.dumps(c5_bench(args.c5_scenes, args.c5_steps, args.c5_iters, args.c5_res, step_rule=args.c5_step)))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design note, "

@@ -164,6 +164,21 @@ int pppm_state_bytes(pppm_solver *s, long long *bytes);
 int pppm_save_state(pppm_solver *s, void *dst_dev);
 int pppm_load_state(pppm_solver *s, const void *src_dev);
 
+/* Batched adjoint: the reverse steps of every scene of a batch, each reverse-step kernel one
+ * launch over all scenes (no counterpart in the reference). Each scene's gradient equals its own
+ * pppm_adjoint_run's bit for bit.
+ * pppm_batch_adjoint_reset: pppm_adjoint_reset(scene, t_last) for every scene.
+ * pppm_batch_adjoint_run: reverses `steps` steps of every scene (their reverse indices must be
+ *   equal: reset together). gout_dev (device, steps x n_scenes x n_listen floats, the layout of
+ *   pppm_batch_run's output): gout_dev[(r n_scenes + i) n_listen + l] = the cotangent of scene i's
+ *   listener l (cell listen_xyz[3 (i n_listen + l) ..], host, n_listen <= 64) at forward step
+ *   T-steps+1+r; grad_dev[i] (device, steps x grad_strides[i] floats, grad_strides[i] >= nt_i):
+ *   scene i's gradient rows as in pppm_adjoint_run. use_graph != 0: 32 reverse batch steps per
+ *   captured graph launch. Synchronous. */
+int pppm_batch_adjoint_reset(pppm_batch *b, int t_last);
+int pppm_batch_adjoint_run(pppm_batch *b, const float *gout_dev, int steps, int n_listen, const int *listen_xyz,
+                           float *const *grad_dev, const long long *grad_strides, int use_graph);
+
 /* Phase entry points used by the reference's tests (src/algorithm/pppm.h:127-135,
  * src/algorithm/fdtd.h:73-87). */
 int pppm_fdtd_step(pppm_solver *s);                              /* FDTD::step */

@@ -110,7 +110,9 @@ class BatchListenerLoss(torch.autograd.Function):
     (pppm_batch_run) in segments of S steps from the zero state and keeps each scene's state at
     the start of every segment; the backward walks the segments from the last, restores every
     scene's checkpoint, recomputes the segment's samples with one batched run and reverses it
-    scene by scene with the adjoint. Returns J as an (n_scenes,) FP64-accumulated float32 tensor;
+    with one batched adjoint run (pppm_batch_adjoint_run: every scene's reverse steps together,
+    each scene's gradient bit for bit its own adjoint's). Returns J as an (n_scenes,)
+    FP64-accumulated float32 tensor;
     the gradient of scene i depends only on its own inputs (the scenes are independent).
     Inputs: batch (PPPMBatch over `solvers`), targets (one (T, L) tensor per scene), listeners
     (L cells per scene), then the per-scene (T, nt_i) inputs g_i."""
@@ -153,16 +155,17 @@ class BatchListenerLoss(torch.autograd.Function):
         T, L = gs[0].shape[0], len(ctx.listeners[0])
         grads = [torch.zeros(T, s.nt, device=g.device, dtype=torch.float32) for g, s in zip(gs, solvers)]
         out = torch.zeros(S, n, L, device=gs[0].device, dtype=torch.float32)
-        for s in solvers:
-            s.adjoint_reset(T - 1)
+        tgt = torch.stack([t.float() for t in ctx.targets], 1)  # (T, n, L)
+        scale = (2.0 * gJ.float())[None, :, None]
+        ctx.batch.adjoint_reset(T - 1)
         for k in reversed(range(ctx.nseg)):
             a, b = k * S, min(T, (k + 1) * S)
             for i, s in enumerate(solvers):
                 s.load_state(ctx.ck[i][k])
             ctx.batch.run([g[a:b] for g in gs], b - a, ctx.listeners, out[:b - a], use_graph=ctx.use_graph)  # recompute
-            for i, s in enumerate(solvers):
-                gout = (2.0 * gJ[i].float()) * (out[:b - a, i] - ctx.targets[i][a:b].float())
-                s.adjoint_run(gout.contiguous(), b - a, ctx.listeners[i], grads[i][a:b], use_graph=ctx.use_graph)
+            gout = scale * (out[:b - a] - tgt[a:b])
+            ctx.batch.adjoint_run(gout.contiguous(), b - a, ctx.listeners, [gr[a:b] for gr in grads],
+                                  use_graph=ctx.use_graph)
         return (None, None, None, None, None, None) + tuple(grads)
 
 

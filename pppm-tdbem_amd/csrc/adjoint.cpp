@@ -133,6 +133,17 @@ int adjoint_setup(pppm_solver *s)
                 ft_slot[at] = fslot[e];
             }
     }
+    // their face records and scales in the same order, so the transposed gather streams them
+    std::vector<uint32_t> tfw((size_t)std::max(P4, 1) * STEPS);
+    std::vector<float2> tfsc(std::max(P4, 1));
+    for (int e = 0; e < P4; e++)
+    {
+        const int sl = ft_slot[e];
+        if (sl < 0 || sl >= pool)
+            return solver_fail(s, PPPM_ERR_STATE, "adjoint: a face pair without a face-cache record");
+        std::memcpy(&tfw[(size_t)e * STEPS], &fw[(size_t)sl * STEPS], STEPS * sizeof(uint32_t));
+        tfsc[e] = fsc[sl];
+    }
     // corners transposed: per C3 cell x, the (triangle, rc x interpolation weight) entries
     std::vector<int> ct_start(n_c3 + 1, 0), ct_c((size_t)8 * nt);
     std::vector<float> ct_w((size_t)8 * nt);
@@ -169,6 +180,8 @@ int adjoint_setup(pppm_solver *s)
     ARC(upload_vec(s, s->adj_ft_start, ft_start));
     ARC(upload_vec(s, s->adj_ft_c, ft_c));
     ARC(upload_vec(s, s->adj_ft_slot, ft_slot));
+    ARC(upload_vec(s, s->adj_tfw, tfw));
+    ARC(upload_vec(s, s->adj_tfsc, tfsc));
     ARC(upload_vec(s, s->adj_ct_start, ct_start));
     ARC(upload_vec(s, s->adj_ct_c, ct_c));
     ARC(upload_vec(s, s->adj_ct_w, ct_w));
@@ -187,7 +200,8 @@ int adjoint_setup(pppm_solver *s)
     return PPPM_OK;
 }
 
-AdjScene adj_scene(pppm_solver *s, const float *gout, int n_listen, float *grad, long long grad_stride, int t_first)
+AdjScene adj_scene(pppm_solver *s, const float *gout, long long gout_stride, int n_listen, float *grad,
+                   long long grad_stride, int t_first)
 {
     AdjScene d{};
     d.fp = solver_fdtd_params(s);
@@ -213,6 +227,8 @@ AdjScene adj_scene(pppm_solver *s, const float *gout, int n_listen, float *grad,
     d.ft_start = s->adj_ft_start.p;
     d.ft_c = s->adj_ft_c.p;
     d.ft_slot = s->adj_ft_slot.p;
+    d.tfw = s->adj_tfw.p;
+    d.tfsc = s->adj_tfsc.p;
     d.ct_start = s->adj_ct_start.p;
     d.ct_c = s->adj_ct_c.p;
     d.ct_w = s->adj_ct_w.p;
@@ -221,6 +237,7 @@ AdjScene adj_scene(pppm_solver *s, const float *gout, int n_listen, float *grad,
     d.lc3 = s->lc3.p;
     d.n_listen = n_listen;
     d.gout = gout;
+    d.gout_stride = gout_stride;
     d.grad = grad;
     d.grad_stride = grad_stride;
     d.t_first = t_first;
@@ -241,6 +258,15 @@ float *level_ptr(pppm_solver *s, int t, int rel)
 
 }  // namespace
 
+namespace pppm_amd
+{
+AdjScene solver_adj_scene(pppm_solver *s, const float *gout, long long gout_stride, int n_listen, float *grad,
+                          long long grad_stride, int t_first)
+{
+    return adj_scene(s, gout, gout_stride, n_listen, grad, grad_stride, t_first);
+}
+}  // namespace pppm_amd
+
 extern "C" {
 
 int pppm_adjoint_reset(pppm_solver *s, int t_last)
@@ -252,6 +278,8 @@ int pppm_adjoint_reset(pppm_solver *s, int t_last)
         return solver_fail(s, PPPM_ERR_ARG, "adjoint: the PML is not differentiated (ABC only)");
     if (t_last < -1)
         return solver_fail(s, PPPM_ERR_ARG, "adjoint_reset: t_last >= -1");
+    if ((long long)s->N * s->N * s->N >= (1LL << 31))
+        return solver_fail(s, PPPM_ERR_ARG, "adjoint: grids below 2^31 cells (32-bit cell indices)");
     ARC(adjoint_setup(s));
     const size_t NN = (size_t)s->N * s->N * s->N;
     ACK(hipMemsetAsync(s->adj_lam3.p, 0, 3 * NN * sizeof(float), s->stream));
@@ -281,7 +309,8 @@ int pppm_adjoint_run(pppm_solver *s, const float *gout_dev, int steps, const int
         return PPPM_OK;
     ARC(solver_set_listeners(s, listen_xyz, n_listen));
     const int t_first = s->adj_t - steps + 1;
-    const AdjScene d = adj_scene(s, n_listen > 0 ? gout_dev : nullptr, n_listen, grad_dev, grad_stride, t_first);
+    const AdjScene d =
+        adj_scene(s, n_listen > 0 ? gout_dev : nullptr, n_listen, n_listen, grad_dev, grad_stride, t_first);
     int done = 0;
     if (use_graph && steps >= 2)
     {

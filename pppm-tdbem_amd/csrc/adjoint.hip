@@ -26,6 +26,8 @@
 // The reverse time index lives in device memory, so a captured graph replays any step.
 #include "internal.hpp"
 
+#include <algorithm>
+
 namespace pppm_amd
 {
 
@@ -44,13 +46,13 @@ __device__ inline float wS(uint32_t w, float2 sc) { return f16_bits_to_f32(w) * 
 __device__ inline float wD(uint32_t w, float2 sc) { return f16_bits_to_f32(w >> 16) * sc.y; }
 
 // 1. listener cotangents and the C3 cells' cotangents u (the near field's input)
-__global__ __launch_bounds__(BLK) void k_adj_read(AdjScene d)
+__device__ __forceinline__ void adj_read(const AdjScene &d, int blk)
 {
     const int t = d.ast->t;
     const size_t nn = (size_t)d.N * d.N * d.N;
     float *lam = alevel(d.lam3, t, 0, nn);
-    const int c = blockIdx.x * BLK + threadIdx.x;
-    const float *go = d.gout ? d.gout + (size_t)(t - d.t_first) * d.n_listen : nullptr;
+    const int c = blk * BLK + threadIdx.x;
+    const float *go = d.gout ? d.gout + (size_t)(t - d.t_first) * d.gout_stride : nullptr;
     if (c < d.n_c3)
     {
         const int q = d.c3_cells[c];
@@ -71,22 +73,60 @@ __global__ __launch_bounds__(BLK) void k_adj_read(AdjScene d)
     }
 }
 
+// The 16 tap threads of a triangle are one aligned 16-lane group (a block's first thread is at
+// a multiple of 16 within its triangle range); lane k fetches the per-record values of records
+// k and k + 16 of the triangle's 27, and every lane takes record o's from lane o % 16 by shuffle.
+struct RecVals
+{
+    float v[2];
+    float2 sc[2];
+};
+__device__ __forceinline__ RecVals rec_vals(const int *__restrict__ tc3, const float *__restrict__ val,
+                                            const float2 *__restrict__ scales, int tri, int k)
+{
+    RecVals r;
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+    {
+        const int o = k + STEPS * h;
+        r.v[h] = 0.0f;
+        r.sc[h] = make_float2(0.0f, 0.0f);
+        if (o < GC)
+        {
+            const int rec = tri * GC + o;
+            r.v[h] = val[tc3[rec]];
+            r.sc[h] = scales[rec];
+        }
+    }
+    return r;
+}
+__device__ __forceinline__ void rec_take(const RecVals &r, int o, float &v, float2 &sc)
+{
+    const int h = o / STEPS, src = o % STEPS;
+    v = __shfl(r.v[h], src, STEPS);
+    sc.x = __shfl(r.sc[h].x, src, STEPS);
+    sc.y = __shfl(r.sc[h].y, src, STEPS);
+}
+
 // 2. near field transposed, one thread per (triangle, tap): the 27 records of a triangle are
 // contiguous, so the 16 threads of a triangle read 64 B of each record
-__global__ __launch_bounds__(BLK) void k_adj_near(AdjScene d)
+__device__ __forceinline__ void adj_near(const AdjScene &d, int blk)
 {
-    const int i = blockIdx.x * BLK + threadIdx.x;
+    const int i = blk * BLK + threadIdx.x;
     if (i >= d.nt * STEPS)
-        return;
+        return;  // whole 16-lane groups
     const int t = d.ast->t;
     const int tri = i / STEPS, k = i % STEPS;
+    const RecVals rv = rec_vals(d.tc3, d.unear, d.sb, tri, k);
     float an = 0.0f, ad = 0.0f;
+#pragma unroll
     for (int o = 0; o < GC; o++)
     {
         const int rec = tri * GC + o;
-        const float u = d.unear[d.tc3[rec]];
+        float u;
+        float2 sc;
+        rec_take(rv, o, u, sc);
         const uint32_t w = d.wb[(size_t)rec * STEPS + k];
-        const float2 sc = d.sb[rec];
         an += wS(w, sc) * u;
         ad += wD(w, sc) * u;
     }
@@ -96,9 +136,9 @@ __global__ __launch_bounds__(BLK) void k_adj_near(AdjScene d)
 }
 
 // 3. the Dirichlet update's divisor
-__global__ __launch_bounds__(BLK) void k_adj_dirv(AdjScene d)
+__device__ __forceinline__ void adj_dirv(const AdjScene &d, int blk)
 {
-    const int c = blockIdx.x * BLK + threadIdx.x;
+    const int c = blk * BLK + threadIdx.x;
     if (c >= d.nt)
         return;
     const int slot = c * RING + (d.ast->t & (RING - 1));
@@ -106,33 +146,55 @@ __global__ __launch_bounds__(BLK) void k_adj_dirv(AdjScene d)
     d.ld[slot] = 0.0f;
 }
 
+// blocks of the C3 part of the Dirichlet and far-field kernels (the triangle part starts at a
+// block boundary, so its 16-lane groups are aligned)
+__device__ __forceinline__ int c3_part_blocks(const AdjScene &d) { return (d.n_c3 + BLK - 1) / BLK; }
+
 // 4. Dirichlet transposed: C3 cells [0, n_c3) gather the face-far-field interpolation; then one
-// thread per (triangle j, tap) gathers the face pairs (c, j)
-__global__ __launch_bounds__(BLK) void k_adj_dir(AdjScene d)
+// thread per (triangle j, tap) gathers the face pairs (c, j) (records in that order: a
+// triangle's 16 lanes read consecutive 64 B records; lane k fetches pair eb + k's cell value and
+// scales, shared by shuffles)
+__device__ __forceinline__ void adj_dir(const AdjScene &d, int blk)
 {
-    const int i = blockIdx.x * BLK + threadIdx.x;
     const int t = d.ast->t;
-    if (i < d.n_c3)
+    const int nb3 = c3_part_blocks(d);
+    if (blk < nb3)
     {
-        float a = d.unear[i];
-        for (int e = d.ct_start[i]; e < d.ct_start[i + 1]; e++)
-            a += d.ct_w[e] * d.v[d.ct_c[e]];
-        d.lfar[i] = a;
+        const int i = blk * BLK + threadIdx.x;
+        if (i < d.n_c3)
+        {
+            float a = d.unear[i];
+            for (int e = d.ct_start[i]; e < d.ct_start[i + 1]; e++)
+                a += d.ct_w[e] * d.v[d.ct_c[e]];
+            d.lfar[i] = a;
+        }
         return;
     }
-    const int q = i - d.n_c3;
+    const int q = (blk - nb3) * BLK + threadIdx.x;
     if (q >= d.nt * STEPS)
-        return;
+        return;  // whole 16-lane groups
     const int j = q / STEPS, k = q % STEPS;
+    const int e0 = d.ft_start[j], e1 = d.ft_start[j + 1];
     float an = 0.0f, ad = 0.0f;
-    for (int e = d.ft_start[j]; e < d.ft_start[j + 1]; e++)
+    for (int eb = e0; eb < e1; eb += STEPS)
     {
-        const int slot = d.ft_slot[e];
-        const float vc = d.v[d.ft_c[e]];
-        const uint32_t w = d.fw[(size_t)slot * STEPS + k];
-        const float2 sc = d.fsc[slot];
-        an += wS(w, sc) * vc;
-        ad += wD(w, sc) * vc;
+        const int el = eb + k;
+        float vl = 0.0f;
+        float2 sl = make_float2(0.0f, 0.0f);
+        if (el < e1)
+        {
+            vl = d.v[d.ft_c[el]];
+            sl = d.tfsc[el];
+        }
+        const int n = min(STEPS, e1 - eb);
+        for (int m = 0; m < n; m++)
+        {
+            const float vc = __shfl(vl, m, STEPS);
+            const float2 sc = make_float2(__shfl(sl.x, m, STEPS), __shfl(sl.y, m, STEPS));
+            const uint32_t w = d.tfw[(size_t)(eb + m) * STEPS + k];
+            an += wS(w, sc) * vc;
+            ad += wD(w, sc) * vc;
+        }
     }
     const int rs = j * RING + ((t - k) & (RING - 1));
     d.ln[rs] += -an;
@@ -141,37 +203,45 @@ __global__ __launch_bounds__(BLK) void k_adj_dir(AdjScene d)
 }
 
 // 5. far field transposed; the (triangle, tap 0) threads write the finished gradient of N(t)
-__global__ __launch_bounds__(BLK) void k_adj_far(AdjScene d)
+__device__ __forceinline__ void adj_far(const AdjScene &d, int blk)
 {
-    const int i = blockIdx.x * BLK + threadIdx.x;
     const int t = d.ast->t;
     const size_t nn = (size_t)d.N * d.N * d.N;
-    if (i < d.n_c3)
+    const int nb3 = c3_part_blocks(d);
+    if (blk < nb3)
     {
-        float *lam = alevel(d.lam3, t, 0, nn);
-        lam[d.c3_cells[i]] += d.lfar[i];
+        const int i = blk * BLK + threadIdx.x;
+        if (i < d.n_c3)
+        {
+            float *lam = alevel(d.lam3, t, 0, nn);
+            lam[d.c3_cells[i]] += d.lfar[i];
+        }
         return;
     }
-    const int q = i - d.n_c3;
+    const int q = (blk - nb3) * BLK + threadIdx.x;
     if (q >= d.nt * STEPS)
-        return;
+        return;  // whole 16-lane groups
     const int tri = q / STEPS, k = q % STEPS;
+    // every lane of the group takes part in the shuffles; tap 0 then writes the gradient instead
+    const RecVals rv = rec_vals(d.tc3, d.lfar, d.sf, tri, k);
+    float an = 0.0f, ad = 0.0f;
+#pragma unroll
+    for (int o = 0; o < GC; o++)
+    {
+        const int rec = tri * GC + o;
+        float lf;
+        float2 sc;
+        rec_take(rv, o, lf, sc);
+        const uint32_t w = d.wf[(size_t)rec * STEPS + k];
+        an += wS(w, sc) * lf;
+        ad += wD(w, sc) * lf;
+    }
     if (k == 0)
     {
         const int slot = tri * RING + (t & (RING - 1));
         d.grad[(size_t)(t - d.t_first) * d.grad_stride + tri] = d.ln[slot];
         d.ln[slot] = 0.0f;
         return;
-    }
-    float an = 0.0f, ad = 0.0f;
-    for (int o = 0; o < GC; o++)
-    {
-        const int rec = tri * GC + o;
-        const float lf = d.lfar[d.tc3[rec]];
-        const uint32_t w = d.wf[(size_t)rec * STEPS + k];
-        const float2 sc = d.sf[rec];
-        an += wS(w, sc) * lf;
-        ad += wD(w, sc) * lf;
     }
     const int slot = tri * RING + ((t - k) & (RING - 1));
     d.ln[slot] += an;
@@ -215,12 +285,12 @@ __device__ inline AbcCoef abc_coef(const FdtdParams &p)
 
 // 6a. per shell cell: b = A^{-T} (ghost cotangents, lambda_P(t)) in FP64 (Gaussian elimination
 // with partial pivoting on the 4x4 transposed system)
-__global__ __launch_bounds__(BLK) void k_adj_abc(AdjScene d)
+__device__ __forceinline__ void adj_abc(const AdjScene &d, int blk)
 {
     const int N = d.fp.N;
     const long long S = N - 2;
     const long long fx = 2 * S * S, fy = 2 * (S - 2) * S, fz = 2 * (S - 2) * (S - 2);
-    long long idx = (long long)blockIdx.x * BLK + threadIdx.x;
+    long long idx = (long long)blk * BLK + threadIdx.x;
     int i, j, k;
     if (idx < fx)
     {
@@ -306,19 +376,43 @@ __global__ __launch_bounds__(BLK) void k_adj_abc(AdjScene d)
 
 // 6b. per cell: the transposed leapfrog (interior cells that read it) and the transposed ABC
 // (shell cells that read it, with the coefficients of fdtd.cu:94-118 for each neighbour slot)
-__global__ __launch_bounds__(BLK) void k_adj_fdtd(AdjScene d)
+__device__ __forceinline__ void adj_fdtd(const AdjScene &d, int blk)
 {
     const int N = d.fp.N;
     const size_t nn = (size_t)N * N * N;
-    const long long id = (long long)blockIdx.x * BLK + threadIdx.x;
-    if (id >= (long long)nn)
+    // 32-bit cell indices (the adjoint takes grids below 2^31 cells)
+    const unsigned NN2 = (unsigned)N * (unsigned)N;
+    const unsigned uid = (unsigned)blk * BLK + threadIdx.x;
+    if (uid >= (unsigned)nn)
         return;
-    const int y0 = (int)(id / ((long long)N * N)), y1 = (int)(id / N % N), y2 = (int)(id % N);
+    const int id = (int)uid;
+    const int y0 = (int)(uid / NN2);
+    const unsigned r = uid - (unsigned)y0 * NN2;
+    const int y1 = (int)(r / (unsigned)N), y2 = (int)(r - (unsigned)y1 * (unsigned)N);
     const int yc[3] = {y0, y1, y2};
     const int t = d.ast->t;
     const float *lam = alevel(d.lam3, t, 0, nn);
     float *lam1 = alevel(d.lam3, t, -1, nn);
     float *lam2 = alevel(d.lam3, t, -2, nn);
+    if (y0 >= 3 && y1 >= 3 && y2 >= 3 && y0 <= N - 4 && y1 <= N - 4 && y2 <= N - 4)
+    {
+        // no ABC shell cell reads this cell or any of its neighbours: the transposed leapfrog
+        // alone, with the general path's operations in its order
+        const double cc = 2.0 - 6.0 * d.fp.coef / d.fp.hh, cn = d.fp.coef / d.fp.hh;
+        const int sN2 = (int)NN2;
+        double ag = 0.0, am = 0.0;
+        ag += cc * (double)lam[id];
+        am -= (double)lam[id];
+        ag += cn * (double)lam[id - sN2];
+        ag += cn * (double)lam[id + sN2];
+        ag += cn * (double)lam[id - N];
+        ag += cn * (double)lam[id + N];
+        ag += cn * (double)lam[id - 1];
+        ag += cn * (double)lam[id + 1];
+        lam1[id] += (float)ag;
+        lam2[id] = (float)am;
+        return;
+    }
     int out = 0;
     for (int e = 0; e < 3; e++)
         out += yc[e] < 1 || yc[e] > N - 2;
@@ -391,27 +485,110 @@ __global__ __launch_bounds__(BLK) void k_adj_fdtd(AdjScene d)
 
 __global__ void k_adj_back(AdjState *st) { st->t = st->t - 1; }
 
+// the reverse step's kernels, in order (kind = index)
+enum AdjKind
+{
+    kRead,
+    kNear,
+    kDirv,
+    kDir,
+    kFar,
+    kAbc,
+    kFdtd
+};
+
+template <int KIND>
+__device__ __forceinline__ void adj_kernel(const AdjScene &d, int blk)
+{
+    if constexpr (KIND == kRead)
+        adj_read(d, blk);
+    else if constexpr (KIND == kNear)
+        adj_near(d, blk);
+    else if constexpr (KIND == kDirv)
+        adj_dirv(d, blk);
+    else if constexpr (KIND == kDir)
+        adj_dir(d, blk);
+    else if constexpr (KIND == kFar)
+        adj_far(d, blk);
+    else if constexpr (KIND == kAbc)
+        adj_abc(d, blk);
+    else
+        adj_fdtd(d, blk);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(BLK) void k_adj(AdjScene d)
+{
+    adj_kernel<KIND>(d, (int)blockIdx.x);
+}
+
+// Batched: one launch covers the blocks of every scene; virtual block v belongs to scene
+// m.scene_of[v] as its local block v - m.first[scene]. Blocks go to the 8 XCDs in turn and block
+// b takes virtual block (b % 8) * m.per + b / 8, so each XCD works through one contiguous run of
+// scenes.
+template <int KIND>
+__global__ __launch_bounds__(BLK) void k_batch_adj(const AdjScene *__restrict__ ds, BatchMap m)
+{
+    const int v = (int)(blockIdx.x & 7) * m.per + (int)(blockIdx.x >> 3);
+    if (v >= m.total)
+        return;
+    const int sc = m.scene_of[v];
+    adj_kernel<KIND>(ds[sc], v - m.first[sc]);
+}
+
+__global__ void k_batch_adj_back(const AdjScene *__restrict__ ds, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        ds[i].ast->t = ds[i].ast->t - 1;
+}
+
 }  // namespace
 
-int adj_c3_blocks(const AdjScene &d) { return (d.n_c3 + 1 + BLK - 1) / BLK; }
-
-hipError_t enqueue_adjoint_step(const AdjScene &d, hipStream_t s)
+int adj_blocks(const AdjScene &d, int kind)
 {
     const int N = d.fp.N;
     const long long S = N - 2;
     const long long shell = 2 * S * S + 2 * (S - 2) * S + 2 * (S - 2) * (S - 2);
     const long long nn = (long long)N * N * N;
-    const int tri_blocks = (d.nt * STEPS + BLK - 1) / BLK;
-    const int c3_blocks = (d.n_c3 + BLK - 1) / BLK;
-    k_adj_read<<<adj_c3_blocks(d), BLK, 0, s>>>(d);
-    k_adj_near<<<tri_blocks, BLK, 0, s>>>(d);
-    k_adj_dirv<<<(d.nt + BLK - 1) / BLK, BLK, 0, s>>>(d);
-    k_adj_dir<<<(int)((d.n_c3 + (long long)d.nt * STEPS + BLK - 1) / BLK), BLK, 0, s>>>(d);
-    k_adj_far<<<(int)((d.n_c3 + (long long)d.nt * STEPS + BLK - 1) / BLK), BLK, 0, s>>>(d);
-    k_adj_abc<<<(int)((shell + BLK - 1) / BLK), BLK, 0, s>>>(d);
-    k_adj_fdtd<<<(int)((nn + BLK - 1) / BLK), BLK, 0, s>>>(d);
+    long long items = 0;
+    switch (kind)
+    {
+    case kRead: items = d.n_c3 + 1; break;  // the extra thread: listeners outside the C3 cells
+    case kNear: items = (long long)d.nt * STEPS; break;
+    case kDirv: items = d.nt; break;
+    case kDir:
+    case kFar:  // the C3 part's blocks, then the triangle part's
+        return (int)std::max<long long>(1, (d.n_c3 + BLK - 1) / BLK + ((long long)d.nt * STEPS + BLK - 1) / BLK);
+    case kAbc: items = shell; break;
+    default: items = nn; break;
+    }
+    return (int)std::max<long long>(1, (items + BLK - 1) / BLK);
+}
+
+hipError_t enqueue_adjoint_step(const AdjScene &d, hipStream_t s)
+{
+    k_adj<kRead><<<adj_blocks(d, kRead), BLK, 0, s>>>(d);
+    k_adj<kNear><<<adj_blocks(d, kNear), BLK, 0, s>>>(d);
+    k_adj<kDirv><<<adj_blocks(d, kDirv), BLK, 0, s>>>(d);
+    k_adj<kDir><<<adj_blocks(d, kDir), BLK, 0, s>>>(d);
+    k_adj<kFar><<<adj_blocks(d, kFar), BLK, 0, s>>>(d);
+    k_adj<kAbc><<<adj_blocks(d, kAbc), BLK, 0, s>>>(d);
+    k_adj<kFdtd><<<adj_blocks(d, kFdtd), BLK, 0, s>>>(d);
     k_adj_back<<<1, 1, 0, s>>>(d.ast);
-    (void)c3_blocks;
+    return hipGetLastError();
+}
+
+hipError_t enqueue_batch_adjoint_step(const AdjScene *ds, const BatchMap *maps, int n, hipStream_t s)
+{
+    k_batch_adj<kRead><<<8 * maps[kRead].per, BLK, 0, s>>>(ds, maps[kRead]);
+    k_batch_adj<kNear><<<8 * maps[kNear].per, BLK, 0, s>>>(ds, maps[kNear]);
+    k_batch_adj<kDirv><<<8 * maps[kDirv].per, BLK, 0, s>>>(ds, maps[kDirv]);
+    k_batch_adj<kDir><<<8 * maps[kDir].per, BLK, 0, s>>>(ds, maps[kDir]);
+    k_batch_adj<kFar><<<8 * maps[kFar].per, BLK, 0, s>>>(ds, maps[kFar]);
+    k_batch_adj<kAbc><<<8 * maps[kAbc].per, BLK, 0, s>>>(ds, maps[kAbc]);
+    k_batch_adj<kFdtd><<<8 * maps[kFdtd].per, BLK, 0, s>>>(ds, maps[kFdtd]);
+    k_batch_adj_back<<<(n + 255) / 256, 256, 0, s>>>(ds, n);
     return hipGetLastError();
 }
 

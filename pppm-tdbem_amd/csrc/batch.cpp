@@ -53,6 +53,9 @@ struct pppm_batch
     DBuf<SceneStep> descs;
     DBuf<FdtdScene> fdescs;
     Map m_fdtd, m_tile[kFdtdVariants], m_win, m_far, m_dir, m_near;
+    // batched adjoint: scene table and one block map per reverse-step kernel
+    DBuf<AdjScene> adescs;
+    Map m_adj[kAdjKernels];
     // per-phase HIP event timing of eager batch steps: 0 FDTD (shell and interiors), 1 history
     // windows and time advance, 2 far field, 3 Dirichlet update, 4 near field
     bool timing = false;
@@ -383,6 +386,110 @@ int pppm_batch_run(pppm_batch *b, const float *const *neumann_series, const long
     BCK(hipStreamSynchronize(b->stream));
     for (pppm_solver *s : b->sc)
         s->t_host += steps;
+    return PPPM_OK;
+}
+
+int pppm_batch_adjoint_reset(pppm_batch *b, int t_last)
+{
+    if (!b)
+        return PPPM_ERR_ARG;
+    BRC(check_scenes(b));
+    for (size_t i = 0; i < b->sc.size(); i++)
+    {
+        const int rc = pppm_adjoint_reset(b->sc[i], t_last);
+        if (rc)
+            return bfail(b, rc, "batch_adjoint_reset: scene " + std::to_string(i) + ": " + b->sc[i]->err);
+    }
+    return PPPM_OK;
+}
+
+int pppm_batch_adjoint_run(pppm_batch *b, const float *gout_dev, int steps, int n_listen, const int *listen_xyz,
+                           float *const *grad_dev, const long long *grad_strides, int use_graph)
+{
+    if (!b)
+        return PPPM_ERR_ARG;
+    const int n = (int)b->sc.size();
+    if (steps < 0 || !grad_dev || !grad_strides || n_listen < 0 || n_listen > 64 ||
+        (n_listen > 0 && (!listen_xyz || !gout_dev)))
+        return bfail(b, PPPM_ERR_ARG, "batch_adjoint_run: bad arguments (at most 64 listeners per scene)");
+    BRC(check_scenes(b));
+    const int t_last = b->sc[0]->adj_t;
+    for (int i = 0; i < n; i++)
+    {
+        pppm_solver *s = b->sc[i];
+        const std::string sn = "batch_adjoint_run: scene " + std::to_string(i) + ": ";
+        if (!grad_dev[i] || grad_strides[i] < s->nt)
+            return bfail(b, PPPM_ERR_ARG, sn + "gradient missing or stride < nt");
+        if (s->adj_version != s->mesh_version)
+            return bfail(b, PPPM_ERR_STATE, sn + "no adjoint_reset since the mesh changed");
+        if (s->adj_t != t_last)
+            return bfail(b, PPPM_ERR_STATE, sn + "its reverse time differs from scene 0's (reset them together)");
+    }
+    if (steps > t_last + 1)
+        return bfail(b, PPPM_ERR_ARG, "batch_adjoint_run: more steps than remain before t = 0");
+    if (steps == 0)
+        return PPPM_OK;
+    // per-scene set-up: listener tables, pending work of the scenes' own streams
+    const int t_first = t_last - steps + 1;
+    std::vector<AdjScene> hd(n);
+    for (int i = 0; i < n; i++)
+    {
+        pppm_solver *s = b->sc[i];
+        const int rc = solver_set_listeners(s, n_listen > 0 ? listen_xyz + 3 * (size_t)i * n_listen : nullptr, n_listen);
+        if (rc)
+            return bfail(b, rc, "batch_adjoint_run: scene " + std::to_string(i) + ": " + s->err);
+        BCK(hipStreamSynchronize(s->stream));
+        // cotangent rows: (steps, n_scenes, n_listen), scene i at column block i
+        hd[i] = solver_adj_scene(s, n_listen > 0 ? gout_dev + (size_t)i * n_listen : nullptr, (long long)n * n_listen,
+                                 n_listen, grad_dev[i], grad_strides[i], t_first);
+    }
+    BCK(b->adescs.ensure(n));
+    BCK(hipMemcpyAsync(b->adescs.p, hd.data(), n * sizeof(AdjScene), hipMemcpyHostToDevice, b->stream));
+    BatchMap maps[kAdjKernels];
+    for (int k = 0; k < kAdjKernels; k++)
+    {
+        std::vector<int> nb(n);
+        for (int i = 0; i < n; i++)
+            nb[i] = adj_blocks(hd[i], k);
+        BRC(build_map(b, b->m_adj[k], nb));
+        maps[k] = b->m_adj[k].view();
+    }
+    int done = 0;
+    if (use_graph && steps >= 2)
+    {
+        const int chunk = std::min(steps, 32);
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        BCK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        for (int i = 0; i < chunk && e == hipSuccess; i++)
+            e = enqueue_batch_adjoint_step(b->adescs.p, maps, n, b->stream);
+        const hipError_t ee = hipStreamEndCapture(b->stream, &g);
+        if (e != hipSuccess || ee != hipSuccess)
+        {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return bfail(b, PPPM_ERR_HIP, "batch adjoint graph capture failed");
+        }
+        if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess)
+        {
+            (void)hipGraphDestroy(g);
+            return bfail(b, PPPM_ERR_HIP, "batch adjoint graph instantiate failed");
+        }
+        hipError_t le = hipSuccess;
+        for (; done + chunk <= steps && le == hipSuccess; done += chunk)
+            le = hipGraphLaunch(ge, b->stream);
+        const hipError_t se = hipStreamSynchronize(b->stream);
+        (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(g);
+        BCK(le);
+        BCK(se);
+    }
+    for (; done < steps; done++)
+        BCK(enqueue_batch_adjoint_step(b->adescs.p, maps, n, b->stream));
+    BCK(hipStreamSynchronize(b->stream));
+    for (pppm_solver *s : b->sc)
+        s->adj_t -= steps;
     return PPPM_OK;
 }
 

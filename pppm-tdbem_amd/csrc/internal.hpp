@@ -271,18 +271,28 @@ struct AdjScene
     const uint32_t *wf, *wb, *fw;    // the forward's weight records
     const float2 *sf, *sb, *fsc;
     const int *ft_start, *ft_c, *ft_slot;  // per triangle j: face pairs (c, j) and their slots
+    const uint32_t *tfw;                   // ... their face records and scales in that order
+    const float2 *tfsc;
     const int *ct_start, *ct_c;            // per C3 cell: triangles with it as a base-cell corner
     const float *ct_w;                     // ... and rc x their interpolation weight
     const double *den;                     // A_c / 2 - sum_j D_0(c, j)
     const int *lcell, *lc3;
     int n_listen;
-    const float *gout;   // listener cotangents, row t - t_first, n_listen per row (null: none)
+    const float *gout;   // listener cotangents, row t - t_first (n_listen used per row; null: none)
+    long long gout_stride;  // floats between rows of gout
     float *grad;         // gradient of the Neumann inputs, row t - t_first
     long long grad_stride;
     int t_first;
 };
 // one reverse step (t = ast->t, then ast->t - 1)
 hipError_t enqueue_adjoint_step(const AdjScene &d, hipStream_t s);
+// kernels of a reverse step (read, near, Dirichlet divisor, Dirichlet, far, ABC, FDTD) and the
+// block count of kernel `kind` for one scene
+constexpr int kAdjKernels = 7;
+int adj_blocks(const AdjScene &d, int kind);
+// one reverse step of n scenes (descriptor table ds in device memory): maps[k] deals kernel k's
+// blocks to the scenes
+hipError_t enqueue_batch_adjoint_step(const AdjScene *ds, const BatchMap *maps, int n, hipStream_t s);
 
 // ---------------------------------------------------------------- noalloc.hip
 // bytes of the per-device scratch array of the allocation-free entry point

@@ -231,6 +231,8 @@ struct pppm_solver
     int adj_t = -1, adj_n_c3 = 0;
     DBuf<int> adj_tc3, adj_ft_start, adj_ft_c, adj_ft_slot, adj_ct_start, adj_ct_c;
     DBuf<float> adj_ct_w;
+    DBuf<uint32_t> adj_tfw;  // face records in the transposed pair order (per j, increasing c)
+    DBuf<float2> adj_tfsc;
     DBuf<double> adj_den;
     DBuf<float> adj_lam3, adj_ln, adj_ld, adj_unear, adj_lfar, adj_v;
     DBuf<float4> adj_bb;
@@ -269,4 +271,8 @@ int solver_ensure_ordered(pppm_solver *s);
 // the step kernels' view of the handle with its own per-step buffers
 SceneStep solver_scene_step(const pppm_solver *s, const float *neumann, int t0, long long stride, int n_listen,
                             float *out);
+// the adjoint kernels' view of the handle (adjoint.cpp): listener cotangents gout (rows
+// gout_stride floats apart), gradient rows grad_stride floats apart, row 0 = step t_first
+AdjScene solver_adj_scene(pppm_solver *s, const float *gout, long long gout_stride, int n_listen, float *grad,
+                          long long grad_stride, int t_first);
 }  // namespace pppm_amd

@@ -39,6 +39,7 @@ EXPORTS = [
     "pppm_batch_autotune", "pppm_batch_variants", "pppm_batch_set_variants", "pppm_batch_set_timing",
     "pppm_batch_get_timing",
     "pppm_adjoint_reset", "pppm_adjoint_run", "pppm_state_bytes", "pppm_save_state", "pppm_load_state",
+    "pppm_batch_adjoint_reset", "pppm_batch_adjoint_run",
 ]
 
 # LayerWeightHalf record (src/algorithm/bem.h:163-169): 16 x fp16, f32, 16 x fp16, f32
@@ -127,6 +128,8 @@ def load_library(path: str = LIB_PATH):
         "pppm_batch_set_timing": ([vp, i], i),
         "pppm_batch_get_timing": ([vp, vp, vp], i),
         "pppm_adjoint_reset": ([vp, i], i),
+        "pppm_batch_adjoint_reset": ([vp, i], i),
+        "pppm_batch_adjoint_run": ([vp, vp, i, i, vp, vp, vp, i], i),
         "pppm_adjoint_run": ([vp, vp, i, vp, i, vp, ll, i], i),
         "pppm_state_bytes": ([vp, P(ll)], i),
         "pppm_save_state": ([vp, vp], i),
@@ -667,6 +670,43 @@ class PPPMBatch:
         strd = (ctypes.c_longlong * n)(*strides)
         self._check(self.lib.pppm_batch_run(self.h, ptrs, strd, int(steps), int(L), lx.ctypes.data_as(ctypes.c_void_p),
                                             out.ptr, int(use_graph)), "batch_run")
+
+    def adjoint_reset(self, t_last):
+        """Zero adjoint state of every scene; the next reverse step is forward step t_last."""
+        self._check(self.lib.pppm_batch_adjoint_reset(self.h, int(t_last)), "batch_adjoint_reset")
+
+    def adjoint_run(self, gout_dev, steps, listeners, grad_devs, use_graph=True, grad_strides=None):
+        """Reverses `steps` steps of every scene: gout_dev (device, steps x n_scenes x L, the
+        layout of run's output, forward time order; L listeners per scene as in run) -> grad_devs
+        (one device tensor per scene, steps x grad_stride_i: the gradient of each step's Neumann
+        input)."""
+        n = len(self.scenes)
+        if len(grad_devs) != n or len(listeners) != n:
+            raise PPPMError("one gradient and one listener set per scene")
+        if steps < 0:
+            raise PPPMError("adjoint_run: steps >= 0")
+        lx = np.asarray(listeners, dtype=np.int32)
+        if lx.ndim == 2:
+            lx = lx[:, None, :]
+        if lx.ndim != 3 or lx.shape[0] != n or lx.shape[2] != 3 or not 0 <= lx.shape[1] <= 64:
+            raise PPPMError("listeners: (x, y, z) per scene, or the same number (0 to 64) of cells per scene")
+        L = lx.shape[1]
+        lx = np.ascontiguousarray(lx)
+        for i, sc in enumerate(self.scenes):
+            if L and (lx[i].min() < 0 or lx[i].max() >= sc.res()):
+                raise PPPMError(f"scene {i}: listener outside the grid")
+        strides = [sc.nt for sc in self.scenes] if grad_strides is None else [int(x) for x in grad_strides]
+        gr = []
+        for i, (sc, g, st) in enumerate(zip(self.scenes, grad_devs, strides)):
+            if st < sc.nt:
+                raise PPPMError(f"scene {i}: grad_stride < triangle count")
+            gr.append(_arg(g, f"grad[{i}]", (int(steps) - 1) * st + sc.nt if steps > 0 else 0, "f32", "device"))
+        go = _arg(gout_dev, "gout", int(steps) * n * L, "f32", "device") if L else None
+        ptrs = (ctypes.c_void_p * n)(*[a.ptr.value for a in gr])
+        strd = (ctypes.c_longlong * n)(*strides)
+        self._check(self.lib.pppm_batch_adjoint_run(self.h, go.ptr if go else None, int(steps), int(L),
+                                                    lx.ctypes.data_as(ctypes.c_void_p), ptrs, strd, int(use_graph)),
+                    "batch_adjoint_run")
 
     def autotune(self, reps=20):
         """Picks the fastest FDTD interior variant per size class; returns {class_lo: {variant: ms}}."""

@@ -109,6 +109,24 @@ def bandlimited_noise(rng, n_series, n_samples, dt, fmax=4000.0):
     return y / np.maximum(rms, 1e-30)
 
 
+def bandlimited_noise_tm(rng, n_samples, n_series, dt, fmax=4000.0):
+    """bandlimited_noise's kind of signal, generated time-major ((n_samples, n_series) float32)
+    with scipy.fft on every core: for series too large to transpose cheaply (the values differ
+    from bandlimited_noise's)."""
+    from scipy import fft as sfft
+    x = rng.standard_normal((n_samples, n_series), dtype=np.float32)
+    X = sfft.rfft(x, axis=0, workers=-1)
+    del x
+    f = np.fft.rfftfreq(n_samples, d=dt)
+    X[f > fmax, :] = 0
+    X[0, :] = 0
+    y = sfft.irfft(X, n=n_samples, axis=0, workers=-1)
+    del X
+    rms = np.sqrt((y * y).mean(axis=0, dtype=np.float64))
+    y /= np.maximum(rms, 1e-30).astype(np.float32)[None, :]
+    return np.ascontiguousarray(y, dtype=np.float32)
+
+
 def pulsating_sphere(g, dt, radius, r, c):
     """Pressure at distance r of a sphere of radius a whose normal derivative dp/dn = g(t).
 
@@ -302,7 +320,7 @@ def c5_scene(index: int, steps: int = 8192, c: float = 343.0, res: int = 128, se
         cell = tuple(int(np.floor((x - m) / float(h))) for x, m in zip(d, mn))
         if all(4 <= q <= res - 5 for q in cell) and cell not in lis:
             lis.append(cell)
-    target_acc = bandlimited_noise(rng, F.shape[0], steps, float(dt)).T.astype(np.float32)
+    target_acc = bandlimited_noise_tm(rng, steps, F.shape[0], float(dt))
     return Scene(f"c5_scene_{index}", res, float(h), float(dt), mn, c, V, F, steps, lis[0], None,
                  info={"listeners": lis, "target_acc": np.ascontiguousarray(target_acc), "radius": radius})
 

@@ -1,12 +1,46 @@
-"""The batched, sqrt(T)-checkpointed listener loss (autograd.BatchListenerLoss: batched forward
-and recompute, adjoint per scene) against the single-scene checkpointed loss of each scene
-(autograd.ListenerLoss): losses and gradients bit for bit, several listeners per scene; and the
-adjoint identity <y, F g> = <F^T y, g> on a scene of the batched-adjoint configuration's kind
-(radially perturbed icosphere, 8 listeners)."""
+"""The batched adjoint (pppm_batch_adjoint_run) against each scene's own adjoint run, bit for
+bit (uneven scenes, several listeners, graph and eager, reversed in two segments); the batched,
+sqrt(T)-checkpointed listener loss (autograd.BatchListenerLoss: batched forward, recompute and
+adjoint) against the single-scene checkpointed loss of each scene (autograd.ListenerLoss):
+losses and gradients bit for bit; and the adjoint identity <y, F g> = <F^T y, g> on a scene of
+the batched-adjoint configuration's kind (radially perturbed icosphere, 8 listeners)."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_adjoint_equals_single_scene_adjoints(amd, scenes_mod, use_graph):
+    import torch
+    T, L = 70, 3
+    cases = [(24, 1), (41, 3), (30, 2), (52, 2)]
+    scs = [scenes_mod.c3_scene(n, lv, seed=n + 7, steps=T) for n, lv in cases]
+    lis = [[(3 * sc.res // 4 + 2, sc.res // 2, sc.res // 2), (sc.res // 2, 2 + sc.res // 8, sc.res // 2),
+            (sc.res // 3, sc.res // 2 + 3, 2 + sc.res // 5)] for sc in scs]
+    print(f"batched adjoint: {len(scs)} scenes, {T} steps, {L} listeners, graph {use_graph}", flush=True)
+
+    def solver(sc):
+        g = amd.PPPMSolver(sc.res, sc.dl, sc.dt, sc.min_pos, 0, 0.83, sc.c)
+        g.set_mesh(sc.verts, sc.tris)
+        return g
+
+    sv = [solver(sc) for sc in scs]
+    b = amd.PPPMBatch(sv)
+    rng = np.random.default_rng(5)
+    y = torch.from_numpy(rng.standard_normal((T, len(scs), L)).astype(np.float32)).cuda()
+    grads = [torch.full((T, sc.tris.shape[0]), float("nan"), device="cuda") for sc in scs]
+    b.adjoint_reset(T - 1)
+    cut = 29  # two consecutive calls: steps [cut, T), then [0, cut)
+    b.adjoint_run(y[cut:].contiguous(), T - cut, lis, [g[cut:] for g in grads], use_graph=use_graph)
+    b.adjoint_run(y[:cut].contiguous(), cut, lis, [g[:cut] for g in grads], use_graph=use_graph)
+    for i, sc in enumerate(scs):
+        s1 = solver(sc)
+        g1 = torch.zeros(T, sc.tris.shape[0], device="cuda")
+        s1.adjoint_reset(T - 1)
+        s1.adjoint_run(y[:, i].contiguous(), T, lis[i], g1, use_graph=use_graph)
+        assert torch.equal(g1, grads[i]), i
+        assert float(g1.abs().max()) > 0, i
 
 
 def test_batch_loss_equals_single_scene_losses(amd, scenes_mod):
