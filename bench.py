@@ -302,7 +302,8 @@ def adjoint_bench(steps=2000):
     }
 
 
-def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11, step_rule="exact"):
+def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11, step_rule="exact", save_dual=None,
+             resume=None):
     """Batched adjoint at scale: n_scenes scenes of res^3 with a radially perturbed icosphere L5
     (20480 triangles) and 8 listeners each, `steps` steps, source reconstruction by gradient
     descent on the per-triangle accelerations: J_i = sum (F_i(g_i) - target_i)^2, target_i the
@@ -313,6 +314,10 @@ def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11, step_rule="
     one batched forward); "bb" takes it on the first iteration and the Barzilai-Borwein step
     alpha_i = s.s / s.y after it (s, y: the changes of the iterate and of the gradient over the
     last iteration; s.y = 2 |F s|^2 > 0 for this J), which needs no extra forward.
+    A descent can span several runs: every gradient is F^T (2 r) (r = the listener residuals, a
+    (steps, L) series per scene), so the iterate from zero is g = F^T w with w = -2 sum alpha r.
+    save_dual writes w and the loss history (a few MB); resume reads them and rebuilds g with one
+    batched adjoint sweep (its first iteration then takes the exact line search).
     Spot checks on scene 0 at full length: the adjoint identity <y, F g> = <F^T y, g> for random
     g, y, and the directional derivative (J(g + d) - J(g - d)) / 2 against <grad J, d>, which J
     being quadratic makes exact up to rounding. Returns the leg's JSON object."""
@@ -383,17 +388,35 @@ def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11, step_rule="
     rhs = float((grads[0].double() * true[0].double()).sum())
     dot_rel = abs(lhs - rhs) / max(abs(lhs), 1e-300)
     del true, grads, y
-    # gradient descent
+    # gradient descent (from zero, or from the iterate F^T w of an earlier run)
     gs = [torch.zeros(steps, nt, device=dev) for _ in range(n_scenes)]
+    w_dual = torch.zeros(steps, n_scenes, L, device=dev, dtype=torch.float64)
+    hist, it0, t_resume = [], 0, 0.0
+    if resume:
+        z = np.load(resume)
+        if tuple(z["w"].shape) != (steps, n_scenes, L) or int(z["res"]) != res:
+            raise SystemExit(f"{resume}: saved for another configuration")
+        w_dual = torch.from_numpy(z["w"]).to(dev)
+        hist = [h for h in z["hist"]]
+        it0 = int(z["iterations"])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        batch.adjoint_reset(steps - 1)
+        batch.adjoint_run(w_dual.float().contiguous(), steps, lis, gs)
+        torch.cuda.synchronize()
+        t_resume = time.perf_counter() - t0
+        note(f"resumed after {it0} iterations: iterate rebuilt from its dual coefficients in {t_resume:.1f} s")
+    samples = torch.zeros(steps, n_scenes, L, device=dev)
+    tgt = torch.stack(targets, 1).double()
     prev_x = prev_g = None
-    hist, t_loss, t_back, t_ls = [], 0.0, 0.0, 0.0
+    t_loss, t_back, t_ls = 0.0, 0.0, 0.0
     ck_bytes = 0
     torch.cuda.reset_peak_memory_stats()
     for it in range(iters):
         xs = [g.detach().requires_grad_(True) for g in gs]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        J = pad.batch_listener_loss(batch, sv, xs, targets, lis)
+        J = pad.batch_listener_loss(batch, sv, xs, targets, lis, samples=samples)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         J.sum().backward()
@@ -403,7 +426,7 @@ def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11, step_rule="
         t_back += t2 - t1
         ck_bytes = sum(sv_.state_bytes() for sv_ in sv) * math.ceil(steps / math.ceil(math.sqrt(steps)))
         hist.append(J.detach().double().cpu().numpy())
-        note(f"iteration {it + 1}/{iters}: loss / initial {float((hist[-1] / hist[0]).mean()):.4f}; "
+        note(f"iteration {it0 + it + 1}/{it0 + iters}: loss / initial {float((hist[-1] / hist[0]).mean()):.4f}; "
              f"forward {t1 - t0:.1f} s, recompute + adjoint {t2 - t1:.1f} s")
         gr = [x.grad.detach() for x in xs]
         alphas = []
@@ -429,6 +452,7 @@ def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11, step_rule="
             prev_x, prev_g = [x.detach() for x in xs], gr
         for i in range(n_scenes):
             gs[i] = (xs[i].detach() - alphas[i] * gr[i]).contiguous()
+        w_dual -= 2.0 * torch.tensor(alphas, device=dev, dtype=torch.float64)[None, :, None] * (samples.double() - tgt)
         del xs, gr
     # directional derivative on scene 0 at the current iterate (any point works for a quadratic),
     # along a random direction with a component along the gradient (no cancellation in <grad, d>)
@@ -451,13 +475,16 @@ def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11, step_rule="
     dir_rel = abs(fd - an) / max(abs(an), 1e-300)
     free1, _ = torch.cuda.mem_get_info()
     hist = np.array(hist)
+    if save_dual:
+        np.savez(save_dual, w=w_dual.cpu().numpy(), hist=hist, iterations=it0 + iters, res=res)
+        note(f"dual coefficients after {it0 + iters} iterations written to {save_dual}")
     cells = n_scenes * res ** 3 * steps
     per_it = (t_loss + t_back + t_ls) / iters
     res_ = {
         "metric": "batched adjoint: source reconstruction by gradient descent",
         "config": {"workload": f"c5_batched_adjoint_{n_scenes}x{res}^3", "scenes": n_scenes, "grid": f"{res}^3",
                    "triangles_per_scene": nt, "listeners_per_scene": L, "steps": steps, "iterations": iters,
-                   "step_rule": step_rule},
+                   "iterations_before": it0, "step_rule": step_rule},
         "forward_cell_updates_per_s": cells / t_fwd,
         "adjoint_cell_updates_per_s": cells / t_adj,
         "forward_s": t_fwd, "adjoint_s": t_adj, "adjoint_over_forward": t_adj / t_fwd,
@@ -471,7 +498,7 @@ def c5_bench(n_scenes=16, steps=8192, iters=3, res=128, seed_dir=11, step_rule="
         "loss_relative_to_initial": (hist[:, :] / hist[0]).mean(axis=1).tolist(),
         "loss_per_scene_first_last": [hist[0].tolist()[:4], hist[-1].tolist()[:4]],
         "spot_checks": {"adjoint_identity_rel": dot_rel, "directional_derivative_rel": dir_rel},
-        "setup_s": {"scene_generation": t_gen, "set_mesh_all": t_setmesh},
+        "setup_s": {"scene_generation": t_gen, "set_mesh_all": t_setmesh, "resume_adjoint_sweep": t_resume},
         "projected_s_for_50_iterations": 50 * per_it,
     }
     return res_
@@ -584,6 +611,9 @@ def main():
     ap.add_argument("--c5-steps", type=int, default=8192)
     ap.add_argument("--c5-iters", type=int, default=3)
     ap.add_argument("--c5-res", type=int, default=128)
+    ap.add_argument("--c5-save-dual", default=None,
+                    help="write the descent's dual coefficients and loss history here (.npz) at the end")
+    ap.add_argument("--c5-resume", default=None, help="continue the descent saved by --c5-save-dual")
     ap.add_argument("--c5-step", default="exact", choices=["exact", "bb"],
                     help="gradient-descent step: exact line search (one batched forward per iteration) or "
                          "Barzilai-Borwein after the first iteration")
@@ -622,7 +652,8 @@ def main():
         ghost_bench(cpu=not args.no_cpu)
         return
     if args.workload == "c5_batched_adjoint":
-        print(json.dumps(c5_bench(args.c5_scenes, args.c5_steps, args.c5_iters, args.c5_res, step_rule=args.c5_step)))
+        print(json.dumps(c5_bench(args.c5_scenes, args.c5_steps, args.c5_iters, args.c5_res, step_rule=args.c5_step,
+                                  save_dual=args.c5_save_dual, resume=args.c5_resume)))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design note, "

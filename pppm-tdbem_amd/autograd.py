@@ -115,10 +115,11 @@ class BatchListenerLoss(torch.autograd.Function):
     FP64-accumulated float32 tensor;
     the gradient of scene i depends only on its own inputs (the scenes are independent).
     Inputs: batch (PPPMBatch over `solvers`), targets (one (T, L) tensor per scene), listeners
-    (L cells per scene), then the per-scene (T, nt_i) inputs g_i."""
+    (L cells per scene), samples (None, or a (T, n_scenes, L) device tensor that receives the
+    listener samples of the forward run), then the per-scene (T, nt_i) inputs g_i."""
 
     @staticmethod
-    def forward(ctx, batch, solvers, targets, listeners, segment, use_graph, *gs):
+    def forward(ctx, batch, solvers, targets, listeners, segment, use_graph, samples, *gs):
         n = len(solvers)
         for g, s in zip(gs, solvers):
             _check_inputs(g, s)
@@ -140,6 +141,8 @@ class BatchListenerLoss(torch.autograd.Function):
             for i, s in enumerate(solvers):
                 s.save_state(ck[i][k])
             batch.run([g[a:b] for g in gs], b - a, listeners, out[:b - a], use_graph=use_graph)
+            if samples is not None:
+                samples[a:b] = out[:b - a]
             for i in range(n):
                 J[i] += ((out[:b - a, i].double() - targets[i][a:b].double()) ** 2).sum()
         ctx.save_for_backward(*gs)
@@ -166,8 +169,8 @@ class BatchListenerLoss(torch.autograd.Function):
             gout = scale * (out[:b - a] - tgt[a:b])
             ctx.batch.adjoint_run(gout.contiguous(), b - a, ctx.listeners, [gr[a:b] for gr in grads],
                                   use_graph=ctx.use_graph)
-        return (None, None, None, None, None, None) + tuple(grads)
+        return (None, None, None, None, None, None, None) + tuple(grads)
 
 
-def batch_listener_loss(batch, solvers, gs, targets, listeners, segment=None, use_graph=True):
-    return BatchListenerLoss.apply(batch, solvers, targets, listeners, segment, use_graph, *gs)
+def batch_listener_loss(batch, solvers, gs, targets, listeners, segment=None, use_graph=True, samples=None):
+    return BatchListenerLoss.apply(batch, solvers, targets, listeners, segment, use_graph, samples, *gs)

@@ -278,8 +278,8 @@ int pppm_adjoint_reset(pppm_solver *s, int t_last)
         return solver_fail(s, PPPM_ERR_ARG, "adjoint: the PML is not differentiated (ABC only)");
     if (t_last < -1)
         return solver_fail(s, PPPM_ERR_ARG, "adjoint_reset: t_last >= -1");
-    if ((long long)s->N * s->N * s->N >= (1LL << 31))
-        return solver_fail(s, PPPM_ERR_ARG, "adjoint: grids below 2^31 cells (32-bit cell indices)");
+    if ((long long)s->N * s->N * s->N >= (1LL << 31) || s->N < 8)
+        return solver_fail(s, PPPM_ERR_ARG, "adjoint: grid edges from 8 and grids below 2^31 cells");
     ARC(adjoint_setup(s));
     const size_t NN = (size_t)s->N * s->N * s->N;
     ACK(hipMemsetAsync(s->adj_lam3.p, 0, 3 * NN * sizeof(float), s->stream));

@@ -375,44 +375,20 @@ __device__ __forceinline__ void adj_abc(const AdjScene &d, int blk)
 }
 
 // 6b. per cell: the transposed leapfrog (interior cells that read it) and the transposed ABC
-// (shell cells that read it, with the coefficients of fdtd.cu:94-118 for each neighbour slot)
-__device__ __forceinline__ void adj_fdtd(const AdjScene &d, int blk)
+// (shell cells that read it, with the coefficients of fdtd.cu:94-118 for each neighbour slot).
+// The blocks first cover the deep interior (every index in [3, N-4]: no shell cell reads the
+// cell or its neighbours, the transposed leapfrog alone), then the frame of the 3 outer layers
+// of every face, so no wave mixes the two paths.
+__device__ __forceinline__ void adj_fdtd_frame_cell(const AdjScene &d, int y0, int y1, int y2)
 {
     const int N = d.fp.N;
     const size_t nn = (size_t)N * N * N;
-    // 32-bit cell indices (the adjoint takes grids below 2^31 cells)
-    const unsigned NN2 = (unsigned)N * (unsigned)N;
-    const unsigned uid = (unsigned)blk * BLK + threadIdx.x;
-    if (uid >= (unsigned)nn)
-        return;
-    const int id = (int)uid;
-    const int y0 = (int)(uid / NN2);
-    const unsigned r = uid - (unsigned)y0 * NN2;
-    const int y1 = (int)(r / (unsigned)N), y2 = (int)(r - (unsigned)y1 * (unsigned)N);
+    const int id = lin3(y0, y1, y2, N);
     const int yc[3] = {y0, y1, y2};
     const int t = d.ast->t;
     const float *lam = alevel(d.lam3, t, 0, nn);
     float *lam1 = alevel(d.lam3, t, -1, nn);
     float *lam2 = alevel(d.lam3, t, -2, nn);
-    if (y0 >= 3 && y1 >= 3 && y2 >= 3 && y0 <= N - 4 && y1 <= N - 4 && y2 <= N - 4)
-    {
-        // no ABC shell cell reads this cell or any of its neighbours: the transposed leapfrog
-        // alone, with the general path's operations in its order
-        const double cc = 2.0 - 6.0 * d.fp.coef / d.fp.hh, cn = d.fp.coef / d.fp.hh;
-        const int sN2 = (int)NN2;
-        double ag = 0.0, am = 0.0;
-        ag += cc * (double)lam[id];
-        am -= (double)lam[id];
-        ag += cn * (double)lam[id - sN2];
-        ag += cn * (double)lam[id + sN2];
-        ag += cn * (double)lam[id - N];
-        ag += cn * (double)lam[id + N];
-        ag += cn * (double)lam[id - 1];
-        ag += cn * (double)lam[id + 1];
-        lam1[id] += (float)ag;
-        lam2[id] = (float)am;
-        return;
-    }
     int out = 0;
     for (int e = 0; e < 3; e++)
         out += yc[e] < 1 || yc[e] > N - 2;
@@ -481,6 +457,71 @@ __device__ __forceinline__ void adj_fdtd(const AdjScene &d, int blk)
     if (out == 0)
         lam1[id] += (float)ag;
     lam2[id] = (float)am;
+}
+
+__device__ __forceinline__ void adj_fdtd(const AdjScene &d, int blk)
+{
+    const int N = d.fp.N;
+    const unsigned M = (unsigned)(N - 6), MM = M * M;
+    const int nbI = (int)((MM * M + BLK - 1) / BLK);
+    if (blk < nbI)
+    {
+        const unsigned u = (unsigned)blk * BLK + threadIdx.x;
+        if (u >= MM * M)
+            return;
+        const unsigned x = u / MM, r = u - x * MM, y = r / M, z = r - y * M;
+        const int id = lin3(3 + (int)x, 3 + (int)y, 3 + (int)z, N);
+        const size_t nn = (size_t)N * N * N;
+        const int t = d.ast->t;
+        const float *lam = alevel(d.lam3, t, 0, nn);
+        float *lam1 = alevel(d.lam3, t, -1, nn);
+        float *lam2 = alevel(d.lam3, t, -2, nn);
+        // the general path's operations, in its order
+        const double cc = 2.0 - 6.0 * d.fp.coef / d.fp.hh, cn = d.fp.coef / d.fp.hh;
+        const int sN2 = N * N;
+        double ag = 0.0, am = 0.0;
+        ag += cc * (double)lam[id];
+        am -= (double)lam[id];
+        ag += cn * (double)lam[id - sN2];
+        ag += cn * (double)lam[id + sN2];
+        ag += cn * (double)lam[id - N];
+        ag += cn * (double)lam[id + N];
+        ag += cn * (double)lam[id - 1];
+        ag += cn * (double)lam[id + 1];
+        lam1[id] += (float)ag;
+        lam2[id] = (float)am;
+        return;
+    }
+    // frame: the 6 outer x planes (6 N^2 cells), then in the other x planes the 6 outer y rows
+    // (6 M N), then in the other (x, y) the 6 outer z cells (6 M^2); outer index k -> {0,1,2,N-3,N-2,N-1}
+    unsigned f = (unsigned)(blk - nbI) * BLK + threadIdx.x;
+    const unsigned uN = (unsigned)N, NN2 = uN * uN;
+    auto outer = [&](unsigned k) { return k < 3 ? (int)k : N - 6 + (int)k; };
+    int x, y, z;
+    if (f < 6 * NN2)
+    {
+        const unsigned p = f / NN2, r = f - p * NN2;
+        x = outer(p);
+        y = (int)(r / uN);
+        z = (int)(r - (unsigned)y * uN);
+    }
+    else if ((f -= 6 * NN2) < 6 * M * uN)
+    {
+        const unsigned xi = f / (6 * uN), r = f - xi * 6 * uN, q = r / uN;
+        x = 3 + (int)xi;
+        y = outer(q);
+        z = (int)(r - q * uN);
+    }
+    else if ((f -= 6 * M * uN) < 6 * MM)
+    {
+        const unsigned xi = f / (6 * M), r = f - xi * 6 * M, yi = r / 6;
+        x = 3 + (int)xi;
+        y = 3 + (int)yi;
+        z = outer(r - yi * 6);
+    }
+    else
+        return;
+    adj_fdtd_frame_cell(d, x, y, z);
 }
 
 __global__ void k_adj_back(AdjState *st) { st->t = st->t - 1; }
@@ -561,7 +602,11 @@ int adj_blocks(const AdjScene &d, int kind)
     case kFar:  // the C3 part's blocks, then the triangle part's
         return (int)std::max<long long>(1, (d.n_c3 + BLK - 1) / BLK + ((long long)d.nt * STEPS + BLK - 1) / BLK);
     case kAbc: items = shell; break;
-    default: items = nn; break;
+    default:  // the deep interior's blocks, then the frame's
+    {
+        const long long M = N - 6;
+        return (int)((M * M * M + BLK - 1) / BLK + (nn - M * M * M + BLK - 1) / BLK);
+    }
     }
     return (int)std::max<long long>(1, (items + BLK - 1) / BLK);
 }

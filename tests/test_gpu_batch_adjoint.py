@@ -95,3 +95,48 @@ def test_adjoint_identity_on_a_perturbed_l5_scene(amd, scenes_mod):
     rel = abs(lhs - rhs) / abs(lhs)
     print(f"  <y, F g> = {lhs:.6e}, <F^T y, g> = {rhs:.6e}, relative difference {rel:.2e}")
     assert abs(lhs) > 0 and rel < 1e-3
+
+
+def test_descent_iterate_equals_adjoint_of_its_dual_coefficients(amd, scenes_mod):
+    """Gradient descent from zero on J_i = |F_i g_i - target_i|^2 keeps g_i = F_i^T w_i with
+    w_i = -2 sum_k alpha_k (F_i g_k - target_i): the iterate rebuilt by one batched adjoint sweep
+    of w matches the accumulated one (how a long descent continues in another run)."""
+    import torch
+    import autograd as pad
+    T, L = 48, 2
+    scs = [scenes_mod.c3_scene(n, lv, seed=n + 3, steps=T) for n, lv in [(24, 1), (30, 2)]]
+    lis = [[(3 * sc.res // 4 + 2, sc.res // 2, sc.res // 2), (sc.res // 2, 2 + sc.res // 8, sc.res // 2)]
+           for sc in scs]
+    print(f"descent dual coefficients: {len(scs)} scenes, {T} steps, {L} listeners", flush=True)
+    sv = []
+    for sc in scs:
+        g = amd.PPPMSolver(sc.res, sc.dl, sc.dt, sc.min_pos, 0, 0.83, sc.c)
+        g.set_mesh(sc.verts, sc.tris)
+        sv.append(g)
+    b = amd.PPPMBatch(sv)
+    rng = np.random.default_rng(9)
+    targets = [torch.from_numpy(rng.standard_normal((T, L)).astype(np.float32) * 1e-4).cuda() for _ in scs]
+    tgt = torch.stack(targets, 1).double()
+    gs = [torch.zeros(T, sc.tris.shape[0], device="cuda") for sc in scs]
+    w = torch.zeros(T, len(scs), L, device="cuda", dtype=torch.float64)
+    smp = torch.zeros(T, len(scs), L, device="cuda")
+    out = torch.zeros(T, len(scs), L, device="cuda")
+    for _ in range(3):
+        xs = [g.detach().requires_grad_(True) for g in gs]
+        J = pad.batch_listener_loss(b, sv, xs, targets, lis, segment=7, samples=smp)
+        J.sum().backward()
+        gr = [x.grad.detach() for x in xs]
+        for s in sv:
+            s.reset()
+        b.run(gr, T, lis, out)
+        alphas = [float((gr[i].double() ** 2).sum()) / (2.0 * float((out[:, i].double() ** 2).sum()))
+                  for i in range(len(scs))]
+        gs = [(xs[i].detach() - alphas[i] * gr[i]).contiguous() for i in range(len(scs))]
+        w -= 2.0 * torch.tensor(alphas, device="cuda", dtype=torch.float64)[None, :, None] * (smp.double() - tgt)
+    rebuilt = [torch.zeros_like(g) for g in gs]
+    b.adjoint_reset(T - 1)
+    b.adjoint_run(w.float().contiguous(), T, lis, rebuilt)
+    for i in range(len(scs)):
+        rel = float((rebuilt[i] - gs[i]).double().norm() / gs[i].double().norm())
+        print(f"  scene {i}: |F^T w - g| / |g| = {rel:.2e}")
+        assert float(gs[i].abs().max()) > 0 and rel < 1e-4, i
