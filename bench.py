@@ -809,6 +809,8 @@ def main():
         "phase_time_share": phase_share,
         "kernels": kernels,
         "listener_final": listener,
+        # face-cache slots handed out since set_mesh (never reclaimed) against the live face pairs
+        "face_cache": {"slots_used": st["pool"], "live_pairs": st["P4"]},
     }
     if not args.no_traffic:
         kname = res["roofline"]["kernel"]
