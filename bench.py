@@ -722,6 +722,23 @@ def main():
         work = solver.get_update_work()
         stream.synchronize()
 
+    # the practical HBM ceiling beside the published peak: a plain 1 GiB device-to-device copy
+    # (read + write bytes), timed with HIP events
+    copy_gbps = None
+    try:
+        ca = torch.ones(1 << 28, device="cuda")
+        cb = torch.empty_like(ca)
+        cb.copy_(ca)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            cb.copy_(ca)
+        e1.record()
+        torch.cuda.synchronize()
+        copy_gbps = 2 * ca.numel() * 4 * 10 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+        del ca, cb
+    except RuntimeError:
+        pass
     if args.calib_copy_mib > 0:
         n = args.calib_copy_mib * (1 << 20) // 4
         a = torch.ones(n, device="cuda")
@@ -803,6 +820,7 @@ def main():
             "traffic": None,
             "bytes_per_launch": bytes_[dom],
             "avg_launch_us": avg[dom] * 1e3,
+            "device_copy_gbps": copy_gbps,
         },
         "hbm_gbps_effective_step": step_bytes / (ms_step * 1e-3) / 1e9,
         "phase_avg_us": {k: (v * 1e3 if v else None) for k, v in avg.items()},
