@@ -88,3 +88,38 @@ class _FakeCuda:
 
     def numel(self):
         return self.t.numel()
+
+
+def _batch(amd, scenes):
+    b = amd.PPPMBatch.__new__(amd.PPPMBatch)
+    b.lib, b.h, b.scenes = _NoCalls(), None, list(scenes)
+    return b
+
+
+def test_batch_run_and_adjoint_inputs_are_checked(amd):
+    import torch
+    sv = [_solver(amd, nt=10, res=16), _solver(amd, nt=12, res=20)]
+    b = _batch(amd, sv)
+    lis = [[(1, 1, 1)], [(2, 2, 2)]]
+    host = [np.zeros(5 * 10, np.float32), np.zeros(5 * 12, np.float32)]
+    out = torch.zeros(5, 2, 1)
+    with pytest.raises(amd.PPPMError, match="one Neumann series and one listener set per scene"):
+        b.run(host[:1], 5, lis, out)
+    with pytest.raises(amd.PPPMError, match="must be device memory"):
+        b.run(host, 5, lis, out)
+    with pytest.raises(amd.PPPMError, match="stride < triangle count"):
+        b.run(host, 5, lis, out, strides=[9, 12])
+    grads = [torch.zeros(5, 10), torch.zeros(5, 12)]   # CPU tensors: not device memory
+    gout = torch.zeros(5, 2, 1)
+    with pytest.raises(amd.PPPMError, match="one gradient and one listener set per scene"):
+        b.adjoint_run(gout, 5, lis, grads[:1])
+    with pytest.raises(amd.PPPMError, match="steps >= 0"):
+        b.adjoint_run(gout, -1, lis, grads)
+    with pytest.raises(amd.PPPMError, match="listener outside the grid"):
+        b.adjoint_run(gout, 5, [[(1, 1, 1)], [(2, 2, 20)]], grads)
+    with pytest.raises(amd.PPPMError, match="grad_stride < triangle count"):
+        b.adjoint_run(gout, 5, lis, grads, grad_strides=[9, 12])
+    with pytest.raises(amd.PPPMError, match="must be device memory"):
+        b.adjoint_run(gout, 5, lis, grads)
+    with pytest.raises(amd.PPPMError, match="needs"):
+        b.adjoint_run(gout, 5, lis, [torch.zeros(4, 10), torch.zeros(5, 12)])
