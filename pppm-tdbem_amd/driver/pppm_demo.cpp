@@ -32,107 +32,36 @@
 namespace pppm_demo
 {
 
-namespace
+// Closest point of triangle (a, b, c) to p by Voronoi regions (Ericson, "Real-Time Collision
+// Detection", 2005, section 5.1.5): the vertex regions, then the edge regions, then the face,
+// each decided from the dot products of the edges with p's offsets from the vertices. A point in
+// a vertex's region gets that vertex itself, so a mesh vertex is at distance 0 from every
+// triangle that holds it.
+F3 closest_point(F3 p, F3 a, F3 b, F3 c)
 {
-struct P2
-{
-    float s, t;
-};
-P2 edge02(float a11, float b1)
-{
-    const int neg = b1 < 0, past = a11 + b1 <= 0;
-    return {0.0f, (float)neg * ((float)past - (float)(1 - past) * (b1 / a11))};
-}
-P2 edge12(float a01, float a11, float b1, float f10, float f01)
-{
-    const float g0 = a01 + b1 - f10, g1 = a11 + b1 - f01;
-    const int neg = g0 < 0, past = g1 <= 0;
-    const float t = (float)neg * ((float)past + (float)(1 - past) * g0 / (g0 - g1));
-    return {1 - t, t};
-}
-P2 inside(P2 q0, float g0, P2 q1, float g1)
-{
-    const float z = g0 / (g0 - g1), w = 1 - z;
-    return {w * q0.s + z * q1.s, w * q0.t + z * q1.t};
-}
-}  // namespace
-
-F3 closest_point(F3 p, F3 v0, F3 v1, F3 v2)
-{
-    const F3 d = p - v0, e0 = v1 - v0, e1 = v2 - v0;
-    const float a00 = dot(e0, e0), a01 = dot(e0, e1), a11 = dot(e1, e1);
-    const float b0 = -dot(d, e0), b1 = -dot(d, e1);
-    const float f00 = b0, f10 = b0 + a00, f01 = b0 + a01;
-    P2 q0{0, 0}, q1{0, 0}, r{0, 0};
-    float g0, g1;
-    if (f00 >= 0)
-    {
-        if (f01 >= 0)
-            r = edge02(a11, b1);
-        else
-        {
-            q0 = {0, f00 / (f00 - f01)};
-            q1.s = f01 / (f01 - f10);
-            q1.t = 1 - q1.s;
-            const float dt1 = q1.t - q0.t;
-            g0 = dt1 * (a11 * q0.t + b1);
-            if (g0 >= 0)
-                r = edge02(a11, b1);
-            else
-            {
-                g1 = dt1 * (a01 * q1.s + a11 * q1.t + b1);
-                r = g1 <= 0 ? edge12(a01, a11, b1, f10, f01) : inside(q0, g0, q1, g1);
-            }
-        }
-    }
-    else if (f01 <= 0)
-    {
-        if (f10 <= 0)
-            r = edge12(a01, a11, b1, f10, f01);
-        else
-        {
-            q0 = {f00 / (f00 - f10), 0};
-            q1.s = f01 / (f01 - f10);
-            q1.t = 1 - q1.s;
-            g0 = q1.t * (a01 * q0.s + b1);
-            if (g0 >= 0)
-                r = q0;
-            else
-            {
-                g1 = q1.t * (a01 * q1.s + a11 * q1.t + b1);
-                r = g1 <= 0 ? edge12(a01, a11, b1, f10, f01) : inside(q0, g0, q1, g1);
-            }
-        }
-    }
-    else if (f10 <= 0)
-    {
-        q0 = {0, f00 / (f00 - f01)};
-        q1.s = f01 / (f01 - f10);
-        q1.t = 1 - q1.s;
-        const float dt1 = q1.t - q0.t;
-        g0 = dt1 * (a11 * q0.t + b1);
-        if (g0 >= 0)
-            r = edge02(a11, b1);
-        else
-        {
-            g1 = dt1 * (a01 * q1.s + a11 * q1.t + b1);
-            r = g1 <= 0 ? edge12(a01, a11, b1, f10, f01) : inside(q0, g0, q1, g1);
-        }
-    }
-    else
-    {
-        q0 = {f00 / (f00 - f10), 0};
-        q1 = {0, f00 / (f00 - f01)};
-        g0 = q1.t * (a01 * q0.s + b1);
-        if (g0 >= 0)
-            r = q0;
-        else
-        {
-            g1 = q1.t * (a11 * q1.t + b1);
-            r = g1 <= 0 ? edge02(a11, b1) : inside(q0, g0, q1, g1);
-        }
-    }
-    return (v0 + r.s * e0) + r.t * e1;
+    const F3 ab = b - a, ac = c - a, ap = p - a;
+    const float d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0 && d2 <= 0)
+        return a;
+    const F3 bp = p - b;
+    const float d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0 && d4 <= d3)
+        return b;
+    const float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0)
+        return a + (d1 / (d1 - d3)) * ab;
+    const F3 cp = p - c;
+    const float d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0 && d5 <= d6)
+        return c;
+    const float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d2 >= 0 && d6 <= 0)
+        return a + (d2 / (d2 - d6)) * ac;
+    const float va = d3 * d6 - d5 * d4;
+    if (va <= 0 && d4 - d3 >= 0 && d5 - d6 >= 0)
+        return b + ((d4 - d3) / ((d4 - d3) + (d5 - d6))) * (c - b);
+    const float den = 1.0f / (va + vb + vc);
+    return a + (vb * den) * ab + (vc * den) * ac;
 }
 
 // A rigid body with modal sound (RigidBody, src/utils/RigidBody.h:53-166): per audio sample one

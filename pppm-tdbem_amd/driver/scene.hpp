@@ -201,9 +201,9 @@ inline F3 rotate(F4 q, F3 v)
     return ((2.0f * dot(u, v)) * u + (s * s - dot(u, u)) * v) + (2.0f * s) * cross(u, v);
 }
 
-// point of triangle (v0, v1, v2) closest to p, as v0 + s e0 + t e1 (Eberly, "Distance from a
-// Point to a Triangle", Geometric Tools 2020; same branches as helper_math.h:1810-1990)
-F3 closest_point(F3 p, F3 v0, F3 v1, F3 v2);
+// point of triangle (a, b, c) closest to p (Voronoi-region tests; a vertex of the triangle is
+// returned exactly, so the triangles that hold a mesh vertex are at distance 0 from it)
+F3 closest_point(F3 p, F3 a, F3 b, F3 c);
 
 // ------------------------------------------------------------------ objects
 struct ObjectState
