@@ -13,8 +13,9 @@
 //
 // A mesh update is a short chain of launches; each launch does every piece of work whose
 // inputs are complete, in block ranges of one grid:
-//   k_tri_update   geometry, cell / base-cell counts, 3x3x3 marks, grid-cache dirty flags,
-//                  interpolation weights (and the vertex copy)
+//   k_tri_update   geometry, cell / base-cell counts, 3x3x3 marks, grid-cache dirty flags and
+//                  the recompute inputs of the dirty triangles, interpolation weights (and the
+//                  vertex copy)
 //   [scan batch]   cell CSR offsets; compaction of base cells, marked cells, dirty triangles
 //   k_lists_count  cell fill (atomic slots), 3x3x3 list lengths of the marked cells, face-list
 //                  lengths (the 4x4x4 block around each triangle's base cell)
@@ -241,8 +242,8 @@ __device__ inline void wave_map(int i, const int *__restrict__ start, int n, int
 __global__ __launch_bounds__(B) void k_tri_update(const float *__restrict__ src, float *__restrict__ dst, int nv3,
                                                   const int *__restrict__ tris, TriGeom *__restrict__ geom,
                                                   BuildArgs ba, WeightArgs wa, const float *__restrict__ gmeta,
-                                                  int first, int *__restrict__ gflag, float *__restrict__ interp,
-                                                  UpdateCounters cn, int *err)
+                                                  int first, int *__restrict__ gflag, GridTriIn *__restrict__ gin,
+                                                  float *__restrict__ interp, UpdateCounters cn, int *err)
 {
     const int i = blockIdx.x * B + threadIdx.x;
     if (dst && i < nv3)
@@ -262,9 +263,39 @@ __global__ __launch_bounds__(B) void k_tri_update(const float *__restrict__ src,
     const int N = ba.N;
     for (int o = 0; o < GC; o++)
         cn.mark[lin3(g.cell3.x + o % 3 - 1, g.cell3.y + o / 3 % 3 - 1, g.cell3.z + o / 9 - 1, N)] = 1;
-    gflag[i] = (first || grid_needs_recompute(g, gmeta + 7 * i, ba.h)) ? 1 : 0;
+    const int dirty = (first || grid_needs_recompute(g, gmeta + 7 * i, ba.h)) ? 1 : 0;
+    gflag[i] = dirty;
     if (!(e & ERR_BAD_INDEX))
+    {
+        if (dirty)
+        {
+            // the grid-cache recompute's inputs (its one-point rule at the centroid)
+            const TriD T = tri_d(src, tris[3 * i], tris[3 * i + 1], tris[3 * i + 2]);
+            const d3 p = l2g(TRI1_X, TRI1_X, T.v);
+            GridTriIn r;
+            r.px = p.x;
+            r.py = p.y;
+            r.pz = p.z;
+            r.nx = T.n.x;
+            r.ny = T.n.y;
+            r.nz = T.n.z;
+            r.jac = T.jac;
+            r.cx = g.cell3.x;
+            r.cy = g.cell3.y;
+            r.cz = g.cell3.z;
+            r.tri = i;
+            r.meta[0] = g.center.x;
+            r.meta[1] = g.center.y;
+            r.meta[2] = g.center.z;
+            r.meta[3] = g.normal.x;
+            r.meta[4] = g.normal.y;
+            r.meta[5] = g.normal.z;
+            r.meta[6] = g.area;
+            r.pad = 0;
+            gin[i] = r;
+        }
         interp_weights(src, tris, i, g.base3, wa, interp + 8 * i);
+    }
 }
 
 // block ranges: [0, r1) cell fill, [r1, r2) 3x3x3 list lengths of the marked cells,
@@ -379,11 +410,12 @@ hipError_t launch_sort_cells(const TriGeom *geom, int nt, const int *start, int 
     return hipGetLastError();
 }
 hipError_t launch_tri_update(const float *src, float *dst, int nv, const int *tris, TriGeom *geom, const BuildArgs &ba,
-                             const WeightArgs &wa, const float *gmeta, int first, int *gflag, float *interp,
-                             const UpdateCounters &cn, int *err, hipStream_t s)
+                             const WeightArgs &wa, const float *gmeta, int first, int *gflag, GridTriIn *gin,
+                             float *interp, const UpdateCounters &cn, int *err, hipStream_t s)
 {
     const long long n = std::max<long long>(ba.nt, dst ? 3LL * nv : 0);
-    k_tri_update<<<nblk(n), B, 0, s>>>(src, dst, 3 * nv, tris, geom, ba, wa, gmeta, first, gflag, interp, cn, err);
+    k_tri_update<<<nblk(n), B, 0, s>>>(src, dst, 3 * nv, tris, geom, ba, wa, gmeta, first, gflag, gin, interp, cn,
+                                       err);
     return hipGetLastError();
 }
 hipError_t launch_lists_count(const ListArgs &a, hipStream_t s)

@@ -84,4 +84,20 @@ struct TriGeom
     int3_ cell3, base3;
 };
 
+// Inputs of a triangle's grid-cache recompute, written by the binning pass for the triangles it
+// flags dirty: the one-point quadrature point (the centroid), the unit normal and twice the
+// area in FP64, the centre's cell, and the centre / normal / area the recompute test compares
+// against next time. 13 eight-byte words, so a wave copies one with a load per lane.
+struct GridTriIn
+{
+    double px, py, pz;
+    double nx, ny, nz;
+    double jac;
+    int cx, cy, cz;
+    int tri;
+    float meta[7];
+    float pad;
+};
+static_assert(sizeof(GridTriIn) == 104 && sizeof(GridTriIn) % 8 == 0, "GridTriIn is 13 eight-byte words");
+
 }  // namespace pppm_amd

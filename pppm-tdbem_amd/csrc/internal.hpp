@@ -152,11 +152,12 @@ struct ListArgs
 // in-cell order of the cell CSR (classification needs ascending triangle indices)
 hipError_t launch_sort_cells(const TriGeom *geom, int nt, const int *start, int *items, const int *rank,
                              int use_base, hipStream_t s);
-// geometry, counts, marks, grid-cache dirty flags, interpolation weights; dst != null copies
-// the nv vertices of src into dst in the same launch
+// geometry, counts, marks, grid-cache dirty flags (and the recompute inputs of the dirty
+// triangles), interpolation weights; dst != null copies the nv vertices of src into dst in the
+// same launch
 hipError_t launch_tri_update(const float *src, float *dst, int nv, const int *tris, TriGeom *geom, const BuildArgs &ba,
-                             const WeightArgs &wa, const float *gmeta, int first, int *gflag, float *interp,
-                             const UpdateCounters &cn, int *err, hipStream_t s);
+                             const WeightArgs &wa, const float *gmeta, int first, int *gflag, GridTriIn *gin,
+                             float *interp, const UpdateCounters &cn, int *err, hipStream_t s);
 hipError_t launch_lists_count(const ListArgs &a, hipStream_t s);
 hipError_t launch_lists_fill(const ListArgs &a, hipStream_t s);
 // 4x4x4 lists of the non-empty base cells: fill = 0 counts (b4_cnt), fill = 1 writes b4_tris
@@ -177,12 +178,11 @@ struct Tdbem
 void tdbem_init(Tdbem &b, float dt, float c, float grid_size);
 
 size_t face_tmp_bytes(int max_dirty);
-// records tri*27+o of the *n_ptr triangles in list (cap bounds *n_ptr). hint_dirty: host
-// estimate of *n_ptr that sizes the grid-stride launch (-1: cap); any value is correct.
-hipError_t launch_grid_weights(const float *verts, const int *tris, const TriGeom *geom, const int *list,
-                               const int *n_ptr, int cap, const WeightArgs &a, const Tdbem &b, uint32_t *wf,
-                               float2 *sf, uint32_t *wb, float2 *sb, float *meta, const int *err,
-                               long long hint_dirty, hipStream_t s);
+// records tri*27+o of the *n_ptr triangles in list (cap bounds *n_ptr), from their inputs in
+// gin (indexed by triangle, written by the binning pass)
+hipError_t launch_grid_weights(const GridTriIn *gin, const int *list, const int *n_ptr, int cap, const WeightArgs &a,
+                               const Tdbem &b, uint32_t *wf, float2 *sf, uint32_t *wb, float2 *sb, float *meta,
+                               const int *err, hipStream_t s);
 // face lists and slots of the previous update (start == null: none); cur_start = this update's
 // face-list offsets
 struct PrevFaces

@@ -362,19 +362,18 @@ int enqueue_rebuild(pppm_solver *s, bool first, const float *verts_series, long 
     const float *src = vsrc ? vsrc : s->verts.p;
     int sp = span_begin(s, 7);
     CK(launch_tri_update(src, vsrc ? s->verts.p : nullptr, s->nv, s->tris.p, s->geom.p, ba, wa, s->gmeta.p,
-                         first ? 1 : 0, s->gflag.p, s->interp.p, cn, err, st));
+                         first ? 1 : 0, s->gflag.p, s->gin.p, s->interp.p, cn, err, st));
 
     // cell CSR offsets; compaction of non-empty base cells, marked cells, dirty triangles
     CK(scan_batch(b1, s->scan_tmp.p, st));
 
-    // grid cache: dirty triangles only. Launch sizes follow the previous update's recompute
-    // counts (the kernels stride, so a larger count is still covered); a first build sizes
-    // for everything.
-    const long long ghint = first ? -1 : 2 * s->last_grid_recomputes + 64;
+    // grid cache: dirty triangles only (one block per CU, striding over the device-side
+    // count). The face-cache launch sizes follow the previous update's recompute count (the
+    // kernels stride, so a larger count is still covered); a first build sizes for everything.
     const long long fhint = first ? -1 : 2 * s->last_face_recomputes + 64;
     auto grid_weights = [&](hipStream_t gs) {
-        return launch_grid_weights(s->verts.p, s->tris.p, s->geom.p, s->glist.p, stp + ST_GDIRTY, nt, wa, s->bem,
-                                   s->wf.p, s->sf.p, s->wb.p, s->sb.p, s->gmeta.p, err, ghint, gs);
+        return launch_grid_weights(s->gin.p, s->glist.p, stp + ST_GDIRTY, nt, wa, s->bem, s->wf.p, s->sf.p, s->wb.p,
+                                   s->sb.p, s->gmeta.p, err, gs);
     };
     const bool fork = fork_update(s);
     if (fork)
@@ -1060,6 +1059,7 @@ int pppm_set_mesh(pppm_solver *s, const float *verts, int nv, const int *tris, i
     CK(s->sf.ensure((size_t)nt * GC));
     CK(s->sb.ensure((size_t)nt * GC));
     CK(s->gmeta.ensure((size_t)nt * 7));
+    CK(s->gin.ensure((size_t)nt));
     CK(s->interp.ensure((size_t)nt * 8));
     // first guess for the 4x4x4 list capacities; rebuild_sync grows them if needed
     s->cap_b4items = std::max(s->cap_b4items, 16 * nt + 1024);

@@ -138,6 +138,7 @@ struct pppm_solver
     DBuf<uint32_t> wf, wb;
     DBuf<float2> sf, sb;
     DBuf<float> gmeta, interp;
+    DBuf<pppm_amd::GridTriIn> gin;  // grid-cache recompute inputs of the dirty triangles (by triangle)
 
     int pool_cap = 0, hcap = 0, pool_count = 0;
     DBuf<uint32_t> fw;

@@ -298,44 +298,191 @@ constexpr Targets81 make_targets81()
     return t;
 }
 __constant__ Targets81 c_t81 = make_targets81();
-// threads per triangle block of the grid-cache weights: one per phase-1 evaluation (729), two
-// rounds for the butterflies and the taps (each phase strides over its items; 256-thread blocks,
-// three triangles per CU, made the moving step 5 us slower)
-constexpr int GW = 768;
-static_assert(GW % STEPS == 0 && GW % WAVE == 0 && GW >= GC * 9, "tap groups stay inside one wave");
+// Threads per block of the grid-cache weights: phase 1 in two rounds over the 729 evaluations,
+// phases 2-4 on the first 27 x 16 (16-lane groups, one offset each) while the last wave loads the
+// next triangle's inputs. Two blocks are resident per CU (up to 128 VGPRs), so one block's
+// latency-bound phases 2-4 overlap the other's FP64-bound phase 1.
+constexpr int GW = 512;
+static_assert(2 * GW >= GC * 9 && GW % WAVE == 0 && WAVE % STEPS == 0 && GC * STEPS <= GW - WAVE,
+              "phase 1 in two rounds; phases 2-4 in one, leaving the last wave free");
 
-// update_grid_cache_kernel (pppm_cache.cu:64-123) for the dirty triangles, one block per
-// triangle, in four phases through LDS:
+// Phases 2-4 of one stencil offset o in the registers of a 16-lane group: lane jj holds entry jj
+// of the bit-reversed input of the group's four series (o, value), i.e. wave number
+// kr = bitrev(jj), whose batch sums it forms itself (kr > 8: the conjugate of 16 - kr's); after
+// the butterflies lane jj holds tap jj of the four series, and tap q of the W^fdtd record takes
+// taps q - 1 and q - 2 from its neighbour lanes. Every value gets the operations and order of
+// the per-(o, k) loop.
+__device__ inline void gw_post(int o, int jj, const cdbl (*__restrict__ gl)[81][2], const GridTriIn &ti,
+                               const WeightArgs &a, const signed char *s_idx, const double *s_twr,
+                               const double *s_twi, const double *s_inv, uint32_t *__restrict__ wf,
+                               float2 *__restrict__ sf, uint32_t *__restrict__ wb, float2 *__restrict__ sb,
+                               float *__restrict__ meta)
+{
+    const int kr = bitrev4(jj), k = kr <= 8 ? kr : STEPS - kr;
+    const int ox = o % 3 - 1, oy = o / 3 % 3 - 1, oz = o / 9 - 1;
+    // phase 2: the 8-target FDTD-stencil batch in the reference's target order, and the
+    // cell-centre target
+    cdbl x[4];
+    {
+        const float s = a.c * a.c * a.dt * a.dt / (a.h * a.h);
+        const int dx[8] = {0, -1, 1, 0, 0, 0, 0, 0}, dy[8] = {0, 0, 0, -1, 1, 0, 0, 0},
+                  dz[8] = {0, 0, 0, 0, 0, -1, 1, 0};
+        const float dsc[8] = {2, s, s, s, s, s, s, -6 * s};
+        double sr0 = 0, si0 = 0, sr1 = 0, si1 = 0;
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+        {
+            const int ex = ox + dx[q], ey = oy + dy[q], ez = oz + dz[q];
+            if (ex == 0 && ey == 0 && ez == 0)
+                continue;  // triangle_in_grid_cache: no target in the triangle's own cell (pppm_cache.cu:7-19)
+            const int slot = s_idx[(ex + 2) * 25 + (ey + 2) * 5 + (ez + 2)];
+            const double sc = dsc[q];
+            sr0 += gl[k][slot][0].re * sc;
+            si0 += gl[k][slot][0].im * sc;
+            sr1 += gl[k][slot][1].re * sc;
+            si1 += gl[k][slot][1].im * sc;
+        }
+        const double f = 0.5 * TRI1_W * ti.jac;
+        x[0] = {f * sr0, f * si0};
+        x[1] = {f * sr1, f * si1};
+        if (!(ox == 0 && oy == 0 && oz == 0))
+        {
+            const int slot = s_idx[(ox + 2) * 25 + (oy + 2) * 5 + (oz + 2)];
+            double cr0 = 0, ci0 = 0, cr1 = 0, ci1 = 0;
+            cr0 += gl[k][slot][0].re * 1.0;
+            ci0 += gl[k][slot][0].im * 1.0;
+            cr1 += gl[k][slot][1].re * 1.0;
+            ci1 += gl[k][slot][1].im * 1.0;
+            x[2] = {f * cr0, f * ci0};
+            x[3] = {f * cr1, f * ci1};
+        }
+        else
+            x[2] = x[3] = {0, 0};
+    }
+    if (kr > 8)
+    {
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            x[v].im = -x[v].im;
+    }
+    // phase 3: scaled_idft of the four series; each radix-2 stage exchanges butterfly partners
+    // with a lane shuffle, and both partners form the twiddled term from the same operands, so
+    // every entry gets the operations of the sequential butterflies
+#pragma unroll
+    for (int L = 2; L <= STEPS; L <<= 1)
+    {
+        const int half = L >> 1, tws = STEPS / L;
+        const bool lower = (jj & half) == 0;
+        const int m = jj & (half - 1);
+        const double cr = s_twr[m * tws], ci = s_twi[m * tws];
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+        {
+            const cdbl other = {__shfl_xor(x[v].re, half, STEPS), __shfl_xor(x[v].im, half, STEPS)};
+            const cdbl y = lower ? other : x[v], a0 = lower ? x[v] : other;
+            const double tr_ = cr * y.re - ci * y.im;
+            const double ti_ = cr * y.im + ci * y.re;
+            x[v] = lower ? cdbl{a0.re + tr_, a0.im + ti_} : cdbl{a0.re - tr_, a0.im - ti_};
+        }
+    }
+    const bool self = (o == 13);
+    double X[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++)  // offset (0, 0, 0) has no centre target: its centre taps are exactly 0
+        X[v] = (v >= 2 && self) ? 0.0 : x[v].re / STEPS * s_inv[jj];
+    // phase 4: W^fdtd = taps shifted by one minus the centre taps shifted by two, W^bem = the
+    // centre taps (GridCache, pppm_cache.cu:99-116); the max-abs scales of LayerWeightHalf::set
+    // are reduced over the 16 lanes of a record (exact, order-free)
+    const int q = jj;
+    const double p0 = __shfl(X[0], q >= 1 ? q - 1 : 0, STEPS), p1 = __shfl(X[1], q >= 1 ? q - 1 : 0, STEPS);
+    const double p2 = __shfl(X[2], q >= 2 ? q - 2 : 0, STEPS), p3 = __shfl(X[3], q >= 2 ? q - 2 : 0, STEPS);
+    double xf = q >= 1 ? p0 : 0.0;
+    double yf = q >= 1 ? p1 : 0.0;
+    if (!self && q >= 2)
+    {
+        xf += p2 * -1.0;
+        yf += p3 * -1.0;
+    }
+    const float fs[2] = {(float)xf, (float)X[2]}, fd[2] = {(float)yf, (float)X[3]};
+    float ms[2], md[2];
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+    {
+        ms[r] = fabsf(fs[r]);
+        md[r] = fabsf(fd[r]);
+    }
+#pragma unroll
+    for (int off = STEPS / 2; off >= 1; off >>= 1)
+    {
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+        {
+            ms[r] = fmaxf(ms[r], __shfl_xor(ms[r], off, STEPS));
+            md[r] = fmaxf(md[r], __shfl_xor(md[r], off, STEPS));
+        }
+    }
+    const size_t rec = (size_t)ti.tri * GC + o;
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+    {
+        if (ms[r] == 0)
+            ms[r] = 1;
+        if (md[r] == 0)
+            md[r] = 1;
+        (r == 0 ? wf : wb)[rec * STEPS + q] = f32_to_f16_bits(fs[r] / ms[r]) | (f32_to_f16_bits(fd[r] / md[r]) << 16);
+        if (q == 0)
+            (r == 0 ? sf : sb)[rec] = make_float2(ms[r], md[r]);
+    }
+    if (o == 0 && q == 0)
+    {
+        float *m = meta + 7 * ti.tri;
+#pragma unroll
+        for (int i = 0; i < 7; i++)
+            m[i] = ti.meta[i];
+    }
+}
+
+// update_grid_cache_kernel (pppm_cache.cu:64-123) for the dirty triangles; block b takes the
+// dirty triangles b, b + gridDim.x, ... in four phases:
 //  1. both layer kernels at the 81 distinct target cells of the 27 stencil batches, for each
-//     wave number k <= 8 (one complex exponential per target, shared by the two layers);
+//     wave number k <= 8 (one complex exponential per target, shared by the two layers), into
+//     LDS;
 //  2. per (offset o, k): the 8-target FDTD-stencil batch, summed in the reference's target
 //     order, and the cell-centre target -> 4 complex values (single / double layer of each);
 //  3. per (o, value): the scaled inverse DFT to 16 real taps;
 //  4. per (o, record, tap): W^fdtd = taps shifted by one minus the centre taps shifted by two
 //     (GridCache, pppm_cache.cu:99-116), W^bem = the centre taps; FP16 + scale each.
-// Every value is computed with the operations and order of a per-(o, k) loop, so the records
-// do not depend on how the work is spread over threads.
-__global__ __launch_bounds__(GW) void k_grid_weights(const float *__restrict__ verts, const int *__restrict__ tris,
-                                                     const TriGeom *__restrict__ geom, const int *__restrict__ list,
+// The inputs come from the record the binning pass wrote for the triangle; during phases 2-4 of
+// triangle j the last wave copies triangle j + 1's record (its number was loaded one iteration
+// ago) and loads triangle j + 2's number. Every value is computed with the operations and order
+// of a per-(o, k) loop, so the records do not depend on how the work is spread over threads.
+__global__ __launch_bounds__(GW) void k_grid_weights(const GridTriIn *__restrict__ gin, const int *__restrict__ list,
                                                      const int *__restrict__ n_ptr, WeightArgs a, Tdbem bem,
                                                      uint32_t *__restrict__ wf, float2 *__restrict__ sf,
                                                      uint32_t *__restrict__ wb, float2 *__restrict__ sb,
                                                      float *__restrict__ meta, const int *__restrict__ err)
 {
-    // phase 1 values, then (same memory) the 108 transforms of phase 3
-    __shared__ cdbl pool[GC * 4 * STEPS];
-    cdbl(*gl)[81][2] = reinterpret_cast<cdbl(*)[81][2]>(pool);
-    cdbl(*xs)[STEPS] = reinterpret_cast<cdbl(*)[STEPS]>(pool);
-    __shared__ cdbl bv[GC][9][4];
-    static_assert(sizeof(cdbl) * 9 * 81 * 2 <= sizeof(pool), "phase-1 values fit the transform buffer");
+    __shared__ cdbl gl[9][81][2];
+    __shared__ GridTriIn tin[2];  // inputs of triangles j, j + 1 (slot = index mod 2)
+    __shared__ int s_tri[2];      // numbers of triangles j + 1, j + 2 (slot = index mod 2)
     // per-block copies of the target table, the wave numbers, twiddles and lambda powers
     __shared__ int s_off[81];
     __shared__ signed char s_idx[125];
     __shared__ double s_wr[9], s_wi[9], s_twr[STEPS / 2], s_twi[STEPS / 2], s_inv[STEPS];
-    const int n = live_count(n_ptr, err);
-    const int tid = threadIdx.x;
+    // the count is read once per block: the error flag can be raised by the face-cache kernels
+    // running beside this one, and every thread must take the same trip count through the
+    // barriers below
+    __shared__ int s_n;
+    constexpr int RW = (int)(sizeof(GridTriIn) / 8);  // words of an input record, one per lane
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1);
+    const bool loader = tid >= GW - WAVE;
+    const int G = (int)gridDim.x;
+    if (tid == 0)
+        s_n = live_count(n_ptr, err);
+    __syncthreads();
+    const int n = s_n;
     if ((int)blockIdx.x >= n)
-        return;  // the launch is sized from an estimate; blocks without a triangle do nothing
+        return;  // the launch covers every CU; blocks without a triangle do nothing
     if (tid < 81)
         s_off[tid] = (c_t81.off[tid][0] + 2) * 25 + (c_t81.off[tid][1] + 2) * 5 + (c_t81.off[tid][2] + 2);
     if (tid < 125)
@@ -352,140 +499,44 @@ __global__ __launch_bounds__(GW) void k_grid_weights(const float *__restrict__ v
     }
     if (tid < STEPS)
         s_inv[tid] = bem.inv_pow[tid];
-    __syncthreads();
-    for (int li = blockIdx.x; li < n; li += gridDim.x)
+    if (loader)
     {
-        const int tri = list[li];
-        const TriGeom g = geom[tri];
-        const int3_ c = g.cell3;
-        const TriD T = tri_d(verts, tris[3 * tri], tris[3 * tri + 1], tris[3 * tri + 2]);
-        const d3 p = l2g(TRI1_X, TRI1_X, T.v);
+        // triangle 0's inputs and triangle 1's number
+        int t0 = lane == 0 ? list[blockIdx.x] : 0;
+        if (lane == 1 && (int)blockIdx.x + G < n)
+            s_tri[1] = list[blockIdx.x + G];
+        t0 = __shfl(t0, 0);
+        if (lane < RW)
+            reinterpret_cast<unsigned long long *>(&tin[0])[lane] =
+                reinterpret_cast<const unsigned long long *>(gin + t0)[lane];
+    }
+    __syncthreads();
+    for (int j = 0;; j++)
+    {
+        const int li = (int)blockIdx.x + j * G;
+        if (li >= n)
+            break;
+        const GridTriIn &ti = tin[j & 1];
         for (int e = tid; e < 9 * 81; e += GW)
         {
-            const int k = e / 81, j = e % 81;
-            const int q = s_off[j];
-            const int ox = q / 25 - 2, oy = q / 5 % 5 - 2, oz = q % 5 - 2;
-            const d3 pt = cell_center_d(c.x + ox, c.y + oy, c.z + oz, a);
-            layer64_pair(p, pt, T.n, s_wr[k], s_wi[k], gl[k][j][0], gl[k][j][1]);
+            const int k = e / 81, jt = e % 81;
+            const int qo = s_off[jt];
+            const int ox = qo / 25 - 2, oy = qo / 5 % 5 - 2, oz = qo % 5 - 2;
+            const d3 pt = cell_center_d(ti.cx + ox, ti.cy + oy, ti.cz + oz, a);
+            layer64_pair(d3{ti.px, ti.py, ti.pz}, pt, d3{ti.nx, ti.ny, ti.nz}, s_wr[k], s_wi[k], gl[k][jt][0],
+                         gl[k][jt][1]);
         }
         __syncthreads();
-        if (tid < GC * 9)
+        if (loader)
         {
-            const int o = tid / 9, k = tid % 9;
-            const int ox = o % 3 - 1, oy = o / 3 % 3 - 1, oz = o / 9 - 1;
-            const float s = a.c * a.c * a.dt * a.dt / (a.h * a.h);
-            const int dx[8] = {0, -1, 1, 0, 0, 0, 0, 0}, dy[8] = {0, 0, 0, -1, 1, 0, 0, 0},
-                      dz[8] = {0, 0, 0, 0, 0, -1, 1, 0};
-            const float dsc[8] = {2, s, s, s, s, s, s, -6 * s};
-            double sr0 = 0, si0 = 0, sr1 = 0, si1 = 0;
-            for (int q = 0; q < 8; q++)
-            {
-                const int ex = ox + dx[q], ey = oy + dy[q], ez = oz + dz[q];
-                if (ex == 0 && ey == 0 && ez == 0)
-                    continue;  // triangle_in_grid_cache: no target in the triangle's own cell (pppm_cache.cu:7-19)
-                const int slot = s_idx[(ex + 2) * 25 + (ey + 2) * 5 + (ez + 2)];
-                const double sc = dsc[q];
-                sr0 += gl[k][slot][0].re * sc;
-                si0 += gl[k][slot][0].im * sc;
-                sr1 += gl[k][slot][1].re * sc;
-                si1 += gl[k][slot][1].im * sc;
-            }
-            const double f = 0.5 * TRI1_W * T.jac;
-            bv[o][k][0] = {f * sr0, f * si0};
-            bv[o][k][1] = {f * sr1, f * si1};
-            if (!(ox == 0 && oy == 0 && oz == 0))
-            {
-                const int slot = s_idx[(ox + 2) * 25 + (oy + 2) * 5 + (oz + 2)];
-                double cr0 = 0, ci0 = 0, cr1 = 0, ci1 = 0;
-                cr0 += gl[k][slot][0].re * 1.0;
-                ci0 += gl[k][slot][0].im * 1.0;
-                cr1 += gl[k][slot][1].re * 1.0;
-                ci1 += gl[k][slot][1].im * 1.0;
-                bv[o][k][2] = {f * cr0, f * ci0};
-                bv[o][k][3] = {f * cr1, f * ci1};
-            }
-            else
-                bv[o][k][2] = bv[o][k][3] = {0, 0};
+            if (li + G < n && lane < RW)
+                reinterpret_cast<unsigned long long *>(&tin[(j + 1) & 1])[lane] =
+                    reinterpret_cast<const unsigned long long *>(gin + s_tri[(j + 1) & 1])[lane];
+            if (li + 2 * G < n && lane == RW)
+                s_tri[j & 1] = list[li + 2 * G];
         }
-        __syncthreads();
-        // phase 3: scaled_idft of the 108 series (o, value), 16 lanes per series, one entry of
-        // the bit-reversed input per lane; each radix-2 stage exchanges butterfly partners with
-        // a lane shuffle, and both partners form the twiddled term from the same operands, so
-        // every entry gets the operations of the sequential butterflies
-        for (int e = tid; e < GC * 4 * STEPS; e += GW)
-        {
-            const int tr = e / STEPS, jj = e % STEPS, o = tr / 4, v = tr % 4;
-            const int k = bitrev4(jj);
-            cdbl x = k <= 8 ? bv[o][k][v] : cdbl{bv[o][16 - k][v].re, -bv[o][16 - k][v].im};
-#pragma unroll
-            for (int L = 2; L <= STEPS; L <<= 1)
-            {
-                const int half = L >> 1, tws = STEPS / L;
-                const bool lower = (jj & half) == 0;
-                const cdbl other = {__shfl_xor(x.re, half, STEPS), __shfl_xor(x.im, half, STEPS)};
-                const cdbl y = lower ? other : x, a0 = lower ? x : other;
-                const int m = jj & (half - 1);
-                const double cr = s_twr[m * tws], ci = s_twi[m * tws];
-                const double tr_ = cr * y.re - ci * y.im;
-                const double ti_ = cr * y.im + ci * y.re;
-                x = lower ? cdbl{a0.re + tr_, a0.im + ti_} : cdbl{a0.re - tr_, a0.im - ti_};
-            }
-            // offset (0, 0, 0) has no centre target: its centre taps are exactly 0
-            xs[tr][jj].re = (v >= 2 && o == 13) ? 0.0 : x.re / STEPS * s_inv[jj];
-        }
-        __syncthreads();
-        // phase 4: one lane per (o, record, tap); the max-abs scales of LayerWeightHalf::set
-        // are reduced over the 16 lanes of a record (exact, order-free)
-        for (int e = tid; e < GC * 2 * STEPS; e += GW)
-        {
-            const int rr = e / STEPS, q = e % STEPS, o = rr / 2, r = rr % 2;
-            const bool self = (o == 13);
-            float fsq, fdq;
-            if (r == 0)
-            {
-                double x = q >= 1 ? xs[4 * o][q - 1].re : 0.0;
-                double y = q >= 1 ? xs[4 * o + 1][q - 1].re : 0.0;
-                if (!self && q >= 2)
-                {
-                    x += xs[4 * o + 2][q - 2].re * -1.0;
-                    y += xs[4 * o + 3][q - 2].re * -1.0;
-                }
-                fsq = (float)x;
-                fdq = (float)y;
-            }
-            else
-            {
-                fsq = (float)xs[4 * o + 2][q].re;
-                fdq = (float)xs[4 * o + 3][q].re;
-            }
-            float ms = fabsf(fsq), md = fabsf(fdq);
-#pragma unroll
-            for (int off = STEPS / 2; off >= 1; off >>= 1)
-            {
-                ms = fmaxf(ms, __shfl_xor(ms, off, STEPS));
-                md = fmaxf(md, __shfl_xor(md, off, STEPS));
-            }
-            if (ms == 0)
-                ms = 1;
-            if (md == 0)
-                md = 1;
-            const size_t rec = (size_t)tri * GC + o;
-            uint32_t *w16 = (r == 0 ? wf : wb) + rec * STEPS;
-            w16[q] = f32_to_f16_bits(fsq / ms) | (f32_to_f16_bits(fdq / md) << 16);
-            if (q == 0)
-                (r == 0 ? sf : sb)[rec] = make_float2(ms, md);
-            if (e == 0)
-            {
-                float *m = meta + 7 * tri;
-                m[0] = g.center.x;
-                m[1] = g.center.y;
-                m[2] = g.center.z;
-                m[3] = g.normal.x;
-                m[4] = g.normal.y;
-                m[5] = g.normal.z;
-                m[6] = g.area;
-            }
-        }
+        else if (tid < GC * STEPS)
+            gw_post(tid / STEPS, tid % STEPS, gl, ti, a, s_idx, s_twr, s_twi, s_inv, wf, sf, wb, sb, meta);
         __syncthreads();
     }
 }
@@ -882,16 +933,21 @@ void tdbem_init(Tdbem &b, float dt, float c, float grid_size)
 
 size_t face_tmp_bytes(int max_dirty) { return (size_t)max_dirty * 18 * sizeof(cdbl); }
 
-hipError_t launch_grid_weights(const float *verts, const int *tris, const TriGeom *geom, const int *list,
-                               const int *n_ptr, int cap, const WeightArgs &a, const Tdbem &b, uint32_t *wf,
-                               float2 *sf, uint32_t *wb, float2 *sb, float *meta, const int *err,
-                               long long hint_dirty, hipStream_t s)
+hipError_t launch_grid_weights(const GridTriIn *gin, const int *list, const int *n_ptr, int cap, const WeightArgs &a,
+                               const Tdbem &b, uint32_t *wf, float2 *sf, uint32_t *wb, float2 *sb, float *meta,
+                               const int *err, hipStream_t s)
 {
-    const long long hint = hint_dirty < 0 ? cap : std::min<long long>(hint_dirty, cap);
-    // blocks stride over the dirty triangles; 1024 blocks keep every CU busy (one 768-thread
-    // block is resident per CU at a time)
-    const long long blocks = std::max<long long>(64, std::min<long long>(hint, 1024));
-    k_grid_weights<<<(unsigned)blocks, GW, 0, s>>>(verts, tris, geom, list, n_ptr, a, b, wf, sf, wb, sb, meta, err);
+    // blocks stride over the dirty triangles (a device-side count), two per CU (two 512-thread
+    // blocks are resident per CU), so a jump in the count (a whole moving object past the
+    // recompute thresholds at once) still fills the device; blocks past the count exit at once
+    static const int cus = [] {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            n = 0;
+        return n > 0 ? n : 256;
+    }();
+    const long long blocks = std::max<long long>(1, std::min<long long>(cap, 2LL * cus));
+    k_grid_weights<<<(unsigned)blocks, GW, 0, s>>>(gin, list, n_ptr, a, b, wf, sf, wb, sb, meta, err);
     return hipGetLastError();
 }
 hipError_t launch_face_lookup(const TriGeom *geom, const int *tris, const int *owner, const int *face_nbr,
