@@ -1,6 +1,6 @@
 // Deterministic FP64 exp, sin and cos for the weight quadrature.
 //
-// Built from + - * / and exact power-of-two scaling only, so every IEEE-conforming build
+// Built from + - * /, truncation and exact power-of-two scaling only, so every IEEE-conforming build
 // (device or host, with contraction off) returns the same bits. The near-zero double-layer
 // weights of coplanar triangle pairs are rounding residues whose FP16 codes depend on the
 // last bit of every exp/sin/cos term; with these functions the device weights equal the CPU
@@ -43,7 +43,9 @@ __host__ __device__ inline double pow2i(int k)  // 2^k for -1022 <= k <= 1023, e
     __builtin_memcpy(&d, &bits, sizeof d);
     return d;
 }
-__host__ __device__ inline double round_half_away(double x) { return x >= 0 ? (double)(long long)(x + 0.5) : -(double)(long long)(-x + 0.5); }
+// nearest integer, halves away from zero: trunc(|x| + 0.5) with x's sign, the value of a round
+// trip through a 64-bit integer (one instruction per truncation instead of an integer conversion)
+__host__ __device__ inline double round_half_away(double x) { return x >= 0 ? __builtin_trunc(x + 0.5) : -__builtin_trunc(-x + 0.5); }
 }  // namespace detm
 
 __host__ __device__ inline double det_exp(double x)
@@ -85,7 +87,7 @@ __host__ __device__ inline void det_sincos(double x, double *s, double *c)
 {
     using namespace detm;
     const double qd = round_half_away(x * TWO_OVER_PI);
-    const long long q = (long long)qd;
+    const int q = (int)qd;  // |qd| < 2^20 in the valid range
     const double r = ((x - qd * PIO2_1) - qd * PIO2_2) - qd * PIO2_3;
     const double z = r * r;
     // sin r = r - r z (1/3! - z (1/5! - z (1/7! - ... - z/17!)))
