@@ -243,28 +243,46 @@ __device__ inline int live_count(const int *n_ptr, const int *err) { return *err
 
 // ---------------------------------------------------------------- grid cache
 // Both layer kernels of one (source point, target) pair: they share r and e^{ikr}, so the
-// values are those of two layer64 calls.
-__device__ inline void layer64_pair(d3 src, d3 trg, d3 n, double kr, double ki, cdbl &g0, cdbl &g1)
+// values are those of two layer64 calls. Split in two so that the target's part is computed once
+// for the 9 wave numbers.
+// orders one wave's LDS writes before its lanes' reads of them
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The wave-number-free part, per (source point, target): r = |trg - src| (0: the target is the
+// source point, both kernels 0), the denominators 4 pi r and 4 pi r^2 r, and (src - trg) . n.
+struct LayerTarget
+{
+    double r, q0, q1, dn;
+};
+__device__ inline LayerTarget layer_target(d3 src, d3 trg, d3 n)
 {
     const d3 s0 = trg - src;
     const double r2 = dotd(s0, s0);
     if (r2 == 0)
+        return {0, 0, 0, 0};
+    const double r = sqrt(r2);
+    const d3 s1 = src - trg;  // = -s0 exactly, so dotd(s1, s1) == r2
+    return {r, 4 * PI_R * r, 4 * PI_R * r2 * r, dotd(s1, n)};
+}
+__device__ inline void layer64_pair(const LayerTarget &t, double kr, double ki, cdbl &g0, cdbl &g1)
+{
+    if (t.r == 0)
     {
         g0 = g1 = {0, 0};
         return;
     }
-    const double r = sqrt(r2);
-    const double zr = -r * ki, zi = r * kr;
+    const double zr = -t.r * ki, zi = t.r * kr;
     const cdbl e = cexp_d(zr, zi);
-    const double q0 = 4 * PI_R * r;
-    g0 = {e.re / q0, e.im / q0};
-    const d3 s1 = src - trg;  // = -s0 exactly, so dotd(s1, s1) == r2
-    const double q1 = 4 * PI_R * r2 * r;
-    const double ar = -e.re / q1, ai = -e.im / q1;
+    g0 = {e.re / t.q0, e.im / t.q0};
+    const double ar = -e.re / t.q1, ai = -e.im / t.q1;
     const double br = 1 - zr, bi = -zi;
     const double pr = ar * br - ai * bi, pi = ar * bi + ai * br;
-    const double dn = dotd(s1, n);
-    g1 = {pr * dn, pi * dn};
+    g1 = {pr * t.dn, pi * t.dn};
 }
 
 // The target cells of one triangle's 27 stencil batches, as offsets from its cell: the
@@ -454,7 +472,8 @@ __device__ inline void gw_post(int o, int jj, const cdbl (*__restrict__ gl)[81][
 //     (GridCache, pppm_cache.cu:99-116), W^bem = the centre taps; FP16 + scale each.
 // The inputs come from the record the binning pass wrote for the triangle; during phases 2-4 of
 // triangle j the last wave copies triangle j + 1's record (its number was loaded one iteration
-// ago) and loads triangle j + 2's number. Every value is computed with the operations and order
+// ago), computes the wave-number-free part of its 81 targets (distance, denominators, normal
+// product) for phase 1, and loads triangle j + 2's number. Every value is computed with the operations and order
 // of a per-(o, k) loop, so the records do not depend on how the work is spread over threads.
 __global__ __launch_bounds__(GW) void k_grid_weights(const GridTriIn *__restrict__ gin, const int *__restrict__ list,
                                                      const int *__restrict__ n_ptr, WeightArgs a, Tdbem bem,
@@ -463,10 +482,10 @@ __global__ __launch_bounds__(GW) void k_grid_weights(const GridTriIn *__restrict
                                                      float *__restrict__ meta, const int *__restrict__ err)
 {
     __shared__ cdbl gl[9][81][2];
-    __shared__ GridTriIn tin[2];  // inputs of triangles j, j + 1 (slot = index mod 2)
-    __shared__ int s_tri[2];      // numbers of triangles j + 1, j + 2 (slot = index mod 2)
+    __shared__ GridTriIn tin[2];       // inputs of triangles j, j + 1 (slot = index mod 2)
+    __shared__ LayerTarget tgt[2][81];  // target parts of triangles j, j + 1
+    __shared__ int s_tri[2];           // numbers of triangles j + 1, j + 2 (slot = index mod 2)
     // per-block copies of the target table, the wave numbers, twiddles and lambda powers
-    __shared__ int s_off[81];
     __shared__ signed char s_idx[125];
     __shared__ double s_wr[9], s_wi[9], s_twr[STEPS / 2], s_twi[STEPS / 2], s_inv[STEPS];
     // the count is read once per block: the error flag can be raised by the face-cache kernels
@@ -483,8 +502,6 @@ __global__ __launch_bounds__(GW) void k_grid_weights(const GridTriIn *__restrict
     const int n = s_n;
     if ((int)blockIdx.x >= n)
         return;  // the launch covers every CU; blocks without a triangle do nothing
-    if (tid < 81)
-        s_off[tid] = (c_t81.off[tid][0] + 2) * 25 + (c_t81.off[tid][1] + 2) * 5 + (c_t81.off[tid][2] + 2);
     if (tid < 125)
         s_idx[tid] = c_t81.idx[tid];
     if (tid < 9)
@@ -499,9 +516,20 @@ __global__ __launch_bounds__(GW) void k_grid_weights(const GridTriIn *__restrict
     }
     if (tid < STEPS)
         s_inv[tid] = bem.inv_pow[tid];
+    // the loader wave's target parts of the triangle in input slot b (written by this wave)
+    auto targets = [&](int b) {
+        wave_sync();
+        const GridTriIn &ti = tin[b];
+        const d3 p{ti.px, ti.py, ti.pz}, nn{ti.nx, ti.ny, ti.nz};
+        for (int t = lane; t < 81; t += WAVE)
+        {
+            const d3 pt = cell_center_d(ti.cx + c_t81.off[t][0], ti.cy + c_t81.off[t][1], ti.cz + c_t81.off[t][2], a);
+            tgt[b][t] = layer_target(p, pt, nn);
+        }
+    };
     if (loader)
     {
-        // triangle 0's inputs and triangle 1's number
+        // triangle 0's inputs and target parts, and triangle 1's number
         int t0 = lane == 0 ? list[blockIdx.x] : 0;
         if (lane == 1 && (int)blockIdx.x + G < n)
             s_tri[1] = list[blockIdx.x + G];
@@ -509,23 +537,21 @@ __global__ __launch_bounds__(GW) void k_grid_weights(const GridTriIn *__restrict
         if (lane < RW)
             reinterpret_cast<unsigned long long *>(&tin[0])[lane] =
                 reinterpret_cast<const unsigned long long *>(gin + t0)[lane];
+        targets(0);
     }
     __syncthreads();
+    // this thread's phase-1 items (at most two): wave number and target
+    const int k0 = tid / 81, jt0 = tid % 81, k1 = (tid + GW) / 81, jt1 = (tid + GW) % 81;
+    const bool second = tid + GW < 9 * 81;
     for (int j = 0;; j++)
     {
         const int li = (int)blockIdx.x + j * G;
         if (li >= n)
             break;
         const GridTriIn &ti = tin[j & 1];
-        for (int e = tid; e < 9 * 81; e += GW)
-        {
-            const int k = e / 81, jt = e % 81;
-            const int qo = s_off[jt];
-            const int ox = qo / 25 - 2, oy = qo / 5 % 5 - 2, oz = qo % 5 - 2;
-            const d3 pt = cell_center_d(ti.cx + ox, ti.cy + oy, ti.cz + oz, a);
-            layer64_pair(d3{ti.px, ti.py, ti.pz}, pt, d3{ti.nx, ti.ny, ti.nz}, s_wr[k], s_wi[k], gl[k][jt][0],
-                         gl[k][jt][1]);
-        }
+        layer64_pair(tgt[j & 1][jt0], s_wr[k0], s_wi[k0], gl[k0][jt0][0], gl[k0][jt0][1]);
+        if (second)
+            layer64_pair(tgt[j & 1][jt1], s_wr[k1], s_wi[k1], gl[k1][jt1][0], gl[k1][jt1][1]);
         __syncthreads();
         if (loader)
         {
@@ -534,6 +560,8 @@ __global__ __launch_bounds__(GW) void k_grid_weights(const GridTriIn *__restrict
                     reinterpret_cast<const unsigned long long *>(gin + s_tri[(j + 1) & 1])[lane];
             if (li + 2 * G < n && lane == RW)
                 s_tri[j & 1] = list[li + 2 * G];
+            if (li + G < n)
+                targets((j + 1) & 1);
         }
         else if (tid < GC * STEPS)
             gw_post(tid / STEPS, tid % STEPS, gl, ti, a, s_idx, s_twr, s_twi, s_inv, wf, sf, wb, sb, meta);
