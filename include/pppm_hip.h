@@ -76,7 +76,9 @@ int pppm_update_mesh_and_step(pppm_solver *s, const float *verts, const float *n
  * steps x stride floats (row i = Neumann input of step i, nt used). After each step
  * the pressure at the n_listen cells (xyz triples, host array) is written to
  * out_dev[i*n_listen + l] (device memory). use_graph != 0 replays the step as a
- * captured HIP graph. Replaces the demo loop's per-step one-float D2H read
+ * captured HIP graph (a run of up to 64 steps as one graph, longer runs 32 steps per graph),
+ * which the handle keeps for later runs whose arguments differ only by the rows of the series
+ * and of the output (the segments of one series). Replaces the demo loop's per-step one-float D2H read
  * (experiments/demo/main.cu:266, src/core/array.h:276-288). */
 int pppm_run(pppm_solver *s, const float *neumann_series, long long stride, int steps, const int *listen_xyz,
              int n_listen, float *out_dev, int use_graph);
@@ -153,7 +155,10 @@ int pppm_batch_get_timing(pppm_batch *b, double *ms8, long long *count8);
  *   of forward step T-steps+1+r (the cells listen_xyz, host, n_listen <= 64); grad_dev (device,
  *   steps x grad_stride, grad_stride >= nt): row r = the gradient with respect to the Neumann
  *   input of step T-steps+1+r. Consecutive calls continue backwards, so a run can be reversed
- *   segment by segment. use_graph != 0 replays 32 reverse steps per captured graph. Synchronous.
+ *   segment by segment. use_graph != 0: a run of up to 64 steps is one captured graph, a longer
+ *   one 32 reverse steps per graph; the handle keeps the graph for later runs whose arguments
+ *   differ only by the rows of gout and grad (segments of one reversal). Asynchronous on the
+ *   handle's stream, as pppm_run.
  * pppm_state_bytes / pppm_save_state / pppm_load_state: a checkpoint of the forward state
  *   (grid levels t and t-1 with the ghost layer, both history rings, t) in caller-owned device
  *   memory; loading one restores the solver to that step (same grid and mesh only). */

@@ -23,15 +23,19 @@ import torch
 
 from pppm_amd import PPPMError
 
-# Checkpoint segments shorter than this run with eager launches by default: a captured graph of a
-# segment is replayed once, so its capture and instantiation cost more than the launches it saves
-# (at 48^3 / 2000 steps, segments of 45 steps: loss and gradient 3.8x the forward eagerly, 4.6x
-# with graphs).
-GRAPH_MIN_SEGMENT = 256
+# Checkpoint segments of a single scene shorter than this run with eager launches by default. The
+# solver keeps a run's captured graph for later runs whose arguments differ only by the rows of
+# one series (pppm_run, pppm_adjoint_run), and the segments below write into full-length
+# sample and cotangent buffers so that they do: a segment length is captured once per
+# direction, and from a few steps on replaying beats launching.
+GRAPH_MIN_SEGMENT = 8
+# The same for a batch, whose runs capture their graph per call: a captured segment is replayed
+# once, so its capture and instantiation cost more than the launches it saves.
+BATCH_GRAPH_MIN_SEGMENT = 256
 
 
-def _segment_graphs(use_graph, S):
-    return S >= GRAPH_MIN_SEGMENT if use_graph is None else bool(use_graph)
+def _segment_graphs(use_graph, S, minimum=GRAPH_MIN_SEGMENT):
+    return S >= minimum if use_graph is None else bool(use_graph)
 
 
 def _check_inputs(g, solver):
@@ -78,14 +82,15 @@ class ListenerLoss(torch.autograd.Function):
         nseg = (T + S - 1) // S
         use_graph = _segment_graphs(use_graph, S)
         ck = torch.empty(nseg, solver.state_bytes(), dtype=torch.uint8, device=g.device)
-        out = torch.zeros(S, L, device=g.device, dtype=torch.float32)
+        # full length: every segment's rows sit where its steps are, so segments share a graph
+        out = torch.zeros(T, L, device=g.device, dtype=torch.float32)
         J = torch.zeros((), device=g.device, dtype=torch.float64)
         solver.reset()
         for k in range(nseg):
             a, b = k * S, min(T, (k + 1) * S)
             solver.save_state(ck[k])
-            solver.run(g[a:b], b - a, listeners, out[:b - a], use_graph=use_graph)
-            J += ((out[:b - a].double() - target[a:b].double()) ** 2).sum()
+            solver.run(g[a:b], b - a, listeners, out[a:b], use_graph=use_graph)
+            J += ((out[a:b].double() - target[a:b].double()) ** 2).sum()
         ctx.save_for_backward(g, target)
         ctx.solver, ctx.listeners, ctx.S, ctx.nseg, ctx.ck, ctx.use_graph = solver, listeners, S, nseg, ck, use_graph
         ctx.checkpoint_bytes = ck.numel()
@@ -97,14 +102,15 @@ class ListenerLoss(torch.autograd.Function):
         s, S, L = ctx.solver, ctx.S, len(ctx.listeners)
         T = g.shape[0]
         grad = torch.zeros(T, s.nt, device=g.device, dtype=torch.float32)
-        out = torch.zeros(S, L, device=g.device, dtype=torch.float32)
+        out = torch.zeros(T, L, device=g.device, dtype=torch.float32)
+        gout = torch.zeros(T, L, device=g.device, dtype=torch.float32)
         s.adjoint_reset(T - 1)
         for k in reversed(range(ctx.nseg)):
             a, b = k * S, min(T, (k + 1) * S)
             s.load_state(ctx.ck[k])
-            s.run(g[a:b], b - a, ctx.listeners, out[:b - a], use_graph=ctx.use_graph)   # recompute
-            gout = (2.0 * gJ.float()) * (out[:b - a] - target[a:b].float())
-            s.adjoint_run(gout.contiguous(), b - a, ctx.listeners, grad[a:b], use_graph=ctx.use_graph)
+            s.run(g[a:b], b - a, ctx.listeners, out[a:b], use_graph=ctx.use_graph)   # recompute
+            torch.mul(out[a:b] - target[a:b].float(), 2.0 * gJ.float(), out=gout[a:b])
+            s.adjoint_run(gout[a:b], b - a, ctx.listeners, grad[a:b], use_graph=ctx.use_graph)
         return grad, None, None, None, None, None
 
 
@@ -142,7 +148,7 @@ class BatchListenerLoss(torch.autograd.Function):
             raise PPPMError("targets must be (steps, listeners) per scene")
         S = int(segment) if segment else max(1, math.ceil(math.sqrt(T)))
         nseg = (T + S - 1) // S
-        use_graph = _segment_graphs(use_graph, S)
+        use_graph = _segment_graphs(use_graph, S, BATCH_GRAPH_MIN_SEGMENT)
         dev = gs[0].device
         ck = [torch.empty(nseg, s.state_bytes(), dtype=torch.uint8, device=dev) for s in solvers]
         out = torch.zeros(S, n, L, device=dev, dtype=torch.float32)

@@ -9,6 +9,7 @@
 #include "pppm_hip.h"
 
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -267,6 +268,31 @@ AdjScene solver_adj_scene(pppm_solver *s, const float *gout, long long gout_stri
 }
 }  // namespace pppm_amd
 
+namespace
+{
+// graph-cache key of an adjoint run: the reverse step's arguments with the per-call rows at t = 0
+struct AdjGraphKey
+{
+    AdjScene d;
+    long long gbase, rbase;  // byte addresses of the cotangent and gradient rows of t = 0
+    int chunk, pad;
+};
+struct AdjChunk
+{
+    pppm_solver *s;
+    const AdjScene *d;
+    int chunk;
+};
+int enqueue_adjoint_chunk(void *p)
+{
+    const AdjChunk &c = *static_cast<const AdjChunk *>(p);
+    for (int i = 0; i < c.chunk; i++)
+        if (enqueue_adjoint_step(*c.d, c.s->stream) != hipSuccess)
+            return solver_fail(c.s, PPPM_ERR_HIP, "adjoint graph capture failed");
+    return PPPM_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int pppm_adjoint_reset(pppm_solver *s, int t_last)
@@ -292,6 +318,7 @@ int pppm_adjoint_reset(pppm_solver *s, int t_last)
     return PPPM_OK;
 }
 
+
 int pppm_adjoint_run(pppm_solver *s, const float *gout_dev, int steps, const int *listen_xyz, int n_listen,
                      float *grad_dev, long long grad_stride, int use_graph)
 {
@@ -314,37 +341,29 @@ int pppm_adjoint_run(pppm_solver *s, const float *gout_dev, int steps, const int
     int done = 0;
     if (use_graph && steps >= 2)
     {
-        const int chunk = std::min(steps, 32);
-        hipGraph_t g = nullptr;
+        // a short run (a checkpoint segment) in one graph, a long one in 32-step graphs; the
+        // graph is kept for later runs whose arguments agree once the cotangent and gradient
+        // rows (read and written only at p + (t - t_first) * row) are shifted to t = 0
+        const int chunk = steps <= 64 ? steps : 32;
+        AdjGraphKey key;
+        std::memset(&key, 0, sizeof key);
+        AdjScene dk = d;
+        canonical_bytes(dk.fp);
+        key.gbase = dk.gout ? (long long)(uintptr_t)dk.gout - (long long)dk.t_first * dk.gout_stride * 4 : 0;
+        key.rbase = (long long)(uintptr_t)dk.grad - (long long)dk.t_first * dk.grad_stride * 4;
+        dk.gout = dk.gout ? s->adj_lam3.p : nullptr;  // presence only
+        dk.grad = s->adj_lam3.p;
+        dk.t_first = 0;
+        std::memcpy(&key.d, &dk, sizeof dk);
+        key.chunk = chunk;
+        AdjChunk ac{s, &d, chunk};
         hipGraphExec_t ge = nullptr;
-        ACK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-        hipError_t e = hipSuccess;
-        for (int i = 0; i < chunk && e == hipSuccess; i++)
-            e = enqueue_adjoint_step(d, s->stream);
-        const hipError_t ee = hipStreamEndCapture(s->stream, &g);
-        if (e != hipSuccess || ee != hipSuccess)
-        {
-            if (g)
-                (void)hipGraphDestroy(g);
-            return solver_fail(s, PPPM_ERR_HIP, "adjoint graph capture failed");
-        }
-        if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess)
-        {
-            (void)hipGraphDestroy(g);
-            return solver_fail(s, PPPM_ERR_HIP, "adjoint graph instantiate failed");
-        }
-        hipError_t le = hipSuccess;
-        for (; done + chunk <= steps && le == hipSuccess; done += chunk)
-            le = hipGraphLaunch(ge, s->stream);
-        const hipError_t se = hipStreamSynchronize(s->stream);
-        (void)hipGraphExecDestroy(ge);
-        (void)hipGraphDestroy(g);
-        ACK(le);
-        ACK(se);
+        ARC(solver_cached_graph(s, s->adj_graphs, &key, sizeof key, enqueue_adjoint_chunk, &ac, &ge));
+        for (; done + chunk <= steps; done += chunk)
+            ACK(hipGraphLaunch(ge, s->stream));
     }
     for (; done < steps; done++)
         ACK(enqueue_adjoint_step(d, s->stream));
-    ACK(hipStreamSynchronize(s->stream));
     s->adj_t -= steps;
     return PPPM_OK;
 }
@@ -372,9 +391,7 @@ int pppm_save_state(pppm_solver *s, void *dst_dev)
     ACK(hipMemcpyAsync(p + NN * 4, level_ptr(s, t, -1), NN * 4, hipMemcpyDeviceToDevice, s->stream));
     ACK(hipMemcpyAsync(p + 2 * NN * 4, s->hn.p, R * 4, hipMemcpyDeviceToDevice, s->stream));
     ACK(hipMemcpyAsync(p + 2 * NN * 4 + R * 4, s->hd.p, R * 4, hipMemcpyDeviceToDevice, s->stream));
-    int trailer[4] = {t, s->N, s->nt, 0x5157};
-    ACK(hipMemcpyAsync(p + 2 * NN * 4 + 2 * R * 4, trailer, sizeof(trailer), hipMemcpyHostToDevice, s->stream));
-    ACK(hipStreamSynchronize(s->stream));
+    ACK(launch_put_ints(reinterpret_cast<int *>(p + 2 * NN * 4 + 2 * R * 4), 4, t, s->N, s->nt, 0x5157, s->stream));
     return PPPM_OK;
 }
 
@@ -396,7 +413,6 @@ int pppm_load_state(pppm_solver *s, const void *src_dev)
     ACK(hipMemcpyAsync(level_ptr(s, t, -1), p + NN * 4, NN * 4, hipMemcpyDeviceToDevice, s->stream));
     ACK(hipMemcpyAsync(s->hn.p, p + 2 * NN * 4, R * 4, hipMemcpyDeviceToDevice, s->stream));
     ACK(hipMemcpyAsync(s->hd.p, p + 2 * NN * 4 + R * 4, R * 4, hipMemcpyDeviceToDevice, s->stream));
-    ACK(hipStreamSynchronize(s->stream));
     return pppm_set_time_idx(s, t);
 }
 

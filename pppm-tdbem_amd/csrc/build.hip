@@ -390,6 +390,19 @@ inline int nblk_waves(long long waves) { return waves > 0 ? (int)((waves + WPB -
 inline int nblk_strided(long long waves) { return std::min(nblk_waves(waves), 1024); }
 }  // namespace
 
+__global__ void k_put_ints(int *__restrict__ a, int n, int v0, int v1, int v2, int v3)
+{
+    const int i = threadIdx.x;
+    if (i < n)
+        a[i] = i == 0 ? v0 : i == 1 ? v1 : i == 2 ? v2 : v3;
+}
+hipError_t launch_put_ints(int *a, int n, int v0, int v1, int v2, int v3, hipStream_t s)
+{
+    if (n < 1 || n > 4)
+        return hipErrorInvalidValue;
+    k_put_ints<<<1, WAVE, 0, s>>>(a, n, v0, v1, v2, v3);
+    return hipGetLastError();
+}
 hipError_t launch_fill_int(int *a, long long n, int value, hipStream_t s)
 {
     if (n <= 0)

@@ -119,6 +119,9 @@ struct BuildArgs
     float minx, miny, minz;
 };
 hipError_t launch_fill_int(int *a, long long n, int value, hipStream_t s);
+// a[0..n) = v[0..n) for n <= 4, from the launch arguments: a stream-ordered small write that
+// needs no host buffer to outlive the call (so no wait for the stream)
+hipError_t launch_put_ints(int *a, int n, int v0, int v1, int v2, int v3, hipStream_t s);
 hipError_t launch_copy_verts(const float *series, long long stride, int t0, const StepState *st, float *verts, int n,
                              hipStream_t s);
 // per-tile counts of the dirty face pairs, accumulated by the face lookup for the face-pair scan

@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <utility>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -774,6 +775,12 @@ int set_listeners(pppm_solver *s, const int *listen_xyz, int n_listen)
             return fail(s, PPPM_ERR_ARG, "listener outside the grid");
         cells[l] = lin3(x, y, z, N);
     }
+    // the same cells on the same mesh version: the device tables already hold them (the C3
+    // lookups below wait for the stream, which a run of checkpoint segments would do per call)
+    const long long version = s->mesh_set ? s->mesh_version : -2;
+    if (n_listen > 0 && version == s->lis_version && cells == s->lis_cells)
+        return PPPM_OK;
+    s->lis_version = -1;
     CK(s->lcell.ensure(std::max(n_listen, 1)));
     CK(s->lc3.ensure(std::max(n_listen, 1)));
     if (n_listen > 0)
@@ -787,7 +794,60 @@ int set_listeners(pppm_solver *s, const int *listen_xyz, int n_listen)
         CK(hipMemcpyAsync(s->lcell.p, cells.data(), n_listen * sizeof(int), hipMemcpyHostToDevice, s->stream));
         CK(hipMemcpyAsync(s->lc3.p, c3.data(), n_listen * sizeof(int), hipMemcpyHostToDevice, s->stream));
         CK(hipStreamSynchronize(s->stream));
+        s->lis_cells = cells;
+        s->lis_version = version;
     }
+    return PPPM_OK;
+}
+
+// Graph-cache key of a static-mesh run: the step's launch arguments with the per-call rows (the
+// Neumann series and the listener output, which the kernels read and write only at
+// p + (t - t0) * row) shifted to t = 0, so the segments of one series, run one after another
+// (a checkpointed loss), replay one graph.
+struct StepGraphKey
+{
+    SceneStep d;
+    FdtdParams fp;
+    long long nbase, obase;  // byte addresses of the rows of t = 0 (0: no series / no output)
+    int chunk, pml;
+};
+
+StepGraphKey step_graph_key(pppm_solver *s, const float *neumann, int t0, long long stride, int n_listen, float *out,
+                            int chunk)
+{
+    StepGraphKey k;
+    std::memset(&k, 0, sizeof k);
+    SceneStep d = scene_step(s, own_bufs(s), neumann, t0, stride, n_listen, out);
+    canonical_bytes(d.a);
+    k.nbase = d.neumann ? (long long)(uintptr_t)d.neumann - (long long)t0 * stride * (long long)sizeof(float) : 0;
+    k.obase = d.out ? (long long)(uintptr_t)d.out - (long long)t0 * d.out_stride * (long long)sizeof(float) : 0;
+    d.neumann = d.neumann ? s->grid.p : nullptr;  // presence only
+    d.out = d.out ? s->grid.p : nullptr;
+    d.a.t0 = 0;
+    std::memcpy(&k.d, &d, sizeof d);
+    FdtdParams fp = fdtd_params(s);
+    canonical_bytes(fp);
+    std::memcpy(&k.fp, &fp, sizeof fp);
+    k.chunk = chunk;
+    k.pml = s->use_pml ? 1 : 0;
+    return k;
+}
+
+struct StaticChunk
+{
+    pppm_solver *s;
+    const float *neumann;
+    int t0;
+    long long stride;
+    int n_listen;
+    float *out;
+    int chunk;
+};
+int enqueue_static_chunk(void *p)
+{
+    const StaticChunk &c = *static_cast<const StaticChunk *>(p);
+    for (int i = 0; i < c.chunk; i++)
+        RC(enqueue_step(c.s, own_bufs(c.s), c.neumann, c.t0, c.stride, c.n_listen, c.out));
     return PPPM_OK;
 }
 
@@ -813,6 +873,7 @@ int run_impl(pppm_solver *s, const float *verts_series, long long vstride, const
     {
         std::vector<int> none(n_listen, -1);
         CK(hipMemcpy(s->lc3.p, none.data(), n_listen * sizeof(int), hipMemcpyHostToDevice));
+        s->lis_version = -1;  // the table no longer holds set_listeners' C3 indices
     }
     float *out = n_listen > 0 ? out_dev : nullptr;
     const int t0 = s->t_host + 1;
@@ -849,7 +910,33 @@ int run_impl(pppm_solver *s, const float *verts_series, long long vstride, const
         RC(ensure_pool(s, s->pool_count + 2 * s->cap_p4));
     }
     int done = 0;
-    if (!use_graph)
+    if (use_graph && !verts_series)
+    {
+        // a short run (a checkpoint segment) in one graph, a long one in 32-step graphs; the
+        // graph is kept for later runs with the same arguments up to the per-call rows
+        const bool timing = s->timing;
+        s->timing = false;
+        const int chunk = steps <= 64 ? steps : 32;
+        const StepGraphKey key = step_graph_key(s, neumann_series, t0, stride, n_listen, out, chunk);
+        StaticChunk sc{s, neumann_series, t0, stride, n_listen, out, chunk};
+        hipGraphExec_t ge = nullptr;
+        int rc = solver_cached_graph(s, s->step_graphs, &key, sizeof key, enqueue_static_chunk, &sc, &ge);
+        for (; rc == PPPM_OK && done + chunk <= steps; done += chunk)
+        {
+            const hipError_t le = hipGraphLaunch(ge, s->stream);
+            if (le != hipSuccess)
+                rc = fail(s, PPPM_ERR_HIP, std::string("graph launch: ") + hipGetErrorString(le));
+        }
+        for (; rc == PPPM_OK && done < steps; done++)
+            rc = one_step(done);
+        s->timing = timing;
+        if (rc)
+        {
+            s->t_host += done;
+            return rc;
+        }
+    }
+    else if (!use_graph)
     {
         for (; done < steps; done++)
         {
@@ -938,6 +1025,66 @@ int solver_set_listeners(pppm_solver *s, const int *listen_xyz, int n_listen)
 FdtdParams solver_fdtd_params(const pppm_solver *s) { return fdtd_params(s); }
 int solver_read_status(pppm_solver *s) { return read_status(s); }
 int solver_ensure_ordered(pppm_solver *s) { return ensure_ordered(s); }
+void canonical_bytes(StepArgs &a)
+{
+    StepArgs z;
+    std::memset(&z, 0, sizeof z);
+    z.N = a.N;
+    z.nt = a.nt;
+    z.rc = a.rc;
+    z.t0 = a.t0;
+    z.stride = a.stride;
+    z.n_listen = a.n_listen;
+    std::memcpy(&a, &z, sizeof z);
+}
+void canonical_bytes(FdtdParams &p)
+{
+    FdtdParams z;
+    std::memset(&z, 0, sizeof z);
+    z.N = p.N;
+    z.coef = p.coef;
+    z.hh = p.hh;
+    z.c = p.c;
+    z.dt = p.dt;
+    z.h = p.h;
+    z.pml = p.pml;
+    z.damp = p.damp;
+    std::memcpy(&p, &z, sizeof z);
+}
+int solver_cached_graph(pppm_solver *s, GraphCache &c, const void *key, size_t n, int (*enqueue)(void *), void *arg,
+                        hipGraphExec_t *out)
+{
+    const unsigned char *kb = static_cast<const unsigned char *>(key);
+    for (const auto &x : c.e)
+        if (x.key.size() == n && std::memcmp(x.key.data(), kb, n) == 0)
+        {
+            *out = x.ge;
+            return PPPM_OK;
+        }
+    hipGraph_t g = nullptr;
+    CK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue(arg);
+    const hipError_t ee = hipStreamEndCapture(s->stream, &g);
+    if (rc || ee != hipSuccess)
+    {
+        if (g)
+            (void)hipGraphDestroy(g);
+        return rc ? rc : fail(s, PPPM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ee));
+    }
+    hipGraphExec_t ge = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    CK(ie);
+    if (c.e.size() >= GraphCache::kMax)
+    {
+        CK(hipStreamSynchronize(s->stream));  // the oldest graph may still be running
+        (void)hipGraphExecDestroy(c.e.front().ge);
+        c.e.erase(c.e.begin());
+    }
+    c.e.push_back(GraphCache::Entry{std::vector<unsigned char>(kb, kb + n), ge});
+    *out = ge;
+    return PPPM_OK;
+}
 SceneStep solver_scene_step(const pppm_solver *s, const float *neumann, int t0, long long stride, int n_listen,
                             float *out)
 {
@@ -1316,9 +1463,8 @@ int pppm_reset(pppm_solver *s)
         CK(hipMemsetAsync(s->hn.p, 0, (size_t)s->nt * RING * sizeof(float), s->stream));
         CK(hipMemsetAsync(s->hd.p, 0, (size_t)s->nt * RING * sizeof(float), s->stream));
     }
-    StepState init{-1, 0};
-    CK(hipMemcpyAsync(s->state.p, &init, sizeof(init), hipMemcpyHostToDevice, s->stream));
-    CK(hipStreamSynchronize(s->stream));
+    static_assert(sizeof(StepState) == 2 * sizeof(int), "StepState is {t, pad}");
+    CK(launch_put_ints(reinterpret_cast<int *>(s->state.p), 2, -1, 0, 0, 0, s->stream));
     s->t_host = -1;
     return PPPM_OK;
 }
@@ -1335,9 +1481,7 @@ int pppm_set_time_idx(pppm_solver *s, int t)
 {
     if (check_handle(s) || t < -1)
         return PPPM_ERR_ARG;
-    StepState v{t, 0};
-    CK(hipMemcpyAsync(s->state.p, &v, sizeof(v), hipMemcpyHostToDevice, s->stream));
-    CK(hipStreamSynchronize(s->stream));
+    CK(launch_put_ints(reinterpret_cast<int *>(s->state.p), 2, t, 0, 0, 0, s->stream));
     s->t_host = t;
     return PPPM_OK;
 }

@@ -75,7 +75,27 @@ enum
     ST_N = 8
 };
 
-
+// Instantiated step graphs of a handle, each with the bytes of the launch arguments it was
+// captured for (kept until evicted or the handle is destroyed; at most kMax).
+struct GraphCache
+{
+    static constexpr size_t kMax = 4;
+    struct Entry
+    {
+        std::vector<unsigned char> key;
+        hipGraphExec_t ge = nullptr;
+    };
+    std::vector<Entry> e;
+    GraphCache() = default;
+    GraphCache(const GraphCache &) = delete;
+    GraphCache &operator=(const GraphCache &) = delete;
+    ~GraphCache()
+    {
+        for (auto &x : e)
+            if (x.ge)
+                (void)hipGraphExecDestroy(x.ge);
+    }
+};
 
 }  // namespace pppm_amd
 
@@ -239,6 +259,12 @@ struct pppm_solver
     DBuf<float4> adj_bb;
     DBuf<pppm_amd::AdjState> adj_state;
     DBuf<unsigned char> state_trailer;
+    // captured graphs of pppm_run (static mesh) and pppm_adjoint_run, replayed by later calls
+    // whose launch arguments agree once the per-call rows are shifted to t = 0
+    pppm_amd::GraphCache step_graphs, adj_graphs;
+    // listener cells and the mesh version the C3 indices of set_listeners were read for
+    std::vector<int> lis_cells;
+    long long lis_version = -1;
 
     // per-phase GPU timing with HIP events on the handle's stream (the reference's
     // log_time / LOG_TIME per phase, src/core/macro.h:35-48)
@@ -276,4 +302,11 @@ SceneStep solver_scene_step(const pppm_solver *s, const float *neumann, int t0, 
 // gout_stride floats apart), gradient rows grad_stride floats apart, row 0 = step t_first
 AdjScene solver_adj_scene(pppm_solver *s, const float *gout, long long gout_stride, int n_listen, float *grad,
                           long long grad_stride, int t_first);
+// the graph of c whose key is the n bytes at key, or one captured now from what enqueue puts on
+// the handle's stream (and then kept in c, evicting the oldest entry past GraphCache::kMax)
+int solver_cached_graph(pppm_solver *s, GraphCache &c, const void *key, size_t n, int (*enqueue)(void *), void *arg,
+                        hipGraphExec_t *out);
+// zeroes the padding of a copy that is compared as bytes: the members are copied one by one
+void canonical_bytes(StepArgs &a);
+void canonical_bytes(FdtdParams &p);
 }  // namespace pppm_amd

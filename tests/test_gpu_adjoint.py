@@ -98,3 +98,25 @@ def test_state_checkpoint_round_trip(amd, scenes_mod):
     assert torch.equal(oa, ob)
     for rel in (0, -1):
         assert np.array_equal(a.get_grid(rel), b.get_grid(rel))
+
+
+def test_segment_runs_replay_one_graph(amd, scenes_mod):
+    """Runs over consecutive segments of one series replay the graph captured for the first one
+    (their launch arguments differ only by the rows of the series and of the output): samples and
+    grid equal an eager run, and a second pass into another output buffer equals the first."""
+    import torch
+    T, S = 120, 20
+    sc = scenes_mod.c3_scene(37, 2, seed=11, steps=T)
+    a, b = _solver(amd, sc), _solver(amd, sc)
+    nm = torch.from_numpy(sc.neumann).cuda()
+    oa = torch.zeros(T, 1, device="cuda")
+    a.run(nm, T, [sc.listener], oa, use_graph=False)
+    for rep in range(2):
+        print(f"segment pass {rep}", flush=True)
+        ob = torch.zeros(T, 1, device="cuda")
+        b.reset()
+        for k in range(T // S):
+            b.run(nm[S * k:S * (k + 1)], S, [sc.listener], ob[S * k:S * (k + 1)], use_graph=True)
+        assert torch.equal(oa, ob)
+        for rel in (0, -1):
+            assert np.array_equal(a.get_grid(rel), b.get_grid(rel))
