@@ -122,6 +122,11 @@ __global__ __launch_bounds__(SB) void k_tile_scan(ScanBatch b, const int *__rest
     const int tile = sums ? (int)blockIdx.x - J.tile0 : 0;
     const long long base = (long long)tile * T;
     const int tid = threadIdx.x;
+    // this thread's items are loaded first, so their loads overlap the tile offset's
+    int it[IT];
+#pragma unroll
+    for (int i = 0; i < IT; i++)
+        it[i] = item_of(J, base + i * SB + tid);
     if (offs)
     {
         if (tid == 0)
@@ -129,20 +134,29 @@ __global__ __launch_bounds__(SB) void k_tile_scan(ScanBatch b, const int *__rest
     }
     else if (tid < 64)
     {
+        // the sums of the job's earlier tiles (at most kDirectTiles of them on this path): all
+        // loads issued before the first is added
         int acc = 0;
         if (sums)
-            for (int t0 = J.tile0; t0 < (int)blockIdx.x; t0 += 64)
+        {
+            int part[kDirectTiles / 64];
+#pragma unroll
+            for (int i = 0; i < kDirectTiles / 64; i++)
             {
-                const int t = t0 + tid;
-                acc += t < (int)blockIdx.x ? sums[t] : 0;
+                const int t = J.tile0 + i * 64 + tid;
+                part[i] = t < (int)blockIdx.x ? sums[t] : 0;
             }
+#pragma unroll
+            for (int i = 0; i < kDirectTiles / 64; i++)
+                acc += part[i];
+        }
         acc = wave_total(acc);
         if (tid == 0)
             s_off = acc;
     }
 #pragma unroll
     for (int i = 0; i < IT; i++)
-        sv[pad(i * SB + tid)] = item_of(J, base + i * SB + tid);
+        sv[pad(i * SB + tid)] = it[i];
     __syncthreads();
     int v[IT], tsum = 0;
 #pragma unroll
