@@ -154,13 +154,19 @@ __device__ __forceinline__ void windows_item(const SceneStep &d, int back, int i
 // of step t is therefore free to be written by the extra waves of this launch.
 __device__ __forceinline__ void far_wave(const SceneStep &d, int wave, int lane, float *__restrict__ vals)
 {
+    // the wave map is read before the error word is waited on (in bounds either way), so the
+    // two loads take one memory round trip
+    int2 w0 = {0, 0}, w1 = {0, 0};
+    if (wave < d.c3_waves)
+    {
+        w0 = reinterpret_cast<const int2 *>(d.c3_wave)[wave];
+        w1 = reinterpret_cast<const int2 *>(d.c3_wave)[wave + 1];
+    }
     if (d.err && *d.err)  // behind a mesh update that failed: the step is run again after the update is
         return;
     const int t = d.st->t;
     if (wave < d.c3_waves)
     {
-        const int2 w0 = reinterpret_cast<const int2 *>(d.c3_wave)[wave];
-        const int2 w1 = reinterpret_cast<const int2 *>(d.c3_wave)[wave + 1];
         const int c0 = w0.x, q0 = w0.y, c1 = w1.x, q1 = w1.y;
         if (c0 >= c1)
             return;
@@ -216,11 +222,14 @@ __device__ __forceinline__ void dirichlet_wave(const SceneStep &d, int wave, int
                                                float *__restrict__ facs)
 {
     const int t = d.st->t;
-    // after a flagged mesh update the face lists may be incomplete: leave D[t] untouched
-    if (wave >= d.face_waves || *d.err != 0)
+    if (wave >= d.face_waves)
         return;
+    // the wave map is read before the error word is waited on (one memory round trip)
     const int2 w0 = reinterpret_cast<const int2 *>(d.face_wave)[wave];
     const int2 w1 = reinterpret_cast<const int2 *>(d.face_wave)[wave + 1];
+    // after a flagged mesh update the face lists may be incomplete: leave D[t] untouched
+    if (*d.err != 0)
+        return;
     const int c0 = w0.x, q0 = w0.y, c1 = w1.x, q1 = w1.y;
     if (c0 >= c1)
         return;
@@ -296,6 +305,13 @@ __device__ __forceinline__ void dirichlet_wave(const SceneStep &d, int wave, int
 // p(c) = far + 0.98 * sum W^bem (*) (N, D) over all 16 taps; records listener samples.
 __device__ __forceinline__ void near_wave(const SceneStep &d, int wave, int lane, float *__restrict__ vals)
 {
+    // the wave map is read before the error word is waited on (in bounds either way)
+    int2 w0 = {0, 0}, w1 = {0, 0};
+    if (wave < d.c3_waves)
+    {
+        w0 = reinterpret_cast<const int2 *>(d.c3_wave)[wave];
+        w1 = reinterpret_cast<const int2 *>(d.c3_wave)[wave + 1];
+    }
     if (d.err && *d.err)
         return;
     const int t = d.st->t;
@@ -303,8 +319,6 @@ __device__ __forceinline__ void near_wave(const SceneStep &d, int wave, int lane
     float *g = d.grid3 + (size_t)(((t % 3) + 3) % 3) * nn;
     if (wave < d.c3_waves)
     {
-        const int2 w0 = reinterpret_cast<const int2 *>(d.c3_wave)[wave];
-        const int2 w1 = reinterpret_cast<const int2 *>(d.c3_wave)[wave + 1];
         const int c0 = w0.x, q0 = w0.y, c1 = w1.x, q1 = w1.y;
         if (c0 >= c1)
             return;
