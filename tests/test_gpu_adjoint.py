@@ -120,3 +120,27 @@ def test_segment_runs_replay_one_graph(amd, scenes_mod):
         assert torch.equal(oa, ob)
         for rel in (0, -1):
             assert np.array_equal(a.get_grid(rel), b.get_grid(rel))
+
+
+def test_segment_runs_past_the_graph_cache(amd, scenes_mod):
+    """Segments of six different lengths (more graphs than a handle keeps, so the oldest are
+    evicted while the run goes on), run twice: both passes equal an eager run."""
+    import torch
+    lens = [5, 7, 9, 11, 13, 15]
+    T = sum(lens)
+    sc = scenes_mod.c3_scene(37, 2, seed=12, steps=T)
+    a, b = _solver(amd, sc), _solver(amd, sc)
+    nm = torch.from_numpy(sc.neumann).cuda()
+    oa = torch.zeros(T, 1, device="cuda")
+    a.run(nm, T, [sc.listener], oa, use_graph=False)
+    for rep in range(2):
+        print(f"segment lengths {lens}, pass {rep}", flush=True)
+        ob = torch.zeros(T, 1, device="cuda")
+        b.reset()
+        t = 0
+        for n in lens:
+            b.run(nm[t:t + n], n, [sc.listener], ob[t:t + n], use_graph=True)
+            t += n
+        assert torch.equal(oa, ob)
+        for rel in (0, -1):
+            assert np.array_equal(a.get_grid(rel), b.get_grid(rel))
