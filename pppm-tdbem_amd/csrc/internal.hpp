@@ -339,8 +339,10 @@ struct SceneStep
     // histories and per-step buffers
     float *hn, *hd, *wn, *wd, *far;
     const float *neumann;  // Neumann series (row t - t0), null: the ring slot is written elsewhere
-    // listener recording: out[(t - t0) * out_stride + l] for l < a.n_listen (null: none)
+    // listener recording: out[(t - t0) * out_stride + l] for l < a.n_listen (null: none); lc3
+    // null: the listeners' C3 indices are read from c3_index (cell -> C3 index) per step
     const int *lcell, *lc3;
+    const int *c3_index;
     float *out;
     long long out_stride;
 };

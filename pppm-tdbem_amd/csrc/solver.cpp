@@ -632,6 +632,7 @@ SceneStep scene_step(const pppm_solver *s, const StepBufs &b, const float *neuma
     d.neumann = neumann;
     d.lcell = b.lcell;
     d.lc3 = b.lc3;
+    d.c3_index = s->c3_index.p;
     d.out = out;
     d.out_stride = n_listen;
     return d;
@@ -707,10 +708,10 @@ int update_and_step(pppm_solver *s, const float *vsrc, const float *neumann, int
     CK(hipEventRecord(c.ev_status, s->stream));
     span_end(s, sp);
     auto step = [&]() -> int {
-        RC(enqueue_step(s, own_bufs(s), neumann, t0, stride, n_listen, nullptr));
-        if (out)
-            CK(launch_listen_direct(scene_step(s, own_bufs(s), nullptr, t0, stride, n_listen, out), s->stream));
-        return PPPM_OK;
+        // the lists moved with this update: the near field looks the listeners' C3 indices up
+        StepBufs b = own_bufs(s);
+        b.lc3 = nullptr;
+        return enqueue_step(s, b, neumann, t0, stride, n_listen, out);
     };
     RC(step());
     CK(hipEventSynchronize(c.ev_status));

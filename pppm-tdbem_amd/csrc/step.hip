@@ -305,13 +305,18 @@ __device__ __forceinline__ void dirichlet_wave(const SceneStep &d, int wave, int
 // p(c) = far + 0.98 * sum W^bem (*) (N, D) over all 16 taps; records listener samples.
 __device__ __forceinline__ void near_wave(const SceneStep &d, int wave, int lane, float *__restrict__ vals)
 {
-    // the wave map is read before the error word is waited on (in bounds either way)
+    // the wave map is read before the error word is waited on (in bounds either way), and lane l
+    // reads listener l's C3 index: the table set with the listeners, or, for lists that move with
+    // the mesh (lc3 null), this update's map of the listener's cell
     int2 w0 = {0, 0}, w1 = {0, 0};
     if (wave < d.c3_waves)
     {
         w0 = reinterpret_cast<const int2 *>(d.c3_wave)[wave];
         w1 = reinterpret_cast<const int2 *>(d.c3_wave)[wave + 1];
     }
+    int lc = -1;
+    if (d.out && lane < d.a.n_listen)
+        lc = d.lc3 ? d.lc3[lane] : d.c3_index[d.lcell[lane]];
     if (d.err && *d.err)
         return;
     const int t = d.st->t;
@@ -349,21 +354,21 @@ __device__ __forceinline__ void near_wave(const SceneStep &d, int wave, int lane
             wave_sync();
             rec = rec_next;
         }
-        if (leader)
-        {
-            const float v = (float)((double)fv + (double)acc * 0.98);
-            if (cell < nn)
-                g[cell] = v;
-            if (d.out)
-                for (int l = 0; l < d.a.n_listen; l++)
-                    if (d.lc3[l] == cs)
-                        d.out[(size_t)(t - d.a.t0) * d.out_stride + l] = v;
-        }
+        const float v = (float)((double)fv + (double)acc * 0.98);
+        if (leader && cell < nn)
+            g[cell] = v;
+        if (d.out)
+            for (int l = 0; l < d.a.n_listen; l++)
+            {
+                const int c3l = __shfl(lc, l);  // every lane takes part
+                if (leader && c3l == cs)
+                    d.out[(size_t)(t - d.a.t0) * d.out_stride + l] = v;
+            }
     }
     else if (d.out && wave == d.c3_waves)
     {
         // listeners outside the corrected region read the grid directly (no wave writes them)
-        if (lane < d.a.n_listen && d.lc3[lane] < 0)
+        if (lane < d.a.n_listen && lc < 0)
             d.out[(size_t)(t - d.a.t0) * d.out_stride + lane] = g[d.lcell[lane]];
     }
 }
