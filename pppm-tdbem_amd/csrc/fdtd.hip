@@ -348,6 +348,10 @@ __device__ __forceinline__ void fdtd_block(float *__restrict__ grid3, const Fdtd
 __global__ __launch_bounds__(BLK) void k_fdtd_abc(float *__restrict__ grid3, FdtdParams p, const StepState *__restrict__ st,
                                                   int nb_inner, int nb_fdtd, const int *__restrict__ skip, StepFuse f)
 {
+    // the update's status block for the host, before the skip: an update that raised an error
+    // is reported too
+    if (f.status_host && blockIdx.x == gridDim.x - 1 && (int)threadIdx.x < f.status_n)
+        f.status_host[threadIdx.x] = f.status[threadIdx.x];
     if (skip && *skip)
         return;
     const int t = st->t;

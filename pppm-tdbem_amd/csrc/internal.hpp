@@ -46,6 +46,11 @@ struct StepFuse
     const float *hn, *hd;
     float *wn, *wd;
     int nt;
+    // status_host != null: the launch also copies the status_n ints of status (the mesh
+    // update's status block) to status_host (pinned host memory), whether or not it steps
+    const int *status;
+    int *status_host;
+    int status_n;
 };
 // skip (may be null): as for launch_advance
 hipError_t launch_fdtd(float *grid3, float *pml3, const FdtdParams &p, const StepState *st, hipStream_t s,

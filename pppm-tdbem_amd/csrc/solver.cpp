@@ -641,8 +641,10 @@ SceneStep scene_step(const pppm_solver *s, const StepBufs &b, const float *neuma
 // one update_grid_and_face on the handle's stream; neumann is device memory (row t - t0).
 // timed: phase spans when timing is on (events are created on first use, so the
 // allocation-free path passes false)
+// status_event != null: the FDTD launch also copies the mesh update's status block to st_h and
+// the event is recorded right behind it (the caller has checked fused_status)
 int enqueue_step(pppm_solver *s, const StepBufs &b, const float *neumann, int t0, long long stride, int n_listen,
-                 float *out, bool timed = true)
+                 float *out, bool timed = true, hipEvent_t status_event = nullptr)
 {
     const FdtdParams fp = fdtd_params(s);
     const SceneStep d = scene_step(s, b, neumann, t0, stride, n_listen, out);
@@ -652,8 +654,16 @@ int enqueue_step(pppm_solver *s, const StepBufs &b, const float *neumann, int t0
     if (s->mesh_set && !s->use_pml)
     {
         // the ABC-shell launch also builds the gathers' history windows
-        const StepFuse fuse{s->hn.p, s->hd.p, d.wn, d.wd, s->nt};
+        StepFuse fuse{s->hn.p, s->hd.p, d.wn, d.wd, s->nt, nullptr, nullptr, 0};
+        if (status_event)
+        {
+            fuse.status = s->status.p;
+            fuse.status_host = s->st_hdev;
+            fuse.status_n = ST_N;
+        }
         CK(launch_fdtd(s->grid.p, s->pmlbuf.p, fp, s->state.p, s->stream, skip, &fuse));
+        if (status_event)
+            CK(hipEventRecord(status_event, s->stream));
         span_end(s, sp);
     }
     else
@@ -702,16 +712,22 @@ int update_and_step(pppm_solver *s, const float *vsrc, const float *neumann, int
     if (!c.ev_status)
         CK(hipEventCreateWithFlags(&c.ev_status, hipEventDisableTiming));
     const bool use_prev = begin_update(s, false);
+    // the step's FDTD launch brings the status block to the host (no copy of its own on the way
+    // to the step), except on the PML path, which launches no fused FDTD
+    const bool fused_status = s->st_hdev && !s->use_pml;
     int sp = span_begin(s, 4);
     RC(enqueue_attempt(s, false, vsrc, use_prev));
-    CK(hipMemcpyAsync(s->st_h, s->status.p, ST_N * sizeof(int), hipMemcpyDeviceToHost, s->stream));
-    CK(hipEventRecord(c.ev_status, s->stream));
+    if (!fused_status)
+    {
+        CK(hipMemcpyAsync(s->st_h, s->status.p, ST_N * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        CK(hipEventRecord(c.ev_status, s->stream));
+    }
     span_end(s, sp);
     auto step = [&]() -> int {
         // the lists moved with this update: the near field looks the listeners' C3 indices up
         StepBufs b = own_bufs(s);
         b.lc3 = nullptr;
-        return enqueue_step(s, b, neumann, t0, stride, n_listen, out);
+        return enqueue_step(s, b, neumann, t0, stride, n_listen, out, true, fused_status ? c.ev_status : nullptr);
     };
     RC(step());
     CK(hipEventSynchronize(c.ev_status));
@@ -1130,6 +1146,8 @@ int pppm_create(pppm_solver **out, int res, float dl, float dt, const float *min
         return PPPM_ERR_HIP;
     }
     s->st_h = s->st_pin.p;
+    if (hipHostGetDevicePointer((void **)&s->st_hdev, s->st_pin.p, 0) != hipSuccess)
+        s->st_hdev = nullptr;
     std::memset(s->st_h, 0, ST_N * sizeof(int));
     s->own_stream = true;
     if (hipStreamCreateWithFlags(&s->uf.aux, hipStreamNonBlocking) != hipSuccess ||
