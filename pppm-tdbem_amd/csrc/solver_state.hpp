@@ -213,9 +213,10 @@ struct pppm_solver
     } st_pin;
     int *st_h = nullptr;
     int *st_hdev = nullptr;  // the device's address of st_h (null: copies only)
-    // update_and_step: recorded behind the status copy of an update whose step is enqueued before
-    // the status is read (a copy on a second stream waiting for the update was slower: the
-    // step's kernels queued behind the cross-stream wait)
+    // update_and_step: recorded behind the step's FDTD launch, which writes the update's status
+    // block into st_h (on the PML path, behind a copy); the step is enqueued before the status is
+    // read (a copy on a second stream waiting for the update was slower: the step's kernels
+    // queued behind the cross-stream wait)
     struct StatusCopy
     {
         hipEvent_t ev_status = nullptr;
